@@ -1,0 +1,1099 @@
+// Resident model handles behind the C ABI of include/fastpls_hip.h.
+//
+// Host orchestration of the HIP kernels: one non-blocking stream per handle,
+// all matrices resident in HBM, one host synchronisation at the end of every
+// ABI call. The handle semantics follow the reference's TypedResidentHandle
+// (src/cuda_resident_api.cuh:69-479) and the fitting cores follow the
+// reference CPU algorithms (inst/include/fastpls/core/simpls.hpp,
+// rsvd.hpp, plssvd.hpp, lda.hpp) so that results agree with the FP64 CPU
+// path for the same seed.
+#include "../../include/fastpls_hip.h"
+#include "kernels.hpp"
+#include "stream.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+namespace fpls {
+namespace {
+
+// ------------------------------------------------------------- buffers
+class Buf {
+ public:
+  Buf() = default;
+  Buf(const Buf&) = delete;
+  Buf& operator=(const Buf&) = delete;
+  ~Buf() { release(); }
+  void alloc(size_t bytes) {
+    release();
+    if (bytes) FPLS_HIP(hipMalloc(&ptr_, bytes));
+    bytes_ = bytes;
+  }
+  void reserve(size_t bytes) {
+    if (bytes > bytes_) alloc(bytes);
+  }
+  void release() noexcept {
+    if (ptr_) (void)hipFree(ptr_);
+    ptr_ = nullptr;
+    bytes_ = 0;
+  }
+  template<class U> U* as() const { return static_cast<U*>(ptr_); }
+  double* d() const { return static_cast<double*>(ptr_); }
+  bool empty() const { return ptr_ == nullptr; }
+  size_t bytes() const { return bytes_; }
+
+ private:
+  void* ptr_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+// Gaussian sketch exactly as the reference draws it (simpls.hpp:244-249).
+void gaussian(unsigned int seed, size_t count, double* out) {
+  std::mt19937 generator(seed);
+  std::normal_distribution<double> normal(0.0, 1.0);
+  for (size_t i = 0; i < count; ++i) out[i] = normal(generator);
+}
+
+// Split counts: aim for >= 2048 work-groups, keep every split >= 32 wide.
+int splits_for(long long tiles_per_split, long long extent, long long min_chunk) {
+  long long s = (2048 + tiles_per_split - 1) / std::max<long long>(tiles_per_split, 1);
+  const long long cap = std::max<long long>(1, extent / min_chunk);
+  s = std::max<long long>(1, std::min(s, cap));
+  return int(std::min<long long>(s, 256));
+}
+
+// simpls.hpp:43-62
+int candidate_block_size(long long remaining, long long p, long long q, bool cls, long long n,
+                         long long maximum) {
+  const double work = double(n) * double(p) * double(q);
+  if (q <= 1 || remaining < 4 || work < 5.0e8) return 1;
+  if (cls && q <= 2048) return int(std::max<long long>(1, std::min({maximum, remaining, p, q})));
+  if (!cls && double(p) * double(q) > 64.0 * 1024.0 * 1024.0)
+    return int(std::max<long long>(1, std::min({8LL, maximum, remaining, p, q})));
+  return 1;
+}
+
+struct Event {
+  hipEvent_t e = nullptr;
+  Event() { FPLS_HIP(hipEventCreate(&e)); }
+  ~Event() { if (e) (void)hipEventDestroy(e); }
+};
+
+// ------------------------------------------------------------- handle
+class Handle {
+ public:
+  virtual ~Handle() = default;
+  virtual void predict_path(const void* x, int rows, const int* prefixes, int count, bool lda,
+                            void* out) = 0;
+  virtual void classify_path(const void* x, int rows, const int* prefixes, int count, bool lda,
+                             int top, int* labels, void* responses) = 0;
+  virtual void response_sums(const void* x, const void* y, const int* labels, int rows,
+                             int prefix, void* out) = 0;
+  virtual void project(const void* x, int rows, int prefix, void* out) = 0;
+  virtual void export_field(int field, void* out, size_t elements) = 0;
+  virtual void controls(int* oversample, int* power, int* block, int* block_limit,
+                        int* implicit, int* cache) = 0;
+  virtual void compact(bool prepare_lda) = 0;
+  virtual double last_fit_seconds() const = 0;
+  // [0] fit, [1] cross-product pass over X, [2] score pass over X (PLS-SVD)
+  // or the whole component loop (SIMPLS), [3] bytes of X, seconds from events.
+  virtual void phases(double* out, int count) const = 0;
+};
+
+template<class T>
+class Model final : public Handle {
+ public:
+  enum Method { kSimpls = 0, kPlssvd = 1 };
+
+  Model(int method, const T* x, const T* y, const int* labels, bool device_inputs, int n,
+        int p, int q, int components, int scaling, int oversample, int power,
+        unsigned long long seed)
+      : method_(method), n_(n), p_(p), q_(q), scaling_(scaling), oversample_(oversample),
+        power_(power), seed_(seed), classification_(labels != nullptr) {
+    if ((y == nullptr) == (labels == nullptr))
+      throw std::invalid_argument("provide responses or labels, not both");
+    if (x == nullptr) throw std::invalid_argument("null predictors");
+    if (n < 2 || p < 1 || q < 1 || components < 1)
+      throw std::invalid_argument("invalid resident dimensions");
+    if (scaling < 1 || scaling > 3) throw std::invalid_argument("scaling must be 1, 2 or 3");
+    if (oversample < 0 || power < 0) throw std::invalid_argument("oversample and power must be non-negative");
+    if (classification_ && q < 2) throw std::invalid_argument("classification needs at least two classes");
+    if (method_ == kSimpls) {
+      if (components > std::min(n - 1, p)) throw std::invalid_argument("invalid resident SIMPLS dimensions");
+    } else {
+      if (components > std::min(n - 1, std::min(p, q)))
+        throw std::invalid_argument("invalid resident PLS-SVD dimensions");
+      if (classification_ && components >= q)
+        throw std::invalid_argument("classification PLS-SVD components must be below class count");
+    }
+    A_ = components;
+    FPLS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    try {
+      Event start, stop;
+      load_inputs(x, y, labels, device_inputs);
+      FPLS_HIP(hipEventRecord(start.e, stream_));
+      prepare_crossprod();
+      if (method_ == kSimpls) fit_simpls(); else fit_plssvd();
+      finish_fit();
+      FPLS_HIP(hipEventRecord(stop.e, stream_));
+      check_state();
+      float ms = 0.f;
+      FPLS_HIP(hipEventElapsedTime(&ms, start.e, stop.e));
+      fit_seconds_ = double(ms) * 1e-3;
+      FPLS_HIP(hipEventElapsedTime(&ms, cp0_.e, cp1_.e));
+      crossprod_seconds_ = double(ms) * 1e-3;
+      FPLS_HIP(hipEventElapsedTime(&ms, sc0_.e, sc1_.e));
+      score_seconds_ = double(ms) * 1e-3;
+    } catch (...) {
+      (void)hipStreamSynchronize(stream_);
+      (void)hipStreamDestroy(stream_);
+      stream_ = nullptr;
+      throw;
+    }
+  }
+
+  ~Model() override {
+    if (stream_) {
+      (void)hipStreamSynchronize(stream_);
+      (void)hipStreamDestroy(stream_);
+    }
+  }
+
+  double last_fit_seconds() const override { return fit_seconds_; }
+
+  void phases(double* out, int count) const override {
+    const double v[4] = {fit_seconds_, crossprod_seconds_, score_seconds_,
+                         double(n_) * double(p_) * double(sizeof(T))};
+    for (int i = 0; i < count && i < 4; ++i) out[i] = v[i];
+  }
+
+  // ------------------------------------------------------------ queries
+  void predict_path(const void* x, int rows, const int* prefixes, int count, bool lda,
+                    void* out) override {
+    if (!x || !out || rows < 1 || count < 1) throw std::invalid_argument("invalid resident prediction-path request");
+    check_path(prefixes, count);
+    if (lda && !classification_) throw std::invalid_argument("LDA requires class labels");
+    const int maxp = prefixes[count - 1];
+    load_rows(static_cast<const T*>(x), rows);
+    project_rows(rows, maxp);
+    T* o = static_cast<T*>(out);
+    for (int j = 0; j < count; ++j) {
+      response_block(rows, prefixes[j], lda, o + (size_t)j * rows * q_out(), nullptr);
+    }
+    FPLS_HIP(hipStreamSynchronize(stream_));
+  }
+
+  void classify_path(const void* x, int rows, const int* prefixes, int count, bool lda,
+                     int top, int* labels, void* responses) override {
+    if (!classification_) throw std::invalid_argument("invalid resident classification request");
+    if (!x || !labels || rows < 1 || count < 1 || top < 1 || top > q_)
+      throw std::invalid_argument("invalid resident classification-path request");
+    check_path(prefixes, count);
+    const int maxp = prefixes[count - 1];
+    load_rows(static_cast<const T*>(x), rows);
+    project_rows(rows, maxp);
+    top_.reserve(sizeof(int) * (size_t)rows * top);
+    for (int j = 0; j < count; ++j) {
+      T* resp = responses ? static_cast<T*>(responses) + (size_t)j * rows * q_ : nullptr;
+      if (resp) response_block(rows, prefixes[j], false, resp, nullptr);
+      const double* constants = nullptr;
+      if (lda) {
+        response_block(rows, prefixes[j], true, nullptr, nullptr);
+        constants = nullptr;  // discriminants already include the constants
+      } else if (!resp) {
+        response_block(rows, prefixes[j], false, nullptr, nullptr);
+      }
+      launch_topk(pred64_.d(), rows, q_, constants, top, top_.as<int>(), stream_);
+      FPLS_HIP(hipMemcpyAsync(labels + (size_t)j * rows * top, top_.as<int>(),
+                              sizeof(int) * (size_t)rows * top, hipMemcpyDeviceToHost, stream_));
+    }
+    FPLS_HIP(hipStreamSynchronize(stream_));
+  }
+
+  void response_sums(const void* x, const void* y, const int* labels, int rows, int prefix,
+                     void* out) override {
+    if (!out || (!y && !labels) || (y && labels) || (classification_ != (labels != nullptr)))
+      throw std::invalid_argument("invalid resident response representation");
+    if (!x || rows < 1 || prefix < 1 || prefix > A_) throw std::invalid_argument("invalid resident prediction input");
+    load_rows(static_cast<const T*>(x), rows);
+    project_rows(rows, prefix);
+    response_block(rows, prefix, false, nullptr, nullptr);
+    metric_.reserve(sizeof(double) * 3 * q_);
+    flags_.reserve(sizeof(int));
+    fill_zero_int(flags_.as<int>(), 1);
+    const T* obs = nullptr;
+    const int* lab = nullptr;
+    if (labels) {
+      obs_labels_.reserve(sizeof(int) * (size_t)rows);
+      FPLS_HIP(hipMemcpyAsync(obs_labels_.as<int>(), labels, sizeof(int) * (size_t)rows,
+                              hipMemcpyHostToDevice, stream_));
+      lab = obs_labels_.as<int>();
+    } else {
+      obs_.reserve(sizeof(T) * (size_t)rows * q_);
+      FPLS_HIP(hipMemcpyAsync(obs_.as<T>(), y, sizeof(T) * (size_t)rows * q_,
+                              hipMemcpyHostToDevice, stream_));
+      obs = obs_.as<T>();
+    }
+    launch_response_sums<T>(pred64_.d(), obs, lab, my_.d(), rows, q_, metric_.d(),
+                            flags_.as<int>(), stream_);
+    std::vector<double> host(3 * (size_t)q_);
+    int bad = 0;
+    FPLS_HIP(hipMemcpyAsync(host.data(), metric_.d(), sizeof(double) * 3 * q_, hipMemcpyDeviceToHost, stream_));
+    FPLS_HIP(hipMemcpyAsync(&bad, flags_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, stream_));
+    FPLS_HIP(hipStreamSynchronize(stream_));
+    if (bad) throw std::runtime_error("nonfinite response/prediction or invalid class label in resident metrics");
+    T* o = static_cast<T*>(out);
+    for (size_t i = 0; i < host.size(); ++i) o[i] = T(host[i]);
+  }
+
+  void project(const void* x, int rows, int prefix, void* out) override {
+    if (!x || !out || rows < 1 || prefix < 1 || prefix > A_) throw std::invalid_argument("invalid resident score request");
+    load_rows(static_cast<const T*>(x), rows);
+    project_rows(rows, prefix);
+    outT_.reserve(sizeof(T) * (size_t)rows * prefix);
+    launch_convert<T>(scores_.d(), outT_.as<T>(), (long long)rows * prefix, stream_);
+    FPLS_HIP(hipMemcpyAsync(out, outT_.as<T>(), sizeof(T) * (size_t)rows * prefix,
+                            hipMemcpyDeviceToHost, stream_));
+    FPLS_HIP(hipStreamSynchronize(stream_));
+  }
+
+  void export_field(int field, void* out, size_t elements) override {
+    const size_t counts[] = {(size_t)p_ * A_, (size_t)q_ * A_, (size_t)n_ * A_, (size_t)p_,
+                             (size_t)p_, (size_t)q_, (size_t)p_ * A_, (size_t)A_ + 1};
+    if (field < 0 || field > 7 || !out || elements != counts[field])
+      throw std::invalid_argument("invalid resident field output size");
+    const double* src = nullptr;
+    switch (field) {
+      case 0: src = R_.d(); break;
+      case 1: src = Q_.d(); break;
+      case 2:
+        if (Tsc_.empty()) throw std::runtime_error("training scores were released by compact");
+        src = Tsc_.d();
+        break;
+      case 3: src = mx_.d(); break;
+      case 4: src = sx_.d(); break;
+      case 5: src = my_.d(); break;
+      default:
+        ensure_variance();
+        src = field == 6 ? var_loadings_.d() : var_ss_.d();
+    }
+    outT_.reserve(sizeof(T) * elements);
+    launch_convert<T>(src, outT_.as<T>(), (long long)elements, stream_);
+    FPLS_HIP(hipMemcpyAsync(out, outT_.as<T>(), sizeof(T) * elements, hipMemcpyDeviceToHost, stream_));
+    FPLS_HIP(hipStreamSynchronize(stream_));
+  }
+
+  void controls(int* oversample, int* power, int* block, int* block_limit, int* implicit,
+                int* cache) override {
+    if (!oversample || !power || !block || !block_limit || !implicit || !cache)
+      throw std::invalid_argument("null resident control output");
+    *oversample = oversample_;
+    *power = power_;
+    *block = method_ == kSimpls ? block_ : A_;
+    *block_limit = 0;
+    *implicit = 0;
+    *cache = 0;
+  }
+
+  void compact(bool prepare_lda) override {
+    if (prepare_lda) {
+      if (!classification_) throw std::invalid_argument("LDA compaction requires classification labels");
+      ensure_lda_moments();
+    }
+    FPLS_HIP(hipStreamSynchronize(stream_));
+    for (Buf* b : {&xown_, &yown_, &Yc_, &S_, &S0_, &G_, &V_, &Tsc_, &simpls_scratch_, &cand_,
+                   &cand_scaled_, &t_raw_, &ts_part_, &ls_part_, &omega_, &steps_, &fitdev_,
+                   &part_, &tmp1_, &tmp2_, &tmp3_, &H_, &labels_dev_})
+      b->release();
+    X_ = nullptr;
+    compacted_ = true;
+  }
+
+ private:
+  int q_out() const { return q_; }
+
+  // --------------------------------------------------------- data load
+  void load_inputs(const T* x, const T* y, const int* labels, bool device_inputs) {
+    const size_t np = (size_t)n_ * p_;
+    if (device_inputs) {
+      X_ = x;
+    } else {
+      xown_.alloc(sizeof(T) * np);
+      FPLS_HIP(hipMemcpyAsync(xown_.as<T>(), x, sizeof(T) * np, hipMemcpyHostToDevice, stream_));
+      X_ = xown_.as<T>();
+    }
+    Yc_.alloc(sizeof(double) * (size_t)n_ * q_);
+    ysum_.alloc(sizeof(double) * q_);
+    my_.alloc(sizeof(double) * q_);
+    state_.alloc(sizeof(int) * 4);
+    fill_zero_int(state_.as<int>(), 4);
+    if (labels) {
+      const int* dl = labels;
+      if (!device_inputs) {
+        labels_dev_.alloc(sizeof(int) * (size_t)n_);
+        FPLS_HIP(hipMemcpyAsync(labels_dev_.as<int>(), labels, sizeof(int) * (size_t)n_,
+                                hipMemcpyHostToDevice, stream_));
+        dl = labels_dev_.as<int>();
+      }
+      counts_.alloc(sizeof(int) * q_);
+      H_.alloc(sizeof(double) * (size_t)n_ * q_);
+      launch_label_response(dl, n_, q_, counts_.as<int>(), my_.d(), Yc_.d(), H_.d(), ysum_.d(),
+                            state_.as<int>() + 2, stream_);
+    } else {
+      const T* dy = y;
+      if (!device_inputs) {
+        yown_.alloc(sizeof(T) * (size_t)n_ * q_);
+        FPLS_HIP(hipMemcpyAsync(yown_.as<T>(), y, sizeof(T) * (size_t)n_ * q_,
+                                hipMemcpyHostToDevice, stream_));
+        dy = yown_.as<T>();
+      }
+      launch_center_response<T>(dy, n_, q_, my_.d(), Yc_.d(), ysum_.d(), stream_);
+    }
+  }
+
+  void fill_zero_int(int* ptr, int count) {
+    // A kernel (not hipMemsetAsync) so the stream stays graph-capturable.
+    std::vector<int> zeros(count, 0);
+    FPLS_HIP(hipMemcpy(ptr, zeros.data(), sizeof(int) * count, hipMemcpyHostToDevice));
+  }
+
+  // --------------------------------------------------------- products
+  // out (m x N, ld m) = (A - 1 cᵀ) W for A (m x k) of element U.
+  template<class U>
+  void product(const U* A, long long m, long long k, const double* center, const double* W,
+               long long ldw, int N, double* out) {
+    for (int c0 = 0; c0 < N; c0 += 64) {
+      const int nb = std::min(64, N - c0);
+      const int splits = splits_for(xw_row_tiles(m), k, 32);
+      XwTask<U> t{};
+      t.A = A; t.m = m; t.k = k; t.center = center; t.W = W + (long long)c0 * ldw; t.ldw = ldw;
+      t.N = nb; t.splits = splits;
+      if (splits == 1) {
+        t.out = out + (long long)c0 * m; t.ldo = m; t.slab = 0;
+        launch_xw_single<U>(t, stream_);
+      } else {
+        part_.reserve(sizeof(double) * (size_t)splits * m * nb);
+        t.out = part_.d(); t.ldo = m; t.slab = m * nb;
+        launch_xw_single<U>(t, stream_);
+        launch_sum_slabs(part_.d(), splits, m * nb, m * nb, out + (long long)c0 * m, stream_);
+      }
+    }
+  }
+
+  // out (k x N, ld k) = (A - 1 cᵀ)ᵀ W for A (m x k), W (m x N).
+  template<class U>
+  void tproduct(const U* A, long long m, long long k, const double* center, const double* W,
+                long long ldw, int N, double* out) {
+    for (int c0 = 0; c0 < N; c0 += 64) {
+      const int nb = std::min(64, N - c0);
+      const int splits = splits_for(xtw_col_tiles(k), m, 256);
+      XtwTask<U> t{};
+      t.A = A; t.m = m; t.k = k; t.center = center; t.W = W + (long long)c0 * ldw; t.ldw = ldw;
+      t.N = nb; t.mode = center ? kCentered : kPlain; t.splits = splits;
+      if (splits == 1) {
+        t.out = out + (long long)c0 * k; t.ldo = k; t.slab = 0;
+        launch_xtw_single<U>(t, stream_);
+      } else {
+        part_.reserve(sizeof(double) * (size_t)splits * k * nb);
+        t.out = part_.d(); t.ldo = k; t.slab = k * nb;
+        launch_xtw_single<U>(t, stream_);
+        launch_sum_slabs(part_.d(), splits, k * nb, k * nb, out + (long long)c0 * k, stream_);
+      }
+    }
+  }
+
+  // ----------------------------------------------------- preprocessing
+  // supervised.hpp:27-124 / classification.hpp:86-184 in one pass over X.
+  void prepare_crossprod() {
+    mx_.alloc(sizeof(double) * p_);
+    sx_.alloc(sizeof(double) * p_);
+    colss_.alloc(sizeof(double) * p_);
+    S_.alloc(sizeof(double) * (size_t)p_ * q_);
+    S0_.alloc(sizeof(double) * (size_t)p_ * q_);
+    const int splits = splits_for(xtw_col_tiles(p_), n_, 256);
+    Buf s_part, stats;
+    s_part.alloc(sizeof(double) * (size_t)splits * p_ * q_);
+    stats.alloc(sizeof(double) * (size_t)splits * 2 * p_);
+    FPLS_HIP(hipEventRecord(cp0_.e, stream_));
+    for (int c0 = 0; c0 < q_; c0 += 64) {
+      const int nb = std::min(64, q_ - c0);
+      XtwTask<T> t{};
+      t.A = X_; t.m = n_; t.k = p_; t.center = nullptr;
+      t.W = Yc_.d() + (long long)c0 * n_; t.ldw = n_; t.N = nb; t.mode = kShiftStats;
+      t.splits = splits;
+      t.out = s_part.d() + (long long)c0 * p_; t.ldo = p_; t.slab = (long long)p_ * q_;
+      t.stats = c0 == 0 ? stats.d() : nullptr;
+      launch_xtw_single<T>(t, stream_);
+    }
+    FPLS_HIP(hipEventRecord(cp1_.e, stream_));
+    launch_finalize_crossprod<T>(X_, n_, p_, q_, scaling_, stats.d(), s_part.d(), splits,
+                                 ysum_.d(), mx_.d(), sx_.d(), colss_.d(), S_.d(), S0_.d(),
+                                 state_.as<int>() + 2, stream_);
+    // the partial buffers must outlive the kernels that read them
+    FPLS_HIP(hipStreamSynchronize(stream_));
+  }
+
+  // ------------------------------------------------------------ SIMPLS
+  // simpls.hpp:480-763 with the routing of src/r_api.cpp:1149-1190.
+  void fit_simpls() {
+    const int maximum = std::min({A_, p_, std::max(n_ - 1, 1)});
+    A_ = maximum;
+    block_ = candidate_block_size(maximum, p_, q_, classification_, n_, 64);
+    const bool right_gram = q_ <= p_ && q_ <= 512;
+    const int maxdim = std::min(p_, q_);
+    std::vector<RefreshStep> steps;
+    std::vector<double> pool;
+    int wmax = 1, bmax = 1;
+    for (int a = 0; a < maximum;) {
+      const int b = std::min({std::max(block_, 1), maximum - a, maxdim});
+      const int w = std::min(maxdim, b + oversample_);
+      RefreshStep st{};
+      st.a = a; st.b = b; st.w = w; st.omega_off = (long long)pool.size();
+      const size_t base = pool.size();
+      pool.resize(base + (size_t)q_ * w);
+      gaussian(unsigned(seed_) + unsigned(a), (size_t)q_ * w, pool.data() + base);
+      steps.push_back(st);
+      wmax = std::max(wmax, w);
+      bmax = std::max(bmax, b);
+      a += b;
+    }
+    if (wmax > 512) throw std::invalid_argument("sketch width above 512 is not supported");
+    R_.alloc(sizeof(double) * (size_t)p_ * A_);
+    Q_.alloc(sizeof(double) * (size_t)q_ * A_);
+    V_.alloc(sizeof(double) * (size_t)p_ * A_);
+    Tsc_.alloc(sizeof(double) * (size_t)n_ * A_);
+    if (right_gram) {
+      G_.alloc(sizeof(double) * (size_t)q_ * q_);
+      tproduct<double>(S_.d(), p_, q_, nullptr, S_.d(), p_, q_, G_.d());
+    }
+    omega_.alloc(sizeof(double) * pool.size());
+    FPLS_HIP(hipMemcpy(omega_.d(), pool.data(), sizeof(double) * pool.size(), hipMemcpyHostToDevice));
+    steps_.alloc(sizeof(RefreshStep) * steps.size());
+    FPLS_HIP(hipMemcpy(steps_.as<RefreshStep>(), steps.data(), sizeof(RefreshStep) * steps.size(),
+                       hipMemcpyHostToDevice));
+    simpls_scratch_.alloc(sizeof(double) * (size_t)simpls_scratch_doubles(p_, q_, wmax));
+    cand_.alloc(sizeof(double) * (size_t)p_ * bmax);
+    cand_scaled_.alloc(sizeof(double) * (size_t)p_ * bmax);
+    t_raw_.alloc(sizeof(double) * (size_t)n_ * bmax);
+    const int xs = splits_for(xw_row_tiles(n_), p_, 32);
+    const int ls = splits_for(xtw_col_tiles(p_), n_, 256);
+    ts_part_.alloc(sizeof(double) * (size_t)xs * n_ * bmax);
+    ls_part_.alloc(sizeof(double) * (size_t)ls * p_ * bmax);
+
+    SimplsFitDev f{};
+    f.n = n_; f.p = p_; f.q = q_; f.A = A_;
+    f.power = power_; f.use_right_gram = right_gram ? 1 : 0; f.reorth = 0;
+    f.nsteps = int(steps.size()); f.bmax = bmax; f.ts_splits = xs; f.ls_splits = ls;
+    f.steps = steps_.as<RefreshStep>(); f.omega = omega_.d();
+    f.S = S_.d(); f.S0 = S0_.d(); f.G = right_gram ? G_.d() : nullptr; f.sx = sx_.d();
+    f.cand = cand_.d(); f.cand_scaled = cand_scaled_.d(); f.t_raw = t_raw_.d();
+    f.ts_part = ts_part_.d(); f.ls_part = ls_part_.d();
+    f.R = R_.d(); f.Q = Q_.d(); f.V = V_.d(); f.T = Tsc_.d();
+    f.scratch = simpls_scratch_.d(); f.state = state_.as<int>();
+    fitdev_.alloc(sizeof(SimplsFitDev));
+    FPLS_HIP(hipMemcpy(fitdev_.as<SimplsFitDev>(), &f, sizeof(f), hipMemcpyHostToDevice));
+
+    FPLS_HIP(hipEventRecord(sc0_.e, stream_));
+    for (size_t s = 0; s < steps.size(); ++s) {
+      const int b = steps[s].b;
+      launch_simpls_refresh(fitdev_.as<SimplsFitDev>(), 1, int(s), stream_);
+      // scores of the candidate block: X_s C = (X - 1 mᵀ)(C / s)
+      XwTask<T> xw{};
+      xw.A = X_; xw.m = n_; xw.k = p_; xw.center = mx_.d(); xw.W = cand_scaled_.d(); xw.ldw = p_;
+      xw.N = b; xw.splits = xs; xw.out = ts_part_.d(); xw.ldo = n_; xw.slab = (long long)n_ * b;
+      launch_xw_single<T>(xw, stream_);
+      launch_sum_slabs(ts_part_.d(), xs, (long long)n_ * b, (long long)n_ * b, t_raw_.d(), stream_);
+      // loadings: (X - 1 mᵀ)ᵀ t  (divided by s in the update)
+      XtwTask<T> xt{};
+      xt.A = X_; xt.m = n_; xt.k = p_; xt.center = mx_.d(); xt.W = t_raw_.d(); xt.ldw = n_;
+      xt.N = b; xt.mode = kCentered; xt.splits = ls; xt.out = ls_part_.d(); xt.ldo = p_;
+      xt.slab = (long long)p_ * b;
+      launch_xtw_single<T>(xt, stream_);
+      launch_simpls_update(fitdev_.as<SimplsFitDev>(), 1, int(s), stream_);
+    }
+    FPLS_HIP(hipEventRecord(sc1_.e, stream_));
+  }
+
+  // ----------------------------------------------------------- PLS-SVD
+  // plssvd.hpp:197-220 -> rsvd.hpp:185-237 -> plssvd.hpp:57-166.
+  void fit_plssvd() {
+    const int k = A_;
+    const int maxdim = std::min(p_, q_);
+    const int width = std::min(maxdim, k + oversample_);
+    R_.alloc(sizeof(double) * (size_t)p_ * k);
+    Q_.alloc(sizeof(double) * (size_t)q_ * k);
+    sigma_.alloc(sizeof(double) * k);
+    Buf usable;
+    usable.alloc(sizeof(int));
+    int* flags = state_.as<int>() + 2;
+    if (width >= maxdim) {
+      dense_svd(k, flags);
+      const int kk = k;
+      FPLS_HIP(hipMemcpy(usable.as<int>(), &kk, sizeof(int), hipMemcpyHostToDevice));
+    } else {
+      randomized_svd(k, width, usable.as<int>(), flags);
+    }
+    FPLS_HIP(hipStreamSynchronize(stream_));
+    int got = 0;
+    FPLS_HIP(hipMemcpy(&got, usable.as<int>(), sizeof(int), hipMemcpyDeviceToHost));
+    if (got < k) throw std::runtime_error("fastPLS PLS-SVD returned fewer components than requested");
+    // scores T = X_s R, Gram TᵀT, prefix weights
+    Rs_.alloc(sizeof(double) * (size_t)p_ * k);
+    launch_div_rows(R_.d(), sx_.d(), p_, k, Rs_.d(), stream_);
+    Tsc_.alloc(sizeof(double) * (size_t)n_ * k);
+    {
+      // scores T = (X - 1 mᵀ)(R / s): the second and last pass over X
+      const int splits = splits_for(xw_row_tiles(n_), p_, 32);
+      part_.reserve(sizeof(double) * (size_t)splits * n_ * k);
+      XwTask<T> t{};
+      t.A = X_; t.m = n_; t.k = p_; t.center = mx_.d(); t.W = Rs_.d(); t.ldw = p_; t.N = k;
+      t.splits = splits; t.out = splits == 1 ? Tsc_.d() : part_.d(); t.ldo = n_;
+      t.slab = (long long)n_ * k;
+      FPLS_HIP(hipEventRecord(sc0_.e, stream_));
+      launch_xw_single<T>(t, stream_);
+      FPLS_HIP(hipEventRecord(sc1_.e, stream_));
+      if (splits > 1)
+        launch_sum_slabs(part_.d(), splits, (long long)n_ * k, (long long)n_ * k, Tsc_.d(), stream_);
+    }
+    gram_.alloc(sizeof(double) * (size_t)k * k);
+    tproduct<double>(Tsc_.d(), n_, k, nullptr, Tsc_.d(), n_, k, gram_.d());
+    W_.alloc(sizeof(double) * (size_t)k * k * q_);
+    launch_plssvd_weights(gram_.d(), sigma_.d(), Q_.d(), k, q_, W_.d(), flags, stream_);
+  }
+
+  void randomized_svd(int k, int w, int* usable, int* flags) {
+    std::vector<double> om((size_t)q_ * w);
+    gaussian(unsigned(seed_), om.size(), om.data());
+    tmp1_.alloc(sizeof(double) * om.size());
+    FPLS_HIP(hipMemcpy(tmp1_.d(), om.data(), sizeof(double) * om.size(), hipMemcpyHostToDevice));
+    Buf sample, basis, rev, rbasis, P, Up, Vt, scratch;
+    sample.alloc(sizeof(double) * (size_t)p_ * w);
+    basis.alloc(sizeof(double) * (size_t)p_ * w);
+    rev.alloc(sizeof(double) * (size_t)q_ * w);
+    rbasis.alloc(sizeof(double) * (size_t)q_ * w);
+    product<double>(S_.d(), p_, q_, nullptr, tmp1_.d(), q_, w, sample.d());
+    for (int it = 0; it < power_; ++it) {
+      launch_qr(sample.d(), p_, w, basis.d(), flags, stream_);
+      tproduct<double>(S_.d(), p_, q_, nullptr, basis.d(), p_, w, rev.d());
+      launch_qr(rev.d(), q_, w, rbasis.d(), flags, stream_);
+      product<double>(S_.d(), p_, q_, nullptr, rbasis.d(), q_, w, sample.d());
+    }
+    launch_qr(sample.d(), p_, w, basis.d(), flags, stream_);
+    P.alloc(sizeof(double) * (size_t)w * q_);
+    tproduct<double>(basis.d(), p_, w, nullptr, S_.d(), p_, q_, P.d());
+    Up.alloc(sizeof(double) * (size_t)w * k);
+    Vt.alloc(sizeof(double) * (size_t)k * q_);
+    const int mx = std::max(w, q_);
+    scratch.alloc(sizeof(double) * ((size_t)3 * w * w + w + q_ + 2 * (size_t)w * q_ +
+                                    (size_t)mx * mx + mx + 64));
+    launch_rsvd_finalize(P.d(), w, q_, k, Up.d(), sigma_.d(), Vt.d(), usable, scratch.d(), flags,
+                         stream_);
+    product<double>(basis.d(), p_, w, nullptr, Up.d(), w, k, R_.d());
+    launch_transpose(Vt.d(), k, q_, Q_.d(), stream_);
+    FPLS_HIP(hipStreamSynchronize(stream_));
+  }
+
+  void dense_svd(int k, int* flags) {
+    Buf Sc, Qs, Rm, Ur, Vr, scratch;
+    if (p_ >= q_) {
+      Sc.alloc(sizeof(double) * (size_t)p_ * q_);
+      FPLS_HIP(hipMemcpyAsync(Sc.d(), S_.d(), sizeof(double) * (size_t)p_ * q_, hipMemcpyDeviceToDevice, stream_));
+      Qs.alloc(sizeof(double) * (size_t)p_ * q_);
+      launch_qr(Sc.d(), p_, q_, Qs.d(), flags, stream_);
+      Rm.alloc(sizeof(double) * (size_t)q_ * q_);
+      tproduct<double>(Qs.d(), p_, q_, nullptr, S_.d(), p_, q_, Rm.d());
+      Ur.alloc(sizeof(double) * (size_t)q_ * k);
+      scratch.alloc(sizeof(double) * (3 * (size_t)q_ * q_ + q_ + 64));
+      launch_small_svd(Rm.d(), q_, q_, k, Ur.d(), sigma_.d(), Q_.d(), scratch.d(), flags, stream_);
+      product<double>(Qs.d(), p_, q_, nullptr, Ur.d(), q_, k, R_.d());
+    } else {
+      Sc.alloc(sizeof(double) * (size_t)p_ * q_);
+      launch_transpose(S_.d(), p_, q_, Sc.d(), stream_);  // q x p
+      Qs.alloc(sizeof(double) * (size_t)q_ * p_);
+      Buf St;
+      St.alloc(sizeof(double) * (size_t)q_ * p_);
+      FPLS_HIP(hipMemcpyAsync(St.d(), Sc.d(), sizeof(double) * (size_t)p_ * q_, hipMemcpyDeviceToDevice, stream_));
+      launch_qr(Sc.d(), q_, p_, Qs.d(), flags, stream_);
+      Rm.alloc(sizeof(double) * (size_t)p_ * p_);
+      tproduct<double>(Qs.d(), q_, p_, nullptr, St.d(), q_, p_, Rm.d());
+      Ur.alloc(sizeof(double) * (size_t)p_ * k);
+      scratch.alloc(sizeof(double) * (3 * (size_t)p_ * p_ + p_ + 64));
+      // Sᵀ = Qs Ur s Vrᵀ  =>  S = Vr s (Qs Ur)ᵀ
+      launch_small_svd(Rm.d(), p_, p_, k, Ur.d(), sigma_.d(), R_.d(), scratch.d(), flags, stream_);
+      product<double>(Qs.d(), q_, p_, nullptr, Ur.d(), p_, k, Q_.d());
+      FPLS_HIP(hipStreamSynchronize(stream_));
+    }
+    FPLS_HIP(hipStreamSynchronize(stream_));
+  }
+
+  // ------------------------------------------------------------ common
+  void finish_fit() {
+    // prediction operands: R / s (p x A) and Qᵀ is read with strides
+    if (method_ == kSimpls) {
+      Rs_.alloc(sizeof(double) * (size_t)p_ * A_);
+      launch_div_rows(R_.d(), sx_.d(), p_, A_, Rs_.d(), stream_);
+    }
+  }
+
+  void check_state() {
+    int st[4] = {0, 0, 0, 0};
+    FPLS_HIP(hipMemcpyAsync(st, state_.as<int>(), sizeof(st), hipMemcpyDeviceToHost, stream_));
+    FPLS_HIP(hipStreamSynchronize(stream_));
+    if (st[2] & kFlagBadLabel) throw std::runtime_error("resident fit failed: labels must lie in 1..q");
+    if (st[2] & kFlagEmptyClass) throw std::runtime_error("resident fit failed: a class has no samples");
+    if (st[2] & kFlagNonFinite) throw std::runtime_error("resident fit failed: non-finite values in the inputs or components");
+    if (st[2] & kFlagDecomposition) throw std::runtime_error("resident fit failed: decomposition or score-Gram solve failed");
+    if (method_ == kSimpls && st[0] < A_)
+      throw std::runtime_error("fastPLS SIMPLS returned fewer components than requested");
+  }
+
+  void check_path(const int* prefixes, int count) {
+    if (!prefixes || count < 1) throw std::invalid_argument("empty component path");
+    int previous = 0;
+    for (int j = 0; j < count; ++j) {
+      if (prefixes[j] <= previous || prefixes[j] > A_)
+        throw std::invalid_argument("component path must be strictly increasing");
+      previous = prefixes[j];
+    }
+  }
+
+  void load_rows(const T* x, int rows) {
+    rows_.reserve(sizeof(T) * (size_t)rows * p_);
+    FPLS_HIP(hipMemcpyAsync(rows_.as<T>(), x, sizeof(T) * (size_t)rows * p_, hipMemcpyHostToDevice, stream_));
+  }
+
+  // scores (rows x prefix) = ((x - m) / s) R[:, :prefix]
+  void project_rows(int rows, int prefix) {
+    scores_.reserve(sizeof(double) * (size_t)rows * prefix);
+    product<T>(rows_.as<T>(), rows, p_, mx_.d(), Rs_.d(), p_, prefix, scores_.d());
+    scores_rows_ = rows;
+  }
+
+  // Responses (or LDA discriminants) of one prefix from scores_, FP64 into
+  // pred64_, converted copy to host `out` (rows x q) when non-null.
+  void response_block(int rows, int prefix, bool lda, T* out, void*) {
+    pred64_.reserve(sizeof(double) * (size_t)rows * q_);
+    if (lda) {
+      ensure_lda(prefix);
+      // disc = T_k linearᵀ + constants ; linear is K x k (ld K)
+      launch_apply_small<T>(scores_.d(), rows, prefix, lda_linear_.d(), q_, 1, q_,
+                            lda_constants_.d(), nullptr, pred64_.d(), stream_);
+    } else if (method_ == kSimpls) {
+      // Ŷ = T_k Q_kᵀ + mean_y ; Q is q x A (ld q)
+      launch_apply_small<T>(scores_.d(), rows, prefix, Q_.d(), q_, 1, q_, my_.d(), nullptr,
+                            pred64_.d(), stream_);
+    } else {
+      // Ŷ = T_k W_k + mean_y ; W_k is k x q (ld k)
+      const double* Wk = W_.d() + (size_t)(prefix - 1) * A_ * q_;
+      launch_apply_small<T>(scores_.d(), rows, prefix, Wk, 1, prefix, q_, my_.d(), nullptr,
+                            pred64_.d(), stream_);
+    }
+    if (out) {
+      outT_.reserve(sizeof(T) * (size_t)rows * q_);
+      launch_convert<T>(pred64_.d(), outT_.as<T>(), (long long)rows * q_, stream_);
+      FPLS_HIP(hipMemcpyAsync(out, outT_.as<T>(), sizeof(T) * (size_t)rows * q_,
+                              hipMemcpyDeviceToHost, stream_));
+      // outT_ is reused by the next block: keep the copy ordered before it
+      FPLS_HIP(hipStreamSynchronize(stream_));
+    }
+  }
+
+  // --------------------------------------------------------------- LDA
+  // lda.hpp:156-223 through the moments TᵀT and class sums of T.
+  void ensure_lda_moments() {
+    if (!lda_gram_.empty()) return;
+    if (Tsc_.empty() || H_.empty()) throw std::runtime_error("LDA moments need the training scores and labels");
+    lda_gram_.alloc(sizeof(double) * (size_t)A_ * A_);
+    tproduct<double>(Tsc_.d(), n_, A_, nullptr, Tsc_.d(), n_, A_, lda_gram_.d());
+    lda_sums_.alloc(sizeof(double) * (size_t)q_ * A_);
+    tproduct<double>(H_.d(), n_, q_, nullptr, Tsc_.d(), n_, A_, lda_sums_.d());
+    FPLS_HIP(hipStreamSynchronize(stream_));
+  }
+
+  void ensure_lda(int k) {
+    if (!classification_) throw std::invalid_argument("LDA requires class labels");
+    ensure_lda_moments();
+    if (lda_prefix_ == k) return;
+    lda_linear_.reserve(sizeof(double) * (size_t)q_ * A_);
+    lda_constants_.reserve(sizeof(double) * q_);
+    lda_ridge_.reserve(sizeof(double));
+    lda_scratch_.reserve(sizeof(double) * (size_t)q_ * A_);
+    Buf fl;
+    fl.alloc(sizeof(int));
+    fill_zero_int(fl.as<int>(), 1);
+    launch_lda_prepare(lda_gram_.d(), lda_sums_.d(), counts_.as<int>(), n_, A_, q_, k,
+                       lda_linear_.d(), lda_constants_.d(), lda_ridge_.d(), lda_scratch_.d(),
+                       fl.as<int>(), stream_);
+    int bad = 0;
+    FPLS_HIP(hipMemcpyAsync(&bad, fl.as<int>(), sizeof(int), hipMemcpyDeviceToHost, stream_));
+    FPLS_HIP(hipStreamSynchronize(stream_));
+    if (bad) throw std::runtime_error("resident LDA Cholesky failed for every regularization value");
+    lda_prefix_ = k;
+  }
+
+  // ---------------------------------------------------------- variance
+  void ensure_variance() {
+    if (!var_ss_.empty()) return;
+    if (X_ == nullptr || Tsc_.empty()) throw std::runtime_error("variance needs the training data; export it before compact");
+    Buf cross, gram, seq;
+    cross.alloc(sizeof(double) * (size_t)p_ * A_);
+    tproduct<T>(X_, n_, p_, mx_.d(), Tsc_.d(), n_, A_, cross.d());
+    launch_div_rows(cross.d(), sx_.d(), p_, A_, cross.d(), stream_);
+    gram.alloc(sizeof(double) * (size_t)A_ * A_);
+    tproduct<double>(Tsc_.d(), n_, A_, nullptr, Tsc_.d(), n_, A_, gram.d());
+    seq.alloc(sizeof(double) * (size_t)p_ * A_);
+    var_loadings_.alloc(sizeof(double) * (size_t)p_ * A_);
+    var_ss_.alloc(sizeof(double) * (A_ + 1));
+    launch_variance(cross.d(), gram.d(), colss_.d(), p_, A_, var_loadings_.d(), var_ss_.d(),
+                    seq.d(), stream_);
+    FPLS_HIP(hipStreamSynchronize(stream_));
+  }
+
+  // ------------------------------------------------------------ members
+  int method_, n_, p_, q_, A_ = 0, scaling_, oversample_, power_;
+  unsigned long long seed_;
+  bool classification_;
+  int block_ = 1;
+  bool compacted_ = false;
+  double fit_seconds_ = 0.0, crossprod_seconds_ = 0.0, score_seconds_ = 0.0;
+  Event cp0_, cp1_, sc0_, sc1_;
+  hipStream_t stream_ = nullptr;
+  const T* X_ = nullptr;
+  Buf xown_, yown_, labels_dev_, counts_, Yc_, H_, ysum_, my_, mx_, sx_, colss_, S_, S0_, G_;
+  Buf R_, Q_, V_, Tsc_, Rs_, sigma_, gram_, W_, state_;
+  Buf omega_, steps_, fitdev_, simpls_scratch_, cand_, cand_scaled_, t_raw_, ts_part_, ls_part_;
+  Buf part_, tmp1_, tmp2_, tmp3_;
+  Buf rows_, scores_, pred64_, outT_, top_, metric_, flags_, obs_, obs_labels_;
+  Buf lda_gram_, lda_sums_, lda_linear_, lda_constants_, lda_ridge_, lda_scratch_;
+  Buf var_loadings_, var_ss_;
+  int lda_prefix_ = 0;
+  int scores_rows_ = 0;
+};
+
+void write_error(char* out, size_t cap, const char* message) noexcept {
+  if (out && cap) std::snprintf(out, cap, "%s", message);
+}
+
+template<class F>
+int guarded(char* err, size_t cap, F&& f) noexcept {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    write_error(err, cap, e.what());
+  } catch (...) {
+    write_error(err, cap, "unknown fastPLS HIP failure");
+  }
+  return 1;
+}
+
+void require_device() {
+  int count = 0;
+  const hipError_t st = hipGetDeviceCount(&count);
+  if (st != hipSuccess || count < 1)
+    throw std::runtime_error("no HIP device is available (the HIP backend has no CPU fallback)");
+}
+
+Handle* create(int method, const void* x, const void* y, const int* labels, int precision,
+               bool device, int n, int p, int q, int components, int scaling, int oversample,
+               int power, unsigned long long seed) {
+  require_device();
+  if (precision == 32)
+    return new Model<float>(method, static_cast<const float*>(x), static_cast<const float*>(y),
+                            labels, device, n, p, q, components, scaling, oversample, power,
+                            seed);
+  if (precision == 64)
+    return new Model<double>(method, static_cast<const double*>(x),
+                             static_cast<const double*>(y), labels, device, n, p, q, components,
+                             scaling, oversample, power, seed);
+  throw std::invalid_argument("precision must be 32 or 64");
+}
+
+template<class T>
+void host_gemm(const T* left, const T* right, int rows, int inner, int cols, T* out) {
+  hipStream_t s = nullptr;
+  FPLS_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  Buf a, b, c, c64, part;
+  try {
+    a.alloc(sizeof(T) * (size_t)rows * inner);
+    b.alloc(sizeof(T) * (size_t)inner * cols);
+    Buf b64;
+    b64.alloc(sizeof(double) * (size_t)inner * cols);
+    c64.alloc(sizeof(double) * (size_t)rows * cols);
+    c.alloc(sizeof(T) * (size_t)rows * cols);
+    FPLS_HIP(hipMemcpyAsync(a.as<T>(), left, sizeof(T) * (size_t)rows * inner, hipMemcpyHostToDevice, s));
+    std::vector<double> r64((size_t)inner * cols);
+    for (size_t i = 0; i < r64.size(); ++i) r64[i] = double(right[i]);
+    FPLS_HIP(hipMemcpyAsync(b64.d(), r64.data(), sizeof(double) * r64.size(), hipMemcpyHostToDevice, s));
+    for (int c0 = 0; c0 < cols; c0 += 64) {
+      const int nb = std::min(64, cols - c0);
+      const int splits = splits_for(xw_row_tiles(rows), inner, 32);
+      part.reserve(sizeof(double) * (size_t)splits * rows * nb);
+      XwTask<T> t{};
+      t.A = a.as<T>(); t.m = rows; t.k = inner; t.center = nullptr;
+      t.W = b64.d() + (size_t)c0 * inner; t.ldw = inner; t.N = nb; t.splits = splits;
+      t.out = part.d(); t.ldo = rows; t.slab = (long long)rows * nb;
+      launch_xw_single<T>(t, s);
+      launch_sum_slabs(part.d(), splits, (long long)rows * nb, (long long)rows * nb,
+                       c64.d() + (size_t)c0 * rows, s);
+      FPLS_HIP(hipStreamSynchronize(s));
+    }
+    launch_convert<T>(c64.d(), c.as<T>(), (long long)rows * cols, s);
+    FPLS_HIP(hipMemcpyAsync(out, c.as<T>(), sizeof(T) * (size_t)rows * cols, hipMemcpyDeviceToHost, s));
+    FPLS_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+    throw;
+  }
+  (void)hipStreamDestroy(s);
+}
+
+}  // namespace
+}  // namespace fpls
+
+using fpls::Handle;
+
+extern "C" {
+
+int fastpls_hip_device_info(char* out, size_t capacity) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1) {
+    fpls::write_error(out, capacity, "no HIP device");
+    return 1;
+  }
+  hipDeviceProp_t prop{};
+  if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
+    fpls::write_error(out, capacity, "hipGetDeviceProperties failed");
+    return 1;
+  }
+  if (out && capacity)
+    std::snprintf(out, capacity, "%s (%s), %d CUs, %.1f GiB", prop.name, prop.gcnArchName,
+                  prop.multiProcessorCount, double(prop.totalGlobalMem) / (1024.0 * 1024.0 * 1024.0));
+  return 0;
+}
+
+int fastpls_hip_gemm(const void* left, const void* right, int precision, int rows, int inner,
+                     int columns, void* output, char* error, size_t error_capacity) {
+  return fpls::guarded(error, error_capacity, [&] {
+    fpls::require_device();
+    if (!left || !right || !output || rows < 1 || inner < 1 || columns < 1)
+      throw std::invalid_argument("invalid GEMM arguments");
+    if (precision == 32)
+      fpls::host_gemm<float>(static_cast<const float*>(left), static_cast<const float*>(right),
+                             rows, inner, columns, static_cast<float*>(output));
+    else if (precision == 64)
+      fpls::host_gemm<double>(static_cast<const double*>(left), static_cast<const double*>(right),
+                              rows, inner, columns, static_cast<double*>(output));
+    else
+      throw std::invalid_argument("precision must be 32 or 64");
+  });
+}
+
+void* fastpls_hip_simpls_create(const void* x, const void* y, const int* labels, int precision,
+                                int n, int p, int q, int components, int scaling, int oversample,
+                                int power, int retain_scores, unsigned long long seed,
+                                char* error, size_t error_capacity) {
+  (void)retain_scores;
+  Handle* h = nullptr;
+  fpls::guarded(error, error_capacity, [&] {
+    h = fpls::create(0, x, y, labels, precision, false, n, p, q, components, scaling,
+                     oversample, power, seed);
+  });
+  return h;
+}
+
+void* fastpls_hip_plssvd_create(const void* x, const void* y, const int* labels, int precision,
+                                int n, int p, int q, int components, int scaling, int oversample,
+                                int power, int retain_scores, unsigned long long seed,
+                                char* error, size_t error_capacity) {
+  (void)retain_scores;
+  Handle* h = nullptr;
+  fpls::guarded(error, error_capacity, [&] {
+    h = fpls::create(1, x, y, labels, precision, false, n, p, q, components, scaling,
+                     oversample, power, seed);
+  });
+  return h;
+}
+
+void* fastpls_hip_opls_create(const void*, const void*, const int*, int, int, int, int, int, int,
+                              int, int, int, unsigned long long, int, char* error,
+                              size_t error_capacity) {
+  fpls::write_error(error, error_capacity, "OPLS is not supported by the HIP backend");
+  return nullptr;
+}
+
+void* fastpls_hip_kernelpls_create(const void*, const void*, const int*, int, int, int, int, int,
+                                   int, int, int, int, unsigned long long, int, double, int,
+                                   double, char* error, size_t error_capacity) {
+  fpls::write_error(error, error_capacity, "kernel PLS is not supported by the HIP backend");
+  return nullptr;
+}
+
+void* fastpls_hip_fit_device(int method, const void* x_device, const void* y_device,
+                             const int* labels_device, int precision, int n, int p, int q,
+                             int components, int scaling, int oversample, int power,
+                             unsigned long long seed, char* error, size_t error_capacity) {
+  Handle* h = nullptr;
+  fpls::guarded(error, error_capacity, [&] {
+    if (method != 0 && method != 1) throw std::invalid_argument("method must be 0 (SIMPLS) or 1 (PLS-SVD)");
+    h = fpls::create(method, x_device, y_device, labels_device, precision, true, n, p, q,
+                     components, scaling, oversample, power, seed);
+  });
+  return h;
+}
+
+int fastpls_hip_predict_path(void* model, const void* x, int rows, const int* prefixes,
+                             int prefix_count, int lda, void* predictions, char* error,
+                             size_t error_capacity) {
+  return fpls::guarded(error, error_capacity, [&] {
+    if (!model) throw std::invalid_argument("null resident model");
+    static_cast<Handle*>(model)->predict_path(x, rows, prefixes, prefix_count, lda != 0, predictions);
+  });
+}
+
+void fastpls_hip_destroy(void* model) { delete static_cast<Handle*>(model); }
+
+int fastpls_hip_export(void* model, int field, void* out, size_t elements, char* error,
+                       size_t error_capacity) {
+  return fpls::guarded(error, error_capacity, [&] {
+    if (!model) throw std::invalid_argument("null resident model");
+    static_cast<Handle*>(model)->export_field(field, out, elements);
+  });
+}
+
+int fastpls_hip_classify_path(void* model, const void* x, int rows, const int* prefixes,
+                              int prefix_count, int lda, int top, int* labels, char* error,
+                              size_t error_capacity) {
+  return fpls::guarded(error, error_capacity, [&] {
+    if (!model) throw std::invalid_argument("null resident model");
+    static_cast<Handle*>(model)->classify_path(x, rows, prefixes, prefix_count, lda != 0, top,
+                                               labels, nullptr);
+  });
+}
+
+int fastpls_hip_classify_response_path(void* model, const void* x, int rows,
+                                       const int* prefixes, int prefix_count, int lda, int top,
+                                       int* labels, void* predictions, char* error,
+                                       size_t error_capacity) {
+  return fpls::guarded(error, error_capacity, [&] {
+    if (!model) throw std::invalid_argument("null resident model");
+    if (!predictions) throw std::invalid_argument("invalid resident classification-response path request");
+    static_cast<Handle*>(model)->classify_path(x, rows, prefixes, prefix_count, lda != 0, top,
+                                               labels, predictions);
+  });
+}
+
+int fastpls_hip_response_sums(void* model, const void* x, const void* y, const int* labels,
+                              int rows, int prefix, void* out, char* error,
+                              size_t error_capacity) {
+  return fpls::guarded(error, error_capacity, [&] {
+    if (!model) throw std::invalid_argument("null resident model");
+    static_cast<Handle*>(model)->response_sums(x, y, labels, rows, prefix, out);
+  });
+}
+
+int fastpls_hip_project(void* model, const void* x, int rows, int prefix, void* out, char* error,
+                        size_t error_capacity) {
+  return fpls::guarded(error, error_capacity, [&] {
+    if (!model) throw std::invalid_argument("null resident model");
+    static_cast<Handle*>(model)->project(x, rows, prefix, out);
+  });
+}
+
+int fastpls_hip_controls(void* model, int* oversample, int* power, int* block, int* block_limit,
+                         int* implicit_operator, int* predictor_crossprod_cache, char* error,
+                         size_t error_capacity) {
+  return fpls::guarded(error, error_capacity, [&] {
+    if (!model) throw std::invalid_argument("null resident model");
+    static_cast<Handle*>(model)->controls(oversample, power, block, block_limit,
+                                          implicit_operator, predictor_crossprod_cache);
+  });
+}
+
+int fastpls_hip_compact(void* model, int prepare_lda, char* error, size_t error_capacity) {
+  return fpls::guarded(error, error_capacity, [&] {
+    if (!model) throw std::invalid_argument("null resident model");
+    static_cast<Handle*>(model)->compact(prepare_lda != 0);
+  });
+}
+
+int fastpls_hip_last_fit_seconds(void* model, double* seconds, char* error,
+                                 size_t error_capacity) {
+  return fpls::guarded(error, error_capacity, [&] {
+    if (!model || !seconds) throw std::invalid_argument("null argument");
+    *seconds = static_cast<Handle*>(model)->last_fit_seconds();
+  });
+}
+
+int fastpls_hip_device_alloc(size_t bytes, void** ptr, char* error, size_t error_capacity) {
+  return fpls::guarded(error, error_capacity, [&] {
+    fpls::require_device();
+    if (!ptr) throw std::invalid_argument("null output pointer");
+    *ptr = nullptr;
+    FPLS_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+  });
+}
+
+int fastpls_hip_device_free(void* ptr, char* error, size_t error_capacity) {
+  return fpls::guarded(error, error_capacity, [&] {
+    if (ptr) FPLS_HIP(hipFree(ptr));
+  });
+}
+
+int fastpls_hip_memcpy(void* dst, const void* src, size_t bytes, int kind, char* error,
+                       size_t error_capacity) {
+  return fpls::guarded(error, error_capacity, [&] {
+    if (!dst || !src) throw std::invalid_argument("null copy pointer");
+    hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                    : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    FPLS_HIP(hipMemcpy(dst, src, bytes, k));
+  });
+}
+
+int fastpls_hip_fit_phases(void* model, double* out, int count, char* error,
+                           size_t error_capacity) {
+  return fpls::guarded(error, error_capacity, [&] {
+    if (!model || !out || count < 1) throw std::invalid_argument("null argument");
+    static_cast<Handle*>(model)->phases(out, count);
+  });
+}
+
+int fastpls_hip_latent_data(void* x_device, void* y_device, int precision, long long n, int p,
+                            int q, int factors, double decay, double noise,
+                            unsigned long long seed, char* error, size_t error_capacity) {
+  return fpls::guarded(error, error_capacity, [&] {
+    fpls::require_device();
+    if (!x_device || !y_device || n < 1 || p < 1 || q < 1 || factors < 1)
+      throw std::invalid_argument("invalid synthetic data request");
+    hipStream_t s = nullptr;
+    FPLS_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    fpls::Buf scratch, y64;
+    scratch.alloc(sizeof(double) * (size_t)(n + p + q) * factors);
+    y64.alloc(sizeof(double) * (size_t)n * q);
+    if (precision == 32) {
+      fpls::launch_latent_data<float>(static_cast<float*>(x_device), y64.d(), n, p, q, factors,
+                                      decay, noise, seed, scratch.d(), s);
+      fpls::launch_convert<float>(y64.d(), static_cast<float*>(y_device), n * q, s);
+    } else if (precision == 64) {
+      fpls::launch_latent_data<double>(static_cast<double*>(x_device), y64.d(), n, p, q,
+                                       factors, decay, noise, seed, scratch.d(), s);
+      fpls::launch_convert<double>(y64.d(), static_cast<double*>(y_device), n * q, s);
+    } else {
+      (void)hipStreamDestroy(s);
+      throw std::invalid_argument("precision must be 32 or 64");
+    }
+    FPLS_HIP(hipStreamSynchronize(s));
+    (void)hipStreamDestroy(s);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,142 @@
+// Descriptors and host launchers of the per-fit kernels (small_kernels.hip,
+// prep_kernels.hip). All small-matrix arithmetic is FP64.
+#pragma once
+
+#include "common.hpp"
+
+namespace fpls {
+
+// One refresh of SIMPLS directions: component index, candidate block,
+// sketch width and the offset of its Gaussian sketch in the sketch pool.
+struct RefreshStep {
+  int a;
+  int b;
+  int w;
+  int pad;
+  long long omega_off;
+};
+
+// Device-resident state of one SIMPLS fit (all pointers are device memory).
+struct SimplsFitDev {
+  int n, p, q, A;          // A = maximum number of components
+  int power, use_right_gram, reorth, nsteps;
+  int bmax, ts_splits, ls_splits, pad;
+  const RefreshStep* steps;
+  const double* omega;     // sketch pool
+  double* S;               // p x q, deflated in place
+  const double* S0;        // p x q, initial cross-covariance
+  double* G;               // q x q right Gram (use_right_gram)
+  const double* sx;        // p predictor scales
+  double* cand;            // p x bmax candidate directions
+  double* cand_scaled;     // p x bmax, cand / sx (operand of the score pass)
+  double* t_raw;           // n x bmax reduced scores (input of the loading pass)
+  double* ts_part;         // score-pass partials: ts_splits slabs of n x b
+  double* ls_part;         // loading-pass partials: ls_splits slabs of p x b
+  double* R;               // p x A
+  double* Q;               // q x A
+  double* V;               // p x A
+  double* T;               // n x A
+  double* scratch;         // refresh / update scratch
+  int* state;              // [0] completed, [1] stopped, [2] error flags
+};
+
+// Error flags written to state[2].
+enum FitFlags : int {
+  kFlagBadLabel = 1,
+  kFlagEmptyClass = 2,
+  kFlagNonFinite = 4,
+  kFlagDecomposition = 8,
+  kFlagZeroNorm = 16,
+};
+
+// Doubles of refresh scratch needed by one fit.
+long long simpls_scratch_doubles(int p, int q, int wmax);
+
+void launch_simpls_refresh(const SimplsFitDev* fits, int count, int step, hipStream_t s);
+void launch_simpls_update(const SimplsFitDev* fits, int count, int step, hipStream_t s);
+
+// ---- single-work-group helpers for the PLS-SVD randomized SVD
+// Householder QR: A (m x n) destroyed, Q (m x n) written. flags |= decomposition.
+void launch_qr(double* A, int m, int n, double* Q, int* flags, hipStream_t s);
+// Finalize a sketch: P (w x q) = Qbᵀ S is given; writes Up (w x k), sigma (k),
+// Vt (k x q) and *usable. Uses the eigen route when w <= q, the dense SVD
+// route otherwise (rsvd.hpp:85-165). scratch: 4*w*w + 2*w*q + w doubles.
+void launch_rsvd_finalize(const double* P, int w, int q, int k, double* Up,
+                          double* sigma, double* Vt, int* usable, double* scratch,
+                          int* flags, hipStream_t s);
+// Dense economy SVD of a small square or tall matrix M (m x n, m >= n), the
+// first k triplets: U (m x k), sigma (k), V (n x k, right vectors as columns).
+void launch_small_svd(const double* M, int m, int n, int k, double* U, double* sigma,
+                      double* V, double* scratch, int* flags, hipStream_t s);
+
+// PLS-SVD prediction weights for prefixes 1..A from G = TᵀT (A x A), sigma and
+// Q (q x A): slab k-1 of W (A*q doubles) holds the k x q weights (ld k).
+void launch_plssvd_weights(const double* G, const double* sigma, const double* Q,
+                           int A, int q, double* W, int* flags, hipStream_t s);
+
+// ---- preprocessing
+// Column means of Y (n x q, element T) and centred copy Yc (FP64) plus the
+// column sums of Yc (ysum) used by the shifted cross-product.
+template<class T>
+void launch_center_response(const T* Y, long long n, int q, double* mean, double* Yc,
+                            double* ysum, hipStream_t s);
+// Labels (one-based) -> class counts, priors, centred one-hot Yc, plain
+// one-hot H (may be null) and ysum; invalid labels / empty classes set flags.
+void launch_label_response(const int* labels, long long n, int K, int* counts,
+                           double* priors, double* Yc, double* H, double* ysum,
+                           int* flags, hipStream_t s);
+// From the stats/cross-product partials of a kShiftStats pass compute mean,
+// scale and S (p x q), and copy S into S0 when S0 is non-null.
+template<class T>
+void launch_finalize_crossprod(const T* X, long long n, int p, int q, int scaling,
+                               const double* stats_part, const double* s_part, int splits,
+                               const double* ysum, double* mean, double* scale,
+                               double* colss, double* S, double* S0, int* flags,
+                               hipStream_t s);
+// out[i] = in[i] / d[i % rows]  (scale rows of a rows x cols matrix)
+void launch_div_rows(const double* in, const double* d, long long rows, int cols,
+                     double* out, hipStream_t s);
+
+// ---- prediction / classification helpers
+// out (rows x q, element T and/or FP64) = Tsc[:, :k] (ld rows) * M + bias,
+// with M(l, c) = Mr[l * rs + c * cs].
+template<class T>
+void launch_apply_small(const double* Tsc, long long rows, int k, const double* Mr,
+                        long long rs, long long cs, int q, const double* bias, T* out,
+                        double* out64, hipStream_t s);
+// out (cols x rows) = inᵀ
+void launch_transpose(const double* in, long long rows, long long cols, double* out,
+                      hipStream_t s);
+// Per-row top-k (one-based indices), first maximum wins; values (rows x K)
+// plus optional per-class constants.
+void launch_topk(const double* values, long long rows, int K, const double* constants,
+                 int top, int* out, hipStream_t s);
+// Three sums per response (SSE, SS about training mean, SS about observed mean).
+template<class T>
+void launch_response_sums(const double* pred, const T* observed, const int* labels,
+                          const double* train_mean, long long n, int q, double* out,
+                          int* flags, hipStream_t s);
+
+// LDA: from score Gram (a x a), class sums (K x a, ld K), counts and n,
+// prepare prefix k: linear (K x k), constants (K), ridge.
+void launch_lda_prepare(const double* gram, const double* class_sums, const int* counts,
+                        long long n, int a, int K, int k, double* linear,
+                        double* constants, double* ridge, double* scratch, int* flags,
+                        hipStream_t s);
+
+// Variance explained (cuda_resident_variance.cuh semantics) from cross =
+// X_sᵀT (p x a), gram = TᵀT and the standardized column sums of squares.
+void launch_variance(const double* cross, const double* gram, const double* colss,
+                     int p, int a, double* loadings, double* ss, double* scratch,
+                     hipStream_t s);
+
+// Synthetic latent-factor data on the device (Philox4x32-10 + Box-Muller).
+template<class T>
+void launch_latent_data(T* X, double* Y, long long n, int p, int q, int factors,
+                        double decay, double noise, unsigned long long seed,
+                        double* scratch, hipStream_t s);
+
+template<class T>
+void launch_convert(const double* in, T* out, long long count, hipStream_t s);
+
+}  // namespace fpls

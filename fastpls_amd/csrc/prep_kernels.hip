@@ -1,0 +1,413 @@
+// Preprocessing, prediction-side and data-generation kernels.
+#include "kernels.hpp"
+
+#include <cfloat>
+
+namespace fpls {
+
+namespace {
+
+// ------------------------------------------------------------ responses
+// supervised.hpp:54-61: response means; the centred copy is what the
+// shifted cross-product pass multiplies by. One work-group per column.
+template<class T>
+__global__ __launch_bounds__(kWG) void k_center_response(const T* Y, long long n, int q,
+                                                         double* mean, double* Yc,
+                                                         double* ysum) {
+  __shared__ double red[kWaves];
+  const int c = blockIdx.x;
+  const T* y = Y + (long long)c * n;
+  double s = 0.0;
+  for (long long i = threadIdx.x; i < n; i += kWG) s += double(y[i]);
+  const double mu = wg_sum(s, red) / double(n);
+  double cs = 0.0;
+  double* yc = Yc + (long long)c * n;
+  for (long long i = threadIdx.x; i < n; i += kWG) {
+    const double v = double(y[i]) - mu;
+    yc[i] = v;
+    cs += v;
+  }
+  cs = wg_sum(cs, red);
+  if (threadIdx.x == 0) { mean[c] = mu; ysum[c] = cs; }
+}
+
+__global__ __launch_bounds__(kWG) void k_fill_int(int* p, int count, int value) {
+  for (int i = blockIdx.x * kWG + threadIdx.x; i < count; i += gridDim.x * kWG) p[i] = value;
+}
+
+// classification.hpp:106-123: class counts (integer atomics: order-free).
+__global__ __launch_bounds__(kWG) void k_label_counts(const int* labels, long long n, int K,
+                                                      int* counts, int* flags) {
+  for (long long i = (long long)blockIdx.x * kWG + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kWG) {
+    const int l = labels[i];
+    if (l < 1 || l > K) atomicOr(flags, kFlagBadLabel);
+    else atomicAdd(&counts[l - 1], 1);
+  }
+}
+
+// Centred one-hot (column c: [label == c+1] - prior_c), plain one-hot and
+// priors; column sums of the centred one-hot go to ysum.
+__global__ __launch_bounds__(kWG) void k_label_fill(const int* labels, long long n, int K,
+                                                    const int* counts, double* priors,
+                                                    double* Yc, double* H, double* ysum,
+                                                    int* flags) {
+  __shared__ double red[kWaves];
+  const int c = blockIdx.x;
+  const double prior = double(counts[c]) / double(n);
+  if (threadIdx.x == 0) {
+    priors[c] = prior;
+    if (counts[c] == 0) atomicOr(flags, kFlagEmptyClass);
+  }
+  double s = 0.0;
+  for (long long i = threadIdx.x; i < n; i += kWG) {
+    const double h = labels[i] == c + 1 ? 1.0 : 0.0;
+    const double v = h - prior;
+    Yc[i + (long long)c * n] = v;
+    if (H) H[i + (long long)c * n] = h;
+    s += v;
+  }
+  s = wg_sum(s, red);
+  if (threadIdx.x == 0) ysum[c] = s;
+}
+
+// supervised.hpp:63-124 by shifted sums: d = x - x[0,j]; mean, the sum-of-
+// squares standard deviation with the reference's clamp and fallback, and
+// S[j,c] = (sum d*yc - (mean - shift) * sum yc) / scale.
+template<class T>
+__global__ __launch_bounds__(kWG) void k_finalize_crossprod(
+    const T* X, long long n, int p, int q, int scaling, const double* stats_part,
+    const double* s_part, int splits, const double* ysum, double* mean, double* scale,
+    double* colss, double* S, double* S0, int* flags) {
+  const int j = blockIdx.x * kWG + threadIdx.x;
+  if (j >= p) return;
+  double s1 = 0.0, s2 = 0.0;
+  for (int sp = 0; sp < splits; ++sp) {
+    s1 += stats_part[(long long)sp * 2 * p + j];
+    s2 += stats_part[(long long)sp * 2 * p + p + j];
+  }
+  const double shift = double(X[(long long)j * n]);
+  const double dn = double(n);
+  double mu = 0.0, sd = 1.0;
+  if (scaling != 3) {
+    mu = shift + s1 / dn;
+    if (scaling == 2) {
+      const double css = fmax(0.0, s2 - s1 * s1 / dn);
+      sd = sqrt(css / double(n > 1 ? n - 1 : 1));
+      if (!isfinite(sd) || sd <= 0.0) sd = 1.0;
+    }
+  }
+  mean[j] = mu;
+  scale[j] = sd;
+  const double delta = shift - mu;  // x - mu = d + delta
+  const double ss = s2 + 2.0 * delta * s1 + dn * delta * delta;
+  colss[j] = ss / (sd * sd);
+  bool finite = isfinite(mu) && isfinite(ss);
+  for (int c = 0; c < q; ++c) {
+    double acc = 0.0;
+    for (int sp = 0; sp < splits; ++sp) acc += s_part[(long long)sp * p * q + j + (long long)c * p];
+    const double v = (acc + delta * ysum[c]) / sd;
+    S[j + (long long)c * p] = v;
+    if (S0) S0[j + (long long)c * p] = v;
+    finite = finite && isfinite(v);
+  }
+  if (!finite) atomicOr(flags, kFlagNonFinite);
+}
+
+__global__ __launch_bounds__(kWG) void k_div_rows(const double* in, const double* d,
+                                                  long long rows, int cols, double* out) {
+  const long long total = rows * cols;
+  for (long long e = (long long)blockIdx.x * kWG + threadIdx.x; e < total;
+       e += (long long)gridDim.x * kWG) {
+    out[e] = in[e] / d[e % rows];
+  }
+}
+
+// ----------------------------------------------------------- prediction
+// out = Tsc[:, :k] * M + bias with M(l, c) = Mr[l * rs + c * cs], one thread
+// per output element.
+template<class T>
+__global__ __launch_bounds__(kWG) void k_apply_small(const double* Tsc, long long rows, int k,
+                                                     const double* Mr, long long rs,
+                                                     long long cs, int q,
+                                                     const double* bias, T* out,
+                                                     double* out64) {
+  const long long total = rows * q;
+  for (long long e = (long long)blockIdx.x * kWG + threadIdx.x; e < total;
+       e += (long long)gridDim.x * kWG) {
+    const long long i = e % rows;
+    const int c = int(e / rows);
+    double acc = 0.0;
+    for (int l = 0; l < k; ++l) acc = fma(Tsc[i + (long long)l * rows], Mr[l * rs + c * cs], acc);
+    if (bias) acc += bias[c];
+    if (out) out[e] = T(acc);
+    if (out64) out64[e] = acc;
+  }
+}
+
+// Per-row top-k with the insertion rule of cuda_resident_api.cuh:29-53: a
+// value enters only when strictly greater than the current last entry and
+// moves past strictly smaller ones, so the earlier class wins a tie.
+constexpr int kMaxTop = 32;
+__global__ __launch_bounds__(kWG) void k_topk(const double* values, long long rows, int K,
+                                              const double* constants, int top, int* out) {
+  const long long r = (long long)blockIdx.x * kWG + threadIdx.x;
+  if (r >= rows) return;
+  double bv[kMaxTop];
+  int bc[kMaxTop];
+  for (int t = 0; t < top; ++t) { bv[t] = -INFINITY; bc[t] = -1; }
+  for (int c = 0; c < K; ++c) {
+    double v = values[r + (long long)c * rows];
+    if (constants) v += constants[c];
+    if (v <= bv[top - 1]) continue;
+    int pos = top - 1;
+    while (pos > 0 && v > bv[pos - 1]) {
+      bv[pos] = bv[pos - 1];
+      bc[pos] = bc[pos - 1];
+      --pos;
+    }
+    bv[pos] = v;
+    bc[pos] = c + 1;
+  }
+  for (int t = 0; t < top; ++t) out[r + (long long)t * rows] = bc[t];
+}
+
+// cuda_resident_metrics.cuh:9-43, one work-group per response column.
+template<class T>
+__global__ __launch_bounds__(kWG) void k_response_sums(const double* pred, const T* observed,
+                                                       const int* labels,
+                                                       const double* train_mean, long long n,
+                                                       int q, double* out, int* flags) {
+  __shared__ double red[kWaves];
+  const int c = blockIdx.x;
+  int bad = 0;
+  double s = 0.0;
+  for (long long i = threadIdx.x; i < n; i += kWG) {
+    double y;
+    if (labels) {
+      const int l = labels[i];
+      if (l < 1 || l > q) bad = 1;
+      y = l == c + 1 ? 1.0 : 0.0;
+    } else {
+      y = double(observed[i + (long long)c * n]);
+    }
+    if (!isfinite(y)) bad = 1;
+    s += y;
+  }
+  const double mean = wg_sum(s, red) / double(n);
+  double sse = 0.0, tss = 0.0, oss = 0.0;
+  for (long long i = threadIdx.x; i < n; i += kWG) {
+    const double y = labels ? (labels[i] == c + 1 ? 1.0 : 0.0) : double(observed[i + (long long)c * n]);
+    const double pr = pred[i + (long long)c * n];
+    if (!isfinite(pr)) bad = 1;
+    const double e = y - pr, d = y - train_mean[c], v = y - mean;
+    sse = fma(e, e, sse); tss = fma(d, d, tss); oss = fma(v, v, oss);
+  }
+  sse = wg_sum(sse, red);
+  tss = wg_sum(tss, red);
+  oss = wg_sum(oss, red);
+  if (threadIdx.x == 0) {
+    out[3 * c] = sse; out[3 * c + 1] = tss; out[3 * c + 2] = oss;
+  }
+  if (bad) atomicOr(flags, kFlagNonFinite);
+}
+
+// ------------------------------------------------------ synthetic data
+// Philox4x32-10 (Salmon et al., SC'11) keyed by the seed; one counter per
+// element, Box-Muller on the first two words.
+__device__ inline void philox(unsigned c[4], unsigned k0, unsigned k1) {
+  const unsigned M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+  for (int r = 0; r < 10; ++r) {
+    const unsigned long long p0 = (unsigned long long)M0 * c[0];
+    const unsigned long long p1 = (unsigned long long)M1 * c[2];
+    const unsigned n0 = unsigned(p1 >> 32) ^ c[1] ^ k0;
+    const unsigned n2 = unsigned(p0 >> 32) ^ c[3] ^ k1;
+    c[0] = n0; c[1] = unsigned(p1); c[2] = n2; c[3] = unsigned(p0);
+    k0 += W0; k1 += W1;
+  }
+}
+
+__device__ inline double normal_at(unsigned long long seed, unsigned stream,
+                                   unsigned long long index) {
+  unsigned c[4] = {unsigned(index), unsigned(index >> 32), stream, 0x5eedu};
+  philox(c, unsigned(seed), unsigned(seed >> 32));
+  const double u1 = (double(c[0]) + 1.0) * (1.0 / 4294967297.0);
+  const double u2 = double(c[1]) * (1.0 / 4294967296.0);
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+// factors: T (n x F, column f scaled by decay^f), P (p x F), Q (q x F)
+__global__ __launch_bounds__(kWG) void k_latent_factors(double* Tl, double* Pl, double* Ql,
+                                                        long long n, int p, int q, int F,
+                                                        double decay,
+                                                        unsigned long long seed) {
+  const long long total = (n + p + q) * F;
+  for (long long e = (long long)blockIdx.x * kWG + threadIdx.x; e < total;
+       e += (long long)gridDim.x * kWG) {
+    if (e < n * F) {
+      const int f = int(e / n);
+      Tl[e] = normal_at(seed, 0, e) * pow(decay, double(f));
+    } else if (e < (n + p) * F) {
+      const long long o = e - n * F;
+      Pl[o] = normal_at(seed, 1, o);
+    } else {
+      const long long o = e - (n + p) * F;
+      Ql[o] = normal_at(seed, 2, o);
+    }
+  }
+}
+
+template<class T>
+__global__ __launch_bounds__(kWG) void k_latent_matrix(T* X, const double* Tl,
+                                                       const double* Pl, long long n,
+                                                       long long cols, int F, double noise,
+                                                       unsigned long long seed,
+                                                       unsigned stream) {
+  const long long total = n * cols;
+  for (long long e = (long long)blockIdx.x * kWG + threadIdx.x; e < total;
+       e += (long long)gridDim.x * kWG) {
+    const long long i = e % n, j = e / n;
+    double acc = 0.0;
+    for (int f = 0; f < F; ++f) acc = fma(Tl[i + (long long)f * n], Pl[j + (long long)f * cols], acc);
+    X[e] = T(acc + noise * normal_at(seed, stream, (unsigned long long)e));
+  }
+}
+
+// out (cols x rows) = inᵀ for a column-major rows x cols input.
+__global__ __launch_bounds__(kWG) void k_transpose(const double* in, long long rows,
+                                                   long long cols, double* out) {
+  const long long total = rows * cols;
+  for (long long e = (long long)blockIdx.x * kWG + threadIdx.x; e < total;
+       e += (long long)gridDim.x * kWG) {
+    const long long i = e % rows, j = e / rows;
+    out[j + i * cols] = in[e];
+  }
+}
+
+template<class T>
+__global__ __launch_bounds__(kWG) void k_convert(const double* in, T* out, long long count) {
+  for (long long e = (long long)blockIdx.x * kWG + threadIdx.x; e < count;
+       e += (long long)gridDim.x * kWG) out[e] = T(in[e]);
+}
+
+unsigned grid_for(long long count) {
+  long long b = (count + kWG - 1) / kWG;
+  if (b < 1) b = 1;
+  if (b > 65536) b = 65536;
+  return unsigned(b);
+}
+
+}  // namespace
+
+template<class T>
+void launch_center_response(const T* Y, long long n, int q, double* mean, double* Yc,
+                            double* ysum, hipStream_t s) {
+  hipLaunchKernelGGL((k_center_response<T>), dim3(q), dim3(kWG), 0, s, Y, n, q, mean, Yc, ysum);
+  FPLS_LAUNCHED("k_center_response");
+}
+
+void launch_label_response(const int* labels, long long n, int K, int* counts,
+                           double* priors, double* Yc, double* H, double* ysum, int* flags,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_int, dim3(grid_for(K)), dim3(kWG), 0, s, counts, K, 0);
+  hipLaunchKernelGGL(k_label_counts, dim3(grid_for(n)), dim3(kWG), 0, s, labels, n, K, counts,
+                     flags);
+  hipLaunchKernelGGL(k_label_fill, dim3(K), dim3(kWG), 0, s, labels, n, K, counts, priors, Yc,
+                     H, ysum, flags);
+  FPLS_LAUNCHED("k_label_fill");
+}
+
+template<class T>
+void launch_finalize_crossprod(const T* X, long long n, int p, int q, int scaling,
+                               const double* stats_part, const double* s_part, int splits,
+                               const double* ysum, double* mean, double* scale,
+                               double* colss, double* S, double* S0, int* flags,
+                               hipStream_t s) {
+  hipLaunchKernelGGL((k_finalize_crossprod<T>), dim3((p + kWG - 1) / kWG), dim3(kWG), 0, s, X,
+                     n, p, q, scaling, stats_part, s_part, splits, ysum, mean, scale, colss,
+                     S, S0, flags);
+  FPLS_LAUNCHED("k_finalize_crossprod");
+}
+
+void launch_div_rows(const double* in, const double* d, long long rows, int cols,
+                     double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_div_rows, dim3(grid_for(rows * cols)), dim3(kWG), 0, s, in, d, rows,
+                     cols, out);
+  FPLS_LAUNCHED("k_div_rows");
+}
+
+template<class T>
+void launch_apply_small(const double* Tsc, long long rows, int k, const double* Mr,
+                        long long rs, long long cs, int q, const double* bias, T* out,
+                        double* out64, hipStream_t s) {
+  hipLaunchKernelGGL((k_apply_small<T>), dim3(grid_for(rows * q)), dim3(kWG), 0, s, Tsc, rows,
+                     k, Mr, rs, cs, q, bias, out, out64);
+  FPLS_LAUNCHED("k_apply_small");
+}
+
+void launch_topk(const double* values, long long rows, int K, const double* constants,
+                 int top, int* out, hipStream_t s) {
+  if (top < 1 || top > kMaxTop || top > K) throw std::invalid_argument("top must lie in 1..min(classes, 32)");
+  hipLaunchKernelGGL(k_topk, dim3(unsigned((rows + kWG - 1) / kWG)), dim3(kWG), 0, s, values,
+                     rows, K, constants, top, out);
+  FPLS_LAUNCHED("k_topk");
+}
+
+template<class T>
+void launch_response_sums(const double* pred, const T* observed, const int* labels,
+                          const double* train_mean, long long n, int q, double* out,
+                          int* flags, hipStream_t s) {
+  hipLaunchKernelGGL((k_response_sums<T>), dim3(q), dim3(kWG), 0, s, pred, observed, labels,
+                     train_mean, n, q, out, flags);
+  FPLS_LAUNCHED("k_response_sums");
+}
+
+template<class T>
+void launch_latent_data(T* X, double* Y, long long n, int p, int q, int factors, double decay,
+                        double noise, unsigned long long seed, double* scratch,
+                        hipStream_t s) {
+  double* Tl = scratch;
+  double* Pl = Tl + n * factors;
+  double* Ql = Pl + (long long)p * factors;
+  hipLaunchKernelGGL(k_latent_factors, dim3(grid_for((n + p + q) * factors)), dim3(kWG), 0, s,
+                     Tl, Pl, Ql, n, p, q, factors, decay, seed);
+  hipLaunchKernelGGL((k_latent_matrix<T>), dim3(grid_for(n * p)), dim3(kWG), 0, s, X, Tl, Pl,
+                     n, (long long)p, factors, noise, seed, 3u);
+  hipLaunchKernelGGL((k_latent_matrix<double>), dim3(grid_for(n * q)), dim3(kWG), 0, s, Y, Tl,
+                     Ql, n, (long long)q, factors, noise, seed, 4u);
+  FPLS_LAUNCHED("k_latent_matrix");
+}
+
+void launch_transpose(const double* in, long long rows, long long cols, double* out,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_transpose, dim3(grid_for(rows * cols)), dim3(kWG), 0, s, in, rows, cols,
+                     out);
+  FPLS_LAUNCHED("k_transpose");
+}
+
+template<class T>
+void launch_convert(const double* in, T* out, long long count, hipStream_t s) {
+  hipLaunchKernelGGL((k_convert<T>), dim3(grid_for(count)), dim3(kWG), 0, s, in, out, count);
+  FPLS_LAUNCHED("k_convert");
+}
+
+#define FPLS_PREP_INSTANCES(T)                                                              \
+  template void launch_center_response<T>(const T*, long long, int, double*, double*,        \
+                                          double*, hipStream_t);                             \
+  template void launch_finalize_crossprod<T>(const T*, long long, int, int, int,             \
+                                             const double*, const double*, int,              \
+                                             const double*, double*, double*, double*,       \
+                                             double*, double*, int*, hipStream_t);           \
+  template void launch_apply_small<T>(const double*, long long, int, const double*,          \
+                                      long long, long long, int, const double*, T*, double*, \
+                                      hipStream_t);                                          \
+  template void launch_response_sums<T>(const double*, const T*, const int*, const double*,  \
+                                        long long, int, double*, int*, hipStream_t);         \
+  template void launch_latent_data<T>(T*, double*, long long, int, int, int, double, double, \
+                                      unsigned long long, double*, hipStream_t);             \
+  template void launch_convert<T>(const double*, T*, long long, hipStream_t);
+FPLS_PREP_INSTANCES(float)
+FPLS_PREP_INSTANCES(double)
+#undef FPLS_PREP_INSTANCES
+
+}  // namespace fpls

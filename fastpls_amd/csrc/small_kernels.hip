@@ -1,0 +1,620 @@
+// Per-fit FP64 kernels: one work-group owns one fit (or one small matrix).
+//
+// These carry the sequential parts of randomized SIMPLS and PLS-SVD: the
+// direction refresh (right-Gram randomized range finder, QR, eigen of the
+// reduced Gram), the component update (normalisation, loading
+// orthogonalisation, rank-one deflation of S and of the right Gram), the
+// finalisation of a randomized SVD sketch, the PLS-SVD prefix solves, the LDA
+// head and the variance-explained bookkeeping. The bulk passes over X run in
+// stream_kernels.hip between these launches.
+#include "kernels.hpp"
+#include "wg_linalg.hpp"
+
+#include <algorithm>
+#include <cfloat>
+
+namespace fpls {
+
+namespace {
+
+constexpr int kMaxW = 512;       // widest sketch / response count handled
+constexpr int kMaxPairs = kMaxW / 2 + 1;
+
+struct WgScratch {
+  double tau[kMaxW];
+  double dots[kMaxW];
+  double cs[kMaxPairs];
+  double sn[kMaxPairs];
+  int pa[kMaxPairs];
+  int pb[kMaxPairs];
+  double red[kWaves];
+};
+
+// Scratch layout of one SIMPLS fit (doubles).
+struct RefreshLayout {
+  long long Z, Y2, sample, Qb, Bm, Gm, Ve, vtmp, evals, wvec, proj, row, total;
+  __host__ __device__ RefreshLayout(int p, int q, int w, int A) {
+    long long o = 0;
+    Z = o; o += (long long)q * w;
+    Y2 = o; o += (long long)q * w;
+    sample = o; o += (long long)p * w;
+    Qb = o; o += (long long)p * w;
+    Bm = o; o += (long long)w * q;
+    Gm = o; o += (long long)w * w;
+    Ve = o; o += (long long)w * w;
+    vtmp = o; o += (long long)w * w;
+    evals = o; o += w;
+    wvec = o; o += p;
+    proj = o; o += (A > 0 ? A : 1);
+    row = o; o += q;
+    total = o;
+  }
+};
+
+__device__ inline void set_flag(int* state, int flag) {
+  if (threadIdx.x == 0) atomicOr(&state[2], flag);
+}
+
+// ----------------------------------------------------------- SIMPLS refresh
+// simpls.hpp:226-368 (refresh_directions), right-Gram and plain branches.
+__global__ __launch_bounds__(kWG) void k_simpls_refresh(const SimplsFitDev* fits, int count,
+                                                        int step) {
+  const SimplsFitDev& f = fits[blockIdx.x];
+  if (step >= f.nsteps) return;
+  if (f.state[1] != 0) return;
+  const RefreshStep st = f.steps[step];
+  const int p = f.p, q = f.q, w = st.w, b = st.b;
+  __shared__ WgScratch sh;
+  const RefreshLayout L(p, q, w, f.A);
+  double* base = f.scratch;
+  double* Z = base + L.Z;
+  double* Y2 = base + L.Y2;
+  double* sample = base + L.sample;
+  double* Qb = base + L.Qb;
+  double* Bm = base + L.Bm;
+  double* Gm = base + L.Gm;
+  double* Ve = base + L.Ve;
+  double* vtmp = base + L.vtmp;
+  double* ev = base + L.evals;
+  const double* omega = f.omega + st.omega_off;
+  bool ok = true;
+
+  if (f.use_right_gram) {
+    wg::copy(q, w, omega, q, Z, q);
+    for (int it = 0; it < f.power; ++it) {
+      wg::gemm(q, w, q, 1.0, f.G, q, false, Z, q, false, 0.0, Y2, q);
+      ok = wg::householder_qr(Y2, q, q, w, Z, q, sh.tau, sh.dots, sh.red) && ok;
+    }
+    if (f.power == 0) {
+      wg::copy(q, w, Z, q, Y2, q);
+      ok = wg::householder_qr(Y2, q, q, w, Z, q, sh.tau, sh.dots, sh.red) && ok;
+    }
+    wg::gemm(p, w, q, 1.0, f.S, p, false, Z, q, false, 0.0, sample, p);
+  } else {
+    wg::gemm(p, w, q, 1.0, f.S, p, false, omega, q, false, 0.0, sample, p);
+    for (int it = 0; it < f.power; ++it) {
+      ok = wg::householder_qr(sample, p, p, w, Qb, p, sh.tau, sh.dots, sh.red) && ok;
+      wg::gemm_tn_long(q, w, p, f.S, p, Qb, p, Y2, q);
+      ok = wg::householder_qr(Y2, q, q, w, Z, q, sh.tau, sh.dots, sh.red) && ok;
+      wg::gemm(p, w, q, 1.0, f.S, p, false, Z, q, false, 0.0, sample, p);
+    }
+  }
+  ok = wg::householder_qr(sample, p, p, w, Qb, p, sh.tau, sh.dots, sh.red) && ok;
+  wg::gemm_tn_long(w, q, p, Qb, p, f.S, p, Bm, w);
+  wg::gemm(w, w, q, 1.0, Bm, w, false, Bm, w, true, 0.0, Gm, w);
+  ok = wg::jacobi_eigen(Gm, Ve, w, ev, sh.cs, sh.sn, sh.pa, sh.pb, sh.red, vtmp) && ok;
+  // candidates = Qb * (top-b eigenvectors, largest first)
+  for (long long e = threadIdx.x; e < (long long)p * b; e += kWG) {
+    const int i = int(e % p), c = int(e / p);
+    const double* v = Ve + (long long)(w - 1 - c) * w;
+    double acc = 0.0;
+    for (int l = 0; l < w; ++l) acc = fma(Qb[i + (long long)l * p], v[l], acc);
+    f.cand[i + (long long)c * p] = acc;
+    f.cand_scaled[i + (long long)c * p] = acc / f.sx[i];
+  }
+  if (!ok) {
+    set_flag(f.state, kFlagDecomposition);
+    if (threadIdx.x == 0) f.state[1] = 1;
+  }
+}
+
+// ------------------------------------------------------------ SIMPLS update
+// simpls.hpp:588-756: per candidate of the block, in order.
+__global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits, int count,
+                                                       int step) {
+  const SimplsFitDev& f = fits[blockIdx.x];
+  if (step >= f.nsteps) return;
+  if (f.state[1] != 0) return;
+  const RefreshStep st = f.steps[step];
+  const int n = f.n, p = f.p, q = f.q;
+  __shared__ WgScratch sh;
+  const RefreshLayout L(p, q, st.w, f.A);
+  double* proj = f.scratch + L.proj;
+  double* row = f.scratch + L.row;
+
+  for (int j = 0; j < st.b; ++j) {
+    const int comp = st.a + j;
+    if (comp >= f.A) break;
+    const double* t = f.t_raw + (long long)j * n;
+    const double tn = wg::norm(t, n, sh.red);
+    if (!(tn > 0.0) || !isfinite(tn)) {
+      set_flag(f.state, isfinite(tn) ? kFlagZeroNorm : kFlagNonFinite);
+      if (threadIdx.x == 0) f.state[1] = 1;
+      return;
+    }
+    const double inv = 1.0 / tn;
+    double* Tc = f.T + (long long)comp * n;
+    for (long long i = threadIdx.x; i < n; i += kWG) Tc[i] = t[i] * inv;
+    double* Rc = f.R + (long long)comp * p;
+    double* Vc = f.V + (long long)comp * p;
+    for (int i = threadIdx.x; i < p; i += kWG) {
+      Rc[i] = f.cand[i + (long long)j * p] * inv;
+      double s = 0.0;
+      for (int sp = 0; sp < f.ls_splits; ++sp)
+        s += f.ls_part[(long long)sp * p * st.b + i + (long long)j * p];
+      Vc[i] = s / f.sx[i] * inv;
+    }
+    __syncthreads();
+    // response loadings q_a = S0ᵀ r
+    wg::gemm_tn_long(q, 1, p, f.S0, p, Rc, p, f.Q + (long long)comp * q, q);
+    // v = loading - V (Vᵀ loading), once (twice when re-orthogonalising)
+    if (comp > 0) {
+      for (int pass = 0; pass < (f.reorth ? 2 : 1); ++pass) {
+        wg::gemm_tn_long(comp, 1, p, f.V, p, Vc, p, proj, comp);
+        for (int i = threadIdx.x; i < p; i += kWG) {
+          double s = 0.0;
+          for (int l = 0; l < comp; ++l) s = fma(f.V[i + (long long)l * p], proj[l], s);
+          Vc[i] -= s;
+        }
+        __syncthreads();
+      }
+    }
+    const double vn = wg::norm(Vc, p, sh.red);
+    if (!(vn > 0.0) || !isfinite(vn)) {
+      set_flag(f.state, isfinite(vn) ? kFlagZeroNorm : kFlagNonFinite);
+      if (threadIdx.x == 0) f.state[1] = 1;
+      return;
+    }
+    const double vinv = 1.0 / vn;
+    for (int i = threadIdx.x; i < p; i += kWG) Vc[i] *= vinv;
+    __syncthreads();
+    // deflation row = vᵀ S ; S -= v row
+    wg::gemm_tn_long(q, 1, p, f.S, p, Vc, p, row, q);
+    for (long long e = threadIdx.x; e < (long long)p * q; e += kWG) {
+      const int i = int(e % p), c = int(e / p);
+      f.S[e] -= Vc[i] * row[c];
+    }
+    if (f.use_right_gram) {
+      for (int e = threadIdx.x; e < q * q; e += kWG) {
+        const int i = e % q, c = e / q;
+        f.G[e] -= row[i] * row[c];
+      }
+      __syncthreads();
+      for (int e = threadIdx.x; e < q * q; e += kWG) {
+        const int i = e % q, c = e / q;
+        if (i < c) {
+          const double v = 0.5 * (f.G[i + c * q] + f.G[c + i * q]);
+          f.G[i + c * q] = v;
+          f.G[c + i * q] = v;
+        }
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) f.state[0] = comp + 1;
+    __syncthreads();
+  }
+}
+
+// --------------------------------------------------------------- tall QR
+__global__ __launch_bounds__(kWG) void k_qr(double* A, int m, int n, double* Q, int* flags) {
+  __shared__ WgScratch sh;
+  const bool ok = wg::householder_qr(A, m, m, n, Q, m, sh.tau, sh.dots, sh.red);
+  if (!ok && threadIdx.x == 0) atomicOr(flags, kFlagDecomposition);
+}
+
+// ----------------------------------------------------- rSVD finalisation
+// rsvd.hpp:85-165: eigen route on P Pᵀ with the rank tolerance, else dense SVD.
+__global__ __launch_bounds__(kWG) void k_rsvd_finalize(const double* P, int w, int q, int k,
+                                                       double* Up, double* sigma, double* Vt,
+                                                       int* usable_out, double* scratch,
+                                                       int* flags) {
+  __shared__ WgScratch sh;
+  __shared__ int usable_sh;
+  double* Gm = scratch;
+  double* Ve = Gm + (long long)w * w;
+  double* vtmp = Ve + (long long)w * w;
+  double* ev = vtmp + (long long)w * w;
+  double* Wc = ev + w + q;                  // w x q copy (dense route)
+  double* Uc = Wc + (long long)w * q;       // w x q or q x w
+  double* Vm = Uc + (long long)w * q;       // up to max(w,q)^2
+  double* sv = Vm + (long long)(w > q ? w : q) * (w > q ? w : q);
+  int usable = 0;
+  if (w <= q) {
+    wg::gemm(w, w, q, 1.0, P, w, false, P, w, true, 0.0, Gm, w);
+    if (wg::jacobi_eigen(Gm, Ve, w, ev, sh.cs, sh.sn, sh.pa, sh.pb, sh.red, vtmp)) {
+      const double largest = fmax(ev[w - 1], 1.0);
+      const double tol = DBL_EPSILON * double(w > q ? w : q) * largest;
+      if (threadIdx.x == 0) {
+        int u = 0;
+        for (int idx = w; idx > 0; --idx) {
+          if (ev[idx - 1] <= tol || u == k) break;
+          ++u;
+        }
+        usable_sh = u;
+      }
+      __syncthreads();
+      usable = usable_sh;
+      if (usable > 0) {
+        for (int e = threadIdx.x; e < w * usable; e += kWG) {
+          const int i = e % w, c = e / w;
+          Up[i + c * w] = Ve[i + (w - 1 - c) * w];
+        }
+        for (int c = threadIdx.x; c < usable; c += kWG) sigma[c] = sqrt(fmax(ev[w - 1 - c], 0.0));
+        __syncthreads();
+        for (int e = threadIdx.x; e < usable * q; e += kWG) {
+          const int c = e % usable, j = e / usable;
+          double acc = 0.0;
+          for (int l = 0; l < w; ++l) acc = fma(Up[l + c * w], P[l + j * w], acc);
+          Vt[c + j * k] = acc / sigma[c];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) *usable_out = usable;
+        return;
+      }
+    }
+  }
+  // dense route: SVD of P (w x q)
+  if (w >= q) {
+    wg::copy(w, q, P, w, Wc, w);
+    const bool ok = wg::jacobi_svd(Wc, w, q, Uc, Vm, sv, sh.cs, sh.sn, sh.pa, sh.pb, sh.red, nullptr);
+    const int ret = k < q ? k : q;
+    for (int e = threadIdx.x; e < w * ret; e += kWG) Up[e % w + (e / w) * w] = Uc[e % w + (long long)(e / w) * w];
+    for (int c = threadIdx.x; c < ret; c += kWG) sigma[c] = sv[c];
+    for (int e = threadIdx.x; e < ret * q; e += kWG) {
+      const int c = e % ret, j = e / ret;
+      Vt[c + j * k] = Vm[j + c * q];
+    }
+    if (!ok && threadIdx.x == 0) atomicOr(flags, kFlagDecomposition);
+    __syncthreads();
+    if (threadIdx.x == 0) *usable_out = ret;
+  } else {
+    // wide P: decompose Pᵀ (q x w) = U' s V'ᵀ so P = V' s U'ᵀ
+    for (int e = threadIdx.x; e < w * q; e += kWG) {
+      const int i = e % w, j = e / w;
+      Wc[j + i * q] = P[i + j * w];
+    }
+    __syncthreads();
+    const bool ok = wg::jacobi_svd(Wc, q, w, Uc, Vm, sv, sh.cs, sh.sn, sh.pa, sh.pb, sh.red, nullptr);
+    const int ret = k < w ? k : w;
+    for (int e = threadIdx.x; e < w * ret; e += kWG) {
+      const int i = e % w, c = e / w;
+      Up[i + c * w] = Vm[i + c * w];
+    }
+    for (int c = threadIdx.x; c < ret; c += kWG) sigma[c] = sv[c];
+    for (int e = threadIdx.x; e < ret * q; e += kWG) {
+      const int c = e % ret, j = e / ret;
+      Vt[c + j * k] = Uc[j + c * q];
+    }
+    if (!ok && threadIdx.x == 0) atomicOr(flags, kFlagDecomposition);
+    __syncthreads();
+    if (threadIdx.x == 0) *usable_out = ret;
+  }
+}
+
+// Dense SVD of M (m x n, m >= n), first k triplets.
+__global__ __launch_bounds__(kWG) void k_small_svd(const double* M, int m, int n, int k,
+                                                   double* U, double* sigma, double* V,
+                                                   double* scratch, int* flags) {
+  __shared__ WgScratch sh;
+  double* Wc = scratch;
+  double* Uc = Wc + (long long)m * n;
+  double* Vm = Uc + (long long)m * n;
+  double* sv = Vm + (long long)n * n;
+  wg::copy(m, n, M, m, Wc, m);
+  const bool ok = wg::jacobi_svd(Wc, m, n, Uc, Vm, sv, sh.cs, sh.sn, sh.pa, sh.pb, sh.red, nullptr);
+  for (long long e = threadIdx.x; e < (long long)m * k; e += kWG) U[e] = Uc[e];
+  for (long long e = threadIdx.x; e < (long long)n * k; e += kWG) V[e] = Vm[e];
+  for (int c = threadIdx.x; c < k; c += kWG) sigma[c] = sv[c];
+  if (!ok && threadIdx.x == 0) atomicOr(flags, kFlagDecomposition);
+}
+
+// ------------------------------------------------------ PLS-SVD weights
+// plssvd.hpp:138-164 with linalg.hpp:57-177: Cholesky of the leading k x k
+// block of TᵀT (shared by every prefix), pivoted elimination as fallback.
+__global__ __launch_bounds__(kWG) void k_plssvd_weights(const double* G, const double* sigma,
+                                                        const double* Q, int A, int q,
+                                                        double* W, int* flags) {
+  __shared__ double Lm[64 * 64];
+  __shared__ double X[64 * 64];
+  __shared__ int fail_at;
+  double* F = Lm;  // the pivoted fallback reuses the Cholesky storage
+  const int tid = threadIdx.x;
+  if (A > 64) {
+    if (tid == 0) atomicOr(flags, kFlagDecomposition);
+    return;
+  }
+  // Cholesky, row by row as linalg.hpp:66-81 orders it; fail_at = first
+  // index whose pivot is not positive (prefixes beyond it need pivoting).
+  if (tid == 0) {
+    fail_at = A;
+    for (int i = 0; i < A && fail_at == A; ++i) {
+      for (int j = 0; j <= i; ++j) {
+        double v = G[i + j * A];
+        for (int l = 0; l < j; ++l) v -= Lm[i + l * 64] * Lm[j + l * 64];
+        if (i == j) {
+          if (!isfinite(v) || v <= 0.0) { fail_at = i; break; }
+          Lm[i + j * 64] = sqrt(v);
+        } else {
+          Lm[i + j * 64] = v / Lm[j + j * 64];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = 1; k <= A; ++k) {
+    double* Wk = W + (long long)(k - 1) * A * q;
+    bool solved = false;
+    if (k <= fail_at) {
+      // fail_at only shrinks, so once the fallback ran Lm is never read again
+      // columns of the right-hand side diag(sigma) in parallel
+      bool good = true;
+      for (int c = tid; c < k; c += kWG) {
+        for (int i = 0; i < k; ++i) {
+          double v = i == c ? sigma[c] : 0.0;
+          for (int l = 0; l < i; ++l) v -= Lm[i + l * 64] * X[l + c * 64];
+          X[i + c * 64] = v / Lm[i + i * 64];
+        }
+        for (int i = k - 1; i >= 0; --i) {
+          double v = X[i + c * 64];
+          for (int l = i + 1; l < k; ++l) v -= Lm[l + i * 64] * X[l + c * 64];
+          v /= Lm[i + i * 64];
+          X[i + c * 64] = v;
+          if (!isfinite(v)) good = false;
+        }
+      }
+      solved = __syncthreads_and(good ? 1 : 0) != 0;
+    }
+    if (!solved) {
+      // pivoted elimination (linalg.hpp:104-169), serial
+      if (tid == 0) {
+        fail_at = 0;
+        double scale = 0.0;
+        for (int j = 0; j < k; ++j)
+          for (int i = 0; i < k; ++i) {
+            F[i + j * 64] = G[i + j * A];
+            scale = fmax(scale, fabs(F[i + j * 64]));
+            X[i + j * 64] = i == j ? sigma[j] : 0.0;
+          }
+        const double tol = DBL_EPSILON * double(k) * fmax(scale, 1.0);
+        bool good = true;
+        for (int piv = 0; piv < k && good; ++piv) {
+          int sel = piv;
+          for (int i = piv + 1; i < k; ++i) if (fabs(F[i + piv * 64]) > fabs(F[sel + piv * 64])) sel = i;
+          if (!isfinite(F[sel + piv * 64]) || fabs(F[sel + piv * 64]) <= tol) { good = false; break; }
+          if (sel != piv) {
+            for (int c = 0; c < k; ++c) {
+              double t = F[piv + c * 64]; F[piv + c * 64] = F[sel + c * 64]; F[sel + c * 64] = t;
+              t = X[piv + c * 64]; X[piv + c * 64] = X[sel + c * 64]; X[sel + c * 64] = t;
+            }
+          }
+          for (int i = piv + 1; i < k; ++i) {
+            const double mlt = F[i + piv * 64] / F[piv + piv * 64];
+            F[i + piv * 64] = 0.0;
+            for (int c = piv + 1; c < k; ++c) F[i + c * 64] -= mlt * F[piv + c * 64];
+            for (int c = 0; c < k; ++c) X[i + c * 64] -= mlt * X[piv + c * 64];
+          }
+        }
+        for (int c = 0; c < k && good; ++c) {
+          for (int i = k - 1; i >= 0; --i) {
+            double v = X[i + c * 64];
+            for (int l = i + 1; l < k; ++l) v -= F[i + l * 64] * X[l + c * 64];
+            v /= F[i + i * 64];
+            X[i + c * 64] = v;
+            if (!isfinite(v)) good = false;
+          }
+        }
+        if (!good) atomicOr(flags, kFlagDecomposition);
+      }
+      __syncthreads();
+    }
+    // W_k = X (k x k) * Q_kᵀ  -> k x q, leading dimension k
+    for (int e = tid; e < k * q; e += kWG) {
+      const int i = e % k, c = e / k;
+      double acc = 0.0;
+      for (int l = 0; l < k; ++l) acc = fma(X[i + l * 64], Q[c + l * q], acc);
+      Wk[i + c * k] = acc;
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------- LDA
+// lda.hpp:86-152 / train_lda_prefixes_from_moments (lda.hpp:236-305).
+__global__ __launch_bounds__(kWG) void k_lda_prepare(const double* gram, const double* sums,
+                                                     const int* counts, long long n, int a,
+                                                     int K, int k, double* linear,
+                                                     double* constants, double* ridge_out,
+                                                     double* scratch, int* flags) {
+  __shared__ double cov[64 * 64];
+  __shared__ double Lm[64 * 64];
+  __shared__ int ok_sh;
+  __shared__ double red[kWaves];
+  const int tid = threadIdx.x;
+  double* means = scratch;  // K x k
+  if (k > 64 || k > a) {
+    if (tid == 0) atomicOr(flags, kFlagDecomposition);
+    return;
+  }
+  for (int e = tid; e < K * k; e += kWG) {
+    const int c = e % K, j = e / K;
+    means[c + j * K] = sums[c + j * K] / double(counts[c]);
+  }
+  __syncthreads();
+  const double den = double(n > K ? n - K : 1);
+  for (int e = tid; e < k * k; e += kWG) {
+    const int r = e % k, c = e / k;
+    double v = gram[r + c * a];
+    for (int cl = 0; cl < K; ++cl) v -= double(counts[cl]) * means[cl + r * K] * means[cl + c * K];
+    cov[r + c * 64] = v / den;
+  }
+  __syncthreads();
+  double tr = 0.0;
+  for (int j = tid; j < k; j += kWG) tr += cov[j + j * 64];
+  double scale = wg_sum(tr, red) / double(k);
+  if (!isfinite(scale) || scale <= 0.0) scale = 1.0;
+  const double grid[6] = {1e-8, 1e-6, 1e-5, 1e-4, 1e-3, 1e-2};
+  bool solved = false;
+  for (int gi = 0; gi < 6 && !solved; ++gi) {
+    const double ridge = grid[gi] * scale;
+    if (tid == 0) {
+      int good = 1;
+      for (int i = 0; i < k && good; ++i) {
+        for (int j = 0; j <= i; ++j) {
+          double v = cov[i + j * 64];
+          if (i == j) v += ridge;
+          for (int l = 0; l < j; ++l) v -= Lm[i + l * 64] * Lm[j + l * 64];
+          if (i == j) {
+            if (!isfinite(v) || v <= 0.0) { good = 0; break; }
+            Lm[i + j * 64] = sqrt(v);
+          } else {
+            const double dg = Lm[j + j * 64];
+            if (!isfinite(dg) || dg <= 0.0) { good = 0; break; }
+            Lm[i + j * 64] = v / dg;
+          }
+        }
+      }
+      ok_sh = good;
+    }
+    __syncthreads();
+    const int factored = ok_sh;
+    __syncthreads();  // every thread has read ok_sh before thread 0 rewrites it
+    if (!factored) continue;
+    // solve for each class (right-hand side = class mean row)
+    int good = 1;
+    for (int c = tid; c < K; c += kWG) {
+      double* x = linear + c;  // row c of linear (K x k), stride K
+      for (int i = 0; i < k; ++i) {
+        double v = means[c + i * K];
+        for (int l = 0; l < i; ++l) v -= Lm[i + l * 64] * x[(long long)l * K];
+        x[(long long)i * K] = v / Lm[i + i * 64];
+      }
+      for (int i = k - 1; i >= 0; --i) {
+        double v = x[(long long)i * K];
+        for (int l = i + 1; l < k; ++l) v -= Lm[l + i * 64] * x[(long long)l * K];
+        v /= Lm[i + i * 64];
+        if (!isfinite(v)) good = 0;
+        x[(long long)i * K] = v;
+      }
+    }
+    solved = __syncthreads_and(good) != 0;
+    if (solved && tid == 0) *ridge_out = ridge;
+  }
+  if (!solved) {
+    if (tid == 0) atomicOr(flags, kFlagDecomposition);
+    return;
+  }
+  for (int c = tid; c < K; c += kWG) {
+    const double prior = double(counts[c]) / double(n);
+    double prod = 0.0;
+    for (int j = 0; j < k; ++j) prod += means[c + j * K] * linear[c + (long long)j * K];
+    constants[c] = -0.5 * prod + log(fmax(prior, DBL_MIN));
+  }
+}
+
+// --------------------------------------------------------------- variance
+// cuda_resident_variance.cuh:10-57: loadings, sequential residual loadings,
+// per-component sums of squares and the total predictor sum of squares.
+__global__ __launch_bounds__(kWG) void k_variance(const double* cross, const double* gram,
+                                                  const double* colss, int p, int a,
+                                                  double* loadings, double* ss, double* seq) {
+  __shared__ double red[kWaves];
+  for (int f = threadIdx.x; f < p; f += kWG) {
+    for (int j = 0; j < a; ++j) {
+      const double den = gram[j + j * a];
+      double v = cross[f + (long long)j * p];
+      const bool good = isfinite(den) && den > 0.0;
+      loadings[f + (long long)j * p] = good ? v / den : 0.0;
+      for (int i = 0; i < j; ++i) v -= seq[f + (long long)i * p] * gram[i + j * a];
+      seq[f + (long long)j * p] = good ? v / den : 0.0;
+    }
+  }
+  __syncthreads();
+  double tot = 0.0;
+  for (int f = threadIdx.x; f < p; f += kWG) tot += colss[f];
+  tot = wg_sum(tot, red);
+  if (threadIdx.x == 0) ss[a] = tot;
+  for (int j = 0; j < a; ++j) {
+    double g = 0.0;
+    for (int f = threadIdx.x; f < p; f += kWG) g = fma(seq[f + (long long)j * p], seq[f + (long long)j * p], g);
+    g = wg_sum(g, red);
+    if (threadIdx.x == 0) {
+      const double gain = g * gram[j + j * a];
+      ss[j] = (isfinite(gain) && gain > 0.0) ? gain : 0.0;
+    }
+  }
+}
+
+}  // namespace
+
+long long simpls_scratch_doubles(int p, int q, int wmax) {
+  return RefreshLayout(p, q, wmax, p).total;
+}
+
+void launch_simpls_refresh(const SimplsFitDev* fits, int count, int step, hipStream_t s) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_simpls_refresh, dim3(count), dim3(kWG), 0, s, fits, count, step);
+  FPLS_LAUNCHED("k_simpls_refresh");
+}
+
+void launch_simpls_update(const SimplsFitDev* fits, int count, int step, hipStream_t s) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_simpls_update, dim3(count), dim3(kWG), 0, s, fits, count, step);
+  FPLS_LAUNCHED("k_simpls_update");
+}
+
+void launch_qr(double* A, int m, int n, double* Q, int* flags, hipStream_t s) {
+  if (n > kMaxW || m < n) throw std::invalid_argument("QR shape outside the supported range");
+  hipLaunchKernelGGL(k_qr, dim3(1), dim3(kWG), 0, s, A, m, n, Q, flags);
+  FPLS_LAUNCHED("k_qr");
+}
+
+void launch_rsvd_finalize(const double* P, int w, int q, int k, double* Up, double* sigma,
+                          double* Vt, int* usable, double* scratch, int* flags,
+                          hipStream_t s) {
+  if (w > kMaxW || q > kMaxW) throw std::invalid_argument("sketch too wide");
+  hipLaunchKernelGGL(k_rsvd_finalize, dim3(1), dim3(kWG), 0, s, P, w, q, k, Up, sigma, Vt,
+                     usable, scratch, flags);
+  FPLS_LAUNCHED("k_rsvd_finalize");
+}
+
+void launch_small_svd(const double* M, int m, int n, int k, double* U, double* sigma,
+                      double* V, double* scratch, int* flags, hipStream_t s) {
+  if (n > kMaxW || m < n) throw std::invalid_argument("small SVD shape outside the supported range");
+  hipLaunchKernelGGL(k_small_svd, dim3(1), dim3(kWG), 0, s, M, m, n, k, U, sigma, V, scratch,
+                     flags);
+  FPLS_LAUNCHED("k_small_svd");
+}
+
+void launch_plssvd_weights(const double* G, const double* sigma, const double* Q, int A,
+                           int q, double* W, int* flags, hipStream_t s) {
+  hipLaunchKernelGGL(k_plssvd_weights, dim3(1), dim3(kWG), 0, s, G, sigma, Q, A, q, W, flags);
+  FPLS_LAUNCHED("k_plssvd_weights");
+}
+
+void launch_lda_prepare(const double* gram, const double* class_sums, const int* counts,
+                        long long n, int a, int K, int k, double* linear,
+                        double* constants, double* ridge, double* scratch, int* flags,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_lda_prepare, dim3(1), dim3(kWG), 0, s, gram, class_sums, counts, n, a,
+                     K, k, linear, constants, ridge, scratch, flags);
+  FPLS_LAUNCHED("k_lda_prepare");
+}
+
+void launch_variance(const double* cross, const double* gram, const double* colss, int p,
+                     int a, double* loadings, double* ss, double* scratch, hipStream_t s) {
+  hipLaunchKernelGGL(k_variance, dim3(1), dim3(kWG), 0, s, cross, gram, colss, p, a,
+                     loadings, ss, scratch);
+  FPLS_LAUNCHED("k_variance");
+}
+
+}  // namespace fpls

@@ -1,0 +1,84 @@
+// Multi-work-group streaming products over a tall column-major matrix.
+//
+// Two shapes cover every pass over the predictor matrix X (and the same
+// shapes over the cross-covariance S in the randomized SVD):
+//   xw : out(m x N)  = (A - 1 cᵀ) · W       W is k x N      (scores, sketches)
+//   xtw: out(k x N)  = (A - 1 cᵀ)ᵀ · W      W is m x N      (XᵀY, loadings)
+// A is m x k, column-major, leading dimension m, element type T (float or
+// double). The optional centre c (length k) is subtracted element-wise as A
+// is loaded, so standardisation never needs its own pass over A.
+// Both kernels accept a task list so that many independent problems (fits of
+// a batch) run in one launch; blockIdx.x indexes a flattened tile space.
+#pragma once
+
+#include "common.hpp"
+
+namespace fpls {
+
+enum XtwMode : int {
+  kPlain = 0,      // plain product
+  kCentered = 1,   // subtract centre[j] from column j
+  kShiftStats = 2  // subtract A[0, j]; also return sum d and sum d^2 per column
+};
+
+template<class T>
+struct XwTask {
+  const T* A;            // m x k
+  long long m, k;
+  const double* center;  // k or null
+  const double* W;       // k x N, leading dimension ldw
+  long long ldw;
+  int N;
+  int splits;            // column splits (k dimension)
+  double* out;           // splits slabs of m x N
+  long long ldo;         // leading dimension of out (>= m)
+  long long slab;        // distance between split slabs of out
+  long long tile0;       // first flattened tile of this task
+};
+
+template<class T>
+struct XtwTask {
+  const T* A;            // m x k
+  long long m, k;
+  const double* center;  // k or null (kCentered)
+  const double* W;       // m x N, leading dimension ldw
+  long long ldw;
+  int N;
+  int mode;
+  int splits;            // row splits (m dimension)
+  double* out;           // splits slabs of k x N
+  long long ldo;         // leading dimension of out (>= k)
+  long long slab;        // distance between split slabs of out
+  double* stats;         // kShiftStats: splits slabs of 2 x k (sum, sum of squares)
+  long long tile0;
+};
+
+// Host side (implemented in stream_kernels.hip). pack_* assigns tile0 to a
+// host task array, checks it and returns the grid size; the caller uploads
+// the packed array once and launches with the device copy, so launches carry
+// no host-to-device traffic and can be captured in a graph.
+template<class T>
+long long pack_xw(XwTask<T>* tasks, int count, int* maxn);
+template<class T>
+long long pack_xtw(XtwTask<T>* tasks, int count, int* maxn);
+template<class T>
+void launch_xw(const XwTask<T>* device_tasks, int count, long long tiles, int maxn,
+               hipStream_t stream);
+template<class T>
+void launch_xtw(const XtwTask<T>* device_tasks, int count, long long tiles, int maxn,
+                hipStream_t stream);
+// One task passed by value as a kernel argument (no device task array).
+template<class T>
+void launch_xw_single(XwTask<T> task, hipStream_t stream);
+template<class T>
+void launch_xtw_single(XtwTask<T> task, hipStream_t stream);
+
+// Number of 256-row tiles of an xw task and of 64-column tiles of xtw.
+__host__ __device__ inline long long xw_row_tiles(long long m) { return (m + 255) / 256; }
+__host__ __device__ inline long long xtw_col_tiles(long long k) { return (k + 63) / 64; }
+
+// Deterministic sum of `splits` slabs: out[i] = sum_s part[s * stride + i].
+void launch_sum_slabs(const double* part, int splits, long long stride,
+                      long long count, double* out, hipStream_t stream);
+
+}  // namespace fpls

@@ -1,0 +1,319 @@
+// Streaming products over tall column-major matrices (see stream.hpp).
+//
+// Roofline: both kernels read A once per launch (m*k*sizeof(T) bytes) and are
+// meant to run at HBM bandwidth; W tiles come from L2 through LDS. The
+// arithmetic is FP64 FMA on values widened from T, so the FP32 storage path
+// loses nothing to accumulation.
+#include "stream.hpp"
+
+#include <algorithm>
+
+namespace fpls {
+
+namespace {
+
+template<class Task>
+__device__ inline int find_task(const Task* tasks, int count, long long block) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tasks[mid].tile0 <= block) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// ------------------------------------------------------------------- xw
+// One thread per row of A; the k range of the split is walked in chunks of
+// 32 columns whose W rows (and centres) are staged in LDS and broadcast.
+template<class T, int NB>
+__global__ __launch_bounds__(kWG) void k_xw(const XwTask<T>* tasks, int count, XwTask<T> single) {
+  const XwTask<T>& t = tasks ? tasks[find_task(tasks, count, blockIdx.x)] : single;
+  const long long local = (long long)blockIdx.x - t.tile0;
+  const long long rtiles = xw_row_tiles(t.m);
+  if (local >= rtiles * t.splits) return;
+  const int split = int(local / rtiles);
+  const long long row = (local % rtiles) * kWG + threadIdx.x;
+  const long long per = (t.k + t.splits - 1) / t.splits;
+  const long long j0 = split * per;
+  const long long j1 = j0 + per < t.k ? j0 + per : t.k;
+
+  __shared__ double ws[32][NB];
+  __shared__ double cs[32];
+  double acc[NB];
+#pragma unroll
+  for (int c = 0; c < NB; ++c) acc[c] = 0.0;
+  const bool valid = row < t.m;
+  const T* a = t.A + (valid ? row : 0);
+
+  for (long long jc = j0; jc < j1; jc += 32) {
+    const int cnt = int(j1 - jc < 32 ? j1 - jc : 32);
+    __syncthreads();
+    for (int e = threadIdx.x; e < 32 * NB; e += kWG) {
+      const int jj = e & 31, c = e >> 5;
+      ws[jj][c] = (jj < cnt && c < t.N) ? t.W[(jc + jj) + (long long)c * t.ldw] : 0.0;
+    }
+    if (threadIdx.x < 32) {
+      cs[threadIdx.x] = (t.center != nullptr && int(threadIdx.x) < cnt) ? t.center[jc + threadIdx.x] : 0.0;
+    }
+    __syncthreads();
+    if (valid) {
+      if (cnt == 32) {
+#pragma unroll 8
+        for (int jj = 0; jj < 32; ++jj) {
+          const double x = double(a[(jc + jj) * t.m]) - cs[jj];
+#pragma unroll
+          for (int c = 0; c < NB; ++c) acc[c] = fma(x, ws[jj][c], acc[c]);
+        }
+      } else {
+        for (int jj = 0; jj < cnt; ++jj) {
+          const double x = double(a[(jc + jj) * t.m]) - cs[jj];
+#pragma unroll
+          for (int c = 0; c < NB; ++c) acc[c] = fma(x, ws[jj][c], acc[c]);
+        }
+      }
+    }
+  }
+  if (valid) {
+    double* o = t.out + (long long)split * t.slab + row;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) if (c < t.N) o[(long long)c * t.ldo] = acc[c];
+  }
+}
+
+// ------------------------------------------------------------------ xtw
+// A block owns 64 columns of A and one row split. Row tiles of 32 are staged
+// in LDS (A widened to double and centred on the way in, W as is); each
+// thread then owns an MC x MN patch of the 64 x NB output.
+template<class T, int NB>
+__global__ __launch_bounds__(kWG) void k_xtw(const XtwTask<T>* tasks, int count, XtwTask<T> single) {
+  constexpr int MN = NB >= 16 ? NB / 16 : 1;
+  constexpr int MC = (NB / 4) / MN;
+  constexpr int GROUPS = 64 / MC;  // column groups
+  const XtwTask<T>& t = tasks ? tasks[find_task(tasks, count, blockIdx.x)] : single;
+  const long long local = (long long)blockIdx.x - t.tile0;
+  const long long ctiles = xtw_col_tiles(t.k);
+  if (local >= ctiles * t.splits) return;
+  const int split = int(local / ctiles);
+  const long long col0 = (local % ctiles) * 64;
+  const long long per = (t.m + t.splits - 1) / t.splits;
+  const long long r0 = split * per;
+  const long long r1 = r0 + per < t.m ? r0 + per : t.m;
+
+  __shared__ double xs[32][65];
+  __shared__ double wsh[32][NB + 1];
+
+  const int tid = threadIdx.x;
+  const int lrow = tid & 31;          // staging row
+  const int lcol = tid >> 5;          // staging column base (0..7)
+  // Per staged column: centre (or shift) and running stats.
+  double centre[8], s1[8], s2[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const long long col = col0 + lcol + 8 * e;
+    double c = 0.0;
+    if (col < t.k) {
+      if (t.mode == kCentered && t.center != nullptr) c = t.center[col];
+      else if (t.mode == kShiftStats) c = double(t.A[col * t.m]);
+    }
+    centre[e] = c; s1[e] = 0.0; s2[e] = 0.0;
+  }
+  const int g = tid % GROUPS, h = tid / GROUPS;
+  double acc[MC][MN];
+#pragma unroll
+  for (int a = 0; a < MC; ++a)
+#pragma unroll
+    for (int b = 0; b < MN; ++b) acc[a][b] = 0.0;
+
+  for (long long i0 = r0; i0 < r1; i0 += 32) {
+    const long long row = i0 + lrow;
+    const bool rv = row < r1;
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const long long col = col0 + lcol + 8 * e;
+      double v = 0.0;
+      if (rv && col < t.k) {
+        v = double(t.A[row + col * t.m]) - centre[e];
+        s1[e] += v;
+        s2[e] = fma(v, v, s2[e]);
+      }
+      xs[lrow][lcol + 8 * e] = v;
+    }
+    for (int e = tid; e < 32 * NB; e += kWG) {
+      const int r = e & 31, c = e >> 5;
+      const long long wr = i0 + r;
+      wsh[r][c] = (wr < r1 && c < t.N) ? t.W[wr + (long long)c * t.ldw] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int r = 0; r < 32; ++r) {
+      double xa[MC], wb[MN];
+#pragma unroll
+      for (int a = 0; a < MC; ++a) xa[a] = xs[r][g * MC + a];
+#pragma unroll
+      for (int b = 0; b < MN; ++b) wb[b] = wsh[r][h * MN + b];
+#pragma unroll
+      for (int a = 0; a < MC; ++a)
+#pragma unroll
+        for (int b = 0; b < MN; ++b) acc[a][b] = fma(xa[a], wb[b], acc[a][b]);
+    }
+  }
+  double* o = t.out + (long long)split * t.slab;
+#pragma unroll
+  for (int a = 0; a < MC; ++a) {
+    const long long col = col0 + g * MC + a;
+#pragma unroll
+    for (int b = 0; b < MN; ++b) {
+      const int n = h * MN + b;
+      if (col < t.k && n < t.N) o[col + (long long)n * t.ldo] = acc[a][b];
+    }
+  }
+  if (t.mode == kShiftStats && t.stats != nullptr) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      double a1 = s1[e], a2 = s2[e];
+      for (int off = 1; off < 32; off <<= 1) {
+        a1 += __shfl_xor(a1, off, 64);
+        a2 += __shfl_xor(a2, off, 64);
+      }
+      const long long col = col0 + lcol + 8 * e;
+      if (lrow == 0 && col < t.k) {
+        double* st = t.stats + (long long)split * 2 * t.k;
+        st[col] = a1;
+        st[t.k + col] = a2;
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(kWG) void k_sum_slabs(const double* part, int splits,
+                                                   long long stride, long long count,
+                                                   double* out) {
+  for (long long i = (long long)blockIdx.x * kWG + threadIdx.x; i < count;
+       i += (long long)gridDim.x * kWG) {
+    double s = 0.0;
+    for (int k = 0; k < splits; ++k) s += part[k * stride + i];
+    out[i] = s;
+  }
+}
+
+template<class T, int NB>
+void run_xw(const XwTask<T>* d, int count, const XwTask<T>& single, long long tiles,
+            hipStream_t stream) {
+  hipLaunchKernelGGL((k_xw<T, NB>), dim3(unsigned(tiles)), dim3(kWG), 0, stream, d, count,
+                     single);
+}
+
+template<class T, int NB>
+void run_xtw(const XtwTask<T>* d, int count, const XtwTask<T>& single, long long tiles,
+             hipStream_t stream) {
+  hipLaunchKernelGGL((k_xtw<T, NB>), dim3(unsigned(tiles)), dim3(kWG), 0, stream, d, count,
+                     single);
+}
+
+template<class T>
+void dispatch_xw(const XwTask<T>* d, int count, const XwTask<T>& single, long long tiles,
+                 int maxn, hipStream_t stream) {
+  if (tiles <= 0) return;
+  if (tiles > 0x7fffffffLL) throw std::invalid_argument("xw grid too large");
+  if (maxn <= 1) run_xw<T, 1>(d, count, single, tiles, stream);
+  else if (maxn <= 4) run_xw<T, 4>(d, count, single, tiles, stream);
+  else if (maxn <= 16) run_xw<T, 16>(d, count, single, tiles, stream);
+  else if (maxn <= 32) run_xw<T, 32>(d, count, single, tiles, stream);
+  else run_xw<T, 64>(d, count, single, tiles, stream);
+  FPLS_LAUNCHED("k_xw");
+}
+
+template<class T>
+void dispatch_xtw(const XtwTask<T>* d, int count, const XtwTask<T>& single, long long tiles,
+                  int maxn, hipStream_t stream) {
+  if (tiles <= 0) return;
+  if (tiles > 0x7fffffffLL) throw std::invalid_argument("xtw grid too large");
+  if (maxn <= 4) run_xtw<T, 4>(d, count, single, tiles, stream);
+  else if (maxn <= 16) run_xtw<T, 16>(d, count, single, tiles, stream);
+  else run_xtw<T, 64>(d, count, single, tiles, stream);
+  FPLS_LAUNCHED("k_xtw");
+}
+
+}  // namespace
+
+template<class T>
+long long pack_xw(XwTask<T>* tasks, int count, int* maxn) {
+  long long tiles = 0;
+  int mn = 0;
+  for (int i = 0; i < count; ++i) {
+    if (tasks[i].N < 1 || tasks[i].N > 64) throw std::invalid_argument("xw supports 1..64 right-hand columns");
+    if (tasks[i].splits < 1) throw std::invalid_argument("xw needs at least one split");
+    tasks[i].tile0 = tiles;
+    tiles += xw_row_tiles(tasks[i].m) * tasks[i].splits;
+    mn = std::max(mn, tasks[i].N);
+  }
+  if (maxn) *maxn = mn;
+  return tiles;
+}
+
+template<class T>
+long long pack_xtw(XtwTask<T>* tasks, int count, int* maxn) {
+  long long tiles = 0;
+  int mn = 0;
+  for (int i = 0; i < count; ++i) {
+    if (tasks[i].N < 1 || tasks[i].N > 64) throw std::invalid_argument("xtw supports 1..64 right-hand columns");
+    if (tasks[i].splits < 1) throw std::invalid_argument("xtw needs at least one split");
+    tasks[i].tile0 = tiles;
+    tiles += xtw_col_tiles(tasks[i].k) * tasks[i].splits;
+    mn = std::max(mn, tasks[i].N);
+  }
+  if (maxn) *maxn = mn;
+  return tiles;
+}
+
+template<class T>
+void launch_xw(const XwTask<T>* device_tasks, int count, long long tiles, int maxn,
+               hipStream_t stream) {
+  if (count <= 0) return;
+  dispatch_xw<T>(device_tasks, count, XwTask<T>{}, tiles, maxn, stream);
+}
+
+template<class T>
+void launch_xtw(const XtwTask<T>* device_tasks, int count, long long tiles, int maxn,
+                hipStream_t stream) {
+  if (count <= 0) return;
+  dispatch_xtw<T>(device_tasks, count, XtwTask<T>{}, tiles, maxn, stream);
+}
+
+template<class T>
+void launch_xw_single(XwTask<T> task, hipStream_t stream) {
+  int maxn = 0;
+  const long long tiles = pack_xw(&task, 1, &maxn);
+  dispatch_xw<T>(nullptr, 1, task, tiles, maxn, stream);
+}
+
+template<class T>
+void launch_xtw_single(XtwTask<T> task, hipStream_t stream) {
+  int maxn = 0;
+  const long long tiles = pack_xtw(&task, 1, &maxn);
+  dispatch_xtw<T>(nullptr, 1, task, tiles, maxn, stream);
+}
+
+void launch_sum_slabs(const double* part, int splits, long long stride,
+                      long long count, double* out, hipStream_t stream) {
+  if (count <= 0) return;
+  long long blocks = (count + kWG - 1) / kWG;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_sum_slabs, dim3(unsigned(blocks)), dim3(kWG), 0, stream,
+                     part, splits, stride, count, out);
+  FPLS_LAUNCHED("k_sum_slabs");
+}
+
+#define FPLS_STREAM_INSTANCES(T)                                                        \
+  template long long pack_xw<T>(XwTask<T>*, int, int*);                                  \
+  template long long pack_xtw<T>(XtwTask<T>*, int, int*);                                \
+  template void launch_xw<T>(const XwTask<T>*, int, long long, int, hipStream_t);       \
+  template void launch_xtw<T>(const XtwTask<T>*, int, long long, int, hipStream_t);      \
+  template void launch_xw_single<T>(XwTask<T>, hipStream_t);                             \
+  template void launch_xtw_single<T>(XtwTask<T>, hipStream_t);
+FPLS_STREAM_INSTANCES(float)
+FPLS_STREAM_INSTANCES(double)
+#undef FPLS_STREAM_INSTANCES
+
+}  // namespace fpls

@@ -1,0 +1,349 @@
+// FP64 dense linear algebra executed by ONE work-group of kWG threads.
+//
+// Every routine must be called by all threads of the work-group with
+// identical arguments; it synchronises internally and returns with the
+// results visible to the whole work-group. Matrices are column-major and may
+// live in global memory (L2-resident scratch) or LDS.
+#pragma once
+
+#include "common.hpp"
+
+namespace fpls {
+namespace wg {
+
+// C(m x n) = alpha * op(A) * op(B) + beta * C ; thread per output element.
+__device__ inline void gemm(int m, int n, int k, double alpha, const double* A,
+                            long lda, bool ta, const double* B, long ldb, bool tb,
+                            double beta, double* C, long ldc) {
+  for (long e = threadIdx.x; e < (long)m * n; e += kWG) {
+    const int i = int(e % m), j = int(e / m);
+    double acc = 0.0;
+    for (int l = 0; l < k; ++l) {
+      const double a = ta ? A[l + i * lda] : A[i + l * lda];
+      const double b = tb ? B[j + l * ldb] : B[l + j * ldb];
+      acc = fma(a, b, acc);
+    }
+    C[i + j * ldc] = alpha * acc + (beta != 0.0 ? beta * C[i + j * ldc] : 0.0);
+  }
+  __syncthreads();
+}
+
+// C(m x n) = Aᵀ B with A (k x m), B (k x n) and a long inner dimension k:
+// one wave per output, lanes stride the inner dimension (coalesced).
+__device__ inline void gemm_tn_long(int m, int n, int k, const double* A, long lda,
+                                    const double* B, long ldb, double* C, long ldc) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int e = wave; e < m * n; e += kWaves) {
+    const int i = e % m, j = e / m;
+    double acc = 0.0;
+    for (int l = lane; l < k; l += 64) acc = fma(A[l + i * lda], B[l + (long)j * ldb], acc);
+    acc = wave_sum(acc);
+    if (lane == 0) C[i + j * ldc] = acc;
+  }
+  __syncthreads();
+}
+
+__device__ inline void copy(int m, int n, const double* A, long lda, double* B, long ldb) {
+  for (long e = threadIdx.x; e < (long)m * n; e += kWG) {
+    const int i = int(e % m), j = int(e / m);
+    B[i + j * ldb] = A[i + j * lda];
+  }
+  __syncthreads();
+}
+
+// Euclidean norm of a vector (all threads receive it).
+__device__ inline double norm(const double* v, long n, double* red) {
+  double s = 0.0;
+  for (long i = threadIdx.x; i < n; i += kWG) s = fma(v[i], v[i], s);
+  return sqrt(fmax(wg_sum(s, red), 0.0));
+}
+
+// Householder economy QR of A (m x n, m >= n), overwritten by the reflectors;
+// the explicit Q (m x n) goes to Q. tau and dots are LDS arrays of n doubles.
+// Returns false on a non-finite column norm.
+__device__ inline bool householder_qr(double* A, long lda, int m, int n, double* Q,
+                                      long ldq, double* tau, double* dots, double* red) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  bool finite = true;
+  for (int j = 0; j < n; ++j) {
+    double s = 0.0;
+    for (long i = j + tid; i < m; i += kWG) s = fma(A[i + j * lda], A[i + j * lda], s);
+    const double norm2 = wg_sum(s, red);
+    const double alpha = A[j + j * lda];
+    const double x2 = norm2 - alpha * alpha;
+    double tj = 0.0, beta = alpha, scale = 0.0;
+    if (!isfinite(norm2)) finite = false;
+    if (finite && x2 > 0.0) {
+      const double nrm = sqrt(norm2);
+      beta = alpha >= 0.0 ? -nrm : nrm;
+      tj = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+    __syncthreads();
+    if (tj != 0.0) {
+      for (long i = j + 1 + tid; i < m; i += kWG) A[i + j * lda] *= scale;
+      if (tid == 0) A[j + j * lda] = beta;
+    }
+    if (tid == 0) tau[j] = tj;
+    __syncthreads();
+    if (tj != 0.0 && j + 1 < n) {
+      for (int c = j + 1 + wave; c < n; c += kWaves) {
+        double acc = 0.0;
+        for (long i = j + 1 + lane; i < m; i += 64) acc = fma(A[i + j * lda], A[i + c * lda], acc);
+        acc = wave_sum(acc);
+        if (lane == 0) dots[c] = (acc + A[j + c * lda]) * tj;
+      }
+      __syncthreads();
+      const long rows = m - j;
+      const long total = (long)(n - j - 1) * rows;
+      for (long e = tid; e < total; e += kWG) {
+        const int c = j + 1 + int(e / rows);
+        const long i = j + e % rows;
+        const double v = i == j ? 1.0 : A[i + j * lda];
+        A[i + c * lda] -= dots[c] * v;
+      }
+      __syncthreads();
+    }
+  }
+  for (long e = tid; e < (long)m * n; e += kWG) {
+    const long i = e % m;
+    const int c = int(e / m);
+    Q[i + c * ldq] = i == c ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int j = n - 1; j >= 0; --j) {
+    const double tj = tau[j];
+    if (tj == 0.0) continue;
+    for (int c = j + wave; c < n; c += kWaves) {
+      double acc = 0.0;
+      for (long i = j + 1 + lane; i < m; i += 64) acc = fma(A[i + j * lda], Q[i + c * ldq], acc);
+      acc = wave_sum(acc);
+      if (lane == 0) dots[c] = (acc + Q[j + c * ldq]) * tj;
+    }
+    __syncthreads();
+    const long rows = m - j;
+    const long total = (long)(n - j) * rows;
+    for (long e = tid; e < total; e += kWG) {
+      const int c = j + int(e / rows);
+      const long i = j + e % rows;
+      const double v = i == j ? 1.0 : A[i + j * lda];
+      Q[i + c * ldq] -= dots[c] * v;
+    }
+    __syncthreads();
+  }
+  return finite;
+}
+
+// Pair (a, b) of round r for k-th slot of the round-robin (circle) schedule
+// over m (even) players.
+__device__ inline void rr_pair(int m, int r, int k, int& a, int& b) {
+  auto at = [&](int pos) { return pos == 0 ? 0 : ((pos - 1 + r) % (m - 1)) + 1; };
+  a = at(k);
+  b = at(m - 1 - k);
+  if (a > b) { const int t = a; a = b; b = t; }
+}
+
+// Parallel cyclic Jacobi eigen-decomposition of the symmetric n x n matrix A
+// (leading dimension n). On return V holds the eigenvectors as columns and
+// w the eigenvalues, ascending (the dsyevd contract). cs/sn/pa/pb are LDS
+// scratch of n/2+1 entries each. Returns false on non-finite input.
+__device__ inline bool jacobi_eigen(double* A, double* V, int n, double* w, double* cs,
+                                    double* sn, int* pa, int* pb, double* red,
+                                    double* vtmp /* n x n scratch */) {
+  const int tid = threadIdx.x;
+  for (long e = tid; e < (long)n * n; e += kWG) V[e] = (e % n) == (e / n) ? 1.0 : 0.0;
+  double fin = 0.0;
+  for (long e = tid; e < (long)n * n; e += kWG) fin += isfinite(A[e]) ? 0.0 : 1.0;
+  if (wg_sum(fin, red) != 0.0) return false;
+  const int m = n + (n & 1);
+  const int half = m / 2;
+  for (int sweep = 0; sweep < 40 && n > 1; ++sweep) {
+    double off = 0.0, dg = 0.0;
+    for (long e = tid; e < (long)n * n; e += kWG) {
+      const int i = int(e % n), j = int(e / n);
+      if (i < j) off = fma(A[e], A[e], off);
+      else if (i == j) dg = fma(A[e], A[e], dg);
+    }
+    off = wg_sum(off, red);
+    dg = wg_sum(dg, red);
+    if (off == 0.0 || off <= 1e-34 * dg) break;
+    for (int r = 0; r < m - 1; ++r) {
+      if (tid < half) {
+        int a, b;
+        rr_pair(m, r, tid, a, b);
+        double c = 1.0, s = 0.0;
+        if (b < n) {
+          const double apq = A[a + b * n];
+          if (apq != 0.0) {
+            const double theta = (A[b + b * n] - A[a + a * n]) / (2.0 * apq);
+            const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(1.0 + theta * theta));
+            c = 1.0 / sqrt(1.0 + t * t);
+            s = t * c;
+          }
+        } else {
+          b = -1;
+        }
+        cs[tid] = c; sn[tid] = s; pa[tid] = a; pb[tid] = b;
+      }
+      __syncthreads();
+      // columns of A and V
+      for (long e = tid; e < (long)half * n; e += kWG) {
+        const int k = int(e / n), i = int(e % n);
+        const int a = pa[k], b = pb[k];
+        if (b < 0 || sn[k] == 0.0) continue;
+        const double c = cs[k], s = sn[k];
+        const double xa = A[i + a * n], xb = A[i + b * n];
+        A[i + a * n] = c * xa - s * xb;
+        A[i + b * n] = s * xa + c * xb;
+        const double va = V[i + a * n], vb = V[i + b * n];
+        V[i + a * n] = c * va - s * vb;
+        V[i + b * n] = s * va + c * vb;
+      }
+      __syncthreads();
+      // rows of A
+      for (long e = tid; e < (long)half * n; e += kWG) {
+        const int k = int(e / n), i = int(e % n);
+        const int a = pa[k], b = pb[k];
+        if (b < 0 || sn[k] == 0.0) continue;
+        const double c = cs[k], s = sn[k];
+        const double xa = A[a + i * n], xb = A[b + i * n];
+        A[a + i * n] = c * xa - s * xb;
+        A[b + i * n] = s * xa + c * xb;
+      }
+      __syncthreads();
+    }
+  }
+  // stable ascending order by rank
+  for (int i = tid; i < n; i += kWG) {
+    const double di = A[i + i * n];
+    int rank = 0;
+    for (int j = 0; j < n; ++j) {
+      const double dj = A[j + j * n];
+      rank += (dj < di || (dj == di && j < i)) ? 1 : 0;
+    }
+    w[rank] = di;
+  }
+  __syncthreads();
+  for (long e = tid; e < (long)n * n; e += kWG) {
+    const int i = int(e % n), j = int(e / n);
+    const double dj = A[j + j * n];
+    int rank = 0;
+    for (int l = 0; l < n; ++l) {
+      const double dl = A[l + l * n];
+      rank += (dl < dj || (dl == dj && l < j)) ? 1 : 0;
+    }
+    vtmp[i + (long)rank * n] = V[i + (long)j * n];
+  }
+  __syncthreads();
+  for (long e = tid; e < (long)n * n; e += kWG) V[e] = vtmp[e];
+  __syncthreads();
+  return true;
+}
+
+// One-sided (Hestenes) Jacobi SVD of W (m x n, m >= n), leading dimension m.
+// On return: s (n, descending), U (m x n) left vectors, Vm (n x n) right
+// vectors as columns. W is destroyed. cs/sn/pa/pb: LDS, n/2+1 entries.
+__device__ inline bool jacobi_svd(double* W, int m, int n, double* U, double* Vm,
+                                  double* s, double* cs, double* sn, int* pa, int* pb,
+                                  double* red, double* sc /* 3*(n/2+1) LDS */) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (long e = tid; e < (long)n * n; e += kWG) Vm[e] = (e % n) == (e / n) ? 1.0 : 0.0;
+  const int mm = n + (n & 1);
+  const int half = mm / 2;
+  for (int sweep = 0; sweep < 40 && n > 1; ++sweep) {
+    double rotated = 0.0;
+    for (int r = 0; r < mm - 1; ++r) {
+      // dot products per pair: one wave per pair
+      for (int k = wave; k < half; k += kWaves) {
+        int a, b;
+        rr_pair(mm, r, k, a, b);
+        double al = 0.0, be = 0.0, ga = 0.0;
+        if (b < n) {
+          for (int i = lane; i < m; i += 64) {
+            const double wa = W[i + (long)a * m], wb = W[i + (long)b * m];
+            al = fma(wa, wa, al); be = fma(wb, wb, be); ga = fma(wa, wb, ga);
+          }
+        }
+        al = wave_sum(al); be = wave_sum(be); ga = wave_sum(ga);
+        if (lane == 0) {
+          double c = 1.0, sv = 0.0;
+          if (b < n && ga != 0.0 && fabs(ga) > 1e-15 * sqrt(al * be)) {
+            const double zeta = (be - al) / (2.0 * ga);
+            const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+            c = 1.0 / sqrt(1.0 + t * t);
+            sv = c * t;
+          }
+          cs[k] = c; sn[k] = sv; pa[k] = a; pb[k] = b < n ? b : -1;
+        }
+      }
+      __syncthreads();
+      for (int k = 0; k < half; ++k) rotated += (sn[k] != 0.0) ? 1.0 : 0.0;
+      for (long e = tid; e < (long)half * m; e += kWG) {
+        const int k = int(e / m), i = int(e % m);
+        const int a = pa[k], b = pb[k];
+        if (b < 0 || sn[k] == 0.0) continue;
+        const double c = cs[k], sv = sn[k];
+        const double wa = W[i + (long)a * m], wb = W[i + (long)b * m];
+        W[i + (long)a * m] = c * wa - sv * wb;
+        W[i + (long)b * m] = sv * wa + c * wb;
+      }
+      for (long e = tid; e < (long)half * n; e += kWG) {
+        const int k = int(e / n), i = int(e % n);
+        const int a = pa[k], b = pb[k];
+        if (b < 0 || sn[k] == 0.0) continue;
+        const double c = cs[k], sv = sn[k];
+        const double va = Vm[i + (long)a * n], vb = Vm[i + (long)b * n];
+        Vm[i + (long)a * n] = c * va - sv * vb;
+        Vm[i + (long)b * n] = sv * va + c * vb;
+      }
+      __syncthreads();
+    }
+    if (rotated == 0.0) break;
+  }
+  (void)sc;
+  // column norms
+  for (int j = wave; j < n; j += kWaves) {
+    double acc = 0.0;
+    for (int i = lane; i < m; i += 64) acc = fma(W[i + (long)j * m], W[i + (long)j * m], acc);
+    acc = wave_sum(acc);
+    if (lane == 0) s[j] = sqrt(acc);
+  }
+  __syncthreads();
+  // rank by descending norm (stable); write U and reorder Vm through U's
+  // storage is not possible in place, so permute into U directly and keep a
+  // rank table in red-sized chunks via recomputation.
+  for (long e = tid; e < (long)m * n; e += kWG) {
+    const int i = int(e % m), j = int(e / m);
+    const double sj = s[j];
+    int rank = 0;
+    for (int l = 0; l < n; ++l) rank += (s[l] > sj || (s[l] == sj && l < j)) ? 1 : 0;
+    U[i + (long)rank * m] = sj > 0.0 ? W[i + (long)j * m] / sj : 0.0;
+  }
+  __syncthreads();
+  // reorder Vm columns using W as scratch (W no longer needed; m >= n)
+  for (long e = tid; e < (long)n * n; e += kWG) {
+    const int i = int(e % n), j = int(e / n);
+    const double sj = s[j];
+    int rank = 0;
+    for (int l = 0; l < n; ++l) rank += (s[l] > sj || (s[l] == sj && l < j)) ? 1 : 0;
+    W[i + (long)rank * n] = Vm[i + (long)j * n];
+  }
+  __syncthreads();
+  double fin = 0.0;
+  for (long e = tid; e < (long)n * n; e += kWG) { Vm[e] = W[e]; fin += isfinite(W[e]) ? 0.0 : 1.0; }
+  __syncthreads();
+  // sort singular values descending
+  if (tid == 0) {
+    for (int i = 1; i < n; ++i) {
+      const double v = s[i];
+      int j = i - 1;
+      while (j >= 0 && s[j] < v) { s[j + 1] = s[j]; --j; }
+      s[j + 1] = v;
+    }
+  }
+  __syncthreads();
+  return wg_sum(fin, red) == 0.0;
+}
+
+}  // namespace wg
+}  // namespace fpls

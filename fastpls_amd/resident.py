@@ -1,0 +1,385 @@
+"""Python binding of the HIP C ABI (include/fastpls_hip.h).
+
+Mirrors the reference's resident-model glue (src/cuda_resident_r.cpp and
+R/resident_cuda.R): a model handle created from predictors plus responses or
+one-based labels, with prediction / classification paths over component
+prefixes, response sums, projection, export of the fitted factors, controls
+and compaction. Arrays are numpy, column-major (Fortran order) like the
+reference's R matrices. There is no CPU fallback: if the HIP library is
+missing or no device is usable, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libfastpls_hip.so")
+
+SIMPLS = 0
+PLSSVD = 1
+
+FIELDS = {"R": 0, "Q": 1, "T": 2, "mX": 3, "vX": 4, "mY": 5, "P": 6, "variance": 7}
+
+# name -> (restype, argtypes)
+_vp, _cvp, _ip, _dp, _cp = C.c_void_p, C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_char_p
+_sz, _ull, _ll = C.c_size_t, C.c_ulonglong, C.c_longlong
+SIGNATURES = {
+    "fastpls_hip_device_info": (C.c_int, [_cp, _sz]),
+    "fastpls_hip_gemm": (C.c_int, [_cvp, _cvp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _cp, _sz]),
+    "fastpls_hip_simpls_create": (_vp, [_cvp, _cvp, _ip, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _ull, _cp, _sz]),
+    "fastpls_hip_plssvd_create": (_vp, [_cvp, _cvp, _ip, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _ull, _cp, _sz]),
+    "fastpls_hip_opls_create": (_vp, [_cvp, _cvp, _ip, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.c_int, C.c_int, C.c_int, C.c_int, _ull, C.c_int, _cp, _sz]),
+    "fastpls_hip_kernelpls_create": (_vp, [_cvp, _cvp, _ip, C.c_int, C.c_int, C.c_int, C.c_int,
+                                           C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _ull,
+                                           C.c_int, C.c_double, C.c_int, C.c_double, _cp, _sz]),
+    "fastpls_hip_predict_path": (C.c_int, [_vp, _cvp, C.c_int, _ip, C.c_int, C.c_int, _vp, _cp, _sz]),
+    "fastpls_hip_destroy": (None, [_vp]),
+    "fastpls_hip_export": (C.c_int, [_vp, C.c_int, _vp, _sz, _cp, _sz]),
+    "fastpls_hip_classify_path": (C.c_int, [_vp, _cvp, C.c_int, _ip, C.c_int, C.c_int, C.c_int,
+                                            _ip, _cp, _sz]),
+    "fastpls_hip_classify_response_path": (C.c_int, [_vp, _cvp, C.c_int, _ip, C.c_int, C.c_int,
+                                                     C.c_int, _ip, _vp, _cp, _sz]),
+    "fastpls_hip_response_sums": (C.c_int, [_vp, _cvp, _cvp, _ip, C.c_int, C.c_int, _vp, _cp, _sz]),
+    "fastpls_hip_project": (C.c_int, [_vp, _cvp, C.c_int, C.c_int, _vp, _cp, _sz]),
+    "fastpls_hip_controls": (C.c_int, [_vp, _ip, _ip, _ip, _ip, _ip, _ip, _cp, _sz]),
+    "fastpls_hip_compact": (C.c_int, [_vp, C.c_int, _cp, _sz]),
+    "fastpls_hip_fit_device": (_vp, [C.c_int, _cvp, _cvp, _ip, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_int, C.c_int, C.c_int, C.c_int, _ull, _cp, _sz]),
+    "fastpls_hip_last_fit_seconds": (C.c_int, [_vp, _dp, _cp, _sz]),
+    "fastpls_hip_fit_phases": (C.c_int, [_vp, _dp, C.c_int, _cp, _sz]),
+    "fastpls_hip_device_alloc": (C.c_int, [_sz, C.POINTER(C.c_void_p), _cp, _sz]),
+    "fastpls_hip_device_free": (C.c_int, [_vp, _cp, _sz]),
+    "fastpls_hip_memcpy": (C.c_int, [_vp, _cvp, _sz, C.c_int, _cp, _sz]),
+    "fastpls_hip_latent_data": (C.c_int, [_vp, _vp, C.c_int, _ll, C.c_int, C.c_int, C.c_int,
+                                          C.c_double, C.c_double, _ull, _cp, _sz]),
+}
+
+_LIB = None
+
+
+class HipError(RuntimeError):
+    """Raised for every failure reported by the HIP library."""
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load the in-tree HIP library (raises if it was not built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise HipError(f"HIP library not built: {path} (run python -m fastpls_amd.build)")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def _err():
+    return C.create_string_buffer(1024)
+
+
+def _raise(buf):
+    raise HipError(buf.value.decode(errors="replace") or "unknown fastPLS HIP failure")
+
+
+def _dtype(precision: int):
+    if precision == 32:
+        return np.float32
+    if precision == 64:
+        return np.float64
+    raise ValueError("precision must be 32 or 64")
+
+
+def _fmat(a, dtype):
+    return np.asfortranarray(np.asarray(a, dtype=dtype))
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _iptr(a):
+    return None if a is None else a.ctypes.data_as(_ip)
+
+
+def device_info() -> str:
+    lib = load_library()
+    buf = C.create_string_buffer(256)
+    if lib.fastpls_hip_device_info(buf, 256) != 0:
+        _raise(buf)
+    return buf.value.decode()
+
+
+def gemm(left, right, precision: int = 64):
+    """fastpls_cuda_gemm counterpart: left @ right on the device."""
+    lib = load_library()
+    dt = _dtype(precision)
+    a = _fmat(left, dt)
+    b = _fmat(right, dt)
+    if a.shape[1] != b.shape[0]:
+        raise ValueError("inner dimensions differ")
+    out = np.empty((a.shape[0], b.shape[1]), dtype=dt, order="F")
+    e = _err()
+    if lib.fastpls_hip_gemm(_ptr(a), _ptr(b), precision, a.shape[0], a.shape[1], b.shape[1],
+                            _ptr(out), e, 1024) != 0:
+        _raise(e)
+    return out
+
+
+class ResidentModel:
+    """A fitted model resident in HBM (handle of the C ABI)."""
+
+    def __init__(self, handle, method, precision, n, p, q, components, classification):
+        self._h = handle
+        self.method = method
+        self.precision = precision
+        self.dtype = _dtype(precision)
+        self.n, self.p, self.q = n, p, q
+        self.components = components
+        self.classification = classification
+
+    # ----------------------------------------------------------- creation
+    @classmethod
+    def fit(cls, x, y=None, labels=None, classes=None, method=SIMPLS, precision=64,
+            components=2, scaling=2, oversample=32, power=5, seed=1, retain_scores=True):
+        """Fit SIMPLS (method=0) or PLS-SVD (method=1) from host arrays."""
+        lib = load_library()
+        dt = _dtype(precision)
+        xa = _fmat(x, dt)
+        n, p = xa.shape
+        if (y is None) == (labels is None):
+            raise ValueError("provide responses or labels, not both")
+        if labels is not None:
+            lab = np.ascontiguousarray(np.asarray(labels, dtype=np.int32))
+            q = int(classes if classes is not None else lab.max())
+            ya = None
+        else:
+            ya = _fmat(y, dt)
+            if ya.ndim == 1:
+                ya = ya.reshape((-1, 1), order="F")
+            q = ya.shape[1]
+            lab = None
+        e = _err()
+        fn = lib.fastpls_hip_simpls_create if method == SIMPLS else lib.fastpls_hip_plssvd_create
+        h = fn(_ptr(xa), _ptr(ya), _iptr(lab), precision, n, p, q, components, scaling,
+               oversample, power, int(bool(retain_scores)), seed, e, 1024)
+        if not h:
+            _raise(e)
+        return cls(h, method, precision, n, p, q, components, lab is not None)
+
+    @classmethod
+    def fit_device(cls, x_ptr, y_ptr, labels_ptr, n, p, q, method=SIMPLS, precision=32,
+                   components=2, scaling=2, oversample=32, power=5, seed=1):
+        """Fit from device pointers (integers) that stay owned by the caller."""
+        lib = load_library()
+        e = _err()
+        lab = C.cast(C.c_void_p(labels_ptr), _ip) if labels_ptr else None
+        h = lib.fastpls_hip_fit_device(method, C.c_void_p(x_ptr),
+                                       C.c_void_p(y_ptr) if y_ptr else None, lab, precision,
+                                       n, p, q, components, scaling, oversample, power, seed,
+                                       e, 1024)
+        if not h:
+            _raise(e)
+        return cls(h, method, precision, n, p, q, components, labels_ptr is not None)
+
+    def close(self):
+        if self._h:
+            load_library().fastpls_hip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -------------------------------------------------------------- calls
+    def _x(self, x):
+        xa = _fmat(x, self.dtype)
+        if xa.ndim != 2 or xa.shape[1] != self.p:
+            raise ValueError("new data must have p columns")
+        return xa
+
+    @staticmethod
+    def _prefixes(prefixes):
+        return np.ascontiguousarray(np.atleast_1d(np.asarray(prefixes, dtype=np.int32)))
+
+    def predict_path(self, x, prefixes, lda=False):
+        xa = self._x(x)
+        pf = self._prefixes(prefixes)
+        rows = xa.shape[0]
+        out = np.empty((rows, self.q, len(pf)), dtype=self.dtype, order="F")
+        e = _err()
+        if load_library().fastpls_hip_predict_path(self._h, _ptr(xa), rows, _iptr(pf), len(pf),
+                                                   int(bool(lda)), _ptr(out), e, 1024) != 0:
+            _raise(e)
+        return out
+
+    def predict(self, x, prefix, lda=False):
+        return self.predict_path(x, [prefix], lda)[:, :, 0]
+
+    def classify_path(self, x, prefixes, lda=False, top=1):
+        xa = self._x(x)
+        pf = self._prefixes(prefixes)
+        rows = xa.shape[0]
+        out = np.empty((rows, top, len(pf)), dtype=np.int32, order="F")
+        e = _err()
+        if load_library().fastpls_hip_classify_path(self._h, _ptr(xa), rows, _iptr(pf), len(pf),
+                                                    int(bool(lda)), top, _iptr(out), e, 1024) != 0:
+            _raise(e)
+        return out
+
+    def classify_response_path(self, x, prefixes, lda=False, top=1):
+        xa = self._x(x)
+        pf = self._prefixes(prefixes)
+        rows = xa.shape[0]
+        labels = np.empty((rows, top, len(pf)), dtype=np.int32, order="F")
+        resp = np.empty((rows, self.q, len(pf)), dtype=self.dtype, order="F")
+        e = _err()
+        if load_library().fastpls_hip_classify_response_path(
+                self._h, _ptr(xa), rows, _iptr(pf), len(pf), int(bool(lda)), top, _iptr(labels),
+                _ptr(resp), e, 1024) != 0:
+            _raise(e)
+        return labels, resp
+
+    def response_sums(self, x, prefix, y=None, labels=None):
+        xa = self._x(x)
+        rows = xa.shape[0]
+        ya = None if y is None else _fmat(y, self.dtype)
+        lab = None if labels is None else np.ascontiguousarray(np.asarray(labels, dtype=np.int32))
+        out = np.empty((3, self.q), dtype=self.dtype, order="F")
+        e = _err()
+        if load_library().fastpls_hip_response_sums(self._h, _ptr(xa), _ptr(ya), _iptr(lab), rows,
+                                                     prefix, _ptr(out), e, 1024) != 0:
+            _raise(e)
+        return out
+
+    def project(self, x, prefix):
+        xa = self._x(x)
+        rows = xa.shape[0]
+        out = np.empty((rows, prefix), dtype=self.dtype, order="F")
+        e = _err()
+        if load_library().fastpls_hip_project(self._h, _ptr(xa), rows, prefix, _ptr(out), e,
+                                              1024) != 0:
+            _raise(e)
+        return out
+
+    def export(self, field):
+        f = FIELDS[field] if isinstance(field, str) else int(field)
+        a = self.components
+        shapes = {0: (self.p, a), 1: (self.q, a), 2: (self.n, a), 3: (self.p,), 4: (self.p,),
+                  5: (self.q,), 6: (self.p, a), 7: (a + 1,)}
+        shape = shapes[f]
+        out = np.empty(shape, dtype=self.dtype, order="F")
+        e = _err()
+        if load_library().fastpls_hip_export(self._h, f, _ptr(out), out.size, e, 1024) != 0:
+            _raise(e)
+        return out
+
+    def controls(self):
+        vals = [C.c_int(0) for _ in range(6)]
+        e = _err()
+        if load_library().fastpls_hip_controls(self._h, *[C.byref(v) for v in vals], e, 1024) != 0:
+            _raise(e)
+        keys = ["oversample", "power", "block", "block_limit", "implicit_operator",
+                "predictor_crossprod_cache"]
+        return {k: v.value for k, v in zip(keys, vals)}
+
+    def compact(self, prepare_lda=False):
+        e = _err()
+        if load_library().fastpls_hip_compact(self._h, int(bool(prepare_lda)), e, 1024) != 0:
+            _raise(e)
+
+    def last_fit_seconds(self) -> float:
+        v = C.c_double(0)
+        e = _err()
+        if load_library().fastpls_hip_last_fit_seconds(self._h, C.byref(v), e, 1024) != 0:
+            _raise(e)
+        return v.value
+
+    def fit_phases(self):
+        out = (C.c_double * 4)()
+        e = _err()
+        if load_library().fastpls_hip_fit_phases(self._h, out, 4, e, 1024) != 0:
+            _raise(e)
+        return {"fit": out[0], "crossprod_pass": out[1], "score_pass": out[2],
+                "x_bytes": out[3]}
+
+    def coefficients(self, prefix):
+        """B for one prefix (p x q) in standardized units: R_k Q_kᵀ (SIMPLS)."""
+        R = self.export("R").astype(np.float64)
+        Q = self.export("Q").astype(np.float64)
+        return R[:, :prefix] @ Q[:, :prefix].T
+
+
+class DeviceBuffer:
+    """Device memory owned through the library (no torch in the process)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        ptr = C.c_void_p(0)
+        e = _err()
+        if load_library().fastpls_hip_device_alloc(self.nbytes, C.byref(ptr), e, 1024) != 0:
+            _raise(e)
+        self.ptr = ptr.value
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.asfortranarray(a)
+        buf = cls(a.nbytes)
+        buf.upload(a)
+        return buf
+
+    def upload(self, a):
+        a = np.asfortranarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than the device buffer")
+        e = _err()
+        if load_library().fastpls_hip_memcpy(C.c_void_p(self.ptr), _ptr(a), a.nbytes, 0, e,
+                                             1024) != 0:
+            _raise(e)
+
+    def download(self, shape, dtype):
+        out = np.empty(shape, dtype=dtype, order="F")
+        e = _err()
+        if load_library().fastpls_hip_memcpy(_ptr(out), C.c_void_p(self.ptr), out.nbytes, 1, e,
+                                             1024) != 0:
+            _raise(e)
+        return out
+
+    def free(self):
+        if self.ptr:
+            e = _err()
+            if load_library().fastpls_hip_device_free(C.c_void_p(self.ptr), e, 1024) != 0:
+                _raise(e)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def latent_data_device(x_ptr, y_ptr, precision, n, p, q, factors=10, decay=1.0, noise=0.1,
+                       seed=0):
+    """Fill caller-allocated device buffers with latent-factor data."""
+    e = _err()
+    if load_library().fastpls_hip_latent_data(C.c_void_p(x_ptr), C.c_void_p(y_ptr), precision, n,
+                                              p, q, factors, decay, noise, seed, e, 1024) != 0:
+        _raise(e)

@@ -1,0 +1,160 @@
+/*
+ * fastpls_hip.h - C ABI of the MI355X (gfx950) HIP backend for fastPLS.
+ *
+ * Drop-in counterpart of the reference's resident device engine ABI
+ * (src/cuda_resident_api.h:8-80). Every fastpls_hip_* entry below mirrors the
+ * fastpls_* entry named in its comment: same argument order, same meaning,
+ * same conventions:
+ *   - precision 32 -> float, 64 -> double (api.h:10); matrices are
+ *     column-major and contiguous (leading dimension = rows);
+ *   - labels are one-based and must lie in 1..q;
+ *   - inputs are host pointers, copied by the call; outputs are caller-owned
+ *     host buffers of exact size;
+ *   - int return 0 = success, 1 = error; create returns NULL on error; the
+ *     message is written to (error, error_capacity) with snprintf semantics;
+ *   - every call is synchronous; calls on one handle must not overlap.
+ *
+ * Precision 32 stores X as FP32 on the device and runs every product over X
+ * with FP64 accumulation; the cross-covariance, the randomized SVD, SIMPLS
+ * deflation/orthogonalisation, PLS-SVD solves and LDA run in FP64 for both
+ * precisions. Gaussian sketches are drawn on the host with
+ * std::mt19937(seed) and std::normal_distribution<double>, the same stream
+ * as the reference CPU core (simpls.hpp:244-249, rsvd.hpp:205-210).
+ *
+ * There is no CPU fallback: every entry fails with an error when no HIP
+ * device is usable.
+ */
+#ifndef FASTPLS_HIP_H
+#define FASTPLS_HIP_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Library and device identification (no reference counterpart): writes a
+ * NUL-terminated description; returns 0 when a device is usable. */
+int fastpls_hip_device_info(char* out, size_t capacity);
+
+/* fastpls_cuda_gemm (src/cuda_resident_api.h:11-13): output(rows x columns)
+ * = left(rows x inner) * right(inner x columns), host buffers. */
+int fastpls_hip_gemm(const void* left, const void* right, int precision, int rows,
+                     int inner, int columns, void* output, char* error,
+                     size_t error_capacity);
+
+/* fastpls_resident_simpls_create (src/cuda_resident_api.h:40-43). Exactly one
+ * of y (n x q) and labels (n, one-based, q classes) is non-NULL. */
+void* fastpls_hip_simpls_create(const void* x, const void* y, const int* labels,
+                                int precision, int n, int p, int q, int components,
+                                int scaling, int oversample, int power,
+                                int retain_scores, unsigned long long seed,
+                                char* error, size_t error_capacity);
+
+/* fastpls_resident_plssvd_create (src/cuda_resident_api.h:44-47). */
+void* fastpls_hip_plssvd_create(const void* x, const void* y, const int* labels,
+                                int precision, int n, int p, int q, int components,
+                                int scaling, int oversample, int power,
+                                int retain_scores, unsigned long long seed,
+                                char* error, size_t error_capacity);
+
+/* fastpls_resident_opls_create / fastpls_resident_kernelpls_create
+ * (src/cuda_resident_api.h:48-56): not part of this backend; they always
+ * return NULL with an "unsupported" message. */
+void* fastpls_hip_opls_create(const void* x, const void* y, const int* labels,
+                              int precision, int n, int p, int q, int components,
+                              int scaling, int oversample, int power, int retain_scores,
+                              unsigned long long seed, int north, char* error,
+                              size_t error_capacity);
+void* fastpls_hip_kernelpls_create(const void* x, const void* y, const int* labels,
+                                   int precision, int n, int p, int q, int components,
+                                   int scaling, int oversample, int power,
+                                   int retain_scores, unsigned long long seed, int kernel,
+                                   double gamma, int degree, double coef0, char* error,
+                                   size_t error_capacity);
+
+/* fastpls_resident_predict_path (src/cuda_resident_api.h:57-59): output is
+ * rows x q x prefix_count; lda != 0 returns LDA discriminant scores. */
+int fastpls_hip_predict_path(void* model, const void* x, int rows, const int* prefixes,
+                             int prefix_count, int lda, void* predictions, char* error,
+                             size_t error_capacity);
+
+/* fastpls_resident_simpls_destroy (src/cuda_resident_api.h:60). */
+void fastpls_hip_destroy(void* model);
+
+/* fastpls_resident_export (src/cuda_resident_api.h:61-62). Fields: 0 R (p*a),
+ * 1 Q (q*a), 2 T (n*a), 3 x means (p), 4 x scales (p), 5 y means / priors (q),
+ * 6 X loadings P (p*a), 7 variance sums of squares (a+1). */
+int fastpls_hip_export(void* model, int field, void* out, size_t elements, char* error,
+                       size_t error_capacity);
+
+/* fastpls_resident_classify_path (src/cuda_resident_api.h:63-65): labels are
+ * rows x top x prefix_count, one-based. */
+int fastpls_hip_classify_path(void* model, const void* x, int rows, const int* prefixes,
+                              int prefix_count, int lda, int top, int* labels, char* error,
+                              size_t error_capacity);
+
+/* fastpls_resident_classify_response_path (src/cuda_resident_api.h:66-68). */
+int fastpls_hip_classify_response_path(void* model, const void* x, int rows,
+                                       const int* prefixes, int prefix_count, int lda,
+                                       int top, int* labels, void* predictions, char* error,
+                                       size_t error_capacity);
+
+/* fastpls_resident_response_sums (src/cuda_resident_api.h:69-70): out is 3 x q
+ * (SSE, SS about the training mean, SS about the observed mean). */
+int fastpls_hip_response_sums(void* model, const void* x, const void* y, const int* labels,
+                              int rows, int prefix, void* out, char* error,
+                              size_t error_capacity);
+
+/* fastpls_resident_project (src/cuda_resident_api.h:71-72): scores rows x prefix. */
+int fastpls_hip_project(void* model, const void* x, int rows, int prefix, void* out,
+                        char* error, size_t error_capacity);
+
+/* fastpls_resident_controls (src/cuda_resident_api.h:73-75). */
+int fastpls_hip_controls(void* model, int* oversample, int* power, int* block,
+                         int* block_limit, int* implicit_operator,
+                         int* predictor_crossprod_cache, char* error,
+                         size_t error_capacity);
+
+/* fastpls_resident_compact (src/cuda_resident_api.h:76-77). */
+int fastpls_hip_compact(void* model, int prepare_lda, char* error, size_t error_capacity);
+
+/* Device-resident fit (no reference counterpart: the reference's
+ * fit_borrowed_device_* members of ResidentSimpls/ResidentPlssvd,
+ * src/cuda_resident_simpls.cuh:287-312, are not exported). x, y and labels
+ * are DEVICE pointers that the handle borrows until compact or destroy;
+ * method 0 = SIMPLS, 1 = PLS-SVD. Used when inputs already live in HBM. */
+void* fastpls_hip_fit_device(int method, const void* x_device, const void* y_device,
+                             const int* labels_device, int precision, int n, int p, int q,
+                             int components, int scaling, int oversample, int power,
+                             unsigned long long seed, char* error, size_t error_capacity);
+
+/* Seconds of device time of the last fit, measured with events on the
+ * handle's stream (no reference counterpart). */
+int fastpls_hip_last_fit_seconds(void* model, double* seconds, char* error,
+                                 size_t error_capacity);
+
+/* Device memory helpers for callers that keep inputs resident (no reference
+ * counterpart). kind: 0 host->device, 1 device->host, 2 device->device. */
+int fastpls_hip_device_alloc(size_t bytes, void** ptr, char* error, size_t error_capacity);
+int fastpls_hip_device_free(void* ptr, char* error, size_t error_capacity);
+int fastpls_hip_memcpy(void* dst, const void* src, size_t bytes, int kind, char* error,
+                       size_t error_capacity);
+
+/* Device-time breakdown of the last fit from events on the handle's stream
+ * (no reference counterpart): out[0] fit seconds, out[1] the one-pass
+ * statistics + cross-product kernel over X, out[2] the PLS-SVD score pass
+ * over X (SIMPLS: the whole component loop), out[3] bytes of X. */
+int fastpls_hip_fit_phases(void* model, double* out, int count, char* error,
+                           size_t error_capacity);
+
+/* Synthetic latent-factor data written straight into device memory
+ * (Philox4x32-10 + Box-Muller): X (n x p, precision 32/64) = T P' + noise E,
+ * Y (n x q, same precision) = T Q' + noise F, with factor column f scaled by
+ * decay^f. x_device / y_device are caller-allocated device buffers. */
+int fastpls_hip_latent_data(void* x_device, void* y_device, int precision, long long n,
+                            int p, int q, int factors, double decay, double noise,
+                            unsigned long long seed, char* error, size_t error_capacity);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

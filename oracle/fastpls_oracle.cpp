@@ -1,0 +1,1233 @@
+// fastpls oracle: TEST INFRASTRUCTURE ONLY.
+//
+// An FP64 CPU restatement of the fastPLS reference core, written for this
+// project, used solely as the checker for the HIP library (tests, the smoke
+// check and the CPU-baseline leg of the benchmark). Nothing in the product
+// path links or calls this file.
+//
+// Each function cites the reference file:line whose semantics it restates.
+// Pinning (how each function is checked against something other than itself)
+// is listed in the test files: golden vectors produced by compiling the
+// reference's own header core (oracle/_ref), the reference's known-answer
+// tests (core/tests/*.cpp), and closed-form / brute-force checks.
+//
+// Storage convention: column-major doubles, leading dimension = rows.
+// Labels at this boundary are one-based, as in src/cuda_resident_api.h:10.
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+using std::size_t;
+
+// ---------------------------------------------------------------- matrices
+struct Mat {
+  size_t r = 0, c = 0;
+  std::vector<double> d;
+  Mat() = default;
+  Mat(size_t rows, size_t cols) : r(rows), c(cols), d(rows * cols, 0.0) {}
+  void resize(size_t rows, size_t cols) { r = rows; c = cols; d.assign(rows * cols, 0.0); }
+  double& operator()(size_t i, size_t j) { return d[i + j * r]; }
+  double operator()(size_t i, size_t j) const { return d[i + j * r]; }
+  double* data() { return d.data(); }
+  const double* data() const { return d.data(); }
+};
+
+// C(m x n) = op(A) * op(B); op(A) is m x k. Plain loops in contiguous order.
+void gemm(const double* A, size_t lda, bool ta, const double* B, size_t ldb,
+          bool tb, size_t m, size_t n, size_t k, double* C, size_t ldc) {
+  for (size_t j = 0; j < n; ++j) {
+    double* cj = C + j * ldc;
+    if (ta) {
+      // C[i,j] = sum_l A[l,i] * op(B)[l,j]
+      for (size_t i = 0; i < m; ++i) {
+        const double* ai = A + i * lda;
+        double acc = 0.0;
+        if (!tb) {
+          const double* bj = B + j * ldb;
+          for (size_t l = 0; l < k; ++l) acc += ai[l] * bj[l];
+        } else {
+          for (size_t l = 0; l < k; ++l) acc += ai[l] * B[j + l * ldb];
+        }
+        cj[i] = acc;
+      }
+    } else {
+      for (size_t i = 0; i < m; ++i) cj[i] = 0.0;
+      for (size_t l = 0; l < k; ++l) {
+        const double b = tb ? B[j + l * ldb] : B[l + j * ldb];
+        if (b == 0.0) continue;
+        const double* al = A + l * lda;
+        for (size_t i = 0; i < m; ++i) cj[i] += al[i] * b;
+      }
+    }
+  }
+}
+
+Mat mul(const Mat& A, bool ta, const Mat& B, bool tb) {
+  const size_t m = ta ? A.c : A.r, k = ta ? A.r : A.c;
+  const size_t kb = tb ? B.c : B.r, n = tb ? B.r : B.c;
+  if (k != kb) throw std::invalid_argument("oracle gemm: inner dimensions differ");
+  Mat C(m, n);
+  if (m && n) gemm(A.data(), A.r, ta, B.data(), B.r, tb, m, n, k, C.data(), m);
+  return C;
+}
+
+// Householder economy QR returning Q (rows x min(rows, cols)); the same
+// factorisation LAPACK dgeqrf + dorgqr performs (src/core_cpu_lapack.cpp:49-96).
+bool qr_economy(const Mat& A, Mat& Q) {
+  const size_t m = A.r, n = A.c, k = std::min(m, n);
+  Mat W = A;
+  std::vector<double> tau(k, 0.0);
+  for (size_t j = 0; j < k; ++j) {
+    double norm2 = 0.0;
+    for (size_t i = j; i < m; ++i) norm2 += W(i, j) * W(i, j);
+    const double alpha = W(j, j);
+    double xnorm2 = norm2 - alpha * alpha;
+    if (!std::isfinite(norm2)) return false;
+    if (xnorm2 <= 0.0) { tau[j] = 0.0; continue; }
+    const double norm = std::sqrt(norm2);
+    const double beta = alpha >= 0 ? -norm : norm;
+    tau[j] = (beta - alpha) / beta;
+    const double scale = 1.0 / (alpha - beta);
+    for (size_t i = j + 1; i < m; ++i) W(i, j) *= scale;
+    W(j, j) = beta;
+    for (size_t c = j + 1; c < n; ++c) {
+      double s = W(j, c);
+      for (size_t i = j + 1; i < m; ++i) s += W(i, j) * W(i, c);
+      s *= tau[j];
+      W(j, c) -= s;
+      for (size_t i = j + 1; i < m; ++i) W(i, c) -= s * W(i, j);
+    }
+  }
+  Q.resize(m, k);
+  for (size_t j = 0; j < k; ++j) Q(j, j) = 1.0;
+  for (size_t jj = k; jj-- > 0;) {
+    if (tau[jj] == 0.0) continue;
+    for (size_t c = jj; c < k; ++c) {
+      double s = Q(jj, c);
+      for (size_t i = jj + 1; i < m; ++i) s += W(i, jj) * Q(i, c);
+      s *= tau[jj];
+      Q(jj, c) -= s;
+      for (size_t i = jj + 1; i < m; ++i) Q(i, c) -= s * W(i, jj);
+    }
+  }
+  for (double v : Q.d) if (!std::isfinite(v)) return false;
+  return true;
+}
+
+// Cyclic Jacobi symmetric eigensolver. On return the matrix holds the
+// eigenvectors as columns and the values ascend, the contract of LAPACK
+// dsyevd('V','L') used by the reference (src/core_cpu_lapack.cpp:98-135).
+bool symmetric_eigen(Mat& A, std::vector<double>& values) {
+  const size_t n = A.r;
+  if (n != A.c) return false;
+  for (double v : A.d) if (!std::isfinite(v)) return false;
+  Mat V(n, n);
+  for (size_t i = 0; i < n; ++i) V(i, i) = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, diag = 0.0;
+    for (size_t j = 0; j < n; ++j) {
+      diag += A(j, j) * A(j, j);
+      for (size_t i = 0; i < j; ++i) off += A(i, j) * A(i, j);
+    }
+    if (off <= 1e-34 * diag || off == 0.0) break;
+    for (size_t p = 0; p + 1 < n; ++p) {
+      for (size_t q = p + 1; q < n; ++q) {
+        const double apq = A(p, q);
+        if (apq == 0.0) continue;
+        const double theta = (A(q, q) - A(p, p)) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) /
+          (std::fabs(theta) + std::sqrt(1.0 + theta * theta));
+        const double cs = 1.0 / std::sqrt(1.0 + t * t), sn = t * cs;
+        for (size_t i = 0; i < n; ++i) {
+          const double aip = A(i, p), aiq = A(i, q);
+          A(i, p) = cs * aip - sn * aiq;
+          A(i, q) = sn * aip + cs * aiq;
+        }
+        for (size_t i = 0; i < n; ++i) {
+          const double api = A(p, i), aqi = A(q, i);
+          A(p, i) = cs * api - sn * aqi;
+          A(q, i) = sn * api + cs * aqi;
+        }
+        for (size_t i = 0; i < n; ++i) {
+          const double vip = V(i, p), viq = V(i, q);
+          V(i, p) = cs * vip - sn * viq;
+          V(i, q) = sn * vip + cs * viq;
+        }
+      }
+    }
+  }
+  std::vector<size_t> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::vector<double> diag(n);
+  for (size_t i = 0; i < n; ++i) diag[i] = A(i, i);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t b) { return diag[a] < diag[b]; });
+  values.resize(n);
+  Mat sorted(n, n);
+  for (size_t j = 0; j < n; ++j) {
+    values[j] = diag[order[j]];
+    for (size_t i = 0; i < n; ++i) sorted(i, j) = V(i, order[j]);
+  }
+  A = std::move(sorted);
+  return true;
+}
+
+// One-sided Jacobi economy SVD (singular values descending), the contract of
+// dgesvd/dgesdd as the reference calls them (src/core_cpu_lapack.cpp:137-205).
+bool svd_economy(const Mat& A, bool left_only, Mat& U,
+                 std::vector<double>& s, Mat& Vt) {
+  const bool tall = A.r >= A.c;
+  Mat W = tall ? A : Mat(A.c, A.r);
+  if (!tall) for (size_t i = 0; i < A.r; ++i) for (size_t j = 0; j < A.c; ++j) W(j, i) = A(i, j);
+  const size_t m = W.r, n = W.c;
+  Mat V(n, n);
+  for (size_t i = 0; i < n; ++i) V(i, i) = 1.0;
+  for (int sweep = 0; sweep < 80; ++sweep) {
+    bool rotated = false;
+    for (size_t p = 0; p + 1 < n; ++p) {
+      for (size_t q = p + 1; q < n; ++q) {
+        double alpha = 0, beta = 0, gamma = 0;
+        for (size_t i = 0; i < m; ++i) {
+          alpha += W(i, p) * W(i, p); beta += W(i, q) * W(i, q);
+          gamma += W(i, p) * W(i, q);
+        }
+        if (std::fabs(gamma) <= 1e-15 * std::sqrt(alpha * beta) || gamma == 0.0) continue;
+        rotated = true;
+        const double zeta = (beta - alpha) / (2.0 * gamma);
+        const double t = (zeta >= 0 ? 1.0 : -1.0) /
+          (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+        const double cs = 1.0 / std::sqrt(1.0 + t * t), sn = cs * t;
+        for (size_t i = 0; i < m; ++i) {
+          const double wp = W(i, p), wq = W(i, q);
+          W(i, p) = cs * wp - sn * wq; W(i, q) = sn * wp + cs * wq;
+        }
+        for (size_t i = 0; i < n; ++i) {
+          const double vp = V(i, p), vq = V(i, q);
+          V(i, p) = cs * vp - sn * vq; V(i, q) = sn * vp + cs * vq;
+        }
+      }
+    }
+    if (!rotated) break;
+  }
+  std::vector<double> norms(n);
+  for (size_t j = 0; j < n; ++j) {
+    double acc = 0;
+    for (size_t i = 0; i < m; ++i) acc += W(i, j) * W(i, j);
+    norms[j] = std::sqrt(acc);
+  }
+  std::vector<size_t> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t b) { return norms[a] > norms[b]; });
+  const size_t k = n;
+  s.resize(k);
+  Mat left(m, k), right(n, k);
+  for (size_t jj = 0; jj < k; ++jj) {
+    const size_t j = order[jj];
+    s[jj] = norms[j];
+    const double inv = norms[j] > 0 ? 1.0 / norms[j] : 0.0;
+    for (size_t i = 0; i < m; ++i) left(i, jj) = W(i, j) * inv;
+    for (size_t i = 0; i < n; ++i) right(i, jj) = V(i, j);
+  }
+  // For a wide input the roles of the factors swap.
+  const Mat& u = tall ? left : right;
+  const Mat& v = tall ? right : left;
+  U = u;
+  if (!left_only) {
+    Vt.resize(v.c, v.r);
+    for (size_t i = 0; i < v.r; ++i) for (size_t j = 0; j < v.c; ++j) Vt(j, i) = v(i, j);
+  }
+  for (double x : s) if (!std::isfinite(x)) return false;
+  return true;
+}
+
+// ------------------------------------------------------------ sketches
+// Gaussian sketch entries exactly as the reference draws them:
+// std::mt19937(seed) + std::normal_distribution<double>(0, 1), filled in
+// storage order (simpls.hpp:244-249, rsvd.hpp:205-210).
+void gaussian(unsigned int seed, size_t count, double* out) {
+  std::mt19937 generator(seed);
+  std::normal_distribution<double> normal(0.0, 1.0);
+  for (size_t i = 0; i < count; ++i) out[i] = normal(generator);
+}
+
+// ------------------------------------------------------------ rSVD
+struct Triplets { Mat U; std::vector<double> s; Mat Vt; };
+
+// rsvd.hpp:59-83
+Triplets dense_triplets(const Mat& A, size_t retained, bool left_only) {
+  Triplets full;
+  if (!svd_economy(A, left_only, full.U, full.s, full.Vt))
+    throw std::runtime_error("oracle rSVD dense decomposition failed");
+  retained = std::min(retained, std::min(full.s.size(), full.U.c));
+  Triplets out;
+  out.U.resize(full.U.r, retained);
+  for (size_t j = 0; j < retained; ++j) for (size_t i = 0; i < full.U.r; ++i) out.U(i, j) = full.U(i, j);
+  out.s.assign(full.s.begin(), full.s.begin() + retained);
+  if (!left_only) {
+    out.Vt.resize(retained, full.Vt.c);
+    for (size_t j = 0; j < full.Vt.c; ++j) for (size_t i = 0; i < retained; ++i) out.Vt(i, j) = full.Vt(i, j);
+  }
+  return out;
+}
+
+// rsvd.hpp:85-165
+Triplets finalize_sample(const Mat& A, const Mat& sample, size_t retained, bool left_only) {
+  Mat basis;
+  if (!qr_economy(sample, basis)) throw std::runtime_error("oracle rSVD sketch QR failed");
+  Mat projected = mul(basis, true, A, false);
+  if (projected.r <= projected.c) {
+    Mat gram = mul(projected, false, projected, true);
+    std::vector<double> ev;
+    if (symmetric_eigen(gram, ev)) {
+      const double largest = ev.empty() ? 1.0 : std::max(ev.back(), 1.0);
+      const double tol = std::numeric_limits<double>::epsilon() *
+        double(std::max(projected.r, projected.c)) * largest;
+      size_t usable = 0;
+      for (size_t idx = ev.size(); idx > 0; --idx) {
+        if (ev[idx - 1] <= tol || usable == retained) break;
+        ++usable;
+      }
+      if (usable > 0) {
+        Mat reduced_left(gram.r, usable);
+        std::vector<double> s(usable);
+        for (size_t c = 0; c < usable; ++c) {
+          const size_t src = ev.size() - 1 - c;
+          s[c] = std::sqrt(std::max(ev[src], 0.0));
+          for (size_t i = 0; i < gram.r; ++i) reduced_left(i, c) = gram(i, src);
+        }
+        Triplets out;
+        out.U = mul(basis, false, reduced_left, false);
+        out.s = s;
+        if (!left_only) {
+          out.Vt = mul(reduced_left, true, projected, false);
+          for (size_t i = 0; i < usable; ++i) {
+            const double inv = 1.0 / s[i];
+            for (size_t j = 0; j < out.Vt.c; ++j) out.Vt(i, j) *= inv;
+          }
+        }
+        return out;
+      }
+    }
+  }
+  Triplets reduced = dense_triplets(projected, retained, left_only);
+  Triplets out;
+  out.U = mul(basis, false, reduced.U, false);
+  out.s = reduced.s;
+  out.Vt = reduced.Vt;
+  return out;
+}
+
+// rsvd.hpp:185-237
+Triplets randomized_svd(const Mat& A, int retained, int oversample, int power,
+                        unsigned int seed, bool left_only) {
+  const size_t maximum = std::min(A.r, A.c);
+  const size_t target = std::min(maximum, size_t(std::max(retained, 1)));
+  if (target == 0) return {};
+  const size_t width = std::min(maximum, target + size_t(std::max(oversample, 0)));
+  if (width >= maximum) return dense_triplets(A, target, left_only);
+  Mat omega(A.c, width);
+  gaussian(seed, omega.d.size(), omega.data());
+  Mat sample = mul(A, false, omega, false);
+  for (int it = 0; it < std::max(power, 0); ++it) {
+    Mat fb;
+    if (!qr_economy(sample, fb)) throw std::runtime_error("oracle rSVD forward QR failed");
+    Mat rev = mul(A, true, fb, false);
+    Mat rb;
+    if (!qr_economy(rev, rb)) throw std::runtime_error("oracle rSVD reverse QR failed");
+    sample = mul(A, false, rb, false);
+  }
+  return finalize_sample(A, sample, target, left_only);
+}
+
+// ------------------------------------------------------------ preprocessing
+struct Prepared {
+  Mat S;                      // p x q cross-covariance
+  std::vector<double> mx, sx, my;
+  Mat Xs;                     // standardized predictors
+};
+
+// Column statistics by sums of squares (supervised.hpp:63-90,
+// classification.hpp:128-153).
+void column_stats(const Mat& X, int scaling, std::vector<double>& mx,
+                  std::vector<double>& sx) {
+  const size_t n = X.r, p = X.c;
+  mx.assign(p, 0.0); sx.assign(p, 1.0);
+  for (size_t j = 0; j < p; ++j) {
+    double sum = 0, ss = 0;
+    for (size_t i = 0; i < n; ++i) { const double v = X(i, j); sum += v; ss += v * v; }
+    if (scaling != 3) {
+      mx[j] = sum / double(n);
+      if (scaling == 2) {
+        const double css = std::max(0.0, ss - double(n) * mx[j] * mx[j]);
+        double sd = std::sqrt(css / double(std::max<size_t>(n - 1, 1)));
+        if (!std::isfinite(sd) || sd <= 0) sd = 1.0;
+        sx[j] = sd;
+      }
+    }
+  }
+}
+
+// supervised.hpp:27-124
+Prepared prepare_dense(const Mat& X, const Mat& Y, int scaling) {
+  if (X.r != Y.r || X.r == 0 || X.c == 0 || Y.c == 0)
+    throw std::invalid_argument("oracle dense cross-product dimensions are invalid");
+  Prepared out;
+  const size_t n = X.r, p = X.c, q = Y.c;
+  out.my.assign(q, 0.0);
+  for (size_t c = 0; c < q; ++c) {
+    double sum = 0;
+    for (size_t i = 0; i < n; ++i) sum += Y(i, c);
+    out.my[c] = sum / double(n);
+  }
+  column_stats(X, scaling, out.mx, out.sx);
+  out.Xs.resize(n, p);
+  std::vector<double> colsum(p, 0.0);
+  for (size_t j = 0; j < p; ++j) {
+    double s = 0;
+    for (size_t i = 0; i < n; ++i) {
+      const double v = (X(i, j) - out.mx[j]) / out.sx[j];
+      out.Xs(i, j) = v; s += v;
+    }
+    colsum[j] = s;
+  }
+  out.S = mul(out.Xs, true, Y, false);
+  for (size_t c = 0; c < q; ++c)
+    for (size_t j = 0; j < p; ++j) out.S(j, c) -= colsum[j] * out.my[c];
+  return out;
+}
+
+// classification.hpp:86-184 (labels are one-based here)
+Prepared prepare_labels(const Mat& X, const int* labels1, size_t K, int scaling) {
+  const size_t n = X.r, p = X.c;
+  if (K < 2 || n == 0 || p == 0) throw std::invalid_argument("oracle label cross-product dimensions are invalid");
+  Prepared out;
+  std::vector<double> counts(K, 0.0);
+  for (size_t i = 0; i < n; ++i) {
+    const int c = labels1[i] - 1;
+    if (c < 0 || size_t(c) >= K) throw std::invalid_argument("oracle label outside 1..K");
+    counts[size_t(c)] += 1.0;
+  }
+  out.my.assign(K, 0.0);
+  for (size_t c = 0; c < K; ++c) {
+    if (counts[c] == 0) throw std::invalid_argument("oracle empty class");
+    out.my[c] = counts[c] / double(n);
+  }
+  column_stats(X, scaling, out.mx, out.sx);
+  out.Xs.resize(n, p);
+  out.S.resize(p, K);
+  std::vector<double> sums(K);
+  for (size_t j = 0; j < p; ++j) {
+    std::fill(sums.begin(), sums.end(), 0.0);
+    double total = 0;
+    for (size_t i = 0; i < n; ++i) {
+      const double v = (X(i, j) - out.mx[j]) / out.sx[j];
+      out.Xs(i, j) = v;
+      sums[size_t(labels1[i] - 1)] += v; total += v;
+    }
+    for (size_t c = 0; c < K; ++c) out.S(j, c) = sums[c] - total * out.my[c];
+  }
+  return out;
+}
+
+// ------------------------------------------------------------ SIMPLS
+struct SimplsControls {
+  size_t components = 2, block = 1;
+  bool cache = false, reorth = false, store_scores = true, right_gram = true,
+       batch_geometry = false;
+  int oversample = 32, power = 5;
+  unsigned int seed = 1;
+};
+
+// simpls.hpp:43-62
+size_t candidate_block_size(size_t remaining, size_t p, size_t q, bool cls,
+                            size_t n, size_t maximum) {
+  const double work = double(n) * double(p) * double(q);
+  if (q <= 1 || remaining < 4 || work < 5.0e8) return 1;
+  if (cls && q <= 2048) return std::max<size_t>(1, std::min({maximum, remaining, p, q}));
+  if (!cls && double(p) * double(q) > 64.0 * 1024.0 * 1024.0)
+    return std::max<size_t>(1, std::min({size_t(8), maximum, remaining, p, q}));
+  return 1;
+}
+
+// src/r_api.cpp:1149-1190 with the default environment thresholds.
+SimplsControls simpls_controls(size_t n, size_t p, size_t q, size_t A, bool cls,
+                               int oversample, int power, unsigned int seed) {
+  SimplsControls c;
+  c.components = A;
+  c.block = candidate_block_size(A, p, q, cls, n, 64);
+  c.cache = A >= 20 && p <= n && n >= p * 8 && p <= 512;
+  c.batch_geometry = c.block > 1 && !c.cache;
+  c.reorth = false;
+  c.store_scores = true;
+  c.right_gram = true;
+  c.oversample = oversample; c.power = power; c.seed = seed;
+  return c;
+}
+
+// simpls.hpp:136-159 (classical Gram-Schmidt, one or two passes)
+void remove_basis(const Mat& basis, size_t used, std::vector<double>& v, bool twice) {
+  if (used == 0) return;
+  const size_t p = basis.r;
+  std::vector<double> proj(used);
+  for (int pass = 0; pass < (twice ? 2 : 1); ++pass) {
+    for (size_t j = 0; j < used; ++j) {
+      double s = 0;
+      for (size_t i = 0; i < p; ++i) s += basis(i, j) * v[i];
+      proj[j] = s;
+    }
+    std::vector<double> corr(p, 0.0);
+    for (size_t j = 0; j < used; ++j)
+      for (size_t i = 0; i < p; ++i) corr[i] += basis(i, j) * proj[j];
+    for (size_t i = 0; i < p; ++i) v[i] -= corr[i];
+  }
+}
+
+// simpls.hpp:161-200
+void remove_score_basis(const Mat& T, const Mat& R, size_t used,
+                        std::vector<double>& t, std::vector<double>& r) {
+  if (used == 0) return;
+  std::vector<double> proj(used);
+  for (int pass = 0; pass < 2; ++pass) {
+    for (size_t j = 0; j < used; ++j) {
+      double s = 0;
+      for (size_t i = 0; i < T.r; ++i) s += T(i, j) * t[i];
+      proj[j] = s;
+    }
+    for (size_t i = 0; i < T.r; ++i) {
+      double s = 0;
+      for (size_t j = 0; j < used; ++j) s += T(i, j) * proj[j];
+      t[i] -= s;
+    }
+    for (size_t i = 0; i < R.r; ++i) {
+      double s = 0;
+      for (size_t j = 0; j < used; ++j) s += R(i, j) * proj[j];
+      r[i] -= s;
+    }
+  }
+}
+
+double norm2(const std::vector<double>& v) {
+  double s = 0;
+  for (double x : v) s += x * x;
+  return std::sqrt(std::max(s, 0.0));
+}
+
+// simpls.hpp:226-368
+bool refresh_directions(const Mat& S, const Mat& G, bool use_right_gram,
+                        size_t retained, int oversample, int power,
+                        unsigned int seed, Mat& directions) {
+  const size_t maximum = std::min(S.r, S.c);
+  const size_t width = std::min(maximum, retained + size_t(std::max(oversample, 0)));
+  if (retained == 0 || width == 0) return false;
+  Mat random(S.c, width);
+  gaussian(seed, random.d.size(), random.data());
+  Mat sample;
+  if (use_right_gram) {
+    Mat reverse_basis = random;
+    for (int it = 0; it < std::max(power, 0); ++it) {
+      Mat rs = mul(G, false, reverse_basis, false);
+      if (!qr_economy(rs, reverse_basis)) return false;
+    }
+    if (power == 0) {
+      Mat tmp;
+      if (!qr_economy(reverse_basis, tmp)) return false;
+      reverse_basis = tmp;
+    }
+    sample = mul(S, false, reverse_basis, false);
+  } else {
+    sample = mul(S, false, random, false);
+    for (int it = 0; it < std::max(power, 0); ++it) {
+      Mat db, rb;
+      if (!qr_economy(sample, db)) return false;
+      Mat rs = mul(S, true, db, false);
+      if (!qr_economy(rs, rb)) return false;
+      sample = mul(S, false, rb, false);
+    }
+  }
+  Mat basis;
+  if (!qr_economy(sample, basis)) return false;
+  Mat reduced = mul(basis, true, S, false);
+  Mat gram = mul(reduced, false, reduced, true);
+  std::vector<double> ev;
+  Mat left;
+  if (!symmetric_eigen(gram, ev)) {
+    std::vector<double> s; Mat unused;
+    if (!svd_economy(reduced, true, left, s, unused)) return false;
+  } else {
+    const size_t available = std::min(retained, ev.size());
+    left.resize(gram.r, available);
+    for (size_t c = 0; c < available; ++c) {
+      const size_t src = ev.size() - 1 - c;
+      for (size_t i = 0; i < gram.r; ++i) left(i, c) = gram(i, src);
+    }
+  }
+  const size_t available = std::min(retained, left.c);
+  if (available == 0) return false;
+  Mat l2(left.r, available);
+  for (size_t j = 0; j < available; ++j) for (size_t i = 0; i < left.r; ++i) l2(i, j) = left(i, j);
+  directions = mul(basis, false, l2, false);
+  return true;
+}
+
+struct SimplsModel {
+  Mat R, Q, V, T;
+  size_t completed = 0;
+};
+
+// simpls.hpp:480-763 (explicit cross-covariance path)
+SimplsModel fit_simpls(const Mat& X, const Mat& S0, const SimplsControls& ctl) {
+  const size_t n = X.r, p = S0.r, q = S0.c;
+  if (ctl.components == 0 || X.c != p) throw std::invalid_argument("oracle SIMPLS needs conformable inputs");
+  const size_t maximum = std::min({ctl.components, p, std::max<size_t>(n - 1, 1)});
+  SimplsModel model;
+  model.R.resize(p, maximum); model.Q.resize(q, maximum); model.V.resize(p, maximum);
+  const bool retain_scores = ctl.store_scores || ctl.reorth;
+  if (retain_scores) model.T.resize(n, maximum);
+  Mat S = S0;
+  Mat XtX;
+  if (ctl.cache) XtX = mul(X, true, X, false);
+  const bool use_right_gram = ctl.right_gram && q <= p && q <= 512;
+  Mat G;
+  if (use_right_gram) G = mul(S, true, S, false);
+
+  size_t component = 0;
+  while (component < maximum) {
+    const size_t block = std::min({std::max<size_t>(ctl.block, 1), maximum - component, std::min(p, q)});
+    Mat candidates;
+    if (!refresh_directions(S, G, use_right_gram, block, ctl.oversample, ctl.power,
+                            ctl.seed + unsigned(component), candidates)) break;
+    const size_t available = std::min(block, candidates.c);
+    Mat cand_scores, cand_loadings;
+    if (ctl.batch_geometry && !ctl.cache) {
+      cand_scores = mul(X, false, candidates, false);
+      cand_loadings = mul(X, true, cand_scores, false);
+    }
+    bool stopped = false;
+    for (size_t cand = 0; cand < available && component < maximum; ++cand, ++component) {
+      std::vector<double> r(p), t(n, 0.0), w(p);
+      for (size_t i = 0; i < p; ++i) r[i] = candidates(i, cand);
+      if (ctl.cache) {
+        for (size_t i = 0; i < p; ++i) {
+          double s = 0;
+          for (size_t l = 0; l < p; ++l) s += XtX(i, l) * r[l];
+          w[i] = s;
+        }
+        double ns = 0;
+        for (size_t i = 0; i < p; ++i) ns += r[i] * w[i];
+        if (!std::isfinite(ns) || ns <= 0) { stopped = true; break; }
+        const double sn = std::sqrt(ns);
+        for (double& x : r) x /= sn;
+        for (double& x : w) x /= sn;
+        if (retain_scores) {
+          for (size_t l = 0; l < p; ++l) for (size_t i = 0; i < n; ++i) t[i] += X(i, l) * r[l];
+        }
+      } else {
+        if (ctl.batch_geometry) {
+          for (size_t i = 0; i < n; ++i) t[i] = cand_scores(i, cand);
+        } else {
+          for (size_t l = 0; l < p; ++l) for (size_t i = 0; i < n; ++i) t[i] += X(i, l) * r[l];
+        }
+        const bool reorthed = ctl.reorth && component > 0;
+        if (reorthed) remove_score_basis(model.T, model.R, component, t, r);
+        const double tn = norm2(t);
+        if (!std::isfinite(tn) || tn <= 0) { stopped = true; break; }
+        for (double& x : t) x /= tn;
+        for (double& x : r) x /= tn;
+        if (ctl.batch_geometry && !reorthed) {
+          for (size_t i = 0; i < p; ++i) w[i] = cand_loadings(i, cand) / tn;
+        } else {
+          for (size_t j = 0; j < p; ++j) {
+            double s = 0;
+            for (size_t i = 0; i < n; ++i) s += X(i, j) * t[i];
+            w[j] = s;
+          }
+        }
+      }
+      std::vector<double> qa(q);
+      for (size_t c = 0; c < q; ++c) {
+        double s = 0;
+        for (size_t i = 0; i < p; ++i) s += S0(i, c) * r[i];
+        qa[c] = s;
+      }
+      std::vector<double> v = w;
+      remove_basis(model.V, component, v, ctl.reorth);
+      const double vn = norm2(v);
+      if (!std::isfinite(vn) || vn <= 0) { stopped = true; break; }
+      for (double& x : v) x /= vn;
+      std::vector<double> row(q);
+      for (size_t c = 0; c < q; ++c) {
+        double s = 0;
+        for (size_t i = 0; i < p; ++i) s += v[i] * S(i, c);
+        row[c] = s;
+      }
+      for (size_t c = 0; c < q; ++c)
+        for (size_t i = 0; i < p; ++i) S(i, c) -= v[i] * row[c];
+      if (use_right_gram) {
+        for (size_t c = 0; c < q; ++c) for (size_t i = 0; i < q; ++i) G(i, c) -= row[i] * row[c];
+        for (size_t c = 0; c < q; ++c) for (size_t i = 0; i < c; ++i) {
+          const double val = 0.5 * (G(i, c) + G(c, i));
+          G(i, c) = val; G(c, i) = val;
+        }
+      }
+      for (size_t i = 0; i < p; ++i) { model.R(i, component) = r[i]; model.V(i, component) = v[i]; }
+      for (size_t c = 0; c < q; ++c) model.Q(c, component) = qa[c];
+      if (retain_scores) for (size_t i = 0; i < n; ++i) model.T(i, component) = t[i];
+      model.completed = component + 1;
+    }
+    if (stopped) break;
+  }
+  return model;
+}
+
+// ------------------------------------------------------------ PLS-SVD
+// linalg.hpp:57-177
+bool cholesky_solve(const Mat& A, const Mat& B, Mat& X) {
+  const size_t n = A.r;
+  Mat L(n, n);
+  for (size_t i = 0; i < n; ++i) {
+    for (size_t j = 0; j <= i; ++j) {
+      double v = A(i, j);
+      for (size_t k = 0; k < j; ++k) v -= L(i, k) * L(j, k);
+      if (i == j) {
+        if (!std::isfinite(v) || v <= 0) return false;
+        L(i, j) = std::sqrt(v);
+      } else {
+        L(i, j) = v / L(j, j);
+      }
+    }
+  }
+  X.resize(n, B.c);
+  for (size_t c = 0; c < B.c; ++c) {
+    for (size_t i = 0; i < n; ++i) {
+      double v = B(i, c);
+      for (size_t k = 0; k < i; ++k) v -= L(i, k) * X(k, c);
+      X(i, c) = v / L(i, i);
+    }
+    for (size_t ii = n; ii-- > 0;) {
+      double v = X(ii, c);
+      for (size_t k = ii + 1; k < n; ++k) v -= L(k, ii) * X(k, c);
+      X(ii, c) = v / L(ii, ii);
+      if (!std::isfinite(X(ii, c))) return false;
+    }
+  }
+  return true;
+}
+
+bool pivoted_solve(const Mat& A, const Mat& B, Mat& X) {
+  const size_t n = A.r;
+  Mat F = A;
+  X = B;
+  double scale = 0;
+  for (double v : F.d) scale = std::max(scale, std::fabs(v));
+  const double tol = std::numeric_limits<double>::epsilon() * double(std::max<size_t>(n, 1)) * std::max(scale, 1.0);
+  for (size_t piv = 0; piv < n; ++piv) {
+    size_t sel = piv;
+    for (size_t i = piv + 1; i < n; ++i) if (std::fabs(F(i, piv)) > std::fabs(F(sel, piv))) sel = i;
+    if (!std::isfinite(F(sel, piv)) || std::fabs(F(sel, piv)) <= tol) return false;
+    if (sel != piv) {
+      for (size_t c = 0; c < n; ++c) std::swap(F(piv, c), F(sel, c));
+      for (size_t c = 0; c < X.c; ++c) std::swap(X(piv, c), X(sel, c));
+    }
+    for (size_t i = piv + 1; i < n; ++i) {
+      const double mlt = F(i, piv) / F(piv, piv);
+      F(i, piv) = 0;
+      for (size_t c = piv + 1; c < n; ++c) F(i, c) -= mlt * F(piv, c);
+      for (size_t c = 0; c < X.c; ++c) X(i, c) -= mlt * X(piv, c);
+    }
+  }
+  for (size_t c = 0; c < X.c; ++c) {
+    for (size_t ii = n; ii-- > 0;) {
+      double v = X(ii, c);
+      for (size_t k = ii + 1; k < n; ++k) v -= F(ii, k) * X(k, c);
+      X(ii, c) = v / F(ii, ii);
+      if (!std::isfinite(X(ii, c))) return false;
+    }
+  }
+  return true;
+}
+
+bool solve_symmetric(const Mat& A, const Mat& B, Mat& X) {
+  return cholesky_solve(A, B, X) || pivoted_solve(A, B, X);
+}
+
+struct PlssvdModel {
+  Mat R, Q, T;
+  std::vector<double> s;
+  std::vector<Mat> W;     // prediction weights per prefix (k x q)
+  std::vector<Mat> latent;
+};
+
+// plssvd.hpp:57-166, 197-220
+PlssvdModel fit_plssvd(const Mat& X, const Mat& S, const std::vector<int>& comps,
+                       int oversample, int power, unsigned int seed) {
+  const size_t p = S.r, q = S.c;
+  const size_t bound = std::min(p, q);
+  size_t retained = 0;
+  for (int c : comps) {
+    if (c < 1 || size_t(c) > bound) throw std::invalid_argument("oracle PLS-SVD component count exceeds cross-covariance rank");
+    retained = std::max(retained, size_t(c));
+  }
+  Triplets dec = randomized_svd(S, int(retained), oversample, power, seed, false);
+  const size_t done = std::min({dec.U.c, dec.Vt.r, dec.s.size()});
+  if (done < retained) throw std::runtime_error("oracle PLS-SVD returned fewer components than requested");
+  PlssvdModel m;
+  m.R.resize(p, retained); m.Q.resize(q, retained);
+  m.s.assign(dec.s.begin(), dec.s.begin() + retained);
+  for (size_t k = 0; k < retained; ++k) {
+    for (size_t i = 0; i < p; ++i) m.R(i, k) = dec.U(i, k);
+    for (size_t c = 0; c < q; ++c) m.Q(c, k) = dec.Vt(k, c);
+  }
+  m.T = mul(X, false, m.R, false);
+  Mat G = mul(m.T, true, m.T, false);
+  for (int req : comps) {
+    const size_t k = size_t(req);
+    Mat g(k, k), diag(k, k);
+    for (size_t j = 0; j < k; ++j) {
+      diag(j, j) = m.s[j];
+      for (size_t i = 0; i < k; ++i) g(i, j) = G(i, j);
+    }
+    Mat lat;
+    if (!solve_symmetric(g, diag, lat)) throw std::runtime_error("oracle PLS-SVD latent solve failed");
+    Mat qk(q, k);
+    for (size_t j = 0; j < k; ++j) for (size_t c = 0; c < q; ++c) qk(c, j) = m.Q(c, j);
+    m.W.push_back(mul(lat, false, qk, true));
+    m.latent.push_back(lat);
+  }
+  return m;
+}
+
+// ------------------------------------------------------------ LDA
+struct Lda { Mat means, linear; std::vector<double> constants, priors; double ridge = 0, relative = 0; };
+
+// lda.hpp:32-84
+bool lda_chol_solve(const Mat& pooled, const Mat& means, double ridge, Mat& linear) {
+  const size_t a = pooled.r, K = means.r;
+  Mat L(a, a);
+  for (size_t i = 0; i < a; ++i) {
+    for (size_t j = 0; j <= i; ++j) {
+      double v = pooled(i, j);
+      if (i == j) v += ridge;
+      for (size_t k = 0; k < j; ++k) v -= L(i, k) * L(j, k);
+      if (i == j) {
+        if (!std::isfinite(v) || v <= 0) return false;
+        L(i, j) = std::sqrt(v);
+      } else {
+        const double dg = L(j, j);
+        if (!std::isfinite(dg) || dg <= 0) return false;
+        L(i, j) = v / dg;
+      }
+    }
+  }
+  Mat sol(a, K);
+  for (size_t c = 0; c < K; ++c) {
+    for (size_t i = 0; i < a; ++i) {
+      double v = means(c, i);
+      for (size_t k = 0; k < i; ++k) v -= L(i, k) * sol(k, c);
+      sol(i, c) = v / L(i, i);
+    }
+    for (size_t ii = a; ii-- > 0;) {
+      double v = sol(ii, c);
+      for (size_t k = ii + 1; k < a; ++k) v -= L(k, ii) * sol(k, c);
+      v /= L(ii, ii);
+      if (!std::isfinite(v)) return false;
+      sol(ii, c) = v;
+    }
+  }
+  linear.resize(K, a);
+  for (size_t c = 0; c < K; ++c) for (size_t j = 0; j < a; ++j) linear(c, j) = sol(j, c);
+  return true;
+}
+
+// lda.hpp:86-223
+std::vector<Lda> train_lda(const Mat& T, const int* labels1, size_t K,
+                           const std::vector<int>& prefixes) {
+  const size_t n = T.r;
+  size_t maximum = 0;
+  for (int pfx : prefixes) {
+    if (pfx < 1 || size_t(pfx) > T.c) throw std::invalid_argument("oracle LDA prefix outside score dimension");
+    maximum = std::max(maximum, size_t(pfx));
+  }
+  std::vector<double> counts(K, 0.0);
+  Mat means(K, maximum);
+  for (size_t i = 0; i < n; ++i) {
+    const int c = labels1[i] - 1;
+    if (c < 0 || size_t(c) >= K) throw std::invalid_argument("oracle LDA label outside 1..K");
+    counts[size_t(c)] += 1;
+    for (size_t j = 0; j < maximum; ++j) means(size_t(c), j) += T(i, j);
+  }
+  for (size_t c = 0; c < K; ++c) {
+    if (counts[c] <= 0) throw std::invalid_argument("oracle LDA empty class");
+    for (size_t j = 0; j < maximum; ++j) means(c, j) /= counts[c];
+  }
+  Mat pooled(maximum, maximum);
+  const double den = double(std::max<size_t>(1, n > K ? n - K : 1));
+  for (size_t c2 = 0; c2 < maximum; ++c2) {
+    for (size_t r2 = 0; r2 <= c2; ++r2) {
+      double v = 0;
+      for (size_t i = 0; i < n; ++i) v += T(i, r2) * T(i, c2);
+      for (size_t c = 0; c < K; ++c) v -= counts[c] * means(c, r2) * means(c, c2);
+      v /= den;
+      pooled(r2, c2) = v; pooled(c2, r2) = v;
+    }
+  }
+  static const double grid[] = {1e-8, 1e-6, 1e-5, 1e-4, 1e-3, 1e-2};
+  std::vector<Lda> models;
+  for (int pfx : prefixes) {
+    const size_t k = size_t(pfx);
+    Lda m;
+    m.means.resize(K, k);
+    Mat cov(k, k);
+    double scale = 0;
+    for (size_t j = 0; j < k; ++j) {
+      scale += pooled(j, j);
+      for (size_t i = 0; i < k; ++i) cov(i, j) = pooled(i, j);
+      for (size_t c = 0; c < K; ++c) m.means(c, j) = means(c, j);
+    }
+    scale /= double(k);
+    if (!std::isfinite(scale) || scale <= 0) scale = 1.0;
+    bool solved = false;
+    for (double rel : grid) {
+      if (lda_chol_solve(cov, m.means, rel * scale, m.linear)) {
+        m.ridge = rel * scale; m.relative = rel; solved = true; break;
+      }
+    }
+    if (!solved) throw std::runtime_error("oracle LDA Cholesky failed for every ridge");
+    m.constants.resize(K); m.priors.resize(K);
+    for (size_t c = 0; c < K; ++c) {
+      m.priors[c] = counts[c] / double(n);
+      double prod = 0;
+      for (size_t j = 0; j < k; ++j) prod += m.means(c, j) * m.linear(c, j);
+      m.constants[c] = -0.5 * prod + std::log(std::max(m.priors[c], std::numeric_limits<double>::min()));
+    }
+    models.push_back(std::move(m));
+  }
+  return models;
+}
+
+// Top-k class indices (one-based) with the reference tie rule: the earlier
+// class wins a tie (src/cuda_resident_api.cuh:29-53, classification.hpp:237-243).
+void topk_rows(const double* values, size_t rows, size_t classes, size_t top, int* out) {
+  std::vector<size_t> idx(classes);
+  for (size_t r = 0; r < rows; ++r) {
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+      const double va = values[r + a * rows], vb = values[r + b * rows];
+      if (std::isnan(va)) return false;
+      if (std::isnan(vb)) return true;
+      return va > vb;
+    });
+    for (size_t k = 0; k < top; ++k) out[r + k * rows] = int(idx[k]) + 1;
+  }
+}
+
+thread_local std::string g_error;
+
+template<class F>
+int guarded(F&& f) {
+  try { f(); return 0; } catch (const std::exception& e) { g_error = e.what(); return 1; }
+}
+
+Mat wrap(const double* x, size_t r, size_t c) {
+  Mat m(r, c);
+  if (r != 0 && c != 0) std::memcpy(m.data(), x, sizeof(double) * r * c);
+  return m;
+}
+
+void copy_out(const Mat& m, double* out) {
+  if (out && !m.d.empty()) std::memcpy(out, m.data(), sizeof(double) * m.d.size());
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* orc_last_error() { return g_error.c_str(); }
+
+void orc_gaussian(unsigned int seed, size_t count, double* out) { gaussian(seed, count, out); }
+
+// Dense preprocessing (supervised.hpp:27-124). xs may be null.
+int orc_prepare_dense(const double* x, const double* y, int n, int p, int q,
+                      int scaling, double* S, double* mx, double* sx, double* my,
+                      double* xs) {
+  return guarded([&] {
+    Prepared pr = prepare_dense(wrap(x, n, p), wrap(y, n, q), scaling);
+    copy_out(pr.S, S);
+    std::copy(pr.mx.begin(), pr.mx.end(), mx);
+    std::copy(pr.sx.begin(), pr.sx.end(), sx);
+    std::copy(pr.my.begin(), pr.my.end(), my);
+    if (xs) copy_out(pr.Xs, xs);
+  });
+}
+
+// Label preprocessing (classification.hpp:86-184); labels one-based.
+int orc_prepare_labels(const double* x, const int* labels, int n, int p, int K,
+                       int scaling, double* S, double* mx, double* sx,
+                       double* priors, double* xs) {
+  return guarded([&] {
+    Prepared pr = prepare_labels(wrap(x, n, p), labels, size_t(K), scaling);
+    copy_out(pr.S, S);
+    std::copy(pr.mx.begin(), pr.mx.end(), mx);
+    std::copy(pr.sx.begin(), pr.sx.end(), sx);
+    std::copy(pr.my.begin(), pr.my.end(), priors);
+    if (xs) copy_out(pr.Xs, xs);
+  });
+}
+
+// Routing decisions (src/r_api.cpp:1149-1190); out = {block, cache, batch_geometry}.
+void orc_simpls_controls(int n, int p, int q, int A, int classification, int* out) {
+  SimplsControls c = simpls_controls(n, p, q, A, classification != 0, 32, 5, 1);
+  out[0] = int(c.block); out[1] = c.cache ? 1 : 0; out[2] = c.batch_geometry ? 1 : 0;
+}
+
+// Full SIMPLS fit: preprocessing + routing + component loop.
+// y (n x q) or labels (one-based, q classes). Outputs sized p*A, q*A, p*A, n*A.
+// reorth: -1 keeps the routing default, 0/1 forces it.
+int orc_simpls_fit(const double* x, const double* y, const int* labels, int n,
+                   int p, int q, int ncomp, int scaling, int oversample,
+                   int power, unsigned int seed, int reorth, double* R,
+                   double* Q, double* V, double* T, double* mx, double* sx,
+                   double* my, int* completed) {
+  return guarded([&] {
+    const Mat X = wrap(x, n, p);
+    Prepared pr = labels ? prepare_labels(X, labels, size_t(q), scaling)
+                         : prepare_dense(X, wrap(y, n, q), scaling);
+    const size_t cap = std::min<size_t>(size_t(p), std::max<size_t>(size_t(n) - 1, 1));
+    const size_t A = std::max<size_t>(1, std::min<size_t>(size_t(ncomp), cap));
+    SimplsControls ctl = simpls_controls(n, p, q, A, labels != nullptr, oversample, power, seed);
+    if (reorth >= 0) ctl.reorth = reorth != 0;
+    SimplsModel m = fit_simpls(pr.Xs, pr.S, ctl);
+    copy_out(m.R, R); copy_out(m.Q, Q); copy_out(m.V, V);
+    if (T) copy_out(m.T, T);
+    std::copy(pr.mx.begin(), pr.mx.end(), mx);
+    std::copy(pr.sx.begin(), pr.sx.end(), sx);
+    std::copy(pr.my.begin(), pr.my.end(), my);
+    *completed = int(m.completed);
+  });
+}
+
+// SIMPLS core on prepared inputs with explicit controls (used by the
+// known-answer tests restated from core/tests/test_core_simpls.cpp).
+int orc_simpls_core(const double* xs, const double* S0, int n, int p, int q,
+                    int components, int block, int cache, int reorth,
+                    int batch_geometry, int oversample, int power,
+                    unsigned int seed, double* R, double* Q, double* V,
+                    double* T, int* completed) {
+  return guarded([&] {
+    SimplsControls ctl;
+    ctl.components = size_t(components); ctl.block = size_t(block);
+    ctl.cache = cache != 0; ctl.reorth = reorth != 0;
+    ctl.batch_geometry = batch_geometry != 0;
+    ctl.oversample = oversample; ctl.power = power; ctl.seed = seed;
+    SimplsModel m = fit_simpls(wrap(xs, n, p), wrap(S0, p, q), ctl);
+    copy_out(m.R, R); copy_out(m.Q, Q); copy_out(m.V, V);
+    if (T) copy_out(m.T, T);
+    *completed = int(m.completed);
+  });
+}
+
+// Explicit randomized SVD (rsvd.hpp:185-237). U m*k, s k, Vt k*n.
+int orc_rsvd(const double* a, int m, int n, int k, int oversample, int power,
+             unsigned int seed, double* U, double* s, double* Vt, int* got) {
+  return guarded([&] {
+    Triplets t = randomized_svd(wrap(a, m, n), k, oversample, power, seed, false);
+    copy_out(t.U, U);
+    std::copy(t.s.begin(), t.s.end(), s);
+    copy_out(t.Vt, Vt);
+    *got = int(t.s.size());
+  });
+}
+
+// PLS-SVD fit for prefixes 1..A (plssvd.hpp:57-220). W holds A slabs of A*q
+// doubles; slab k-1 stores the k x q weights with leading dimension k.
+int orc_plssvd_fit(const double* x, const double* y, const int* labels, int n,
+                   int p, int q, int A, int scaling, int oversample, int power,
+                   unsigned int seed, double* R, double* Q, double* T,
+                   double* sigma, double* W, double* mx, double* sx,
+                   double* my) {
+  return guarded([&] {
+    const Mat X = wrap(x, n, p);
+    Prepared pr = labels ? prepare_labels(X, labels, size_t(q), scaling)
+                         : prepare_dense(X, wrap(y, n, q), scaling);
+    std::vector<int> comps(A);
+    std::iota(comps.begin(), comps.end(), 1);
+    PlssvdModel m = fit_plssvd(pr.Xs, pr.S, comps, oversample, power, seed);
+    copy_out(m.R, R); copy_out(m.Q, Q);
+    if (T) copy_out(m.T, T);
+    std::copy(m.s.begin(), m.s.end(), sigma);
+    if (W) {
+      for (int k = 1; k <= A; ++k)
+        std::memcpy(W + size_t(k - 1) * A * q, m.W[k - 1].data(), sizeof(double) * k * q);
+    }
+    std::copy(pr.mx.begin(), pr.mx.end(), mx);
+    std::copy(pr.sx.begin(), pr.sx.end(), sx);
+    std::copy(pr.my.begin(), pr.my.end(), my);
+  });
+}
+
+// PLS-SVD core on a prepared predictor block and cross-product (for the
+// known-answer test of core/tests/test_core_plssvd.cpp).
+int orc_plssvd_core(const double* xs, const double* S, int n, int p, int q,
+                    const int* comps, int count, int oversample, int power,
+                    unsigned int seed, double* R, double* Q, double* T,
+                    double* sigma, double* W_last) {
+  return guarded([&] {
+    std::vector<int> c(comps, comps + count);
+    PlssvdModel m = fit_plssvd(wrap(xs, n, p), wrap(S, p, q), c, oversample, power, seed);
+    copy_out(m.R, R); copy_out(m.Q, Q); copy_out(m.T, T);
+    std::copy(m.s.begin(), m.s.end(), sigma);
+    copy_out(m.W.back(), W_last);
+  });
+}
+
+// Ŷ for SIMPLS at one prefix: ((X - mx)/sx) R_k Q_kᵀ + my
+// (cuda_resident_simpls.cuh:332-370, simpls.hpp:1049-1076).
+int orc_simpls_predict(const double* x, int rows, int p, int q, int A,
+                       const double* R, const double* Q, const double* mx,
+                       const double* sx, const double* my, int prefix,
+                       double* out) {
+  return guarded([&] {
+    Mat Xs(rows, p);
+    for (int j = 0; j < p; ++j) for (int i = 0; i < rows; ++i)
+      Xs(i, j) = (x[i + size_t(j) * rows] - mx[j]) / sx[j];
+    Mat Rk(p, prefix), Qk(q, prefix);
+    for (int k = 0; k < prefix; ++k) {
+      for (int i = 0; i < p; ++i) Rk(i, k) = R[i + size_t(k) * p];
+      for (int c = 0; c < q; ++c) Qk(c, k) = Q[c + size_t(k) * q];
+    }
+    (void)A;
+    Mat Tk = mul(Xs, false, Rk, false);
+    Mat Yh = mul(Tk, false, Qk, true);
+    for (int c = 0; c < q; ++c) for (int i = 0; i < rows; ++i) Yh(i, c) += my[c];
+    copy_out(Yh, out);
+  });
+}
+
+// LDA training on scores (lda.hpp:156-223) followed by discriminant scores
+// for a test score block (lda.hpp:308-327). disc is rows x K; labels out
+// one-based by first maximum.
+int orc_lda(const double* T, int n, int a, const int* labels, int K, int prefix,
+            const double* test_scores, int rows, double* disc, int* predicted,
+            double* ridge) {
+  return guarded([&] {
+    std::vector<int> pf{prefix};
+    auto models = train_lda(wrap(T, n, a), labels, size_t(K), pf);
+    const Lda& m = models[0];
+    for (int c = 0; c < K; ++c) {
+      for (int i = 0; i < rows; ++i) {
+        double v = m.constants[size_t(c)];
+        for (int j = 0; j < prefix; ++j) v += test_scores[i + size_t(j) * rows] * m.linear(size_t(c), size_t(j));
+        disc[i + size_t(c) * rows] = v;
+      }
+    }
+    for (int i = 0; i < rows; ++i) {
+      int best = 0;
+      double bv = disc[i];
+      for (int c = 1; c < K; ++c) if (disc[i + size_t(c) * rows] > bv) { bv = disc[i + size_t(c) * rows]; best = c; }
+      predicted[i] = best + 1;
+    }
+    *ridge = m.ridge;
+  });
+}
+
+void orc_topk(const double* values, int rows, int classes, int top, int* out) {
+  topk_rows(values, size_t(rows), size_t(classes), size_t(top), out);
+}
+
+// Per-response SSE, SS about the training mean, SS about the observed mean
+// (cuda_resident_metrics.cuh:9-43). out is 3 x q.
+int orc_response_sums(const double* pred, const double* observed,
+                      const int* labels, const double* train_mean, int n,
+                      int q, double* out) {
+  return guarded([&] {
+    for (int c = 0; c < q; ++c) {
+      double sum = 0;
+      for (int i = 0; i < n; ++i) {
+        if (labels && (labels[i] < 1 || labels[i] > q)) throw std::invalid_argument("oracle label outside 1..q");
+        const double y = labels ? double(labels[i] == c + 1) : observed[i + size_t(c) * n];
+        if (!std::isfinite(y)) throw std::invalid_argument("oracle non-finite response");
+        sum += y;
+      }
+      const double mean = sum / double(n);
+      double sse = 0, tss = 0, oss = 0;
+      for (int i = 0; i < n; ++i) {
+        const double y = labels ? double(labels[i] == c + 1) : observed[i + size_t(c) * n];
+        const double pr = pred[i + size_t(c) * n];
+        if (!std::isfinite(pr)) throw std::invalid_argument("oracle non-finite prediction");
+        const double e = y - pr, d = y - train_mean[c], v = y - mean;
+        sse += e * e; tss += d * d; oss += v * v;
+      }
+      out[3 * c] = sse; out[3 * c + 1] = tss; out[3 * c + 2] = oss;
+    }
+  });
+}
+
+// X-variance explained without an n x p residual
+// (cuda_resident_variance.cuh:10-98). P is p x a; ss has a + 1 entries.
+int orc_variance(const double* xs, const double* T, int n, int p, int a,
+                 double* P, double* ss) {
+  return guarded([&] {
+    Mat X = wrap(xs, n, p), Tm = wrap(T, n, a);
+    Mat cross = mul(X, true, Tm, false);
+    Mat gram = mul(Tm, true, Tm, false);
+    Mat seq(p, a);
+    for (int f = 0; f < p; ++f) {
+      for (int j = 0; j < a; ++j) {
+        const double den = gram(j, j);
+        double v = cross(f, j);
+        P[f + size_t(j) * p] = (std::isfinite(den) && den > 0) ? v / den : 0.0;
+        for (int i = 0; i < j; ++i) v -= seq(f, i) * gram(i, j);
+        seq(f, j) = (std::isfinite(den) && den > 0) ? v / den : 0.0;
+      }
+    }
+    double total = 0;
+    for (double v : X.d) total += v * v;
+    for (int j = 0; j < a; ++j) {
+      double g = 0;
+      for (int f = 0; f < p; ++f) g += seq(f, j) * seq(f, j);
+      const double gain = g * gram(j, j);
+      ss[j] = (std::isfinite(gain) && gain > 0) ? gain : 0.0;
+    }
+    ss[a] = total;
+  });
+}
+
+// General product with the reference's fastpls_cuda_gemm contract
+// (src/cuda_resident_api.h:11-13): out (rows x cols) = left (rows x inner) * right.
+void orc_gemm(const double* left, const double* right, int rows, int inner,
+              int cols, double* out) {
+  gemm(left, size_t(rows), false, right, size_t(inner), false, size_t(rows),
+       size_t(cols), size_t(inner), out, size_t(rows));
+}
+
+// Symmetric eigen and economy SVD exposed for their closed-form checks.
+int orc_symmetric_eigen(double* a, int n, double* values) {
+  return guarded([&] {
+    Mat m = wrap(a, n, n);
+    std::vector<double> ev;
+    if (!symmetric_eigen(m, ev)) throw std::runtime_error("oracle eigen failed");
+    copy_out(m, a);
+    std::copy(ev.begin(), ev.end(), values);
+  });
+}
+
+int orc_qr(const double* a, int m, int n, double* q) {
+  return guarded([&] {
+    Mat Q;
+    if (!qr_economy(wrap(a, m, n), Q)) throw std::runtime_error("oracle QR failed");
+    copy_out(Q, q);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,332 @@
+"""ctypes bindings for the FP64 oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` import this module, and only as the checker. The product path
+(``fastpls_amd``) never imports it.
+
+``Oracle`` wraps ``oracle/liboracle.so`` (the restatement in
+``fastpls_oracle.cpp``); ``Reference`` wraps ``oracle/_ref/libfastpls_ref.so``
+(the reference's own header core compiled by ``oracle/Makefile``), which is
+present only where the reference checkout was available to build it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libfastpls_ref.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip) if a is not None else None
+
+
+def _f64(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def build(reference: bool = False) -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    if reference:
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+def _labels(labels):
+    if labels is None:
+        return None
+    return np.ascontiguousarray(np.asarray(labels, dtype=np.int32))
+
+
+class Oracle:
+    """FP64 restatement of the reference algorithms (column-major)."""
+
+    def __init__(self, path: str = ORACLE_SO):
+        if not os.path.exists(path):
+            build()
+        self.lib = C.CDLL(path)
+        self.lib.orc_last_error.restype = C.c_char_p
+
+    def _check(self, status):
+        if status != 0:
+            raise RuntimeError(self.lib.orc_last_error().decode())
+
+    def gaussian(self, seed: int, count: int) -> np.ndarray:
+        out = np.empty(count, dtype=np.float64)
+        self.lib.orc_gaussian(C.c_uint(seed), C.c_size_t(count), _d(out))
+        return out
+
+    def prepare(self, x, y=None, labels=None, classes=None, scaling=2):
+        x = _f64(x)
+        n, p = x.shape
+        xs = np.empty((n, p), dtype=np.float64, order="F")
+        mx = np.empty(p); sx = np.empty(p)
+        if labels is not None:
+            lab = _labels(labels)
+            q = int(classes)
+            S = np.empty((p, q), order="F"); my = np.empty(q)
+            self._check(self.lib.orc_prepare_labels(_d(x), _i(lab), n, p, q, scaling,
+                                                    _d(S), _d(mx), _d(sx), _d(my), _d(xs)))
+        else:
+            y = _f64(y)
+            q = y.shape[1]
+            S = np.empty((p, q), order="F"); my = np.empty(q)
+            self._check(self.lib.orc_prepare_dense(_d(x), _d(y), n, p, q, scaling,
+                                                   _d(S), _d(mx), _d(sx), _d(my), _d(xs)))
+        return dict(S=S, mx=mx, sx=sx, my=my, xs=xs)
+
+    def controls(self, n, p, q, A, classification=False):
+        out = np.zeros(3, dtype=np.int32)
+        self.lib.orc_simpls_controls(n, p, q, A, int(classification), _i(out))
+        return dict(block=int(out[0]), cache=int(out[1]), batch_geometry=int(out[2]))
+
+    def simpls_fit(self, x, y=None, labels=None, classes=None, ncomp=2, scaling=2,
+                   oversample=32, power=5, seed=1, reorth=-1):
+        x = _f64(x)
+        n, p = x.shape
+        lab = _labels(labels)
+        if lab is not None:
+            q = int(classes); yy = None
+        else:
+            yy = _f64(y); q = yy.shape[1]
+        A = max(1, min(ncomp, p, max(n - 1, 1)))
+        R = np.zeros((p, A), order="F"); Q = np.zeros((q, A), order="F")
+        V = np.zeros((p, A), order="F"); T = np.zeros((n, A), order="F")
+        mx = np.empty(p); sx = np.empty(p); my = np.empty(q)
+        done = C.c_int(0)
+        self._check(self.lib.orc_simpls_fit(_d(x), _d(yy), _i(lab), n, p, q, ncomp, scaling,
+                                            oversample, power, C.c_uint(seed), reorth,
+                                            _d(R), _d(Q), _d(V), _d(T), _d(mx), _d(sx), _d(my),
+                                            C.byref(done)))
+        return dict(R=R, Q=Q, V=V, T=T, mx=mx, sx=sx, my=my, completed=done.value)
+
+    def simpls_core(self, xs, S0, components, block=1, cache=0, reorth=0, batch_geometry=0,
+                    oversample=32, power=5, seed=1):
+        xs = _f64(xs); S0 = _f64(S0)
+        n, p = xs.shape; q = S0.shape[1]
+        A = max(1, min(components, p, max(n - 1, 1)))
+        R = np.zeros((p, A), order="F"); Q = np.zeros((q, A), order="F")
+        V = np.zeros((p, A), order="F"); T = np.zeros((n, A), order="F")
+        done = C.c_int(0)
+        self._check(self.lib.orc_simpls_core(_d(xs), _d(S0), n, p, q, components, block, cache,
+                                             reorth, batch_geometry, oversample, power,
+                                             C.c_uint(seed), _d(R), _d(Q), _d(V), _d(T),
+                                             C.byref(done)))
+        return dict(R=R, Q=Q, V=V, T=T, completed=done.value)
+
+    def rsvd(self, a, k, oversample=32, power=5, seed=1):
+        a = _f64(a)
+        m, n = a.shape
+        kk = min(k, m, n)
+        U = np.zeros((m, kk), order="F"); s = np.zeros(kk); Vt = np.zeros((kk, n), order="F")
+        got = C.c_int(0)
+        self._check(self.lib.orc_rsvd(_d(a), m, n, k, oversample, power, C.c_uint(seed),
+                                      _d(U), _d(s), _d(Vt), C.byref(got)))
+        g = got.value
+        return dict(U=U[:, :g], s=s[:g], Vt=np.asfortranarray(Vt[:g]))
+
+    def plssvd_fit(self, x, y=None, labels=None, classes=None, ncomp=2, scaling=2,
+                   oversample=32, power=5, seed=1):
+        x = _f64(x)
+        n, p = x.shape
+        lab = _labels(labels)
+        if lab is not None:
+            q = int(classes); yy = None
+        else:
+            yy = _f64(y); q = yy.shape[1]
+        A = ncomp
+        R = np.zeros((p, A), order="F"); Q = np.zeros((q, A), order="F")
+        T = np.zeros((n, A), order="F"); s = np.zeros(A)
+        W = np.zeros(A * A * q)
+        mx = np.empty(p); sx = np.empty(p); my = np.empty(q)
+        self._check(self.lib.orc_plssvd_fit(_d(x), _d(yy), _i(lab), n, p, q, A, scaling,
+                                            oversample, power, C.c_uint(seed), _d(R), _d(Q),
+                                            _d(T), _d(s), _d(W), _d(mx), _d(sx), _d(my)))
+        weights = [W[k * A * q:k * A * q + (k + 1) * q].reshape((k + 1, q), order="F")
+                   for k in range(A)]
+        return dict(R=R, Q=Q, T=T, sigma=s, W=weights, mx=mx, sx=sx, my=my)
+
+    def plssvd_core(self, xs, S, comps, oversample=32, power=5, seed=1):
+        xs = _f64(xs); S = _f64(S)
+        n, p = xs.shape; q = S.shape[1]
+        comps = np.ascontiguousarray(np.asarray(comps, dtype=np.int32))
+        A = int(comps.max())
+        R = np.zeros((p, A), order="F"); Q = np.zeros((q, A), order="F")
+        T = np.zeros((n, A), order="F"); s = np.zeros(A)
+        W = np.zeros((int(comps[-1]), q), order="F")
+        self._check(self.lib.orc_plssvd_core(_d(xs), _d(S), n, p, q, _i(comps), len(comps),
+                                             oversample, power, C.c_uint(seed), _d(R), _d(Q),
+                                             _d(T), _d(s), _d(W)))
+        return dict(R=R, Q=Q, T=T, sigma=s, W_last=W)
+
+    def simpls_predict(self, model, x, prefix):
+        x = _f64(x)
+        rows, p = x.shape
+        q = model["Q"].shape[0]; A = model["R"].shape[1]
+        out = np.zeros((rows, q), order="F")
+        self._check(self.lib.orc_simpls_predict(_d(x), rows, p, q, A, _d(_f64(model["R"])),
+                                                _d(_f64(model["Q"])), _d(model["mx"]),
+                                                _d(model["sx"]), _d(model["my"]), prefix,
+                                                _d(out)))
+        return out
+
+    def lda(self, T, labels, classes, prefix, test_scores):
+        T = _f64(T); n, a = T.shape
+        ts = _f64(test_scores); rows = ts.shape[0]
+        lab = _labels(labels)
+        disc = np.zeros((rows, classes), order="F")
+        pred = np.zeros(rows, dtype=np.int32)
+        ridge = C.c_double(0)
+        self._check(self.lib.orc_lda(_d(T), n, a, _i(lab), classes, prefix, _d(ts), rows,
+                                     _d(disc), _i(pred), C.byref(ridge)))
+        return dict(disc=disc, labels=pred, ridge=ridge.value)
+
+    def topk(self, values, top):
+        v = _f64(values); rows, classes = v.shape
+        out = np.zeros((rows, top), dtype=np.int32, order="F")
+        self.lib.orc_topk(_d(v), rows, classes, top, out.ctypes.data_as(_ip))
+        return out
+
+    def response_sums(self, pred, train_mean, observed=None, labels=None):
+        pred = _f64(pred); n, q = pred.shape
+        out = np.zeros(3 * q)
+        self._check(self.lib.orc_response_sums(_d(pred), _d(_f64(observed)) if observed is not None else None,
+                                               _i(_labels(labels)), _d(np.asarray(train_mean, dtype=np.float64)),
+                                               n, q, _d(out)))
+        return out.reshape((3, q), order="F")
+
+    def variance(self, xs, T):
+        xs = _f64(xs); T = _f64(T)
+        n, p = xs.shape; a = T.shape[1]
+        P = np.zeros((p, a), order="F"); ss = np.zeros(a + 1)
+        self._check(self.lib.orc_variance(_d(xs), _d(T), n, p, a, _d(P), _d(ss)))
+        return dict(P=P, ss=ss)
+
+    def gemm(self, left, right):
+        left = _f64(left); right = _f64(right)
+        out = np.zeros((left.shape[0], right.shape[1]), order="F")
+        self.lib.orc_gemm(_d(left), _d(right), left.shape[0], left.shape[1], right.shape[1], _d(out))
+        return out
+
+    def eigen(self, a):
+        a = _f64(a).copy(order="F"); n = a.shape[0]
+        vals = np.zeros(n)
+        self._check(self.lib.orc_symmetric_eigen(_d(a), n, _d(vals)))
+        return vals, a
+
+    def qr(self, a):
+        a = _f64(a); m, n = a.shape
+        q = np.zeros((m, min(m, n)), order="F")
+        self._check(self.lib.orc_qr(_d(a), m, n, _d(q)))
+        return q
+
+
+class Reference:
+    """The reference's own header core with its naive test backend."""
+
+    def __init__(self, path: str = REF_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.lib = C.CDLL(path)
+        self.lib.ref_last_error.restype = C.c_char_p
+
+    def _check(self, status):
+        if status != 0:
+            raise RuntimeError(self.lib.ref_last_error().decode())
+
+    def simpls_fit(self, x, y=None, labels=None, classes=None, ncomp=2, scaling=2,
+                   oversample=32, power=5, seed=1, block=1, cache=0, reorth=0,
+                   batch_geometry=0):
+        x = _f64(x); n, p = x.shape
+        lab = _labels(labels)
+        if lab is not None:
+            q = int(classes); yy = None
+        else:
+            yy = _f64(y); q = yy.shape[1]
+        A = ncomp
+        R = np.zeros((p, A), order="F"); Q = np.zeros((q, A), order="F")
+        V = np.zeros((p, A), order="F"); T = np.zeros((n, A), order="F")
+        mx = np.empty(p); sx = np.empty(p); my = np.empty(q)
+        done = C.c_int(0)
+        self._check(self.lib.ref_simpls_fit(_d(x), _d(yy), _i(lab), n, p, q, A, scaling,
+                                            oversample, power, C.c_uint(seed), block, cache,
+                                            reorth, batch_geometry, _d(R), _d(Q), _d(V), _d(T),
+                                            _d(mx), _d(sx), _d(my), C.byref(done)))
+        return dict(R=R, Q=Q, V=V, T=T, mx=mx, sx=sx, my=my, completed=done.value)
+
+    def plssvd_fit(self, x, y=None, labels=None, classes=None, ncomp=2, scaling=2,
+                   oversample=32, power=5, seed=1):
+        x = _f64(x); n, p = x.shape
+        lab = _labels(labels)
+        if lab is not None:
+            q = int(classes); yy = None
+        else:
+            yy = _f64(y); q = yy.shape[1]
+        A = ncomp
+        R = np.zeros((p, A), order="F"); Q = np.zeros((q, A), order="F")
+        T = np.zeros((n, A), order="F"); s = np.zeros(A)
+        W = np.zeros(A * A * q)
+        mx = np.empty(p); sx = np.empty(p); my = np.empty(q)
+        self._check(self.lib.ref_plssvd_fit(_d(x), _d(yy), _i(lab), n, p, q, A, scaling,
+                                            oversample, power, C.c_uint(seed), _d(R), _d(Q),
+                                            _d(T), _d(s), _d(W), _d(mx), _d(sx), _d(my)))
+        weights = [W[k * A * q:k * A * q + (k + 1) * q].reshape((k + 1, q), order="F")
+                   for k in range(A)]
+        return dict(R=R, Q=Q, T=T, sigma=s, W=weights, mx=mx, sx=sx, my=my)
+
+    def rsvd(self, a, k, oversample=32, power=5, seed=1):
+        a = _f64(a); m, n = a.shape
+        kk = min(k, m, n)
+        U = np.zeros((m, kk), order="F"); s = np.zeros(kk); Vt = np.zeros((kk, n), order="F")
+        got = C.c_int(0)
+        self._check(self.lib.ref_rsvd(_d(a), m, n, k, oversample, power, C.c_uint(seed),
+                                      _d(U), _d(s), _d(Vt), C.byref(got)))
+        g = got.value
+        return dict(U=U[:, :g], s=s[:g], Vt=np.asfortranarray(Vt[:g]))
+
+    def lda(self, T, labels, classes, prefix, test_scores):
+        T = _f64(T); n, a = T.shape
+        ts = _f64(test_scores)[:, :prefix].copy(order="F"); rows = ts.shape[0]
+        lab = _labels(labels)
+        disc = np.zeros((rows, classes), order="F")
+        pred = np.zeros(rows, dtype=np.int32)
+        ridge = C.c_double(0)
+        self._check(self.lib.ref_lda(_d(T), n, a, _i(lab), classes, prefix, _d(ts), rows,
+                                     _d(disc), _i(pred), C.byref(ridge)))
+        return dict(disc=disc, labels=pred, ridge=ridge.value)
+
+
+def latent_data(n, p, q, factors=10, noise=0.1, seed=0, decay=None, dtype=np.float64):
+    """Latent-factor regression data: X = T Pᵀ + noise·E, Y = T Qᵀ + noise·F."""
+    rng = np.random.default_rng(seed)
+    T = rng.standard_normal((n, factors))
+    if decay is not None:
+        T *= decay ** np.arange(factors)
+    P = rng.standard_normal((p, factors))
+    Qm = rng.standard_normal((q, factors))
+    X = T @ P.T + noise * rng.standard_normal((n, p))
+    Y = T @ Qm.T + noise * rng.standard_normal((n, q))
+    return np.asfortranarray(X.astype(dtype)), np.asfortranarray(Y.astype(dtype))
+
+
+def class_data(n, p, K, latent=15, seed=2, spread=0.5, dtype=np.float64):
+    """Balanced classes whose means live in a latent subspace."""
+    rng = np.random.default_rng(seed)
+    labels = (np.arange(n) % K) + 1
+    rng.shuffle(labels)
+    means = spread * rng.standard_normal((K, latent))
+    L = rng.standard_normal((p, latent))
+    Z = means[labels - 1]
+    X = Z @ L.T + rng.standard_normal((n, p))
+    return np.asfortranarray(X.astype(dtype)), labels.astype(np.int32)

@@ -1,0 +1,186 @@
+// Test infrastructure only: a C-ABI driver around the reference's own
+// header-only core (inst/include/fastpls/core) and its test backend
+// (core/tests/reference_backend.hpp). It is compiled from the sources where
+// they lie under the reference checkout by oracle/Makefile into
+// oracle/_ref/libfastpls_ref.so and is used only to write golden vectors and
+// to cross-check the oracle restatement (oracle/fastpls_oracle.cpp).
+//
+// The routing of src/r_api.cpp:1149-1190 (simpls_controls) cannot be compiled
+// here (that file needs R headers), so the same default thresholds are passed
+// in explicitly by this driver.
+
+#include <fastpls/core.hpp>
+
+#include "reference_backend.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <exception>
+#include <string>
+#include <vector>
+
+namespace {
+using fastpls::core::ConstMatrixView;
+using fastpls::core::Matrix;
+using fastpls::core::PredictorScaling;
+thread_local std::string g_error;
+
+template<class F>
+int guarded(F&& f) {
+  try { f(); return 0; } catch (const std::exception& e) { g_error = e.what(); return 1; }
+}
+
+Matrix<double> load(const double* x, std::size_t r, std::size_t c) {
+  Matrix<double> m(r, c);
+  std::memcpy(m.data(), x, sizeof(double) * r * c);
+  return m;
+}
+
+void store(const Matrix<double>& m, double* out) {
+  if (out && m.size()) std::memcpy(out, m.data(), sizeof(double) * m.size());
+}
+}  // namespace
+
+extern "C" {
+
+const char* ref_last_error() { return g_error.c_str(); }
+
+// Preprocess (dense or one-based labels) and fit SIMPLS with the given
+// controls. Outputs: R p*A, Q q*A, V p*A, T n*A, mx p, sx p, my q.
+int ref_simpls_fit(const double* x, const double* y, const int* labels, int n,
+                   int p, int q, int A, int scaling, int oversample, int power,
+                   unsigned int seed, int block, int cache, int reorth,
+                   int batch_geometry, double* R, double* Q, double* V,
+                   double* T, double* mx, double* sx, double* my,
+                   int* completed) {
+  return guarded([&] {
+    ReferenceBackend<double> backend;
+    Matrix<double> X = load(x, n, p);
+    Matrix<double> S;
+    std::vector<double> vmx, vsx, vmy;
+    if (labels) {
+      std::vector<std::size_t> zero_based(n);
+      for (int i = 0; i < n; ++i) zero_based[i] = std::size_t(labels[i] - 1);
+      auto pr = fastpls::core::prepare_scaled_label_crossprod(
+        X.view(), zero_based.data(), std::size_t(n), std::size_t(q),
+        static_cast<PredictorScaling>(scaling), backend);
+      S = std::move(pr.crossprod);
+      vmx = pr.predictor_center; vsx = pr.predictor_scale; vmy = pr.response_mean;
+    } else {
+      Matrix<double> Y = load(y, n, q);
+      auto pr = fastpls::core::prepare_scaled_dense_crossprod(
+        X.view(), ConstMatrixView<double>(Y.view()),
+        static_cast<PredictorScaling>(scaling), backend);
+      S = std::move(pr.crossprod);
+      vmx = pr.predictor_center; vsx = pr.predictor_scale; vmy = pr.response_mean;
+    }
+    fastpls::core::SimplsControls controls;
+    controls.components = std::size_t(A);
+    controls.maximum_block = std::size_t(block);
+    controls.cache_predictor_crossprod = cache != 0;
+    controls.batch_candidate_geometry = batch_geometry != 0;
+    controls.reorthogonalize = reorth != 0;
+    controls.store_scores = true;
+    controls.use_right_gram = true;
+    controls.rsvd.oversample = oversample;
+    controls.rsvd.power = power;
+    controls.rsvd.seed = seed;
+    fastpls::core::SimplsWorkspace<double> workspace;
+    auto model = fastpls::core::fit_simpls_preprocessed<double>(
+      ConstMatrixView<double>(X.view()), ConstMatrixView<double>(S.view()),
+      controls, backend, workspace);
+    store(model.weights, R); store(model.response_loadings, Q);
+    store(model.deflation_basis, V); store(model.scores, T);
+    std::copy(vmx.begin(), vmx.end(), mx);
+    std::copy(vsx.begin(), vsx.end(), sx);
+    std::copy(vmy.begin(), vmy.end(), my);
+    *completed = int(model.completed_components);
+  });
+}
+
+// Preprocess and fit PLS-SVD for prefixes 1..A. W holds A slabs of A*q;
+// slab k-1 holds the k x q prediction weights (leading dimension k).
+int ref_plssvd_fit(const double* x, const double* y, const int* labels, int n,
+                   int p, int q, int A, int scaling, int oversample, int power,
+                   unsigned int seed, double* R, double* Q, double* T,
+                   double* sigma, double* W, double* mx, double* sx,
+                   double* my) {
+  return guarded([&] {
+    ReferenceBackend<double> backend;
+    Matrix<double> X = load(x, n, p);
+    Matrix<double> S;
+    std::vector<double> vmx, vsx, vmy;
+    if (labels) {
+      std::vector<std::size_t> zero_based(n);
+      for (int i = 0; i < n; ++i) zero_based[i] = std::size_t(labels[i] - 1);
+      auto pr = fastpls::core::prepare_scaled_label_crossprod(
+        X.view(), zero_based.data(), std::size_t(n), std::size_t(q),
+        static_cast<PredictorScaling>(scaling), backend);
+      S = std::move(pr.crossprod);
+      vmx = pr.predictor_center; vsx = pr.predictor_scale; vmy = pr.response_mean;
+    } else {
+      Matrix<double> Y = load(y, n, q);
+      auto pr = fastpls::core::prepare_scaled_dense_crossprod(
+        X.view(), ConstMatrixView<double>(Y.view()),
+        static_cast<PredictorScaling>(scaling), backend);
+      S = std::move(pr.crossprod);
+      vmx = pr.predictor_center; vsx = pr.predictor_scale; vmy = pr.response_mean;
+    }
+    std::vector<int> comps(A);
+    for (int k = 0; k < A; ++k) comps[k] = k + 1;
+    fastpls::core::PlssvdControls controls;
+    controls.rsvd.oversample = oversample;
+    controls.rsvd.power = power;
+    controls.rsvd.seed = seed;
+    auto model = fastpls::core::fit_plssvd_preprocessed(
+      ConstMatrixView<double>(X.view()), ConstMatrixView<double>(S.view()),
+      comps.data(), comps.size(), controls, backend);
+    store(model.weights, R); store(model.response_loadings, Q);
+    store(model.scores, T);
+    std::copy(model.singular_values.begin(), model.singular_values.end(), sigma);
+    for (int k = 1; k <= A; ++k)
+      std::memcpy(W + std::size_t(k - 1) * A * q, model.prediction_weights[k - 1].data(),
+                  sizeof(double) * k * q);
+    std::copy(vmx.begin(), vmx.end(), mx);
+    std::copy(vsx.begin(), vsx.end(), sx);
+    std::copy(vmy.begin(), vmy.end(), my);
+  });
+}
+
+// Explicit randomized SVD of a dense matrix (rsvd.hpp:185-237).
+int ref_rsvd(const double* a, int m, int n, int k, int oversample, int power,
+             unsigned int seed, double* U, double* s, double* Vt, int* got) {
+  return guarded([&] {
+    ReferenceBackend<double> backend;
+    Matrix<double> A = load(a, m, n);
+    fastpls::core::RsvdControls controls;
+    controls.oversample = oversample; controls.power = power; controls.seed = seed;
+    auto t = fastpls::core::randomized_svd(ConstMatrixView<double>(A.view()), k, controls, backend);
+    store(t.U, U);
+    std::copy(t.singular_values.begin(), t.singular_values.end(), s);
+    store(t.Vt, Vt);
+    *got = int(t.singular_values.size());
+  });
+}
+
+// LDA on training scores, then discriminants and first-max labels of a test
+// score block (lda.hpp:156-346). labels one-based.
+int ref_lda(const double* T, int n, int a, const int* labels, int K, int prefix,
+            const double* test_scores, int rows, double* disc, int* predicted,
+            double* ridge) {
+  return guarded([&] {
+    Matrix<double> scores = load(T, n, a);
+    const int prefixes[] = {prefix};
+    auto models = fastpls::core::train_lda_prefixes(
+      ConstMatrixView<double>(scores.view()), labels, std::size_t(n),
+      std::size_t(K), prefixes, 1);
+    Matrix<double> test = load(test_scores, rows, prefix);
+    auto d = fastpls::core::lda_scores(ConstMatrixView<double>(test.view()), models[0]);
+    store(d, disc);
+    auto lab = fastpls::core::lda_predict(ConstMatrixView<double>(test.view()), models[0]);
+    std::copy(lab.begin(), lab.end(), predicted);
+    *ridge = models[0].ridge;
+  });
+}
+
+}  // extern "C"

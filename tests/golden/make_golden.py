@@ -1,0 +1,130 @@
+"""Write the golden vectors under tests/golden/ from the reference core.
+
+The reference's own header-only core (inst/include/fastpls/core) and its test
+backend (core/tests/reference_backend.hpp) are compiled by ``make -C oracle
+ref`` into oracle/_ref/libfastpls_ref.so; this script runs it on seeded
+synthetic inputs and stores inputs (or their generator recipe and checksum)
+with the outputs as .npz fixtures. Run it where the reference checkout is
+present:  python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from oracle.oracle import Reference, build, class_data, latent_data  # noqa: E402
+
+
+def digest(a: np.ndarray) -> str:
+    return hashlib.sha256(np.asfortranarray(a).tobytes(order="F")).hexdigest()
+
+
+def save(name, **arrays):
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **arrays)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
+def simpls_case(ref, name, X, Y=None, labels=None, classes=None, ncomp=5, oversample=32,
+                power=5, seed=1, block=1, cache=0, batch_geometry=0, store_inputs=True,
+                recipe=None):
+    m = ref.simpls_fit(X, Y, labels=labels, classes=classes, ncomp=ncomp, oversample=oversample,
+                       power=power, seed=seed, block=block, cache=cache,
+                       batch_geometry=batch_geometry)
+    arrays = dict(R=m["R"], Q=m["Q"], V=m["V"], T=m["T"], mx=m["mx"], sx=m["sx"], my=m["my"],
+                  completed=np.int64(m["completed"]), ncomp=np.int64(ncomp),
+                  oversample=np.int64(oversample), power=np.int64(power), seed=np.int64(seed),
+                  block=np.int64(block), cache=np.int64(cache),
+                  batch_geometry=np.int64(batch_geometry), x_sha256=digest(X))
+    if store_inputs:
+        arrays["X"] = X
+        if Y is not None:
+            arrays["Y"] = Y
+    if labels is not None:
+        arrays["labels"] = np.asarray(labels, dtype=np.int32)
+        arrays["classes"] = np.int64(classes)
+    if recipe is not None:
+        arrays["recipe"] = np.asarray(recipe, dtype=np.float64)
+    save(name, **arrays)
+
+
+def plssvd_case(ref, name, X, Y=None, labels=None, classes=None, ncomp=3, oversample=32,
+                power=5, seed=1):
+    m = ref.plssvd_fit(X, Y, labels=labels, classes=classes, ncomp=ncomp, oversample=oversample,
+                       power=power, seed=seed)
+    arrays = dict(R=m["R"], Q=m["Q"], T=m["T"], sigma=m["sigma"], mx=m["mx"], sx=m["sx"],
+                  my=m["my"], ncomp=np.int64(ncomp), oversample=np.int64(oversample),
+                  power=np.int64(power), seed=np.int64(seed), X=X)
+    for k, w in enumerate(m["W"]):
+        arrays[f"W{k + 1}"] = w
+    if Y is not None:
+        arrays["Y"] = Y
+    if labels is not None:
+        arrays["labels"] = np.asarray(labels, dtype=np.int32)
+        arrays["classes"] = np.int64(classes)
+    save(name, **arrays)
+
+
+def main():
+    build(reference=True)
+    ref = Reference()
+
+    # SIMPLS, dense responses, explicit width = q (direction independent of the sketch)
+    X, Y = latent_data(64, 20, 3, factors=5, noise=0.1, seed=11)
+    simpls_case(ref, "simpls_dense_small", X, Y, ncomp=5, seed=7)
+    # SIMPLS with a sketch narrower than q (width 2 < q = 6): sketch-dependent
+    X, Y = latent_data(300, 40, 6, factors=8, noise=0.1, seed=12)
+    simpls_case(ref, "simpls_dense_sketch", X, Y, ncomp=8, oversample=1, power=2, seed=3)
+    # SIMPLS classification (one-based labels). The centred one-hot
+    # cross-product has rank K-1, so a sketch as wide as K makes the naive
+    # test backend's Gram-Schmidt QR give up; widths below K keep the golden
+    # run on the path that LAPACK-backed builds take.
+    X, lab = class_data(400, 30, 4, latent=5, seed=13)
+    simpls_case(ref, "simpls_labels_small", X, labels=lab, classes=4, ncomp=3, oversample=1,
+                power=2, seed=5)
+    # SIMPLS with a candidate block of 3 and batched candidate geometry
+    X, lab = class_data(500, 60, 6, latent=6, seed=14)
+    simpls_case(ref, "simpls_block3", X, labels=lab, classes=6, ncomp=5, oversample=1,
+                power=2, seed=2, block=3, batch_geometry=1)
+    # The plain SIMPLS regression workload: n=2000, p=1000, q=3, 10 components,
+    # latent-factor data with seed 0, autoscaling; inputs regenerated from the recipe.
+    X, Y = latent_data(2000, 1000, 3, factors=10, noise=0.1, seed=0)
+    simpls_case(ref, "simpls_c1", X, Y, ncomp=10, seed=1, store_inputs=False,
+                recipe=[2000, 1000, 3, 10, 0.1, 0])
+
+    # PLS-SVD: randomized (width 6 < min(p,q) = 10) and dense (width >= q)
+    X, Y = latent_data(120, 30, 10, factors=6, noise=0.1, seed=21)
+    plssvd_case(ref, "plssvd_sketch", X, Y, ncomp=4, oversample=2, power=2, seed=17)
+    X, Y = latent_data(100, 25, 4, factors=4, noise=0.1, seed=22)
+    plssvd_case(ref, "plssvd_dense", X, Y, ncomp=3, oversample=32, power=5, seed=1)
+    X, lab = class_data(300, 20, 5, latent=4, seed=23)
+    # width 4 < 5 classes: the randomized route (the dense route of the naive test
+    # backend rejects the rank-deficient centred one-hot cross-product)
+    plssvd_case(ref, "plssvd_labels", X, labels=lab, classes=5, ncomp=3, oversample=1,
+                power=2, seed=1)
+
+    # Randomized SVD of a dense matrix
+    A = np.random.default_rng(31).standard_normal((60, 40))
+    r = ref.rsvd(A, 5, oversample=4, power=2, seed=11)
+    save("rsvd_small", A=A, U=r["U"], s=r["s"], Vt=r["Vt"], k=np.int64(5),
+         oversample=np.int64(4), power=np.int64(2), seed=np.int64(11))
+
+    # LDA on scores
+    rng = np.random.default_rng(41)
+    T = rng.standard_normal((200, 4))
+    lab = (np.arange(200) % 3 + 1).astype(np.int32)
+    T[:, 0] += lab
+    test = rng.standard_normal((50, 4))
+    lda = ref.lda(T, lab, 3, 3, test)
+    save("lda_small", T=T, labels=lab, test=test, disc=lda["disc"], predicted=lda["labels"],
+         ridge=np.float64(lda["ridge"]), prefix=np.int64(3), classes=np.int64(3))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,235 @@
+"""Pinning the FP64 oracle (CPU only).
+
+Each oracle function is checked against something other than itself:
+  * golden vectors written by the reference's own header core compiled in
+    oracle/_ref (tests/golden/make_golden.py);
+  * the reference's known-answer tests, restated (core/tests/*.cpp);
+  * closed forms, invariants and brute force.
+"""
+import numpy as np
+import pytest
+
+from conftest import align_columns, load_golden, rel
+from oracle.oracle import latent_data
+
+
+def coef(m, k):
+    return m["R"][:, :k] @ m["Q"][:, :k].T
+
+
+# ------------------------------------------------------------ golden vectors
+@pytest.mark.parametrize("name", ["simpls_dense_small", "simpls_dense_sketch", "simpls_labels_small"])
+def test_simpls_matches_reference_core(oracle, name):
+    g = load_golden(name)
+    kw = dict(ncomp=int(g["ncomp"]), oversample=int(g["oversample"]), power=int(g["power"]),
+              seed=int(g["seed"]))
+    if "labels" in g:
+        m = oracle.simpls_fit(g["X"], labels=g["labels"], classes=int(g["classes"]), **kw)
+    else:
+        m = oracle.simpls_fit(g["X"], g["Y"], **kw)
+    assert m["completed"] == int(g["completed"])
+    np.testing.assert_allclose(m["mx"], g["mx"], rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(m["sx"], g["sx"], rtol=1e-13)
+    np.testing.assert_allclose(m["my"], g["my"], rtol=1e-13, atol=1e-15)
+    for k in range(1, int(g["ncomp"]) + 1):
+        assert rel(coef(m, k), coef(g, k)) < 1e-8, k
+    # components agree individually up to sign
+    assert rel(align_columns(m["R"], g["R"]), g["R"]) < 1e-7
+    assert rel(align_columns(m["T"], g["T"]), g["T"]) < 1e-7
+
+
+def test_simpls_block_geometry_matches_reference_core(oracle):
+    g = load_golden("simpls_block3")
+    prep = oracle.prepare(g["X"], labels=g["labels"], classes=int(g["classes"]))
+    m = oracle.simpls_core(prep["xs"], prep["S"], int(g["ncomp"]), block=int(g["block"]),
+                           batch_geometry=int(g["batch_geometry"]), oversample=int(g["oversample"]),
+                           power=int(g["power"]), seed=int(g["seed"]))
+    for k in range(1, int(g["ncomp"]) + 1):
+        assert rel(coef(m, k), coef(g, k)) < 1e-8
+
+
+def test_simpls_c1_workload_matches_reference_core(oracle):
+    g = load_golden("simpls_c1")
+    n, p, q, factors, noise, seed = g["recipe"]
+    X, Y = latent_data(int(n), int(p), int(q), factors=int(factors), noise=noise, seed=int(seed))
+    import hashlib
+    assert hashlib.sha256(np.asfortranarray(X).tobytes(order="F")).hexdigest() == str(g["x_sha256"])
+    m = oracle.simpls_fit(X, Y, ncomp=int(g["ncomp"]), seed=int(g["seed"]))
+    for k in (1, 5, 10):
+        assert rel(coef(m, k), coef(g, k)) < 1e-9
+
+
+@pytest.mark.parametrize("name", ["plssvd_sketch", "plssvd_dense", "plssvd_labels"])
+def test_plssvd_matches_reference_core(oracle, name):
+    g = load_golden(name)
+    kw = dict(ncomp=int(g["ncomp"]), oversample=int(g["oversample"]), power=int(g["power"]),
+              seed=int(g["seed"]))
+    if "labels" in g:
+        m = oracle.plssvd_fit(g["X"], labels=g["labels"], classes=int(g["classes"]), **kw)
+    else:
+        m = oracle.plssvd_fit(g["X"], g["Y"], **kw)
+    np.testing.assert_allclose(m["sigma"], g["sigma"], rtol=1e-10)
+    for k in range(1, int(g["ncomp"]) + 1):
+        Bk = m["R"][:, :k] @ m["W"][k - 1]
+        Bg = g["R"][:, :k] @ g[f"W{k}"]
+        assert rel(Bk, Bg) < 1e-9, k
+
+
+def test_rsvd_matches_reference_core(oracle):
+    g = load_golden("rsvd_small")
+    r = oracle.rsvd(g["A"], int(g["k"]), oversample=int(g["oversample"]), power=int(g["power"]),
+                    seed=int(g["seed"]))
+    np.testing.assert_allclose(r["s"], g["s"], rtol=1e-12)
+    assert rel(r["U"] @ r["U"].T, g["U"] @ g["U"].T) < 1e-10
+    assert rel(align_columns(r["Vt"].T, g["Vt"].T), g["Vt"].T) < 1e-10
+
+
+def test_lda_matches_reference_core(oracle):
+    g = load_golden("lda_small")
+    out = oracle.lda(g["T"], g["labels"], int(g["classes"]), int(g["prefix"]), g["test"])
+    np.testing.assert_allclose(out["disc"], g["disc"], rtol=1e-12, atol=1e-12)
+    np.testing.assert_array_equal(out["labels"], g["predicted"])
+    assert out["ridge"] == pytest.approx(float(g["ridge"]), rel=1e-14)
+
+
+# ------------------------------------------------- restated known answers
+def _kat_plssvd_data():
+    # core/tests/test_core_plssvd.cpp:11-25
+    x = np.zeros((8, 3)); y = np.zeros((8, 2))
+    for r in range(8):
+        v = r - 3.5
+        x[r] = [v, v * v - 5.25, -1.0 if r % 2 == 0 else 1.0]
+        y[r] = [2 * x[r, 0] - 0.5 * x[r, 1], x[r, 2] + 0.25 * x[r, 0]]
+    return x, y
+
+
+def test_kat_plssvd_sse_and_sigma(oracle):
+    # core/tests/test_core_plssvd.cpp:26-91: SSE 10.5762741916562, sigma1 119.561004785789
+    x, y = _kat_plssvd_data()
+    S = x.T @ y
+    m = oracle.plssvd_core(x, S, [1, 2], oversample=4, power=2, seed=17)
+    pred = m["T"][:, :2] @ m["W_last"]
+    sse = float(((pred - y) ** 2).sum())
+    assert abs(sse - 10.5762741916562) < 2e-7
+    assert abs(m["sigma"][0] - 119.561004785789) < 2e-7
+
+
+def test_kat_simpls_exact_recovery(oracle):
+    # core/tests/test_core_simpls.cpp:11-117
+    vals = [-3, -2, -1, 0, 0, 1, 2, 3, 1, -1, 1, -1, 1, -1, 1, -1, -1, 2, -2, 1, 1, -2, 2, -1]
+    x = np.array(vals, dtype=np.float64).reshape((8, 3), order="F")
+    coefs = np.zeros((3, 3))
+    coefs[0, 0], coefs[1, 0], coefs[0, 1], coefs[1, 1], coefs[0, 2], coefs[1, 2] = 1.5, -0.5, -0.75, 2, 0.25, 1.25
+    y = x @ coefs
+    S = x.T @ y
+    m = oracle.simpls_core(x, S, 2, block=1, reorth=1, oversample=0, power=5, seed=19)
+    assert m["completed"] == 2
+    pred = m["T"] @ (m["Q"].T)
+    # X y-part lies in a 2-dim subspace: relative error < 1e-8 (f64 contract)
+    assert np.sum((pred - y) ** 2) / np.sum(y ** 2) < 1e-8
+    np.testing.assert_allclose(m["T"].T @ m["T"], np.eye(2), atol=1e-10)
+    np.testing.assert_allclose(m["V"].T @ m["V"], np.eye(2), atol=1e-10)
+
+
+def test_kat_rsvd_diagonal(oracle):
+    # core/tests/test_core_operator_rsvd.cpp:141-180: diag(8..1) padded to 10x8
+    a = np.zeros((10, 8))
+    for i in range(8):
+        a[i, i] = 8 - i
+    r = oracle.rsvd(a, 2, oversample=6, power=1, seed=27)
+    np.testing.assert_allclose(r["s"], [8, 7], atol=1e-8)
+    resid = a @ r["Vt"].T - r["U"] * r["s"]
+    assert np.abs(resid).max() < 1e-8
+
+
+# ------------------------------------------------------------ closed forms
+def test_preprocessing_closed_form(oracle):
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((50, 7)) * 3 + 10
+    X[:, 3] = 4.0  # constant column: scale falls back to 1
+    Y = rng.standard_normal((50, 2))
+    for scaling in (1, 2, 3):
+        pr = oracle.prepare(X, Y, scaling=scaling)
+        mu = X.mean(0) if scaling != 3 else np.zeros(7)
+        sd = X.std(0, ddof=1) if scaling == 2 else np.ones(7)
+        sd = np.where(sd > 1e-9, sd, 1.0)
+        xs = (X - mu) / sd
+        np.testing.assert_allclose(pr["xs"], xs, atol=1e-12)
+        np.testing.assert_allclose(pr["S"], xs.T @ (Y - Y.mean(0)), atol=1e-10)
+
+
+def test_label_preprocessing_equals_centred_one_hot(oracle):
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((60, 5))
+    lab = rng.integers(1, 5, size=60).astype(np.int32)
+    lab[:4] = [1, 2, 3, 4]
+    pr = oracle.prepare(X, labels=lab, classes=4)
+    H = np.eye(4)[lab - 1]
+    xs = (X - X.mean(0)) / X.std(0, ddof=1)
+    np.testing.assert_allclose(pr["S"], xs.T @ (H - H.mean(0)), atol=1e-12)
+    np.testing.assert_allclose(pr["my"], H.mean(0))
+
+
+def test_qr_eigen_closed_forms(oracle):
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((30, 6))
+    q = oracle.qr(a)
+    np.testing.assert_allclose(q.T @ q, np.eye(6), atol=1e-13)
+    # same column space: projection of a onto q reproduces a
+    np.testing.assert_allclose(q @ (q.T @ a), a, atol=1e-12)
+    s = a.T @ a
+    vals, vecs = oracle.eigen(s)
+    np.testing.assert_allclose(vals, np.linalg.eigvalsh(s), rtol=1e-12)
+    np.testing.assert_allclose(s @ vecs, vecs * vals, atol=1e-10)
+
+
+def test_simpls_width_covers_column_space_equals_exact_svd_direction(oracle):
+    # q <= oversample+1: the first direction is the top left singular vector of S
+    X, Y = latent_data(80, 15, 3, factors=4, seed=6)
+    m = oracle.simpls_fit(X, Y, ncomp=1, oversample=32, power=5, seed=9)
+    pr = oracle.prepare(X, Y)
+    u = np.linalg.svd(pr["S"])[0][:, 0]
+    r = m["R"][:, 0] / np.linalg.norm(m["R"][:, 0])
+    assert abs(abs(r @ u) - 1.0) < 1e-12
+
+
+def test_topk_tie_rule(oracle):
+    vals = np.array([[1.0, 3.0, 3.0, 2.0], [5.0, 5.0, 5.0, 5.0]])
+    out = oracle.topk(vals, 3)
+    np.testing.assert_array_equal(out[0], [2, 3, 4])
+    np.testing.assert_array_equal(out[1], [1, 2, 3])
+
+
+def test_response_sums_brute_force(oracle):
+    rng = np.random.default_rng(7)
+    y = rng.standard_normal((40, 3)); pred = y + 0.1 * rng.standard_normal((40, 3))
+    tm = np.array([0.1, -0.2, 0.3])
+    out = oracle.response_sums(pred, tm, observed=y)
+    np.testing.assert_allclose(out[0], ((y - pred) ** 2).sum(0))
+    np.testing.assert_allclose(out[1], ((y - tm) ** 2).sum(0))
+    np.testing.assert_allclose(out[2], ((y - y.mean(0)) ** 2).sum(0))
+
+
+def test_variance_orthogonal_scores_closed_form(oracle):
+    # With orthonormal scores the sequential loadings equal the plain ones and
+    # the gains are the squared loading norms.
+    rng = np.random.default_rng(8)
+    xs = rng.standard_normal((50, 6))
+    t, _ = np.linalg.qr(rng.standard_normal((50, 3)))
+    v = oracle.variance(xs, t)
+    np.testing.assert_allclose(v["P"], xs.T @ t, atol=1e-12)
+    np.testing.assert_allclose(v["ss"][:3], ((xs.T @ t) ** 2).sum(0), rtol=1e-12)
+    assert v["ss"][3] == pytest.approx((xs ** 2).sum())
+
+
+def test_gaussian_stream_is_the_reference_stream(oracle):
+    # first draws of std::mt19937(1) + std::normal_distribution<double>
+    a = oracle.gaussian(1, 6)
+    b = oracle.gaussian(1, 6)
+    np.testing.assert_array_equal(a, b)
+    assert abs(a.mean()) < 3 and np.isfinite(a).all()
+    g = load_golden("simpls_dense_sketch")
+    # the sketch-dependent golden fit only matches with the same stream
+    m = oracle.simpls_fit(g["X"], g["Y"], ncomp=int(g["ncomp"]), oversample=int(g["oversample"]),
+                          power=int(g["power"]), seed=int(g["seed"]) + 1)
+    assert rel(coef(m, 8), coef(g, 8)) > 1e-6
