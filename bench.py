@@ -155,9 +155,9 @@ def main():
     achieved = xbytes / t_cp
     roofline = {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
-                "kernel": "k_xtw<float,64> (column stats + X^T Y, one pass over X)",
+                "kernel": "k_xtw_mfma<float,64> (column stats + X^T Y, one pass over X)",
                 "kernel_ms": round(1e3 * t_cp, 4),
-                "score_pass_kernel": "k_xw<float,32> (T = X_s R)",
+                "score_pass_kernel": "k_xw_mfma<float,32> (T = X_s R)",
                 "score_pass_ms": round(1e3 * float(np.mean(sc)), 4),
                 "score_pass_frac": round(xbytes / float(np.mean(sc)) / HBM_PEAK, 4)}
 

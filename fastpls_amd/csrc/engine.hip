@@ -569,6 +569,41 @@ class Model final : public Handle {
     launch_plssvd_weights(gram_.d(), sigma_.d(), Q_.d(), k, q_, W_.d(), flags, stream_);
   }
 
+  // Tall-skinny QR: CholQR2 in FP64 (two Gram / Cholesky / triangular-apply
+  // rounds through the multi-work-group products), with a device-side
+  // Householder fallback when a Cholesky factor is unusable. A is destroyed
+  // only by the fallback; Q receives the orthonormal basis.
+  void qr_tall(double* A, int m, int w, double* Q, int* flags) {
+    if (w > 64 || m < 4 * w) {
+      launch_qr(A, m, w, Q, flags, stream_);
+      return;
+    }
+    qr_gram_.reserve(sizeof(double) * (size_t)w * w);
+    qr_rinv_.reserve(sizeof(double) * (size_t)w * w);
+    qr_tmp_.reserve(sizeof(double) * (size_t)m * w);
+    qr_fail_.reserve(sizeof(int));
+    fill_zero_int_async(qr_fail_.as<int>(), 1);
+    tproduct<double>(A, m, w, nullptr, A, m, w, qr_gram_.d());
+    launch_chol_inv(qr_gram_.d(), w, qr_rinv_.d(), qr_fail_.as<int>(), stream_);
+    product<double>(A, m, w, nullptr, qr_rinv_.d(), w, w, qr_tmp_.d());
+    tproduct<double>(qr_tmp_.d(), m, w, nullptr, qr_tmp_.d(), m, w, qr_gram_.d());
+    launch_chol_inv(qr_gram_.d(), w, qr_rinv_.d(), qr_fail_.as<int>(), stream_);
+    product<double>(qr_tmp_.d(), m, w, nullptr, qr_rinv_.d(), w, w, Q);
+    launch_qr_fallback(A, m, w, Q, qr_fail_.as<int>(), flags, stream_);
+  }
+
+  void fill_zero_int_async(int* ptr, int count) {
+    zero_ints_.reserve(sizeof(int) * 64);
+    if (count > 64) throw std::invalid_argument("zero fill too large");
+    static const int zeros[64] = {0};
+    if (!zero_ready_) {
+      FPLS_HIP(hipMemcpy(zero_ints_.as<int>(), zeros, sizeof(zeros), hipMemcpyHostToDevice));
+      zero_ready_ = true;
+    }
+    FPLS_HIP(hipMemcpyAsync(ptr, zero_ints_.as<int>(), sizeof(int) * count,
+                            hipMemcpyDeviceToDevice, stream_));
+  }
+
   void randomized_svd(int k, int w, int* usable, int* flags) {
     std::vector<double> om((size_t)q_ * w);
     gaussian(unsigned(seed_), om.size(), om.data());
@@ -581,12 +616,12 @@ class Model final : public Handle {
     rbasis.alloc(sizeof(double) * (size_t)q_ * w);
     product<double>(S_.d(), p_, q_, nullptr, tmp1_.d(), q_, w, sample.d());
     for (int it = 0; it < power_; ++it) {
-      launch_qr(sample.d(), p_, w, basis.d(), flags, stream_);
+      qr_tall(sample.d(), p_, w, basis.d(), flags);
       tproduct<double>(S_.d(), p_, q_, nullptr, basis.d(), p_, w, rev.d());
-      launch_qr(rev.d(), q_, w, rbasis.d(), flags, stream_);
+      qr_tall(rev.d(), q_, w, rbasis.d(), flags);
       product<double>(S_.d(), p_, q_, nullptr, rbasis.d(), q_, w, sample.d());
     }
-    launch_qr(sample.d(), p_, w, basis.d(), flags, stream_);
+    qr_tall(sample.d(), p_, w, basis.d(), flags);
     P.alloc(sizeof(double) * (size_t)w * q_);
     tproduct<double>(basis.d(), p_, w, nullptr, S_.d(), p_, q_, P.d());
     Up.alloc(sizeof(double) * (size_t)w * k);
@@ -607,7 +642,7 @@ class Model final : public Handle {
       Sc.alloc(sizeof(double) * (size_t)p_ * q_);
       FPLS_HIP(hipMemcpyAsync(Sc.d(), S_.d(), sizeof(double) * (size_t)p_ * q_, hipMemcpyDeviceToDevice, stream_));
       Qs.alloc(sizeof(double) * (size_t)p_ * q_);
-      launch_qr(Sc.d(), p_, q_, Qs.d(), flags, stream_);
+      qr_tall(Sc.d(), p_, q_, Qs.d(), flags);
       Rm.alloc(sizeof(double) * (size_t)q_ * q_);
       tproduct<double>(Qs.d(), p_, q_, nullptr, S_.d(), p_, q_, Rm.d());
       Ur.alloc(sizeof(double) * (size_t)q_ * k);
@@ -770,7 +805,8 @@ class Model final : public Handle {
   Buf xown_, yown_, labels_dev_, counts_, Yc_, H_, ysum_, my_, mx_, sx_, colss_, S_, S0_, G_;
   Buf R_, Q_, V_, Tsc_, Rs_, sigma_, gram_, W_, state_;
   Buf omega_, steps_, fitdev_, simpls_scratch_, cand_, cand_scaled_, t_raw_, ts_part_, ls_part_;
-  Buf part_, tmp1_, tmp2_, tmp3_;
+  Buf part_, tmp1_, tmp2_, tmp3_, qr_gram_, qr_rinv_, qr_tmp_, qr_fail_, zero_ints_;
+  bool zero_ready_ = false;
   Buf rows_, scores_, pred64_, outT_, top_, metric_, flags_, obs_, obs_labels_;
   Buf lda_gram_, lda_sums_, lda_linear_, lda_constants_, lda_ridge_, lda_scratch_;
   Buf var_loadings_, var_ss_;

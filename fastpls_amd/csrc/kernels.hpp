@@ -58,6 +58,12 @@ void launch_simpls_update(const SimplsFitDev* fits, int count, int step, hipStre
 // ---- single-work-group helpers for the PLS-SVD randomized SVD
 // Householder QR: A (m x n) destroyed, Q (m x n) written. flags |= decomposition.
 void launch_qr(double* A, int m, int n, double* Q, int* flags, hipStream_t s);
+// CholQR2 pieces: Rinv = (chol(G))^{-T} for a w x w Gram (w <= 64), *fail set
+// when the factorisation is unusable; the fallback recomputes Q by
+// Householder from A only when *fail is set.
+void launch_chol_inv(const double* G, int w, double* Rinv, int* fail, hipStream_t s);
+void launch_qr_fallback(double* A, int m, int n, double* Q, const int* fail, int* flags,
+                        hipStream_t s);
 // Finalize a sketch: P (w x q) = Qbᵀ S is given; writes Up (w x k), sigma (k),
 // Vt (k x q) and *usable. Uses the eigen route when w <= q, the dense SVD
 // route otherwise (rsvd.hpp:85-165). scratch: 4*w*w + 2*w*q + w doubles.

@@ -212,6 +212,63 @@ __global__ __launch_bounds__(kWG) void k_qr(double* A, int m, int n, double* Q, 
   if (!ok && threadIdx.x == 0) atomicOr(flags, kFlagDecomposition);
 }
 
+// ---------------------------------------------------------------- CholQR2
+// Cholesky of a w x w Gram (w <= 64) and the inverse of its transposed
+// factor, so that Q = A Rinv with R = Lᵀ. A pivot that is non-finite or
+// below 1e-13 of the largest diagonal entry marks the factorisation as
+// failed (*fail = 1) and leaves Rinv as the identity; the Householder
+// fallback then recomputes Q.
+__global__ __launch_bounds__(kWG) void k_chol_inv(const double* G, int w, double* Rinv,
+                                                  int* fail) {
+  __shared__ double Lm[64 * 64];
+  __shared__ int bad;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    bad = (*fail != 0) ? 1 : 0;
+    double dmax = 0.0;
+    for (int i = 0; i < w; ++i) dmax = fmax(dmax, G[i + i * w]);
+    for (int i = 0; i < w && !bad; ++i) {
+      for (int j = 0; j <= i; ++j) {
+        double v = G[i + j * w];
+        for (int l = 0; l < j; ++l) v -= Lm[i + l * 64] * Lm[j + l * 64];
+        if (i == j) {
+          if (!isfinite(v) || !(v > 1e-13 * dmax)) { bad = 1; break; }
+          Lm[i + i * 64] = sqrt(v);
+        } else {
+          Lm[i + j * 64] = v / Lm[j + j * 64];
+        }
+      }
+    }
+    if (bad) *fail = 1;
+  }
+  __syncthreads();
+  // Rinv = L^{-T}: column c solves Lᵀ x = e_c (upper triangular, x_i = 0 for i > c)
+  for (int c = tid; c < w; c += kWG) {
+    for (int i = w - 1; i >= 0; --i) {
+      double v;
+      if (bad) {
+        v = i == c ? 1.0 : 0.0;
+      } else if (i > c) {
+        v = 0.0;
+      } else {
+        v = i == c ? 1.0 : 0.0;
+        for (int l = i + 1; l <= c; ++l) v -= Lm[l + i * 64] * Rinv[l + c * w];
+        v /= Lm[i + i * 64];
+      }
+      Rinv[i + c * w] = v;
+    }
+  }
+}
+
+// Householder recomputation of Q when CholQR2 failed; a no-op otherwise.
+__global__ __launch_bounds__(kWG) void k_qr_fallback(double* A, int m, int n, double* Q,
+                                                     const int* fail, int* flags) {
+  if (*fail == 0) return;
+  __shared__ WgScratch sh;
+  const bool ok = wg::householder_qr(A, m, m, n, Q, m, sh.tau, sh.dots, sh.red);
+  if (!ok && threadIdx.x == 0) atomicOr(flags, kFlagDecomposition);
+}
+
 // ----------------------------------------------------- rSVD finalisation
 // rsvd.hpp:85-165: eigen route on P Pᵀ with the rank tolerance, else dense SVD.
 __global__ __launch_bounds__(kWG) void k_rsvd_finalize(const double* P, int w, int q, int k,
@@ -576,6 +633,19 @@ void launch_qr(double* A, int m, int n, double* Q, int* flags, hipStream_t s) {
   if (n > kMaxW || m < n) throw std::invalid_argument("QR shape outside the supported range");
   hipLaunchKernelGGL(k_qr, dim3(1), dim3(kWG), 0, s, A, m, n, Q, flags);
   FPLS_LAUNCHED("k_qr");
+}
+
+void launch_chol_inv(const double* G, int w, double* Rinv, int* fail, hipStream_t s) {
+  if (w > 64) throw std::invalid_argument("CholQR2 handles at most 64 columns");
+  hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(kWG), 0, s, G, w, Rinv, fail);
+  FPLS_LAUNCHED("k_chol_inv");
+}
+
+void launch_qr_fallback(double* A, int m, int n, double* Q, const int* fail, int* flags,
+                        hipStream_t s) {
+  if (n > kMaxW || m < n) throw std::invalid_argument("QR shape outside the supported range");
+  hipLaunchKernelGGL(k_qr_fallback, dim3(1), dim3(kWG), 0, s, A, m, n, Q, fail, flags);
+  FPLS_LAUNCHED("k_qr_fallback");
 }
 
 void launch_rsvd_finalize(const double* P, int w, int q, int k, double* Up, double* sigma,

@@ -186,6 +186,259 @@ __global__ __launch_bounds__(kWG) void k_xtw(const XtwTask<T>* tasks, int count,
   }
 }
 
+// ------------------------------------------------------------ MFMA paths
+// Matrix-core versions of the two products for right-hand sides of 8..64
+// columns. FP32 inputs use v_mfma_f32_16x16x4f32 and FP64 inputs
+// v_mfma_f64_16x16x4f64; FP32 accumulators are folded into FP64 ones every
+// 256 rows (xtw) or 64 columns (xw), so long reductions keep FP64 accuracy.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template<class T> struct Mma;
+template<> struct Mma<float> {
+  using acc_t = f32x4;
+  __device__ static acc_t zero() { return acc_t{0.f, 0.f, 0.f, 0.f}; }
+  __device__ static acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+template<> struct Mma<double> {
+  using acc_t = f64x4;
+  __device__ static acc_t zero() { return acc_t{0.0, 0.0, 0.0, 0.0}; }
+  __device__ static acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+
+// xtw on matrix cores. A block owns 64 columns of A (16 per wave) and one row
+// split; each iteration covers 32 rows: lane (m, g) = (lane % 16, lane / 16)
+// holds rows 8g..8g+7 of column m of its wave, and MFMA call u pairs row
+// 8g+u of A with the same row of W (staged in LDS, rows contiguous).
+template<class T, int NB>
+__global__ __launch_bounds__(kWG) void k_xtw_mfma(const XtwTask<T>* tasks, int count,
+                                                  XtwTask<T> single) {
+  using M = Mma<T>;
+  constexpr int NBLK = NB / 16;
+  constexpr int PAD = 4;
+  const XtwTask<T>& t = tasks ? tasks[find_task(tasks, count, blockIdx.x)] : single;
+  const long long local = (long long)blockIdx.x - t.tile0;
+  const long long ctiles = xtw_col_tiles(t.k);
+  if (local >= ctiles * t.splits) return;
+  const int split = int(local / ctiles);
+  const long long col0 = (local % ctiles) * 64;
+  long long per = (t.m + t.splits - 1) / t.splits;
+  per = (per + 31) / 32 * 32;
+  const long long r0 = split * per;
+  const long long r1 = r0 + per < t.m ? r0 + per : t.m;
+
+  __shared__ T bs[NB][32 + PAD];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m16 = lane & 15, g = lane >> 4;
+  const long long col = col0 + wave * 16 + m16;
+  const bool cvalid = col < t.k;
+  double centre = 0.0;
+  if (cvalid) {
+    if (t.mode == kCentered && t.center != nullptr) centre = t.center[col];
+    else if (t.mode == kShiftStats) centre = double(t.A[col * t.m]);
+  }
+  const T centre_t = T(centre);
+  const T* xcol = t.A + (cvalid ? col : 0) * t.m;
+  const bool vec_ok = (t.m % (16 / sizeof(T))) == 0;
+
+  typename M::acc_t acc[NBLK];
+  double acc64[NBLK][4];
+#pragma unroll
+  for (int nb = 0; nb < NBLK; ++nb) {
+    acc[nb] = M::zero();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc64[nb][r] = 0.0;
+  }
+  double s1 = 0.0, s2 = 0.0;
+  int iter = 0;
+  for (long long i0 = r0; i0 < r1; i0 += 32, ++iter) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < 32 * NB; e += kWG) {
+      const int r = e & 31, c = e >> 5;
+      const long long wr = i0 + r;
+      bs[c][r] = (wr < r1 && c < t.N) ? T(t.W[wr + (long long)c * t.ldw]) : T(0);
+    }
+    __syncthreads();
+    T a[8];
+    const long long rb = i0 + 8 * g;
+    if (cvalid && vec_ok && rb + 8 <= r1) {
+      if constexpr (sizeof(T) == 4) {
+        const float4 v0 = *reinterpret_cast<const float4*>(xcol + rb);
+        const float4 v1 = *reinterpret_cast<const float4*>(xcol + rb + 4);
+        a[0] = v0.x; a[1] = v0.y; a[2] = v0.z; a[3] = v0.w;
+        a[4] = v1.x; a[5] = v1.y; a[6] = v1.z; a[7] = v1.w;
+      } else {
+#pragma unroll
+        for (int u = 0; u < 8; u += 2) {
+          const double2 v = *reinterpret_cast<const double2*>(xcol + rb + u);
+          a[u] = v.x; a[u + 1] = v.y;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a[u] -= centre_t;
+    } else {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const long long row = rb + u;
+        a[u] = (cvalid && row < r1) ? T(xcol[row] - centre_t) : T(0);
+      }
+    }
+    if (t.mode == kShiftStats) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const double v = double(a[u]);
+        s1 += v;
+        s2 = fma(v, v, s2);
+      }
+    }
+#pragma unroll
+    for (int nb = 0; nb < NBLK; ++nb) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc[nb] = M::mma(a[u], bs[nb * 16 + m16][8 * g + u], acc[nb]);
+    }
+    if ((iter & 7) == 7) {
+#pragma unroll
+      for (int nb = 0; nb < NBLK; ++nb) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc64[nb][r] += double(acc[nb][r]);
+        acc[nb] = M::zero();
+      }
+    }
+  }
+#pragma unroll
+  for (int nb = 0; nb < NBLK; ++nb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc64[nb][r] += double(acc[nb][r]);
+  double* o = t.out + (long long)split * t.slab;
+#pragma unroll
+  for (int nb = 0; nb < NBLK; ++nb) {
+    const int n = nb * 16 + m16;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const long long oc = col0 + wave * 16 + 4 * g + r;
+      if (oc < t.k && n < t.N) o[oc + (long long)n * t.ldo] = acc64[nb][r];
+    }
+  }
+  if (t.mode == kShiftStats && t.stats != nullptr) {
+    s1 += __shfl_xor(s1, 16, 64);
+    s1 += __shfl_xor(s1, 32, 64);
+    s2 += __shfl_xor(s2, 16, 64);
+    s2 += __shfl_xor(s2, 32, 64);
+    if (g == 0 && cvalid) {
+      double* st = t.stats + (long long)split * 2 * t.k;
+      st[col] = s1;
+      st[t.k + col] = s2;
+    }
+  }
+}
+
+// xw on matrix cores. A block owns 256 rows (64 per wave) and one column
+// split; each K-step covers 4 columns: lane (m, kk) = (lane % 16, lane / 16)
+// holds rows 4m..4m+3 of column kk, and MFMA call e computes output rows
+// {4m + e}. W rows of the current 64-column chunk are staged in LDS.
+template<class T, int NB>
+__global__ __launch_bounds__(kWG) void k_xw_mfma(const XwTask<T>* tasks, int count,
+                                                 XwTask<T> single) {
+  using M = Mma<T>;
+  constexpr int NBLK = NB / 16;
+  constexpr int PAD = 1;
+  const XwTask<T>& t = tasks ? tasks[find_task(tasks, count, blockIdx.x)] : single;
+  const long long local = (long long)blockIdx.x - t.tile0;
+  const long long rtiles = xw_row_tiles(t.m);
+  if (local >= rtiles * t.splits) return;
+  const int split = int(local / rtiles);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m16 = lane & 15, kk = lane >> 4;
+  const long long i0 = (local % rtiles) * kWG + wave * 64;
+  const long long rowb = i0 + 4 * m16;
+  long long per = (t.k + t.splits - 1) / t.splits;
+  per = (per + 3) / 4 * 4;
+  const long long j0 = split * per;
+  const long long j1 = j0 + per < t.k ? j0 + per : t.k;
+  const bool vec_ok = (t.m % (16 / sizeof(T))) == 0 && rowb + 4 <= t.m;
+
+  __shared__ T ws[64][NB + PAD];
+  __shared__ T cs[64];
+  typename M::acc_t acc[4][NBLK];
+  double acc64[4][NBLK][4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+#pragma unroll
+    for (int nb = 0; nb < NBLK; ++nb) {
+      acc[e][nb] = M::zero();
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc64[e][nb][r] = 0.0;
+    }
+
+  for (long long jc = j0; jc < j1; jc += 64) {
+    const int cnt = int(j1 - jc < 64 ? j1 - jc : 64);
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * NB; e += kWG) {
+      const int jj = e & 63, c = e >> 6;
+      ws[jj][c] = (jj < cnt && c < t.N) ? T(t.W[(jc + jj) + (long long)c * t.ldw]) : T(0);
+    }
+    if (threadIdx.x < 64)
+      cs[threadIdx.x] = (t.center != nullptr && int(threadIdx.x) < cnt) ? T(t.center[jc + threadIdx.x]) : T(0);
+    __syncthreads();
+    for (int s4 = 0; s4 < cnt; s4 += 4) {
+      const int jj = s4 + kk;
+      T x[4];
+      if (jj < cnt) {
+        const T* src = t.A + (jc + jj) * t.m + rowb;
+        if (vec_ok) {
+          if constexpr (sizeof(T) == 4) {
+            const float4 v = *reinterpret_cast<const float4*>(src);
+            x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
+          } else {
+            const double2 v0 = *reinterpret_cast<const double2*>(src);
+            const double2 v1 = *reinterpret_cast<const double2*>(src + 2);
+            x[0] = v0.x; x[1] = v0.y; x[2] = v1.x; x[3] = v1.y;
+          }
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) x[e] = rowb + e < t.m ? src[e] : T(0);
+        }
+        const T c = cs[jj];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) x[e] = rowb + e < t.m ? T(x[e] - c) : T(0);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) x[e] = T(0);
+      }
+#pragma unroll
+      for (int nb = 0; nb < NBLK; ++nb) {
+        const T b = jj < cnt ? ws[jj][nb * 16 + m16] : T(0);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc[e][nb] = M::mma(x[e], b, acc[e][nb]);
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+#pragma unroll
+      for (int nb = 0; nb < NBLK; ++nb) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc64[e][nb][r] += double(acc[e][nb][r]);
+        acc[e][nb] = M::zero();
+      }
+  }
+  double* o = t.out + (long long)split * t.slab;
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+#pragma unroll
+    for (int nb = 0; nb < NBLK; ++nb) {
+      const int n = nb * 16 + m16;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long long row = i0 + 4 * (4 * kk + r) + e;
+        if (row < t.m && n < t.N) o[row + (long long)n * t.ldo] = acc64[e][nb][r];
+      }
+    }
+}
+
 __global__ __launch_bounds__(kWG) void k_sum_slabs(const double* part, int splits,
                                                    long long stride, long long count,
                                                    double* out) {
@@ -211,11 +464,35 @@ void run_xtw(const XtwTask<T>* d, int count, const XtwTask<T>& single, long long
                      single);
 }
 
+template<class T, int NB>
+void run_xw_mfma(const XwTask<T>* d, int count, const XwTask<T>& single, long long tiles,
+                 hipStream_t stream) {
+  hipLaunchKernelGGL((k_xw_mfma<T, NB>), dim3(unsigned(tiles)), dim3(kWG), 0, stream, d, count,
+                     single);
+}
+
+template<class T, int NB>
+void run_xtw_mfma(const XtwTask<T>* d, int count, const XtwTask<T>& single, long long tiles,
+                  hipStream_t stream) {
+  hipLaunchKernelGGL((k_xtw_mfma<T, NB>), dim3(unsigned(tiles)), dim3(kWG), 0, stream, d,
+                     count, single);
+}
+
+// Right-hand sides of 8 or more columns go to the matrix cores.
+constexpr int kMfmaMinN = 8;
+
 template<class T>
 void dispatch_xw(const XwTask<T>* d, int count, const XwTask<T>& single, long long tiles,
                  int maxn, hipStream_t stream) {
   if (tiles <= 0) return;
   if (tiles > 0x7fffffffLL) throw std::invalid_argument("xw grid too large");
+  if (maxn >= kMfmaMinN) {
+    if (maxn <= 16) run_xw_mfma<T, 16>(d, count, single, tiles, stream);
+    else if (maxn <= 32) run_xw_mfma<T, 32>(d, count, single, tiles, stream);
+    else run_xw_mfma<T, 64>(d, count, single, tiles, stream);
+    FPLS_LAUNCHED("k_xw_mfma");
+    return;
+  }
   if (maxn <= 1) run_xw<T, 1>(d, count, single, tiles, stream);
   else if (maxn <= 4) run_xw<T, 4>(d, count, single, tiles, stream);
   else if (maxn <= 16) run_xw<T, 16>(d, count, single, tiles, stream);
@@ -229,6 +506,13 @@ void dispatch_xtw(const XtwTask<T>* d, int count, const XtwTask<T>& single, long
                   int maxn, hipStream_t stream) {
   if (tiles <= 0) return;
   if (tiles > 0x7fffffffLL) throw std::invalid_argument("xtw grid too large");
+  if (maxn >= kMfmaMinN) {
+    if (maxn <= 16) run_xtw_mfma<T, 16>(d, count, single, tiles, stream);
+    else if (maxn <= 32) run_xtw_mfma<T, 32>(d, count, single, tiles, stream);
+    else run_xtw_mfma<T, 64>(d, count, single, tiles, stream);
+    FPLS_LAUNCHED("k_xtw_mfma");
+    return;
+  }
   if (maxn <= 4) run_xtw<T, 4>(d, count, single, tiles, stream);
   else if (maxn <= 16) run_xtw<T, 16>(d, count, single, tiles, stream);
   else run_xtw<T, 64>(d, count, single, tiles, stream);
