@@ -8,6 +8,7 @@
 // rsvd.hpp, plssvd.hpp, lda.hpp) so that results agree with the FP64 CPU
 // path for the same seed.
 #include "../../include/fastpls_hip.h"
+#include "host_util.hpp"
 #include "kernels.hpp"
 #include "stream.hpp"
 
@@ -24,68 +25,6 @@
 
 namespace fpls {
 namespace {
-
-// ------------------------------------------------------------- buffers
-class Buf {
- public:
-  Buf() = default;
-  Buf(const Buf&) = delete;
-  Buf& operator=(const Buf&) = delete;
-  ~Buf() { release(); }
-  void alloc(size_t bytes) {
-    release();
-    if (bytes) FPLS_HIP(hipMalloc(&ptr_, bytes));
-    bytes_ = bytes;
-  }
-  void reserve(size_t bytes) {
-    if (bytes > bytes_) alloc(bytes);
-  }
-  void release() noexcept {
-    if (ptr_) (void)hipFree(ptr_);
-    ptr_ = nullptr;
-    bytes_ = 0;
-  }
-  template<class U> U* as() const { return static_cast<U*>(ptr_); }
-  double* d() const { return static_cast<double*>(ptr_); }
-  bool empty() const { return ptr_ == nullptr; }
-  size_t bytes() const { return bytes_; }
-
- private:
-  void* ptr_ = nullptr;
-  size_t bytes_ = 0;
-};
-
-// Gaussian sketch exactly as the reference draws it (simpls.hpp:244-249).
-void gaussian(unsigned int seed, size_t count, double* out) {
-  std::mt19937 generator(seed);
-  std::normal_distribution<double> normal(0.0, 1.0);
-  for (size_t i = 0; i < count; ++i) out[i] = normal(generator);
-}
-
-// Split counts: aim for >= 2048 work-groups, keep every split >= 32 wide.
-int splits_for(long long tiles_per_split, long long extent, long long min_chunk) {
-  long long s = (2048 + tiles_per_split - 1) / std::max<long long>(tiles_per_split, 1);
-  const long long cap = std::max<long long>(1, extent / min_chunk);
-  s = std::max<long long>(1, std::min(s, cap));
-  return int(std::min<long long>(s, 256));
-}
-
-// simpls.hpp:43-62
-int candidate_block_size(long long remaining, long long p, long long q, bool cls, long long n,
-                         long long maximum) {
-  const double work = double(n) * double(p) * double(q);
-  if (q <= 1 || remaining < 4 || work < 5.0e8) return 1;
-  if (cls && q <= 2048) return int(std::max<long long>(1, std::min({maximum, remaining, p, q})));
-  if (!cls && double(p) * double(q) > 64.0 * 1024.0 * 1024.0)
-    return int(std::max<long long>(1, std::min({8LL, maximum, remaining, p, q})));
-  return 1;
-}
-
-struct Event {
-  hipEvent_t e = nullptr;
-  Event() { FPLS_HIP(hipEventCreate(&e)); }
-  ~Event() { if (e) (void)hipEventDestroy(e); }
-};
 
 // ------------------------------------------------------------- handle
 class Handle {
@@ -394,7 +333,7 @@ class Model final : public Handle {
                 long long ldw, int N, double* out) {
     for (int c0 = 0; c0 < N; c0 += 64) {
       const int nb = std::min(64, N - c0);
-      const int splits = splits_for(xtw_col_tiles(k), m, 256);
+      const int splits = xtw_splits(m, k, nb);
       XtwTask<U> t{};
       t.A = A; t.m = m; t.k = k; t.center = center; t.W = W + (long long)c0 * ldw; t.ldw = ldw;
       t.N = nb; t.mode = center ? kCentered : kPlain; t.splits = splits;
@@ -418,16 +357,24 @@ class Model final : public Handle {
     colss_.alloc(sizeof(double) * p_);
     S_.alloc(sizeof(double) * (size_t)p_ * q_);
     S0_.alloc(sizeof(double) * (size_t)p_ * q_);
-    const int splits = splits_for(xtw_col_tiles(p_), n_, 256);
+    const int splits = xtw_splits(n_, p_, std::min(q_, 64));
     Buf s_part, stats;
     s_part.alloc(sizeof(double) * (size_t)splits * p_ * q_);
     stats.alloc(sizeof(double) * (size_t)splits * 2 * p_);
+    // the centred responses in the element type of X (the matrix-core operand)
+    const T* ycT = nullptr;
+    if (std::is_same<T, float>::value) {
+      ycT_.alloc(sizeof(T) * (size_t)n_ * q_);
+      launch_convert<T>(Yc_.d(), ycT_.as<T>(), (long long)n_ * q_, stream_);
+      ycT = ycT_.as<T>();
+    }
     FPLS_HIP(hipEventRecord(cp0_.e, stream_));
     for (int c0 = 0; c0 < q_; c0 += 64) {
       const int nb = std::min(64, q_ - c0);
       XtwTask<T> t{};
       t.A = X_; t.m = n_; t.k = p_; t.center = nullptr;
       t.W = Yc_.d() + (long long)c0 * n_; t.ldw = n_; t.N = nb; t.mode = kShiftStats;
+      t.Wt = ycT ? ycT + (long long)c0 * n_ : nullptr;
       t.splits = splits;
       t.out = s_part.d() + (long long)c0 * p_; t.ldo = p_; t.slab = (long long)p_ * q_;
       t.stats = c0 == 0 ? stats.d() : nullptr;
@@ -484,7 +431,7 @@ class Model final : public Handle {
     cand_scaled_.alloc(sizeof(double) * (size_t)p_ * bmax);
     t_raw_.alloc(sizeof(double) * (size_t)n_ * bmax);
     const int xs = splits_for(xw_row_tiles(n_), p_, 32);
-    const int ls = splits_for(xtw_col_tiles(p_), n_, 256);
+    const int ls = xtw_splits(n_, p_, bmax);
     ts_part_.alloc(sizeof(double) * (size_t)xs * n_ * bmax);
     ls_part_.alloc(sizeof(double) * (size_t)ls * p_ * bmax);
 
@@ -492,6 +439,7 @@ class Model final : public Handle {
     f.n = n_; f.p = p_; f.q = q_; f.A = A_;
     f.power = power_; f.use_right_gram = right_gram ? 1 : 0; f.reorth = 0;
     f.nsteps = int(steps.size()); f.bmax = bmax; f.ts_splits = xs; f.ls_splits = ls;
+    f.ls_slab = (long long)p_ * bmax;
     f.steps = steps_.as<RefreshStep>(); f.omega = omega_.d();
     f.S = S_.d(); f.S0 = S0_.d(); f.G = right_gram ? G_.d() : nullptr; f.sx = sx_.d();
     f.cand = cand_.d(); f.cand_scaled = cand_scaled_.d(); f.t_raw = t_raw_.d();
@@ -515,7 +463,7 @@ class Model final : public Handle {
       XtwTask<T> xt{};
       xt.A = X_; xt.m = n_; xt.k = p_; xt.center = mx_.d(); xt.W = t_raw_.d(); xt.ldw = n_;
       xt.N = b; xt.mode = kCentered; xt.splits = ls; xt.out = ls_part_.d(); xt.ldo = p_;
-      xt.slab = (long long)p_ * b;
+      xt.slab = (long long)p_ * bmax;
       launch_xtw_single<T>(xt, stream_);
       launch_simpls_update(fitdev_.as<SimplsFitDev>(), 1, int(s), stream_);
     }
@@ -555,6 +503,11 @@ class Model final : public Handle {
       part_.reserve(sizeof(double) * (size_t)splits * n_ * k);
       XwTask<T> t{};
       t.A = X_; t.m = n_; t.k = p_; t.center = mx_.d(); t.W = Rs_.d(); t.ldw = p_; t.N = k;
+      if (std::is_same<T, float>::value) {
+        rsT_.alloc(sizeof(T) * (size_t)p_ * k);
+        launch_convert<T>(Rs_.d(), rsT_.as<T>(), (long long)p_ * k, stream_);
+        t.Wt = rsT_.as<T>();
+      }
       t.splits = splits; t.out = splits == 1 ? Tsc_.d() : part_.d(); t.ldo = n_;
       t.slab = (long long)n_ * k;
       FPLS_HIP(hipEventRecord(sc0_.e, stream_));
@@ -805,6 +758,7 @@ class Model final : public Handle {
   Buf xown_, yown_, labels_dev_, counts_, Yc_, H_, ysum_, my_, mx_, sx_, colss_, S_, S0_, G_;
   Buf R_, Q_, V_, Tsc_, Rs_, sigma_, gram_, W_, state_;
   Buf omega_, steps_, fitdev_, simpls_scratch_, cand_, cand_scaled_, t_raw_, ts_part_, ls_part_;
+  Buf ycT_, rsT_;
   Buf part_, tmp1_, tmp2_, tmp3_, qr_gram_, qr_rinv_, qr_tmp_, qr_fail_, zero_ints_;
   bool zero_ready_ = false;
   Buf rows_, scores_, pred64_, outT_, top_, metric_, flags_, obs_, obs_labels_;

@@ -1,0 +1,87 @@
+// Host-side helpers shared by the single-fit engine and the batch engine.
+#pragma once
+
+#include "common.hpp"
+#include "stream.hpp"
+
+#include <algorithm>
+#include <cstddef>
+#include <random>
+
+namespace fpls {
+
+// ------------------------------------------------------------- buffers
+class Buf {
+ public:
+  Buf() = default;
+  Buf(const Buf&) = delete;
+  Buf& operator=(const Buf&) = delete;
+  ~Buf() { release(); }
+  void alloc(size_t bytes) {
+    release();
+    if (bytes) FPLS_HIP(hipMalloc(&ptr_, bytes));
+    bytes_ = bytes;
+  }
+  void reserve(size_t bytes) {
+    if (bytes > bytes_) alloc(bytes);
+  }
+  void release() noexcept {
+    if (ptr_) (void)hipFree(ptr_);
+    ptr_ = nullptr;
+    bytes_ = 0;
+  }
+  template<class U> U* as() const { return static_cast<U*>(ptr_); }
+  double* d() const { return static_cast<double*>(ptr_); }
+  bool empty() const { return ptr_ == nullptr; }
+  size_t bytes() const { return bytes_; }
+
+ private:
+  void* ptr_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+// Gaussian sketch exactly as the reference draws it (simpls.hpp:244-249).
+inline void gaussian(unsigned int seed, size_t count, double* out) {
+  std::mt19937 generator(seed);
+  std::normal_distribution<double> normal(0.0, 1.0);
+  for (size_t i = 0; i < count; ++i) out[i] = normal(generator);
+}
+
+// Split counts: aim for >= 2048 work-groups, keep every split >= 32 wide.
+inline int splits_for(long long tiles_per_split, long long extent, long long min_chunk) {
+  long long s = (2048 + tiles_per_split - 1) / std::max<long long>(tiles_per_split, 1);
+  const long long cap = std::max<long long>(1, extent / min_chunk);
+  s = std::max<long long>(1, std::min(s, cap));
+  return int(std::min<long long>(s, 256));
+}
+
+// Row splits of an xtw pass over an m x k matrix with N right-hand columns:
+// enough work-groups to fill the device, and no split longer than the FP32
+// accumulation bound of the matrix-core kernel.
+inline int xtw_splits(long long m, long long k, int N) {
+  int s = splits_for(xtw_tiles(k, N), m, 256);
+  if (N >= kMfmaMinN) {
+    const long long need = (m + kXtwSplitRows - 1) / kXtwSplitRows;
+    if (need > s) s = int(std::min<long long>(need, 4096));
+  }
+  return s;
+}
+
+// simpls.hpp:43-62
+inline int candidate_block_size(long long remaining, long long p, long long q, bool cls, long long n,
+                         long long maximum) {
+  const double work = double(n) * double(p) * double(q);
+  if (q <= 1 || remaining < 4 || work < 5.0e8) return 1;
+  if (cls && q <= 2048) return int(std::max<long long>(1, std::min({maximum, remaining, p, q})));
+  if (!cls && double(p) * double(q) > 64.0 * 1024.0 * 1024.0)
+    return int(std::max<long long>(1, std::min({8LL, maximum, remaining, p, q})));
+  return 1;
+}
+
+struct Event {
+  hipEvent_t e = nullptr;
+  Event() { FPLS_HIP(hipEventCreate(&e)); }
+  ~Event() { if (e) (void)hipEventDestroy(e); }
+};
+
+}  // namespace fpls

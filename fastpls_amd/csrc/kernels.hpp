@@ -21,6 +21,7 @@ struct SimplsFitDev {
   int n, p, q, A;          // A = maximum number of components
   int power, use_right_gram, reorth, nsteps;
   int bmax, ts_splits, ls_splits, pad;
+  long long ls_slab;       // distance between loading-pass split slabs
   const RefreshStep* steps;
   const double* omega;     // sketch pool
   double* S;               // p x q, deflated in place

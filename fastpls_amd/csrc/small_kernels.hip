@@ -151,7 +151,7 @@ __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits,
       Rc[i] = f.cand[i + (long long)j * p] * inv;
       double s = 0.0;
       for (int sp = 0; sp < f.ls_splits; ++sp)
-        s += f.ls_part[(long long)sp * p * st.b + i + (long long)j * p];
+        s += f.ls_part[(long long)sp * f.ls_slab + i + (long long)j * p];
       Vc[i] = s / f.sx[i] * inv;
     }
     __syncthreads();
@@ -206,9 +206,20 @@ __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits,
 }
 
 // --------------------------------------------------------------- tall QR
+// Householder QR; a matrix of up to 6144 doubles is factorised in LDS.
 __global__ __launch_bounds__(kWG) void k_qr(double* A, int m, int n, double* Q, int* flags) {
   __shared__ WgScratch sh;
-  const bool ok = wg::householder_qr(A, m, m, n, Q, m, sh.tau, sh.dots, sh.red);
+  __shared__ double la[6144];
+  __shared__ double lq[6144];
+  bool ok;
+  if ((long long)m * n <= 6144) {
+    for (int e = threadIdx.x; e < m * n; e += kWG) la[e] = A[e];
+    __syncthreads();
+    ok = wg::householder_qr(la, m, m, n, lq, m, sh.tau, sh.dots, sh.red);
+    for (int e = threadIdx.x; e < m * n; e += kWG) Q[e] = lq[e];
+  } else {
+    ok = wg::householder_qr(A, m, m, n, Q, m, sh.tau, sh.dots, sh.red);
+  }
   if (!ok && threadIdx.x == 0) atomicOr(flags, kFlagDecomposition);
 }
 
@@ -220,44 +231,56 @@ __global__ __launch_bounds__(kWG) void k_qr(double* A, int m, int n, double* Q, 
 // fallback then recomputes Q.
 __global__ __launch_bounds__(kWG) void k_chol_inv(const double* G, int w, double* Rinv,
                                                   int* fail) {
-  __shared__ double Lm[64 * 64];
+  __shared__ double Lm[64 * 65];
+  __shared__ double Ri[64 * 65];
   __shared__ int bad;
+  __shared__ double dmax;
   const int tid = threadIdx.x;
+  for (int e = tid; e < w * w; e += kWG) Lm[(e % w) + (e / w) * 65] = G[e];
   if (tid == 0) {
     bad = (*fail != 0) ? 1 : 0;
-    double dmax = 0.0;
-    for (int i = 0; i < w; ++i) dmax = fmax(dmax, G[i + i * w]);
-    for (int i = 0; i < w && !bad; ++i) {
-      for (int j = 0; j <= i; ++j) {
-        double v = G[i + j * w];
-        for (int l = 0; l < j; ++l) v -= Lm[i + l * 64] * Lm[j + l * 64];
-        if (i == j) {
-          if (!isfinite(v) || !(v > 1e-13 * dmax)) { bad = 1; break; }
-          Lm[i + i * 64] = sqrt(v);
-        } else {
-          Lm[i + j * 64] = v / Lm[j + j * 64];
-        }
-      }
-    }
-    if (bad) *fail = 1;
+    double d = 0.0;
+    for (int i = 0; i < w; ++i) d = fmax(d, G[i + i * w]);
+    dmax = d;
   }
   __syncthreads();
+  // right-looking Cholesky in LDS, one column per step (lower triangle)
+  for (int j = 0; j < w; ++j) {
+    if (tid == 0 && !bad) {
+      const double v = Lm[j + j * 65];
+      if (!isfinite(v) || !(v > 1e-13 * dmax)) bad = 1;
+      else Lm[j + j * 65] = sqrt(v);
+    }
+    __syncthreads();
+    if (bad) break;
+    const double djj = Lm[j + j * 65];
+    for (int i = j + 1 + tid; i < w; i += kWG) Lm[i + j * 65] /= djj;
+    __syncthreads();
+    for (int e = tid; e < (w - j - 1) * (w - j - 1); e += kWG) {
+      const int i = j + 1 + e % (w - j - 1), c = j + 1 + e / (w - j - 1);
+      if (c <= i) Lm[i + c * 65] -= Lm[i + j * 65] * Lm[c + j * 65];
+    }
+    __syncthreads();
+  }
+  if (bad) {
+    if (tid == 0) *fail = 1;
+    for (int e = tid; e < w * w; e += kWG) Rinv[e] = (e % w) == (e / w) ? 1.0 : 0.0;
+    return;
+  }
   // Rinv = L^{-T}: column c solves Lᵀ x = e_c (upper triangular, x_i = 0 for i > c)
   for (int c = tid; c < w; c += kWG) {
     for (int i = w - 1; i >= 0; --i) {
-      double v;
-      if (bad) {
+      double v = 0.0;
+      if (i <= c) {
         v = i == c ? 1.0 : 0.0;
-      } else if (i > c) {
-        v = 0.0;
-      } else {
-        v = i == c ? 1.0 : 0.0;
-        for (int l = i + 1; l <= c; ++l) v -= Lm[l + i * 64] * Rinv[l + c * w];
-        v /= Lm[i + i * 64];
+        for (int l = i + 1; l <= c; ++l) v -= Lm[l + i * 65] * Ri[l + c * 65];
+        v /= Lm[i + i * 65];
       }
-      Rinv[i + c * w] = v;
+      Ri[i + c * 65] = v;
     }
   }
+  __syncthreads();
+  for (int e = tid; e < w * w; e += kWG) Rinv[e] = Ri[(e % w) + (e / w) * 65];
 }
 
 // Householder recomputation of Q when CholQR2 failed; a no-op otherwise.

@@ -27,6 +27,7 @@ struct XwTask {
   long long m, k;
   const double* center;  // k or null
   const double* W;       // k x N, leading dimension ldw
+  const T* Wt;           // optional copy of W in the element type (preferred)
   long long ldw;
   int N;
   int splits;            // column splits (k dimension)
@@ -42,6 +43,7 @@ struct XtwTask {
   long long m, k;
   const double* center;  // k or null (kCentered)
   const double* W;       // m x N, leading dimension ldw
+  const T* Wt;           // optional copy of W in the element type (preferred)
   long long ldw;
   int N;
   int mode;
@@ -76,6 +78,15 @@ void launch_xtw_single(XtwTask<T> task, hipStream_t stream);
 // Number of 256-row tiles of an xw task and of 64-column tiles of xtw.
 __host__ __device__ inline long long xw_row_tiles(long long m) { return (m + 255) / 256; }
 __host__ __device__ inline long long xtw_col_tiles(long long k) { return (k + 63) / 64; }
+// Right-hand sides of this many columns or more run on the matrix cores.
+constexpr int kMfmaMinN = 8;
+// Column tiles of an xtw launch whose widest task has maxn columns: the
+// matrix-core kernel owns 128 columns per work-group, the VALU kernel 64.
+__host__ __device__ inline long long xtw_tiles(long long k, int maxn) {
+  return maxn >= kMfmaMinN ? (k + 127) / 128 : (k + 63) / 64;
+}
+// Longest row split of the matrix-core xtw (FP32 accumulators run over it).
+constexpr long long kXtwSplitRows = 4096;
 
 // Deterministic sum of `splits` slabs: out[i] = sum_s part[s * stride + i].
 void launch_sum_slabs(const double* part, int splits, long long stride,

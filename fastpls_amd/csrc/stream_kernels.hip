@@ -194,9 +194,13 @@ __global__ __launch_bounds__(kWG) void k_xtw(const XtwTask<T>* tasks, int count,
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
+// Output layout of the 16x16 tiles (measured on gfx950): item r of lane l
+// holds D[row(l / 16, r)][l % 16], with row(g, r) = 4g + r for FP32 and
+// g + 4r for FP64.
 template<class T> struct Mma;
 template<> struct Mma<float> {
   using acc_t = f32x4;
+  __device__ static int row(int g, int r) { return 4 * g + r; }
   __device__ static acc_t zero() { return acc_t{0.f, 0.f, 0.f, 0.f}; }
   __device__ static acc_t mma(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
@@ -204,142 +208,174 @@ template<> struct Mma<float> {
 };
 template<> struct Mma<double> {
   using acc_t = f64x4;
+  __device__ static int row(int g, int r) { return g + 4 * r; }
   __device__ static acc_t zero() { return acc_t{0.0, 0.0, 0.0, 0.0}; }
   __device__ static acc_t mma(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
 };
 
-// xtw on matrix cores. A block owns 64 columns of A (16 per wave) and one row
-// split; each iteration covers 32 rows: lane (m, g) = (lane % 16, lane / 16)
-// holds rows 8g..8g+7 of column m of its wave, and MFMA call u pairs row
-// 8g+u of A with the same row of W (staged in LDS, rows contiguous).
+// xtw on matrix cores. A block owns 128 columns of A (32 per wave, two
+// 16-column M-blocks) and one row split of at most kXtwSplitRows rows, over
+// which the FP32 accumulators run before being written as FP64 partials.
+// Each iteration covers 64 rows: lane (m, g) = (lane % 16, lane / 16) holds
+// rows 16g..16g+15 of column m of each M-block (all loads issued before the
+// first MFMA), and MFMA call u pairs row 16g+u of A with the same row of W.
+// W tiles (64 rows x NB) are double-buffered in LDS: one barrier per
+// iteration.
+constexpr int kXtwMfmaCols = 128;
+constexpr int kXtwRows = 64;
+
 template<class T, int NB>
 __global__ __launch_bounds__(kWG) void k_xtw_mfma(const XtwTask<T>* tasks, int count,
                                                   XtwTask<T> single) {
   using M = Mma<T>;
   constexpr int NBLK = NB / 16;
-  constexpr int PAD = 4;
+  constexpr int LD = kXtwRows + 4;
   const XtwTask<T>& t = tasks ? tasks[find_task(tasks, count, blockIdx.x)] : single;
   const long long local = (long long)blockIdx.x - t.tile0;
-  const long long ctiles = xtw_col_tiles(t.k);
+  const long long ctiles = (t.k + kXtwMfmaCols - 1) / kXtwMfmaCols;
   if (local >= ctiles * t.splits) return;
   const int split = int(local / ctiles);
-  const long long col0 = (local % ctiles) * 64;
+  const long long col0 = (local % ctiles) * kXtwMfmaCols;
   long long per = (t.m + t.splits - 1) / t.splits;
-  per = (per + 31) / 32 * 32;
+  per = (per + kXtwRows - 1) / kXtwRows * kXtwRows;
   const long long r0 = split * per;
   const long long r1 = r0 + per < t.m ? r0 + per : t.m;
 
-  __shared__ T bs[NB][32 + PAD];
+  __shared__ T bs[2][NB][LD];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m16 = lane & 15, g = lane >> 4;
-  const long long col = col0 + wave * 16 + m16;
-  const bool cvalid = col < t.k;
-  double centre = 0.0;
-  if (cvalid) {
-    if (t.mode == kCentered && t.center != nullptr) centre = t.center[col];
-    else if (t.mode == kShiftStats) centre = double(t.A[col * t.m]);
-  }
-  const T centre_t = T(centre);
-  const T* xcol = t.A + (cvalid ? col : 0) * t.m;
-  const bool vec_ok = (t.m % (16 / sizeof(T))) == 0;
-
-  typename M::acc_t acc[NBLK];
-  double acc64[NBLK][4];
+  long long col[2];
+  bool cvalid[2];
+  T centre[2];
+  const T* xcol[2];
 #pragma unroll
-  for (int nb = 0; nb < NBLK; ++nb) {
-    acc[nb] = M::zero();
-#pragma unroll
-    for (int r = 0; r < 4; ++r) acc64[nb][r] = 0.0;
-  }
-  double s1 = 0.0, s2 = 0.0;
-  int iter = 0;
-  for (long long i0 = r0; i0 < r1; i0 += 32, ++iter) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < 32 * NB; e += kWG) {
-      const int r = e & 31, c = e >> 5;
-      const long long wr = i0 + r;
-      bs[c][r] = (wr < r1 && c < t.N) ? T(t.W[wr + (long long)c * t.ldw]) : T(0);
+  for (int mb = 0; mb < 2; ++mb) {
+    col[mb] = col0 + wave * 32 + mb * 16 + m16;
+    cvalid[mb] = col[mb] < t.k;
+    double c = 0.0;
+    if (cvalid[mb]) {
+      if (t.mode == kCentered && t.center != nullptr) c = t.center[col[mb]];
+      else if (t.mode == kShiftStats) c = double(t.A[col[mb] * t.m]);
     }
-    __syncthreads();
-    T a[8];
-    const long long rb = i0 + 8 * g;
-    if (cvalid && vec_ok && rb + 8 <= r1) {
-      if constexpr (sizeof(T) == 4) {
-        const float4 v0 = *reinterpret_cast<const float4*>(xcol + rb);
-        const float4 v1 = *reinterpret_cast<const float4*>(xcol + rb + 4);
-        a[0] = v0.x; a[1] = v0.y; a[2] = v0.z; a[3] = v0.w;
-        a[4] = v1.x; a[5] = v1.y; a[6] = v1.z; a[7] = v1.w;
+    centre[mb] = T(c);
+    xcol[mb] = t.A + (cvalid[mb] ? col[mb] : 0) * t.m;
+  }
+  const bool vec_ok = (t.m % (16 / sizeof(T))) == 0;
+  const T* Wt = t.Wt;
+
+  // stage W rows [i0, i0 + 64) into buffer b (zero outside [r0, r1) or N)
+  auto stage = [&](int b, long long i0) {
+    for (int e = threadIdx.x; e < kXtwRows * NB; e += kWG) {
+      const int r = e % kXtwRows, c = e / kXtwRows;
+      const long long wr = i0 + r;
+      T v = T(0);
+      if (wr < r1 && c < t.N) v = Wt ? Wt[wr + (long long)c * t.ldw] : T(t.W[wr + (long long)c * t.ldw]);
+      bs[b][c][r] = v;
+    }
+  };
+
+  typename M::acc_t acc[2][NBLK];
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < NBLK; ++nb) acc[mb][nb] = M::zero();
+  double s1[2] = {0.0, 0.0}, s2[2] = {0.0, 0.0};
+
+  if (r0 < r1) stage(0, r0);
+  __syncthreads();
+  int buf = 0;
+  for (long long i0 = r0; i0 < r1; i0 += kXtwRows) {
+    T a[2][16];
+    const long long rb = i0 + 16 * g;
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+      if (cvalid[mb] && vec_ok && rb + 16 <= r1) {
+        if constexpr (sizeof(T) == 4) {
+#pragma unroll
+          for (int u = 0; u < 16; u += 4) {
+            const float4 v = *reinterpret_cast<const float4*>(xcol[mb] + rb + u);
+            a[mb][u] = v.x; a[mb][u + 1] = v.y; a[mb][u + 2] = v.z; a[mb][u + 3] = v.w;
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < 16; u += 2) {
+            const double2 v = *reinterpret_cast<const double2*>(xcol[mb] + rb + u);
+            a[mb][u] = v.x; a[mb][u + 1] = v.y;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) a[mb][u] -= centre[mb];
       } else {
 #pragma unroll
-        for (int u = 0; u < 8; u += 2) {
-          const double2 v = *reinterpret_cast<const double2*>(xcol + rb + u);
-          a[u] = v.x; a[u + 1] = v.y;
+        for (int u = 0; u < 16; ++u) {
+          const long long row = rb + u;
+          a[mb][u] = (cvalid[mb] && row < r1) ? T(xcol[mb][row] - centre[mb]) : T(0);
         }
       }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) a[u] -= centre_t;
-    } else {
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const long long row = rb + u;
-        a[u] = (cvalid && row < r1) ? T(xcol[row] - centre_t) : T(0);
-      }
     }
+    // prefetch the next W tile into the other buffer while this one is used
+    if (i0 + kXtwRows < r1) stage(buf ^ 1, i0 + kXtwRows);
     if (t.mode == kShiftStats) {
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const double v = double(a[u]);
-        s1 += v;
-        s2 = fma(v, v, s2);
-      }
+      for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const double v = double(a[mb][u]);
+          s1[mb] += v;
+          s2[mb] = fma(v, v, s2[mb]);
+        }
     }
 #pragma unroll
     for (int nb = 0; nb < NBLK; ++nb) {
+      T b[16];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) acc[nb] = M::mma(a[u], bs[nb * 16 + m16][8 * g + u], acc[nb]);
-    }
-    if ((iter & 7) == 7) {
+      for (int u = 0; u < 16; ++u) b[u] = bs[buf][nb * 16 + m16][16 * g + u];
 #pragma unroll
-      for (int nb = 0; nb < NBLK; ++nb) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) acc64[nb][r] += double(acc[nb][r]);
-        acc[nb] = M::zero();
+      for (int u = 0; u < 16; ++u) {
+        acc[0][nb] = M::mma(a[0][u], b[u], acc[0][nb]);
+        acc[1][nb] = M::mma(a[1][u], b[u], acc[1][nb]);
       }
     }
+    __syncthreads();
+    buf ^= 1;
   }
-#pragma unroll
-  for (int nb = 0; nb < NBLK; ++nb)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) acc64[nb][r] += double(acc[nb][r]);
   double* o = t.out + (long long)split * t.slab;
 #pragma unroll
-  for (int nb = 0; nb < NBLK; ++nb) {
-    const int n = nb * 16 + m16;
+  for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const long long oc = col0 + wave * 16 + 4 * g + r;
-      if (oc < t.k && n < t.N) o[oc + (long long)n * t.ldo] = acc64[nb][r];
+    for (int nb = 0; nb < NBLK; ++nb) {
+      const int n = nb * 16 + m16;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long long oc = col0 + wave * 32 + mb * 16 + M::row(g, r);
+        if (oc < t.k && n < t.N) o[oc + (long long)n * t.ldo] = double(acc[mb][nb][r]);
+      }
     }
-  }
   if (t.mode == kShiftStats && t.stats != nullptr) {
-    s1 += __shfl_xor(s1, 16, 64);
-    s1 += __shfl_xor(s1, 32, 64);
-    s2 += __shfl_xor(s2, 16, 64);
-    s2 += __shfl_xor(s2, 32, 64);
-    if (g == 0 && cvalid) {
-      double* st = t.stats + (long long)split * 2 * t.k;
-      st[col] = s1;
-      st[t.k + col] = s2;
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+      double a1 = s1[mb], a2 = s2[mb];
+      a1 += __shfl_xor(a1, 16, 64);
+      a1 += __shfl_xor(a1, 32, 64);
+      a2 += __shfl_xor(a2, 16, 64);
+      a2 += __shfl_xor(a2, 32, 64);
+      if (g == 0 && cvalid[mb]) {
+        double* st = t.stats + (long long)split * 2 * t.k;
+        st[col[mb]] = a1;
+        st[t.k + col[mb]] = a2;
+      }
     }
   }
 }
 
 // xw on matrix cores. A block owns 256 rows (64 per wave) and one column
-// split; each K-step covers 4 columns: lane (m, kk) = (lane % 16, lane / 16)
-// holds rows 4m..4m+3 of column kk, and MFMA call e computes output rows
-// {4m + e}. W rows of the current 64-column chunk are staged in LDS.
+// split; the split is walked in chunks of 64 columns whose W rows are staged
+// in LDS. Within a chunk lane (m, kk) = (lane % 16, lane / 16) loads rows
+// 4m..4m+3 of columns 4s+kk for all 16 K-steps s before the first MFMA, and
+// MFMA call e of step s produces output rows {4m + e}. FP32 accumulators are
+// folded into FP64 ones after every chunk.
 template<class T, int NB>
 __global__ __launch_bounds__(kWG) void k_xw_mfma(const XwTask<T>* tasks, int count,
                                                  XwTask<T> single) {
@@ -356,10 +392,11 @@ __global__ __launch_bounds__(kWG) void k_xw_mfma(const XwTask<T>* tasks, int cou
   const long long i0 = (local % rtiles) * kWG + wave * 64;
   const long long rowb = i0 + 4 * m16;
   long long per = (t.k + t.splits - 1) / t.splits;
-  per = (per + 3) / 4 * 4;
+  per = (per + 63) / 64 * 64;
   const long long j0 = split * per;
   const long long j1 = j0 + per < t.k ? j0 + per : t.k;
   const bool vec_ok = (t.m % (16 / sizeof(T))) == 0 && rowb + 4 <= t.m;
+  const bool rows_ok = rowb < t.m;
 
   __shared__ T ws[64][NB + PAD];
   __shared__ T cs[64];
@@ -379,41 +416,46 @@ __global__ __launch_bounds__(kWG) void k_xw_mfma(const XwTask<T>* tasks, int cou
     __syncthreads();
     for (int e = threadIdx.x; e < 64 * NB; e += kWG) {
       const int jj = e & 63, c = e >> 6;
-      ws[jj][c] = (jj < cnt && c < t.N) ? T(t.W[(jc + jj) + (long long)c * t.ldw]) : T(0);
+      T v = T(0);
+      if (jj < cnt && c < t.N)
+        v = t.Wt ? t.Wt[(jc + jj) + (long long)c * t.ldw] : T(t.W[(jc + jj) + (long long)c * t.ldw]);
+      ws[jj][c] = v;
     }
     if (threadIdx.x < 64)
       cs[threadIdx.x] = (t.center != nullptr && int(threadIdx.x) < cnt) ? T(t.center[jc + threadIdx.x]) : T(0);
-    __syncthreads();
-    for (int s4 = 0; s4 < cnt; s4 += 4) {
-      const int jj = s4 + kk;
-      T x[4];
-      if (jj < cnt) {
-        const T* src = t.A + (jc + jj) * t.m + rowb;
-        if (vec_ok) {
-          if constexpr (sizeof(T) == 4) {
-            const float4 v = *reinterpret_cast<const float4*>(src);
-            x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
-          } else {
-            const double2 v0 = *reinterpret_cast<const double2*>(src);
-            const double2 v1 = *reinterpret_cast<const double2*>(src + 2);
-            x[0] = v0.x; x[1] = v0.y; x[2] = v1.x; x[3] = v1.y;
-          }
+    // all 16 K-steps of this chunk in flight at once
+    T x[16][4];
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      const int jj = 4 * s + kk;
+      const T* src = t.A + (jc + jj) * t.m + rowb;
+      if (jj < cnt && vec_ok) {
+        if constexpr (sizeof(T) == 4) {
+          const float4 v = *reinterpret_cast<const float4*>(src);
+          x[s][0] = v.x; x[s][1] = v.y; x[s][2] = v.z; x[s][3] = v.w;
         } else {
-#pragma unroll
-          for (int e = 0; e < 4; ++e) x[e] = rowb + e < t.m ? src[e] : T(0);
+          const double2 v0 = *reinterpret_cast<const double2*>(src);
+          const double2 v1 = *reinterpret_cast<const double2*>(src + 2);
+          x[s][0] = v0.x; x[s][1] = v0.y; x[s][2] = v1.x; x[s][3] = v1.y;
         }
-        const T c = cs[jj];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) x[e] = rowb + e < t.m ? T(x[e] - c) : T(0);
       } else {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) x[e] = T(0);
+        for (int e = 0; e < 4; ++e)
+          x[s][e] = (jj < cnt && rows_ok && rowb + e < t.m) ? src[e] : T(0);
       }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      const int jj = 4 * s + kk;
+      const T c = cs[jj];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) x[s][e] = (jj < cnt && rowb + e < t.m) ? T(x[s][e] - c) : T(0);
 #pragma unroll
       for (int nb = 0; nb < NBLK; ++nb) {
-        const T b = jj < cnt ? ws[jj][nb * 16 + m16] : T(0);
+        const T b = ws[jj][nb * 16 + m16];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) acc[e][nb] = M::mma(x[e], b, acc[e][nb]);
+        for (int e = 0; e < 4; ++e) acc[e][nb] = M::mma(x[s][e], b, acc[e][nb]);
       }
     }
 #pragma unroll
@@ -433,7 +475,7 @@ __global__ __launch_bounds__(kWG) void k_xw_mfma(const XwTask<T>* tasks, int cou
       const int n = nb * 16 + m16;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const long long row = i0 + 4 * (4 * kk + r) + e;
+        const long long row = i0 + 4 * M::row(kk, r) + e;
         if (row < t.m && n < t.N) o[row + (long long)n * t.ldo] = acc64[e][nb][r];
       }
     }
@@ -477,9 +519,6 @@ void run_xtw_mfma(const XtwTask<T>* d, int count, const XtwTask<T>& single, long
   hipLaunchKernelGGL((k_xtw_mfma<T, NB>), dim3(unsigned(tiles)), dim3(kWG), 0, stream, d,
                      count, single);
 }
-
-// Right-hand sides of 8 or more columns go to the matrix cores.
-constexpr int kMfmaMinN = 8;
 
 template<class T>
 void dispatch_xw(const XwTask<T>* d, int count, const XwTask<T>& single, long long tiles,
@@ -538,14 +577,16 @@ long long pack_xw(XwTask<T>* tasks, int count, int* maxn) {
 
 template<class T>
 long long pack_xtw(XtwTask<T>* tasks, int count, int* maxn) {
-  long long tiles = 0;
   int mn = 0;
   for (int i = 0; i < count; ++i) {
     if (tasks[i].N < 1 || tasks[i].N > 64) throw std::invalid_argument("xtw supports 1..64 right-hand columns");
     if (tasks[i].splits < 1) throw std::invalid_argument("xtw needs at least one split");
-    tasks[i].tile0 = tiles;
-    tiles += xtw_col_tiles(tasks[i].k) * tasks[i].splits;
     mn = std::max(mn, tasks[i].N);
+  }
+  long long tiles = 0;
+  for (int i = 0; i < count; ++i) {
+    tasks[i].tile0 = tiles;
+    tiles += xtw_tiles(tasks[i].k, mn) * tasks[i].splits;
   }
   if (maxn) *maxn = mn;
   return tiles;
