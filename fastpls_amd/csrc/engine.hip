@@ -298,6 +298,18 @@ class Model final : public Handle {
     }
   }
 
+  // Operand workspace of at least `bytes` (null when no launch needs one).
+  // Growing it first drains the stream, so no queued kernel still reads the
+  // old buffer.
+  void* frag(size_t bytes) {
+    if (bytes == 0) return nullptr;
+    if (bytes > frag_.bytes()) {
+      FPLS_HIP(hipStreamSynchronize(stream_));
+      frag_.alloc(bytes);
+    }
+    return frag_.as<void>();
+  }
+
   void fill_zero_int(int* ptr, int count) {
     // A kernel (not hipMemsetAsync) so the stream stays graph-capturable.
     std::vector<int> zeros(count, 0);
@@ -311,10 +323,10 @@ class Model final : public Handle {
                long long ldw, int N, double* out) {
     for (int c0 = 0; c0 < N; c0 += 64) {
       const int nb = std::min(64, N - c0);
-      const int splits = splits_for(xw_row_tiles(m), k, 32);
+      const int splits = xw_splits(m, k, nb);
       XwTask<U> t{};
       t.A = A; t.m = m; t.k = k; t.center = center; t.W = W + (long long)c0 * ldw; t.ldw = ldw;
-      t.N = nb; t.splits = splits;
+      t.N = nb; t.splits = splits; t.frag = frag(xw_frag_bytes<U>(k, nb));
       if (splits == 1) {
         t.out = out + (long long)c0 * m; t.ldo = m; t.slab = 0;
         launch_xw_single<U>(t, stream_);
@@ -337,6 +349,7 @@ class Model final : public Handle {
       XtwTask<U> t{};
       t.A = A; t.m = m; t.k = k; t.center = center; t.W = W + (long long)c0 * ldw; t.ldw = ldw;
       t.N = nb; t.mode = center ? kCentered : kPlain; t.splits = splits;
+      t.frag = frag(xtw_frag_bytes<U>(m, nb));
       if (splits == 1) {
         t.out = out + (long long)c0 * k; t.ldo = k; t.slab = 0;
         launch_xtw_single<U>(t, stream_);
@@ -361,20 +374,14 @@ class Model final : public Handle {
     Buf s_part, stats;
     s_part.alloc(sizeof(double) * (size_t)splits * p_ * q_);
     stats.alloc(sizeof(double) * (size_t)splits * 2 * p_);
-    // the centred responses in the element type of X (the matrix-core operand)
-    const T* ycT = nullptr;
-    if (std::is_same<T, float>::value) {
-      ycT_.alloc(sizeof(T) * (size_t)n_ * q_);
-      launch_convert<T>(Yc_.d(), ycT_.as<T>(), (long long)n_ * q_, stream_);
-      ycT = ycT_.as<T>();
-    }
+    void* wf = frag(xtw_frag_bytes<T>(n_, std::min(q_, 64)));
     FPLS_HIP(hipEventRecord(cp0_.e, stream_));
     for (int c0 = 0; c0 < q_; c0 += 64) {
       const int nb = std::min(64, q_ - c0);
       XtwTask<T> t{};
       t.A = X_; t.m = n_; t.k = p_; t.center = nullptr;
       t.W = Yc_.d() + (long long)c0 * n_; t.ldw = n_; t.N = nb; t.mode = kShiftStats;
-      t.Wt = ycT ? ycT + (long long)c0 * n_ : nullptr;
+      t.frag = wf;
       t.splits = splits;
       t.out = s_part.d() + (long long)c0 * p_; t.ldo = p_; t.slab = (long long)p_ * q_;
       t.stats = c0 == 0 ? stats.d() : nullptr;
@@ -430,7 +437,7 @@ class Model final : public Handle {
     cand_.alloc(sizeof(double) * (size_t)p_ * bmax);
     cand_scaled_.alloc(sizeof(double) * (size_t)p_ * bmax);
     t_raw_.alloc(sizeof(double) * (size_t)n_ * bmax);
-    const int xs = splits_for(xw_row_tiles(n_), p_, 32);
+    const int xs = xw_splits(n_, p_, bmax);
     const int ls = xtw_splits(n_, p_, bmax);
     ts_part_.alloc(sizeof(double) * (size_t)xs * n_ * bmax);
     ls_part_.alloc(sizeof(double) * (size_t)ls * p_ * bmax);
@@ -457,6 +464,7 @@ class Model final : public Handle {
       XwTask<T> xw{};
       xw.A = X_; xw.m = n_; xw.k = p_; xw.center = mx_.d(); xw.W = cand_scaled_.d(); xw.ldw = p_;
       xw.N = b; xw.splits = xs; xw.out = ts_part_.d(); xw.ldo = n_; xw.slab = (long long)n_ * b;
+      xw.frag = frag(xw_frag_bytes<T>(p_, b));
       launch_xw_single<T>(xw, stream_);
       launch_sum_slabs(ts_part_.d(), xs, (long long)n_ * b, (long long)n_ * b, t_raw_.d(), stream_);
       // loadings: (X - 1 mᵀ)ᵀ t  (divided by s in the update)
@@ -464,6 +472,7 @@ class Model final : public Handle {
       xt.A = X_; xt.m = n_; xt.k = p_; xt.center = mx_.d(); xt.W = t_raw_.d(); xt.ldw = n_;
       xt.N = b; xt.mode = kCentered; xt.splits = ls; xt.out = ls_part_.d(); xt.ldo = p_;
       xt.slab = (long long)p_ * bmax;
+      xt.frag = frag(xtw_frag_bytes<T>(n_, b));
       launch_xtw_single<T>(xt, stream_);
       launch_simpls_update(fitdev_.as<SimplsFitDev>(), 1, int(s), stream_);
     }
@@ -499,15 +508,11 @@ class Model final : public Handle {
     Tsc_.alloc(sizeof(double) * (size_t)n_ * k);
     {
       // scores T = (X - 1 mᵀ)(R / s): the second and last pass over X
-      const int splits = splits_for(xw_row_tiles(n_), p_, 32);
+      const int splits = xw_splits(n_, p_, k);
       part_.reserve(sizeof(double) * (size_t)splits * n_ * k);
       XwTask<T> t{};
       t.A = X_; t.m = n_; t.k = p_; t.center = mx_.d(); t.W = Rs_.d(); t.ldw = p_; t.N = k;
-      if (std::is_same<T, float>::value) {
-        rsT_.alloc(sizeof(T) * (size_t)p_ * k);
-        launch_convert<T>(Rs_.d(), rsT_.as<T>(), (long long)p_ * k, stream_);
-        t.Wt = rsT_.as<T>();
-      }
+      t.frag = frag(xw_frag_bytes<T>(p_, k));
       t.splits = splits; t.out = splits == 1 ? Tsc_.d() : part_.d(); t.ldo = n_;
       t.slab = (long long)n_ * k;
       FPLS_HIP(hipEventRecord(sc0_.e, stream_));
@@ -758,7 +763,7 @@ class Model final : public Handle {
   Buf xown_, yown_, labels_dev_, counts_, Yc_, H_, ysum_, my_, mx_, sx_, colss_, S_, S0_, G_;
   Buf R_, Q_, V_, Tsc_, Rs_, sigma_, gram_, W_, state_;
   Buf omega_, steps_, fitdev_, simpls_scratch_, cand_, cand_scaled_, t_raw_, ts_part_, ls_part_;
-  Buf ycT_, rsT_;
+  Buf frag_;  // matrix-core operand workspace of the streaming products
   Buf part_, tmp1_, tmp2_, tmp3_, qr_gram_, qr_rinv_, qr_tmp_, qr_fail_, zero_ints_;
   bool zero_ready_ = false;
   Buf rows_, scores_, pred64_, outT_, top_, metric_, flags_, obs_, obs_labels_;
@@ -811,7 +816,7 @@ template<class T>
 void host_gemm(const T* left, const T* right, int rows, int inner, int cols, T* out) {
   hipStream_t s = nullptr;
   FPLS_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-  Buf a, b, c, c64, part;
+  Buf a, b, c, c64, part, wf;
   try {
     a.alloc(sizeof(T) * (size_t)rows * inner);
     b.alloc(sizeof(T) * (size_t)inner * cols);
@@ -825,11 +830,13 @@ void host_gemm(const T* left, const T* right, int rows, int inner, int cols, T* 
     FPLS_HIP(hipMemcpyAsync(b64.d(), r64.data(), sizeof(double) * r64.size(), hipMemcpyHostToDevice, s));
     for (int c0 = 0; c0 < cols; c0 += 64) {
       const int nb = std::min(64, cols - c0);
-      const int splits = splits_for(xw_row_tiles(rows), inner, 32);
+      const int splits = xw_splits(rows, inner, nb);
       part.reserve(sizeof(double) * (size_t)splits * rows * nb);
+      wf.reserve(xw_frag_bytes<T>(inner, nb));
       XwTask<T> t{};
       t.A = a.as<T>(); t.m = rows; t.k = inner; t.center = nullptr;
       t.W = b64.d() + (size_t)c0 * inner; t.ldw = inner; t.N = nb; t.splits = splits;
+      t.frag = wf.as<void>();
       t.out = part.d(); t.ldo = rows; t.slab = (long long)rows * nb;
       launch_xw_single<T>(t, s);
       launch_sum_slabs(part.d(), splits, (long long)rows * nb, (long long)rows * nb,

@@ -67,6 +67,18 @@ inline int xtw_splits(long long m, long long k, int N) {
   return s;
 }
 
+// Column splits of an xw pass (m x k times k x N): enough work-groups to fill
+// the device, and no split wider than the FP32 accumulation bound of the
+// matrix-core kernel.
+inline int xw_splits(long long m, long long k, int N) {
+  int s = splits_for(xw_row_tiles(m), k, 32);
+  if (N >= kMfmaMinN) {
+    const long long need = (k + kXwSplitCols - 1) / kXwSplitCols;
+    if (need > s) s = int(std::min<long long>(need, 4096));
+  }
+  return s;
+}
+
 // simpls.hpp:43-62
 inline int candidate_block_size(long long remaining, long long p, long long q, bool cls, long long n,
                          long long maximum) {

@@ -27,7 +27,7 @@ struct XwTask {
   long long m, k;
   const double* center;  // k or null
   const double* W;       // k x N, leading dimension ldw
-  const T* Wt;           // optional copy of W in the element type (preferred)
+  void* frag;            // matrix-core operand workspace, xw_frag_bytes<T>(k, maxn) bytes
   long long ldw;
   int N;
   int splits;            // column splits (k dimension)
@@ -43,7 +43,7 @@ struct XtwTask {
   long long m, k;
   const double* center;  // k or null (kCentered)
   const double* W;       // m x N, leading dimension ldw
-  const T* Wt;           // optional copy of W in the element type (preferred)
+  void* frag;            // matrix-core operand workspace, xtw_frag_bytes<T>(m, maxn) bytes
   long long ldw;
   int N;
   int mode;
@@ -85,8 +85,33 @@ constexpr int kMfmaMinN = 8;
 __host__ __device__ inline long long xtw_tiles(long long k, int maxn) {
   return maxn >= kMfmaMinN ? (k + 127) / 128 : (k + 63) / 64;
 }
-// Longest row split of the matrix-core xtw (FP32 accumulators run over it).
+// Longest row split of the matrix-core xtw and longest column split of the
+// matrix-core xw (their FP32 accumulators run over one split).
 constexpr long long kXtwSplitRows = 4096;
+constexpr long long kXwSplitCols = 4096;
+
+// Right-hand width a matrix-core launch is compiled for.
+inline int mfma_width(int maxn) { return maxn <= 16 ? 16 : maxn <= 32 ? 32 : maxn <= 48 ? 48 : 64; }
+
+// Bytes of the operand workspace (XwTask::frag / XtwTask::frag) a task needs
+// when the widest task of its launch has maxn right-hand columns; 0 means the
+// launch does not read it. For FP32 A the matrix-core kernels run in bf16x3
+// split precision and W is split once per launch into bf16 high and low parts
+// laid out in fragment order:
+//   xw : [hi, lo][NB][k rounded up to 32] (W transposed) + FP32 centre [k32]
+//   xtw: [hi, lo][NB][m rounded up to 64], rows permuted within 64-row tiles
+template<class T>
+inline size_t xw_frag_bytes(long long k, int maxn) {
+  if (sizeof(T) != 4 || maxn < kMfmaMinN) return 0;
+  const size_t k32 = size_t((k + 31) / 32 * 32);
+  return k32 * (size_t)mfma_width(maxn) * 4 + k32 * 4;
+}
+template<class T>
+inline size_t xtw_frag_bytes(long long m, int maxn) {
+  if (sizeof(T) != 4 || maxn < kMfmaMinN) return 0;
+  const size_t m64 = size_t((m + 63) / 64 * 64);
+  return m64 * (size_t)mfma_width(maxn) * 4;
+}
 
 // Deterministic sum of `splits` slabs: out[i] = sum_s part[s * stride + i].
 void launch_sum_slabs(const double* part, int splits, long long stride,

@@ -1,9 +1,16 @@
 // Streaming products over tall column-major matrices (see stream.hpp).
 //
-// Roofline: both kernels read A once per launch (m*k*sizeof(T) bytes) and are
-// meant to run at HBM bandwidth; W tiles come from L2 through LDS. The
-// arithmetic is FP64 FMA on values widened from T, so the FP32 storage path
-// loses nothing to accumulation.
+// Roofline: every kernel reads A once per launch (m*k*sizeof(T) bytes) and is
+// meant to run at HBM bandwidth; W operands come from L2 (directly or through
+// LDS). Three arithmetic paths:
+//   * VALU, FP64 FMA on values widened from T (right-hand sides below 8);
+//   * FP32 A on the matrix cores in bf16x3 split precision:
+//     x·w ~ xh·wh + xh·wl + xl·wh with xh = bf16(x), xl = bf16(x - xh), FP32
+//     accumulation over at most 4096 rows / columns, FP64 partial slabs;
+//   * FP64 A on the FP64 matrix cores.
+// Kernel arguments are read once into registers (task lists through the
+// constant address space) and A is read through global-address-space
+// pointers, so no generic (flat) access stalls the load pipeline.
 #include "stream.hpp"
 
 #include <algorithm>
@@ -12,14 +19,58 @@ namespace fpls {
 
 namespace {
 
+// Address-space qualifiers exist for the device pass only; the host pass
+// merely parses the kernel bodies.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define FPLS_GLOBAL __attribute__((address_space(1)))
+#define FPLS_CONSTANT __attribute__((address_space(4)))
+#else
+#define FPLS_GLOBAL
+#define FPLS_CONSTANT
+#endif
+
+template<class T>
+__device__ __forceinline__ const T FPLS_GLOBAL* gp(const T* p) { return (const T FPLS_GLOBAL*)p; }
+template<class T>
+__device__ __forceinline__ T FPLS_GLOBAL* gpw(T* p) { return (T FPLS_GLOBAL*)p; }
+
+// The task of this work-group: the single by-value task, or a binary search
+// of the flattened tile space of a device task list.
 template<class Task>
-__device__ inline int find_task(const Task* tasks, int count, long long block) {
+__device__ __forceinline__ Task load_task(const Task* tasks, int count, const Task& single) {
+  if (tasks == nullptr) return single;
+  const Task FPLS_CONSTANT* ct = (const Task FPLS_CONSTANT*)tasks;
   int lo = 0, hi = count - 1;
+  const long long b = blockIdx.x;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (tasks[mid].tile0 <= block) lo = mid; else hi = mid - 1;
+    if (ct[mid].tile0 <= b) lo = mid; else hi = mid - 1;
   }
-  return lo;
+  return ct[lo];
+}
+
+// Task `index` of a device task list, or the single by-value task.
+template<class Task>
+__device__ __forceinline__ Task task_at(const Task* tasks, unsigned index, const Task& single) {
+  if (tasks == nullptr) return single;
+  return ((const Task FPLS_CONSTANT*)tasks)[index];
+}
+
+// Rows of one split of an xtw task (a multiple of 64) and columns of one split
+// of an xw task (a multiple of 32).
+__device__ __forceinline__ long long split_extent(long long extent, int splits, int quantum) {
+  const long long per = (extent + splits - 1) / splits;
+  return (per + quantum - 1) / quantum * quantum;
+}
+
+// Tiles are handed to the 8 XCDs round-robin (tile b runs on XCD b % 8).
+// Renumber them so that each XCD walks a contiguous range of the task's tile
+// space: work-groups sharing an L2 then share row splits (and W tiles).
+__device__ __forceinline__ long long xcd_local(long long local, long long total) {
+  constexpr long long kXcds = 8;
+  const long long q = total / kXcds, r = total % kXcds;
+  const long long xcd = local % kXcds, pos = local / kXcds;
+  return xcd < r ? xcd * (q + 1) + pos : r * (q + 1) + (xcd - r) * q + pos;
 }
 
 // ------------------------------------------------------------------- xw
@@ -27,7 +78,7 @@ __device__ inline int find_task(const Task* tasks, int count, long long block) {
 // 32 columns whose W rows (and centres) are staged in LDS and broadcast.
 template<class T, int NB>
 __global__ __launch_bounds__(kWG) void k_xw(const XwTask<T>* tasks, int count, XwTask<T> single) {
-  const XwTask<T>& t = tasks ? tasks[find_task(tasks, count, blockIdx.x)] : single;
+  const XwTask<T> t = load_task(tasks, count, single);
   const long long local = (long long)blockIdx.x - t.tile0;
   const long long rtiles = xw_row_tiles(t.m);
   if (local >= rtiles * t.splits) return;
@@ -43,17 +94,18 @@ __global__ __launch_bounds__(kWG) void k_xw(const XwTask<T>* tasks, int count, X
 #pragma unroll
   for (int c = 0; c < NB; ++c) acc[c] = 0.0;
   const bool valid = row < t.m;
-  const T* a = t.A + (valid ? row : 0);
+  const T FPLS_GLOBAL* a = gp(t.A) + (valid ? row : 0);
+  const double FPLS_GLOBAL* W = gp(t.W);
 
   for (long long jc = j0; jc < j1; jc += 32) {
     const int cnt = int(j1 - jc < 32 ? j1 - jc : 32);
     __syncthreads();
     for (int e = threadIdx.x; e < 32 * NB; e += kWG) {
       const int jj = e & 31, c = e >> 5;
-      ws[jj][c] = (jj < cnt && c < t.N) ? t.W[(jc + jj) + (long long)c * t.ldw] : 0.0;
+      ws[jj][c] = (jj < cnt && c < t.N) ? W[(jc + jj) + (long long)c * t.ldw] : 0.0;
     }
     if (threadIdx.x < 32) {
-      cs[threadIdx.x] = (t.center != nullptr && int(threadIdx.x) < cnt) ? t.center[jc + threadIdx.x] : 0.0;
+      cs[threadIdx.x] = (t.center != nullptr && int(threadIdx.x) < cnt) ? gp(t.center)[jc + threadIdx.x] : 0.0;
     }
     __syncthreads();
     if (valid) {
@@ -74,7 +126,7 @@ __global__ __launch_bounds__(kWG) void k_xw(const XwTask<T>* tasks, int count, X
     }
   }
   if (valid) {
-    double* o = t.out + (long long)split * t.slab + row;
+    double FPLS_GLOBAL* o = gpw(t.out) + (long long)split * t.slab + row;
 #pragma unroll
     for (int c = 0; c < NB; ++c) if (c < t.N) o[(long long)c * t.ldo] = acc[c];
   }
@@ -89,7 +141,7 @@ __global__ __launch_bounds__(kWG) void k_xtw(const XtwTask<T>* tasks, int count,
   constexpr int MN = NB >= 16 ? NB / 16 : 1;
   constexpr int MC = (NB / 4) / MN;
   constexpr int GROUPS = 64 / MC;  // column groups
-  const XtwTask<T>& t = tasks ? tasks[find_task(tasks, count, blockIdx.x)] : single;
+  const XtwTask<T> t = load_task(tasks, count, single);
   const long long local = (long long)blockIdx.x - t.tile0;
   const long long ctiles = xtw_col_tiles(t.k);
   if (local >= ctiles * t.splits) return;
@@ -98,6 +150,8 @@ __global__ __launch_bounds__(kWG) void k_xtw(const XtwTask<T>* tasks, int count,
   const long long per = (t.m + t.splits - 1) / t.splits;
   const long long r0 = split * per;
   const long long r1 = r0 + per < t.m ? r0 + per : t.m;
+  const T FPLS_GLOBAL* A = gp(t.A);
+  const double FPLS_GLOBAL* W = gp(t.W);
 
   __shared__ double xs[32][65];
   __shared__ double wsh[32][NB + 1];
@@ -112,8 +166,8 @@ __global__ __launch_bounds__(kWG) void k_xtw(const XtwTask<T>* tasks, int count,
     const long long col = col0 + lcol + 8 * e;
     double c = 0.0;
     if (col < t.k) {
-      if (t.mode == kCentered && t.center != nullptr) c = t.center[col];
-      else if (t.mode == kShiftStats) c = double(t.A[col * t.m]);
+      if (t.mode == kCentered && t.center != nullptr) c = gp(t.center)[col];
+      else if (t.mode == kShiftStats) c = double(A[col * t.m]);
     }
     centre[e] = c; s1[e] = 0.0; s2[e] = 0.0;
   }
@@ -133,7 +187,7 @@ __global__ __launch_bounds__(kWG) void k_xtw(const XtwTask<T>* tasks, int count,
       const long long col = col0 + lcol + 8 * e;
       double v = 0.0;
       if (rv && col < t.k) {
-        v = double(t.A[row + col * t.m]) - centre[e];
+        v = double(A[row + col * t.m]) - centre[e];
         s1[e] += v;
         s2[e] = fma(v, v, s2[e]);
       }
@@ -142,7 +196,7 @@ __global__ __launch_bounds__(kWG) void k_xtw(const XtwTask<T>* tasks, int count,
     for (int e = tid; e < 32 * NB; e += kWG) {
       const int r = e & 31, c = e >> 5;
       const long long wr = i0 + r;
-      wsh[r][c] = (wr < r1 && c < t.N) ? t.W[wr + (long long)c * t.ldw] : 0.0;
+      wsh[r][c] = (wr < r1 && c < t.N) ? W[wr + (long long)c * t.ldw] : 0.0;
     }
     __syncthreads();
 #pragma unroll 4
@@ -158,7 +212,7 @@ __global__ __launch_bounds__(kWG) void k_xtw(const XtwTask<T>* tasks, int count,
         for (int b = 0; b < MN; ++b) acc[a][b] = fma(xa[a], wb[b], acc[a][b]);
     }
   }
-  double* o = t.out + (long long)split * t.slab;
+  double FPLS_GLOBAL* o = gpw(t.out) + (long long)split * t.slab;
 #pragma unroll
   for (int a = 0; a < MC; ++a) {
     const long long col = col0 + g * MC + a;
@@ -178,7 +232,7 @@ __global__ __launch_bounds__(kWG) void k_xtw(const XtwTask<T>* tasks, int count,
       }
       const long long col = col0 + lcol + 8 * e;
       if (lrow == 0 && col < t.k) {
-        double* st = t.stats + (long long)split * 2 * t.k;
+        double FPLS_GLOBAL* st = gpw(t.stats) + (long long)split * 2 * t.k;
         st[col] = a1;
         st[t.k + col] = a2;
       }
@@ -187,161 +241,286 @@ __global__ __launch_bounds__(kWG) void k_xtw(const XtwTask<T>* tasks, int count,
 }
 
 // ------------------------------------------------------------ MFMA paths
-// Matrix-core versions of the two products for right-hand sides of 8..64
-// columns. FP32 inputs use v_mfma_f32_16x16x4f32 and FP64 inputs
-// v_mfma_f64_16x16x4f64; FP32 accumulators are folded into FP64 ones every
-// 256 rows (xtw) or 64 columns (xw), so long reductions keep FP64 accuracy.
+// Fragment conventions of the 16x16 matrix-core instructions (gfx950): for
+// D = A·B with A 16 x K and B K x 16, lane l supplies row l % 16 of A and
+// column l % 16 of B over the K slice owned by lane group l / 16 (one value
+// per lane for the x4 FP64 shape, eight bf16 for the x32 shape); the products
+// only pair A and B values held in the same lane slot, so any assignment of
+// rows of the reduction to slots is valid as long as A and B agree. Results
+// (measured on gfx950): item r of lane l holds D[4(l/16) + r][l % 16] for FP32
+// accumulators and D[l/16 + 4r][l % 16] for FP64 accumulators.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-// Output layout of the 16x16 tiles (measured on gfx950): item r of lane l
-// holds D[row(l / 16, r)][l % 16], with row(g, r) = 4g + r for FP32 and
-// g + 4r for FP64.
-template<class T> struct Mma;
-template<> struct Mma<float> {
-  using acc_t = f32x4;
-  __device__ static int row(int g, int r) { return 4 * g + r; }
-  __device__ static acc_t zero() { return acc_t{0.f, 0.f, 0.f, 0.f}; }
-  __device__ static acc_t mma(float a, float b, acc_t c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+__device__ __forceinline__ unsigned pack_bf16(float a, float b) {
+  const bf16x2 v = {(__bf16)a, (__bf16)b};
+  return __builtin_bit_cast(unsigned, v);
+}
+
+// hi = bf16(x) and lo = bf16(x - hi), both rounded to nearest even: hi + lo
+// holds x to within 2^-16 |x|. Two values per 32-bit word, first in the low half.
+__device__ __forceinline__ void split_pair(float a, float b, unsigned& hi, unsigned& lo) {
+  hi = pack_bf16(a, b);
+  const float ha = __builtin_bit_cast(float, hi << 16);
+  const float hb = __builtin_bit_cast(float, hi & 0xffff0000u);
+  lo = pack_bf16(a - ha, b - hb);
+}
+
+__device__ __forceinline__ void split8(const float (&v)[8], u32x4& hi, u32x4& lo) {
+  unsigned h0, h1, h2, h3, l0, l1, l2, l3;
+  split_pair(v[0], v[1], h0, l0);
+  split_pair(v[2], v[3], h1, l1);
+  split_pair(v[4], v[5], h2, l2);
+  split_pair(v[6], v[7], h3, l3);
+  hi = u32x4{h0, h1, h2, h3};
+  lo = u32x4{l0, l1, l2, l3};
+}
+
+// acc += a·b in bf16x3: the two cross terms first, then the leading term.
+__device__ __forceinline__ f32x4 mma3(const u32x4& ah, const u32x4& al, const u32x4& bh,
+                                      const u32x4& bl, f32x4 acc) {
+  const bf16x8 AH = __builtin_bit_cast(bf16x8, ah), AL = __builtin_bit_cast(bf16x8, al);
+  const bf16x8 BH = __builtin_bit_cast(bf16x8, bh), BL = __builtin_bit_cast(bf16x8, bl);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(AL, BH, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(AH, BL, acc, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(AH, BH, acc, 0, 0, 0);
+}
+
+__device__ __forceinline__ f64x4 mma64(double a, double b, f64x4 acc) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+}
+
+// ---------------------------------------------- bf16x3 operand splitting
+// xw operand: W (k x N) -> hi/lo [2][NB][k32] in W's own column order, plus
+// the centre in FP32 (zero past k and past N). Grid: (blocks, tasks).
+template<int NB>
+__global__ __launch_bounds__(kWG) void k_xw_split(const XwTask<float>* tasks, XwTask<float> single) {
+  const XwTask<float> t = task_at(tasks, blockIdx.y, single);
+  const long long k32 = (t.k + 31) / 32 * 32;
+  unsigned short FPLS_GLOBAL* hi = (unsigned short FPLS_GLOBAL*)t.frag;
+  unsigned short FPLS_GLOBAL* lo = hi + (long long)NB * k32;
+  float FPLS_GLOBAL* cf = (float FPLS_GLOBAL*)(hi + 2LL * NB * k32);
+  const double FPLS_GLOBAL* W = gp(t.W);
+  const long long stride = (long long)gridDim.x * kWG;
+  // two adjacent elements per thread
+  for (long long e = ((long long)blockIdx.x * kWG + threadIdx.x) * 2; e < (long long)NB * k32; e += 2 * stride) {
+    const long long n = e / k32, j = e % k32;  // k32 is even: j, j + 1 share n
+    const float a = (n < t.N && j < t.k) ? float(W[j + n * t.ldw]) : 0.f;
+    const float b = (n < t.N && j + 1 < t.k) ? float(W[j + 1 + n * t.ldw]) : 0.f;
+    unsigned h, l;
+    split_pair(a, b, h, l);
+    *(unsigned FPLS_GLOBAL*)(hi + e) = h;
+    *(unsigned FPLS_GLOBAL*)(lo + e) = l;
   }
-};
-template<> struct Mma<double> {
-  using acc_t = f64x4;
-  __device__ static int row(int g, int r) { return g + 4 * r; }
-  __device__ static acc_t zero() { return acc_t{0.0, 0.0, 0.0, 0.0}; }
-  __device__ static acc_t mma(double a, double b, acc_t c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  for (long long j = (long long)blockIdx.x * kWG + threadIdx.x; j < k32; j += stride)
+    cf[j] = (t.center != nullptr && j < t.k) ? float(gp(t.center)[j]) : 0.f;
+}
+
+// xtw operand: W (m x N) -> hi/lo [2][NB][m64]; within each 64-row tile,
+// position 32h + 8g + j holds row 16(2h + j/4) + 4g + j%4, so that lane group
+// g of reduction step h reads its eight values as one 16-byte chunk.
+template<int NB>
+__global__ __launch_bounds__(kWG) void k_xtw_split(const XtwTask<float>* tasks, XtwTask<float> single) {
+  const XtwTask<float> t = task_at(tasks, blockIdx.y, single);
+  const long long m64 = (t.m + 63) / 64 * 64;
+  unsigned short FPLS_GLOBAL* hi = (unsigned short FPLS_GLOBAL*)t.frag;
+  unsigned short FPLS_GLOBAL* lo = hi + (long long)NB * m64;
+  const double FPLS_GLOBAL* W = gp(t.W);
+  const long long stride = (long long)gridDim.x * kWG;
+  for (long long e = ((long long)blockIdx.x * kWG + threadIdx.x) * 2; e < (long long)NB * m64; e += 2 * stride) {
+    const long long n = e / m64, p = e % m64;
+    const long long tile = p >> 6;
+    float v[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int pp = int((p + u) & 63);
+      const int h = pp >> 5, g = (pp >> 3) & 3, j = pp & 7;
+      const long long row = tile * 64 + 16 * (2 * h + (j >> 2)) + 4 * g + (j & 3);
+      v[u] = (n < t.N && row < t.m) ? float(W[row + n * t.ldw]) : 0.f;
+    }
+    unsigned h, l;
+    split_pair(v[0], v[1], h, l);
+    *(unsigned FPLS_GLOBAL*)(hi + e) = h;
+    *(unsigned FPLS_GLOBAL*)(lo + e) = l;
   }
-};
+}
 
-// xtw on matrix cores. A block owns 128 columns of A (32 per wave, two
-// 16-column M-blocks) and one row split of at most kXtwSplitRows rows, over
-// which the FP32 accumulators run before being written as FP64 partials.
-// Each iteration covers 64 rows: lane (m, g) = (lane % 16, lane / 16) holds
-// rows 16g..16g+15 of column m of each M-block (all loads issued before the
-// first MFMA), and MFMA call u pairs row 16g+u of A with the same row of W.
-// W tiles (64 rows x NB) are double-buffered in LDS: one barrier per
-// iteration.
-constexpr int kXtwMfmaCols = 128;
-constexpr int kXtwRows = 64;
+// --------------------------------------------------- xtw, bf16x3 (FP32 A)
+// A work-group owns 128 columns of A (32 per wave: two 16-column M-blocks)
+// and one row split. Each 64-row step: lane (c, g) = (lane % 16, lane / 16)
+// holds rows 16v + 4g + e (v, e < 4) of its column of each M-block, loaded as
+// float4; reduction step h pairs rows {16(2h + j/4) + 4g + j%4} with the same
+// rows of W, whose split fragments are staged in LDS (double-buffered). The
+// next step's A and W loads are issued before the current step's arithmetic.
+constexpr int kXtwCols = 128;
 
-template<class T, int NB>
-__global__ __launch_bounds__(kWG) void k_xtw_mfma(const XtwTask<T>* tasks, int count,
-                                                  XtwTask<T> single) {
-  using M = Mma<T>;
+template<int NB>
+__global__ __launch_bounds__(kWG) void k_xtw_bf3(const XtwTask<float>* tasks, int count,
+                                                 XtwTask<float> single) {
   constexpr int NBLK = NB / 16;
-  constexpr int LD = kXtwRows + 4;
-  const XtwTask<T>& t = tasks ? tasks[find_task(tasks, count, blockIdx.x)] : single;
-  const long long local = (long long)blockIdx.x - t.tile0;
-  const long long ctiles = (t.k + kXtwMfmaCols - 1) / kXtwMfmaCols;
-  if (local >= ctiles * t.splits) return;
+  const XtwTask<float> t = load_task(tasks, count, single);
+  const long long ctiles = (t.k + kXtwCols - 1) / kXtwCols;
+  const long long total = ctiles * t.splits;
+  long long local = (long long)blockIdx.x - t.tile0;
+  if (local >= total) return;
+  local = xcd_local(local, total);
   const int split = int(local / ctiles);
-  const long long col0 = (local % ctiles) * kXtwMfmaCols;
-  long long per = (t.m + t.splits - 1) / t.splits;
-  per = (per + kXtwRows - 1) / kXtwRows * kXtwRows;
+  const long long col0 = (local % ctiles) * kXtwCols;
+  const long long per = split_extent(t.m, t.splits, 64);
   const long long r0 = split * per;
   const long long r1 = r0 + per < t.m ? r0 + per : t.m;
+  const long long m64 = (t.m + 63) / 64 * 64;
 
-  __shared__ T bs[2][NB][LD];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // [buffer][hi rows 0..NB-1, lo rows NB..2NB-1][eight 16-byte chunks + pad]
+  __shared__ u32x4 wl[2][2 * NB][9];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int m16 = lane & 15, g = lane >> 4;
-  long long col[2];
+  const bool vec = (t.m & 3) == 0 && (((unsigned long long)t.A) & 15) == 0;
+  const bool stats = t.mode == kShiftStats;
+  // Fast loads: a wave-uniform base (column bc, advanced per step) plus one
+  // 32-bit lane offset per M-block; columns past k re-read column bc.
+  const long long cw = col0 + wave * 32;
+  const long long bc = cw < t.k ? cw : t.k - 1;
+  const char FPLS_GLOBAL* xbase = (const char FPLS_GLOBAL*)(gp(t.A) + bc * t.m);
+  unsigned xoff[2];
+
+  const float FPLS_GLOBAL* xcol[2];
+  float centre[2];
   bool cvalid[2];
-  T centre[2];
-  const T* xcol[2];
 #pragma unroll
   for (int mb = 0; mb < 2; ++mb) {
-    col[mb] = col0 + wave * 32 + mb * 16 + m16;
-    cvalid[mb] = col[mb] < t.k;
-    double c = 0.0;
+    const long long col = col0 + wave * 32 + mb * 16 + m16;
+    cvalid[mb] = col < t.k;
+    xcol[mb] = gp(t.A) + (cvalid[mb] ? col : 0) * t.m;
+    xoff[mb] = unsigned(((cvalid[mb] ? col - bc : 0) * t.m + 4 * g) * 4);
+    float c = 0.f;
     if (cvalid[mb]) {
-      if (t.mode == kCentered && t.center != nullptr) c = t.center[col[mb]];
-      else if (t.mode == kShiftStats) c = double(t.A[col[mb] * t.m]);
+      if (t.mode == kCentered && t.center != nullptr) c = float(gp(t.center)[col]);
+      else if (stats) c = xcol[mb][0];
     }
-    centre[mb] = T(c);
-    xcol[mb] = t.A + (cvalid[mb] ? col[mb] : 0) * t.m;
+    centre[mb] = c;
   }
-  const bool vec_ok = (t.m % (16 / sizeof(T))) == 0;
-  const T* Wt = t.Wt;
+  const char FPLS_GLOBAL* frag = (const char FPLS_GLOBAL*)t.frag;
+  // W chunk c = threadIdx.x + s * kWG of a step: fragment row c / 8, 16 bytes
+  // at 8 (c % 8); row s * kWG / 8 + threadIdx.x / 8 of the [2 NB][m64] array.
+  const unsigned woff = unsigned(((threadIdx.x >> 3) * m64 + 8 * (threadIdx.x & 7)) * 2);
 
-  // stage W rows [i0, i0 + 64) into buffer b (zero outside [r0, r1) or N)
-  auto stage = [&](int b, long long i0) {
-    for (int e = threadIdx.x; e < kXtwRows * NB; e += kWG) {
-      const int r = e % kXtwRows, c = e / kXtwRows;
-      const long long wr = i0 + r;
-      T v = T(0);
-      if (wr < r1 && c < t.N) v = Wt ? Wt[wr + (long long)c * t.ldw] : T(t.W[wr + (long long)c * t.ldw]);
-      bs[b][c][r] = v;
+  auto load_w = [&](u32x4 (&wr)[NBLK], long long i0) {
+#pragma unroll
+    for (int s = 0; s < NBLK; ++s) {
+      const char FPLS_GLOBAL* b = frag + ((long long)s * (kWG / 8) * m64 + i0) * 2;
+      wr[s] = *(const u32x4 FPLS_GLOBAL*)(b + woff);
     }
   };
+  auto store_w = [&](const u32x4 (&wr)[NBLK], int b) {
+#pragma unroll
+    for (int s = 0; s < NBLK; ++s) {
+      const int c = threadIdx.x + s * kWG;
+      wl[b][c >> 3][c & 7] = wr[s];
+    }
+  };
+  // Full 64-row steps of 16-byte-aligned columns: four float4 per M-block.
+  auto load_x = [&](float (&x)[2][16], long long i0) {
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const float4 f = *(const float4 FPLS_GLOBAL*)(xbase + i0 * 4 + 64 * v + xoff[mb]);
+        x[mb][4 * v] = f.x; x[mb][4 * v + 1] = f.y; x[mb][4 * v + 2] = f.z; x[mb][4 * v + 3] = f.w;
+      }
+  };
+  // Any step: rows past r1 and columns past k load the centre, so that they
+  // contribute exactly zero after centring.
+  auto load_x_edge = [&](float (&x)[2][16], long long i0) {
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const long long row = i0 + 16 * v + 4 * g + e;
+          x[mb][4 * v + e] = (cvalid[mb] && row < r1) ? xcol[mb][row] : centre[mb];
+        }
+  };
 
-  typename M::acc_t acc[2][NBLK];
+  f32x4 acc[2][NBLK];
 #pragma unroll
   for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-    for (int nb = 0; nb < NBLK; ++nb) acc[mb][nb] = M::zero();
+    for (int nb = 0; nb < NBLK; ++nb) acc[mb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
   double s1[2] = {0.0, 0.0}, s2[2] = {0.0, 0.0};
 
-  if (r0 < r1) stage(0, r0);
-  __syncthreads();
-  int buf = 0;
-  for (long long i0 = r0; i0 < r1; i0 += kXtwRows) {
-    T a[2][16];
-    const long long rb = i0 + 16 * g;
+  auto compute = [&](const float (&x)[2][16], int b) {
+    float p1[2] = {0.f, 0.f}, p2[2] = {0.f, 0.f};
 #pragma unroll
-    for (int mb = 0; mb < 2; ++mb) {
-      if (cvalid[mb] && vec_ok && rb + 16 <= r1) {
-        if constexpr (sizeof(T) == 4) {
+    for (int h = 0; h < 2; ++h) {
+      u32x4 ah[2], al[2];
 #pragma unroll
-          for (int u = 0; u < 16; u += 4) {
-            const float4 v = *reinterpret_cast<const float4*>(xcol[mb] + rb + u);
-            a[mb][u] = v.x; a[mb][u + 1] = v.y; a[mb][u + 2] = v.z; a[mb][u + 3] = v.w;
-          }
-        } else {
+      for (int mb = 0; mb < 2; ++mb) {
+        float v[8];
 #pragma unroll
-          for (int u = 0; u < 16; u += 2) {
-            const double2 v = *reinterpret_cast<const double2*>(xcol[mb] + rb + u);
-            a[mb][u] = v.x; a[mb][u + 1] = v.y;
-          }
+        for (int j = 0; j < 8; ++j) v[j] = x[mb][8 * h + j] - centre[mb];
+        if (stats) {
+#pragma unroll
+          for (int j = 0; j < 8; ++j) { p1[mb] += v[j]; p2[mb] = fmaf(v[j], v[j], p2[mb]); }
         }
+        split8(v, ah[mb], al[mb]);
+      }
 #pragma unroll
-        for (int u = 0; u < 16; ++u) a[mb][u] -= centre[mb];
-      } else {
+      for (int nb = 0; nb < NBLK; ++nb) {
+        const u32x4 bh = wl[b][nb * 16 + m16][4 * h + g];
+        const u32x4 bl = wl[b][NB + nb * 16 + m16][4 * h + g];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-          const long long row = rb + u;
-          a[mb][u] = (cvalid[mb] && row < r1) ? T(xcol[mb][row] - centre[mb]) : T(0);
-        }
+        for (int mb = 0; mb < 2; ++mb) acc[mb][nb] = mma3(ah[mb], al[mb], bh, bl, acc[mb][nb]);
       }
     }
-    // prefetch the next W tile into the other buffer while this one is used
-    if (i0 + kXtwRows < r1) stage(buf ^ 1, i0 + kXtwRows);
-    if (t.mode == kShiftStats) {
+    if (stats) {
 #pragma unroll
-      for (int mb = 0; mb < 2; ++mb)
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-          const double v = double(a[mb][u]);
-          s1[mb] += v;
-          s2[mb] = fma(v, v, s2[mb]);
-        }
+      for (int mb = 0; mb < 2; ++mb) { s1[mb] += double(p1[mb]); s2[mb] += double(p2[mb]); }
     }
-#pragma unroll
-    for (int nb = 0; nb < NBLK; ++nb) {
-      T b[16];
-#pragma unroll
-      for (int u = 0; u < 16; ++u) b[u] = bs[buf][nb * 16 + m16][16 * g + u];
-#pragma unroll
-      for (int u = 0; u < 16; ++u) {
-        acc[0][nb] = M::mma(a[0][u], b[u], acc[0][nb]);
-        acc[1][nb] = M::mma(a[1][u], b[u], acc[1][nb]);
-      }
-    }
+  };
+
+  // Branch-free two-stage pipeline over the full steps (the last prefetch
+  // repeats a step), LDS buffers alternating with the register buffers.
+  float xa[2][16], xb[2][16];
+  u32x4 wr[NBLK];
+  const long long nfull = vec ? (r1 - r0) / 64 : 0;
+  long long s = 0;
+  if (nfull > 0) {
+    load_w(wr, r0);
+    load_x(xa, r0);
+    store_w(wr, 0);
     __syncthreads();
-    buf ^= 1;
+    for (; s + 1 < nfull; s += 2) {
+      load_w(wr, r0 + 64 * (s + 1));
+      load_x(xb, r0 + 64 * (s + 1));
+      compute(xa, 0);
+      store_w(wr, 1);
+      __syncthreads();
+      const long long nx = r0 + 64 * (s + 2 < nfull ? s + 2 : s + 1);
+      load_w(wr, nx);
+      load_x(xa, nx);
+      compute(xb, 1);
+      store_w(wr, 0);
+      __syncthreads();
+    }
+    if (s < nfull) { compute(xa, 0); ++s; __syncthreads(); }
   }
-  double* o = t.out + (long long)split * t.slab;
+  // the ragged last step (or every step of unaligned columns), unpipelined
+  for (long long i0 = r0 + 64 * s; i0 < r1; i0 += 64) {
+    load_w(wr, i0);
+    store_w(wr, 0);
+    load_x_edge(xa, i0);
+    __syncthreads();
+    compute(xa, 0);
+    __syncthreads();
+  }
+
+  double FPLS_GLOBAL* o = gpw(t.out) + (long long)split * t.slab;
 #pragma unroll
   for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
@@ -349,11 +528,11 @@ __global__ __launch_bounds__(kWG) void k_xtw_mfma(const XtwTask<T>* tasks, int c
       const int n = nb * 16 + m16;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const long long oc = col0 + wave * 32 + mb * 16 + M::row(g, r);
+        const long long oc = col0 + wave * 32 + mb * 16 + 4 * g + r;
         if (oc < t.k && n < t.N) o[oc + (long long)n * t.ldo] = double(acc[mb][nb][r]);
       }
     }
-  if (t.mode == kShiftStats && t.stats != nullptr) {
+  if (stats && t.stats != nullptr) {
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb) {
       double a1 = s1[mb], a2 = s2[mb];
@@ -362,162 +541,391 @@ __global__ __launch_bounds__(kWG) void k_xtw_mfma(const XtwTask<T>* tasks, int c
       a2 += __shfl_xor(a2, 16, 64);
       a2 += __shfl_xor(a2, 32, 64);
       if (g == 0 && cvalid[mb]) {
-        double* st = t.stats + (long long)split * 2 * t.k;
-        st[col[mb]] = a1;
-        st[t.k + col[mb]] = a2;
+        const long long col = col0 + wave * 32 + mb * 16 + m16;
+        double FPLS_GLOBAL* st = gpw(t.stats) + (long long)split * 2 * t.k;
+        st[col] = a1;
+        st[t.k + col] = a2;
       }
     }
   }
 }
 
-// xw on matrix cores. A block owns 256 rows (64 per wave) and one column
-// split; the split is walked in chunks of 64 columns whose W rows are staged
-// in LDS. Within a chunk lane (m, kk) = (lane % 16, lane / 16) loads rows
-// 4m..4m+3 of columns 4s+kk for all 16 K-steps s before the first MFMA, and
-// MFMA call e of step s produces output rows {4m + e}. FP32 accumulators are
-// folded into FP64 ones after every chunk.
-template<class T, int NB>
-__global__ __launch_bounds__(kWG) void k_xw_mfma(const XwTask<T>* tasks, int count,
-                                                 XwTask<T> single) {
-  using M = Mma<T>;
+// ---------------------------------------------------- xw, bf16x3 (FP32 A)
+// A work-group owns 256 rows (64 per wave) and one column split, walked in
+// 32-column reduction steps. Lane (r, g) = (lane % 16, lane / 16) loads rows
+// 4r..4r+3 of columns 8g..8g+7 of the step (eight float4); MFMA call e uses
+// row 4r + e of those columns as the A fragment of output rows {4r' + e}.
+// W fragments (already split, W's own column order) and the FP32 centre come
+// straight from L2; the next step's loads are issued before the current
+// step's arithmetic. No LDS and no barriers.
+template<int NB>
+__global__ __launch_bounds__(kWG) void k_xw_bf3(const XwTask<float>* tasks, int count,
+                                                XwTask<float> single) {
   constexpr int NBLK = NB / 16;
-  constexpr int PAD = 1;
-  const XwTask<T>& t = tasks ? tasks[find_task(tasks, count, blockIdx.x)] : single;
+  const XwTask<float> t = load_task(tasks, count, single);
   const long long local = (long long)blockIdx.x - t.tile0;
   const long long rtiles = xw_row_tiles(t.m);
   if (local >= rtiles * t.splits) return;
   const int split = int(local / rtiles);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int m16 = lane & 15, kk = lane >> 4;
-  const long long i0 = (local % rtiles) * kWG + wave * 64;
-  const long long rowb = i0 + 4 * m16;
-  long long per = (t.k + t.splits - 1) / t.splits;
-  per = (per + 63) / 64 * 64;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int m16 = lane & 15, g = lane >> 4;
+  const long long i0w = (local % rtiles) * kWG + wave * 64;
+  const long long rowb = i0w + 4 * m16;
+  const long long per = split_extent(t.k, t.splits, 32);
   const long long j0 = split * per;
   const long long j1 = j0 + per < t.k ? j0 + per : t.k;
-  const bool vec_ok = (t.m % (16 / sizeof(T))) == 0 && rowb + 4 <= t.m;
-  const bool rows_ok = rowb < t.m;
+  const long long k32 = (t.k + 31) / 32 * 32;
+  const bool vec = (t.m & 3) == 0 && (((unsigned long long)t.A) & 15) == 0;
+  const bool rows_full = i0w + 64 <= t.m;
 
-  __shared__ T ws[64][NB + PAD];
-  __shared__ T cs[64];
-  typename M::acc_t acc[4][NBLK];
-  double acc64[4][NBLK][4];
+  const float FPLS_GLOBAL* xr = gp(t.A) + (rowb < t.m ? rowb : 0);
+  const unsigned short FPLS_GLOBAL* wh = (const unsigned short FPLS_GLOBAL*)t.frag;
+  const unsigned short FPLS_GLOBAL* wlo = wh + (long long)NB * k32;
+  const float FPLS_GLOBAL* cf = (const float FPLS_GLOBAL*)(wh + 2LL * NB * k32);
+  // Fast loads: wave-uniform bases plus one 32-bit lane offset per operand.
+  const long long colbytes = t.m * 4;
+  const char FPLS_GLOBAL* xbase = (const char FPLS_GLOBAL*)(gp(t.A) + i0w);
+  const unsigned xoff = unsigned(8 * g * colbytes + 16 * m16);
+  const char FPLS_GLOBAL* fbase = (const char FPLS_GLOBAL*)t.frag;
+  const unsigned woff = unsigned((m16 * k32 + 8 * g) * 2);
+  const unsigned coff = unsigned(4LL * NB * k32 + 32 * g);  // FP32 centre block, lane's 8
+
+  struct Step {
+    float x[8][4];
+    float c[8];
+    u32x4 bh[NBLK], bl[NBLK];
+  };
+  // centre and W fragments of the step starting at column jc
+  auto load_wc = [&](Step& s, long long jc) {
+    const char FPLS_GLOBAL* cb = fbase + jc * 4;
+    const float4 c0 = *(const float4 FPLS_GLOBAL*)(cb + coff);
+    const float4 c1 = *(const float4 FPLS_GLOBAL*)(cb + coff + 16);
+    s.c[0] = c0.x; s.c[1] = c0.y; s.c[2] = c0.z; s.c[3] = c0.w;
+    s.c[4] = c1.x; s.c[5] = c1.y; s.c[6] = c1.z; s.c[7] = c1.w;
+#pragma unroll
+    for (int nb = 0; nb < NBLK; ++nb) {
+      const char FPLS_GLOBAL* bh = fbase + ((long long)nb * 16 * k32 + jc) * 2;
+      const char FPLS_GLOBAL* bl = bh + (long long)NB * k32 * 2;
+      s.bh[nb] = *(const u32x4 FPLS_GLOBAL*)(bh + woff);
+      s.bl[nb] = *(const u32x4 FPLS_GLOBAL*)(bl + woff);
+    }
+  };
+  // a full 32-column step of 16-byte-aligned rows
+  auto load = [&](Step& s, long long jc) {
+    const char FPLS_GLOBAL* b = xbase + jc * colbytes;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const float4 f = *(const float4 FPLS_GLOBAL*)(b + j * colbytes + xoff);
+      s.x[j][0] = f.x; s.x[j][1] = f.y; s.x[j][2] = f.z; s.x[j][3] = f.w;
+    }
+    load_wc(s, jc);
+  };
+  // any step: columns past j1 and rows past m read as zero
+  auto load_edge = [&](Step& s, long long jc) {
+    const long long cb = jc + 8 * g;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        s.x[j][e] = (cb + j < j1 && rowb + e < t.m) ? xr[(cb + j) * t.m + e] : 0.f;
+    load_wc(s, jc);
+  };
+
+  f32x4 acc[4][NBLK];
 #pragma unroll
   for (int e = 0; e < 4; ++e)
 #pragma unroll
-    for (int nb = 0; nb < NBLK; ++nb) {
-      acc[e][nb] = M::zero();
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc64[e][nb][r] = 0.0;
-    }
+    for (int nb = 0; nb < NBLK; ++nb) acc[e][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  for (long long jc = j0; jc < j1; jc += 64) {
-    const int cnt = int(j1 - jc < 64 ? j1 - jc : 64);
-    __syncthreads();
-    for (int e = threadIdx.x; e < 64 * NB; e += kWG) {
-      const int jj = e & 63, c = e >> 6;
-      T v = T(0);
-      if (jj < cnt && c < t.N)
-        v = t.Wt ? t.Wt[(jc + jj) + (long long)c * t.ldw] : T(t.W[(jc + jj) + (long long)c * t.ldw]);
-      ws[jj][c] = v;
-    }
-    if (threadIdx.x < 64)
-      cs[threadIdx.x] = (t.center != nullptr && int(threadIdx.x) < cnt) ? T(t.center[jc + threadIdx.x]) : T(0);
-    // all 16 K-steps of this chunk in flight at once
-    T x[16][4];
+  auto compute = [&](const Step& s) {
 #pragma unroll
-    for (int s = 0; s < 16; ++s) {
-      const int jj = 4 * s + kk;
-      const T* src = t.A + (jc + jj) * t.m + rowb;
-      if (jj < cnt && vec_ok) {
-        if constexpr (sizeof(T) == 4) {
-          const float4 v = *reinterpret_cast<const float4*>(src);
-          x[s][0] = v.x; x[s][1] = v.y; x[s][2] = v.z; x[s][3] = v.w;
-        } else {
-          const double2 v0 = *reinterpret_cast<const double2*>(src);
-          const double2 v1 = *reinterpret_cast<const double2*>(src + 2);
-          x[s][0] = v0.x; x[s][1] = v0.y; x[s][2] = v1.x; x[s][3] = v1.y;
+    for (int e = 0; e < 4; ++e) {
+      float v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = s.x[j][e] - s.c[j];
+      u32x4 ah, al;
+      split8(v, ah, al);
+#pragma unroll
+      for (int nb = 0; nb < NBLK; ++nb) acc[e][nb] = mma3(ah, al, s.bh[nb], s.bl[nb], acc[e][nb]);
+    }
+  };
+
+  // Branch-free two-stage pipeline over the full steps: the load of the next
+  // step is always issued (the last one repeats a step), so the wait before
+  // each step's arithmetic leaves the next step's loads in flight.
+  Step sa, sb;
+  const long long nfull = (vec && rows_full) ? (j1 - j0) / 32 : 0;
+  long long s = 0;
+  if (nfull > 0) {
+    load(sa, j0);
+    for (; s + 1 < nfull; s += 2) {
+      load(sb, j0 + 32 * (s + 1));
+      compute(sa);
+      load(sa, j0 + 32 * (s + 2 < nfull ? s + 2 : s + 1));
+      compute(sb);
+    }
+    if (s < nfull) { compute(sa); ++s; }
+  }
+  // the ragged last step (or every step of a ragged or unaligned row tile)
+  for (long long jc = j0 + 32 * s; jc < j1; jc += 32) {
+    load_edge(sa, jc);
+    compute(sa);
+  }
+
+  double FPLS_GLOBAL* o = gpw(t.out) + (long long)split * t.slab;
+#pragma unroll
+  for (int nb = 0; nb < NBLK; ++nb) {
+    const int n = nb * 16 + m16;
+    if (n >= t.N) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const long long row = i0w + 16 * g + 4 * r + e;
+        if (row < t.m) o[row + (long long)n * t.ldo] = double(acc[e][nb][r]);
+      }
+  }
+}
+
+// ------------------------------------------------------ xtw, FP64 MFMA
+// Same tiling as the bf16x3 kernel (128 columns, 64-row steps, lane rows
+// 16v + 4g + e); MFMA call (v, e) pairs row 16v + 4g + e of A with the same
+// row of W, which is staged in LDS as FP64. The FP64 matrix cores bound this
+// kernel, so A is single-buffered.
+template<int NB>
+__global__ __launch_bounds__(kWG) void k_xtw_f64(const XtwTask<double>* tasks, int count,
+                                                 XtwTask<double> single) {
+  constexpr int NBLK = NB / 16;
+  constexpr int LD = 64 + 2;
+  const XtwTask<double> t = load_task(tasks, count, single);
+  const long long ctiles = (t.k + kXtwCols - 1) / kXtwCols;
+  const long long total = ctiles * t.splits;
+  long long local = (long long)blockIdx.x - t.tile0;
+  if (local >= total) return;
+  local = xcd_local(local, total);
+  const int split = int(local / ctiles);
+  const long long col0 = (local % ctiles) * kXtwCols;
+  const long long per = split_extent(t.m, t.splits, 64);
+  const long long r0 = split * per;
+  const long long r1 = r0 + per < t.m ? r0 + per : t.m;
+
+  __shared__ double ws[NB][LD];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m16 = lane & 15, g = lane >> 4;
+  const bool vec = (t.m & 1) == 0 && (((unsigned long long)t.A) & 15) == 0;
+  const bool stats = t.mode == kShiftStats;
+  const double FPLS_GLOBAL* W = gp(t.W);
+
+  const double FPLS_GLOBAL* xcol[2];
+  double centre[2];
+  bool cvalid[2];
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb) {
+    const long long col = col0 + wave * 32 + mb * 16 + m16;
+    cvalid[mb] = col < t.k;
+    xcol[mb] = gp(t.A) + (cvalid[mb] ? col : 0) * t.m;
+    double c = 0.0;
+    if (cvalid[mb]) {
+      if (t.mode == kCentered && t.center != nullptr) c = gp(t.center)[col];
+      else if (stats) c = xcol[mb][0];
+    }
+    centre[mb] = c;
+  }
+
+  f64x4 acc[2][NBLK];
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < NBLK; ++nb) acc[mb][nb] = f64x4{0.0, 0.0, 0.0, 0.0};
+  double s1[2] = {0.0, 0.0}, s2[2] = {0.0, 0.0};
+
+  for (long long i0 = r0; i0 < r1; i0 += 64) {
+    double x[2][16];
+    const bool full = vec && i0 + 64 <= r1;
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+      if (full) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const double2 a = *(const double2 FPLS_GLOBAL*)(xcol[mb] + i0 + 16 * v + 4 * g);
+          const double2 b = *(const double2 FPLS_GLOBAL*)(xcol[mb] + i0 + 16 * v + 4 * g + 2);
+          x[mb][4 * v] = a.x; x[mb][4 * v + 1] = a.y; x[mb][4 * v + 2] = b.x; x[mb][4 * v + 3] = b.y;
         }
       } else {
 #pragma unroll
-        for (int e = 0; e < 4; ++e)
-          x[s][e] = (jj < cnt && rows_ok && rowb + e < t.m) ? src[e] : T(0);
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const long long row = i0 + 16 * v + 4 * g + e;
+            x[mb][4 * v + e] = (cvalid[mb] && row < r1) ? xcol[mb][row] : centre[mb];
+          }
       }
     }
     __syncthreads();
-#pragma unroll
-    for (int s = 0; s < 16; ++s) {
-      const int jj = 4 * s + kk;
-      const T c = cs[jj];
-#pragma unroll
-      for (int e = 0; e < 4; ++e) x[s][e] = (jj < cnt && rowb + e < t.m) ? T(x[s][e] - c) : T(0);
-#pragma unroll
-      for (int nb = 0; nb < NBLK; ++nb) {
-        const T b = ws[jj][nb * 16 + m16];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) acc[e][nb] = M::mma(x[s][e], b, acc[e][nb]);
-      }
+    for (int e = threadIdx.x; e < 64 * NB; e += kWG) {
+      const int r = e & 63, n = e >> 6;
+      const long long wr = i0 + r;
+      ws[n][r] = (wr < r1 && n < t.N) ? W[wr + (long long)n * t.ldw] : 0.0;
     }
+    __syncthreads();
 #pragma unroll
-    for (int e = 0; e < 4; ++e)
+    for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-      for (int nb = 0; nb < NBLK; ++nb) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) acc64[e][nb][r] += double(acc[e][nb][r]);
-        acc[e][nb] = M::zero();
+      for (int u = 0; u < 16; ++u) {
+        x[mb][u] -= centre[mb];
+        if (stats) { s1[mb] += x[mb][u]; s2[mb] = fma(x[mb][u], x[mb][u], s2[mb]); }
       }
-  }
-  double* o = t.out + (long long)split * t.slab;
 #pragma unroll
-  for (int e = 0; e < 4; ++e)
+    for (int nb = 0; nb < NBLK; ++nb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const double b = ws[nb * 16 + m16][16 * v + 4 * g + e];
+#pragma unroll
+          for (int mb = 0; mb < 2; ++mb) acc[mb][nb] = mma64(x[mb][4 * v + e], b, acc[mb][nb]);
+        }
+  }
+
+  double FPLS_GLOBAL* o = gpw(t.out) + (long long)split * t.slab;
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
     for (int nb = 0; nb < NBLK; ++nb) {
       const int n = nb * 16 + m16;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const long long row = i0 + 4 * M::row(kk, r) + e;
-        if (row < t.m && n < t.N) o[row + (long long)n * t.ldo] = acc64[e][nb][r];
+        const long long oc = col0 + wave * 32 + mb * 16 + g + 4 * r;
+        if (oc < t.k && n < t.N) o[oc + (long long)n * t.ldo] = acc[mb][nb][r];
       }
     }
+  if (stats && t.stats != nullptr) {
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+      double a1 = s1[mb], a2 = s2[mb];
+      a1 += __shfl_xor(a1, 16, 64);
+      a1 += __shfl_xor(a1, 32, 64);
+      a2 += __shfl_xor(a2, 16, 64);
+      a2 += __shfl_xor(a2, 32, 64);
+      if (g == 0 && cvalid[mb]) {
+        const long long col = col0 + wave * 32 + mb * 16 + m16;
+        double FPLS_GLOBAL* st = gpw(t.stats) + (long long)split * 2 * t.k;
+        st[col] = a1;
+        st[t.k + col] = a2;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------- xw, FP64 MFMA
+// Same tiling as the bf16x3 kernel (256 rows, 32-column steps, lane rows
+// 4r..4r+3 of columns 8g..8g+7); MFMA call (e, j) pairs column 8g + j of row
+// 4r + e with row 8g + j of W, read straight from W's columns in L2.
+template<int NB>
+__global__ __launch_bounds__(kWG) void k_xw_f64(const XwTask<double>* tasks, int count,
+                                                XwTask<double> single) {
+  constexpr int NBLK = NB / 16;
+  const XwTask<double> t = load_task(tasks, count, single);
+  const long long local = (long long)blockIdx.x - t.tile0;
+  const long long rtiles = xw_row_tiles(t.m);
+  if (local >= rtiles * t.splits) return;
+  const int split = int(local / rtiles);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m16 = lane & 15, g = lane >> 4;
+  const long long i0w = (local % rtiles) * kWG + wave * 64;
+  const long long rowb = i0w + 4 * m16;
+  const long long per = split_extent(t.k, t.splits, 32);
+  const long long j0 = split * per;
+  const long long j1 = j0 + per < t.k ? j0 + per : t.k;
+  const bool vec = (t.m & 1) == 0 && (((unsigned long long)t.A) & 15) == 0;
+  const bool rows_full = i0w + 64 <= t.m;
+  const double FPLS_GLOBAL* xr = gp(t.A) + (rowb < t.m ? rowb : 0);
+  const double FPLS_GLOBAL* W = gp(t.W);
+  const double FPLS_GLOBAL* C = t.center ? gp(t.center) : nullptr;
+
+  f64x4 acc[4][NBLK];
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+#pragma unroll
+    for (int nb = 0; nb < NBLK; ++nb) acc[e][nb] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+  for (long long jc = j0; jc < j1; jc += 32) {
+    const long long cb = jc + 8 * g;
+    const bool full = jc + 32 <= j1;
+    double x[8][4], c[8];
+    if (vec && rows_full && full) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const double2 a = *(const double2 FPLS_GLOBAL*)(xr + (cb + j) * t.m);
+        const double2 b = *(const double2 FPLS_GLOBAL*)(xr + (cb + j) * t.m + 2);
+        x[j][0] = a.x; x[j][1] = a.y; x[j][2] = b.x; x[j][3] = b.y;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          x[j][e] = (cb + j < j1 && rowb + e < t.m) ? xr[(cb + j) * t.m + e] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) c[j] = (C != nullptr && cb + j < j1) ? C[cb + j] : 0.0;
+#pragma unroll
+    for (int nb = 0; nb < NBLK; ++nb) {
+      const int n = nb * 16 + m16;
+      double w[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        w[j] = (n < t.N && cb + j < j1) ? W[(cb + j) + (long long)n * t.ldw] : 0.0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[e][nb] = mma64(x[j][e] - c[j], w[j], acc[e][nb]);
+    }
+  }
+
+  double FPLS_GLOBAL* o = gpw(t.out) + (long long)split * t.slab;
+#pragma unroll
+  for (int nb = 0; nb < NBLK; ++nb) {
+    const int n = nb * 16 + m16;
+    if (n >= t.N) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const long long row = i0w + 4 * (g + 4 * r) + e;
+        if (row < t.m) o[row + (long long)n * t.ldo] = acc[e][nb][r];
+      }
+  }
 }
 
 __global__ __launch_bounds__(kWG) void k_sum_slabs(const double* part, int splits,
                                                    long long stride, long long count,
                                                    double* out) {
+  const double FPLS_GLOBAL* P = gp(part);
+  double FPLS_GLOBAL* O = gpw(out);
   for (long long i = (long long)blockIdx.x * kWG + threadIdx.x; i < count;
        i += (long long)gridDim.x * kWG) {
     double s = 0.0;
-    for (int k = 0; k < splits; ++k) s += part[k * stride + i];
-    out[i] = s;
+    for (int k = 0; k < splits; ++k) s += P[k * stride + i];
+    O[i] = s;
   }
 }
 
-template<class T, int NB>
-void run_xw(const XwTask<T>* d, int count, const XwTask<T>& single, long long tiles,
-            hipStream_t stream) {
-  hipLaunchKernelGGL((k_xw<T, NB>), dim3(unsigned(tiles)), dim3(kWG), 0, stream, d, count,
-                     single);
-}
+// ------------------------------------------------------------- dispatch
+// Lane offsets of the bf16x3 kernels are 32-bit byte offsets spanning at most
+// 32 columns of A.
+constexpr long long kMfmaMaxRows = 1LL << 24;
+constexpr unsigned kSplitBlocks = 512;
 
-template<class T, int NB>
-void run_xtw(const XtwTask<T>* d, int count, const XtwTask<T>& single, long long tiles,
-             hipStream_t stream) {
-  hipLaunchKernelGGL((k_xtw<T, NB>), dim3(unsigned(tiles)), dim3(kWG), 0, stream, d, count,
-                     single);
-}
-
-template<class T, int NB>
-void run_xw_mfma(const XwTask<T>* d, int count, const XwTask<T>& single, long long tiles,
-                 hipStream_t stream) {
-  hipLaunchKernelGGL((k_xw_mfma<T, NB>), dim3(unsigned(tiles)), dim3(kWG), 0, stream, d, count,
-                     single);
-}
-
-template<class T, int NB>
-void run_xtw_mfma(const XtwTask<T>* d, int count, const XtwTask<T>& single, long long tiles,
-                  hipStream_t stream) {
-  hipLaunchKernelGGL((k_xtw_mfma<T, NB>), dim3(unsigned(tiles)), dim3(kWG), 0, stream, d,
-                     count, single);
+template<class Fn>
+void with_width(int maxn, Fn&& fn) {
+  switch (mfma_width(maxn)) {
+    case 16: fn(std::integral_constant<int, 16>{}); break;
+    case 32: fn(std::integral_constant<int, 32>{}); break;
+    case 48: fn(std::integral_constant<int, 48>{}); break;
+    default: fn(std::integral_constant<int, 64>{}); break;
+  }
 }
 
 template<class T>
@@ -525,18 +933,26 @@ void dispatch_xw(const XwTask<T>* d, int count, const XwTask<T>& single, long lo
                  int maxn, hipStream_t stream) {
   if (tiles <= 0) return;
   if (tiles > 0x7fffffffLL) throw std::invalid_argument("xw grid too large");
+  const dim3 grid((unsigned)tiles), block((unsigned)kWG);
   if (maxn >= kMfmaMinN) {
-    if (maxn <= 16) run_xw_mfma<T, 16>(d, count, single, tiles, stream);
-    else if (maxn <= 32) run_xw_mfma<T, 32>(d, count, single, tiles, stream);
-    else run_xw_mfma<T, 64>(d, count, single, tiles, stream);
-    FPLS_LAUNCHED("k_xw_mfma");
+    with_width(maxn, [&](auto w) {
+      constexpr int NB = decltype(w)::value;
+      if constexpr (sizeof(T) == 4) {
+        hipLaunchKernelGGL((k_xw_split<NB>), dim3(kSplitBlocks, unsigned(d ? count : 1)), block, 0,
+                           stream, d, single);
+        FPLS_LAUNCHED("k_xw_split");
+        hipLaunchKernelGGL((k_xw_bf3<NB>), grid, block, 0, stream, d, count, single);
+        FPLS_LAUNCHED("k_xw_bf3");
+      } else {
+        hipLaunchKernelGGL((k_xw_f64<NB>), grid, block, 0, stream, d, count, single);
+        FPLS_LAUNCHED("k_xw_f64");
+      }
+    });
     return;
   }
-  if (maxn <= 1) run_xw<T, 1>(d, count, single, tiles, stream);
-  else if (maxn <= 4) run_xw<T, 4>(d, count, single, tiles, stream);
-  else if (maxn <= 16) run_xw<T, 16>(d, count, single, tiles, stream);
-  else if (maxn <= 32) run_xw<T, 32>(d, count, single, tiles, stream);
-  else run_xw<T, 64>(d, count, single, tiles, stream);
+  if (maxn <= 1) hipLaunchKernelGGL((k_xw<T, 1>), grid, block, 0, stream, d, count, single);
+  else if (maxn <= 4) hipLaunchKernelGGL((k_xw<T, 4>), grid, block, 0, stream, d, count, single);
+  else hipLaunchKernelGGL((k_xw<T, 8>), grid, block, 0, stream, d, count, single);
   FPLS_LAUNCHED("k_xw");
 }
 
@@ -545,16 +961,25 @@ void dispatch_xtw(const XtwTask<T>* d, int count, const XtwTask<T>& single, long
                   int maxn, hipStream_t stream) {
   if (tiles <= 0) return;
   if (tiles > 0x7fffffffLL) throw std::invalid_argument("xtw grid too large");
+  const dim3 grid((unsigned)tiles), block((unsigned)kWG);
   if (maxn >= kMfmaMinN) {
-    if (maxn <= 16) run_xtw_mfma<T, 16>(d, count, single, tiles, stream);
-    else if (maxn <= 32) run_xtw_mfma<T, 32>(d, count, single, tiles, stream);
-    else run_xtw_mfma<T, 64>(d, count, single, tiles, stream);
-    FPLS_LAUNCHED("k_xtw_mfma");
+    with_width(maxn, [&](auto w) {
+      constexpr int NB = decltype(w)::value;
+      if constexpr (sizeof(T) == 4) {
+        hipLaunchKernelGGL((k_xtw_split<NB>), dim3(kSplitBlocks, unsigned(d ? count : 1)), block, 0,
+                           stream, d, single);
+        FPLS_LAUNCHED("k_xtw_split");
+        hipLaunchKernelGGL((k_xtw_bf3<NB>), grid, block, 0, stream, d, count, single);
+        FPLS_LAUNCHED("k_xtw_bf3");
+      } else {
+        hipLaunchKernelGGL((k_xtw_f64<NB>), grid, block, 0, stream, d, count, single);
+        FPLS_LAUNCHED("k_xtw_f64");
+      }
+    });
     return;
   }
-  if (maxn <= 4) run_xtw<T, 4>(d, count, single, tiles, stream);
-  else if (maxn <= 16) run_xtw<T, 16>(d, count, single, tiles, stream);
-  else run_xtw<T, 64>(d, count, single, tiles, stream);
+  if (maxn <= 4) hipLaunchKernelGGL((k_xtw<T, 4>), grid, block, 0, stream, d, count, single);
+  else hipLaunchKernelGGL((k_xtw<T, 16>), grid, block, 0, stream, d, count, single);
   FPLS_LAUNCHED("k_xtw");
 }
 
@@ -564,12 +989,16 @@ template<class T>
 long long pack_xw(XwTask<T>* tasks, int count, int* maxn) {
   long long tiles = 0;
   int mn = 0;
+  for (int i = 0; i < count; ++i) mn = std::max(mn, tasks[i].N);
   for (int i = 0; i < count; ++i) {
     if (tasks[i].N < 1 || tasks[i].N > 64) throw std::invalid_argument("xw supports 1..64 right-hand columns");
     if (tasks[i].splits < 1) throw std::invalid_argument("xw needs at least one split");
+    if (xw_frag_bytes<T>(tasks[i].k, mn) > 0 && tasks[i].frag == nullptr)
+      throw std::invalid_argument("xw task without its operand workspace");
+    if (xw_frag_bytes<T>(tasks[i].k, mn) > 0 && tasks[i].m > kMfmaMaxRows)
+      throw std::invalid_argument("matrix-core products support at most 2^24 rows");
     tasks[i].tile0 = tiles;
     tiles += xw_row_tiles(tasks[i].m) * tasks[i].splits;
-    mn = std::max(mn, tasks[i].N);
   }
   if (maxn) *maxn = mn;
   return tiles;
@@ -585,6 +1014,10 @@ long long pack_xtw(XtwTask<T>* tasks, int count, int* maxn) {
   }
   long long tiles = 0;
   for (int i = 0; i < count; ++i) {
+    if (xtw_frag_bytes<T>(tasks[i].m, mn) > 0 && tasks[i].frag == nullptr)
+      throw std::invalid_argument("xtw task without its operand workspace");
+    if (xtw_frag_bytes<T>(tasks[i].m, mn) > 0 && tasks[i].m > kMfmaMaxRows)
+      throw std::invalid_argument("matrix-core products support at most 2^24 rows");
     tasks[i].tile0 = tiles;
     tiles += xtw_tiles(tasks[i].k, mn) * tasks[i].splits;
   }
@@ -596,6 +1029,7 @@ template<class T>
 void launch_xw(const XwTask<T>* device_tasks, int count, long long tiles, int maxn,
                hipStream_t stream) {
   if (count <= 0) return;
+  if (count > 65535) throw std::invalid_argument("xw task list longer than 65535");
   dispatch_xw<T>(device_tasks, count, XwTask<T>{}, tiles, maxn, stream);
 }
 
@@ -603,6 +1037,7 @@ template<class T>
 void launch_xtw(const XtwTask<T>* device_tasks, int count, long long tiles, int maxn,
                 hipStream_t stream) {
   if (count <= 0) return;
+  if (count > 65535) throw std::invalid_argument("xtw task list longer than 65535");
   dispatch_xtw<T>(device_tasks, count, XtwTask<T>{}, tiles, maxn, stream);
 }
 
