@@ -4,15 +4,24 @@ Workload (default): PLS-SVD through the randomized SVD of XᵀY,
 n=100000, p=20000, q=50, 20 components, oversample 10, 2 power iterations,
 rSVD seed 1; latent-factor data (30 factors, column norms 0.8^j, noise 0.05,
 data seed 0) generated directly in HBM. X is FP32 (8 GB) for the mixed path
-and FP64 (16 GB) for the plain path. A step is one complete fit
+(bf16x3 matrix-core products, FP32 accumulation, FP64 small math) and FP64
+(16 GB) for the plain path (FP64 throughout). A step is one complete fit
 (preprocessing, cross-product, randomized SVD, scores, prefix weights) with
 inputs already resident in HBM; value = fits per second of the mixed path.
 
+Accuracy of mixed against plain is reported on identical inputs (the plain
+path's FP64 buffers hold the FP32 values): Ŷ on the first 4096 training rows,
+and B through 4096 random probes x = mu + sigma z (z ~ N(0, I)):
+|Ŷ32(x) - Ŷ64(x)| / |Ŷ64(x) - mu_y| (Frobenius norms).
+
 Prints ONE JSON line on stdout; progress goes to stderr.
+`--calibrate-copy MIB` instead times one device-to-device copy of MIB MiB
+(the reference kernel for scaling rocprof FETCH_SIZE / WRITE_SIZE) and exits.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -30,20 +39,21 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def lib_call(name, *args):
+    from fastpls_amd.resident import _err, _raise, load_library
+    e = _err()
+    if getattr(load_library(), name)(*args, e, 1024) != 0:
+        _raise(e)
+
+
 def gather_rows(buf, n, p, rows, dtype):
     """First `rows` rows of a column-major device matrix, to the host."""
-    import ctypes as C
-    from fastpls_amd.resident import _err, _raise, load_library
-    lib = load_library()
     item = np.dtype(dtype).itemsize
     out = np.empty((rows, p), dtype=dtype, order="F")
     base = out.ctypes.data
-    e = _err()
     for j in range(p):
-        if lib.fastpls_hip_memcpy(C.c_void_p(base + j * rows * item),
-                                  C.c_void_p(buf.ptr + j * n * item), rows * item, 1, e,
-                                  1024) != 0:
-            _raise(e)
+        lib_call("fastpls_hip_memcpy", C.c_void_p(base + j * rows * item),
+                 C.c_void_p(buf.ptr + j * n * item), C.c_size_t(rows * item), 1)
     return out
 
 
@@ -74,14 +84,24 @@ def run_fits(fp, xbuf, ybuf, n, p, q, precision, k, steps, warmup, label):
     return model, np.array(lat), phases
 
 
-def cpu_baseline(rows, p, q, k, seconds_budget):
+def cpu_baseline(rows, p, q, k):
     from oracle.oracle import Oracle, latent_data
     orc = Oracle()
     X, Y = latent_data(rows, p, q, factors=30, noise=0.05, seed=0, decay=0.8)
     t0 = time.perf_counter()
     orc.plssvd_fit(X, Y, ncomp=k, oversample=10, power=2, seed=1)
+    return time.perf_counter() - t0
+
+
+def calibrate_copy(fp, mib):
+    nbytes = int(mib) << 20
+    a, b = fp.DeviceBuffer(nbytes), fp.DeviceBuffer(nbytes)
+    lib_call("fastpls_hip_memcpy", C.c_void_p(b.ptr), C.c_void_p(a.ptr), C.c_size_t(nbytes), 2)
+    t0 = time.perf_counter()
+    lib_call("fastpls_hip_memcpy", C.c_void_p(b.ptr), C.c_void_p(a.ptr), C.c_size_t(nbytes), 2)
     dt = time.perf_counter() - t0
-    return dt
+    print(json.dumps({"copy_bytes": nbytes, "seconds": dt, "GBps_read_plus_write": 2 * nbytes / dt / 1e9}),
+          flush=True)
 
 
 def main():
@@ -97,16 +117,21 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=6000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU; this build has no multi-GPU path "
+        print("bench.py measures one GPU: this build has no multi-GPU path "
               "(independent fits would shard one scheduler per GPU).", file=sys.stderr)
         sys.exit(2)
 
     import fastpls_amd as fp
+    if args.calibrate_copy:
+        calibrate_copy(fp, args.calibrate_copy)
+        return
     info = fp.device_info()
     log(f"device: {info}")
     n, p, q, k = args.n, args.p, args.q, args.k
+    rows = min(4096, n)
 
     # ---------------------------------------------------- mixed (FP32 X)
     log("generating FP32 latent-factor data in HBM")
@@ -116,56 +141,72 @@ def main():
     m32, lat32, ph32 = run_fits(fp, x32, y32, n, p, q, 32, k, args.steps, args.warmup, "mixed")
     wall32 = float(lat32.sum())
     fps = args.steps / wall32
-
-    rows = 4096
     xr32 = gather_rows(x32, n, p, rows, np.float32)
-    yhat32 = m32.predict_path(xr32, [k]).astype(np.float64)[:, :, 0]
-    R32 = m32.export("R").astype(np.float64)
+    x32.free(); y32.free()
 
     # ---------------------------------------------------- plain (FP64 X)
     plain = {}
     if not args.no_plain:
-        x32.free(); y32.free()
-        log("generating FP64 latent-factor data in HBM")
+        # the same data as the mixed path (FP32 values held in FP64), so the
+        # comparison measures the mixed path's arithmetic alone
+        log("generating the same latent-factor data in FP64 storage")
         x64 = fp.DeviceBuffer(n * p * 8)
         y64 = fp.DeviceBuffer(n * q * 8)
         fp.latent_data_device(x64.ptr, y64.ptr, 64, n, p, q, factors=30, decay=0.8, noise=0.05,
-                              seed=0)
+                              seed=0, round32=True)
         m64, lat64, ph64 = run_fits(fp, x64, y64, n, p, q, 64, k, args.plain_steps, 1, "plain")
-        xr64 = gather_rows(x64, n, p, rows, np.float64)
-        yhat64 = m64.predict_path(xr64, [k])[:, :, 0]
-        R64 = m64.export("R")
-        # sign-align components before comparing the weights themselves
-        s = np.sign(np.sum(R32 * R64, axis=0))
+        x64.free(); y64.free()
+        yhat32 = m32.predict_path(xr32, [k]).astype(np.float64)[:, :, 0]
+        yhat64 = m64.predict_path(xr32.astype(np.float64), [k])[:, :, 0]
+        mu, sd, my = m64.export("mX"), m64.export("vX"), m64.export("mY")
+        z = np.random.default_rng(7).standard_normal((rows, p))
+        probe = np.asfortranarray(mu[None, :] + sd[None, :] * z)
+        b32 = m32.predict_path(probe.astype(np.float32), [k]).astype(np.float64)[:, :, 0]
+        b64 = m64.predict_path(probe, [k])[:, :, 0]
         plain = {
             "plain_fits_per_s": round(args.plain_steps / float(lat64.sum()), 4),
             "plain_ms_per_fit": round(1e3 * float(np.median(lat64)), 3),
             "speedup_mixed_vs_plain": round(float(np.median(lat64)) / float(np.median(lat32)), 3),
             "rel_err_yhat_mixed_vs_plain": rel(yhat32, yhat64),
-            "rel_err_R_mixed_vs_plain": rel(R32 * s, R64),
+            "rel_err_B_probe_mixed_vs_plain": float(np.linalg.norm(b32 - b64) /
+                                                    np.linalg.norm(b64 - my[None, :])),
         }
-        m64.close(); x64.free(); y64.free()
+        m64.close()
     m32.close()
 
     # ------------------------------------------------- roofline (X passes)
+    # Each event interval spans the operand split kernel (tens of µs) and the
+    # matrix-core pass that reads X once: algorithmic bytes = n p 4 (X) +
+    # n q 4 (Y) for the cross-product pass and n p 4 + n k 8 for the scores.
     cp = np.array([ph["crossprod_pass"] for ph in ph32])
     sc = np.array([ph["score_pass"] for ph in ph32])
-    xbytes = float(n) * p * 4
-    t_cp = float(np.mean(cp))
-    achieved = xbytes / t_cp
+    cp_bytes = float(n) * p * 4 + float(n) * q * 4
+    sc_bytes = float(n) * p * 4 + float(n) * k * 8
+    t_cp, t_sc = float(np.mean(cp)), float(np.mean(sc))
+    achieved = cp_bytes / t_cp
+    # HBM bytes per launch of the same kernel from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE runs (scaled by a copy of known size), when
+    # they were taken at this workload
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "plssvd_c2_traffic.json")
+    if os.path.exists(tpath):
+        tr = json.load(open(tpath))
+        if tr.get("workload", "").startswith(f"plssvd_rsvd n={n} p={p} q={q} k={k}"):
+            traffic = tr["traffic_bytes"]
     roofline = {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
-                "kernel": "k_xtw_mfma<float,64> (column stats + X^T Y, one pass over X)",
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
+                "algorithmic_bytes": cp_bytes,
+                "kernel": "k_xtw_bf3<64> (column stats + X^T Y, one pass over X, bf16x3 MFMA)",
                 "kernel_ms": round(1e3 * t_cp, 4),
-                "score_pass_kernel": "k_xw_mfma<float,32> (T = X_s R)",
-                "score_pass_ms": round(1e3 * float(np.mean(sc)), 4),
-                "score_pass_frac": round(xbytes / float(np.mean(sc)) / HBM_PEAK, 4)}
+                "score_pass_kernel": "k_xw_bf3<32> (T = X_s R, bf16x3 MFMA)",
+                "score_pass_ms": round(1e3 * t_sc, 4),
+                "score_pass_frac": round(sc_bytes / t_sc / HBM_PEAK, 4)}
 
     # ------------------------------------------------------ CPU baseline
     cpu = None
     if not args.no_cpu:
         log(f"CPU oracle on a {args.cpu_rows}-row sample (1 thread)")
-        dt = cpu_baseline(args.cpu_rows, p, q, k, 30)
+        dt = cpu_baseline(args.cpu_rows, p, q, k)
         t_full = dt * n / args.cpu_rows
         cpu = {"value": round(1.0 / t_full, 6), "unit": "fits/s", "cores": 1, "kind": "port",
                "sample": f"FP64 oracle PLS-SVD fit on {args.cpu_rows} of {n} rows (same p, q, k, "
@@ -181,13 +222,14 @@ def main():
         "ms_per_step": round(1e3 * wall32 / args.steps, 3),
         "p50_ms": round(1e3 * float(np.percentile(lat32, 50)), 3),
         "p99_ms": round(1e3 * float(np.percentile(lat32, 99)), 3),
+        "device_ms_per_fit": round(1e3 * float(np.mean([ph["fit"] for ph in ph32])), 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 storage, f64 accumulation",
+        "dtype": "f32 storage; bf16x3 MFMA with f32 accumulation; f64 small math",
         "data": "synthetic latent-factor data generated in HBM (Philox4x32-10)",
         "config": {"workload": f"plssvd_rsvd n={n} p={p} q={q} k={k} oversample=10 power=2 "
-                               f"(X fp32 {xbytes / 1e9:.1f} GB)",
+                               f"(X fp32 {float(n) * p * 4 / 1e9:.1f} GB)",
                    "n": n, "p": p, "q": q, "components": k},
         "device": info,
         "roofline": roofline,

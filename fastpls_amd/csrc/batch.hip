@@ -1,0 +1,756 @@
+// Grouped execution of many independent SIMPLS fits.
+//
+// The reference fits cross-validation folds one after another on one stream
+// (src/cuda_resident_api.cuh:770-1023) and runs permutation refits on the CPU
+// (R/main.R:9419-9450). Here every phase of every fit in a batch is one
+// launch: the column statistics and cross-products of all fits, the right
+// Grams, then per component step one refresh launch (one work-group per fit),
+// one score pass and one loading pass over every distinct X (fits that share
+// X, e.g. the permutation refits of one fold, form one wider product whose
+// right-hand side holds all their candidates), and one update launch. A fit
+// whose component count is reached simply idles in later steps.
+#include "../../include/fastpls_hip.h"
+#include "host_util.hpp"
+#include "kernels.hpp"
+#include "stream.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <vector>
+
+namespace fpls {
+
+// Row gather: dst (count x cols) = src[idx, :] for a column-major src (n x cols).
+template<class T>
+void launch_gather_rows(const T* src, long long n, long long cols, const int* idx, long long count,
+                        T* dst, hipStream_t s);
+
+namespace {
+
+// A bump allocator over one device allocation (256-byte aligned pieces).
+class Arena {
+ public:
+  size_t add(size_t bytes) {
+    const size_t off = total_;
+    total_ += (bytes + 255) / 256 * 256;
+    return off;
+  }
+  void allocate() { buf_.alloc(total_ ? total_ : 256); }
+  template<class U> U* at(size_t off) const { return reinterpret_cast<U*>(buf_.as<char>() + off); }
+  size_t bytes() const { return total_; }
+
+ private:
+  size_t total_ = 0;
+  Buf buf_;
+};
+
+struct GroupIn {
+  const void* x;
+  int n, p;
+};
+
+struct FitIn {
+  int group, q, A;
+  const void* y;        // device responses (n x q) or null
+  const int* labels;    // device one-based labels or null
+  const int* perm;      // device permutation of response rows or null
+  unsigned long long seed;
+};
+
+template<class T>
+class SimplsBatch {
+ public:
+  SimplsBatch(const std::vector<GroupIn>& groups, const std::vector<FitIn>& fits, int scaling,
+              int oversample, int power)
+      : groups_(groups), fits_(fits), scaling_(scaling), oversample_(oversample), power_(power) {
+    if (groups_.empty() || fits_.empty()) throw std::invalid_argument("empty batch");
+    if (scaling < 1 || scaling > 3) throw std::invalid_argument("scaling must be 1, 2 or 3");
+    if (oversample < 0 || power < 0) throw std::invalid_argument("oversample and power must be non-negative");
+    FPLS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    try {
+      plan();
+      Event start, stop;
+      FPLS_HIP(hipEventRecord(start.e, stream_));
+      run();
+      FPLS_HIP(hipEventRecord(stop.e, stream_));
+      collect();
+      float ms = 0.f;
+      FPLS_HIP(hipEventElapsedTime(&ms, start.e, stop.e));
+      seconds_ = double(ms) * 1e-3;
+    } catch (...) {
+      (void)hipStreamSynchronize(stream_);
+      (void)hipStreamDestroy(stream_);
+      stream_ = nullptr;
+      throw;
+    }
+  }
+
+  ~SimplsBatch() {
+    if (stream_) {
+      (void)hipStreamSynchronize(stream_);
+      (void)hipStreamDestroy(stream_);
+    }
+  }
+
+  int count() const { return int(fits_.size()); }
+  double seconds() const { return seconds_; }
+  int completed(int f) const { return completed_[f]; }
+  int flags(int f) const { return flags_[f]; }
+
+  // Fields as fastpls_hip_export: 0 R, 1 Q, 2 T, 3 x means, 4 x scales, 5 y means.
+  void export_field(int f, int field, void* out, size_t elements) {
+    check_fit(f);
+    const Fit& F = plan_[f];
+    const Group& G = gplan_[fits_[f].group];
+    const size_t counts[] = {(size_t)G.p * F.A, (size_t)F.q * F.A, (size_t)G.n * F.A,
+                             (size_t)G.p, (size_t)G.p, (size_t)F.q};
+    if (field < 0 || field > 5 || !out || elements != counts[field])
+      throw std::invalid_argument("invalid batch field output size");
+    const double* src = field == 0 ? F.R : field == 1 ? F.Q : field == 2 ? F.T
+                      : field == 3 ? G.mx : field == 4 ? G.sx : F.my;
+    std::vector<double> host(elements);
+    FPLS_HIP(hipMemcpy(host.data(), src, sizeof(double) * elements, hipMemcpyDeviceToHost));
+    T* o = static_cast<T*>(out);
+    for (size_t i = 0; i < elements; ++i) o[i] = T(host[i]);
+  }
+
+  // Scores of device rows (rows x p, element T) for fit f: rows x prefix, FP64 (device).
+  void project_device(int f, const T* x, int rows, int prefix, double* scores, Buf& part) {
+    const Fit& F = plan_[f];
+    const Group& G = gplan_[fits_[f].group];
+    XwTask<T> t{};
+    t.A = x; t.m = rows; t.k = G.p; t.center = G.mx; t.W = F.Rs; t.ldw = G.p; t.N = prefix;
+    const int splits = splits_for(xw_row_tiles(rows), G.p, 32);
+    t.splits = splits; t.ldo = rows; t.slab = (long long)rows * prefix;
+    if (splits == 1) {
+      t.out = scores;
+      launch_xw_single<T>(t, stream_);
+    } else {
+      part.reserve(sizeof(double) * (size_t)splits * rows * prefix);
+      t.out = part.d();
+      launch_xw_single<T>(t, stream_);
+      launch_sum_slabs(part.d(), splits, (long long)rows * prefix, (long long)rows * prefix,
+                       scores, stream_);
+    }
+  }
+
+  // Ŷ (rows x q, FP64 device) = scores[:, :k] Q_kᵀ + mean_y of fit f.
+  void predict_from_scores(int f, const double* scores, int rows, int k, double* out) {
+    const Fit& F = plan_[f];
+    launch_apply_small<T>(scores, rows, k, F.Q, F.q, 1, F.q, F.my, nullptr, out, stream_);
+  }
+
+  const double* y_mean(int f) const { return plan_[f].my; }
+  hipStream_t stream() const { return stream_; }
+
+ private:
+  struct Group {
+    const T* X;
+    int n, p, qcat = 0, bcat = 0;
+    double *Yc, *ysum, *S, *S0, *mx, *sx, *colss, *stats, *spart;
+    double *cand, *cand_scaled, *t_raw, *loads;
+    int splits = 1, ls_splits = 1;
+    int* flags;
+  };
+  struct Fit {
+    int q, A, bmax, wmax, nsteps, off, boff;
+    bool right_gram;
+    double *my, *G, *R, *Q, *V, *T, *Rs, *scratch;
+    int* state;
+    int* counts;
+    T* yperm;
+    int* lperm;
+    RefreshStep* steps;
+    double* omega;
+    std::vector<RefreshStep> host_steps;
+    std::vector<double> host_omega;
+  };
+
+  void check_fit(int f) const {
+    if (f < 0 || f >= count()) throw std::invalid_argument("fit index outside the batch");
+  }
+
+  // Sizes, schedules and one arena for everything.
+  void plan() {
+    gplan_.resize(groups_.size());
+    for (size_t g = 0; g < groups_.size(); ++g) {
+      const GroupIn& in = groups_[g];
+      if (!in.x || in.n < 2 || in.p < 1) throw std::invalid_argument("invalid batch group");
+      gplan_[g].X = static_cast<const T*>(in.x);
+      gplan_[g].n = in.n;
+      gplan_[g].p = in.p;
+    }
+    plan_.resize(fits_.size());
+    for (size_t f = 0; f < fits_.size(); ++f) {
+      const FitIn& in = fits_[f];
+      if (in.group < 0 || in.group >= int(groups_.size())) throw std::invalid_argument("fit refers to a missing group");
+      if ((in.y == nullptr) == (in.labels == nullptr)) throw std::invalid_argument("provide responses or labels, not both");
+      Group& G = gplan_[in.group];
+      Fit& F = plan_[f];
+      F.q = in.q;
+      if (F.q < 1 || (in.labels && F.q < 2)) throw std::invalid_argument("invalid response count");
+      if (in.A < 1 || in.A > std::min(G.n - 1, G.p)) throw std::invalid_argument("invalid batch SIMPLS dimensions");
+      F.A = in.A;
+      const int block = candidate_block_size(F.A, G.p, F.q, in.labels != nullptr, G.n, 64);
+      const int maxdim = std::min(G.p, F.q);
+      F.right_gram = F.q <= G.p && F.q <= 512;
+      F.bmax = 1; F.wmax = 1;
+      for (int a = 0; a < F.A;) {
+        const int b = std::min({std::max(block, 1), F.A - a, maxdim});
+        const int w = std::min(maxdim, b + oversample_);
+        RefreshStep st{};
+        st.a = a; st.b = b; st.w = w; st.omega_off = (long long)F.host_omega.size();
+        const size_t base = F.host_omega.size();
+        F.host_omega.resize(base + (size_t)F.q * w);
+        gaussian(unsigned(in.seed) + unsigned(a), (size_t)F.q * w, F.host_omega.data() + base);
+        F.host_steps.push_back(st);
+        F.bmax = std::max(F.bmax, b);
+        F.wmax = std::max(F.wmax, w);
+        a += b;
+      }
+      if (F.wmax > 512) throw std::invalid_argument("sketch width above 512 is not supported");
+      F.nsteps = int(F.host_steps.size());
+      F.off = G.qcat;
+      G.qcat += F.q;
+      F.boff = G.bcat;
+      G.bcat += F.bmax;
+      max_steps_ = std::max(max_steps_, F.nsteps);
+    }
+    // preprocessing splits: one launch over every group
+    long long tiles1 = 0;
+    for (const Group& G : gplan_) tiles1 += xtw_col_tiles(G.p) * ((G.qcat + 63) / 64);
+    for (Group& G : gplan_) {
+      long long s = (2048 + tiles1 - 1) / std::max<long long>(tiles1, 1);
+      s = std::max<long long>(1, std::min<long long>(s, G.n / 256));
+      s = std::max<long long>(s, (G.n + kXtwSplitRows - 1) / kXtwSplitRows);
+      G.splits = int(std::min<long long>(s, 4096));
+    }
+    for (Group& G : gplan_) G.ls_splits = G.bcat >= kMfmaMinN ? int((G.n + kXtwSplitRows - 1) / kXtwSplitRows) : 1;
+    // arena layout
+    std::vector<size_t> go(gplan_.size() * 15), fo(plan_.size() * 14);
+    for (size_t g = 0; g < gplan_.size(); ++g) {
+      const Group& G = gplan_[g];
+      const size_t n = G.n, p = G.p, Q = G.qcat, B = G.bcat;
+      size_t* o = &go[g * 15];
+      o[0] = arena_.add(8 * n * Q);                    // Yc
+      o[1] = arena_.add(8 * Q);                        // ysum
+      o[2] = arena_.add(8 * p * Q);                    // S
+      o[3] = arena_.add(8 * p * Q);                    // S0
+      o[4] = arena_.add(8 * p);                        // mx
+      o[5] = arena_.add(8 * p);                        // sx
+      o[6] = arena_.add(8 * p);                        // colss
+      o[7] = arena_.add(8 * (size_t)G.splits * 2 * p); // stats
+      o[8] = arena_.add(8 * (size_t)G.splits * p * Q); // spart
+      o[9] = arena_.add(8 * p * B);                    // cand
+      o[10] = arena_.add(8 * p * B);                   // cand_scaled
+      o[11] = arena_.add(8 * n * B);                   // t_raw
+      o[12] = arena_.add(8 * (size_t)G.ls_splits * p * B);  // loads (split slabs)
+      o[13] = arena_.add(sizeof(int) * 4);             // flags
+    }
+    for (size_t f = 0; f < plan_.size(); ++f) {
+      const Fit& F = plan_[f];
+      const Group& G = gplan_[fits_[f].group];
+      const size_t n = G.n, p = G.p, q = F.q, A = F.A;
+      size_t* o = &fo[f * 14];
+      o[0] = arena_.add(8 * q);                                   // my
+      o[1] = arena_.add(8 * q * q);                               // G
+      o[2] = arena_.add(8 * p * A);                               // R
+      o[3] = arena_.add(8 * q * A);                               // Q
+      o[4] = arena_.add(8 * p * A);                               // V
+      o[5] = arena_.add(8 * n * A);                               // T
+      o[6] = arena_.add(8 * p * A);                               // Rs
+      o[7] = arena_.add(8 * (size_t)simpls_scratch_doubles(G.p, F.q, F.wmax));
+      o[8] = arena_.add(sizeof(int) * 4);                         // state
+      o[9] = arena_.add(sizeof(int) * q);                         // counts
+      o[10] = fits_[f].perm ? arena_.add(sizeof(T) * n * q) : 0;  // permuted responses
+      o[11] = fits_[f].perm ? arena_.add(sizeof(int) * n) : 0;    // permuted labels
+      o[12] = arena_.add(sizeof(RefreshStep) * F.host_steps.size());
+      o[13] = arena_.add(8 * F.host_omega.size());
+    }
+    devfits_off_ = arena_.add(sizeof(SimplsFitDev) * plan_.size());
+    arena_.allocate();
+    for (size_t g = 0; g < gplan_.size(); ++g) {
+      Group& G = gplan_[g];
+      const size_t* o = &go[g * 15];
+      G.Yc = arena_.at<double>(o[0]); G.ysum = arena_.at<double>(o[1]);
+      G.S = arena_.at<double>(o[2]); G.S0 = arena_.at<double>(o[3]);
+      G.mx = arena_.at<double>(o[4]); G.sx = arena_.at<double>(o[5]);
+      G.colss = arena_.at<double>(o[6]); G.stats = arena_.at<double>(o[7]);
+      G.spart = arena_.at<double>(o[8]); G.cand = arena_.at<double>(o[9]);
+      G.cand_scaled = arena_.at<double>(o[10]); G.t_raw = arena_.at<double>(o[11]);
+      G.loads = arena_.at<double>(o[12]); G.flags = arena_.at<int>(o[13]);
+    }
+    std::vector<SimplsFitDev> dev(plan_.size());
+    for (size_t f = 0; f < plan_.size(); ++f) {
+      Fit& F = plan_[f];
+      Group& G = gplan_[fits_[f].group];
+      const size_t* o = &fo[f * 14];
+      F.my = arena_.at<double>(o[0]); F.G = arena_.at<double>(o[1]);
+      F.R = arena_.at<double>(o[2]); F.Q = arena_.at<double>(o[3]);
+      F.V = arena_.at<double>(o[4]); F.T = arena_.at<double>(o[5]);
+      F.Rs = arena_.at<double>(o[6]); F.scratch = arena_.at<double>(o[7]);
+      F.state = arena_.at<int>(o[8]); F.counts = arena_.at<int>(o[9]);
+      F.yperm = fits_[f].perm ? arena_.at<T>(o[10]) : nullptr;
+      F.lperm = fits_[f].perm ? arena_.at<int>(o[11]) : nullptr;
+      F.steps = arena_.at<RefreshStep>(o[12]);
+      F.omega = arena_.at<double>(o[13]);
+      FPLS_HIP(hipMemcpy(F.steps, F.host_steps.data(), sizeof(RefreshStep) * F.host_steps.size(),
+                         hipMemcpyHostToDevice));
+      FPLS_HIP(hipMemcpy(F.omega, F.host_omega.data(), sizeof(double) * F.host_omega.size(),
+                         hipMemcpyHostToDevice));
+      SimplsFitDev& d = dev[f];
+      d.n = G.n; d.p = G.p; d.q = F.q; d.A = F.A;
+      d.power = power_; d.use_right_gram = F.right_gram ? 1 : 0; d.reorth = 0;
+      d.nsteps = F.nsteps; d.bmax = F.bmax; d.ts_splits = 1; d.ls_splits = G.ls_splits;
+      d.ls_slab = (long long)G.p * G.bcat;
+      d.steps = F.steps; d.omega = F.omega;
+      d.S = G.S + (size_t)F.off * G.p; d.S0 = G.S0 + (size_t)F.off * G.p;
+      d.G = F.right_gram ? F.G : nullptr; d.sx = G.sx;
+      d.cand = G.cand + (size_t)F.boff * G.p;
+      d.cand_scaled = G.cand_scaled + (size_t)F.boff * G.p;
+      d.t_raw = G.t_raw + (size_t)F.boff * G.n;
+      d.ts_part = nullptr;
+      d.ls_part = G.loads + (size_t)F.boff * G.p;
+      d.R = F.R; d.Q = F.Q; d.V = F.V; d.T = F.T;
+      d.scratch = F.scratch; d.state = F.state;
+    }
+    devfits_ = arena_.at<SimplsFitDev>(devfits_off_);
+    FPLS_HIP(hipMemcpy(devfits_, dev.data(), sizeof(SimplsFitDev) * dev.size(), hipMemcpyHostToDevice));
+    // zero states and flags
+    std::vector<int> zeros(4, 0);
+    for (Fit& F : plan_) FPLS_HIP(hipMemcpy(F.state, zeros.data(), sizeof(int) * 4, hipMemcpyHostToDevice));
+    for (Group& G : gplan_) FPLS_HIP(hipMemcpy(G.flags, zeros.data(), sizeof(int) * 4, hipMemcpyHostToDevice));
+    // task lists of the per-step passes (identical for every step)
+    std::vector<XwTask<T>> xw;
+    std::vector<XtwTask<T>> xt;
+    for (Group& G : gplan_) {
+      for (int c0 = 0; c0 < G.bcat; c0 += 64) {
+        const int nb = std::min(64, G.bcat - c0);
+        XwTask<T> a{};
+        a.A = G.X; a.m = G.n; a.k = G.p; a.center = G.mx;
+        a.W = G.cand_scaled + (size_t)c0 * G.p; a.ldw = G.p; a.N = nb; a.splits = 1;
+        a.out = G.t_raw + (size_t)c0 * G.n; a.ldo = G.n; a.slab = 0;
+        xw.push_back(a);
+        XtwTask<T> b{};
+        b.A = G.X; b.m = G.n; b.k = G.p; b.center = G.mx; b.mode = kCentered;
+        b.W = G.t_raw + (size_t)c0 * G.n; b.ldw = G.n; b.N = nb; b.splits = G.ls_splits;
+        b.out = G.loads + (size_t)c0 * G.p; b.ldo = G.p; b.slab = (long long)G.p * G.bcat;
+        xt.push_back(b);
+      }
+    }
+    xw_tiles_ = pack_xw(xw.data(), int(xw.size()), &xw_maxn_);
+    xt_tiles_ = pack_xtw(xt.data(), int(xt.size()), &xt_maxn_);
+    xw_count_ = int(xw.size());
+    xt_count_ = int(xt.size());
+    xw_tasks_.alloc(sizeof(XwTask<T>) * xw.size());
+    xt_tasks_.alloc(sizeof(XtwTask<T>) * xt.size());
+    FPLS_HIP(hipMemcpy(xw_tasks_.as<XwTask<T>>(), xw.data(), sizeof(XwTask<T>) * xw.size(), hipMemcpyHostToDevice));
+    FPLS_HIP(hipMemcpy(xt_tasks_.as<XtwTask<T>>(), xt.data(), sizeof(XtwTask<T>) * xt.size(), hipMemcpyHostToDevice));
+  }
+
+  void run() {
+    // responses of every fit into its columns of the group's centred block
+    for (size_t f = 0; f < plan_.size(); ++f) {
+      Fit& F = plan_[f];
+      Group& G = gplan_[fits_[f].group];
+      const FitIn& in = fits_[f];
+      double* yc = G.Yc + (size_t)F.off * G.n;
+      if (in.labels) {
+        const int* lab = in.labels;
+        if (in.perm) {
+          launch_gather_rows<int>(in.labels, G.n, 1, in.perm, G.n, F.lperm, stream_);
+          lab = F.lperm;
+        }
+        launch_label_response(lab, G.n, F.q, F.counts, F.my, yc, nullptr, G.ysum + F.off,
+                              F.state + 2, stream_);
+      } else {
+        const T* y = static_cast<const T*>(in.y);
+        if (in.perm) {
+          launch_gather_rows<T>(y, G.n, F.q, in.perm, G.n, F.yperm, stream_);
+          y = F.yperm;
+        }
+        launch_center_response<T>(y, G.n, F.q, F.my, yc, G.ysum + F.off, stream_);
+      }
+    }
+    // one launch: column statistics + cross-products of every group
+    std::vector<XtwTask<T>> pre;
+    for (Group& G : gplan_) {
+      for (int c0 = 0; c0 < G.qcat; c0 += 64) {
+        const int nb = std::min(64, G.qcat - c0);
+        XtwTask<T> t{};
+        t.A = G.X; t.m = G.n; t.k = G.p; t.W = G.Yc + (size_t)c0 * G.n; t.ldw = G.n; t.N = nb;
+        t.mode = kShiftStats; t.splits = G.splits;
+        t.out = G.spart + (size_t)c0 * G.p; t.ldo = G.p; t.slab = (long long)G.p * G.qcat;
+        t.stats = c0 == 0 ? G.stats : nullptr;
+        pre.push_back(t);
+      }
+    }
+    int maxn = 0;
+    const long long tiles = pack_xtw(pre.data(), int(pre.size()), &maxn);
+    Buf pre_dev;
+    pre_dev.alloc(sizeof(XtwTask<T>) * pre.size());
+    FPLS_HIP(hipMemcpy(pre_dev.as<XtwTask<T>>(), pre.data(), sizeof(XtwTask<T>) * pre.size(), hipMemcpyHostToDevice));
+    launch_xtw<T>(pre_dev.as<XtwTask<T>>(), int(pre.size()), tiles, maxn, stream_);
+    for (Group& G : gplan_)
+      launch_finalize_crossprod<T>(G.X, G.n, G.p, G.qcat, scaling_, G.stats, G.spart, G.splits,
+                                   G.ysum, G.mx, G.sx, G.colss, G.S, G.S0, G.flags, stream_);
+    // right Grams SᵀS of every fit: one launch
+    std::vector<XtwTask<double>> gram;
+    for (size_t f = 0; f < plan_.size(); ++f) {
+      Fit& F = plan_[f];
+      if (!F.right_gram) continue;
+      Group& G = gplan_[fits_[f].group];
+      const double* S = G.S + (size_t)F.off * G.p;
+      for (int c0 = 0; c0 < F.q; c0 += 64) {
+        const int nb = std::min(64, F.q - c0);
+        XtwTask<double> t{};
+        t.A = S; t.m = G.p; t.k = F.q; t.W = S + (size_t)c0 * G.p; t.ldw = G.p; t.N = nb;
+        t.mode = kPlain; t.splits = 1; t.out = F.G + (size_t)c0 * F.q; t.ldo = F.q; t.slab = 0;
+        gram.push_back(t);
+      }
+    }
+    Buf gram_dev;
+    if (!gram.empty()) {
+      const long long gt = pack_xtw(gram.data(), int(gram.size()), &maxn);
+      gram_dev.alloc(sizeof(XtwTask<double>) * gram.size());
+      FPLS_HIP(hipMemcpy(gram_dev.as<XtwTask<double>>(), gram.data(), sizeof(XtwTask<double>) * gram.size(), hipMemcpyHostToDevice));
+      launch_xtw<double>(gram_dev.as<XtwTask<double>>(), int(gram.size()), gt, maxn, stream_);
+    }
+    // component steps: four launches per step for the whole batch
+    const int nf = int(plan_.size());
+    for (int s = 0; s < max_steps_; ++s) {
+      launch_simpls_refresh(devfits_, nf, s, stream_);
+      launch_xw<T>(xw_tasks_.as<XwTask<T>>(), xw_count_, xw_tiles_, xw_maxn_, stream_);
+      launch_xtw<T>(xt_tasks_.as<XtwTask<T>>(), xt_count_, xt_tiles_, xt_maxn_, stream_);
+      launch_simpls_update(devfits_, nf, s, stream_);
+    }
+    for (size_t f = 0; f < plan_.size(); ++f) {
+      const Fit& F = plan_[f];
+      const Group& G = gplan_[fits_[f].group];
+      launch_div_rows(F.R, G.sx, G.p, F.A, F.Rs, stream_);
+    }
+    FPLS_HIP(hipStreamSynchronize(stream_));
+  }
+
+  void collect() {
+    completed_.assign(plan_.size(), 0);
+    flags_.assign(plan_.size(), 0);
+    for (size_t f = 0; f < plan_.size(); ++f) {
+      int st[4];
+      int gf[4];
+      FPLS_HIP(hipMemcpy(st, plan_[f].state, sizeof(st), hipMemcpyDeviceToHost));
+      FPLS_HIP(hipMemcpy(gf, gplan_[fits_[f].group].flags, sizeof(gf), hipMemcpyDeviceToHost));
+      completed_[f] = st[0];
+      flags_[f] = st[2] | gf[0];
+    }
+  }
+
+  std::vector<GroupIn> groups_;
+  std::vector<FitIn> fits_;
+  std::vector<Group> gplan_;
+  std::vector<Fit> plan_;
+  int scaling_, oversample_, power_;
+  int max_steps_ = 0;
+  Arena arena_;
+  size_t devfits_off_ = 0;
+  SimplsFitDev* devfits_ = nullptr;
+  Buf xw_tasks_, xt_tasks_;
+  long long xw_tiles_ = 0, xt_tiles_ = 0;
+  int xw_maxn_ = 0, xt_maxn_ = 0, xw_count_ = 0, xt_count_ = 0;
+  hipStream_t stream_ = nullptr;
+  double seconds_ = 0.0;
+  std::vector<int> completed_, flags_;
+};
+
+class BatchHandle {
+ public:
+  virtual ~BatchHandle() = default;
+  virtual int count() const = 0;
+  virtual double seconds() const = 0;
+  virtual int completed(int f) const = 0;
+  virtual int flags(int f) const = 0;
+  virtual void export_field(int f, int field, void* out, size_t elements) = 0;
+};
+
+template<class T>
+class TypedBatch final : public BatchHandle {
+ public:
+  explicit TypedBatch(std::unique_ptr<SimplsBatch<T>> b) : b_(std::move(b)) {}
+  int count() const override { return b_->count(); }
+  double seconds() const override { return b_->seconds(); }
+  int completed(int f) const override { return b_->completed(f); }
+  int flags(int f) const override { return b_->flags(f); }
+  void export_field(int f, int field, void* out, size_t elements) override {
+    b_->export_field(f, field, out, elements);
+  }
+
+ private:
+  std::unique_ptr<SimplsBatch<T>> b_;
+};
+
+void write_err(char* out, size_t cap, const char* message) noexcept {
+  if (out && cap) std::snprintf(out, cap, "%s", message);
+}
+
+template<class F>
+int guard(char* err, size_t cap, F&& f) noexcept {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    write_err(err, cap, e.what());
+  } catch (...) {
+    write_err(err, cap, "unknown fastPLS HIP failure");
+  }
+  return 1;
+}
+
+void need_device() {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
+    throw std::runtime_error("no HIP device is available (the HIP backend has no CPU fallback)");
+}
+
+// ------------------------------------------------------------------- CV
+// src/cuda_resident_api.cuh:770-1023: regression CV with per-fold fits, test
+// predictions for every prefix, SSE / training-mean SS per fold and the
+// metric ladder (2 = observed R², 3 = Q², 4 = RMSD). All folds are fitted
+// in one batch instead of one after another.
+template<class T>
+void simpls_cv_regression(const T* predictors, const T* responses, const int* folds, int n, int p,
+                          int q, const int* prefixes, int prefix_count, int scaling, int metric,
+                          int oversample, int power, unsigned long long seed,
+                          bool store_predictions, double* prediction_output, int* status,
+                          double* metrics, double* q2_values, double* rmsd_values,
+                          double* observed_r2_values) {
+  if (!predictors || !responses || !folds || !prefixes || !status || !metrics || n < 2 || p < 1 ||
+      q < 1 || prefix_count < 1 || scaling < 1 || scaling > 3 || metric < 2 || metric > 4)
+    throw std::invalid_argument("invalid resident HIP regression CV input");
+  int maxp = 0, fold_count = 0;
+  for (int j = 0; j < prefix_count; ++j) {
+    if (prefixes[j] <= maxp) throw std::invalid_argument("HIP CV component path must be strictly increasing");
+    maxp = prefixes[j];
+  }
+  for (int r = 0; r < n; ++r) {
+    if (folds[r] < 1) throw std::invalid_argument("invalid HIP CV fold");
+    fold_count = std::max(fold_count, folds[r]);
+  }
+  if (maxp > std::min(n - 1, p) || (store_predictions && !prediction_output))
+    throw std::invalid_argument("invalid HIP regression CV output or rank");
+  std::vector<std::vector<int>> test_rows(fold_count), train_rows(fold_count);
+  for (int r = 0; r < n; ++r) test_rows[folds[r] - 1].push_back(r);
+  for (int f = 0; f < fold_count; ++f) {
+    std::vector<char> held(n, 0);
+    for (int r : test_rows[f]) held[r] = 1;
+    for (int r = 0; r < n; ++r) if (!held[r]) train_rows[f].push_back(r);
+  }
+  long double total_ss = 0.0L;
+  for (int c = 0; c < q; ++c) {
+    long double s = 0.0L, s2 = 0.0L;
+    for (int r = 0; r < n; ++r) {
+      const long double v = responses[(size_t)c * n + r];
+      s += v; s2 += v * v;
+    }
+    total_ss += s2 - s * s / n;
+  }
+  hipStream_t stream = nullptr;
+  FPLS_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() { if (s) { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); } }
+  } guard_stream{stream};
+  Buf xfull, yfull, idx;
+  xfull.alloc(sizeof(T) * (size_t)n * p);
+  yfull.alloc(sizeof(T) * (size_t)n * q);
+  FPLS_HIP(hipMemcpy(xfull.as<T>(), predictors, sizeof(T) * (size_t)n * p, hipMemcpyHostToDevice));
+  FPLS_HIP(hipMemcpy(yfull.as<T>(), responses, sizeof(T) * (size_t)n * q, hipMemcpyHostToDevice));
+  std::vector<int> all_idx;
+  std::vector<size_t> tr_off(fold_count), te_off(fold_count);
+  for (int f = 0; f < fold_count; ++f) {
+    tr_off[f] = all_idx.size();
+    all_idx.insert(all_idx.end(), train_rows[f].begin(), train_rows[f].end());
+    te_off[f] = all_idx.size();
+    all_idx.insert(all_idx.end(), test_rows[f].begin(), test_rows[f].end());
+  }
+  idx.alloc(sizeof(int) * all_idx.size());
+  FPLS_HIP(hipMemcpy(idx.as<int>(), all_idx.data(), sizeof(int) * all_idx.size(), hipMemcpyHostToDevice));
+  std::vector<std::unique_ptr<Buf>> xtr(fold_count), ytr(fold_count), xte(fold_count), yte(fold_count);
+  std::vector<GroupIn> groups;
+  std::vector<FitIn> fits;
+  std::vector<int> fold_of;
+  for (int f = 0; f < fold_count; ++f) {
+    const int ntr = int(train_rows[f].size()), nte = int(test_rows[f].size());
+    if (nte == 0) { status[f] = 2; continue; }
+    if (ntr < 2) throw std::invalid_argument("HIP CV contains an empty train or test fold");
+    if (maxp > std::min(ntr - 1, p)) throw std::invalid_argument("HIP CV component count exceeds a fold rank");
+    xtr[f].reset(new Buf); ytr[f].reset(new Buf); xte[f].reset(new Buf); yte[f].reset(new Buf);
+    xtr[f]->alloc(sizeof(T) * (size_t)ntr * p);
+    ytr[f]->alloc(sizeof(T) * (size_t)ntr * q);
+    xte[f]->alloc(sizeof(T) * (size_t)nte * p);
+    yte[f]->alloc(sizeof(T) * (size_t)nte * q);
+    launch_gather_rows<T>(xfull.as<T>(), n, p, idx.as<int>() + tr_off[f], ntr, xtr[f]->as<T>(), stream);
+    launch_gather_rows<T>(yfull.as<T>(), n, q, idx.as<int>() + tr_off[f], ntr, ytr[f]->as<T>(), stream);
+    launch_gather_rows<T>(xfull.as<T>(), n, p, idx.as<int>() + te_off[f], nte, xte[f]->as<T>(), stream);
+    launch_gather_rows<T>(yfull.as<T>(), n, q, idx.as<int>() + te_off[f], nte, yte[f]->as<T>(), stream);
+    groups.push_back({xtr[f]->as<T>(), ntr, p});
+    FitIn fi{};
+    fi.group = int(groups.size()) - 1; fi.q = q; fi.A = maxp; fi.y = ytr[f]->as<T>();
+    fi.labels = nullptr; fi.perm = nullptr; fi.seed = seed;  // SIMPLS folds share the seed
+    fits.push_back(fi);
+    fold_of.push_back(f);
+  }
+  FPLS_HIP(hipStreamSynchronize(stream));
+  if (fits.empty()) throw std::invalid_argument("HIP CV has no non-empty fold");
+  SimplsBatch<T> batch(groups, fits, scaling, oversample, power);
+  for (int i = 0; i < batch.count(); ++i)
+    if (batch.completed(i) < maxp || batch.flags(i))
+      throw std::runtime_error("resident SIMPLS failed: invalid labels, decomposition or component norm");
+  std::fill(metrics, metrics + prefix_count, 0.0);
+  std::vector<double> counts(prefix_count, 0.0), fold_training_ss(prefix_count, 0.0);
+  Buf scores, part, pred, sums, flag;
+  sums.alloc(sizeof(double) * 3 * q);
+  flag.alloc(sizeof(int));
+  hipStream_t bs = batch.stream();
+  for (int i = 0; i < batch.count(); ++i) {
+    const int f = fold_of[i];
+    const int nte = int(test_rows[f].size());
+    scores.reserve(sizeof(double) * (size_t)nte * maxp);
+    pred.reserve(sizeof(double) * (size_t)nte * q);
+    batch.project_device(i, xte[f]->as<T>(), nte, maxp, scores.d(), part);
+    std::vector<double> host_metric(3 * (size_t)q * prefix_count);
+    std::vector<double> host_pred(store_predictions ? (size_t)nte * q * prefix_count : 0);
+    for (int j = 0; j < prefix_count; ++j) {
+      batch.predict_from_scores(i, scores.d(), nte, prefixes[j], pred.d());
+      if (store_predictions)
+        FPLS_HIP(hipMemcpyAsync(host_pred.data() + (size_t)j * nte * q, pred.d(),
+                                sizeof(double) * (size_t)nte * q, hipMemcpyDeviceToHost, bs));
+      const int zero = 0;
+      FPLS_HIP(hipMemcpyAsync(flag.as<int>(), &zero, sizeof(int), hipMemcpyHostToDevice, bs));
+      launch_response_sums<T>(pred.d(), yte[f]->as<T>(), nullptr, batch.y_mean(i), nte, q,
+                              sums.d(), flag.as<int>(), bs);
+      FPLS_HIP(hipMemcpyAsync(host_metric.data() + (size_t)j * 3 * q, sums.d(),
+                              sizeof(double) * 3 * q, hipMemcpyDeviceToHost, bs));
+      FPLS_HIP(hipStreamSynchronize(bs));
+    }
+    for (int j = 0; j < prefix_count; ++j) {
+      long double sse = 0.0L, tss = 0.0L;
+      for (int c = 0; c < q; ++c) {
+        sse += host_metric[(size_t)j * 3 * q + (size_t)c * 3];
+        tss += host_metric[(size_t)j * 3 * q + (size_t)c * 3 + 1];
+      }
+      metrics[j] += double(sse);
+      fold_training_ss[j] += double(tss);
+      counts[j] += double(nte) * q;
+      if (store_predictions)
+        for (int c = 0; c < q; ++c)
+          for (int r = 0; r < nte; ++r)
+            prediction_output[(size_t)j * n * q + (size_t)c * n + test_rows[f][r]] =
+                host_pred[(size_t)j * nte * q + (size_t)c * nte + r];
+    }
+    status[f] = 1;
+  }
+  for (int j = 0; j < prefix_count; ++j) {
+    const double sse = metrics[j];
+    rmsd_values[j] = counts[j] > 0.0 ? std::sqrt(sse / counts[j]) : std::numeric_limits<double>::quiet_NaN();
+    q2_values[j] = fold_training_ss[j] > 0.0 ? 1.0 - sse / fold_training_ss[j] : std::numeric_limits<double>::quiet_NaN();
+    observed_r2_values[j] = total_ss > 0.0L ? 1.0 - sse / double(total_ss) : std::numeric_limits<double>::quiet_NaN();
+    metrics[j] = metric == 4 ? rmsd_values[j] : metric == 3 ? q2_values[j] : observed_r2_values[j];
+  }
+}
+
+}  // namespace
+}  // namespace fpls
+
+using fpls::BatchHandle;
+
+extern "C" {
+
+void* fastpls_hip_simpls_batch_create(int precision, int group_count, const void* const* x_device,
+                                      const int* n, const int* p, int fit_count,
+                                      const int* group, const void* const* y_device,
+                                      const int* const* labels_device,
+                                      const int* const* perm_device, const int* q,
+                                      const int* components, const unsigned long long* seeds,
+                                      int scaling, int oversample, int power, char* error,
+                                      size_t error_capacity) {
+  BatchHandle* h = nullptr;
+  fpls::guard(error, error_capacity, [&] {
+    fpls::need_device();
+    if (group_count < 1 || fit_count < 1 || !x_device || !n || !p || !group || !q ||
+        !components || !seeds)
+      throw std::invalid_argument("invalid batch description");
+    std::vector<fpls::GroupIn> groups(group_count);
+    for (int g = 0; g < group_count; ++g) groups[g] = {x_device[g], n[g], p[g]};
+    std::vector<fpls::FitIn> fits(fit_count);
+    for (int f = 0; f < fit_count; ++f) {
+      fits[f].group = group[f];
+      fits[f].q = q[f];
+      fits[f].A = components[f];
+      fits[f].y = y_device ? y_device[f] : nullptr;
+      fits[f].labels = labels_device ? labels_device[f] : nullptr;
+      fits[f].perm = perm_device ? perm_device[f] : nullptr;
+      fits[f].seed = seeds[f];
+    }
+    if (precision == 32)
+      h = new fpls::TypedBatch<float>(std::make_unique<fpls::SimplsBatch<float>>(groups, fits, scaling, oversample, power));
+    else if (precision == 64)
+      h = new fpls::TypedBatch<double>(std::make_unique<fpls::SimplsBatch<double>>(groups, fits, scaling, oversample, power));
+    else
+      throw std::invalid_argument("precision must be 32 or 64");
+  });
+  return h;
+}
+
+int fastpls_hip_batch_status(void* batch, int* completed, int* flags, double* seconds,
+                             char* error, size_t error_capacity) {
+  return fpls::guard(error, error_capacity, [&] {
+    if (!batch) throw std::invalid_argument("null batch");
+    BatchHandle* b = static_cast<BatchHandle*>(batch);
+    for (int f = 0; f < b->count(); ++f) {
+      if (completed) completed[f] = b->completed(f);
+      if (flags) flags[f] = b->flags(f);
+    }
+    if (seconds) *seconds = b->seconds();
+  });
+}
+
+int fastpls_hip_batch_export(void* batch, int fit, int field, void* out, size_t elements,
+                             char* error, size_t error_capacity) {
+  return fpls::guard(error, error_capacity, [&] {
+    if (!batch) throw std::invalid_argument("null batch");
+    static_cast<BatchHandle*>(batch)->export_field(fit, field, out, elements);
+  });
+}
+
+void fastpls_hip_batch_destroy(void* batch) { delete static_cast<BatchHandle*>(batch); }
+
+int fastpls_hip_simpls_cv_regression(const void* predictors, const void* responses,
+                                     const int* folds, int precision, int n, int p, int q,
+                                     const int* prefixes, int prefix_count, int scaling,
+                                     int metric, int oversample, int power,
+                                     unsigned long long seed, int store_predictions,
+                                     double* predictions, int* status, double* metrics,
+                                     double* q2, double* rmsd, double* observed_r2, char* error,
+                                     size_t error_capacity) {
+  return fpls::guard(error, error_capacity, [&] {
+    fpls::need_device();
+    if (precision == 32)
+      fpls::simpls_cv_regression<float>(static_cast<const float*>(predictors),
+          static_cast<const float*>(responses), folds, n, p, q, prefixes, prefix_count, scaling,
+          metric, oversample, power, seed, store_predictions != 0, predictions, status, metrics,
+          q2, rmsd, observed_r2);
+    else if (precision == 64)
+      fpls::simpls_cv_regression<double>(static_cast<const double*>(predictors),
+          static_cast<const double*>(responses), folds, n, p, q, prefixes, prefix_count, scaling,
+          metric, oversample, power, seed, store_predictions != 0, predictions, status, metrics,
+          q2, rmsd, observed_r2);
+    else
+      throw std::invalid_argument("precision must be 32 or 64");
+  });
+}
+
+}  // extern "C"

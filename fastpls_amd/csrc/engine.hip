@@ -294,7 +294,8 @@ class Model final : public Handle {
                                 hipMemcpyHostToDevice, stream_));
         dy = yown_.as<T>();
       }
-      launch_center_response<T>(dy, n_, q_, my_.d(), Yc_.d(), ysum_.d(), stream_);
+      part_.reserve(sizeof(double) * (size_t)kCenterScratch * q_);
+      launch_center_response<T>(dy, n_, q_, my_.d(), Yc_.d(), ysum_.d(), part_.d(), stream_);
     }
   }
 
@@ -1066,7 +1067,8 @@ int fastpls_hip_fit_phases(void* model, double* out, int count, char* error,
 
 int fastpls_hip_latent_data(void* x_device, void* y_device, int precision, long long n, int p,
                             int q, int factors, double decay, double noise,
-                            unsigned long long seed, char* error, size_t error_capacity) {
+                            unsigned long long seed, int round32, char* error,
+                            size_t error_capacity) {
   return fpls::guarded(error, error_capacity, [&] {
     fpls::require_device();
     if (!x_device || !y_device || n < 1 || p < 1 || q < 1 || factors < 1)
@@ -1078,11 +1080,12 @@ int fastpls_hip_latent_data(void* x_device, void* y_device, int precision, long 
     y64.alloc(sizeof(double) * (size_t)n * q);
     if (precision == 32) {
       fpls::launch_latent_data<float>(static_cast<float*>(x_device), y64.d(), n, p, q, factors,
-                                      decay, noise, seed, scratch.d(), s);
+                                      decay, noise, seed, 1, scratch.d(), s);
       fpls::launch_convert<float>(y64.d(), static_cast<float*>(y_device), n * q, s);
     } else if (precision == 64) {
       fpls::launch_latent_data<double>(static_cast<double*>(x_device), y64.d(), n, p, q,
-                                       factors, decay, noise, seed, scratch.d(), s);
+                                       factors, decay, noise, seed, round32 ? 1 : 0, scratch.d(),
+                                       s);
       fpls::launch_convert<double>(y64.d(), static_cast<double*>(y_device), n * q, s);
     } else {
       (void)hipStreamDestroy(s);

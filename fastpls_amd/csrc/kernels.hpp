@@ -84,9 +84,11 @@ void launch_plssvd_weights(const double* G, const double* sigma, const double* Q
 // ---- preprocessing
 // Column means of Y (n x q, element T) and centred copy Yc (FP64) plus the
 // column sums of Yc (ysum) used by the shifted cross-product.
+// `scratch` holds kCenterScratch doubles per response.
+constexpr int kCenterScratch = 128;
 template<class T>
 void launch_center_response(const T* Y, long long n, int q, double* mean, double* Yc,
-                            double* ysum, hipStream_t s);
+                            double* ysum, double* scratch, hipStream_t s);
 // Labels (one-based) -> class counts, priors, centred one-hot Yc, plain
 // one-hot H (may be null) and ysum; invalid labels / empty classes set flags.
 void launch_label_response(const int* labels, long long n, int K, int* counts,
@@ -138,9 +140,10 @@ void launch_variance(const double* cross, const double* gram, const double* cols
                      hipStream_t s);
 
 // Synthetic latent-factor data on the device (Philox4x32-10 + Box-Muller).
+// round32 rounds every value to FP32 before it is stored (in T).
 template<class T>
 void launch_latent_data(T* X, double* Y, long long n, int p, int q, int factors,
-                        double decay, double noise, unsigned long long seed,
+                        double decay, double noise, unsigned long long seed, int round32,
                         double* scratch, hipStream_t s);
 
 template<class T>

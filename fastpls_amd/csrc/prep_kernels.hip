@@ -10,25 +10,58 @@ namespace {
 // ------------------------------------------------------------ responses
 // supervised.hpp:54-61: response means; the centred copy is what the
 // shifted cross-product pass multiplies by. One work-group per column.
+// Response centring in three fixed-order phases over (kCenterBlocks x q)
+// work-groups: chunk sums, then the mean and the centred copy with chunk sums
+// of the centred values, then one thread per response folds the chunks.
+constexpr int kCenterBlocks = 64;
+
 template<class T>
-__global__ __launch_bounds__(kWG) void k_center_response(const T* Y, long long n, int q,
-                                                         double* mean, double* Yc,
-                                                         double* ysum) {
+__global__ __launch_bounds__(kWG) void k_center_sums(const T* Y, long long n, double* part) {
   __shared__ double red[kWaves];
-  const int c = blockIdx.x;
+  const int c = blockIdx.y, b = blockIdx.x;
+  const long long per = (n + kCenterBlocks - 1) / kCenterBlocks;
+  const long long i0 = b * per, i1 = i0 + per < n ? i0 + per : n;
   const T* y = Y + (long long)c * n;
   double s = 0.0;
-  for (long long i = threadIdx.x; i < n; i += kWG) s += double(y[i]);
-  const double mu = wg_sum(s, red) / double(n);
-  double cs = 0.0;
+  for (long long i = i0 + threadIdx.x; i < i1; i += kWG) s += double(y[i]);
+  s = wg_sum(s, red);
+  if (threadIdx.x == 0) part[(long long)c * kCenterBlocks + b] = s;
+}
+
+template<class T>
+__global__ __launch_bounds__(kWG) void k_center_apply(const T* Y, long long n, const double* part,
+                                                      double* Yc, double* part2) {
+  __shared__ double red[kWaves];
+  const int c = blockIdx.y, b = blockIdx.x;
+  double tot = 0.0;
+  for (int k = 0; k < kCenterBlocks; ++k) tot += part[(long long)c * kCenterBlocks + k];
+  const double mu = tot / double(n);
+  const long long per = (n + kCenterBlocks - 1) / kCenterBlocks;
+  const long long i0 = b * per, i1 = i0 + per < n ? i0 + per : n;
+  const T* y = Y + (long long)c * n;
   double* yc = Yc + (long long)c * n;
-  for (long long i = threadIdx.x; i < n; i += kWG) {
+  double cs = 0.0;
+  for (long long i = i0 + threadIdx.x; i < i1; i += kWG) {
     const double v = double(y[i]) - mu;
     yc[i] = v;
     cs += v;
   }
   cs = wg_sum(cs, red);
-  if (threadIdx.x == 0) { mean[c] = mu; ysum[c] = cs; }
+  if (threadIdx.x == 0) part2[(long long)c * kCenterBlocks + b] = cs;
+}
+
+__global__ __launch_bounds__(kWG) void k_center_fold(const double* part, const double* part2,
+                                                     long long n, int q, double* mean,
+                                                     double* ysum) {
+  const int c = blockIdx.x * kWG + threadIdx.x;
+  if (c >= q) return;
+  double tot = 0.0, cs = 0.0;
+  for (int k = 0; k < kCenterBlocks; ++k) {
+    tot += part[(long long)c * kCenterBlocks + k];
+    cs += part2[(long long)c * kCenterBlocks + k];
+  }
+  mean[c] = tot / double(n);
+  ysum[c] = cs;
 }
 
 __global__ __launch_bounds__(kWG) void k_fill_int(int* p, int count, int value) {
@@ -75,10 +108,9 @@ __global__ __launch_bounds__(kWG) void k_label_fill(const int* labels, long long
 // squares standard deviation with the reference's clamp and fallback, and
 // S[j,c] = (sum d*yc - (mean - shift) * sum yc) / scale.
 template<class T>
-__global__ __launch_bounds__(kWG) void k_finalize_crossprod(
-    const T* X, long long n, int p, int q, int scaling, const double* stats_part,
-    const double* s_part, int splits, const double* ysum, double* mean, double* scale,
-    double* colss, double* S, double* S0, int* flags) {
+__global__ __launch_bounds__(kWG) void k_finalize_stats(
+    const T* X, long long n, int p, int scaling, const double* stats_part, int splits,
+    double* mean, double* scale, double* colss, int* flags) {
   const int j = blockIdx.x * kWG + threadIdx.x;
   if (j >= p) return;
   double s1 = 0.0, s2 = 0.0;
@@ -102,16 +134,26 @@ __global__ __launch_bounds__(kWG) void k_finalize_crossprod(
   const double delta = shift - mu;  // x - mu = d + delta
   const double ss = s2 + 2.0 * delta * s1 + dn * delta * delta;
   colss[j] = ss / (sd * sd);
-  bool finite = isfinite(mu) && isfinite(ss);
-  for (int c = 0; c < q; ++c) {
-    double acc = 0.0;
-    for (int sp = 0; sp < splits; ++sp) acc += s_part[(long long)sp * p * q + j + (long long)c * p];
-    const double v = (acc + delta * ysum[c]) / sd;
-    S[j + (long long)c * p] = v;
-    if (S0) S0[j + (long long)c * p] = v;
-    finite = finite && isfinite(v);
-  }
-  if (!finite) atomicOr(flags, kFlagNonFinite);
+  if (!(isfinite(mu) && isfinite(ss))) atomicOr(flags, kFlagNonFinite);
+}
+
+// S[j, c] = (sum_i d_ij y~_ic + (shift_j - mu_j) sum_i y~_ic) / sd_j over the
+// row-split partials; grid (column blocks, responses).
+template<class T>
+__global__ __launch_bounds__(kWG) void k_finalize_cross(
+    const T* X, long long n, int p, int q, const double* s_part, int splits,
+    const double* ysum, const double* mean, const double* scale, double* S, double* S0,
+    int* flags) {
+  const int j = blockIdx.x * kWG + threadIdx.x;
+  const int c = blockIdx.y;
+  if (j >= p) return;
+  const double delta = double(X[(long long)j * n]) - mean[j];
+  double acc = 0.0;
+  for (int sp = 0; sp < splits; ++sp) acc += s_part[(long long)sp * p * q + j + (long long)c * p];
+  const double v = (acc + delta * ysum[c]) / scale[j];
+  S[j + (long long)c * p] = v;
+  if (S0) S0[j + (long long)c * p] = v;
+  if (!isfinite(v)) atomicOr(flags, kFlagNonFinite);
 }
 
 __global__ __launch_bounds__(kWG) void k_div_rows(const double* in, const double* d,
@@ -262,14 +304,15 @@ __global__ __launch_bounds__(kWG) void k_latent_matrix(T* X, const double* Tl,
                                                        const double* Pl, long long n,
                                                        long long cols, int F, double noise,
                                                        unsigned long long seed,
-                                                       unsigned stream) {
+                                                       unsigned stream, int round32) {
   const long long total = n * cols;
   for (long long e = (long long)blockIdx.x * kWG + threadIdx.x; e < total;
        e += (long long)gridDim.x * kWG) {
     const long long i = e % n, j = e / n;
     double acc = 0.0;
     for (int f = 0; f < F; ++f) acc = fma(Tl[i + (long long)f * n], Pl[j + (long long)f * cols], acc);
-    X[e] = T(acc + noise * normal_at(seed, stream, (unsigned long long)e));
+    const double v = acc + noise * normal_at(seed, stream, (unsigned long long)e);
+    X[e] = T(round32 ? double(float(v)) : v);
   }
 }
 
@@ -301,9 +344,16 @@ unsigned grid_for(long long count) {
 
 template<class T>
 void launch_center_response(const T* Y, long long n, int q, double* mean, double* Yc,
-                            double* ysum, hipStream_t s) {
-  hipLaunchKernelGGL((k_center_response<T>), dim3(q), dim3(kWG), 0, s, Y, n, q, mean, Yc, ysum);
-  FPLS_LAUNCHED("k_center_response");
+                            double* ysum, double* scratch, hipStream_t s) {
+  static_assert(kCenterBlocks * 2 == kCenterScratch, "centring scratch layout");
+  if (q <= 0) return;
+  double* part = scratch;
+  hipLaunchKernelGGL((k_center_sums<T>), dim3(kCenterBlocks, q), dim3(kWG), 0, s, Y, n, part);
+  hipLaunchKernelGGL((k_center_apply<T>), dim3(kCenterBlocks, q), dim3(kWG), 0, s, Y, n, part, Yc,
+                     part + (long long)q * kCenterBlocks);
+  hipLaunchKernelGGL(k_center_fold, dim3((q + kWG - 1) / kWG), dim3(kWG), 0, s, part,
+                     part + (long long)q * kCenterBlocks, n, q, mean, ysum);
+  FPLS_LAUNCHED("k_center_fold");
 }
 
 void launch_label_response(const int* labels, long long n, int K, int* counts,
@@ -323,10 +373,13 @@ void launch_finalize_crossprod(const T* X, long long n, int p, int q, int scalin
                                const double* ysum, double* mean, double* scale,
                                double* colss, double* S, double* S0, int* flags,
                                hipStream_t s) {
-  hipLaunchKernelGGL((k_finalize_crossprod<T>), dim3((p + kWG - 1) / kWG), dim3(kWG), 0, s, X,
-                     n, p, q, scaling, stats_part, s_part, splits, ysum, mean, scale, colss,
-                     S, S0, flags);
-  FPLS_LAUNCHED("k_finalize_crossprod");
+  const unsigned blocks = unsigned((p + kWG - 1) / kWG);
+  hipLaunchKernelGGL((k_finalize_stats<T>), dim3(blocks), dim3(kWG), 0, s, X, n, p, scaling,
+                     stats_part, splits, mean, scale, colss, flags);
+  if (q > 0)
+    hipLaunchKernelGGL((k_finalize_cross<T>), dim3(blocks, q), dim3(kWG), 0, s, X, n, p, q, s_part,
+                       splits, ysum, mean, scale, S, S0, flags);
+  FPLS_LAUNCHED("k_finalize_cross");
 }
 
 void launch_div_rows(const double* in, const double* d, long long rows, int cols,
@@ -364,7 +417,7 @@ void launch_response_sums(const double* pred, const T* observed, const int* labe
 
 template<class T>
 void launch_latent_data(T* X, double* Y, long long n, int p, int q, int factors, double decay,
-                        double noise, unsigned long long seed, double* scratch,
+                        double noise, unsigned long long seed, int round32, double* scratch,
                         hipStream_t s) {
   double* Tl = scratch;
   double* Pl = Tl + n * factors;
@@ -372,9 +425,9 @@ void launch_latent_data(T* X, double* Y, long long n, int p, int q, int factors,
   hipLaunchKernelGGL(k_latent_factors, dim3(grid_for((n + p + q) * factors)), dim3(kWG), 0, s,
                      Tl, Pl, Ql, n, p, q, factors, decay, seed);
   hipLaunchKernelGGL((k_latent_matrix<T>), dim3(grid_for(n * p)), dim3(kWG), 0, s, X, Tl, Pl,
-                     n, (long long)p, factors, noise, seed, 3u);
+                     n, (long long)p, factors, noise, seed, 3u, round32);
   hipLaunchKernelGGL((k_latent_matrix<double>), dim3(grid_for(n * q)), dim3(kWG), 0, s, Y, Tl,
-                     Ql, n, (long long)q, factors, noise, seed, 4u);
+                     Ql, n, (long long)q, factors, noise, seed, 4u, round32);
   FPLS_LAUNCHED("k_latent_matrix");
 }
 
@@ -393,7 +446,7 @@ void launch_convert(const double* in, T* out, long long count, hipStream_t s) {
 
 #define FPLS_PREP_INSTANCES(T)                                                              \
   template void launch_center_response<T>(const T*, long long, int, double*, double*,        \
-                                          double*, hipStream_t);                             \
+                                          double*, double*, hipStream_t);                    \
   template void launch_finalize_crossprod<T>(const T*, long long, int, int, int,             \
                                              const double*, const double*, int,              \
                                              const double*, double*, double*, double*,       \
@@ -404,7 +457,7 @@ void launch_convert(const double* in, T* out, long long count, hipStream_t s) {
   template void launch_response_sums<T>(const double*, const T*, const int*, const double*,  \
                                         long long, int, double*, int*, hipStream_t);         \
   template void launch_latent_data<T>(T*, double*, long long, int, int, int, double, double, \
-                                      unsigned long long, double*, hipStream_t);             \
+                                      unsigned long long, int, double*, hipStream_t);             \
   template void launch_convert<T>(const double*, T*, long long, hipStream_t);
 FPLS_PREP_INSTANCES(float)
 FPLS_PREP_INSTANCES(double)

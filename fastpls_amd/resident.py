@@ -57,7 +57,7 @@ SIGNATURES = {
     "fastpls_hip_device_free": (C.c_int, [_vp, _cp, _sz]),
     "fastpls_hip_memcpy": (C.c_int, [_vp, _cvp, _sz, C.c_int, _cp, _sz]),
     "fastpls_hip_latent_data": (C.c_int, [_vp, _vp, C.c_int, _ll, C.c_int, C.c_int, C.c_int,
-                                          C.c_double, C.c_double, _ull, _cp, _sz]),
+                                          C.c_double, C.c_double, _ull, C.c_int, _cp, _sz]),
 }
 
 _LIB = None
@@ -377,9 +377,11 @@ class DeviceBuffer:
 
 
 def latent_data_device(x_ptr, y_ptr, precision, n, p, q, factors=10, decay=1.0, noise=0.1,
-                       seed=0):
-    """Fill caller-allocated device buffers with latent-factor data."""
+                       seed=0, round32=False):
+    """Fill caller-allocated device buffers with latent-factor data (round32:
+    FP64 buffers hold the FP32-rounded values)."""
     e = _err()
     if load_library().fastpls_hip_latent_data(C.c_void_p(x_ptr), C.c_void_p(y_ptr), precision, n,
-                                              p, q, factors, decay, noise, seed, e, 1024) != 0:
+                                              p, q, factors, decay, noise, seed, int(round32), e,
+                                              1024) != 0:
         _raise(e)

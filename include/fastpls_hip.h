@@ -149,10 +149,14 @@ int fastpls_hip_fit_phases(void* model, double* out, int count, char* error,
 /* Synthetic latent-factor data written straight into device memory
  * (Philox4x32-10 + Box-Muller): X (n x p, precision 32/64) = T P' + noise E,
  * Y (n x q, same precision) = T Q' + noise F, with factor column f scaled by
- * decay^f. x_device / y_device are caller-allocated device buffers. */
+ * decay^f. x_device / y_device are caller-allocated device buffers. With
+ * round32 = 1 and precision 64 the FP64 buffers hold exactly the FP32 data
+ * (every value rounded to FP32), so both precisions can fit identical inputs.
+ * No reference counterpart (benchmark input). */
 int fastpls_hip_latent_data(void* x_device, void* y_device, int precision, long long n,
                             int p, int q, int factors, double decay, double noise,
-                            unsigned long long seed, char* error, size_t error_capacity);
+                            unsigned long long seed, int round32, char* error,
+                            size_t error_capacity);
 
 #ifdef __cplusplus
 }

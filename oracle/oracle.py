@@ -245,6 +245,12 @@ class Reference:
         if status != 0:
             raise RuntimeError(self.lib.ref_last_error().decode())
 
+    def block_size(self, remaining, p, q, classification, n, maximum):
+        f = self.lib.ref_block_size
+        f.restype = C.c_longlong
+        f.argtypes = [C.c_longlong] * 3 + [C.c_int] + [C.c_longlong] * 2
+        return int(f(remaining, p, q, int(classification), n, maximum))
+
     def simpls_fit(self, x, y=None, labels=None, classes=None, ncomp=2, scaling=2,
                    oversample=32, power=5, seed=1, block=1, cache=0, reorth=0,
                    batch_geometry=0):

@@ -45,6 +45,14 @@ extern "C" {
 
 const char* ref_last_error() { return g_error.c_str(); }
 
+// simpls_candidate_block_size (simpls.hpp:43-62) as compiled from the header.
+long long ref_block_size(long long remaining, long long p, long long q, int classification,
+                         long long n, long long maximum) {
+  return (long long)fastpls::core::simpls_candidate_block_size(
+      std::size_t(remaining), std::size_t(p), std::size_t(q), classification != 0,
+      std::size_t(n), std::size_t(maximum));
+}
+
 // Preprocess (dense or one-based labels) and fit SIMPLS with the given
 // controls. Outputs: R p*A, Q q*A, V p*A, T n*A, mx p, sx p, my q.
 int ref_simpls_fit(const double* x, const double* y, const int* labels, int n,

@@ -233,3 +233,45 @@ def test_gaussian_stream_is_the_reference_stream(oracle):
     m = oracle.simpls_fit(g["X"], g["Y"], ncomp=int(g["ncomp"]), oversample=int(g["oversample"]),
                           power=int(g["power"]), seed=int(g["seed"]) + 1)
     assert rel(coef(m, 8), coef(g, 8)) > 1e-6
+
+
+def _reference_or_skip():
+    import os
+    from oracle.oracle import REF_SO, Reference
+    if not os.path.exists(REF_SO):
+        pytest.skip("oracle/_ref not built (needs the reference checkout)")
+    return Reference()
+
+
+def test_block_routing_matches_reference_core(oracle):
+    # simpls_candidate_block_size (simpls.hpp:43-62) compiled from the header,
+    # over a grid that straddles every threshold: work 5e8, responses 1 / 2048,
+    # remaining 4, cross-covariance 64 Mi elements.
+    ref = _reference_or_skip()
+    grid = [(n, p, q, A, cls)
+            for n in (10, 2000, 50000, 100000)
+            for p in (3, 100, 10000, 20000)
+            for q in (1, 2, 3, 20, 2048, 2049, 4000)
+            for A in (1, 3, 4, 15, 80)
+            for cls in (False, True)]
+    for n, p, q, A, cls in grid:
+        got = oracle.controls(n, p, q, A, classification=cls)["block"]
+        want = ref.block_size(A, p, q, cls, n, 64)
+        assert got == want, (n, p, q, A, cls, got, want)
+    # the wide-regression branch (p q > 64 Mi) caps the block at 8
+    assert ref.block_size(30, 40000, 2000, False, 100, 64) == 8
+    assert oracle.controls(100, 40000, 2000, 30)["block"] == 8
+
+
+def test_crossprod_cache_rule_cases(oracle):
+    # src/r_api.cpp:1165-1176 (not compilable here: needs R): the X^T X cache
+    # is used iff A >= 20, p <= n, n >= 8 p and p <= 512; batch geometry iff
+    # block > 1 and no cache.
+    assert oracle.controls(5000, 500, 3, 20)["cache"] == 1
+    assert oracle.controls(5000, 500, 3, 19)["cache"] == 0
+    assert oracle.controls(3999, 500, 3, 20)["cache"] == 0
+    assert oracle.controls(8000, 513, 3, 20)["cache"] == 0
+    c = oracle.controls(50000, 10000, 20, 15, classification=True)
+    assert (c["block"], c["cache"], c["batch_geometry"]) == (15, 0, 1)
+    c = oracle.controls(2000, 1000, 3, 10)
+    assert (c["block"], c["cache"], c["batch_geometry"]) == (1, 0, 0)
