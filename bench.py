@@ -4,7 +4,7 @@ Workload (default): PLS-SVD through the randomized SVD of XᵀY,
 n=100000, p=20000, q=50, 20 components, oversample 10, 2 power iterations,
 rSVD seed 1; latent-factor data (30 factors, column norms 0.8^j, noise 0.05,
 data seed 0) generated directly in HBM. X is FP32 (8 GB) for the mixed path
-(bf16x3 matrix-core products, FP32 accumulation, FP64 small math) and FP64
+(bf16x6 matrix-core products, FP32 accumulation, FP64 small math) and FP64
 (16 GB) for the plain path (FP64 throughout). A step is one complete fit
 (preprocessing, cross-product, randomized SVD, scores, prefix weights) with
 inputs already resident in HBM; value = fits per second of the mixed path.
@@ -191,14 +191,15 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "plssvd_c2_traffic.json")
     if os.path.exists(tpath):
         tr = json.load(open(tpath))
-        if tr.get("workload", "").startswith(f"plssvd_rsvd n={n} p={p} q={q} k={k}"):
+        if (tr.get("kernel") == "k_xtw_bf6<64>" and
+                tr.get("workload", "").startswith(f"plssvd_rsvd n={n} p={p} q={q} k={k}")):
             traffic = tr["traffic_bytes"]
     roofline = {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
                 "algorithmic_bytes": cp_bytes,
-                "kernel": "k_xtw_bf3<64> (column stats + X^T Y, one pass over X, bf16x3 MFMA)",
+                "kernel": "k_xtw_bf6<64> (column stats + X^T Y, one pass over X, bf16x6 MFMA)",
                 "kernel_ms": round(1e3 * t_cp, 4),
-                "score_pass_kernel": "k_xw_bf3<32> (T = X_s R, bf16x3 MFMA)",
+                "score_pass_kernel": "k_xw_bf6<32> (T = X_s R, bf16x6 MFMA)",
                 "score_pass_ms": round(1e3 * t_sc, 4),
                 "score_pass_frac": round(sc_bytes / t_sc / HBM_PEAK, 4)}
 
@@ -226,7 +227,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 storage; bf16x3 MFMA with f32 accumulation; f64 small math",
+        "dtype": "f32 storage; bf16x6 MFMA with f32 accumulation; f64 small math",
         "data": "synthetic latent-factor data generated in HBM (Philox4x32-10)",
         "config": {"workload": f"plssvd_rsvd n={n} p={p} q={q} k={k} oversample=10 power=2 "
                                f"(X fp32 {float(n) * p * 4 / 1e9:.1f} GB)",

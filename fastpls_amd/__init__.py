@@ -4,7 +4,9 @@ The compute path is the C ABI library ``lib/libfastpls_hip.so`` (declared in
 ``include/fastpls_hip.h``); this package is its Python binding.
 """
 from .resident import (FIELDS, PLSSVD, SIMPLS, DeviceBuffer, HipError, ResidentModel,
-                       device_info, gemm, latent_data_device, load_library)
+                       SimplsBatch, device_info, gemm, latent_data_device, load_library,
+                       simpls_cv_regression)
 
-__all__ = ["FIELDS", "PLSSVD", "SIMPLS", "DeviceBuffer", "HipError", "ResidentModel", "device_info", "gemm",
-           "latent_data_device", "load_library"]
+__all__ = ["FIELDS", "PLSSVD", "SIMPLS", "DeviceBuffer", "HipError", "ResidentModel",
+           "SimplsBatch", "device_info", "gemm", "latent_data_device", "load_library",
+           "simpls_cv_regression"]

@@ -24,11 +24,6 @@
 
 namespace fpls {
 
-// Row gather: dst (count x cols) = src[idx, :] for a column-major src (n x cols).
-template<class T>
-void launch_gather_rows(const T* src, long long n, long long cols, const int* idx, long long count,
-                        T* dst, hipStream_t s);
-
 namespace {
 
 // A bump allocator over one device allocation (256-byte aligned pieces).
@@ -63,6 +58,10 @@ struct FitIn {
 
 template<class T>
 class SimplsBatch {
+  // right-hand chunk of the per-step score pass (FP32 matrix-core launches
+  // take at most 32 columns)
+  static constexpr int kXwChunk = sizeof(T) == 4 ? 32 : 64;
+
  public:
   SimplsBatch(const std::vector<GroupIn>& groups, const std::vector<FitIn>& fits, int scaling,
               int oversample, int power)
@@ -77,6 +76,7 @@ class SimplsBatch {
       FPLS_HIP(hipEventRecord(start.e, stream_));
       run();
       FPLS_HIP(hipEventRecord(stop.e, stream_));
+      FPLS_HIP(hipEventSynchronize(stop.e));
       collect();
       float ms = 0.f;
       FPLS_HIP(hipEventElapsedTime(&ms, start.e, stop.e));
@@ -110,7 +110,7 @@ class SimplsBatch {
                              (size_t)G.p, (size_t)G.p, (size_t)F.q};
     if (field < 0 || field > 5 || !out || elements != counts[field])
       throw std::invalid_argument("invalid batch field output size");
-    const double* src = field == 0 ? F.R : field == 1 ? F.Q : field == 2 ? F.T
+    const double* src = field == 0 ? F.R : field == 1 ? F.Q : field == 2 ? F.Tsc
                       : field == 3 ? G.mx : field == 4 ? G.sx : F.my;
     std::vector<double> host(elements);
     FPLS_HIP(hipMemcpy(host.data(), src, sizeof(double) * elements, hipMemcpyDeviceToHost));
@@ -124,13 +124,23 @@ class SimplsBatch {
     const Group& G = gplan_[fits_[f].group];
     XwTask<T> t{};
     t.A = x; t.m = rows; t.k = G.p; t.center = G.mx; t.W = F.Rs; t.ldw = G.p; t.N = prefix;
-    const int splits = splits_for(xw_row_tiles(rows), G.p, 32);
+    const int splits = xw_splits(rows, G.p, prefix);
     t.splits = splits; t.ldo = rows; t.slab = (long long)rows * prefix;
+    const size_t fb = xw_frag_bytes<T>(G.p, prefix);
+    if (fb > proj_frag_.bytes()) {
+      FPLS_HIP(hipStreamSynchronize(stream_));
+      proj_frag_.alloc(fb);
+    }
+    t.frag = fb ? proj_frag_.as<void>() : nullptr;
     if (splits == 1) {
       t.out = scores;
       launch_xw_single<T>(t, stream_);
     } else {
-      part.reserve(sizeof(double) * (size_t)splits * rows * prefix);
+      const size_t pb = sizeof(double) * (size_t)splits * rows * prefix;
+      if (pb > part.bytes()) {
+        FPLS_HIP(hipStreamSynchronize(stream_));
+        part.alloc(pb);
+      }
       t.out = part.d();
       launch_xw_single<T>(t, stream_);
       launch_sum_slabs(part.d(), splits, (long long)rows * prefix, (long long)rows * prefix,
@@ -150,16 +160,16 @@ class SimplsBatch {
  private:
   struct Group {
     const T* X;
-    int n, p, qcat = 0, bcat = 0;
+    int n, p, qcat = 0, bcat = 0, max_steps = 0;
     double *Yc, *ysum, *S, *S0, *mx, *sx, *colss, *stats, *spart;
-    double *cand, *cand_scaled, *t_raw, *loads;
-    int splits = 1, ls_splits = 1;
+    double *cand, *cand_scaled, *t_raw, *tpart, *loads;
+    int splits = 1, ls_splits = 1, xs_splits = 1;
     int* flags;
   };
   struct Fit {
     int q, A, bmax, wmax, nsteps, off, boff;
     bool right_gram;
-    double *my, *G, *R, *Q, *V, *T, *Rs, *scratch;
+    double *my, *G, *R, *Q, *V, *Tsc, *Rs, *scratch, *center;
     int* state;
     int* counts;
     T* yperm;
@@ -193,6 +203,7 @@ class SimplsBatch {
       Fit& F = plan_[f];
       F.q = in.q;
       if (F.q < 1 || (in.labels && F.q < 2)) throw std::invalid_argument("invalid response count");
+      if (F.q > 512) throw std::invalid_argument("more than 512 responses is not supported");
       if (in.A < 1 || in.A > std::min(G.n - 1, G.p)) throw std::invalid_argument("invalid batch SIMPLS dimensions");
       F.A = in.A;
       const int block = candidate_block_size(F.A, G.p, F.q, in.labels != nullptr, G.n, 64);
@@ -218,24 +229,40 @@ class SimplsBatch {
       G.qcat += F.q;
       F.boff = G.bcat;
       G.bcat += F.bmax;
+      G.max_steps = std::max(G.max_steps, F.nsteps);
       max_steps_ = std::max(max_steps_, F.nsteps);
     }
-    // preprocessing splits: one launch over every group
+    // groups in decreasing order of component steps: the groups still active
+    // at step s are a prefix of this order
+    order_.resize(gplan_.size());
+    for (size_t g = 0; g < gplan_.size(); ++g) order_[g] = int(g);
+    std::stable_sort(order_.begin(), order_.end(),
+                     [&](int a, int b) { return gplan_[a].max_steps > gplan_[b].max_steps; });
+    // preprocessing splits (one launch over every group) and the per-step splits
     long long tiles1 = 0;
-    for (const Group& G : gplan_) tiles1 += xtw_col_tiles(G.p) * ((G.qcat + 63) / 64);
+    int maxn_pre = 0, maxn_step = 0, maxn_xw = 0;
+    for (const Group& G : gplan_) {
+      tiles1 += xtw_col_tiles(G.p) * ((G.qcat + 63) / 64);
+      maxn_pre = std::max(maxn_pre, std::min(G.qcat, 64));
+      maxn_step = std::max(maxn_step, std::min(G.bcat, 64));
+      maxn_xw = std::max(maxn_xw, std::min(G.bcat, kXwChunk));
+    }
     for (Group& G : gplan_) {
       long long s = (2048 + tiles1 - 1) / std::max<long long>(tiles1, 1);
       s = std::max<long long>(1, std::min<long long>(s, G.n / 256));
       s = std::max<long long>(s, (G.n + kXtwSplitRows - 1) / kXtwSplitRows);
       G.splits = int(std::min<long long>(s, 4096));
+      const bool mfma = std::min(G.bcat, 64) >= kMfmaMinN;
+      G.ls_splits = mfma ? int((G.n + kXtwSplitRows - 1) / kXtwSplitRows) : 1;
+      G.xs_splits = (mfma && sizeof(T) == 4) ? int((G.p + kXwSplitCols - 1) / kXwSplitCols) : 1;
     }
-    for (Group& G : gplan_) G.ls_splits = G.bcat >= kMfmaMinN ? int((G.n + kXtwSplitRows - 1) / kXtwSplitRows) : 1;
     // arena layout
-    std::vector<size_t> go(gplan_.size() * 15), fo(plan_.size() * 14);
+    constexpr int kGo = 16, kFo = 15;
+    std::vector<size_t> go(gplan_.size() * kGo), fo(plan_.size() * kFo);
     for (size_t g = 0; g < gplan_.size(); ++g) {
       const Group& G = gplan_[g];
       const size_t n = G.n, p = G.p, Q = G.qcat, B = G.bcat;
-      size_t* o = &go[g * 15];
+      size_t* o = &go[g * kGo];
       o[0] = arena_.add(8 * n * Q);                    // Yc
       o[1] = arena_.add(8 * Q);                        // ysum
       o[2] = arena_.add(8 * p * Q);                    // S
@@ -250,12 +277,20 @@ class SimplsBatch {
       o[11] = arena_.add(8 * n * B);                   // t_raw
       o[12] = arena_.add(8 * (size_t)G.ls_splits * p * B);  // loads (split slabs)
       o[13] = arena_.add(sizeof(int) * 4);             // flags
+      o[14] = G.xs_splits > 1 ? arena_.add(8 * (size_t)G.xs_splits * n * B) : 0;  // score slabs
+      // operand workspaces: one per right-hand chunk of each pass
+      const size_t xw_chunks = (B + kXwChunk - 1) / kXwChunk, xt_chunks = (B + 63) / 64;
+      const size_t pre_chunks = (Q + 63) / 64;
+      o[15] = arena_.add(xw_chunks * xw_frag_bytes<T>(G.p, maxn_xw) +
+                         xt_chunks * xtw_frag_bytes<T>(G.n, maxn_step) +
+                         pre_chunks * xtw_frag_bytes<T>(G.n, maxn_pre) +
+                         256 * (xw_chunks + xt_chunks + pre_chunks));
     }
     for (size_t f = 0; f < plan_.size(); ++f) {
       const Fit& F = plan_[f];
       const Group& G = gplan_[fits_[f].group];
       const size_t n = G.n, p = G.p, q = F.q, A = F.A;
-      size_t* o = &fo[f * 14];
+      size_t* o = &fo[f * kFo];
       o[0] = arena_.add(8 * q);                                   // my
       o[1] = arena_.add(8 * q * q);                               // G
       o[2] = arena_.add(8 * p * A);                               // R
@@ -270,12 +305,14 @@ class SimplsBatch {
       o[11] = fits_[f].perm ? arena_.add(sizeof(int) * n) : 0;    // permuted labels
       o[12] = arena_.add(sizeof(RefreshStep) * F.host_steps.size());
       o[13] = arena_.add(8 * F.host_omega.size());
+      o[14] = arena_.add(8 * (size_t)kCenterScratch * q);        // centring scratch
     }
     devfits_off_ = arena_.add(sizeof(SimplsFitDev) * plan_.size());
     arena_.allocate();
+    std::vector<char*> frag_cursor(gplan_.size());
     for (size_t g = 0; g < gplan_.size(); ++g) {
       Group& G = gplan_[g];
-      const size_t* o = &go[g * 15];
+      const size_t* o = &go[g * kGo];
       G.Yc = arena_.at<double>(o[0]); G.ysum = arena_.at<double>(o[1]);
       G.S = arena_.at<double>(o[2]); G.S0 = arena_.at<double>(o[3]);
       G.mx = arena_.at<double>(o[4]); G.sx = arena_.at<double>(o[5]);
@@ -283,28 +320,38 @@ class SimplsBatch {
       G.spart = arena_.at<double>(o[8]); G.cand = arena_.at<double>(o[9]);
       G.cand_scaled = arena_.at<double>(o[10]); G.t_raw = arena_.at<double>(o[11]);
       G.loads = arena_.at<double>(o[12]); G.flags = arena_.at<int>(o[13]);
+      G.tpart = G.xs_splits > 1 ? arena_.at<double>(o[14]) : nullptr;
+      frag_cursor[g] = arena_.at<char>(o[15]);
     }
+    auto take_frag = [&](int g, size_t bytes) -> void* {
+      if (bytes == 0) return nullptr;
+      void* ptr = frag_cursor[g];
+      frag_cursor[g] += (bytes + 255) / 256 * 256;
+      return ptr;
+    };
     std::vector<SimplsFitDev> dev(plan_.size());
     for (size_t f = 0; f < plan_.size(); ++f) {
       Fit& F = plan_[f];
       Group& G = gplan_[fits_[f].group];
-      const size_t* o = &fo[f * 14];
+      const size_t* o = &fo[f * kFo];
       F.my = arena_.at<double>(o[0]); F.G = arena_.at<double>(o[1]);
       F.R = arena_.at<double>(o[2]); F.Q = arena_.at<double>(o[3]);
-      F.V = arena_.at<double>(o[4]); F.T = arena_.at<double>(o[5]);
+      F.V = arena_.at<double>(o[4]); F.Tsc = arena_.at<double>(o[5]);
       F.Rs = arena_.at<double>(o[6]); F.scratch = arena_.at<double>(o[7]);
       F.state = arena_.at<int>(o[8]); F.counts = arena_.at<int>(o[9]);
       F.yperm = fits_[f].perm ? arena_.at<T>(o[10]) : nullptr;
       F.lperm = fits_[f].perm ? arena_.at<int>(o[11]) : nullptr;
       F.steps = arena_.at<RefreshStep>(o[12]);
       F.omega = arena_.at<double>(o[13]);
+      F.center = arena_.at<double>(o[14]);
       FPLS_HIP(hipMemcpy(F.steps, F.host_steps.data(), sizeof(RefreshStep) * F.host_steps.size(),
                          hipMemcpyHostToDevice));
       FPLS_HIP(hipMemcpy(F.omega, F.host_omega.data(), sizeof(double) * F.host_omega.size(),
                          hipMemcpyHostToDevice));
       SimplsFitDev& d = dev[f];
       d.n = G.n; d.p = G.p; d.q = F.q; d.A = F.A;
-      d.power = power_; d.use_right_gram = F.right_gram ? 1 : 0; d.reorth = 0;
+      d.power = power_; d.use_right_gram = F.right_gram ? 1 : 0;
+      d.reorth = sizeof(T) == 4 ? 1 : 0;  // src/r_api.cpp:1416: two passes for float
       d.nsteps = F.nsteps; d.bmax = F.bmax; d.ts_splits = 1; d.ls_splits = G.ls_splits;
       d.ls_slab = (long long)G.p * G.bcat;
       d.steps = F.steps; d.omega = F.omega;
@@ -315,7 +362,7 @@ class SimplsBatch {
       d.t_raw = G.t_raw + (size_t)F.boff * G.n;
       d.ts_part = nullptr;
       d.ls_part = G.loads + (size_t)F.boff * G.p;
-      d.R = F.R; d.Q = F.Q; d.V = F.V; d.T = F.T;
+      d.R = F.R; d.Q = F.Q; d.V = F.V; d.T = F.Tsc;
       d.scratch = F.scratch; d.state = F.state;
     }
     devfits_ = arena_.at<SimplsFitDev>(devfits_off_);
@@ -324,32 +371,116 @@ class SimplsBatch {
     std::vector<int> zeros(4, 0);
     for (Fit& F : plan_) FPLS_HIP(hipMemcpy(F.state, zeros.data(), sizeof(int) * 4, hipMemcpyHostToDevice));
     for (Group& G : gplan_) FPLS_HIP(hipMemcpy(G.flags, zeros.data(), sizeof(int) * 4, hipMemcpyHostToDevice));
-    // task lists of the per-step passes (identical for every step)
+    // task lists of the per-step passes (identical for every step), in group
+    // order; prefix bookkeeping for the active groups of each step
     std::vector<XwTask<T>> xw;
     std::vector<XtwTask<T>> xt;
-    for (Group& G : gplan_) {
-      for (int c0 = 0; c0 < G.bcat; c0 += 64) {
-        const int nb = std::min(64, G.bcat - c0);
+    std::vector<SlabSum> sums;
+    std::vector<int> xw_end(order_.size() + 1, 0), xt_end(order_.size() + 1, 0),
+        sum_end(order_.size() + 1, 0), xw_mn(order_.size() + 1, 0), xt_mn(order_.size() + 1, 0);
+    std::vector<long long> sum_max(order_.size() + 1, 0);
+    for (size_t gi = 0; gi < order_.size(); ++gi) {
+      const int g = order_[gi];
+      Group& G = gplan_[g];
+      for (int c0 = 0; c0 < G.bcat; c0 += kXwChunk) {
+        const int nb = std::min(kXwChunk, G.bcat - c0);
         XwTask<T> a{};
         a.A = G.X; a.m = G.n; a.k = G.p; a.center = G.mx;
-        a.W = G.cand_scaled + (size_t)c0 * G.p; a.ldw = G.p; a.N = nb; a.splits = 1;
-        a.out = G.t_raw + (size_t)c0 * G.n; a.ldo = G.n; a.slab = 0;
+        a.W = G.cand_scaled + (size_t)c0 * G.p; a.ldw = G.p; a.N = nb; a.splits = G.xs_splits;
+        if (G.xs_splits > 1) {
+          a.out = G.tpart + (size_t)c0 * G.n; a.slab = (long long)G.n * G.bcat;
+        } else {
+          a.out = G.t_raw + (size_t)c0 * G.n; a.slab = 0;
+        }
+        a.ldo = G.n;
+        a.frag = take_frag(g, xw_frag_bytes<T>(G.p, maxn_xw));
         xw.push_back(a);
+        xw_mn[gi + 1] = std::max(xw_mn[gi + 1], nb);
+      }
+      for (int c0 = 0; c0 < G.bcat; c0 += 64) {
+        const int nb = std::min(64, G.bcat - c0);
         XtwTask<T> b{};
         b.A = G.X; b.m = G.n; b.k = G.p; b.center = G.mx; b.mode = kCentered;
         b.W = G.t_raw + (size_t)c0 * G.n; b.ldw = G.n; b.N = nb; b.splits = G.ls_splits;
         b.out = G.loads + (size_t)c0 * G.p; b.ldo = G.p; b.slab = (long long)G.p * G.bcat;
+        b.frag = take_frag(g, xtw_frag_bytes<T>(G.n, maxn_step));
         xt.push_back(b);
+        xt_mn[gi + 1] = std::max(xt_mn[gi + 1], nb);
+      }
+      if (G.xs_splits > 1) {
+        const long long cnt = (long long)G.n * G.bcat;
+        sums.push_back(SlabSum{G.tpart, G.t_raw, cnt, cnt, G.xs_splits});
+        sum_max[gi + 1] = std::max(sum_max[gi], cnt);
+      } else {
+        sum_max[gi + 1] = sum_max[gi];
+      }
+      xw_mn[gi + 1] = std::max(xw_mn[gi + 1], xw_mn[gi]);
+      xt_mn[gi + 1] = std::max(xt_mn[gi + 1], xt_mn[gi]);
+      xw_end[gi + 1] = int(xw.size());
+      xt_end[gi + 1] = int(xt.size());
+      sum_end[gi + 1] = int(sums.size());
+    }
+    int mn = 0;
+    const long long xw_total = pack_xw(xw.data(), int(xw.size()), &mn);
+    const long long xt_total = pack_xtw(xt.data(), int(xt.size()), &mn);
+    // tiles of a prefix: tile0 of the first task after it. The xtw tile count
+    // depends on the launch width, so packs of each prefix are recomputed.
+    steps_.assign(order_.size() + 1, StepLaunch{});
+    for (size_t na = 1; na <= order_.size(); ++na) {
+      StepLaunch& L = steps_[na];
+      L.xw_count = xw_end[na];
+      L.xt_count = xt_end[na];
+      L.sum_count = sum_end[na];
+      L.sum_max = sum_max[na];
+      L.xw_maxn = xw_mn[na];
+      L.xt_maxn = xt_mn[na];
+      L.xw_tiles = L.xw_count < int(xw.size()) ? xw[L.xw_count].tile0 : xw_total;
+      L.xt_tiles = L.xt_count < int(xt.size()) ? xt[L.xt_count].tile0 : xt_total;
+    }
+    // xtw tiles per task depend on the widest task of the launch (128 vs 64
+    // columns per work-group); the prefixes above assumed the widest over all
+    // tasks, so a prefix whose own width selects the other kernel repacks
+    for (size_t na = 1; na <= order_.size(); ++na) {
+      StepLaunch& L = steps_[na];
+      const bool all_mfma = mn >= kMfmaMinN;
+      const bool pre_mfma = L.xt_maxn >= kMfmaMinN;
+      L.xt_list = 0;
+      if (all_mfma != pre_mfma) {
+        std::vector<XtwTask<T>> sub(xt.begin(), xt.begin() + L.xt_count);
+        int m2 = 0;
+        L.xt_tiles = pack_xtw(sub.data(), int(sub.size()), &m2);
+        extra_xt_.push_back(std::make_unique<Buf>());
+        extra_xt_.back()->alloc(sizeof(XtwTask<T>) * sub.size());
+        FPLS_HIP(hipMemcpy(extra_xt_.back()->as<XtwTask<T>>(), sub.data(),
+                           sizeof(XtwTask<T>) * sub.size(), hipMemcpyHostToDevice));
+        L.xt_list = int(extra_xt_.size());
       }
     }
-    xw_tiles_ = pack_xw(xw.data(), int(xw.size()), &xw_maxn_);
-    xt_tiles_ = pack_xtw(xt.data(), int(xt.size()), &xt_maxn_);
-    xw_count_ = int(xw.size());
-    xt_count_ = int(xt.size());
     xw_tasks_.alloc(sizeof(XwTask<T>) * xw.size());
     xt_tasks_.alloc(sizeof(XtwTask<T>) * xt.size());
     FPLS_HIP(hipMemcpy(xw_tasks_.as<XwTask<T>>(), xw.data(), sizeof(XwTask<T>) * xw.size(), hipMemcpyHostToDevice));
     FPLS_HIP(hipMemcpy(xt_tasks_.as<XtwTask<T>>(), xt.data(), sizeof(XtwTask<T>) * xt.size(), hipMemcpyHostToDevice));
+    if (!sums.empty()) {
+      sums_.alloc(sizeof(SlabSum) * sums.size());
+      FPLS_HIP(hipMemcpy(sums_.as<SlabSum>(), sums.data(), sizeof(SlabSum) * sums.size(), hipMemcpyHostToDevice));
+    }
+    // preprocessing task list: column statistics + cross-products of every group
+    for (const int g : order_) {
+      Group& G = gplan_[g];
+      for (int c0 = 0; c0 < G.qcat; c0 += 64) {
+        const int nb = std::min(64, G.qcat - c0);
+        XtwTask<T> t{};
+        t.A = G.X; t.m = G.n; t.k = G.p; t.W = G.Yc + (size_t)c0 * G.n; t.ldw = G.n; t.N = nb;
+        t.mode = kShiftStats; t.splits = G.splits;
+        t.out = G.spart + (size_t)c0 * G.p; t.ldo = G.p; t.slab = (long long)G.p * G.qcat;
+        t.stats = c0 == 0 ? G.stats : nullptr;
+        t.frag = take_frag(g, xtw_frag_bytes<T>(G.n, maxn_pre));
+        pre_.push_back(t);
+      }
+    }
+    pre_tiles_ = pack_xtw(pre_.data(), int(pre_.size()), &pre_maxn_);
+    pre_dev_.alloc(sizeof(XtwTask<T>) * pre_.size());
+    FPLS_HIP(hipMemcpy(pre_dev_.as<XtwTask<T>>(), pre_.data(), sizeof(XtwTask<T>) * pre_.size(), hipMemcpyHostToDevice));
   }
 
   void run() {
@@ -373,28 +504,11 @@ class SimplsBatch {
           launch_gather_rows<T>(y, G.n, F.q, in.perm, G.n, F.yperm, stream_);
           y = F.yperm;
         }
-        launch_center_response<T>(y, G.n, F.q, F.my, yc, G.ysum + F.off, stream_);
+        launch_center_response<T>(y, G.n, F.q, F.my, yc, G.ysum + F.off, F.center, stream_);
       }
     }
     // one launch: column statistics + cross-products of every group
-    std::vector<XtwTask<T>> pre;
-    for (Group& G : gplan_) {
-      for (int c0 = 0; c0 < G.qcat; c0 += 64) {
-        const int nb = std::min(64, G.qcat - c0);
-        XtwTask<T> t{};
-        t.A = G.X; t.m = G.n; t.k = G.p; t.W = G.Yc + (size_t)c0 * G.n; t.ldw = G.n; t.N = nb;
-        t.mode = kShiftStats; t.splits = G.splits;
-        t.out = G.spart + (size_t)c0 * G.p; t.ldo = G.p; t.slab = (long long)G.p * G.qcat;
-        t.stats = c0 == 0 ? G.stats : nullptr;
-        pre.push_back(t);
-      }
-    }
-    int maxn = 0;
-    const long long tiles = pack_xtw(pre.data(), int(pre.size()), &maxn);
-    Buf pre_dev;
-    pre_dev.alloc(sizeof(XtwTask<T>) * pre.size());
-    FPLS_HIP(hipMemcpy(pre_dev.as<XtwTask<T>>(), pre.data(), sizeof(XtwTask<T>) * pre.size(), hipMemcpyHostToDevice));
-    launch_xtw<T>(pre_dev.as<XtwTask<T>>(), int(pre.size()), tiles, maxn, stream_);
+    launch_xtw<T>(pre_dev_.as<XtwTask<T>>(), int(pre_.size()), pre_tiles_, pre_maxn_, stream_);
     for (Group& G : gplan_)
       launch_finalize_crossprod<T>(G.X, G.n, G.p, G.qcat, scaling_, G.stats, G.spart, G.splits,
                                    G.ysum, G.mx, G.sx, G.colss, G.S, G.S0, G.flags, stream_);
@@ -415,17 +529,25 @@ class SimplsBatch {
     }
     Buf gram_dev;
     if (!gram.empty()) {
+      int maxn = 0;
       const long long gt = pack_xtw(gram.data(), int(gram.size()), &maxn);
       gram_dev.alloc(sizeof(XtwTask<double>) * gram.size());
       FPLS_HIP(hipMemcpy(gram_dev.as<XtwTask<double>>(), gram.data(), sizeof(XtwTask<double>) * gram.size(), hipMemcpyHostToDevice));
       launch_xtw<double>(gram_dev.as<XtwTask<double>>(), int(gram.size()), gt, maxn, stream_);
     }
-    // component steps: four launches per step for the whole batch
+    // component steps: refresh, score pass, loading pass, update over the
+    // groups still active at the step (a prefix of the step order)
     const int nf = int(plan_.size());
     for (int s = 0; s < max_steps_; ++s) {
+      int na = 0;
+      while (na < int(order_.size()) && gplan_[order_[na]].max_steps > s) ++na;
+      const StepLaunch& L = steps_[na];
       launch_simpls_refresh(devfits_, nf, s, stream_);
-      launch_xw<T>(xw_tasks_.as<XwTask<T>>(), xw_count_, xw_tiles_, xw_maxn_, stream_);
-      launch_xtw<T>(xt_tasks_.as<XtwTask<T>>(), xt_count_, xt_tiles_, xt_maxn_, stream_);
+      launch_xw<T>(xw_tasks_.as<XwTask<T>>(), L.xw_count, L.xw_tiles, L.xw_maxn, stream_);
+      if (L.sum_count > 0) launch_sum_slabs_list(sums_.as<SlabSum>(), L.sum_count, L.sum_max, stream_);
+      const XtwTask<T>* xl = L.xt_list ? extra_xt_[L.xt_list - 1]->template as<XtwTask<T>>()
+                                       : xt_tasks_.as<XtwTask<T>>();
+      launch_xtw<T>(xl, L.xt_count, L.xt_tiles, L.xt_maxn, stream_);
       launch_simpls_update(devfits_, nf, s, stream_);
     }
     for (size_t f = 0; f < plan_.size(); ++f) {
@@ -433,7 +555,6 @@ class SimplsBatch {
       const Group& G = gplan_[fits_[f].group];
       launch_div_rows(F.R, G.sx, G.p, F.A, F.Rs, stream_);
     }
-    FPLS_HIP(hipStreamSynchronize(stream_));
   }
 
   void collect() {
@@ -449,18 +570,27 @@ class SimplsBatch {
     }
   }
 
+  struct StepLaunch {
+    int xw_count = 0, xt_count = 0, sum_count = 0, xw_maxn = 0, xt_maxn = 0, xt_list = 0;
+    long long xw_tiles = 0, xt_tiles = 0, sum_max = 0;
+  };
+
   std::vector<GroupIn> groups_;
   std::vector<FitIn> fits_;
   std::vector<Group> gplan_;
   std::vector<Fit> plan_;
+  std::vector<int> order_;
+  std::vector<StepLaunch> steps_;
   int scaling_, oversample_, power_;
   int max_steps_ = 0;
   Arena arena_;
   size_t devfits_off_ = 0;
   SimplsFitDev* devfits_ = nullptr;
-  Buf xw_tasks_, xt_tasks_;
-  long long xw_tiles_ = 0, xt_tiles_ = 0;
-  int xw_maxn_ = 0, xt_maxn_ = 0, xw_count_ = 0, xt_count_ = 0;
+  Buf xw_tasks_, xt_tasks_, sums_, pre_dev_, proj_frag_;
+  std::vector<std::unique_ptr<Buf>> extra_xt_;
+  std::vector<XtwTask<T>> pre_;
+  long long pre_tiles_ = 0;
+  int pre_maxn_ = 0;
   hipStream_t stream_ = nullptr;
   double seconds_ = 0.0;
   std::vector<int> completed_, flags_;
@@ -628,8 +758,7 @@ void simpls_cv_regression(const T* predictors, const T* responses, const int* fo
       if (store_predictions)
         FPLS_HIP(hipMemcpyAsync(host_pred.data() + (size_t)j * nte * q, pred.d(),
                                 sizeof(double) * (size_t)nte * q, hipMemcpyDeviceToHost, bs));
-      const int zero = 0;
-      FPLS_HIP(hipMemcpyAsync(flag.as<int>(), &zero, sizeof(int), hipMemcpyHostToDevice, bs));
+      launch_fill_int(flag.as<int>(), 1, 0, bs);
       launch_response_sums<T>(pred.d(), yte[f]->as<T>(), nullptr, batch.y_mean(i), nte, q,
                               sums.d(), flag.as<int>(), bs);
       FPLS_HIP(hipMemcpyAsync(host_metric.data() + (size_t)j * 3 * q, sums.d(),

@@ -23,7 +23,25 @@ inline void hip_check(hipError_t status, const char* what) {
 #define FPLS_LAUNCHED(name) ::fpls::hip_check(hipGetLastError(), name)
 
 // ---------------------------------------------------------------- device
+// Address-space qualifiers exist for the device pass only; the host pass
+// merely parses kernel bodies.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define FPLS_GLOBAL __attribute__((address_space(1)))
+#define FPLS_CONSTANT __attribute__((address_space(4)))
+#else
+#define FPLS_GLOBAL
+#define FPLS_CONSTANT
+#endif
+
 #if defined(__HIP_DEVICE_COMPILE__) || defined(__HIPCC__)
+
+// Element `index` of a device array that no kernel writes while it runs,
+// copied into registers through the constant address space (scalar loads), so
+// later field reads are not generic memory accesses.
+template<class T>
+__device__ __forceinline__ T load_const(const T* array, long long index) {
+  return ((const T FPLS_CONSTANT*)array)[index];
+}
 
 __device__ inline double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

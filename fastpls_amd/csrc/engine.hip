@@ -445,7 +445,8 @@ class Model final : public Handle {
 
     SimplsFitDev f{};
     f.n = n_; f.p = p_; f.q = q_; f.A = A_;
-    f.power = power_; f.use_right_gram = right_gram ? 1 : 0; f.reorth = 0;
+    f.power = power_; f.use_right_gram = right_gram ? 1 : 0;
+    f.reorth = sizeof(T) == 4 ? 1 : 0;  // src/r_api.cpp:1416: two passes for float
     f.nsteps = int(steps.size()); f.bmax = bmax; f.ts_splits = xs; f.ls_splits = ls;
     f.ls_slab = (long long)p_ * bmax;
     f.steps = steps_.as<RefreshStep>(); f.omega = omega_.d();

@@ -139,6 +139,14 @@ void launch_variance(const double* cross, const double* gram, const double* cols
                      int p, int a, double* loadings, double* ss, double* scratch,
                      hipStream_t s);
 
+// p[0..count) = value (a kernel, so streams stay graph-capturable).
+void launch_fill_int(int* p, int count, int value, hipStream_t s);
+
+// Row gather: dst (count x cols) = src[idx, :] for a column-major src (n x cols).
+template<class T>
+void launch_gather_rows(const T* src, long long n, long long cols, const int* idx, long long count,
+                        T* dst, hipStream_t s);
+
 // Synthetic latent-factor data on the device (Philox4x32-10 + Box-Muller).
 // round32 rounds every value to FP32 before it is stored (in T).
 template<class T>

@@ -156,6 +156,18 @@ __global__ __launch_bounds__(kWG) void k_finalize_cross(
   if (!isfinite(v)) atomicOr(flags, kFlagNonFinite);
 }
 
+// dst (count x cols) = src[idx[i], j] for a column-major src with n rows.
+template<class T>
+__global__ __launch_bounds__(kWG) void k_gather_rows(const T* src, long long n, long long cols,
+                                                     const int* idx, long long count, T* dst) {
+  const long long total = count * cols;
+  for (long long e = (long long)blockIdx.x * kWG + threadIdx.x; e < total;
+       e += (long long)gridDim.x * kWG) {
+    const long long i = e % count, j = e / count;
+    dst[e] = src[(long long)idx[i] + j * n];
+  }
+}
+
 __global__ __launch_bounds__(kWG) void k_div_rows(const double* in, const double* d,
                                                   long long rows, int cols, double* out) {
   const long long total = rows * cols;
@@ -356,6 +368,12 @@ void launch_center_response(const T* Y, long long n, int q, double* mean, double
   FPLS_LAUNCHED("k_center_fold");
 }
 
+void launch_fill_int(int* p, int count, int value, hipStream_t s) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_fill_int, dim3(grid_for(count)), dim3(kWG), 0, s, p, count, value);
+  FPLS_LAUNCHED("k_fill_int");
+}
+
 void launch_label_response(const int* labels, long long n, int K, int* counts,
                            double* priors, double* Yc, double* H, double* ysum, int* flags,
                            hipStream_t s) {
@@ -381,6 +399,21 @@ void launch_finalize_crossprod(const T* X, long long n, int p, int q, int scalin
                        splits, ysum, mean, scale, S, S0, flags);
   FPLS_LAUNCHED("k_finalize_cross");
 }
+
+template<class T>
+void launch_gather_rows(const T* src, long long n, long long cols, const int* idx,
+                        long long count, T* dst, hipStream_t s) {
+  if (count <= 0 || cols <= 0) return;
+  hipLaunchKernelGGL((k_gather_rows<T>), dim3(grid_for(count * cols)), dim3(kWG), 0, s, src, n,
+                     cols, idx, count, dst);
+  FPLS_LAUNCHED("k_gather_rows");
+}
+template void launch_gather_rows<float>(const float*, long long, long long, const int*, long long,
+                                        float*, hipStream_t);
+template void launch_gather_rows<double>(const double*, long long, long long, const int*,
+                                         long long, double*, hipStream_t);
+template void launch_gather_rows<int>(const int*, long long, long long, const int*, long long,
+                                      int*, hipStream_t);
 
 void launch_div_rows(const double* in, const double* d, long long rows, int cols,
                      double* out, hipStream_t s) {

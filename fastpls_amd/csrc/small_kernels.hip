@@ -59,7 +59,7 @@ __device__ inline void set_flag(int* state, int flag) {
 // simpls.hpp:226-368 (refresh_directions), right-Gram and plain branches.
 __global__ __launch_bounds__(kWG) void k_simpls_refresh(const SimplsFitDev* fits, int count,
                                                         int step) {
-  const SimplsFitDev& f = fits[blockIdx.x];
+  const SimplsFitDev f = load_const(fits, blockIdx.x);
   if (step >= f.nsteps) return;
   if (f.state[1] != 0) return;
   const RefreshStep st = f.steps[step];
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(kWG) void k_simpls_refresh(const SimplsFitDev* fits
 // simpls.hpp:588-756: per candidate of the block, in order.
 __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits, int count,
                                                        int step) {
-  const SimplsFitDev& f = fits[blockIdx.x];
+  const SimplsFitDev f = load_const(fits, blockIdx.x);
   if (step >= f.nsteps) return;
   if (f.state[1] != 0) return;
   const RefreshStep st = f.steps[step];

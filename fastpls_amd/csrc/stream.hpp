@@ -95,26 +95,39 @@ inline int mfma_width(int maxn) { return maxn <= 16 ? 16 : maxn <= 32 ? 32 : max
 
 // Bytes of the operand workspace (XwTask::frag / XtwTask::frag) a task needs
 // when the widest task of its launch has maxn right-hand columns; 0 means the
-// launch does not read it. For FP32 A the matrix-core kernels run in bf16x3
-// split precision and W is split once per launch into bf16 high and low parts
-// laid out in fragment order:
-//   xw : [hi, lo][NB][k rounded up to 32] (W transposed) + FP32 centre [k32]
-//   xtw: [hi, lo][NB][m rounded up to 64], rows permuted within 64-row tiles
+// launch does not read it. For FP32 A the matrix-core kernels run in bf16x6
+// split precision (three bf16 terms per value, six products) and W is split
+// once per launch into its high, middle and low bf16 parts, laid out in
+// fragment order:
+//   xw : [hi, mid, lo][NB][k rounded up to 64] (W transposed) + FP32 centre [k64]
+//   xtw: [hi, mid, lo][NB][m rounded up to 64], rows permuted within 64-row tiles
+constexpr int kSplitTerms = 3;
 template<class T>
 inline size_t xw_frag_bytes(long long k, int maxn) {
   if (sizeof(T) != 4 || maxn < kMfmaMinN) return 0;
-  const size_t k32 = size_t((k + 31) / 32 * 32);
-  return k32 * (size_t)mfma_width(maxn) * 4 + k32 * 4;
+  const size_t k64 = size_t((k + 63) / 64 * 64);
+  return k64 * (size_t)mfma_width(maxn) * 2 * kSplitTerms + k64 * 4;
 }
 template<class T>
 inline size_t xtw_frag_bytes(long long m, int maxn) {
   if (sizeof(T) != 4 || maxn < kMfmaMinN) return 0;
   const size_t m64 = size_t((m + 63) / 64 * 64);
-  return m64 * (size_t)mfma_width(maxn) * 4;
+  return m64 * (size_t)mfma_width(maxn) * 2 * kSplitTerms;
 }
+
 
 // Deterministic sum of `splits` slabs: out[i] = sum_s part[s * stride + i].
 void launch_sum_slabs(const double* part, int splits, long long stride,
                       long long count, double* out, hipStream_t stream);
+
+// The same over a device list of independent slab sums (one launch).
+struct SlabSum {
+  const double* part;
+  double* out;
+  long long stride, count;
+  int splits;
+};
+void launch_sum_slabs_list(const SlabSum* device_list, int count, long long max_count,
+                           hipStream_t stream);
 
 }  // namespace fpls

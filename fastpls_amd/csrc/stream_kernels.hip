@@ -4,9 +4,11 @@
 // meant to run at HBM bandwidth; W operands come from L2 (directly or through
 // LDS). Three arithmetic paths:
 //   * VALU, FP64 FMA on values widened from T (right-hand sides below 8);
-//   * FP32 A on the matrix cores in bf16x3 split precision:
-//     x·w ~ xh·wh + xh·wl + xl·wh with xh = bf16(x), xl = bf16(x - xh), FP32
-//     accumulation over at most 4096 rows / columns, FP64 partial slabs;
+//   * FP32 A on the matrix cores in bf16x6 split precision: x = xh + xm + xl
+//     and w = wh + wm + wl (three bf16 terms each, residual 2^-24), products
+//     hh + hm + mh + hl + mm + lh (dropped terms of order 2^-24), each
+//     32-long reduction step in one FP32 matrix-core chain folded into FP64
+//     accumulators, FP64 partial slabs;
 //   * FP64 A on the FP64 matrix cores.
 // Kernel arguments are read once into registers (task lists through the
 // constant address space) and A is read through global-address-space
@@ -19,15 +21,6 @@ namespace fpls {
 
 namespace {
 
-// Address-space qualifiers exist for the device pass only; the host pass
-// merely parses the kernel bodies.
-#if defined(__HIP_DEVICE_COMPILE__)
-#define FPLS_GLOBAL __attribute__((address_space(1)))
-#define FPLS_CONSTANT __attribute__((address_space(4)))
-#else
-#define FPLS_GLOBAL
-#define FPLS_CONSTANT
-#endif
 
 template<class T>
 __device__ __forceinline__ const T FPLS_GLOBAL* gp(const T* p) { return (const T FPLS_GLOBAL*)p; }
@@ -260,49 +253,71 @@ __device__ __forceinline__ unsigned pack_bf16(float a, float b) {
   return __builtin_bit_cast(unsigned, v);
 }
 
-// hi = bf16(x) and lo = bf16(x - hi), both rounded to nearest even: hi + lo
-// holds x to within 2^-16 |x|. Two values per 32-bit word, first in the low half.
-__device__ __forceinline__ void split_pair(float a, float b, unsigned& hi, unsigned& lo) {
+// Three-term bf16 split, each term rounded to nearest even: x = hi + mid + lo
+// + r with |r| <= 2^-24 |x| (hi carries the leading 8 significant bits, mid
+// the next 8, lo the next 8; both residuals are exact in FP32). Two values per
+// 32-bit word, the first in the low half.
+__device__ __forceinline__ void split_pair(float a, float b, unsigned& hi, unsigned& mid,
+                                           unsigned& lo) {
   hi = pack_bf16(a, b);
-  const float ha = __builtin_bit_cast(float, hi << 16);
-  const float hb = __builtin_bit_cast(float, hi & 0xffff0000u);
-  lo = pack_bf16(a - ha, b - hb);
+  const float ra = a - __builtin_bit_cast(float, hi << 16);
+  const float rb = b - __builtin_bit_cast(float, hi & 0xffff0000u);
+  mid = pack_bf16(ra, rb);
+  const float sa = ra - __builtin_bit_cast(float, mid << 16);
+  const float sb = rb - __builtin_bit_cast(float, mid & 0xffff0000u);
+  lo = pack_bf16(sa, sb);
 }
 
-__device__ __forceinline__ void split8(const float (&v)[8], u32x4& hi, u32x4& lo) {
-  unsigned h0, h1, h2, h3, l0, l1, l2, l3;
-  split_pair(v[0], v[1], h0, l0);
-  split_pair(v[2], v[3], h1, l1);
-  split_pair(v[4], v[5], h2, l2);
-  split_pair(v[6], v[7], h3, l3);
-  hi = u32x4{h0, h1, h2, h3};
-  lo = u32x4{l0, l1, l2, l3};
+struct Frag3 {
+  u32x4 h, m, l;
+};
+
+__device__ __forceinline__ Frag3 split8(const float (&v)[8]) {
+  unsigned h[4], m[4], l[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) split_pair(v[2 * i], v[2 * i + 1], h[i], m[i], l[i]);
+  return Frag3{u32x4{h[0], h[1], h[2], h[3]}, u32x4{m[0], m[1], m[2], m[3]},
+               u32x4{l[0], l[1], l[2], l[3]}};
 }
 
-// acc += a·b in bf16x3: the two cross terms first, then the leading term.
-__device__ __forceinline__ f32x4 mma3(const u32x4& ah, const u32x4& al, const u32x4& bh,
+__device__ __forceinline__ f32x4 mma_bf16(const u32x4& a, const u32x4& b, f32x4 acc) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                 __builtin_bit_cast(bf16x8, b), acc, 0, 0, 0);
+}
+
+// acc += a·b to FP32 accuracy from six bf16 products (bf16x6): the terms of
+// order 2^-16 (hi·lo, mid·mid, lo·hi) first, then those of order 2^-8
+// (hi·mid, mid·hi), then hi·hi; the dropped terms are of order 2^-24.
+__device__ __forceinline__ f32x4 mma6(const Frag3& a, const u32x4& bh, const u32x4& bm,
                                       const u32x4& bl, f32x4 acc) {
-  const bf16x8 AH = __builtin_bit_cast(bf16x8, ah), AL = __builtin_bit_cast(bf16x8, al);
-  const bf16x8 BH = __builtin_bit_cast(bf16x8, bh), BL = __builtin_bit_cast(bf16x8, bl);
-  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(AL, BH, acc, 0, 0, 0);
-  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(AH, BL, acc, 0, 0, 0);
-  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(AH, BH, acc, 0, 0, 0);
+  acc = mma_bf16(a.h, bl, acc);
+  acc = mma_bf16(a.m, bm, acc);
+  acc = mma_bf16(a.l, bh, acc);
+  acc = mma_bf16(a.h, bm, acc);
+  acc = mma_bf16(a.m, bh, acc);
+  return mma_bf16(a.h, bh, acc);
+}
+
+// FP64 accumulation of one short FP32 matrix-core chain.
+__device__ __forceinline__ void fold(f64x4& acc, const f32x4& c) {
+  acc[0] += double(c[0]); acc[1] += double(c[1]); acc[2] += double(c[2]); acc[3] += double(c[3]);
 }
 
 __device__ __forceinline__ f64x4 mma64(double a, double b, f64x4 acc) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
 }
 
-// ---------------------------------------------- bf16x3 operand splitting
-// xw operand: W (k x N) -> hi/lo [2][NB][k32] in W's own column order, plus
-// the centre in FP32 (zero past k and past N). Grid: (blocks, tasks).
+// ---------------------------------------------- bf16x6 operand splitting
+// xw operand: W (k x N) -> hi/mid/lo [3][NB][k64] in W's own column order,
+// plus the centre in FP32 (zero past k and past N). Grid: (blocks, tasks).
 template<int NB>
 __global__ __launch_bounds__(kWG) void k_xw_split(const XwTask<float>* tasks, XwTask<float> single) {
   const XwTask<float> t = task_at(tasks, blockIdx.y, single);
-  const long long k32 = (t.k + 31) / 32 * 32;
+  const long long k32 = (t.k + 63) / 64 * 64;  // padded to 64 columns
   unsigned short FPLS_GLOBAL* hi = (unsigned short FPLS_GLOBAL*)t.frag;
-  unsigned short FPLS_GLOBAL* lo = hi + (long long)NB * k32;
-  float FPLS_GLOBAL* cf = (float FPLS_GLOBAL*)(hi + 2LL * NB * k32);
+  unsigned short FPLS_GLOBAL* mid = hi + (long long)NB * k32;
+  unsigned short FPLS_GLOBAL* lo = mid + (long long)NB * k32;
+  float FPLS_GLOBAL* cf = (float FPLS_GLOBAL*)(hi + 3LL * NB * k32);
   const double FPLS_GLOBAL* W = gp(t.W);
   const long long stride = (long long)gridDim.x * kWG;
   // two adjacent elements per thread
@@ -310,16 +325,17 @@ __global__ __launch_bounds__(kWG) void k_xw_split(const XwTask<float>* tasks, Xw
     const long long n = e / k32, j = e % k32;  // k32 is even: j, j + 1 share n
     const float a = (n < t.N && j < t.k) ? float(W[j + n * t.ldw]) : 0.f;
     const float b = (n < t.N && j + 1 < t.k) ? float(W[j + 1 + n * t.ldw]) : 0.f;
-    unsigned h, l;
-    split_pair(a, b, h, l);
+    unsigned h, md, l;
+    split_pair(a, b, h, md, l);
     *(unsigned FPLS_GLOBAL*)(hi + e) = h;
+    *(unsigned FPLS_GLOBAL*)(mid + e) = md;
     *(unsigned FPLS_GLOBAL*)(lo + e) = l;
   }
   for (long long j = (long long)blockIdx.x * kWG + threadIdx.x; j < k32; j += stride)
     cf[j] = (t.center != nullptr && j < t.k) ? float(gp(t.center)[j]) : 0.f;
 }
 
-// xtw operand: W (m x N) -> hi/lo [2][NB][m64]; within each 64-row tile,
+// xtw operand: W (m x N) -> hi/mid/lo [3][NB][m64]; within each 64-row tile,
 // position 32h + 8g + j holds row 16(2h + j/4) + 4g + j%4, so that lane group
 // g of reduction step h reads its eight values as one 16-byte chunk.
 template<int NB>
@@ -327,7 +343,8 @@ __global__ __launch_bounds__(kWG) void k_xtw_split(const XtwTask<float>* tasks, 
   const XtwTask<float> t = task_at(tasks, blockIdx.y, single);
   const long long m64 = (t.m + 63) / 64 * 64;
   unsigned short FPLS_GLOBAL* hi = (unsigned short FPLS_GLOBAL*)t.frag;
-  unsigned short FPLS_GLOBAL* lo = hi + (long long)NB * m64;
+  unsigned short FPLS_GLOBAL* mid = hi + (long long)NB * m64;
+  unsigned short FPLS_GLOBAL* lo = mid + (long long)NB * m64;
   const double FPLS_GLOBAL* W = gp(t.W);
   const long long stride = (long long)gridDim.x * kWG;
   for (long long e = ((long long)blockIdx.x * kWG + threadIdx.x) * 2; e < (long long)NB * m64; e += 2 * stride) {
@@ -341,26 +358,35 @@ __global__ __launch_bounds__(kWG) void k_xtw_split(const XtwTask<float>* tasks, 
       const long long row = tile * 64 + 16 * (2 * h + (j >> 2)) + 4 * g + (j & 3);
       v[u] = (n < t.N && row < t.m) ? float(W[row + n * t.ldw]) : 0.f;
     }
-    unsigned h, l;
-    split_pair(v[0], v[1], h, l);
+    unsigned h, md, l;
+    split_pair(v[0], v[1], h, md, l);
     *(unsigned FPLS_GLOBAL*)(hi + e) = h;
+    *(unsigned FPLS_GLOBAL*)(mid + e) = md;
     *(unsigned FPLS_GLOBAL*)(lo + e) = l;
   }
 }
 
-// --------------------------------------------------- xtw, bf16x3 (FP32 A)
-// A work-group owns 128 columns of A (32 per wave: two 16-column M-blocks)
-// and one row split. Each 64-row step: lane (c, g) = (lane % 16, lane / 16)
-// holds rows 16v + 4g + e (v, e < 4) of its column of each M-block, loaded as
-// float4; reduction step h pairs rows {16(2h + j/4) + 4g + j%4} with the same
-// rows of W, whose split fragments are staged in LDS (double-buffered). The
-// next step's A and W loads are issued before the current step's arithmetic.
+// --------------------------------------------------- xtw, bf16x6 (FP32 A)
+// A work-group of 512 threads owns 128 columns of A (16 per wave: one
+// 16-column M-block) and one row split, walked in 64-row steps. Lane (c, g)
+// = (lane % 16, lane / 16) holds rows 16v + 4g + e (v, e < 4) of its
+// column, loaded as float4; reduction sub-step h (32 rows) pairs rows
+// {16(2h + j/4) + 4g + j%4} with the same rows of W, whose split fragments
+// are staged in LDS (double-buffered, shared by the eight waves). The next
+// step's A and W loads are issued before the current step's arithmetic.
+// Each sub-step's six products run in a fresh FP32 chain that is folded into
+// FP64 accumulators.
 constexpr int kXtwCols = 128;
+constexpr int kBf6Threads = 512;
 
 template<int NB>
-__global__ __launch_bounds__(kWG) void k_xtw_bf3(const XtwTask<float>* tasks, int count,
-                                                 XtwTask<float> single) {
+__global__ __launch_bounds__(kBf6Threads) void k_xtw_bf6(const XtwTask<float>* tasks, int count,
+                                                         XtwTask<float> single) {
   constexpr int NBLK = NB / 16;
+  constexpr int ROWS = 64;                                     // rows per step
+  constexpr int CPR = ROWS / 8;                                // 16-byte chunks per W row
+  constexpr int WCH = 3 * NB * CPR;                            // chunks of one step
+  constexpr int WPT = (WCH + kBf6Threads - 1) / kBf6Threads;   // chunks per thread
   const XtwTask<float> t = load_task(tasks, count, single);
   const long long ctiles = (t.k + kXtwCols - 1) / kXtwCols;
   const long long total = ctiles * t.splits;
@@ -374,121 +400,106 @@ __global__ __launch_bounds__(kWG) void k_xtw_bf3(const XtwTask<float>* tasks, in
   const long long r1 = r0 + per < t.m ? r0 + per : t.m;
   const long long m64 = (t.m + 63) / 64 * 64;
 
-  // [buffer][hi rows 0..NB-1, lo rows NB..2NB-1][eight 16-byte chunks + pad]
-  __shared__ u32x4 wl[2][2 * NB][9];
+  // [buffer][hi rows 0..NB-1, mid NB..2NB-1, lo 2NB..3NB-1][8 chunks + pad]
+  __shared__ u32x4 wl[2][3 * NB][CPR + 1];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int m16 = lane & 15, g = lane >> 4;
   const bool vec = (t.m & 3) == 0 && (((unsigned long long)t.A) & 15) == 0;
   const bool stats = t.mode == kShiftStats;
   // Fast loads: a wave-uniform base (column bc, advanced per step) plus one
-  // 32-bit lane offset per M-block; columns past k re-read column bc.
-  const long long cw = col0 + wave * 32;
+  // 32-bit lane offset; columns past k re-read column bc.
+  const long long cw = col0 + wave * 16;
   const long long bc = cw < t.k ? cw : t.k - 1;
   const char FPLS_GLOBAL* xbase = (const char FPLS_GLOBAL*)(gp(t.A) + bc * t.m);
-  unsigned xoff[2];
-
-  const float FPLS_GLOBAL* xcol[2];
-  float centre[2];
-  bool cvalid[2];
-#pragma unroll
-  for (int mb = 0; mb < 2; ++mb) {
-    const long long col = col0 + wave * 32 + mb * 16 + m16;
-    cvalid[mb] = col < t.k;
-    xcol[mb] = gp(t.A) + (cvalid[mb] ? col : 0) * t.m;
-    xoff[mb] = unsigned(((cvalid[mb] ? col - bc : 0) * t.m + 4 * g) * 4);
-    float c = 0.f;
-    if (cvalid[mb]) {
-      if (t.mode == kCentered && t.center != nullptr) c = float(gp(t.center)[col]);
-      else if (stats) c = xcol[mb][0];
-    }
-    centre[mb] = c;
+  const long long col = cw + m16;
+  const bool cvalid = col < t.k;
+  const float FPLS_GLOBAL* xcol = gp(t.A) + (cvalid ? col : 0) * t.m;
+  const unsigned xoff = unsigned(((cvalid ? col - bc : 0) * t.m + 4 * g) * 4);
+  float centre = 0.f;
+  if (cvalid) {
+    if (t.mode == kCentered && t.center != nullptr) centre = float(gp(t.center)[col]);
+    else if (stats) centre = xcol[0];
   }
   const char FPLS_GLOBAL* frag = (const char FPLS_GLOBAL*)t.frag;
-  // W chunk c = threadIdx.x + s * kWG of a step: fragment row c / 8, 16 bytes
-  // at 8 (c % 8); row s * kWG / 8 + threadIdx.x / 8 of the [2 NB][m64] array.
+  // W chunk c = threadIdx.x + s * 512 of a step: fragment row c / 8, 16 bytes
+  // at 8 (c % 8); row 64 s + threadIdx.x / 8 of the [3 NB][m64] array.
+  static_assert(CPR == 8, "one 64-row tile of W per step");
   const unsigned woff = unsigned(((threadIdx.x >> 3) * m64 + 8 * (threadIdx.x & 7)) * 2);
 
-  auto load_w = [&](u32x4 (&wr)[NBLK], long long i0) {
+  auto load_w = [&](u32x4 (&wr)[WPT], long long i0) {
 #pragma unroll
-    for (int s = 0; s < NBLK; ++s) {
-      const char FPLS_GLOBAL* b = frag + ((long long)s * (kWG / 8) * m64 + i0) * 2;
-      wr[s] = *(const u32x4 FPLS_GLOBAL*)(b + woff);
-    }
-  };
-  auto store_w = [&](const u32x4 (&wr)[NBLK], int b) {
-#pragma unroll
-    for (int s = 0; s < NBLK; ++s) {
-      const int c = threadIdx.x + s * kWG;
-      wl[b][c >> 3][c & 7] = wr[s];
-    }
-  };
-  // Full 64-row steps of 16-byte-aligned columns: four float4 per M-block.
-  auto load_x = [&](float (&x)[2][16], long long i0) {
-#pragma unroll
-    for (int mb = 0; mb < 2; ++mb)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const float4 f = *(const float4 FPLS_GLOBAL*)(xbase + i0 * 4 + 64 * v + xoff[mb]);
-        x[mb][4 * v] = f.x; x[mb][4 * v + 1] = f.y; x[mb][4 * v + 2] = f.z; x[mb][4 * v + 3] = f.w;
+    for (int s = 0; s < WPT; ++s) {
+      if (WCH % kBf6Threads == 0 || int(threadIdx.x) + s * kBf6Threads < WCH) {
+        const char FPLS_GLOBAL* b =
+            frag + ((long long)s * (kBf6Threads / CPR) * m64 + i0) * 2;
+        wr[s] = *(const u32x4 FPLS_GLOBAL*)(b + woff);
       }
+    }
+  };
+  auto store_w = [&](const u32x4 (&wr)[WPT], int b) {
+#pragma unroll
+    for (int s = 0; s < WPT; ++s) {
+      const int c = threadIdx.x + s * kBf6Threads;
+      if (WCH % kBf6Threads == 0 || c < WCH) wl[b][c / CPR][c % CPR] = wr[s];
+    }
+  };
+  // Full 64-row steps of 16-byte-aligned columns: four float4.
+  auto load_x = [&](float (&x)[ROWS / 4], long long i0) {
+#pragma unroll
+    for (int v = 0; v < ROWS / 16; ++v) {
+      const float4 f = *(const float4 FPLS_GLOBAL*)(xbase + i0 * 4 + 64 * v + xoff);
+      x[4 * v] = f.x; x[4 * v + 1] = f.y; x[4 * v + 2] = f.z; x[4 * v + 3] = f.w;
+    }
   };
   // Any step: rows past r1 and columns past k load the centre, so that they
   // contribute exactly zero after centring.
-  auto load_x_edge = [&](float (&x)[2][16], long long i0) {
+  auto load_x_edge = [&](float (&x)[ROWS / 4], long long i0) {
 #pragma unroll
-    for (int mb = 0; mb < 2; ++mb)
+    for (int v = 0; v < ROWS / 16; ++v)
 #pragma unroll
-      for (int v = 0; v < 4; ++v)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const long long row = i0 + 16 * v + 4 * g + e;
-          x[mb][4 * v + e] = (cvalid[mb] && row < r1) ? xcol[mb][row] : centre[mb];
-        }
+      for (int e = 0; e < 4; ++e) {
+        const long long row = i0 + 16 * v + 4 * g + e;
+        x[4 * v + e] = (cvalid && row < r1) ? xcol[row] : centre;
+      }
   };
 
-  f32x4 acc[2][NBLK];
+  f64x4 acc[NBLK];
 #pragma unroll
-  for (int mb = 0; mb < 2; ++mb)
-#pragma unroll
-    for (int nb = 0; nb < NBLK; ++nb) acc[mb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
-  double s1[2] = {0.0, 0.0}, s2[2] = {0.0, 0.0};
+  for (int nb = 0; nb < NBLK; ++nb) acc[nb] = f64x4{0.0, 0.0, 0.0, 0.0};
+  double s1 = 0.0, s2 = 0.0;
 
-  auto compute = [&](const float (&x)[2][16], int b) {
-    float p1[2] = {0.f, 0.f}, p2[2] = {0.f, 0.f};
+  auto compute = [&](const float (&x)[ROWS / 4], int b) {
+    float p1 = 0.f, p2 = 0.f;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      u32x4 ah[2], al[2];
+    for (int h = 0; h < ROWS / 32; ++h) {
+      float v[8];
 #pragma unroll
-      for (int mb = 0; mb < 2; ++mb) {
-        float v[8];
+      for (int j = 0; j < 8; ++j) v[j] = x[8 * h + j] - centre;
+      if (stats) {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = x[mb][8 * h + j] - centre[mb];
-        if (stats) {
-#pragma unroll
-          for (int j = 0; j < 8; ++j) { p1[mb] += v[j]; p2[mb] = fmaf(v[j], v[j], p2[mb]); }
-        }
-        split8(v, ah[mb], al[mb]);
+        for (int j = 0; j < 8; ++j) { p1 += v[j]; p2 = fmaf(v[j], v[j], p2); }
       }
+      const Frag3 a = split8(v);
 #pragma unroll
       for (int nb = 0; nb < NBLK; ++nb) {
         const u32x4 bh = wl[b][nb * 16 + m16][4 * h + g];
-        const u32x4 bl = wl[b][NB + nb * 16 + m16][4 * h + g];
-#pragma unroll
-        for (int mb = 0; mb < 2; ++mb) acc[mb][nb] = mma3(ah[mb], al[mb], bh, bl, acc[mb][nb]);
+        const u32x4 bm = wl[b][NB + nb * 16 + m16][4 * h + g];
+        const u32x4 bl = wl[b][2 * NB + nb * 16 + m16][4 * h + g];
+        fold(acc[nb], mma6(a, bh, bm, bl, f32x4{0.f, 0.f, 0.f, 0.f}));
       }
+      // one sub-step's fragment reads at a time (bounded register pressure;
+      // the other waves hide the LDS latency)
+      __builtin_amdgcn_sched_barrier(0);
     }
-    if (stats) {
-#pragma unroll
-      for (int mb = 0; mb < 2; ++mb) { s1[mb] += double(p1[mb]); s2[mb] += double(p2[mb]); }
-    }
+    if (stats) { s1 += double(p1); s2 += double(p2); }
   };
 
   // Branch-free two-stage pipeline over the full steps (the last prefetch
   // repeats a step), LDS buffers alternating with the register buffers.
-  float xa[2][16], xb[2][16];
-  u32x4 wr[NBLK];
-  const long long nfull = vec ? (r1 - r0) / 64 : 0;
+  float xa[ROWS / 4], xb[ROWS / 4];
+  u32x4 wr[WPT];
+  const long long nfull = vec ? (r1 - r0) / ROWS : 0;
   long long s = 0;
   if (nfull > 0) {
     load_w(wr, r0);
@@ -496,12 +507,12 @@ __global__ __launch_bounds__(kWG) void k_xtw_bf3(const XtwTask<float>* tasks, in
     store_w(wr, 0);
     __syncthreads();
     for (; s + 1 < nfull; s += 2) {
-      load_w(wr, r0 + 64 * (s + 1));
-      load_x(xb, r0 + 64 * (s + 1));
+      load_w(wr, r0 + ROWS * (s + 1));
+      load_x(xb, r0 + ROWS * (s + 1));
       compute(xa, 0);
       store_w(wr, 1);
       __syncthreads();
-      const long long nx = r0 + 64 * (s + 2 < nfull ? s + 2 : s + 1);
+      const long long nx = r0 + ROWS * (s + 2 < nfull ? s + 2 : s + 1);
       load_w(wr, nx);
       load_x(xa, nx);
       compute(xb, 1);
@@ -510,8 +521,9 @@ __global__ __launch_bounds__(kWG) void k_xtw_bf3(const XtwTask<float>* tasks, in
     }
     if (s < nfull) { compute(xa, 0); ++s; __syncthreads(); }
   }
-  // the ragged last step (or every step of unaligned columns), unpipelined
-  for (long long i0 = r0 + 64 * s; i0 < r1; i0 += 64) {
+  // the ragged last step (or every step of unaligned columns), unpipelined;
+  // the W operand is padded to m64 rows, so its tile is always whole
+  for (long long i0 = r0 + ROWS * s; i0 < r1; i0 += ROWS) {
     load_w(wr, i0);
     store_w(wr, 0);
     load_x_edge(xa, i0);
@@ -522,46 +534,48 @@ __global__ __launch_bounds__(kWG) void k_xtw_bf3(const XtwTask<float>* tasks, in
 
   double FPLS_GLOBAL* o = gpw(t.out) + (long long)split * t.slab;
 #pragma unroll
-  for (int mb = 0; mb < 2; ++mb)
+  for (int nb = 0; nb < NBLK; ++nb) {
+    const int n = nb * 16 + m16;
 #pragma unroll
-    for (int nb = 0; nb < NBLK; ++nb) {
-      const int n = nb * 16 + m16;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const long long oc = col0 + wave * 32 + mb * 16 + 4 * g + r;
-        if (oc < t.k && n < t.N) o[oc + (long long)n * t.ldo] = double(acc[mb][nb][r]);
-      }
+    for (int r = 0; r < 4; ++r) {
+      const long long oc = cw + 4 * g + r;
+      if (oc < t.k && n < t.N) o[oc + (long long)n * t.ldo] = acc[nb][r];
     }
+  }
   if (stats && t.stats != nullptr) {
-#pragma unroll
-    for (int mb = 0; mb < 2; ++mb) {
-      double a1 = s1[mb], a2 = s2[mb];
-      a1 += __shfl_xor(a1, 16, 64);
-      a1 += __shfl_xor(a1, 32, 64);
-      a2 += __shfl_xor(a2, 16, 64);
-      a2 += __shfl_xor(a2, 32, 64);
-      if (g == 0 && cvalid[mb]) {
-        const long long col = col0 + wave * 32 + mb * 16 + m16;
-        double FPLS_GLOBAL* st = gpw(t.stats) + (long long)split * 2 * t.k;
-        st[col] = a1;
-        st[t.k + col] = a2;
-      }
+    double a1 = s1, a2 = s2;
+    a1 += __shfl_xor(a1, 16, 64);
+    a1 += __shfl_xor(a1, 32, 64);
+    a2 += __shfl_xor(a2, 16, 64);
+    a2 += __shfl_xor(a2, 32, 64);
+    if (g == 0 && cvalid) {
+      double FPLS_GLOBAL* st = gpw(t.stats) + (long long)split * 2 * t.k;
+      st[col] = a1;
+      st[t.k + col] = a2;
     }
   }
 }
 
-// ---------------------------------------------------- xw, bf16x3 (FP32 A)
-// A work-group owns 256 rows (64 per wave) and one column split, walked in
-// 32-column reduction steps. Lane (r, g) = (lane % 16, lane / 16) loads rows
-// 4r..4r+3 of columns 8g..8g+7 of the step (eight float4); MFMA call e uses
-// row 4r + e of those columns as the A fragment of output rows {4r' + e}.
-// W fragments (already split, W's own column order) and the FP32 centre come
-// straight from L2; the next step's loads are issued before the current
-// step's arithmetic. No LDS and no barriers.
+// ---------------------------------------------------- xw, bf16x6 (FP32 A)
+// A work-group of 512 threads owns 256 rows (32 per wave) and one column
+// split, walked in 64-column steps of two 32-column reduction sub-steps.
+// Lane (r, g) = (lane % 16, lane / 16) loads rows 2r, 2r + 1 of columns
+// 32u + 8g..32u + 8g + 7 of sub-step u (eight float2 per sub-step, a full
+// 128-byte line per column and instruction); MFMA call e uses row 2r + e of
+// those columns as the A fragment of output rows {2r' + e}. The W fragments
+// (already split, W's own column order) and the FP32 centre of a step are
+// staged in LDS (double-buffered, shared by the eight waves); the next
+// step's loads are issued before the current step's arithmetic. Each
+// sub-step's products run in a fresh FP32 chain folded into FP64
+// accumulators.
 template<int NB>
-__global__ __launch_bounds__(kWG) void k_xw_bf3(const XwTask<float>* tasks, int count,
-                                                XwTask<float> single) {
+__global__ __launch_bounds__(kBf6Threads) void k_xw_bf6(const XwTask<float>* tasks, int count,
+                                                        XwTask<float> single) {
   constexpr int NBLK = NB / 16;
+  constexpr int COLS = 64;                                     // columns per step
+  constexpr int CPR = COLS / 8;                                // 16-byte chunks per W row
+  constexpr int WCH = 3 * NB * CPR;                            // W chunks of one step
+  constexpr int WPT = (WCH + kBf6Threads - 1) / kBf6Threads;
   const XwTask<float> t = load_task(tasks, count, single);
   const long long local = (long long)blockIdx.x - t.tile0;
   const long long rtiles = xw_row_tiles(t.m);
@@ -570,107 +584,134 @@ __global__ __launch_bounds__(kWG) void k_xw_bf3(const XwTask<float>* tasks, int 
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int m16 = lane & 15, g = lane >> 4;
-  const long long i0w = (local % rtiles) * kWG + wave * 64;
-  const long long rowb = i0w + 4 * m16;
-  const long long per = split_extent(t.k, t.splits, 32);
+  const long long i0w = (local % rtiles) * 256 + wave * 32;
+  const long long rowb = i0w + 2 * m16;
+  const long long per = split_extent(t.k, t.splits, COLS);
   const long long j0 = split * per;
   const long long j1 = j0 + per < t.k ? j0 + per : t.k;
-  const long long k32 = (t.k + 31) / 32 * 32;
-  const bool vec = (t.m & 3) == 0 && (((unsigned long long)t.A) & 15) == 0;
-  const bool rows_full = i0w + 64 <= t.m;
+  const long long k64 = (t.k + 63) / 64 * 64;
+  const bool vec = (t.m & 1) == 0 && (((unsigned long long)t.A) & 7) == 0;
+  const bool rows_full = i0w + 32 <= t.m;
+
+  __shared__ u32x4 wl[2][3 * NB][CPR + 1];
+  __shared__ float cs[2][COLS];
 
   const float FPLS_GLOBAL* xr = gp(t.A) + (rowb < t.m ? rowb : 0);
-  const unsigned short FPLS_GLOBAL* wh = (const unsigned short FPLS_GLOBAL*)t.frag;
-  const unsigned short FPLS_GLOBAL* wlo = wh + (long long)NB * k32;
-  const float FPLS_GLOBAL* cf = (const float FPLS_GLOBAL*)(wh + 2LL * NB * k32);
-  // Fast loads: wave-uniform bases plus one 32-bit lane offset per operand.
+  // Fast loads: wave-uniform bases plus one 32-bit lane offset.
   const long long colbytes = t.m * 4;
   const char FPLS_GLOBAL* xbase = (const char FPLS_GLOBAL*)(gp(t.A) + i0w);
-  const unsigned xoff = unsigned(8 * g * colbytes + 16 * m16);
+  const unsigned xoff = unsigned(8 * g * colbytes + 8 * m16);
   const char FPLS_GLOBAL* fbase = (const char FPLS_GLOBAL*)t.frag;
-  const unsigned woff = unsigned((m16 * k32 + 8 * g) * 2);
-  const unsigned coff = unsigned(4LL * NB * k32 + 32 * g);  // FP32 centre block, lane's 8
+  const float FPLS_GLOBAL* cf = (const float FPLS_GLOBAL*)(fbase + 6LL * NB * k64);
+  // W chunk c = threadIdx.x + s * 512: fragment row c / 8, 16 bytes at 8 (c % 8)
+  const unsigned woff = unsigned(((threadIdx.x >> 3) * k64 + 8 * (threadIdx.x & 7)) * 2);
 
-  struct Step {
-    float x[8][4];
-    float c[8];
-    u32x4 bh[NBLK], bl[NBLK];
-  };
-  // centre and W fragments of the step starting at column jc
-  auto load_wc = [&](Step& s, long long jc) {
-    const char FPLS_GLOBAL* cb = fbase + jc * 4;
-    const float4 c0 = *(const float4 FPLS_GLOBAL*)(cb + coff);
-    const float4 c1 = *(const float4 FPLS_GLOBAL*)(cb + coff + 16);
-    s.c[0] = c0.x; s.c[1] = c0.y; s.c[2] = c0.z; s.c[3] = c0.w;
-    s.c[4] = c1.x; s.c[5] = c1.y; s.c[6] = c1.z; s.c[7] = c1.w;
+  auto load_w = [&](u32x4 (&wr)[WPT], float& cr, long long jc) {
 #pragma unroll
-    for (int nb = 0; nb < NBLK; ++nb) {
-      const char FPLS_GLOBAL* bh = fbase + ((long long)nb * 16 * k32 + jc) * 2;
-      const char FPLS_GLOBAL* bl = bh + (long long)NB * k32 * 2;
-      s.bh[nb] = *(const u32x4 FPLS_GLOBAL*)(bh + woff);
-      s.bl[nb] = *(const u32x4 FPLS_GLOBAL*)(bl + woff);
+    for (int s = 0; s < WPT; ++s) {
+      if (WCH % kBf6Threads == 0 || int(threadIdx.x) + s * kBf6Threads < WCH) {
+        const char FPLS_GLOBAL* b =
+            fbase + ((long long)s * (kBf6Threads / CPR) * k64 + jc) * 2;
+        wr[s] = *(const u32x4 FPLS_GLOBAL*)(b + woff);
+      }
     }
+    if (threadIdx.x < COLS) cr = cf[jc + threadIdx.x];
   };
-  // a full 32-column step of 16-byte-aligned rows
-  auto load = [&](Step& s, long long jc) {
-    const char FPLS_GLOBAL* b = xbase + jc * colbytes;
+  auto store_w = [&](const u32x4 (&wr)[WPT], float cr, int b) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const float4 f = *(const float4 FPLS_GLOBAL*)(b + j * colbytes + xoff);
-      s.x[j][0] = f.x; s.x[j][1] = f.y; s.x[j][2] = f.z; s.x[j][3] = f.w;
+    for (int s = 0; s < WPT; ++s) {
+      const int c = threadIdx.x + s * kBf6Threads;
+      if (WCH % kBf6Threads == 0 || c < WCH) wl[b][c / CPR][c % CPR] = wr[s];
     }
-    load_wc(s, jc);
+    if (threadIdx.x < COLS) cs[b][threadIdx.x] = cr;
+  };
+  // a full 64-column step of 8-byte-aligned row pairs
+  auto load_x = [&](float (&x)[2][8][2], long long jc) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const char FPLS_GLOBAL* b = xbase + (jc + 32 * u) * colbytes;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const float2 f = *(const float2 FPLS_GLOBAL*)(b + j * colbytes + xoff);
+        x[u][j][0] = f.x; x[u][j][1] = f.y;
+      }
+    }
   };
   // any step: columns past j1 and rows past m read as zero
-  auto load_edge = [&](Step& s, long long jc) {
-    const long long cb = jc + 8 * g;
+  auto load_x_edge = [&](float (&x)[2][8][2], long long jc) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j)
+    for (int u = 0; u < 2; ++u)
 #pragma unroll
-      for (int e = 0; e < 4; ++e)
-        s.x[j][e] = (cb + j < j1 && rowb + e < t.m) ? xr[(cb + j) * t.m + e] : 0.f;
-    load_wc(s, jc);
+      for (int j = 0; j < 8; ++j)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const long long cc = jc + 32 * u + 8 * g + j;
+          x[u][j][e] = (cc < j1 && rowb + e < t.m) ? xr[cc * t.m + e] : 0.f;
+        }
   };
 
-  f32x4 acc[4][NBLK];
+  f64x4 acc[2][NBLK];
 #pragma unroll
-  for (int e = 0; e < 4; ++e)
+  for (int e = 0; e < 2; ++e)
 #pragma unroll
-    for (int nb = 0; nb < NBLK; ++nb) acc[e][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int nb = 0; nb < NBLK; ++nb) acc[e][nb] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-  auto compute = [&](const Step& s) {
+  auto compute = [&](const float (&x)[2][8][2], int b) {
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      float v[8];
+    for (int u = 0; u < 2; ++u) {
+      float c[8];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) v[j] = s.x[j][e] - s.c[j];
-      u32x4 ah, al;
-      split8(v, ah, al);
+      for (int j = 0; j < 8; ++j) c[j] = cs[b][32 * u + 8 * g + j];
 #pragma unroll
-      for (int nb = 0; nb < NBLK; ++nb) acc[e][nb] = mma3(ah, al, s.bh[nb], s.bl[nb], acc[e][nb]);
+      for (int e = 0; e < 2; ++e) {
+        float v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = x[u][j][e] - c[j];
+        const Frag3 a = split8(v);
+#pragma unroll
+        for (int nb = 0; nb < NBLK; ++nb) {
+          const u32x4 bh = wl[b][nb * 16 + m16][4 * u + g];
+          const u32x4 bm = wl[b][NB + nb * 16 + m16][4 * u + g];
+          const u32x4 bl = wl[b][2 * NB + nb * 16 + m16][4 * u + g];
+          fold(acc[e][nb], mma6(a, bh, bm, bl, f32x4{0.f, 0.f, 0.f, 0.f}));
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
   };
 
-  // Branch-free two-stage pipeline over the full steps: the load of the next
-  // step is always issued (the last one repeats a step), so the wait before
-  // each step's arithmetic leaves the next step's loads in flight.
-  Step sa, sb;
-  const long long nfull = (vec && rows_full) ? (j1 - j0) / 32 : 0;
-  long long s = 0;
-  if (nfull > 0) {
-    load(sa, j0);
-    for (; s + 1 < nfull; s += 2) {
-      load(sb, j0 + 32 * (s + 1));
-      compute(sa);
-      load(sa, j0 + 32 * (s + 2 < nfull ? s + 2 : s + 1));
-      compute(sb);
+  // Branch-free two-stage pipeline over the steps of the split. Every wave
+  // of the work-group walks the same steps (barriers); a wave whose rows are
+  // ragged or unaligned uses masked loads. Steps past j1 read zero W (the
+  // operand is padded to k64 columns) and masked A.
+  float xa[2][8][2], xb[2][8][2];
+  u32x4 wr[WPT];
+  float cr = 0.f;
+  const bool fast = vec && rows_full;
+  const long long nsteps = (j1 - j0 + COLS - 1) / COLS;
+  auto load_any = [&](float (&x)[2][8][2], long long jc) {
+    if (fast && jc + COLS <= j1) load_x(x, jc); else load_x_edge(x, jc);
+  };
+  if (nsteps > 0) {
+    load_w(wr, cr, j0);
+    load_any(xa, j0);
+    store_w(wr, cr, 0);
+    __syncthreads();
+    long long s = 0;
+    for (; s + 1 < nsteps; s += 2) {
+      load_w(wr, cr, j0 + COLS * (s + 1));
+      load_any(xb, j0 + COLS * (s + 1));
+      compute(xa, 0);
+      store_w(wr, cr, 1);
+      __syncthreads();
+      const long long nx = j0 + COLS * (s + 2 < nsteps ? s + 2 : s + 1);
+      load_w(wr, cr, nx);
+      load_any(xa, nx);
+      compute(xb, 1);
+      store_w(wr, cr, 0);
+      __syncthreads();
     }
-    if (s < nfull) { compute(sa); ++s; }
-  }
-  // the ragged last step (or every step of a ragged or unaligned row tile)
-  for (long long jc = j0 + 32 * s; jc < j1; jc += 32) {
-    load_edge(sa, jc);
-    compute(sa);
+    if (s < nsteps) compute(xa, 0);
   }
 
   double FPLS_GLOBAL* o = gpw(t.out) + (long long)split * t.slab;
@@ -681,15 +722,15 @@ __global__ __launch_bounds__(kWG) void k_xw_bf3(const XwTask<float>* tasks, int 
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const long long row = i0w + 16 * g + 4 * r + e;
-        if (row < t.m) o[row + (long long)n * t.ldo] = double(acc[e][nb][r]);
+      for (int e = 0; e < 2; ++e) {
+        const long long row = i0w + 8 * g + 2 * r + e;
+        if (row < t.m) o[row + (long long)n * t.ldo] = acc[e][nb][r];
       }
   }
 }
 
 // ------------------------------------------------------ xtw, FP64 MFMA
-// Same tiling as the bf16x3 kernel (128 columns, 64-row steps, lane rows
+// Same tiling as the bf16x6 kernel (128 columns, 64-row steps, lane rows
 // 16v + 4g + e); MFMA call (v, e) pairs row 16v + 4g + e of A with the same
 // row of W, which is staged in LDS as FP64. The FP64 matrix cores bound this
 // kernel, so A is single-buffered.
@@ -819,7 +860,7 @@ __global__ __launch_bounds__(kWG) void k_xtw_f64(const XtwTask<double>* tasks, i
 }
 
 // ------------------------------------------------------- xw, FP64 MFMA
-// Same tiling as the bf16x3 kernel (256 rows, 32-column steps, lane rows
+// Same tiling as the bf16x6 kernel (256 rows, 32-column steps, lane rows
 // 4r..4r+3 of columns 8g..8g+7); MFMA call (e, j) pairs column 8g + j of row
 // 4r + e with row 8g + j of W, read straight from W's columns in L2.
 template<int NB>
@@ -912,8 +953,24 @@ __global__ __launch_bounds__(kWG) void k_sum_slabs(const double* part, int split
   }
 }
 
+// Grid (blocks, list entries): entry blockIdx.y of a slab-sum list.
+__global__ __launch_bounds__(kWG) void k_sum_slabs_list(const SlabSum* list) {
+  const SlabSum e = load_const(list, blockIdx.y);
+  const double FPLS_GLOBAL* P = gp(e.part);
+  double FPLS_GLOBAL* O = gpw(e.out);
+  for (long long i = (long long)blockIdx.x * kWG + threadIdx.x; i < e.count;
+       i += (long long)gridDim.x * kWG) {
+    double s = 0.0;
+    for (int k = 0; k < e.splits; ++k) s += P[k * e.stride + i];
+    O[i] = s;
+  }
+}
+
 // ------------------------------------------------------------- dispatch
-// Lane offsets of the bf16x3 kernels are 32-bit byte offsets spanning at most
+// Widest right-hand side of one FP32 matrix-core xw launch (its FP64
+// accumulators and double-buffered operands fill the register file at 32).
+constexpr int kXwBf6MaxN = 32;
+// Lane offsets of the bf16x6 kernels are 32-bit byte offsets spanning at most
 // 32 columns of A.
 constexpr long long kMfmaMaxRows = 1LL << 24;
 constexpr unsigned kSplitBlocks = 512;
@@ -935,15 +992,17 @@ void dispatch_xw(const XwTask<T>* d, int count, const XwTask<T>& single, long lo
   if (tiles > 0x7fffffffLL) throw std::invalid_argument("xw grid too large");
   const dim3 grid((unsigned)tiles), block((unsigned)kWG);
   if (maxn >= kMfmaMinN) {
+    if (sizeof(T) == 4 && maxn > kXwBf6MaxN)
+      throw std::invalid_argument("FP32 matrix-core xw launches take at most 32 right-hand columns");
     with_width(maxn, [&](auto w) {
       constexpr int NB = decltype(w)::value;
-      if constexpr (sizeof(T) == 4) {
+      if constexpr (sizeof(T) == 4 && NB <= kXwBf6MaxN) {
         hipLaunchKernelGGL((k_xw_split<NB>), dim3(kSplitBlocks, unsigned(d ? count : 1)), block, 0,
                            stream, d, single);
         FPLS_LAUNCHED("k_xw_split");
-        hipLaunchKernelGGL((k_xw_bf3<NB>), grid, block, 0, stream, d, count, single);
-        FPLS_LAUNCHED("k_xw_bf3");
-      } else {
+        hipLaunchKernelGGL((k_xw_bf6<NB>), grid, dim3(kBf6Threads), 0, stream, d, count, single);
+        FPLS_LAUNCHED("k_xw_bf6");
+      } else if constexpr (sizeof(T) == 8) {
         hipLaunchKernelGGL((k_xw_f64<NB>), grid, block, 0, stream, d, count, single);
         FPLS_LAUNCHED("k_xw_f64");
       }
@@ -969,8 +1028,8 @@ void dispatch_xtw(const XtwTask<T>* d, int count, const XtwTask<T>& single, long
         hipLaunchKernelGGL((k_xtw_split<NB>), dim3(kSplitBlocks, unsigned(d ? count : 1)), block, 0,
                            stream, d, single);
         FPLS_LAUNCHED("k_xtw_split");
-        hipLaunchKernelGGL((k_xtw_bf3<NB>), grid, block, 0, stream, d, count, single);
-        FPLS_LAUNCHED("k_xtw_bf3");
+        hipLaunchKernelGGL((k_xtw_bf6<NB>), grid, dim3(kBf6Threads), 0, stream, d, count, single);
+        FPLS_LAUNCHED("k_xtw_bf6");
       } else {
         hipLaunchKernelGGL((k_xtw_f64<NB>), grid, block, 0, stream, d, count, single);
         FPLS_LAUNCHED("k_xtw_f64");
@@ -1043,9 +1102,19 @@ void launch_xtw(const XtwTask<T>* device_tasks, int count, long long tiles, int 
 
 template<class T>
 void launch_xw_single(XwTask<T> task, hipStream_t stream) {
-  int maxn = 0;
-  const long long tiles = pack_xw(&task, 1, &maxn);
-  dispatch_xw<T>(nullptr, 1, task, tiles, maxn, stream);
+  // FP32 matrix-core launches take at most 32 right-hand columns: wider
+  // tasks run as consecutive 32-column pieces (same stream, so the pieces
+  // may share the operand workspace)
+  const int step = (sizeof(T) == 4 && task.N >= kMfmaMinN) ? kXwBf6MaxN : task.N;
+  for (int c0 = 0; c0 < task.N; c0 += step) {
+    XwTask<T> piece = task;
+    piece.N = std::min(step, task.N - c0);
+    piece.W = task.W + (long long)c0 * task.ldw;
+    piece.out = task.out + (long long)c0 * task.ldo;
+    int maxn = 0;
+    const long long tiles = pack_xw(&piece, 1, &maxn);
+    dispatch_xw<T>(nullptr, 1, piece, tiles, maxn, stream);
+  }
 }
 
 template<class T>
@@ -1063,6 +1132,17 @@ void launch_sum_slabs(const double* part, int splits, long long stride,
   hipLaunchKernelGGL(k_sum_slabs, dim3(unsigned(blocks)), dim3(kWG), 0, stream,
                      part, splits, stride, count, out);
   FPLS_LAUNCHED("k_sum_slabs");
+}
+
+void launch_sum_slabs_list(const SlabSum* device_list, int count, long long max_count,
+                           hipStream_t stream) {
+  if (count <= 0 || max_count <= 0) return;
+  if (count > 65535) throw std::invalid_argument("slab-sum list longer than 65535");
+  long long blocks = (max_count + kWG - 1) / kWG;
+  if (blocks > 1024) blocks = 1024;
+  hipLaunchKernelGGL(k_sum_slabs_list, dim3(unsigned(blocks), unsigned(count)), dim3(kWG), 0,
+                     stream, device_list);
+  FPLS_LAUNCHED("k_sum_slabs_list");
 }
 
 #define FPLS_STREAM_INSTANCES(T)                                                        \

@@ -58,6 +58,16 @@ SIGNATURES = {
     "fastpls_hip_memcpy": (C.c_int, [_vp, _cvp, _sz, C.c_int, _cp, _sz]),
     "fastpls_hip_latent_data": (C.c_int, [_vp, _vp, C.c_int, _ll, C.c_int, C.c_int, C.c_int,
                                           C.c_double, C.c_double, _ull, C.c_int, _cp, _sz]),
+    "fastpls_hip_simpls_batch_create": (_vp, [C.c_int, C.c_int, _vp, _ip, _ip, C.c_int, _ip, _vp,
+                                              _vp, _vp, _ip, _ip, C.POINTER(_ull), C.c_int,
+                                              C.c_int, C.c_int, _cp, _sz]),
+    "fastpls_hip_batch_status": (C.c_int, [_vp, _ip, _ip, _dp, _cp, _sz]),
+    "fastpls_hip_batch_export": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _sz, _cp, _sz]),
+    "fastpls_hip_batch_destroy": (None, [_vp]),
+    "fastpls_hip_simpls_cv_regression": (C.c_int, [_cvp, _cvp, _ip, C.c_int, C.c_int, C.c_int,
+                                                   C.c_int, _ip, C.c_int, C.c_int, C.c_int,
+                                                   C.c_int, C.c_int, _ull, C.c_int, _dp, _ip,
+                                                   _dp, _dp, _dp, _dp, _cp, _sz]),
 }
 
 _LIB = None
@@ -109,6 +119,10 @@ def _ptr(a):
 
 def _iptr(a):
     return None if a is None else a.ctypes.data_as(_ip)
+
+
+def _dptr(a):
+    return None if a is None else a.ctypes.data_as(_dp)
 
 
 def device_info() -> str:
@@ -385,3 +399,105 @@ def latent_data_device(x_ptr, y_ptr, precision, n, p, q, factors=10, decay=1.0, 
                                               p, q, factors, decay, noise, seed, int(round32), e,
                                               1024) != 0:
         _raise(e)
+
+
+BATCH_FIELDS = {"R": 0, "Q": 1, "T": 2, "mX": 3, "vX": 4, "mY": 5}
+
+
+class SimplsBatch:
+    """Many SIMPLS fits in one grouped execution (fastpls_hip_simpls_batch_create).
+
+    ``groups``: list of (device pointer of X, n, p). ``fits``: list of dicts with
+    keys group, q, components, seed and exactly one of y (device pointer of the
+    n x q responses) / labels (device pointer of one-based labels), plus an
+    optional perm (device pointer of a row permutation of the responses).
+    """
+
+    def __init__(self, precision, groups, fits, scaling=2, oversample=32, power=5):
+        self.precision = int(precision)
+        self.dtype = np.float64 if self.precision == 64 else np.float32
+        self.groups = [(int(g[1]), int(g[2])) for g in groups]
+        self.fits = [dict(f) for f in fits]
+        G, F = len(groups), len(fits)
+        xs = (C.c_void_p * G)(*[C.c_void_p(g[0]) for g in groups])
+        ns = (C.c_int * G)(*[g[1] for g in groups])
+        ps = (C.c_int * G)(*[g[2] for g in groups])
+        grp = (C.c_int * F)(*[f["group"] for f in fits])
+        has_y = any(f.get("y") for f in fits)
+        has_l = any(f.get("labels") for f in fits)
+        has_p = any(f.get("perm") for f in fits)
+        ys = (C.c_void_p * F)(*[C.c_void_p(f.get("y") or 0) for f in fits]) if has_y else None
+        ls = (C.c_void_p * F)(*[C.c_void_p(f.get("labels") or 0) for f in fits]) if has_l else None
+        pm = (C.c_void_p * F)(*[C.c_void_p(f.get("perm") or 0) for f in fits]) if has_p else None
+        qs = (C.c_int * F)(*[f["q"] for f in fits])
+        cs = (C.c_int * F)(*[f["components"] for f in fits])
+        sd = (C.c_ulonglong * F)(*[int(f.get("seed", 1)) for f in fits])
+        e = _err()
+        self._h = load_library().fastpls_hip_simpls_batch_create(
+            self.precision, G, C.cast(xs, C.c_void_p), ns, ps, F, grp,
+            C.cast(ys, C.c_void_p) if ys is not None else None,
+            C.cast(ls, C.c_void_p) if ls is not None else None,
+            C.cast(pm, C.c_void_p) if pm is not None else None, qs, cs, sd, int(scaling),
+            int(oversample), int(power), e, 1024)
+        if not self._h:
+            _raise(e)
+
+    def status(self):
+        F = len(self.fits)
+        done = (C.c_int * F)()
+        flags = (C.c_int * F)()
+        sec = C.c_double(0.0)
+        e = _err()
+        if load_library().fastpls_hip_batch_status(self._h, done, flags, C.byref(sec), e,
+                                                   1024) != 0:
+            _raise(e)
+        return list(done), list(flags), sec.value
+
+    def export(self, fit, field):
+        f = BATCH_FIELDS[field] if isinstance(field, str) else int(field)
+        n, p = self.groups[self.fits[fit]["group"]]
+        q, a = self.fits[fit]["q"], self.fits[fit]["components"]
+        shape = {0: (p, a), 1: (q, a), 2: (n, a), 3: (p,), 4: (p,), 5: (q,)}[f]
+        out = np.empty(shape, dtype=self.dtype, order="F")
+        e = _err()
+        if load_library().fastpls_hip_batch_export(self._h, int(fit), f, _ptr(out), out.size, e,
+                                                   1024) != 0:
+            _raise(e)
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().fastpls_hip_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def simpls_cv_regression(x, y, folds, prefixes, precision=64, scaling=2, metric=3,
+                         oversample=32, power=5, seed=1, store_predictions=True):
+    """fastpls_hip_simpls_cv_regression: fold metrics for every prefix."""
+    dt = np.float64 if precision == 64 else np.float32
+    xa = np.asfortranarray(x, dtype=dt)
+    ya = np.asfortranarray(np.asarray(y).reshape(len(x), -1), dtype=dt)
+    n, p = xa.shape
+    q = ya.shape[1]
+    fo = np.ascontiguousarray(folds, dtype=np.int32)
+    pr = np.ascontiguousarray(prefixes, dtype=np.int32)
+    k = len(pr)
+    nf = int(fo.max())
+    status = np.zeros(nf, dtype=np.int32)
+    metrics, q2, rmsd, r2 = (np.zeros(k) for _ in range(4))
+    preds = np.zeros((n, q, k), order="F") if store_predictions else None
+    e = _err()
+    rc = load_library().fastpls_hip_simpls_cv_regression(
+        _ptr(xa), _ptr(ya), _iptr(fo), int(precision), n, p, q, _iptr(pr), k, int(scaling),
+        int(metric), int(oversample), int(power), int(seed), int(bool(store_predictions)),
+        _dptr(preds), _iptr(status), _dptr(metrics), _dptr(q2), _dptr(rmsd), _dptr(r2), e, 1024)
+    if rc != 0:
+        _raise(e)
+    return dict(status=status, metrics=metrics, q2=q2, rmsd=rmsd, observed_r2=r2,
+                predictions=preds)

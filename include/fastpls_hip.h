@@ -146,6 +146,49 @@ int fastpls_hip_memcpy(void* dst, const void* src, size_t bytes, int kind, char*
 int fastpls_hip_fit_phases(void* model, double* out, int count, char* error,
                            size_t error_capacity);
 
+/* ------------------------------------------------------------------ batches
+ * Many independent SIMPLS fits in one grouped execution (no reference
+ * counterpart: the reference fits CV folds one after another,
+ * src/cuda_resident_api.cuh:770-1023, and permutation refits on the CPU,
+ * R/main.R:9419-9450). Fits are grouped by a shared device-resident X
+ * (x_device[g], n[g] x p[g], precision 32/64); fit f uses group[f], either
+ * y_device[f] (n x q[f], same precision) or labels_device[f] (one-based,
+ * q[f] classes), an optional device permutation perm_device[f] of the
+ * response rows, components[f] and seeds[f] (rSVD seed, component a uses
+ * seed + a as in simpls.hpp:555-556). Every phase of every fit is one launch
+ * per step; the call returns when all fits are done. Arrays of pointers are
+ * host arrays; y_device / labels_device / perm_device may be NULL as a whole. */
+void* fastpls_hip_simpls_batch_create(int precision, int group_count, const void* const* x_device,
+                                      const int* n, const int* p, int fit_count,
+                                      const int* group, const void* const* y_device,
+                                      const int* const* labels_device,
+                                      const int* const* perm_device, const int* q,
+                                      const int* components, const unsigned long long* seeds,
+                                      int scaling, int oversample, int power, char* error,
+                                      size_t error_capacity);
+/* Per-fit completed component counts and status flags (0 = ok) and the device
+ * seconds of the whole batch. */
+int fastpls_hip_batch_status(void* batch, int* completed, int* flags, double* seconds,
+                             char* error, size_t error_capacity);
+/* Fields 0 R (p*A), 1 Q (q*A), 2 T (n*A), 3 x means (p), 4 x scales (p),
+ * 5 y means (q) of fit `fit`, in the batch precision. */
+int fastpls_hip_batch_export(void* batch, int fit, int field, void* out, size_t elements,
+                             char* error, size_t error_capacity);
+void fastpls_hip_batch_destroy(void* batch);
+
+/* fastpls_resident_simpls_cv_regression (src/cuda_resident_api.h:20-26):
+ * same arguments, outputs and metric ladder (2 observed R^2, 3 Q^2, 4 RMSD;
+ * src/cuda_resident_api.cuh:772-1023). All folds are fitted in one batch;
+ * SIMPLS folds share the seed, as in the reference. */
+int fastpls_hip_simpls_cv_regression(const void* predictors, const void* responses,
+                                     const int* folds, int precision, int n, int p, int q,
+                                     const int* prefixes, int prefix_count, int scaling,
+                                     int metric, int oversample, int power,
+                                     unsigned long long seed, int store_predictions,
+                                     double* predictions, int* status, double* metrics,
+                                     double* q2, double* rmsd, double* observed_r2, char* error,
+                                     size_t error_capacity);
+
 /* Synthetic latent-factor data written straight into device memory
  * (Philox4x32-10 + Box-Muller): X (n x p, precision 32/64) = T P' + noise E,
  * Y (n x q, same precision) = T Q' + noise F, with factor column f scaled by
