@@ -104,6 +104,130 @@ def calibrate_copy(fp, mib):
           flush=True)
 
 
+def c3_batched(fp, seed=2026, details_out=None):
+    """Batched SIMPLS: 256 fits = 2 datasets x 5-fold CV x 20 row permutations
+    of the training responses (the fits of one fold share X) + 56 uneven
+    singletons; per dataset n ~ U[500, 20000], p ~ U[100, 5000],
+    q ~ U[1, 10], ncomp ~ U[2, 30]; latent-factor data (10 factors, noise
+    0.1); per-fit rSVD seed 1000 + i. Times the whole batch in FP32 (mixed)
+    and in FP64 (plain, the same values) and, as the baseline, the plain FP64
+    fits one at a time through the single-fit entry (device-resident
+    inputs). Inputs are staged before the timed regions."""
+    rng = np.random.default_rng(seed)
+    keep, groups32, groups64, fits32, fits64, seq = [], [], [], [], [], []
+    for d in range(58):
+        n = int(rng.integers(500, 20001)); p = int(rng.integers(100, 5001))
+        q = int(rng.integers(1, 11)); A = int(rng.integers(2, 31))
+        xb = fp.DeviceBuffer(n * p * 4); yb = fp.DeviceBuffer(n * q * 4)
+        fp.latent_data_device(xb.ptr, yb.ptr, 32, n, p, q, factors=10, noise=0.1, seed=100 + d)
+        X = xb.download((n, p), np.float32); Y = yb.download((n, q), np.float32)
+        xb.free(); yb.free()
+        parts = []
+        if d < 2:
+            folds = np.arange(n) % 5
+            rng.shuffle(folds)
+            parts = [(X[folds != f], Y[folds != f], 20) for f in range(5)]
+        else:
+            parts = [(X, Y, 0)]
+        for Xg, Yg, nperm in parts:
+            ng = Xg.shape[0]
+            bufs = [fp.DeviceBuffer.from_array(a) for a in
+                    (Xg, Xg.astype(np.float64), Yg, Yg.astype(np.float64))]
+            keep += bufs
+            groups32.append((bufs[0].ptr, ng, p)); groups64.append((bufs[1].ptr, ng, p))
+            g = len(groups32) - 1
+            for r in range(max(nperm, 1)):
+                i = len(fits32)
+                base = dict(group=g, q=q, components=A, seed=1000 + i)
+                ys = Yg
+                if nperm:
+                    perm = rng.permutation(ng).astype(np.int32)
+                    pb = fp.DeviceBuffer.from_array(perm); keep.append(pb)
+                    base["perm"] = pb.ptr
+                    ys = Yg[perm]
+                fits32.append(dict(base, y=bufs[2].ptr))
+                fits64.append(dict(base, y=bufs[3].ptr))
+                yseq = fp.DeviceBuffer.from_array(ys.astype(np.float64)); keep.append(yseq)
+                seq.append((bufs[1].ptr, yseq.ptr, ng, p, q, A, 1000 + i, nperm > 0))
+    assert len(fits32) == 256
+    log(f"[batched] 256 fits over {len(groups32)} shared-X groups staged")
+
+    def run_batch(precision, groups, fits, reorth=-1):
+        t0 = time.perf_counter()
+        b = fp.SimplsBatch(precision, groups, fits, scaling=2, oversample=10, power=2,
+                           reorth=reorth)
+        wall = time.perf_counter() - t0
+        done, flags, dev = b.status()
+        assert all(f == 0 for f in flags) and all(d == f["components"] for d, f in zip(done, fits))
+        return b, wall, dev
+
+    # warm-up, then timed batches
+    b32, _, _ = run_batch(32, groups32, fits32); b32.close()
+    b32, wall32, dev32 = run_batch(32, groups32, fits32)
+    log(f"[batched] mixed: {1e3 * wall32:.1f} ms wall, {1e3 * dev32:.1f} ms device")
+    b64, wall64, dev64 = run_batch(64, groups64, fits64)
+    log(f"[batched] plain: {1e3 * wall64:.1f} ms wall, {1e3 * dev64:.1f} ms device")
+    # mixed against plain on the same values, at the latent rank of the data
+    # (k = min(A, 10)) and at k = A. Past the rank SIMPLS components fit
+    # noise with vanishing singular gaps: FP32-level rounding of the sums
+    # (the mixed path) moves their directions by up to ~1e-3 relative, as any
+    # change of summation order would in FP32.
+    err = dict(fit_rank=0.0, b_rank=0.0, fit_all=0.0, b_all=0.0)
+    details = []
+    for i, f in enumerate(fits32):
+        T32, Q32 = b32.export(i, "T").astype(np.float64), b32.export(i, "Q").astype(np.float64)
+        T64, Q64 = b64.export(i, "T"), b64.export(i, "Q")
+        A = f["components"]
+        kr = min(A, 10)
+        e = dict(fit_all=rel(T32 @ Q32.T, T64 @ Q64.T),
+                 fit_rank=rel(T32[:, :kr] @ Q32[:, :kr].T, T64[:, :kr] @ Q64[:, :kr].T))
+        if "perm" not in f:
+            R32, R64 = b32.export(i, "R").astype(np.float64), b64.export(i, "R")
+            e["b_all"] = rel(R32 @ Q32.T, R64 @ Q64.T)
+            e["b_rank"] = rel(R32[:, :kr] @ Q32[:, :kr].T, R64[:, :kr] @ Q64[:, :kr].T)
+        for key, v in e.items():
+            if "perm" in f and key == "fit_rank":
+                continue  # permuted fits have no latent structure to hold to
+            err[key] = max(err[key], v)
+        g = groups32[f["group"]]
+        details.append(dict(fit=i, n=g[1], p=g[2], q=f["q"], A=A, permuted="perm" in f, **e))
+    b32.close(); b64.close()
+    if details_out is not None:
+        details_out.extend(details)
+    # baseline: plain FP64, one fit at a time
+    lat = []
+    for (xp, yp, n, p, q, A, sd, _) in seq:
+        t0 = time.perf_counter()
+        m = fp.ResidentModel.fit_device(xp, yp, None, n, p, q, method=fp.SIMPLS, precision=64,
+                                        components=A, scaling=2, oversample=10, power=2, seed=sd)
+        lat.append(time.perf_counter() - t0)
+        m.close()
+    lat = np.array(lat)
+    log(f"[batched] one at a time (plain FP64): {1e3 * lat.sum():.1f} ms")
+    for b in keep:
+        b.free()
+    return {
+        "workload": "256 SIMPLS fits: 2 x (5-fold CV x 20 response-row permutations) + 56 "
+                    "singletons; n~U[500,2e4], p~U[100,5e3], q~U[1,10], ncomp~U[2,30]; "
+                    "oversample 10, power 2",
+        "batched_mixed_fits_per_s": round(256 / wall32, 2),
+        "batched_mixed_wall_ms": round(1e3 * wall32, 2),
+        "batched_plain_fits_per_s": round(256 / wall64, 2),
+        "sequential_plain_fits_per_s": round(256 / float(lat.sum()), 2),
+        "sequential_plain_p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+        "sequential_plain_p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
+        "speedup_batched_mixed_vs_sequential_plain": round(float(lat.sum()) / wall32, 2),
+        "speedup_batched_mixed_vs_batched_plain": round(wall64 / wall32, 2),
+        "max_rel_err_fitted_mixed_vs_plain_at_rank": err["fit_rank"],
+        "max_rel_err_B_mixed_vs_plain_at_rank": err["b_rank"],
+        "max_rel_err_fitted_mixed_vs_plain_all_components": err["fit_all"],
+        "max_rel_err_B_mixed_vs_plain_all_components": err["b_all"],
+        "accuracy_note": "errors over unpermuted fits at k = min(ncomp, 10) (the data's latent "
+                         "rank) and at k = ncomp; components past the rank fit noise and are "
+                         "ill-conditioned (FP32-level rounding of the sums moves them)",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +242,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -203,6 +328,11 @@ def main():
                 "score_pass_ms": round(1e3 * t_sc, 4),
                 "score_pass_frac": round(sc_bytes / t_sc / HBM_PEAK, 4)}
 
+    # ----------------------------------------------- batched SIMPLS fits
+    batched = None
+    if not args.no_batched:
+        batched = c3_batched(fp)
+
     # ------------------------------------------------------ CPU baseline
     cpu = None
     if not args.no_cpu:
@@ -236,6 +366,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         **plain,
+        "batched_simpls": batched,
     }
     print(json.dumps(out), flush=True)
 

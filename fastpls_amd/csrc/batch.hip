@@ -64,8 +64,10 @@ class SimplsBatch {
 
  public:
   SimplsBatch(const std::vector<GroupIn>& groups, const std::vector<FitIn>& fits, int scaling,
-              int oversample, int power)
-      : groups_(groups), fits_(fits), scaling_(scaling), oversample_(oversample), power_(power) {
+              int oversample, int power, int reorth)
+      : groups_(groups), fits_(fits), scaling_(scaling), oversample_(oversample), power_(power),
+        reorth_(reorth) {
+    if (reorth < -1 || reorth > 1) throw std::invalid_argument("reorth must be -1, 0 or 1");
     if (groups_.empty() || fits_.empty()) throw std::invalid_argument("empty batch");
     if (scaling < 1 || scaling > 3) throw std::invalid_argument("scaling must be 1, 2 or 3");
     if (oversample < 0 || power < 0) throw std::invalid_argument("oversample and power must be non-negative");
@@ -351,7 +353,8 @@ class SimplsBatch {
       SimplsFitDev& d = dev[f];
       d.n = G.n; d.p = G.p; d.q = F.q; d.A = F.A;
       d.power = power_; d.use_right_gram = F.right_gram ? 1 : 0;
-      d.reorth = sizeof(T) == 4 ? 1 : 0;  // src/r_api.cpp:1416: two passes for float
+      // src/r_api.cpp:1416: two passes for float, one for double, unless forced
+      d.reorth = reorth_ >= 0 ? reorth_ : (sizeof(T) == 4 ? 1 : 0);
       d.nsteps = F.nsteps; d.bmax = F.bmax; d.ts_splits = 1; d.ls_splits = G.ls_splits;
       d.ls_slab = (long long)G.p * G.bcat;
       d.steps = F.steps; d.omega = F.omega;
@@ -581,7 +584,7 @@ class SimplsBatch {
   std::vector<Fit> plan_;
   std::vector<int> order_;
   std::vector<StepLaunch> steps_;
-  int scaling_, oversample_, power_;
+  int scaling_, oversample_, power_, reorth_;
   int max_steps_ = 0;
   Arena arena_;
   size_t devfits_off_ = 0;
@@ -735,7 +738,7 @@ void simpls_cv_regression(const T* predictors, const T* responses, const int* fo
   }
   FPLS_HIP(hipStreamSynchronize(stream));
   if (fits.empty()) throw std::invalid_argument("HIP CV has no non-empty fold");
-  SimplsBatch<T> batch(groups, fits, scaling, oversample, power);
+  SimplsBatch<T> batch(groups, fits, scaling, oversample, power, -1);
   for (int i = 0; i < batch.count(); ++i)
     if (batch.completed(i) < maxp || batch.flags(i))
       throw std::runtime_error("resident SIMPLS failed: invalid labels, decomposition or component norm");
@@ -804,8 +807,8 @@ void* fastpls_hip_simpls_batch_create(int precision, int group_count, const void
                                       const int* const* labels_device,
                                       const int* const* perm_device, const int* q,
                                       const int* components, const unsigned long long* seeds,
-                                      int scaling, int oversample, int power, char* error,
-                                      size_t error_capacity) {
+                                      int scaling, int oversample, int power, int reorth,
+                                      char* error, size_t error_capacity) {
   BatchHandle* h = nullptr;
   fpls::guard(error, error_capacity, [&] {
     fpls::need_device();
@@ -825,9 +828,9 @@ void* fastpls_hip_simpls_batch_create(int precision, int group_count, const void
       fits[f].seed = seeds[f];
     }
     if (precision == 32)
-      h = new fpls::TypedBatch<float>(std::make_unique<fpls::SimplsBatch<float>>(groups, fits, scaling, oversample, power));
+      h = new fpls::TypedBatch<float>(std::make_unique<fpls::SimplsBatch<float>>(groups, fits, scaling, oversample, power, reorth));
     else if (precision == 64)
-      h = new fpls::TypedBatch<double>(std::make_unique<fpls::SimplsBatch<double>>(groups, fits, scaling, oversample, power));
+      h = new fpls::TypedBatch<double>(std::make_unique<fpls::SimplsBatch<double>>(groups, fits, scaling, oversample, power, reorth));
     else
       throw std::invalid_argument("precision must be 32 or 64");
   });

@@ -60,7 +60,7 @@ SIGNATURES = {
                                           C.c_double, C.c_double, _ull, C.c_int, _cp, _sz]),
     "fastpls_hip_simpls_batch_create": (_vp, [C.c_int, C.c_int, _vp, _ip, _ip, C.c_int, _ip, _vp,
                                               _vp, _vp, _ip, _ip, C.POINTER(_ull), C.c_int,
-                                              C.c_int, C.c_int, _cp, _sz]),
+                                              C.c_int, C.c_int, C.c_int, _cp, _sz]),
     "fastpls_hip_batch_status": (C.c_int, [_vp, _ip, _ip, _dp, _cp, _sz]),
     "fastpls_hip_batch_export": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _sz, _cp, _sz]),
     "fastpls_hip_batch_destroy": (None, [_vp]),
@@ -413,7 +413,7 @@ class SimplsBatch:
     optional perm (device pointer of a row permutation of the responses).
     """
 
-    def __init__(self, precision, groups, fits, scaling=2, oversample=32, power=5):
+    def __init__(self, precision, groups, fits, scaling=2, oversample=32, power=5, reorth=-1):
         self.precision = int(precision)
         self.dtype = np.float64 if self.precision == 64 else np.float32
         self.groups = [(int(g[1]), int(g[2])) for g in groups]
@@ -438,7 +438,7 @@ class SimplsBatch:
             C.cast(ys, C.c_void_p) if ys is not None else None,
             C.cast(ls, C.c_void_p) if ls is not None else None,
             C.cast(pm, C.c_void_p) if pm is not None else None, qs, cs, sd, int(scaling),
-            int(oversample), int(power), e, 1024)
+            int(oversample), int(power), int(reorth), e, 1024)
         if not self._h:
             _raise(e)
 

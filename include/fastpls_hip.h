@@ -157,15 +157,18 @@ int fastpls_hip_fit_phases(void* model, double* out, int count, char* error,
  * response rows, components[f] and seeds[f] (rSVD seed, component a uses
  * seed + a as in simpls.hpp:555-556). Every phase of every fit is one launch
  * per step; the call returns when all fits are done. Arrays of pointers are
- * host arrays; y_device / labels_device / perm_device may be NULL as a whole. */
+ * host arrays; y_device / labels_device / perm_device may be NULL as a whole.
+ * reorth: -1 follows the reference routing (two Gram-Schmidt passes of the
+ * loadings for precision 32, one for 64; src/r_api.cpp:1416), 0 / 1 force
+ * one / two passes. */
 void* fastpls_hip_simpls_batch_create(int precision, int group_count, const void* const* x_device,
                                       const int* n, const int* p, int fit_count,
                                       const int* group, const void* const* y_device,
                                       const int* const* labels_device,
                                       const int* const* perm_device, const int* q,
                                       const int* components, const unsigned long long* seeds,
-                                      int scaling, int oversample, int power, char* error,
-                                      size_t error_capacity);
+                                      int scaling, int oversample, int power, int reorth,
+                                      char* error, size_t error_capacity);
 /* Per-fit completed component counts and status flags (0 = ok) and the device
  * seconds of the whole batch. */
 int fastpls_hip_batch_status(void* batch, int* completed, int* flags, double* seconds,
