@@ -5,8 +5,10 @@ The compute path is the C ABI library ``lib/libfastpls_hip.so`` (declared in
 """
 from .resident import (FIELDS, PLSSVD, SIMPLS, DeviceBuffer, HipError, ResidentModel,
                        SimplsBatch, device_info, gemm, latent_data_device, load_library,
+                       plssvd_cv_classification, plssvd_cv_regression, simpls_cv_classification,
                        simpls_cv_regression)
 
 __all__ = ["FIELDS", "PLSSVD", "SIMPLS", "DeviceBuffer", "HipError", "ResidentModel",
            "SimplsBatch", "device_info", "gemm", "latent_data_device", "load_library",
+           "plssvd_cv_classification", "plssvd_cv_regression", "simpls_cv_classification",
            "simpls_cv_regression"]

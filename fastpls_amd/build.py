@@ -26,7 +26,8 @@ CFLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall",
 ]
-SOURCES = ["stream_kernels.hip", "small_kernels.hip", "prep_kernels.hip", "engine.hip", "batch.hip"]
+SOURCES = ["stream_kernels.hip", "small_kernels.hip", "prep_kernels.hip", "engine.hip", "batch.hip",
+           "cv.hip"]
 
 
 def _compile(src: str) -> str:

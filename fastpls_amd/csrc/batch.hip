@@ -10,6 +10,7 @@
 // right-hand side holds all their candidates), and one update launch. A fit
 // whose component count is reached simply idles in later steps.
 #include "../../include/fastpls_hip.h"
+#include "cv_common.hpp"
 #include "host_util.hpp"
 #include "kernels.hpp"
 #include "stream.hpp"
@@ -625,29 +626,6 @@ class TypedBatch final : public BatchHandle {
   std::unique_ptr<SimplsBatch<T>> b_;
 };
 
-void write_err(char* out, size_t cap, const char* message) noexcept {
-  if (out && cap) std::snprintf(out, cap, "%s", message);
-}
-
-template<class F>
-int guard(char* err, size_t cap, F&& f) noexcept {
-  try {
-    f();
-    return 0;
-  } catch (const std::exception& e) {
-    write_err(err, cap, e.what());
-  } catch (...) {
-    write_err(err, cap, "unknown fastPLS HIP failure");
-  }
-  return 1;
-}
-
-void need_device() {
-  int count = 0;
-  if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
-    throw std::runtime_error("no HIP device is available (the HIP backend has no CPU fallback)");
-}
-
 // ------------------------------------------------------------------- CV
 // src/cuda_resident_api.cuh:770-1023: regression CV with per-fold fits, test
 // predictions for every prefix, SSE / training-mean SS per fold and the
@@ -660,36 +638,20 @@ void simpls_cv_regression(const T* predictors, const T* responses, const int* fo
                           bool store_predictions, double* prediction_output, int* status,
                           double* metrics, double* q2_values, double* rmsd_values,
                           double* observed_r2_values) {
-  if (!predictors || !responses || !folds || !prefixes || !status || !metrics || n < 2 || p < 1 ||
-      q < 1 || prefix_count < 1 || scaling < 1 || scaling > 3 || metric < 2 || metric > 4)
+  if (!predictors || !responses || !folds || !prefixes || !status || !metrics || !q2_values ||
+      !rmsd_values || !observed_r2_values || n < 2 || p < 1 || q < 1 || prefix_count < 1 ||
+      scaling < 1 || scaling > 3 || metric < 2 || metric > 4)
     throw std::invalid_argument("invalid resident HIP regression CV input");
-  int maxp = 0, fold_count = 0;
-  for (int j = 0; j < prefix_count; ++j) {
-    if (prefixes[j] <= maxp) throw std::invalid_argument("HIP CV component path must be strictly increasing");
-    maxp = prefixes[j];
-  }
-  for (int r = 0; r < n; ++r) {
-    if (folds[r] < 1) throw std::invalid_argument("invalid HIP CV fold");
-    fold_count = std::max(fold_count, folds[r]);
-  }
+  const int maxp = cv_path_maximum(prefixes, prefix_count);
+  const FoldSplit split(folds, n);
+  const int fold_count = split.count;
   if (maxp > std::min(n - 1, p) || (store_predictions && !prediction_output))
     throw std::invalid_argument("invalid HIP regression CV output or rank");
-  std::vector<std::vector<int>> test_rows(fold_count), train_rows(fold_count);
-  for (int r = 0; r < n; ++r) test_rows[folds[r] - 1].push_back(r);
-  for (int f = 0; f < fold_count; ++f) {
-    std::vector<char> held(n, 0);
-    for (int r : test_rows[f]) held[r] = 1;
-    for (int r = 0; r < n; ++r) if (!held[r]) train_rows[f].push_back(r);
-  }
-  long double total_ss = 0.0L;
-  for (int c = 0; c < q; ++c) {
-    long double s = 0.0L, s2 = 0.0L;
-    for (int r = 0; r < n; ++r) {
-      const long double v = responses[(size_t)c * n + r];
-      s += v; s2 += v * v;
-    }
-    total_ss += s2 - s * s / n;
-  }
+  split.require_usable();
+  const std::vector<std::vector<int>>& test_rows = split.test;
+  std::vector<std::vector<int>> train_rows(fold_count);
+  for (int f = 0; f < fold_count; ++f) train_rows[f] = split.training(f, n);
+  const long double total_ss = cv_total_ss(responses, n, q);
   hipStream_t stream = nullptr;
   FPLS_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   struct StreamGuard {
@@ -718,7 +680,6 @@ void simpls_cv_regression(const T* predictors, const T* responses, const int* fo
   for (int f = 0; f < fold_count; ++f) {
     const int ntr = int(train_rows[f].size()), nte = int(test_rows[f].size());
     if (nte == 0) { status[f] = 2; continue; }
-    if (ntr < 2) throw std::invalid_argument("HIP CV contains an empty train or test fold");
     if (maxp > std::min(ntr - 1, p)) throw std::invalid_argument("HIP CV component count exceeds a fold rank");
     xtr[f].reset(new Buf); ytr[f].reset(new Buf); xte[f].reset(new Buf); yte[f].reset(new Buf);
     xtr[f]->alloc(sizeof(T) * (size_t)ntr * p);
@@ -742,8 +703,7 @@ void simpls_cv_regression(const T* predictors, const T* responses, const int* fo
   for (int i = 0; i < batch.count(); ++i)
     if (batch.completed(i) < maxp || batch.flags(i))
       throw std::runtime_error("resident SIMPLS failed: invalid labels, decomposition or component norm");
-  std::fill(metrics, metrics + prefix_count, 0.0);
-  std::vector<double> counts(prefix_count, 0.0), fold_training_ss(prefix_count, 0.0);
+  RegressionTally tally(prefix_count);
   Buf scores, part, pred, sums, flag;
   sums.alloc(sizeof(double) * 3 * q);
   flag.alloc(sizeof(int));
@@ -768,30 +728,12 @@ void simpls_cv_regression(const T* predictors, const T* responses, const int* fo
                               sizeof(double) * 3 * q, hipMemcpyDeviceToHost, bs));
       FPLS_HIP(hipStreamSynchronize(bs));
     }
-    for (int j = 0; j < prefix_count; ++j) {
-      long double sse = 0.0L, tss = 0.0L;
-      for (int c = 0; c < q; ++c) {
-        sse += host_metric[(size_t)j * 3 * q + (size_t)c * 3];
-        tss += host_metric[(size_t)j * 3 * q + (size_t)c * 3 + 1];
-      }
-      metrics[j] += double(sse);
-      fold_training_ss[j] += double(tss);
-      counts[j] += double(nte) * q;
-      if (store_predictions)
-        for (int c = 0; c < q; ++c)
-          for (int r = 0; r < nte; ++r)
-            prediction_output[(size_t)j * n * q + (size_t)c * n + test_rows[f][r]] =
-                host_pred[(size_t)j * nte * q + (size_t)c * nte + r];
-    }
+    tally.add_fold(host_metric.data(), q, nte);
+    if (store_predictions)
+      scatter_predictions(host_pred.data(), test_rows[f], n, q, prefix_count, prediction_output);
     status[f] = 1;
   }
-  for (int j = 0; j < prefix_count; ++j) {
-    const double sse = metrics[j];
-    rmsd_values[j] = counts[j] > 0.0 ? std::sqrt(sse / counts[j]) : std::numeric_limits<double>::quiet_NaN();
-    q2_values[j] = fold_training_ss[j] > 0.0 ? 1.0 - sse / fold_training_ss[j] : std::numeric_limits<double>::quiet_NaN();
-    observed_r2_values[j] = total_ss > 0.0L ? 1.0 - sse / double(total_ss) : std::numeric_limits<double>::quiet_NaN();
-    metrics[j] = metric == 4 ? rmsd_values[j] : metric == 3 ? q2_values[j] : observed_r2_values[j];
-  }
+  tally.finish(total_ss, metric, metrics, q2_values, rmsd_values, observed_r2_values);
 }
 
 }  // namespace
@@ -810,8 +752,8 @@ void* fastpls_hip_simpls_batch_create(int precision, int group_count, const void
                                       int scaling, int oversample, int power, int reorth,
                                       char* error, size_t error_capacity) {
   BatchHandle* h = nullptr;
-  fpls::guard(error, error_capacity, [&] {
-    fpls::need_device();
+  fpls::guarded(error, error_capacity, [&] {
+    fpls::require_device();
     if (group_count < 1 || fit_count < 1 || !x_device || !n || !p || !group || !q ||
         !components || !seeds)
       throw std::invalid_argument("invalid batch description");
@@ -839,7 +781,7 @@ void* fastpls_hip_simpls_batch_create(int precision, int group_count, const void
 
 int fastpls_hip_batch_status(void* batch, int* completed, int* flags, double* seconds,
                              char* error, size_t error_capacity) {
-  return fpls::guard(error, error_capacity, [&] {
+  return fpls::guarded(error, error_capacity, [&] {
     if (!batch) throw std::invalid_argument("null batch");
     BatchHandle* b = static_cast<BatchHandle*>(batch);
     for (int f = 0; f < b->count(); ++f) {
@@ -852,7 +794,7 @@ int fastpls_hip_batch_status(void* batch, int* completed, int* flags, double* se
 
 int fastpls_hip_batch_export(void* batch, int fit, int field, void* out, size_t elements,
                              char* error, size_t error_capacity) {
-  return fpls::guard(error, error_capacity, [&] {
+  return fpls::guarded(error, error_capacity, [&] {
     if (!batch) throw std::invalid_argument("null batch");
     static_cast<BatchHandle*>(batch)->export_field(fit, field, out, elements);
   });
@@ -868,8 +810,8 @@ int fastpls_hip_simpls_cv_regression(const void* predictors, const void* respons
                                      double* predictions, int* status, double* metrics,
                                      double* q2, double* rmsd, double* observed_r2, char* error,
                                      size_t error_capacity) {
-  return fpls::guard(error, error_capacity, [&] {
-    fpls::need_device();
+  return fpls::guarded(error, error_capacity, [&] {
+    fpls::require_device();
     if (precision == 32)
       fpls::simpls_cv_regression<float>(static_cast<const float*>(predictors),
           static_cast<const float*>(responses), folds, n, p, q, prefixes, prefix_count, scaling,

@@ -8,6 +8,7 @@
 // rsvd.hpp, plssvd.hpp, lda.hpp) so that results agree with the FP64 CPU
 // path for the same seed.
 #include "../../include/fastpls_hip.h"
+#include "handle.hpp"
 #include "host_util.hpp"
 #include "kernels.hpp"
 #include "stream.hpp"
@@ -25,27 +26,6 @@
 
 namespace fpls {
 namespace {
-
-// ------------------------------------------------------------- handle
-class Handle {
- public:
-  virtual ~Handle() = default;
-  virtual void predict_path(const void* x, int rows, const int* prefixes, int count, bool lda,
-                            void* out) = 0;
-  virtual void classify_path(const void* x, int rows, const int* prefixes, int count, bool lda,
-                             int top, int* labels, void* responses) = 0;
-  virtual void response_sums(const void* x, const void* y, const int* labels, int rows,
-                             int prefix, void* out) = 0;
-  virtual void project(const void* x, int rows, int prefix, void* out) = 0;
-  virtual void export_field(int field, void* out, size_t elements) = 0;
-  virtual void controls(int* oversample, int* power, int* block, int* block_limit,
-                        int* implicit, int* cache) = 0;
-  virtual void compact(bool prepare_lda) = 0;
-  virtual double last_fit_seconds() const = 0;
-  // [0] fit, [1] cross-product pass over X, [2] score pass over X (PLS-SVD)
-  // or the whole component loop (SIMPLS), [3] bytes of X, seconds from events.
-  virtual void phases(double* out, int count) const = 0;
-};
 
 template<class T>
 class Model final : public Handle {
@@ -121,8 +101,7 @@ class Model final : public Handle {
     check_path(prefixes, count);
     if (lda && !classification_) throw std::invalid_argument("LDA requires class labels");
     const int maxp = prefixes[count - 1];
-    load_rows(static_cast<const T*>(x), rows);
-    project_rows(rows, maxp);
+    project_rows(load_rows(static_cast<const T*>(x), rows), rows, maxp);
     T* o = static_cast<T*>(out);
     for (int j = 0; j < count; ++j) {
       response_block(rows, prefixes[j], lda, o + (size_t)j * rows * q_out(), nullptr);
@@ -131,14 +110,13 @@ class Model final : public Handle {
   }
 
   void classify_path(const void* x, int rows, const int* prefixes, int count, bool lda,
-                     int top, int* labels, void* responses) override {
+                     int top, int* labels, void* responses, bool device_rows) override {
     if (!classification_) throw std::invalid_argument("invalid resident classification request");
     if (!x || !labels || rows < 1 || count < 1 || top < 1 || top > q_)
       throw std::invalid_argument("invalid resident classification-path request");
     check_path(prefixes, count);
     const int maxp = prefixes[count - 1];
-    load_rows(static_cast<const T*>(x), rows);
-    project_rows(rows, maxp);
+    project_rows(load_rows(static_cast<const T*>(x), rows, device_rows), rows, maxp);
     top_.reserve(sizeof(int) * (size_t)rows * top);
     for (int j = 0; j < count; ++j) {
       T* resp = responses ? static_cast<T*>(responses) + (size_t)j * rows * q_ : nullptr;
@@ -162,8 +140,7 @@ class Model final : public Handle {
     if (!out || (!y && !labels) || (y && labels) || (classification_ != (labels != nullptr)))
       throw std::invalid_argument("invalid resident response representation");
     if (!x || rows < 1 || prefix < 1 || prefix > A_) throw std::invalid_argument("invalid resident prediction input");
-    load_rows(static_cast<const T*>(x), rows);
-    project_rows(rows, prefix);
+    project_rows(load_rows(static_cast<const T*>(x), rows), rows, prefix);
     response_block(rows, prefix, false, nullptr, nullptr);
     metric_.reserve(sizeof(double) * 3 * q_);
     flags_.reserve(sizeof(int));
@@ -193,10 +170,31 @@ class Model final : public Handle {
     for (size_t i = 0; i < host.size(); ++i) o[i] = T(host[i]);
   }
 
+  void heldout_regression(const void* x, const void* y, int rows, const int* prefixes, int count,
+                          double* predictions, double* sums) override {
+    if (classification_) throw std::invalid_argument("held-out regression needs a response model");
+    if (!x || !y || !sums || rows < 1) throw std::invalid_argument("invalid held-out regression request");
+    check_path(prefixes, count);
+    project_rows(static_cast<const T*>(x), rows, prefixes[count - 1]);
+    metric_.reserve(sizeof(double) * 3 * q_);
+    flags_.reserve(sizeof(int));
+    for (int j = 0; j < count; ++j) {
+      response_block(rows, prefixes[j], false, nullptr, nullptr);
+      if (predictions)
+        FPLS_HIP(hipMemcpyAsync(predictions + (size_t)j * rows * q_, pred64_.d(),
+                                sizeof(double) * (size_t)rows * q_, hipMemcpyDeviceToHost, stream_));
+      launch_fill_int(flags_.as<int>(), 1, 0, stream_);
+      launch_response_sums<T>(pred64_.d(), static_cast<const T*>(y), nullptr, my_.d(), rows, q_,
+                              metric_.d(), flags_.as<int>(), stream_);
+      FPLS_HIP(hipMemcpyAsync(sums + (size_t)j * 3 * q_, metric_.d(), sizeof(double) * 3 * q_,
+                              hipMemcpyDeviceToHost, stream_));
+    }
+    FPLS_HIP(hipStreamSynchronize(stream_));
+  }
+
   void project(const void* x, int rows, int prefix, void* out) override {
     if (!x || !out || rows < 1 || prefix < 1 || prefix > A_) throw std::invalid_argument("invalid resident score request");
-    load_rows(static_cast<const T*>(x), rows);
-    project_rows(rows, prefix);
+    project_rows(load_rows(static_cast<const T*>(x), rows), rows, prefix);
     outT_.reserve(sizeof(T) * (size_t)rows * prefix);
     launch_convert<T>(scores_.d(), outT_.as<T>(), (long long)rows * prefix, stream_);
     FPLS_HIP(hipMemcpyAsync(out, outT_.as<T>(), sizeof(T) * (size_t)rows * prefix,
@@ -660,15 +658,19 @@ class Model final : public Handle {
     }
   }
 
-  void load_rows(const T* x, int rows) {
+  // Device address of `rows` x p query rows: host rows are staged in rows_,
+  // device rows are read in place.
+  const T* load_rows(const T* x, int rows, bool on_device = false) {
+    if (on_device) return x;
     rows_.reserve(sizeof(T) * (size_t)rows * p_);
     FPLS_HIP(hipMemcpyAsync(rows_.as<T>(), x, sizeof(T) * (size_t)rows * p_, hipMemcpyHostToDevice, stream_));
+    return rows_.as<T>();
   }
 
   // scores (rows x prefix) = ((x - m) / s) R[:, :prefix]
-  void project_rows(int rows, int prefix) {
+  void project_rows(const T* x, int rows, int prefix) {
     scores_.reserve(sizeof(double) * (size_t)rows * prefix);
-    product<T>(rows_.as<T>(), rows, p_, mx_.d(), Rs_.d(), p_, prefix, scores_.d());
+    product<T>(x, rows, p_, mx_.d(), Rs_.d(), p_, prefix, scores_.d());
     scores_rows_ = rows;
   }
 
@@ -775,29 +777,7 @@ class Model final : public Handle {
   int scores_rows_ = 0;
 };
 
-void write_error(char* out, size_t cap, const char* message) noexcept {
-  if (out && cap) std::snprintf(out, cap, "%s", message);
-}
-
-template<class F>
-int guarded(char* err, size_t cap, F&& f) noexcept {
-  try {
-    f();
-    return 0;
-  } catch (const std::exception& e) {
-    write_error(err, cap, e.what());
-  } catch (...) {
-    write_error(err, cap, "unknown fastPLS HIP failure");
-  }
-  return 1;
-}
-
-void require_device() {
-  int count = 0;
-  const hipError_t st = hipGetDeviceCount(&count);
-  if (st != hipSuccess || count < 1)
-    throw std::runtime_error("no HIP device is available (the HIP backend has no CPU fallback)");
-}
+}  // namespace
 
 Handle* create(int method, const void* x, const void* y, const int* labels, int precision,
                bool device, int n, int p, int q, int components, int scaling, int oversample,
@@ -813,6 +793,8 @@ Handle* create(int method, const void* x, const void* y, const int* labels, int 
                              scaling, oversample, power, seed);
   throw std::invalid_argument("precision must be 32 or 64");
 }
+
+namespace {
 
 template<class T>
 void host_gemm(const T* left, const T* right, int rows, int inner, int cols, T* out) {
@@ -975,7 +957,7 @@ int fastpls_hip_classify_path(void* model, const void* x, int rows, const int* p
   return fpls::guarded(error, error_capacity, [&] {
     if (!model) throw std::invalid_argument("null resident model");
     static_cast<Handle*>(model)->classify_path(x, rows, prefixes, prefix_count, lda != 0, top,
-                                               labels, nullptr);
+                                               labels, nullptr, false);
   });
 }
 
@@ -987,7 +969,7 @@ int fastpls_hip_classify_response_path(void* model, const void* x, int rows,
     if (!model) throw std::invalid_argument("null resident model");
     if (!predictions) throw std::invalid_argument("invalid resident classification-response path request");
     static_cast<Handle*>(model)->classify_path(x, rows, prefixes, prefix_count, lda != 0, top,
-                                               labels, predictions);
+                                               labels, predictions, false);
   });
 }
 

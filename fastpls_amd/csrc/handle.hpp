@@ -1,0 +1,43 @@
+// The resident model interface shared by the C ABI thunks and the CV drivers.
+#pragma once
+
+namespace fpls {
+
+// One fitted model with its device buffers and stream
+// (src/cuda_resident_api.cuh:69-479). Row inputs are host arrays unless a
+// call says they are on the device.
+class Handle {
+ public:
+  virtual ~Handle() = default;
+  virtual void predict_path(const void* x, int rows, const int* prefixes, int count, bool lda,
+                            void* out) = 0;
+  // Top-`top` class indices (1-based) per prefix; `responses` (rows x q x
+  // count, optional) receives the response scores. `device_rows` reads x
+  // from device memory.
+  virtual void classify_path(const void* x, int rows, const int* prefixes, int count, bool lda,
+                             int top, int* labels, void* responses, bool device_rows) = 0;
+  virtual void response_sums(const void* x, const void* y, const int* labels, int rows,
+                             int prefix, void* out) = 0;
+  // Held-out regression evaluation of device rows x (rows x p) against
+  // device responses y (rows x q): per prefix, FP64 predictions (rows x q,
+  // optional) and the 3 x q response sums of src/cuda_resident_metrics.cuh:9-43.
+  virtual void heldout_regression(const void* x, const void* y, int rows, const int* prefixes,
+                                  int count, double* predictions, double* sums) = 0;
+  virtual void project(const void* x, int rows, int prefix, void* out) = 0;
+  virtual void export_field(int field, void* out, size_t elements) = 0;
+  virtual void controls(int* oversample, int* power, int* block, int* block_limit,
+                        int* implicit, int* cache) = 0;
+  virtual void compact(bool prepare_lda) = 0;
+  virtual double last_fit_seconds() const = 0;
+  // [0] fit, [1] cross-product pass over X, [2] score pass over X (PLS-SVD)
+  // or the whole component loop (SIMPLS), [3] bytes of X, seconds from events.
+  virtual void phases(double* out, int count) const = 0;
+};
+
+// method 0 = SIMPLS, 1 = PLS-SVD; precision 32 or 64. With `device_inputs`
+// x, y and labels are device arrays that must outlive the handle.
+Handle* create(int method, const void* x, const void* y, const int* labels, int precision,
+               bool device_inputs, int n, int p, int q, int components, int scaling,
+               int oversample, int power, unsigned long long seed);
+
+}  // namespace fpls

@@ -6,6 +6,8 @@
 
 #include <algorithm>
 #include <cstddef>
+#include <cstdio>
+#include <exception>
 #include <random>
 
 namespace fpls {
@@ -95,5 +97,33 @@ struct Event {
   Event() { FPLS_HIP(hipEventCreate(&e)); }
   ~Event() { if (e) (void)hipEventDestroy(e); }
 };
+
+// ------------------------------------------------------------ C ABI glue
+// Exceptions never cross the C ABI: every entry point runs its body through
+// `guarded`, which returns 0 on success and 1 with the message copied into
+// the caller's buffer (src/cuda_resident_api.cuh:480-482).
+inline void write_error(char* out, size_t cap, const char* message) noexcept {
+  if (out && cap) std::snprintf(out, cap, "%s", message);
+}
+
+template<class F>
+int guarded(char* err, size_t cap, F&& f) noexcept {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    write_error(err, cap, e.what());
+  } catch (...) {
+    write_error(err, cap, "unknown fastPLS HIP failure");
+  }
+  return 1;
+}
+
+inline void require_device() {
+  int count = 0;
+  const hipError_t st = hipGetDeviceCount(&count);
+  if (st != hipSuccess || count < 1)
+    throw std::runtime_error("no HIP device is available (the HIP backend has no CPU fallback)");
+}
 
 }  // namespace fpls

@@ -26,6 +26,13 @@ FIELDS = {"R": 0, "Q": 1, "T": 2, "mX": 3, "vX": 4, "mY": 5, "P": 6, "variance":
 # name -> (restype, argtypes)
 _vp, _cvp, _ip, _dp, _cp = C.c_void_p, C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_char_p
 _sz, _ull, _ll = C.c_size_t, C.c_ulonglong, C.c_longlong
+_CV_REGRESSION = (C.c_int, [_cvp, _cvp, _ip, C.c_int, C.c_int, C.c_int, C.c_int, _ip, C.c_int,
+                            C.c_int, C.c_int, C.c_int, C.c_int, _ull, C.c_int, _dp, _ip, _dp,
+                            _dp, _dp, _dp, _cp, _sz])
+_CV_CLASSIFICATION = (C.c_int, [_cvp, _ip, _ip, C.c_int, C.c_int, C.c_int, C.c_int, _ip, C.c_int,
+                                C.c_int, C.c_int, C.c_int, C.c_int, _ull, C.c_int, C.c_int, _ip,
+                                _vp, _ip, _dp, _cp, _sz])
+
 SIGNATURES = {
     "fastpls_hip_device_info": (C.c_int, [_cp, _sz]),
     "fastpls_hip_gemm": (C.c_int, [_cvp, _cvp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _cp, _sz]),
@@ -64,10 +71,10 @@ SIGNATURES = {
     "fastpls_hip_batch_status": (C.c_int, [_vp, _ip, _ip, _dp, _cp, _sz]),
     "fastpls_hip_batch_export": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _sz, _cp, _sz]),
     "fastpls_hip_batch_destroy": (None, [_vp]),
-    "fastpls_hip_simpls_cv_regression": (C.c_int, [_cvp, _cvp, _ip, C.c_int, C.c_int, C.c_int,
-                                                   C.c_int, _ip, C.c_int, C.c_int, C.c_int,
-                                                   C.c_int, C.c_int, _ull, C.c_int, _dp, _ip,
-                                                   _dp, _dp, _dp, _dp, _cp, _sz]),
+    "fastpls_hip_simpls_cv_regression": _CV_REGRESSION,
+    "fastpls_hip_plssvd_cv_regression": _CV_REGRESSION,
+    "fastpls_hip_simpls_cv_classification": _CV_CLASSIFICATION,
+    "fastpls_hip_plssvd_cv_classification": _CV_CLASSIFICATION,
 }
 
 _LIB = None
@@ -477,9 +484,8 @@ class SimplsBatch:
             pass
 
 
-def simpls_cv_regression(x, y, folds, prefixes, precision=64, scaling=2, metric=3,
-                         oversample=32, power=5, seed=1, store_predictions=True):
-    """fastpls_hip_simpls_cv_regression: fold metrics for every prefix."""
+def _cv_regression(entry, x, y, folds, prefixes, precision, scaling, metric, oversample, power,
+                   seed, store_predictions):
     dt = np.float64 if precision == 64 else np.float32
     xa = np.asfortranarray(x, dtype=dt)
     ya = np.asfortranarray(np.asarray(y).reshape(len(x), -1), dtype=dt)
@@ -488,12 +494,11 @@ def simpls_cv_regression(x, y, folds, prefixes, precision=64, scaling=2, metric=
     fo = np.ascontiguousarray(folds, dtype=np.int32)
     pr = np.ascontiguousarray(prefixes, dtype=np.int32)
     k = len(pr)
-    nf = int(fo.max())
-    status = np.zeros(nf, dtype=np.int32)
+    status = np.zeros(max(int(fo.max()), 1), dtype=np.int32)
     metrics, q2, rmsd, r2 = (np.zeros(k) for _ in range(4))
     preds = np.zeros((n, q, k), order="F") if store_predictions else None
     e = _err()
-    rc = load_library().fastpls_hip_simpls_cv_regression(
+    rc = getattr(load_library(), entry)(
         _ptr(xa), _ptr(ya), _iptr(fo), int(precision), n, p, q, _iptr(pr), k, int(scaling),
         int(metric), int(oversample), int(power), int(seed), int(bool(store_predictions)),
         _dptr(preds), _iptr(status), _dptr(metrics), _dptr(q2), _dptr(rmsd), _dptr(r2), e, 1024)
@@ -501,3 +506,62 @@ def simpls_cv_regression(x, y, folds, prefixes, precision=64, scaling=2, metric=
         _raise(e)
     return dict(status=status, metrics=metrics, q2=q2, rmsd=rmsd, observed_r2=r2,
                 predictions=preds)
+
+
+def simpls_cv_regression(x, y, folds, prefixes, precision=64, scaling=2, metric=3,
+                         oversample=32, power=5, seed=1, store_predictions=True):
+    """fastpls_hip_simpls_cv_regression: fold metrics for every prefix (all
+    folds fitted in one batch, one shared seed)."""
+    return _cv_regression("fastpls_hip_simpls_cv_regression", x, y, folds, prefixes, precision,
+                          scaling, metric, oversample, power, seed, store_predictions)
+
+
+def plssvd_cv_regression(x, y, folds, prefixes, precision=64, scaling=2, metric=3,
+                         oversample=32, power=5, seed=1, store_predictions=True):
+    """fastpls_hip_plssvd_cv_regression: fold metrics for every prefix (fold
+    f fitted with seed + f)."""
+    return _cv_regression("fastpls_hip_plssvd_cv_regression", x, y, folds, prefixes, precision,
+                          scaling, metric, oversample, power, seed, store_predictions)
+
+
+def _cv_classification(entry, x, labels, folds, classes, prefixes, precision, scaling, lda,
+                       oversample, power, seed, store_predictions, store_scores):
+    dt = np.float64 if precision == 64 else np.float32
+    xa = np.asfortranarray(x, dtype=dt)
+    n, p = xa.shape
+    lab = np.ascontiguousarray(labels, dtype=np.int32)
+    fo = np.ascontiguousarray(folds, dtype=np.int32)
+    pr = np.ascontiguousarray(prefixes, dtype=np.int32)
+    k = len(pr)
+    status = np.zeros(max(int(fo.max()), 1), dtype=np.int32)
+    metrics = np.zeros(k)
+    preds = np.zeros((n, k), dtype=np.int32, order="F") if store_predictions else None
+    scores = np.zeros((n, int(classes), k), dtype=dt, order="F") if store_scores else None
+    e = _err()
+    rc = getattr(load_library(), entry)(
+        _ptr(xa), _iptr(lab), _iptr(fo), int(precision), n, p, int(classes), _iptr(pr), k,
+        int(scaling), int(bool(lda)), int(oversample), int(power), int(seed),
+        int(bool(store_predictions)), int(bool(store_scores)), _iptr(preds), _ptr(scores),
+        _iptr(status), _dptr(metrics), e, 1024)
+    if rc != 0:
+        _raise(e)
+    return dict(status=status, metrics=metrics, predictions=preds, scores=scores)
+
+
+def simpls_cv_classification(x, labels, folds, classes, prefixes, precision=64, scaling=2,
+                             lda=False, oversample=32, power=5, seed=1, store_predictions=True,
+                             store_scores=False):
+    """fastpls_hip_simpls_cv_classification: accuracy per prefix."""
+    return _cv_classification("fastpls_hip_simpls_cv_classification", x, labels, folds, classes,
+                              prefixes, precision, scaling, lda, oversample, power, seed,
+                              store_predictions, store_scores)
+
+
+def plssvd_cv_classification(x, labels, folds, classes, prefixes, precision=64, scaling=2,
+                             lda=False, oversample=32, power=5, seed=1, store_predictions=True,
+                             store_scores=False):
+    """fastpls_hip_plssvd_cv_classification: accuracy per prefix (fold f
+    fitted with seed + f)."""
+    return _cv_classification("fastpls_hip_plssvd_cv_classification", x, labels, folds, classes,
+                              prefixes, precision, scaling, lda, oversample, power, seed,
+                              store_predictions, store_scores)

@@ -192,6 +192,45 @@ int fastpls_hip_simpls_cv_regression(const void* predictors, const void* respons
                                      double* q2, double* rmsd, double* observed_r2, char* error,
                                      size_t error_capacity);
 
+/* fastpls_resident_plssvd_cv_regression (src/cuda_resident_api.h:33-39):
+ * the same contract as the SIMPLS entry above; folds are fitted one after
+ * another and fold f (0-based) uses seed + f (src/cuda_resident_api.cuh:934-937). */
+int fastpls_hip_plssvd_cv_regression(const void* predictors, const void* responses,
+                                     const int* folds, int precision, int n, int p, int q,
+                                     const int* prefixes, int prefix_count, int scaling,
+                                     int metric, int oversample, int power,
+                                     unsigned long long seed, int store_predictions,
+                                     double* predictions, int* status, double* metrics,
+                                     double* q2, double* rmsd, double* observed_r2, char* error,
+                                     size_t error_capacity);
+
+/* fastpls_resident_simpls_cv_classification (src/cuda_resident_api.h:14-19)
+ * and fastpls_resident_plssvd_cv_classification (src/cuda_resident_api.h:27-32);
+ * semantics of src/cuda_resident_api.cuh:538-768. Labels are 1..classes.
+ * Per fold: status 1 fitted, 2 no held-out rows, 4 training rows hold one
+ * class or none (that class, or 1, is predicted for the fold). Classes absent
+ * from a fold's training rows are dropped from its fit. predictions
+ * (n x prefix_count, int) and scores (n x classes x prefix_count, in the
+ * call's precision, zero for dropped classes) are written when requested;
+ * metrics are accuracies per prefix. SIMPLS folds share the seed, PLS-SVD
+ * fold f uses seed + f. */
+int fastpls_hip_simpls_cv_classification(const void* predictors, const int* labels,
+                                         const int* folds, int precision, int n, int p,
+                                         int classes, const int* prefixes, int prefix_count,
+                                         int scaling, int lda, int oversample, int power,
+                                         unsigned long long seed, int store_predictions,
+                                         int store_scores, int* predictions, void* scores,
+                                         int* status, double* metrics, char* error,
+                                         size_t error_capacity);
+int fastpls_hip_plssvd_cv_classification(const void* predictors, const int* labels,
+                                         const int* folds, int precision, int n, int p,
+                                         int classes, const int* prefixes, int prefix_count,
+                                         int scaling, int lda, int oversample, int power,
+                                         unsigned long long seed, int store_predictions,
+                                         int store_scores, int* predictions, void* scores,
+                                         int* status, double* metrics, char* error,
+                                         size_t error_capacity);
+
 /* Synthetic latent-factor data written straight into device memory
  * (Philox4x32-10 + Box-Muller): X (n x p, precision 32/64) = T P' + noise E,
  * Y (n x q, same precision) = T Q' + noise F, with factor column f scaled by

@@ -1,4 +1,4 @@
-"""Grouped multi-fit execution and the batched CV driver (GPU, through the C ABI).
+"""Grouped multi-fit execution (GPU, through the C ABI).
 
 * A batch of uneven fits must give, for every fit, the same model as fitting
   it alone through fastpls_hip_simpls_create: groups that share X (row
@@ -13,10 +13,8 @@
   gaps, so FP32-level differences between the batched and the single-fit
   arithmetic (FP32-accumulated matrix-core passes against FP64 VALU passes)
   move them by ~1e-5..1e-4 relative while the fitted values agree.
-* fastpls_hip_simpls_cv_regression must reproduce a fold-by-fold
-  restatement on the FP64 oracle (src/cuda_resident_api.cuh:772-1023: the
-  same fold rows, the shared seed, SSE / training-mean SS per fold, the
-  metric ladder and the prediction layout).
+
+The batched CV driver is tested with the other CV entries in test_gpu_cv.py.
 """
 import numpy as np
 import pytest
@@ -113,9 +111,9 @@ def test_batch_equals_alone(hip, precision):
             assert rel(coef(Tb, Qb, k), coef(Ta, Qa, k)) < TOL[precision], (precision, i, k)
             if precision == 64 or k <= a["rank"]:
                 assert rel(coef(Rb, Qb, k), coef(Ra, Qa, k)) < TOL[precision], (precision, i, k)
-        # column means: FP64 sums of per-lane partials (FP32 partials over 16
-        # rows on the bf16x3 path), so different split layouts agree to
-        # rounding relative to the column scale
+        # column means: FP64 sums of per-lane partials (FP32 partials over
+        # one 64-row step on the matrix-core path), so different split
+        # layouts agree to rounding relative to the column scale
         scale = float(np.mean(m.export("vX")))
         np.testing.assert_allclose(batch.export(i, "mX"), m.export("mX"), rtol=0,
                                    atol=(1e-12 if precision == 64 else 1e-6) * scale)
@@ -131,61 +129,3 @@ def test_batch_rejects_bad_descriptions(hip):
         fp.SimplsBatch(64, [(xb.ptr, 50, 10)], [dict(group=3, q=1, components=2, y=yb.ptr)])
     with pytest.raises(fp.HipError, match="dimensions"):
         fp.SimplsBatch(64, [(xb.ptr, 50, 10)], [dict(group=0, q=1, components=11, y=yb.ptr)])
-
-
-def _oracle_cv(oracle, X, Y, folds, prefixes, seed, oversample, power, scaling=2):
-    """Fold-by-fold restatement of src/cuda_resident_api.cuh:772-1023."""
-    n, q = Y.shape
-    F = folds.max()
-    sse = np.zeros(len(prefixes)); tss = np.zeros(len(prefixes)); cnt = np.zeros(len(prefixes))
-    pred = np.zeros((n, q, len(prefixes)))
-    status = np.zeros(F, dtype=np.int32)
-    for f in range(F):
-        te = np.where(folds == f + 1)[0]
-        if te.size == 0:
-            status[f] = 2
-            continue
-        tr = np.where(folds != f + 1)[0]
-        m = oracle.simpls_fit(X[tr], Y[tr], ncomp=max(prefixes), scaling=scaling,
-                              oversample=oversample, power=power, seed=seed)
-        Xs = (X[te] - m["mx"]) / m["sx"]
-        for j, k in enumerate(prefixes):
-            yh = Xs @ (m["R"][:, :k] @ m["Q"][:, :k].T) + m["my"]
-            pred[te, :, j] = yh
-            sse[j] += ((Y[te] - yh) ** 2).sum()
-            tss[j] += ((Y[te] - m["my"]) ** 2).sum()
-            cnt[j] += te.size * q
-        status[f] = 1
-    total = ((Y - Y.mean(0)) ** 2).sum()
-    return dict(status=status, sse=sse, q2=1 - sse / tss, rmsd=np.sqrt(sse / cnt),
-                observed_r2=1 - sse / total, predictions=pred)
-
-
-@pytest.mark.parametrize("metric", [2, 3, 4])
-def test_cv_regression_matches_oracle(hip, oracle, metric):
-    fp = hip[0]
-    X, Y = latent_data(300, 40, 2, factors=5, noise=0.3, seed=6)
-    folds = (np.arange(300) % 5 + 1).astype(np.int32)
-    np.random.default_rng(2).shuffle(folds)
-    prefixes = [1, 3, 5]
-    got = fp.simpls_cv_regression(X, Y, folds, prefixes, precision=64, metric=metric,
-                                  oversample=6, power=2, seed=5)
-    want = _oracle_cv(oracle, X, Y, folds, prefixes, seed=5, oversample=6, power=2)
-    np.testing.assert_array_equal(got["status"], want["status"])
-    np.testing.assert_allclose(got["q2"], want["q2"], rtol=1e-9)
-    np.testing.assert_allclose(got["rmsd"], want["rmsd"], rtol=1e-9)
-    np.testing.assert_allclose(got["observed_r2"], want["observed_r2"], rtol=1e-9)
-    expect = {2: want["observed_r2"], 3: want["q2"], 4: want["rmsd"]}[metric]
-    np.testing.assert_allclose(got["metrics"], expect, rtol=1e-9)
-    assert rel(got["predictions"], want["predictions"]) < 1e-9
-
-
-def test_cv_regression_empty_fold_status(hip, oracle):
-    fp = hip[0]
-    X, Y = latent_data(120, 15, 1, factors=3, seed=7)
-    folds = np.where(np.arange(120) % 2 == 0, 1, 3).astype(np.int32)  # fold 2 empty
-    got = fp.simpls_cv_regression(X, Y, folds, [2], precision=64, metric=3, oversample=4,
-                                  power=1, seed=1)
-    assert list(got["status"]) == [1, 2, 1]
-    with pytest.raises(fp.HipError, match="strictly increasing"):
-        fp.simpls_cv_regression(X, Y, folds, [2, 2], precision=64)
