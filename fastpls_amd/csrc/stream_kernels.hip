@@ -367,15 +367,20 @@ __global__ __launch_bounds__(kWG) void k_xtw_split(const XtwTask<float>* tasks, 
 }
 
 // --------------------------------------------------- xtw, bf16x6 (FP32 A)
-// A work-group of 512 threads owns 128 columns of A (16 per wave: one
-// 16-column M-block) and one row split, walked in 64-row steps. Lane (c, g)
-// = (lane % 16, lane / 16) holds rows 16v + 4g + e (v, e < 4) of its
-// column, loaded as float4; reduction sub-step h (32 rows) pairs rows
-// {16(2h + j/4) + 4g + j%4} with the same rows of W, whose split fragments
-// are staged in LDS (double-buffered, shared by the eight waves). The next
-// step's A and W loads are issued before the current step's arithmetic.
-// Each sub-step's six products run in a fresh FP32 chain that is folded into
-// FP64 accumulators.
+// A work-group of 512 threads owns 128 columns of A and one row split,
+// walked in 64-row steps, with the work split by role. Waves 0-3 produce:
+// wave p loads columns 32p..32p+31 (two 16-column M-blocks; lane (c, g) =
+// (lane % 16, lane / 16) holds rows 16v + 4g + e, v, e < 4, of column c of
+// each block, as float4), centres them, gathers the column statistics,
+// splits every 32-row sub-step h (rows {16(2h + j/4) + 4g + j%4}) into bf16
+// hi/mid/lo fragments and stores them in LDS next to the step's W tile.
+// Waves 4-7 consume: wave 4 + c runs the matrix-core chains of producer c's
+// M-blocks against every 16-column block of W. Waves w and w + 4 share a
+// SIMD, so one step's vector work overlaps the previous step's matrix
+// products. LDS holds two steps; producers prefetch the A rows and W tile of
+// the step after the one they prepare. Each (M-block, W block) runs one FP32
+// chain of twelve products per step (order-2^-16 terms of both sub-steps,
+// then order-2^-8, then hi·hi) that is folded into FP64 accumulators.
 constexpr int kXtwCols = 128;
 constexpr int kBf6Threads = 512;
 
@@ -383,10 +388,12 @@ template<int NB>
 __global__ __launch_bounds__(kBf6Threads) void k_xtw_bf6(const XtwTask<float>* tasks, int count,
                                                          XtwTask<float> single) {
   constexpr int NBLK = NB / 16;
-  constexpr int ROWS = 64;                                     // rows per step
-  constexpr int CPR = ROWS / 8;                                // 16-byte chunks per W row
-  constexpr int WCH = 3 * NB * CPR;                            // chunks of one step
-  constexpr int WPT = (WCH + kBf6Threads - 1) / kBf6Threads;   // chunks per thread
+  constexpr int ROWS = 64;                        // rows per step
+  constexpr int CPR = ROWS / 8;                   // 16-byte W chunks per fragment row
+  constexpr int WCH = 3 * NB * CPR;               // W chunks of one step
+  constexpr int PT = kBf6Threads / 2;             // producer threads
+  constexpr int WPT = (WCH + PT - 1) / PT;        // W chunks per producer thread
+  static_assert(CPR == 8, "one 64-row tile of W per step");
   const XtwTask<float> t = load_task(tasks, count, single);
   const long long ctiles = (t.k + kXtwCols - 1) / kXtwCols;
   const long long total = ctiles * t.splits;
@@ -400,160 +407,230 @@ __global__ __launch_bounds__(kBf6Threads) void k_xtw_bf6(const XtwTask<float>* t
   const long long r1 = r0 + per < t.m ? r0 + per : t.m;
   const long long m64 = (t.m + 63) / 64 * 64;
 
-  // [buffer][hi rows 0..NB-1, mid NB..2NB-1, lo 2NB..3NB-1][8 chunks + pad]
-  __shared__ u32x4 wl[2][3 * NB][CPR + 1];
+  __shared__ u32x4 af[2][8][2][3][64];            // [buffer][M-block][sub-step][plane][lane]
+  __shared__ u32x4 wl[2][3 * NB][CPR + 1];        // [buffer][plane NB + W column][chunk + pad]
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int role = wave & 3;
   const int m16 = lane & 15, g = lane >> 4;
   const bool vec = (t.m & 3) == 0 && (((unsigned long long)t.A) & 15) == 0;
   const bool stats = t.mode == kShiftStats;
-  // Fast loads: a wave-uniform base (column bc, advanced per step) plus one
-  // 32-bit lane offset; columns past k re-read column bc.
-  const long long cw = col0 + wave * 16;
-  const long long bc = cw < t.k ? cw : t.k - 1;
-  const char FPLS_GLOBAL* xbase = (const char FPLS_GLOBAL*)(gp(t.A) + bc * t.m);
-  const long long col = cw + m16;
-  const bool cvalid = col < t.k;
-  const float FPLS_GLOBAL* xcol = gp(t.A) + (cvalid ? col : 0) * t.m;
-  const unsigned xoff = unsigned(((cvalid ? col - bc : 0) * t.m + 4 * g) * 4);
-  float centre = 0.f;
-  if (cvalid) {
-    if (t.mode == kCentered && t.center != nullptr) centre = float(gp(t.center)[col]);
-    else if (stats) centre = xcol[0];
-  }
-  const char FPLS_GLOBAL* frag = (const char FPLS_GLOBAL*)t.frag;
-  // W chunk c = threadIdx.x + s * 512 of a step: fragment row c / 8, 16 bytes
-  // at 8 (c % 8); row 64 s + threadIdx.x / 8 of the [3 NB][m64] array.
-  static_assert(CPR == 8, "one 64-row tile of W per step");
-  const unsigned woff = unsigned(((threadIdx.x >> 3) * m64 + 8 * (threadIdx.x & 7)) * 2);
+  const long long nfull = vec ? (r1 - r0) / ROWS : 0;
+  const long long cw = col0 + 32 * role;          // first column of the wave's M-blocks
 
-  auto load_w = [&](u32x4 (&wr)[WPT], long long i0) {
+  if (wave < 4) {
+    // ------------------------------------------------------------ producer
+    // Fast loads: a wave-uniform base (column bc) plus 32-bit lane offsets;
+    // columns past k re-read column bc (their outputs are never written).
+    const long long bc = cw < t.k ? cw : t.k - 1;
+    const char FPLS_GLOBAL* xbase = (const char FPLS_GLOBAL*)(gp(t.A) + bc * t.m);
+    unsigned xoff[2];
+    bool cvalid[2];
+    float centre[2];
+    const float FPLS_GLOBAL* xcol[2];
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+      const long long col = cw + 16 * mb + m16;
+      cvalid[mb] = col < t.k;
+      xcol[mb] = gp(t.A) + (cvalid[mb] ? col : 0) * t.m;
+      xoff[mb] = unsigned(((cvalid[mb] ? col - bc : 0) * t.m + 4 * g) * 4);
+      centre[mb] = 0.f;
+      if (cvalid[mb]) {
+        if (t.mode == kCentered && t.center != nullptr) centre[mb] = float(gp(t.center)[col]);
+        else if (stats) centre[mb] = xcol[mb][0];
+      }
+    }
+    auto load_x = [&](float (&x)[2][16], long long i0) {
+#pragma unroll
+      for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const float4 f = *(const float4 FPLS_GLOBAL*)(xbase + i0 * 4 + 64 * v + xoff[mb]);
+          x[mb][4 * v] = f.x; x[mb][4 * v + 1] = f.y; x[mb][4 * v + 2] = f.z; x[mb][4 * v + 3] = f.w;
+        }
+    };
+    // rows past r1 and columns past k load the centre: they contribute zero
+    auto load_x_edge = [&](float (&x)[2][16], long long i0) {
+#pragma unroll
+      for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const long long row = i0 + 16 * v + 4 * g + e;
+            x[mb][4 * v + e] = (cvalid[mb] && row < r1) ? xcol[mb][row] : centre[mb];
+          }
+    };
+    double s1[2] = {0.0, 0.0}, s2[2] = {0.0, 0.0};
+    auto prepare = [&](const float (&x)[2][16], int b) {
+#pragma unroll
+      for (int mb = 0; mb < 2; ++mb) {
+        float p1 = 0.f, p2 = 0.f;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          float v[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) v[j] = x[mb][8 * h + j] - centre[mb];
+          if (stats) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) { p1 += v[j]; p2 = fmaf(v[j], v[j], p2); }
+          }
+          const Frag3 a = split8(v);
+          af[b][2 * role + mb][h][0][lane] = a.h;
+          af[b][2 * role + mb][h][1][lane] = a.m;
+          af[b][2 * role + mb][h][2][lane] = a.l;
+        }
+        if (stats) { s1[mb] += double(p1); s2[mb] += double(p2); }
+      }
+    };
+
+    float xa[2][16], xb[2][16];
+    if (nfull > 0) {
+      auto at = [&](long long step) { return r0 + ROWS * (step < nfull ? step : nfull - 1); };
+      load_x(xa, at(0));
+      load_x(xb, at(1));
+      prepare(xa, 0);
+      __syncthreads();
+      for (long long s = 0; s < nfull; s += 2) {
+        load_x(xa, at(s + 2));
+        if (s + 1 < nfull) prepare(xb, 1);
+        __syncthreads();
+        if (s + 1 >= nfull) break;
+        load_x(xb, at(s + 3));
+        if (s + 2 < nfull) prepare(xa, 0);
+        __syncthreads();
+      }
+    }
+    // the ragged last step (or every step of unaligned columns)
+    for (long long i0 = r0 + ROWS * nfull; i0 < r1; i0 += ROWS) {
+      load_x_edge(xa, i0);
+      prepare(xa, 0);
+      __syncthreads();
+      __syncthreads();
+    }
+    if (stats && t.stats != nullptr) {
+#pragma unroll
+      for (int mb = 0; mb < 2; ++mb) {
+        double a1 = s1[mb], a2 = s2[mb];
+        a1 += __shfl_xor(a1, 16, 64);
+        a1 += __shfl_xor(a1, 32, 64);
+        a2 += __shfl_xor(a2, 16, 64);
+        a2 += __shfl_xor(a2, 32, 64);
+        const long long col = cw + 16 * mb + m16;
+        if (g == 0 && cvalid[mb]) {
+          double FPLS_GLOBAL* st = gpw(t.stats) + (long long)split * 2 * t.k;
+          st[col] = a1;
+          st[t.k + col] = a2;
+        }
+      }
+    }
+    return;
+  }
+
+  // -------------------------------------------------------------- consumer
+  // The consumers also stage W: chunk c = (threadIdx.x - PT) + s * PT of a
+  // step is fragment row c / 8, 16 bytes at 8 (c % 8), i.e. row 32 s +
+  // (threadIdx.x - PT) / 8 of the [3 NB][m64] operand.
+  const int ct = threadIdx.x - PT;
+  const char FPLS_GLOBAL* frag = (const char FPLS_GLOBAL*)t.frag;
+  const unsigned woff = unsigned(((ct >> 3) * m64 + 8 * (ct & 7)) * 2);
+  u32x4 wr[WPT];
+  auto load_w = [&](long long i0) {
 #pragma unroll
     for (int s = 0; s < WPT; ++s) {
-      if (WCH % kBf6Threads == 0 || int(threadIdx.x) + s * kBf6Threads < WCH) {
-        const char FPLS_GLOBAL* b =
-            frag + ((long long)s * (kBf6Threads / CPR) * m64 + i0) * 2;
+      if (WCH % PT == 0 || ct + s * PT < WCH) {
+        const char FPLS_GLOBAL* b = frag + ((long long)s * (PT / CPR) * m64 + i0) * 2;
         wr[s] = *(const u32x4 FPLS_GLOBAL*)(b + woff);
       }
     }
   };
-  auto store_w = [&](const u32x4 (&wr)[WPT], int b) {
+  auto store_w = [&](int b) {
 #pragma unroll
     for (int s = 0; s < WPT; ++s) {
-      const int c = threadIdx.x + s * kBf6Threads;
-      if (WCH % kBf6Threads == 0 || c < WCH) wl[b][c / CPR][c % CPR] = wr[s];
+      const int c = ct + s * PT;
+      if (WCH % PT == 0 || c < WCH) wl[b][c / CPR][c % CPR] = wr[s];
     }
   };
-  // Full 64-row steps of 16-byte-aligned columns: four float4.
-  auto load_x = [&](float (&x)[ROWS / 4], long long i0) {
+  f64x4 acc[2][NBLK];
 #pragma unroll
-    for (int v = 0; v < ROWS / 16; ++v) {
-      const float4 f = *(const float4 FPLS_GLOBAL*)(xbase + i0 * 4 + 64 * v + xoff);
-      x[4 * v] = f.x; x[4 * v + 1] = f.y; x[4 * v + 2] = f.z; x[4 * v + 3] = f.w;
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < NBLK; ++nb) acc[mb][nb] = f64x4{0.0, 0.0, 0.0, 0.0};
+  auto compute = [&](int b) {
+    u32x4 a[2][2][3];
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int pl = 0; pl < 3; ++pl) a[mb][h][pl] = af[b][2 * role + mb][h][pl][lane];
+#pragma unroll
+    for (int nb = 0; nb < NBLK; ++nb) {
+      u32x4 w[2][3];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int pl = 0; pl < 3; ++pl) w[h][pl] = wl[b][pl * NB + nb * 16 + m16][4 * h + g];
+#pragma unroll
+      for (int mb = 0; mb < 2; ++mb) {
+        f32x4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          c = mma_bf16(a[mb][h][0], w[h][2], c);
+          c = mma_bf16(a[mb][h][1], w[h][1], c);
+          c = mma_bf16(a[mb][h][2], w[h][0], c);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          c = mma_bf16(a[mb][h][0], w[h][1], c);
+          c = mma_bf16(a[mb][h][1], w[h][0], c);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) c = mma_bf16(a[mb][h][0], w[h][0], c);
+        fold(acc[mb][nb], c);
+      }
     }
   };
-  // Any step: rows past r1 and columns past k load the centre, so that they
-  // contribute exactly zero after centring.
-  auto load_x_edge = [&](float (&x)[ROWS / 4], long long i0) {
-#pragma unroll
-    for (int v = 0; v < ROWS / 16; ++v)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const long long row = i0 + 16 * v + 4 * g + e;
-        x[4 * v + e] = (cvalid && row < r1) ? xcol[row] : centre;
-      }
-  };
-
-  f64x4 acc[NBLK];
-#pragma unroll
-  for (int nb = 0; nb < NBLK; ++nb) acc[nb] = f64x4{0.0, 0.0, 0.0, 0.0};
-  double s1 = 0.0, s2 = 0.0;
-
-  auto compute = [&](const float (&x)[ROWS / 4], int b) {
-    float p1 = 0.f, p2 = 0.f;
-#pragma unroll
-    for (int h = 0; h < ROWS / 32; ++h) {
-      float v[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) v[j] = x[8 * h + j] - centre;
-      if (stats) {
-#pragma unroll
-        for (int j = 0; j < 8; ++j) { p1 += v[j]; p2 = fmaf(v[j], v[j], p2); }
-      }
-      const Frag3 a = split8(v);
-#pragma unroll
-      for (int nb = 0; nb < NBLK; ++nb) {
-        const u32x4 bh = wl[b][nb * 16 + m16][4 * h + g];
-        const u32x4 bm = wl[b][NB + nb * 16 + m16][4 * h + g];
-        const u32x4 bl = wl[b][2 * NB + nb * 16 + m16][4 * h + g];
-        fold(acc[nb], mma6(a, bh, bm, bl, f32x4{0.f, 0.f, 0.f, 0.f}));
-      }
-      // one sub-step's fragment reads at a time (bounded register pressure;
-      // the other waves hide the LDS latency)
-      __builtin_amdgcn_sched_barrier(0);
-    }
-    if (stats) { s1 += double(p1); s2 += double(p2); }
-  };
-
-  // Branch-free two-stage pipeline over the full steps (the last prefetch
-  // repeats a step), LDS buffers alternating with the register buffers.
-  float xa[ROWS / 4], xb[ROWS / 4];
-  u32x4 wr[WPT];
-  const long long nfull = vec ? (r1 - r0) / ROWS : 0;
-  long long s = 0;
+  // phase s: W of step s + 1 goes from registers into LDS buffer (s + 1) % 2,
+  // the W loads of step s + 2 are issued, then step s is computed
   if (nfull > 0) {
-    load_w(wr, r0);
-    load_x(xa, r0);
-    store_w(wr, 0);
+    auto at = [&](long long step) { return r0 + ROWS * (step < nfull ? step : nfull - 1); };
+    load_w(at(0));
+    store_w(0);
+    load_w(at(1));
     __syncthreads();
-    for (; s + 1 < nfull; s += 2) {
-      load_w(wr, r0 + ROWS * (s + 1));
-      load_x(xb, r0 + ROWS * (s + 1));
-      compute(xa, 0);
-      store_w(wr, 1);
+    for (long long s = 0; s < nfull; s += 2) {
+      store_w(1);
+      load_w(at(s + 2));
+      compute(0);
       __syncthreads();
-      const long long nx = r0 + ROWS * (s + 2 < nfull ? s + 2 : s + 1);
-      load_w(wr, nx);
-      load_x(xa, nx);
-      compute(xb, 1);
-      store_w(wr, 0);
+      if (s + 1 >= nfull) break;
+      store_w(0);
+      load_w(at(s + 3));
+      compute(1);
       __syncthreads();
     }
-    if (s < nfull) { compute(xa, 0); ++s; __syncthreads(); }
   }
-  // the ragged last step (or every step of unaligned columns), unpipelined;
   // the W operand is padded to m64 rows, so its tile is always whole
-  for (long long i0 = r0 + ROWS * s; i0 < r1; i0 += ROWS) {
-    load_w(wr, i0);
-    store_w(wr, 0);
-    load_x_edge(xa, i0);
+  for (long long i0 = r0 + ROWS * nfull; i0 < r1; i0 += ROWS) {
+    load_w(i0);
+    store_w(0);
     __syncthreads();
-    compute(xa, 0);
+    compute(0);
     __syncthreads();
   }
-
   double FPLS_GLOBAL* o = gpw(t.out) + (long long)split * t.slab;
 #pragma unroll
-  for (int nb = 0; nb < NBLK; ++nb) {
-    const int n = nb * 16 + m16;
+  for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const long long oc = cw + 4 * g + r;
-      if (oc < t.k && n < t.N) o[oc + (long long)n * t.ldo] = acc[nb][r];
+    for (int nb = 0; nb < NBLK; ++nb) {
+      const int n = nb * 16 + m16;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long long oc = cw + 16 * mb + 4 * g + r;
+        if (oc < t.k && n < t.N) o[oc + (long long)n * t.ldo] = acc[mb][nb][r];
+      }
     }
-  }
-  if (stats && t.stats != nullptr) {
-    double a1 = s1, a2 = s2;
-    a1 += __shfl_xor(a1, 16, 64);
-    a1 += __shfl_xor(a1, 32, 64);
-    a2 += __shfl_xor(a2, 16, 64);
-    a2 += __shfl_xor(a2, 32, 64);
-    if (g == 0 && cvalid) {
-      double FPLS_GLOBAL* st = gpw(t.stats) + (long long)split * 2 * t.k;
-      st[col] = a1;
-      st[t.k + col] = a2;
-    }
-  }
 }
 
 // ---------------------------------------------------- xw, bf16x6 (FP32 A)
