@@ -43,6 +43,23 @@ __device__ __forceinline__ T load_const(const T* array, long long index) {
   return ((const T FPLS_CONSTANT*)array)[index];
 }
 
+// p[0] + p[stride] + ... + p[(count - 1) stride], summed left to right (the
+// order is fixed, so results are reproducible) with eight loads in flight.
+template<class P>
+__device__ __forceinline__ double sum_strided(P p, long long stride, int count) {
+  double s = 0.0;
+  int k = 0;
+  for (; k + 8 <= count; k += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = p[(k + u) * stride];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u];
+  }
+  for (; k < count; ++k) s += p[k * stride];
+  return s;
+}
+
 __device__ inline double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
@@ -65,6 +82,24 @@ __device__ inline double wg_sum(double v, double* red) {
 #pragma unroll
   for (int w = 0; w < kWaves; ++w) s += red[w];
   return s;
+}
+
+// The same over a work-group of NT threads (NT a multiple of 64); one wave
+// needs no LDS and no barrier.
+template<int NT>
+__device__ inline double wg_sum_n(double v, double* red) {
+  if constexpr (NT == 64) {
+    return wave_sum(v);
+  } else {
+    v = wave_sum(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) s += red[w];
+    return s;
+  }
 }
 
 #endif

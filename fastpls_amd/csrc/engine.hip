@@ -370,7 +370,8 @@ class Model final : public Handle {
     S_.alloc(sizeof(double) * (size_t)p_ * q_);
     S0_.alloc(sizeof(double) * (size_t)p_ * q_);
     const int splits = xtw_splits(n_, p_, std::min(q_, 64));
-    Buf s_part, stats;
+    Buf& s_part = cp_part_;
+    Buf& stats = cp_stats_;
     s_part.alloc(sizeof(double) * (size_t)splits * p_ * q_);
     stats.alloc(sizeof(double) * (size_t)splits * 2 * p_);
     void* wf = frag(xtw_frag_bytes<T>(n_, std::min(q_, 64)));
@@ -390,8 +391,6 @@ class Model final : public Handle {
     launch_finalize_crossprod<T>(X_, n_, p_, q_, scaling_, stats.d(), s_part.d(), splits,
                                  ysum_.d(), mx_.d(), sx_.d(), colss_.d(), S_.d(), S0_.d(),
                                  state_.as<int>() + 2, stream_);
-    // the partial buffers must outlive the kernels that read them
-    FPLS_HIP(hipStreamSynchronize(stream_));
   }
 
   // ------------------------------------------------------------ SIMPLS
@@ -488,20 +487,16 @@ class Model final : public Handle {
     R_.alloc(sizeof(double) * (size_t)p_ * k);
     Q_.alloc(sizeof(double) * (size_t)q_ * k);
     sigma_.alloc(sizeof(double) * k);
-    Buf usable;
-    usable.alloc(sizeof(int));
+    usable_.alloc(sizeof(int));
     int* flags = state_.as<int>() + 2;
+    reserve_plssvd_parts(k, width);
     if (width >= maxdim) {
       dense_svd(k, flags);
-      const int kk = k;
-      FPLS_HIP(hipMemcpy(usable.as<int>(), &kk, sizeof(int), hipMemcpyHostToDevice));
+      launch_fill_int(usable_.as<int>(), 1, k, stream_);
     } else {
-      randomized_svd(k, width, usable.as<int>(), flags);
+      randomized_svd(k, width, usable_.as<int>(), flags);
     }
-    FPLS_HIP(hipStreamSynchronize(stream_));
-    int got = 0;
-    FPLS_HIP(hipMemcpy(&got, usable.as<int>(), sizeof(int), hipMemcpyDeviceToHost));
-    if (got < k) throw std::runtime_error("fastPLS PLS-SVD returned fewer components than requested");
+    // (the usable component count is checked with the other fit flags)
     // scores T = X_s R, Gram TᵀT, prefix weights
     Rs_.alloc(sizeof(double) * (size_t)p_ * k);
     launch_div_rows(R_.d(), sx_.d(), p_, k, Rs_.d(), stream_);
@@ -525,6 +520,25 @@ class Model final : public Handle {
     tproduct<double>(Tsc_.d(), n_, k, nullptr, Tsc_.d(), n_, k, gram_.d());
     W_.alloc(sizeof(double) * (size_t)k * k * q_);
     launch_plssvd_weights(gram_.d(), sigma_.d(), Q_.d(), k, q_, W_.d(), flags, stream_);
+  }
+
+  // Split partials of every product of a PLS-SVD fit fit in part_ at once,
+  // so no product has to grow (and free) it while queued kernels read it.
+  void reserve_plssvd_parts(int k, int w) {
+    auto xtw_part = [&](long long m, long long kk, int N) {
+      return (long long)xtw_splits(m, kk, std::min(N, 64)) * kk * std::min(N, 64);
+    };
+    auto xw_part = [&](long long m, long long kk, int N) {
+      return (long long)xw_splits(m, kk, std::min(N, 64)) * m * std::min(N, 64);
+    };
+    const int mx = std::max({w, q_, k, 1});
+    long long need = 0;
+    for (long long v : {xtw_part(p_, q_, mx), xtw_part(p_, mx, mx), xtw_part(q_, mx, mx),
+                        xtw_part(p_, mx, q_), xtw_part(q_, p_, mx), xw_part(p_, q_, mx),
+                        xw_part(p_, mx, mx), xw_part(q_, p_, mx), xw_part(q_, mx, mx),
+                        xw_part(p_, mx, k), xtw_part(n_, k, k), xw_part(n_, p_, k)})
+      need = std::max(need, v);
+    part_.reserve(sizeof(double) * (size_t)need);
   }
 
   // Tall-skinny QR: CholQR2 in FP64 (two Gram / Cholesky / triangular-apply
@@ -563,11 +577,15 @@ class Model final : public Handle {
   }
 
   void randomized_svd(int k, int w, int* usable, int* flags) {
-    std::vector<double> om((size_t)q_ * w);
+    std::vector<double>& om = omega_host_;
+    om.resize((size_t)q_ * w);
     gaussian(unsigned(seed_), om.size(), om.data());
     tmp1_.alloc(sizeof(double) * om.size());
-    FPLS_HIP(hipMemcpy(tmp1_.d(), om.data(), sizeof(double) * om.size(), hipMemcpyHostToDevice));
-    Buf sample, basis, rev, rbasis, P, Up, Vt, scratch;
+    // om is a member: it outlives the asynchronous copy
+    FPLS_HIP(hipMemcpyAsync(tmp1_.d(), om.data(), sizeof(double) * om.size(), hipMemcpyHostToDevice,
+                            stream_));
+    Buf &sample = svd_ws_[0], &basis = svd_ws_[1], &rev = svd_ws_[2], &rbasis = svd_ws_[3],
+        &P = svd_ws_[4], &Up = svd_ws_[5], &Vt = svd_ws_[6], &scratch = svd_ws_[7];
     sample.alloc(sizeof(double) * (size_t)p_ * w);
     basis.alloc(sizeof(double) * (size_t)p_ * w);
     rev.alloc(sizeof(double) * (size_t)q_ * w);
@@ -591,11 +609,11 @@ class Model final : public Handle {
                          stream_);
     product<double>(basis.d(), p_, w, nullptr, Up.d(), w, k, R_.d());
     launch_transpose(Vt.d(), k, q_, Q_.d(), stream_);
-    FPLS_HIP(hipStreamSynchronize(stream_));
   }
 
   void dense_svd(int k, int* flags) {
-    Buf Sc, Qs, Rm, Ur, Vr, scratch;
+    Buf &Sc = svd_ws_[0], &Qs = svd_ws_[1], &Rm = svd_ws_[2], &Ur = svd_ws_[3],
+        &scratch = svd_ws_[4], &St = svd_ws_[5];
     if (p_ >= q_) {
       Sc.alloc(sizeof(double) * (size_t)p_ * q_);
       FPLS_HIP(hipMemcpyAsync(Sc.d(), S_.d(), sizeof(double) * (size_t)p_ * q_, hipMemcpyDeviceToDevice, stream_));
@@ -611,7 +629,6 @@ class Model final : public Handle {
       Sc.alloc(sizeof(double) * (size_t)p_ * q_);
       launch_transpose(S_.d(), p_, q_, Sc.d(), stream_);  // q x p
       Qs.alloc(sizeof(double) * (size_t)q_ * p_);
-      Buf St;
       St.alloc(sizeof(double) * (size_t)q_ * p_);
       FPLS_HIP(hipMemcpyAsync(St.d(), Sc.d(), sizeof(double) * (size_t)p_ * q_, hipMemcpyDeviceToDevice, stream_));
       launch_qr(Sc.d(), q_, p_, Qs.d(), flags, stream_);
@@ -622,9 +639,7 @@ class Model final : public Handle {
       // Sᵀ = Qs Ur s Vrᵀ  =>  S = Vr s (Qs Ur)ᵀ
       launch_small_svd(Rm.d(), p_, p_, k, Ur.d(), sigma_.d(), R_.d(), scratch.d(), flags, stream_);
       product<double>(Qs.d(), q_, p_, nullptr, Ur.d(), p_, k, Q_.d());
-      FPLS_HIP(hipStreamSynchronize(stream_));
     }
-    FPLS_HIP(hipStreamSynchronize(stream_));
   }
 
   // ------------------------------------------------------------ common
@@ -646,6 +661,11 @@ class Model final : public Handle {
     if (st[2] & kFlagDecomposition) throw std::runtime_error("resident fit failed: decomposition or score-Gram solve failed");
     if (method_ == kSimpls && st[0] < A_)
       throw std::runtime_error("fastPLS SIMPLS returned fewer components than requested");
+    if (method_ == kPlssvd && !usable_.empty()) {
+      int got = 0;
+      FPLS_HIP(hipMemcpy(&got, usable_.as<int>(), sizeof(int), hipMemcpyDeviceToHost));
+      if (got < A_) throw std::runtime_error("fastPLS PLS-SVD returned fewer components than requested");
+    }
   }
 
   void check_path(const int* prefixes, int count) {
@@ -769,6 +789,9 @@ class Model final : public Handle {
   Buf omega_, steps_, fitdev_, simpls_scratch_, cand_, cand_scaled_, t_raw_, ts_part_, ls_part_;
   Buf frag_;  // matrix-core operand workspace of the streaming products
   Buf part_, tmp1_, tmp2_, tmp3_, qr_gram_, qr_rinv_, qr_tmp_, qr_fail_, zero_ints_;
+  // fit workspaces that outlive the kernels reading them (no mid-fit frees)
+  Buf cp_part_, cp_stats_, usable_, svd_ws_[8];
+  std::vector<double> omega_host_;
   bool zero_ready_ = false;
   Buf rows_, scores_, pred64_, outT_, top_, metric_, flags_, obs_, obs_labels_;
   Buf lda_gram_, lda_sums_, lda_linear_, lda_constants_, lda_ridge_, lda_scratch_;

@@ -113,11 +113,8 @@ __global__ __launch_bounds__(kWG) void k_finalize_stats(
     double* mean, double* scale, double* colss, int* flags) {
   const int j = blockIdx.x * kWG + threadIdx.x;
   if (j >= p) return;
-  double s1 = 0.0, s2 = 0.0;
-  for (int sp = 0; sp < splits; ++sp) {
-    s1 += stats_part[(long long)sp * 2 * p + j];
-    s2 += stats_part[(long long)sp * 2 * p + p + j];
-  }
+  const double s1 = sum_strided(stats_part + j, 2LL * p, splits);
+  const double s2 = sum_strided(stats_part + p + j, 2LL * p, splits);
   const double shift = double(X[(long long)j * n]);
   const double dn = double(n);
   double mu = 0.0, sd = 1.0;
@@ -148,8 +145,7 @@ __global__ __launch_bounds__(kWG) void k_finalize_cross(
   const int c = blockIdx.y;
   if (j >= p) return;
   const double delta = double(X[(long long)j * n]) - mean[j];
-  double acc = 0.0;
-  for (int sp = 0; sp < splits; ++sp) acc += s_part[(long long)sp * p * q + j + (long long)c * p];
+  const double acc = sum_strided(s_part + j + (long long)c * p, (long long)p * q, splits);
   const double v = (acc + delta * ysum[c]) / scale[j];
   S[j + (long long)c * p] = v;
   if (S0) S0[j + (long long)c * p] = v;

@@ -149,9 +149,7 @@ __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits,
     double* Vc = f.V + (long long)comp * p;
     for (int i = threadIdx.x; i < p; i += kWG) {
       Rc[i] = f.cand[i + (long long)j * p] * inv;
-      double s = 0.0;
-      for (int sp = 0; sp < f.ls_splits; ++sp)
-        s += f.ls_part[(long long)sp * f.ls_slab + i + (long long)j * p];
+      const double s = sum_strided(f.ls_part + i + (long long)j * p, f.ls_slab, f.ls_splits);
       Vc[i] = s / f.sx[i] * inv;
     }
     __syncthreads();
@@ -207,18 +205,19 @@ __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits,
 
 // --------------------------------------------------------------- tall QR
 // Householder QR; a matrix of up to 6144 doubles is factorised in LDS.
-__global__ __launch_bounds__(kWG) void k_qr(double* A, int m, int n, double* Q, int* flags) {
+template<int NT>
+__global__ __launch_bounds__(NT) void k_qr(double* A, int m, int n, double* Q, int* flags) {
   __shared__ WgScratch sh;
   __shared__ double la[6144];
   __shared__ double lq[6144];
   bool ok;
   if ((long long)m * n <= 6144) {
-    for (int e = threadIdx.x; e < m * n; e += kWG) la[e] = A[e];
+    for (int e = threadIdx.x; e < m * n; e += NT) la[e] = A[e];
     __syncthreads();
-    ok = wg::householder_qr(la, m, m, n, lq, m, sh.tau, sh.dots, sh.red);
-    for (int e = threadIdx.x; e < m * n; e += kWG) Q[e] = lq[e];
+    ok = wg::householder_qr<NT>(la, m, m, n, lq, m, sh.tau, sh.dots, sh.red);
+    for (int e = threadIdx.x; e < m * n; e += NT) Q[e] = lq[e];
   } else {
-    ok = wg::householder_qr(A, m, m, n, Q, m, sh.tau, sh.dots, sh.red);
+    ok = wg::householder_qr<NT>(A, m, m, n, Q, m, sh.tau, sh.dots, sh.red);
   }
   if (!ok && threadIdx.x == 0) atomicOr(flags, kFlagDecomposition);
 }
@@ -237,11 +236,12 @@ __global__ __launch_bounds__(kWG) void k_chol_inv(const double* G, int w, double
   __shared__ double dmax;
   const int tid = threadIdx.x;
   for (int e = tid; e < w * w; e += kWG) Lm[(e % w) + (e / w) * 65] = G[e];
-  if (tid == 0) {
-    bad = (*fail != 0) ? 1 : 0;
-    double d = 0.0;
-    for (int i = 0; i < w; ++i) d = fmax(d, G[i + i * w]);
-    dmax = d;
+  if (tid == 0) bad = (*fail != 0) ? 1 : 0;
+  __syncthreads();
+  if (tid < 64) {
+    double d = tid < w ? Lm[tid + tid * 65] : 0.0;
+    for (int off = 32; off > 0; off >>= 1) d = fmax(d, __shfl_xor(d, off, 64));
+    if (tid == 0) dmax = d;
   }
   __syncthreads();
   // right-looking Cholesky in LDS, one column per step (lower triangle)
@@ -256,8 +256,9 @@ __global__ __launch_bounds__(kWG) void k_chol_inv(const double* G, int w, double
     const double djj = Lm[j + j * 65];
     for (int i = j + 1 + tid; i < w; i += kWG) Lm[i + j * 65] /= djj;
     __syncthreads();
-    for (int e = tid; e < (w - j - 1) * (w - j - 1); e += kWG) {
-      const int i = j + 1 + e % (w - j - 1), c = j + 1 + e / (w - j - 1);
+    const int r = w - j - 1;
+    for (int e = tid; e < r * r; e += kWG) {
+      const int i = j + 1 + e % r, c = j + 1 + e / r;
       if (c <= i) Lm[i + c * 65] -= Lm[i + j * 65] * Lm[c + j * 65];
     }
     __syncthreads();
@@ -294,24 +295,35 @@ __global__ __launch_bounds__(kWG) void k_qr_fallback(double* A, int m, int n, do
 
 // ----------------------------------------------------- rSVD finalisation
 // rsvd.hpp:85-165: eigen route on P Pᵀ with the rank tolerance, else dense SVD.
-__global__ __launch_bounds__(kWG) void k_rsvd_finalize(const double* P, int w, int q, int k,
+// The Gram of a sketch up to 64 wide stays in LDS.
+template<int NT>
+__global__ __launch_bounds__(NT) void k_rsvd_finalize(const double* P, int w, int q, int k,
                                                        double* Up, double* sigma, double* Vt,
                                                        int* usable_out, double* scratch,
                                                        int* flags) {
   __shared__ WgScratch sh;
   __shared__ int usable_sh;
-  double* Gm = scratch;
-  double* Ve = Gm + (long long)w * w;
-  double* vtmp = Ve + (long long)w * w;
-  double* ev = vtmp + (long long)w * w;
-  double* Wc = ev + w + q;                  // w x q copy (dense route)
+  // the eigen route of a sketch up to 64 wide runs in LDS
+  __shared__ double lg[64 * 64], lv[64 * 64], lt[64 * 64], lev[64];
+  __shared__ double lp[4096];
+  if (w * q <= 4096) {
+    for (int e = threadIdx.x; e < w * q; e += NT) lp[e] = P[e];
+    __syncthreads();
+    P = lp;
+  }
+  const bool in_lds = w <= 64;
+  double* Gm = in_lds ? lg : scratch;
+  double* Ve = in_lds ? lv : scratch + (long long)w * w;
+  double* vtmp = in_lds ? lt : scratch + 2LL * w * w;
+  double* ev = in_lds ? lev : scratch + 3LL * w * w;
+  double* Wc = scratch + 3LL * w * w + w + q;  // w x q copy (dense route)
   double* Uc = Wc + (long long)w * q;       // w x q or q x w
   double* Vm = Uc + (long long)w * q;       // up to max(w,q)^2
   double* sv = Vm + (long long)(w > q ? w : q) * (w > q ? w : q);
   int usable = 0;
   if (w <= q) {
-    wg::gemm(w, w, q, 1.0, P, w, false, P, w, true, 0.0, Gm, w);
-    if (wg::jacobi_eigen(Gm, Ve, w, ev, sh.cs, sh.sn, sh.pa, sh.pb, sh.red, vtmp)) {
+    wg::gemm<NT>(w, w, q, 1.0, P, w, false, P, w, true, 0.0, Gm, w);
+    if (wg::jacobi_eigen<NT>(Gm, Ve, w, ev, sh.cs, sh.sn, sh.pa, sh.pb, sh.red, vtmp)) {
       const double largest = fmax(ev[w - 1], 1.0);
       const double tol = DBL_EPSILON * double(w > q ? w : q) * largest;
       if (threadIdx.x == 0) {
@@ -325,13 +337,13 @@ __global__ __launch_bounds__(kWG) void k_rsvd_finalize(const double* P, int w, i
       __syncthreads();
       usable = usable_sh;
       if (usable > 0) {
-        for (int e = threadIdx.x; e < w * usable; e += kWG) {
+        for (int e = threadIdx.x; e < w * usable; e += NT) {
           const int i = e % w, c = e / w;
           Up[i + c * w] = Ve[i + (w - 1 - c) * w];
         }
-        for (int c = threadIdx.x; c < usable; c += kWG) sigma[c] = sqrt(fmax(ev[w - 1 - c], 0.0));
+        for (int c = threadIdx.x; c < usable; c += NT) sigma[c] = sqrt(fmax(ev[w - 1 - c], 0.0));
         __syncthreads();
-        for (int e = threadIdx.x; e < usable * q; e += kWG) {
+        for (int e = threadIdx.x; e < usable * q; e += NT) {
           const int c = e % usable, j = e / usable;
           double acc = 0.0;
           for (int l = 0; l < w; ++l) acc = fma(Up[l + c * w], P[l + j * w], acc);
@@ -345,12 +357,12 @@ __global__ __launch_bounds__(kWG) void k_rsvd_finalize(const double* P, int w, i
   }
   // dense route: SVD of P (w x q)
   if (w >= q) {
-    wg::copy(w, q, P, w, Wc, w);
-    const bool ok = wg::jacobi_svd(Wc, w, q, Uc, Vm, sv, sh.cs, sh.sn, sh.pa, sh.pb, sh.red, nullptr);
+    wg::copy<NT>(w, q, P, w, Wc, w);
+    const bool ok = wg::jacobi_svd<NT>(Wc, w, q, Uc, Vm, sv, sh.cs, sh.sn, sh.pa, sh.pb, sh.red, nullptr);
     const int ret = k < q ? k : q;
-    for (int e = threadIdx.x; e < w * ret; e += kWG) Up[e % w + (e / w) * w] = Uc[e % w + (long long)(e / w) * w];
-    for (int c = threadIdx.x; c < ret; c += kWG) sigma[c] = sv[c];
-    for (int e = threadIdx.x; e < ret * q; e += kWG) {
+    for (int e = threadIdx.x; e < w * ret; e += NT) Up[e % w + (e / w) * w] = Uc[e % w + (long long)(e / w) * w];
+    for (int c = threadIdx.x; c < ret; c += NT) sigma[c] = sv[c];
+    for (int e = threadIdx.x; e < ret * q; e += NT) {
       const int c = e % ret, j = e / ret;
       Vt[c + j * k] = Vm[j + c * q];
     }
@@ -359,19 +371,19 @@ __global__ __launch_bounds__(kWG) void k_rsvd_finalize(const double* P, int w, i
     if (threadIdx.x == 0) *usable_out = ret;
   } else {
     // wide P: decompose Pᵀ (q x w) = U' s V'ᵀ so P = V' s U'ᵀ
-    for (int e = threadIdx.x; e < w * q; e += kWG) {
+    for (int e = threadIdx.x; e < w * q; e += NT) {
       const int i = e % w, j = e / w;
       Wc[j + i * q] = P[i + j * w];
     }
     __syncthreads();
-    const bool ok = wg::jacobi_svd(Wc, q, w, Uc, Vm, sv, sh.cs, sh.sn, sh.pa, sh.pb, sh.red, nullptr);
+    const bool ok = wg::jacobi_svd<NT>(Wc, q, w, Uc, Vm, sv, sh.cs, sh.sn, sh.pa, sh.pb, sh.red, nullptr);
     const int ret = k < w ? k : w;
-    for (int e = threadIdx.x; e < w * ret; e += kWG) {
+    for (int e = threadIdx.x; e < w * ret; e += NT) {
       const int i = e % w, c = e / w;
       Up[i + c * w] = Vm[i + c * w];
     }
-    for (int c = threadIdx.x; c < ret; c += kWG) sigma[c] = sv[c];
-    for (int e = threadIdx.x; e < ret * q; e += kWG) {
+    for (int c = threadIdx.x; c < ret; c += NT) sigma[c] = sv[c];
+    for (int e = threadIdx.x; e < ret * q; e += NT) {
       const int c = e % ret, j = e / ret;
       Vt[c + j * k] = Uc[j + c * q];
     }
@@ -399,112 +411,120 @@ __global__ __launch_bounds__(kWG) void k_small_svd(const double* M, int m, int n
 }
 
 // ------------------------------------------------------ PLS-SVD weights
-// plssvd.hpp:138-164 with linalg.hpp:57-177: Cholesky of the leading k x k
-// block of TᵀT (shared by every prefix), pivoted elimination as fallback.
+// plssvd.hpp:138-164 with linalg.hpp:57-177: prefix k (work-group k - 1)
+// solves G_k L = diag(sigma_1..k) for the leading k x k block of TᵀT by
+// Cholesky (linalg.hpp:4-47), or by pivoted elimination (linalg.hpp:51-114)
+// when a pivot is not positive or the solution is not finite, and writes
+// W_k = L Q_kᵀ (k x q, leading dimension k). Prefixes are independent.
 __global__ __launch_bounds__(kWG) void k_plssvd_weights(const double* G, const double* sigma,
                                                         const double* Q, int A, int q,
                                                         double* W, int* flags) {
-  __shared__ double Lm[64 * 64];
-  __shared__ double X[64 * 64];
-  __shared__ int fail_at;
-  double* F = Lm;  // the pivoted fallback reuses the Cholesky storage
+  __shared__ double F[64 * 65];   // factor (Cholesky or elimination)
+  __shared__ double X[64 * 65];   // right-hand side / solution
+  __shared__ int ok_sh;
   const int tid = threadIdx.x;
+  const int k = blockIdx.x + 1;
   if (A > 64) {
-    if (tid == 0) atomicOr(flags, kFlagDecomposition);
+    if (tid == 0 && blockIdx.x == 0) atomicOr(flags, kFlagDecomposition);
     return;
   }
-  // Cholesky, row by row as linalg.hpp:66-81 orders it; fail_at = first
-  // index whose pivot is not positive (prefixes beyond it need pivoting).
-  if (tid == 0) {
-    fail_at = A;
-    for (int i = 0; i < A && fail_at == A; ++i) {
-      for (int j = 0; j <= i; ++j) {
-        double v = G[i + j * A];
-        for (int l = 0; l < j; ++l) v -= Lm[i + l * 64] * Lm[j + l * 64];
-        if (i == j) {
-          if (!isfinite(v) || v <= 0.0) { fail_at = i; break; }
-          Lm[i + j * 64] = sqrt(v);
-        } else {
-          Lm[i + j * 64] = v / Lm[j + j * 64];
-        }
+  constexpr int L = 65;
+  for (int e = tid; e < k * k; e += kWG) {
+    const int i = e % k, j = e / k;
+    F[i + j * L] = G[i + j * A];
+    X[i + j * L] = i == j ? sigma[j] : 0.0;
+  }
+  if (tid == 0) ok_sh = 1;
+  __syncthreads();
+  // Cholesky, right-looking over columns (lower triangle; the update order
+  // differs from the row-oriented reference only by rounding)
+  for (int j = 0; j < k; ++j) {
+    if (tid == 0) {
+      const double v = F[j + j * L];
+      if (!isfinite(v) || v <= 0.0) ok_sh = 0;
+      else F[j + j * L] = sqrt(v);
+    }
+    __syncthreads();
+    if (!ok_sh) break;
+    const double djj = F[j + j * L];
+    for (int i = j + 1 + tid; i < k; i += kWG) F[i + j * L] /= djj;
+    __syncthreads();
+    const int r = k - j - 1;
+    for (int e = tid; e < r * r; e += kWG) {
+      const int i = j + 1 + e % r, c = j + 1 + e / r;
+      if (c <= i) F[i + c * L] -= F[i + j * L] * F[c + j * L];
+    }
+    __syncthreads();
+  }
+  bool solved = false;
+  if (ok_sh) {
+    // forward then backward substitution, one column of diag(sigma) per thread
+    bool good = true;
+    for (int c = tid; c < k; c += kWG) {
+      for (int i = c; i < k; ++i) {   // rows above c of L⁻¹ e_c sigma_c are zero
+        double v = i == c ? sigma[c] : 0.0;
+        for (int l = c; l < i; ++l) v -= F[i + l * L] * X[l + c * L];
+        X[i + c * L] = v / F[i + i * L];
+      }
+      for (int i = k - 1; i >= 0; --i) {
+        double v = i >= c ? X[i + c * L] : 0.0;
+        for (int l = i + 1; l < k; ++l) v -= F[l + i * L] * X[l + c * L];
+        v /= F[i + i * L];
+        X[i + c * L] = v;
+        if (!isfinite(v)) good = false;
       }
     }
+    solved = __syncthreads_and(good ? 1 : 0) != 0;
   }
-  __syncthreads();
-  for (int k = 1; k <= A; ++k) {
-    double* Wk = W + (long long)(k - 1) * A * q;
-    bool solved = false;
-    if (k <= fail_at) {
-      // fail_at only shrinks, so once the fallback ran Lm is never read again
-      // columns of the right-hand side diag(sigma) in parallel
-      bool good = true;
-      for (int c = tid; c < k; c += kWG) {
+  if (!solved) {
+    // pivoted elimination (linalg.hpp:51-114), serial
+    if (tid == 0) {
+      double scale = 0.0;
+      for (int j = 0; j < k; ++j)
         for (int i = 0; i < k; ++i) {
-          double v = i == c ? sigma[c] : 0.0;
-          for (int l = 0; l < i; ++l) v -= Lm[i + l * 64] * X[l + c * 64];
-          X[i + c * 64] = v / Lm[i + i * 64];
+          F[i + j * L] = G[i + j * A];
+          scale = fmax(scale, fabs(F[i + j * L]));
+          X[i + j * L] = i == j ? sigma[j] : 0.0;
         }
+      const double tol = DBL_EPSILON * double(k) * fmax(scale, 1.0);
+      bool good = true;
+      for (int piv = 0; piv < k && good; ++piv) {
+        int sel = piv;
+        for (int i = piv + 1; i < k; ++i) if (fabs(F[i + piv * L]) > fabs(F[sel + piv * L])) sel = i;
+        if (!isfinite(F[sel + piv * L]) || fabs(F[sel + piv * L]) <= tol) { good = false; break; }
+        if (sel != piv) {
+          for (int c = 0; c < k; ++c) {
+            double t = F[piv + c * L]; F[piv + c * L] = F[sel + c * L]; F[sel + c * L] = t;
+            t = X[piv + c * L]; X[piv + c * L] = X[sel + c * L]; X[sel + c * L] = t;
+          }
+        }
+        for (int i = piv + 1; i < k; ++i) {
+          const double mlt = F[i + piv * L] / F[piv + piv * L];
+          F[i + piv * L] = 0.0;
+          for (int c = piv + 1; c < k; ++c) F[i + c * L] -= mlt * F[piv + c * L];
+          for (int c = 0; c < k; ++c) X[i + c * L] -= mlt * X[piv + c * L];
+        }
+      }
+      for (int c = 0; c < k && good; ++c) {
         for (int i = k - 1; i >= 0; --i) {
-          double v = X[i + c * 64];
-          for (int l = i + 1; l < k; ++l) v -= Lm[l + i * 64] * X[l + c * 64];
-          v /= Lm[i + i * 64];
-          X[i + c * 64] = v;
+          double v = X[i + c * L];
+          for (int l = i + 1; l < k; ++l) v -= F[i + l * L] * X[l + c * L];
+          v /= F[i + i * L];
+          X[i + c * L] = v;
           if (!isfinite(v)) good = false;
         }
       }
-      solved = __syncthreads_and(good ? 1 : 0) != 0;
-    }
-    if (!solved) {
-      // pivoted elimination (linalg.hpp:104-169), serial
-      if (tid == 0) {
-        fail_at = 0;
-        double scale = 0.0;
-        for (int j = 0; j < k; ++j)
-          for (int i = 0; i < k; ++i) {
-            F[i + j * 64] = G[i + j * A];
-            scale = fmax(scale, fabs(F[i + j * 64]));
-            X[i + j * 64] = i == j ? sigma[j] : 0.0;
-          }
-        const double tol = DBL_EPSILON * double(k) * fmax(scale, 1.0);
-        bool good = true;
-        for (int piv = 0; piv < k && good; ++piv) {
-          int sel = piv;
-          for (int i = piv + 1; i < k; ++i) if (fabs(F[i + piv * 64]) > fabs(F[sel + piv * 64])) sel = i;
-          if (!isfinite(F[sel + piv * 64]) || fabs(F[sel + piv * 64]) <= tol) { good = false; break; }
-          if (sel != piv) {
-            for (int c = 0; c < k; ++c) {
-              double t = F[piv + c * 64]; F[piv + c * 64] = F[sel + c * 64]; F[sel + c * 64] = t;
-              t = X[piv + c * 64]; X[piv + c * 64] = X[sel + c * 64]; X[sel + c * 64] = t;
-            }
-          }
-          for (int i = piv + 1; i < k; ++i) {
-            const double mlt = F[i + piv * 64] / F[piv + piv * 64];
-            F[i + piv * 64] = 0.0;
-            for (int c = piv + 1; c < k; ++c) F[i + c * 64] -= mlt * F[piv + c * 64];
-            for (int c = 0; c < k; ++c) X[i + c * 64] -= mlt * X[piv + c * 64];
-          }
-        }
-        for (int c = 0; c < k && good; ++c) {
-          for (int i = k - 1; i >= 0; --i) {
-            double v = X[i + c * 64];
-            for (int l = i + 1; l < k; ++l) v -= F[i + l * 64] * X[l + c * 64];
-            v /= F[i + i * 64];
-            X[i + c * 64] = v;
-            if (!isfinite(v)) good = false;
-          }
-        }
-        if (!good) atomicOr(flags, kFlagDecomposition);
-      }
-      __syncthreads();
-    }
-    // W_k = X (k x k) * Q_kᵀ  -> k x q, leading dimension k
-    for (int e = tid; e < k * q; e += kWG) {
-      const int i = e % k, c = e / k;
-      double acc = 0.0;
-      for (int l = 0; l < k; ++l) acc = fma(X[i + l * 64], Q[c + l * q], acc);
-      Wk[i + c * k] = acc;
+      if (!good) atomicOr(flags, kFlagDecomposition);
     }
     __syncthreads();
+  }
+  // W_k = X (k x k) Q_kᵀ -> k x q, leading dimension k
+  double* Wk = W + (long long)(k - 1) * A * q;
+  for (int e = tid; e < k * q; e += kWG) {
+    const int i = e % k, c = e / k;
+    double acc = 0.0;
+    for (int l = 0; l < k; ++l) acc = fma(X[i + l * L], Q[c + l * q], acc);
+    Wk[i + c * k] = acc;
   }
 }
 
@@ -654,7 +674,7 @@ void launch_simpls_update(const SimplsFitDev* fits, int count, int step, hipStre
 
 void launch_qr(double* A, int m, int n, double* Q, int* flags, hipStream_t s) {
   if (n > kMaxW || m < n) throw std::invalid_argument("QR shape outside the supported range");
-  hipLaunchKernelGGL(k_qr, dim3(1), dim3(kWG), 0, s, A, m, n, Q, flags);
+  hipLaunchKernelGGL((k_qr<kWG>), dim3(1), dim3(kWG), 0, s, A, m, n, Q, flags);
   FPLS_LAUNCHED("k_qr");
 }
 
@@ -675,8 +695,8 @@ void launch_rsvd_finalize(const double* P, int w, int q, int k, double* Up, doub
                           double* Vt, int* usable, double* scratch, int* flags,
                           hipStream_t s) {
   if (w > kMaxW || q > kMaxW) throw std::invalid_argument("sketch too wide");
-  hipLaunchKernelGGL(k_rsvd_finalize, dim3(1), dim3(kWG), 0, s, P, w, q, k, Up, sigma, Vt,
-                     usable, scratch, flags);
+  hipLaunchKernelGGL((k_rsvd_finalize<kWG>), dim3(1), dim3(kWG), 0, s, P, w, q, k, Up, sigma,
+                     Vt, usable, scratch, flags);
   FPLS_LAUNCHED("k_rsvd_finalize");
 }
 
@@ -690,7 +710,7 @@ void launch_small_svd(const double* M, int m, int n, int k, double* U, double* s
 
 void launch_plssvd_weights(const double* G, const double* sigma, const double* Q, int A,
                            int q, double* W, int* flags, hipStream_t s) {
-  hipLaunchKernelGGL(k_plssvd_weights, dim3(1), dim3(kWG), 0, s, G, sigma, Q, A, q, W, flags);
+  hipLaunchKernelGGL(k_plssvd_weights, dim3(unsigned(A > 0 ? A : 1)), dim3(kWG), 0, s, G, sigma, Q, A, q, W, flags);
   FPLS_LAUNCHED("k_plssvd_weights");
 }
 

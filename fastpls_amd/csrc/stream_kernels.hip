@@ -1024,9 +1024,7 @@ __global__ __launch_bounds__(kWG) void k_sum_slabs(const double* part, int split
   double FPLS_GLOBAL* O = gpw(out);
   for (long long i = (long long)blockIdx.x * kWG + threadIdx.x; i < count;
        i += (long long)gridDim.x * kWG) {
-    double s = 0.0;
-    for (int k = 0; k < splits; ++k) s += P[k * stride + i];
-    O[i] = s;
+    O[i] = sum_strided(P + i, stride, splits);
   }
 }
 
@@ -1037,9 +1035,7 @@ __global__ __launch_bounds__(kWG) void k_sum_slabs_list(const SlabSum* list) {
   double FPLS_GLOBAL* O = gpw(e.out);
   for (long long i = (long long)blockIdx.x * kWG + threadIdx.x; i < e.count;
        i += (long long)gridDim.x * kWG) {
-    double s = 0.0;
-    for (int k = 0; k < e.splits; ++k) s += P[k * e.stride + i];
-    O[i] = s;
+    O[i] = sum_strided(P + i, e.stride, e.splits);
   }
 }
 

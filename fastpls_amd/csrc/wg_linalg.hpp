@@ -1,4 +1,5 @@
-// FP64 dense linear algebra executed by ONE work-group of kWG threads.
+// FP64 dense linear algebra executed by ONE work-group of NT threads (NT a
+// multiple of 64, kWG unless stated).
 //
 // Every routine must be called by all threads of the work-group with
 // identical arguments; it synchronises internally and returns with the
@@ -12,11 +13,12 @@ namespace fpls {
 namespace wg {
 
 // C(m x n) = alpha * op(A) * op(B) + beta * C ; thread per output element.
+template<int NT = kWG>
 __device__ inline void gemm(int m, int n, int k, double alpha, const double* A,
                             long lda, bool ta, const double* B, long ldb, bool tb,
                             double beta, double* C, long ldc) {
-  for (long e = threadIdx.x; e < (long)m * n; e += kWG) {
-    const int i = int(e % m), j = int(e / m);
+  for (int e = threadIdx.x; e < m * n; e += NT) {
+    const int i = e % m, j = e / m;
     double acc = 0.0;
     for (int l = 0; l < k; ++l) {
       const double a = ta ? A[l + i * lda] : A[i + l * lda];
@@ -30,10 +32,11 @@ __device__ inline void gemm(int m, int n, int k, double alpha, const double* A,
 
 // C(m x n) = Aᵀ B with A (k x m), B (k x n) and a long inner dimension k:
 // one wave per output, lanes stride the inner dimension (coalesced).
+template<int NT = kWG>
 __device__ inline void gemm_tn_long(int m, int n, int k, const double* A, long lda,
                                     const double* B, long ldb, double* C, long ldc) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int e = wave; e < m * n; e += kWaves) {
+  for (int e = wave; e < m * n; e += (NT / 64)) {
     const int i = e % m, j = e / m;
     double acc = 0.0;
     for (int l = lane; l < k; l += 64) acc = fma(A[l + i * lda], B[l + (long)j * ldb], acc);
@@ -43,32 +46,35 @@ __device__ inline void gemm_tn_long(int m, int n, int k, const double* A, long l
   __syncthreads();
 }
 
+template<int NT = kWG>
 __device__ inline void copy(int m, int n, const double* A, long lda, double* B, long ldb) {
-  for (long e = threadIdx.x; e < (long)m * n; e += kWG) {
-    const int i = int(e % m), j = int(e / m);
+  for (int e = threadIdx.x; e < m * n; e += NT) {
+    const int i = e % m, j = e / m;
     B[i + j * ldb] = A[i + j * lda];
   }
   __syncthreads();
 }
 
 // Euclidean norm of a vector (all threads receive it).
+template<int NT = kWG>
 __device__ inline double norm(const double* v, long n, double* red) {
   double s = 0.0;
-  for (long i = threadIdx.x; i < n; i += kWG) s = fma(v[i], v[i], s);
-  return sqrt(fmax(wg_sum(s, red), 0.0));
+  for (long i = threadIdx.x; i < n; i += NT) s = fma(v[i], v[i], s);
+  return sqrt(fmax(wg_sum_n<NT>(s, red), 0.0));
 }
 
 // Householder economy QR of A (m x n, m >= n), overwritten by the reflectors;
 // the explicit Q (m x n) goes to Q. tau and dots are LDS arrays of n doubles.
 // Returns false on a non-finite column norm.
+template<int NT = kWG>
 __device__ inline bool householder_qr(double* A, long lda, int m, int n, double* Q,
                                       long ldq, double* tau, double* dots, double* red) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   bool finite = true;
   for (int j = 0; j < n; ++j) {
     double s = 0.0;
-    for (long i = j + tid; i < m; i += kWG) s = fma(A[i + j * lda], A[i + j * lda], s);
-    const double norm2 = wg_sum(s, red);
+    for (int i = j + tid; i < m; i += NT) s = fma(A[i + j * lda], A[i + j * lda], s);
+    const double norm2 = wg_sum_n<NT>(s, red);
     const double alpha = A[j + j * lda];
     const double x2 = norm2 - alpha * alpha;
     double tj = 0.0, beta = alpha, scale = 0.0;
@@ -81,53 +87,49 @@ __device__ inline bool householder_qr(double* A, long lda, int m, int n, double*
     }
     __syncthreads();
     if (tj != 0.0) {
-      for (long i = j + 1 + tid; i < m; i += kWG) A[i + j * lda] *= scale;
+      for (int i = j + 1 + tid; i < m; i += NT) A[i + j * lda] *= scale;
       if (tid == 0) A[j + j * lda] = beta;
     }
     if (tid == 0) tau[j] = tj;
     __syncthreads();
     if (tj != 0.0 && j + 1 < n) {
-      for (int c = j + 1 + wave; c < n; c += kWaves) {
+      for (int c = j + 1 + wave; c < n; c += (NT / 64)) {
         double acc = 0.0;
-        for (long i = j + 1 + lane; i < m; i += 64) acc = fma(A[i + j * lda], A[i + c * lda], acc);
+        for (int i = j + 1 + lane; i < m; i += 64) acc = fma(A[i + j * lda], A[i + c * lda], acc);
         acc = wave_sum(acc);
         if (lane == 0) dots[c] = (acc + A[j + c * lda]) * tj;
       }
       __syncthreads();
-      const long rows = m - j;
-      const long total = (long)(n - j - 1) * rows;
-      for (long e = tid; e < total; e += kWG) {
-        const int c = j + 1 + int(e / rows);
-        const long i = j + e % rows;
-        const double v = i == j ? 1.0 : A[i + j * lda];
-        A[i + c * lda] -= dots[c] * v;
+      // one wave per column, lanes over the rows (no index division)
+      for (int c = j + 1 + wave; c < n; c += (NT / 64)) {
+        const double d = dots[c];
+        for (int i = j + lane; i < m; i += 64) {
+          const double v = i == j ? 1.0 : A[i + j * lda];
+          A[i + c * lda] -= d * v;
+        }
       }
       __syncthreads();
     }
   }
-  for (long e = tid; e < (long)m * n; e += kWG) {
-    const long i = e % m;
-    const int c = int(e / m);
-    Q[i + c * ldq] = i == c ? 1.0 : 0.0;
-  }
+  for (int c = wave; c < n; c += (NT / 64))
+    for (int i = lane; i < m; i += 64) Q[i + c * ldq] = i == c ? 1.0 : 0.0;
   __syncthreads();
   for (int j = n - 1; j >= 0; --j) {
     const double tj = tau[j];
     if (tj == 0.0) continue;
-    for (int c = j + wave; c < n; c += kWaves) {
+    for (int c = j + wave; c < n; c += (NT / 64)) {
       double acc = 0.0;
-      for (long i = j + 1 + lane; i < m; i += 64) acc = fma(A[i + j * lda], Q[i + c * ldq], acc);
+      for (int i = j + 1 + lane; i < m; i += 64) acc = fma(A[i + j * lda], Q[i + c * ldq], acc);
       acc = wave_sum(acc);
       if (lane == 0) dots[c] = (acc + Q[j + c * ldq]) * tj;
     }
     __syncthreads();
-    const long rows = m - j;
-    const long total = (long)(n - j) * rows;
-    for (long e = tid; e < total; e += kWG) {
-      const int c = j + int(e / rows);
-      const long i = j + e % rows;
-      const double v = i == j ? 1.0 : A[i + j * lda];
-      Q[i + c * ldq] -= dots[c] * v;
+    for (int c = j + wave; c < n; c += (NT / 64)) {
+      const double d = dots[c];
+      for (int i = j + lane; i < m; i += 64) {
+        const double v = i == j ? 1.0 : A[i + j * lda];
+        Q[i + c * ldq] -= d * v;
+      }
     }
     __syncthreads();
   }
@@ -136,6 +138,7 @@ __device__ inline bool householder_qr(double* A, long lda, int m, int n, double*
 
 // Pair (a, b) of round r for k-th slot of the round-robin (circle) schedule
 // over m (even) players.
+template<int NT = kWG>
 __device__ inline void rr_pair(int m, int r, int k, int& a, int& b) {
   auto at = [&](int pos) { return pos == 0 ? 0 : ((pos - 1 + r) % (m - 1)) + 1; };
   a = at(k);
@@ -147,25 +150,26 @@ __device__ inline void rr_pair(int m, int r, int k, int& a, int& b) {
 // (leading dimension n). On return V holds the eigenvectors as columns and
 // w the eigenvalues, ascending (the dsyevd contract). cs/sn/pa/pb are LDS
 // scratch of n/2+1 entries each. Returns false on non-finite input.
+template<int NT = kWG>
 __device__ inline bool jacobi_eigen(double* A, double* V, int n, double* w, double* cs,
                                     double* sn, int* pa, int* pb, double* red,
                                     double* vtmp /* n x n scratch */) {
   const int tid = threadIdx.x;
-  for (long e = tid; e < (long)n * n; e += kWG) V[e] = (e % n) == (e / n) ? 1.0 : 0.0;
+  for (int e = tid; e < n * n; e += NT) V[e] = (e % n) == (e / n) ? 1.0 : 0.0;
   double fin = 0.0;
-  for (long e = tid; e < (long)n * n; e += kWG) fin += isfinite(A[e]) ? 0.0 : 1.0;
-  if (wg_sum(fin, red) != 0.0) return false;
+  for (int e = tid; e < n * n; e += NT) fin += isfinite(A[e]) ? 0.0 : 1.0;
+  if (wg_sum_n<NT>(fin, red) != 0.0) return false;
   const int m = n + (n & 1);
   const int half = m / 2;
   for (int sweep = 0; sweep < 40 && n > 1; ++sweep) {
     double off = 0.0, dg = 0.0;
-    for (long e = tid; e < (long)n * n; e += kWG) {
+    for (int e = tid; e < n * n; e += NT) {
       const int i = int(e % n), j = int(e / n);
       if (i < j) off = fma(A[e], A[e], off);
       else if (i == j) dg = fma(A[e], A[e], dg);
     }
-    off = wg_sum(off, red);
-    dg = wg_sum(dg, red);
+    off = wg_sum_n<NT>(off, red);
+    dg = wg_sum_n<NT>(dg, red);
     if (off == 0.0 || off <= 1e-34 * dg) break;
     for (int r = 0; r < m - 1; ++r) {
       if (tid < half) {
@@ -187,7 +191,7 @@ __device__ inline bool jacobi_eigen(double* A, double* V, int n, double* w, doub
       }
       __syncthreads();
       // columns of A and V
-      for (long e = tid; e < (long)half * n; e += kWG) {
+      for (int e = tid; e < half * n; e += NT) {
         const int k = int(e / n), i = int(e % n);
         const int a = pa[k], b = pb[k];
         if (b < 0 || sn[k] == 0.0) continue;
@@ -201,7 +205,7 @@ __device__ inline bool jacobi_eigen(double* A, double* V, int n, double* w, doub
       }
       __syncthreads();
       // rows of A
-      for (long e = tid; e < (long)half * n; e += kWG) {
+      for (int e = tid; e < half * n; e += NT) {
         const int k = int(e / n), i = int(e % n);
         const int a = pa[k], b = pb[k];
         if (b < 0 || sn[k] == 0.0) continue;
@@ -214,7 +218,7 @@ __device__ inline bool jacobi_eigen(double* A, double* V, int n, double* w, doub
     }
   }
   // stable ascending order by rank
-  for (int i = tid; i < n; i += kWG) {
+  for (int i = tid; i < n; i += NT) {
     const double di = A[i + i * n];
     int rank = 0;
     for (int j = 0; j < n; ++j) {
@@ -224,7 +228,7 @@ __device__ inline bool jacobi_eigen(double* A, double* V, int n, double* w, doub
     w[rank] = di;
   }
   __syncthreads();
-  for (long e = tid; e < (long)n * n; e += kWG) {
+  for (int e = tid; e < n * n; e += NT) {
     const int i = int(e % n), j = int(e / n);
     const double dj = A[j + j * n];
     int rank = 0;
@@ -235,7 +239,7 @@ __device__ inline bool jacobi_eigen(double* A, double* V, int n, double* w, doub
     vtmp[i + (long)rank * n] = V[i + (long)j * n];
   }
   __syncthreads();
-  for (long e = tid; e < (long)n * n; e += kWG) V[e] = vtmp[e];
+  for (int e = tid; e < n * n; e += NT) V[e] = vtmp[e];
   __syncthreads();
   return true;
 }
@@ -243,18 +247,19 @@ __device__ inline bool jacobi_eigen(double* A, double* V, int n, double* w, doub
 // One-sided (Hestenes) Jacobi SVD of W (m x n, m >= n), leading dimension m.
 // On return: s (n, descending), U (m x n) left vectors, Vm (n x n) right
 // vectors as columns. W is destroyed. cs/sn/pa/pb: LDS, n/2+1 entries.
+template<int NT = kWG>
 __device__ inline bool jacobi_svd(double* W, int m, int n, double* U, double* Vm,
                                   double* s, double* cs, double* sn, int* pa, int* pb,
                                   double* red, double* sc /* 3*(n/2+1) LDS */) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (long e = tid; e < (long)n * n; e += kWG) Vm[e] = (e % n) == (e / n) ? 1.0 : 0.0;
+  for (int e = tid; e < n * n; e += NT) Vm[e] = (e % n) == (e / n) ? 1.0 : 0.0;
   const int mm = n + (n & 1);
   const int half = mm / 2;
   for (int sweep = 0; sweep < 40 && n > 1; ++sweep) {
     double rotated = 0.0;
     for (int r = 0; r < mm - 1; ++r) {
       // dot products per pair: one wave per pair
-      for (int k = wave; k < half; k += kWaves) {
+      for (int k = wave; k < half; k += (NT / 64)) {
         int a, b;
         rr_pair(mm, r, k, a, b);
         double al = 0.0, be = 0.0, ga = 0.0;
@@ -278,7 +283,7 @@ __device__ inline bool jacobi_svd(double* W, int m, int n, double* U, double* Vm
       }
       __syncthreads();
       for (int k = 0; k < half; ++k) rotated += (sn[k] != 0.0) ? 1.0 : 0.0;
-      for (long e = tid; e < (long)half * m; e += kWG) {
+      for (int e = tid; e < half * m; e += NT) {
         const int k = int(e / m), i = int(e % m);
         const int a = pa[k], b = pb[k];
         if (b < 0 || sn[k] == 0.0) continue;
@@ -287,7 +292,7 @@ __device__ inline bool jacobi_svd(double* W, int m, int n, double* U, double* Vm
         W[i + (long)a * m] = c * wa - sv * wb;
         W[i + (long)b * m] = sv * wa + c * wb;
       }
-      for (long e = tid; e < (long)half * n; e += kWG) {
+      for (int e = tid; e < half * n; e += NT) {
         const int k = int(e / n), i = int(e % n);
         const int a = pa[k], b = pb[k];
         if (b < 0 || sn[k] == 0.0) continue;
@@ -302,7 +307,7 @@ __device__ inline bool jacobi_svd(double* W, int m, int n, double* U, double* Vm
   }
   (void)sc;
   // column norms
-  for (int j = wave; j < n; j += kWaves) {
+  for (int j = wave; j < n; j += (NT / 64)) {
     double acc = 0.0;
     for (int i = lane; i < m; i += 64) acc = fma(W[i + (long)j * m], W[i + (long)j * m], acc);
     acc = wave_sum(acc);
@@ -312,7 +317,7 @@ __device__ inline bool jacobi_svd(double* W, int m, int n, double* U, double* Vm
   // rank by descending norm (stable); write U and reorder Vm through U's
   // storage is not possible in place, so permute into U directly and keep a
   // rank table in red-sized chunks via recomputation.
-  for (long e = tid; e < (long)m * n; e += kWG) {
+  for (int e = tid; e < m * n; e += NT) {
     const int i = int(e % m), j = int(e / m);
     const double sj = s[j];
     int rank = 0;
@@ -321,7 +326,7 @@ __device__ inline bool jacobi_svd(double* W, int m, int n, double* U, double* Vm
   }
   __syncthreads();
   // reorder Vm columns using W as scratch (W no longer needed; m >= n)
-  for (long e = tid; e < (long)n * n; e += kWG) {
+  for (int e = tid; e < n * n; e += NT) {
     const int i = int(e % n), j = int(e / n);
     const double sj = s[j];
     int rank = 0;
@@ -330,7 +335,7 @@ __device__ inline bool jacobi_svd(double* W, int m, int n, double* U, double* Vm
   }
   __syncthreads();
   double fin = 0.0;
-  for (long e = tid; e < (long)n * n; e += kWG) { Vm[e] = W[e]; fin += isfinite(W[e]) ? 0.0 : 1.0; }
+  for (int e = tid; e < n * n; e += NT) { Vm[e] = W[e]; fin += isfinite(W[e]) ? 0.0 : 1.0; }
   __syncthreads();
   // sort singular values descending
   if (tid == 0) {
@@ -342,7 +347,7 @@ __device__ inline bool jacobi_svd(double* W, int m, int n, double* U, double* Vm
     }
   }
   __syncthreads();
-  return wg_sum(fin, red) == 0.0;
+  return wg_sum_n<NT>(fin, red) == 0.0;
 }
 
 }  // namespace wg
