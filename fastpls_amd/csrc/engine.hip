@@ -88,6 +88,24 @@ class Model final : public Handle {
 
   double last_fit_seconds() const override { return fit_seconds_; }
 
+  void set_batch_invariant(bool on) override { invariant_ = on; }
+
+  size_t device_bytes() const override {
+    size_t total = 0;
+    for (const Buf* b : {&xown_, &yown_, &labels_dev_, &counts_, &Yc_, &H_, &ysum_, &my_, &mx_,
+                         &sx_, &colss_, &S_, &S0_, &G_, &R_, &Q_, &V_, &Tsc_, &Rs_, &sigma_,
+                         &gram_, &W_, &state_, &omega_, &steps_, &fitdev_, &simpls_scratch_,
+                         &cand_, &cand_scaled_, &t_raw_, &ts_part_, &ls_part_, &frag_, &part_,
+                         &tmp1_, &tmp2_, &tmp3_, &qr_gram_, &qr_rinv_, &qr_tmp_, &qr_fail_,
+                         &zero_ints_, &cp_part_, &cp_stats_, &usable_, &rows_, &scores_,
+                         &pred64_, &outT_, &top_, &metric_, &flags_, &obs_, &obs_labels_,
+                         &lda_gram_, &lda_sums_, &lda_linear_, &lda_constants_, &lda_ridge_,
+                         &lda_scratch_, &var_loadings_, &var_ss_})
+      total += b->bytes();
+    for (const Buf& b : svd_ws_) total += b.bytes();
+    return total;
+  }
+
   void phases(double* out, int count) const override {
     const double v[4] = {fit_seconds_, crossprod_seconds_, score_seconds_,
                          double(n_) * double(p_) * double(sizeof(T))};
@@ -96,12 +114,12 @@ class Model final : public Handle {
 
   // ------------------------------------------------------------ queries
   void predict_path(const void* x, int rows, const int* prefixes, int count, bool lda,
-                    void* out) override {
+                    void* out, bool device_rows) override {
     if (!x || !out || rows < 1 || count < 1) throw std::invalid_argument("invalid resident prediction-path request");
     check_path(prefixes, count);
     if (lda && !classification_) throw std::invalid_argument("LDA requires class labels");
     const int maxp = prefixes[count - 1];
-    project_rows(load_rows(static_cast<const T*>(x), rows), rows, maxp);
+    project_rows(load_rows(static_cast<const T*>(x), rows, device_rows), rows, maxp);
     T* o = static_cast<T*>(out);
     for (int j = 0; j < count; ++j) {
       response_block(rows, prefixes[j], lda, o + (size_t)j * rows * q_out(), nullptr);
@@ -248,8 +266,10 @@ class Model final : public Handle {
     FPLS_HIP(hipStreamSynchronize(stream_));
     for (Buf* b : {&xown_, &yown_, &Yc_, &S_, &S0_, &G_, &V_, &Tsc_, &simpls_scratch_, &cand_,
                    &cand_scaled_, &t_raw_, &ts_part_, &ls_part_, &omega_, &steps_, &fitdev_,
-                   &part_, &tmp1_, &tmp2_, &tmp3_, &H_, &labels_dev_})
+                   &part_, &tmp1_, &tmp2_, &tmp3_, &H_, &labels_dev_, &frag_, &cp_part_,
+                   &cp_stats_, &qr_gram_, &qr_rinv_, &qr_tmp_, &qr_fail_})
       b->release();
+    for (Buf& b : svd_ws_) b.release();
     X_ = nullptr;
     compacted_ = true;
   }
@@ -322,7 +342,7 @@ class Model final : public Handle {
                long long ldw, int N, double* out) {
     for (int c0 = 0; c0 < N; c0 += 64) {
       const int nb = std::min(64, N - c0);
-      const int splits = xw_splits(m, k, nb);
+      const int splits = invariant_ ? xw_splits_by_depth(k) : xw_splits(m, k, nb);
       XwTask<U> t{};
       t.A = A; t.m = m; t.k = k; t.center = center; t.W = W + (long long)c0 * ldw; t.ldw = ldw;
       t.N = nb; t.splits = splits; t.frag = frag(xw_frag_bytes<U>(k, nb));
@@ -798,6 +818,7 @@ class Model final : public Handle {
   Buf var_loadings_, var_ss_;
   int lda_prefix_ = 0;
   int scores_rows_ = 0;
+  bool invariant_ = false;
 };
 
 }  // namespace
@@ -960,7 +981,8 @@ int fastpls_hip_predict_path(void* model, const void* x, int rows, const int* pr
                              size_t error_capacity) {
   return fpls::guarded(error, error_capacity, [&] {
     if (!model) throw std::invalid_argument("null resident model");
-    static_cast<Handle*>(model)->predict_path(x, rows, prefixes, prefix_count, lda != 0, predictions);
+    static_cast<Handle*>(model)->predict_path(x, rows, prefixes, prefix_count, lda != 0, predictions,
+                                              false);
   });
 }
 

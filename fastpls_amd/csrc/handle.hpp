@@ -1,6 +1,8 @@
 // The resident model interface shared by the C ABI thunks and the CV drivers.
 #pragma once
 
+#include <cstddef>
+
 namespace fpls {
 
 // One fitted model with its device buffers and stream
@@ -9,8 +11,10 @@ namespace fpls {
 class Handle {
  public:
   virtual ~Handle() = default;
+  // Responses (or LDA discriminants) per prefix, rows x q x count, in the
+  // model's precision; `device_rows` reads x from device memory.
   virtual void predict_path(const void* x, int rows, const int* prefixes, int count, bool lda,
-                            void* out) = 0;
+                            void* out, bool device_rows) = 0;
   // Top-`top` class indices (1-based) per prefix; `responses` (rows x q x
   // count, optional) receives the response scores. `device_rows` reads x
   // from device memory.
@@ -29,6 +33,11 @@ class Handle {
                         int* implicit, int* cache) = 0;
   virtual void compact(bool prepare_lda) = 0;
   virtual double last_fit_seconds() const = 0;
+  // Query products split the reduction over p by a rule that depends on p
+  // only, so a row's result does not depend on the rows queried with it.
+  virtual void set_batch_invariant(bool on) = 0;
+  // Device bytes the handle holds.
+  virtual size_t device_bytes() const = 0;
   // [0] fit, [1] cross-product pass over X, [2] score pass over X (PLS-SVD)
   // or the whole component loop (SIMPLS), [3] bytes of X, seconds from events.
   virtual void phases(double* out, int count) const = 0;

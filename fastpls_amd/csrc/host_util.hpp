@@ -81,6 +81,12 @@ inline int xw_splits(long long m, long long k, int N) {
   return s;
 }
 
+// Column splits of an xw pass that depend on the depth k only (batch-
+// invariant queries): no split deeper than 2048 columns.
+inline int xw_splits_by_depth(long long k) {
+  return int(std::min<long long>(4096, std::max<long long>(1, (k + 2047) / 2048)));
+}
+
 // simpls.hpp:43-62
 inline int candidate_block_size(long long remaining, long long p, long long q, bool cls, long long n,
                          long long maximum) {

@@ -34,6 +34,17 @@ _CV_CLASSIFICATION = (C.c_int, [_cvp, _ip, _ip, C.c_int, C.c_int, C.c_int, C.c_i
                                 _vp, _ip, _dp, _cp, _sz])
 
 SIGNATURES = {
+    "fastpls_hip_server_create": (_vp, [C.c_double, C.c_int, C.c_int, _cp, _sz]),
+    "fastpls_hip_server_submit_fit": (_ll, [_vp, C.c_int, _cvp, _cvp, _ip, C.c_int, C.c_int,
+                                            C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                            C.c_int, _ull, _cp, _sz]),
+    "fastpls_hip_server_submit_predict": (_ll, [_vp, _ll, _cvp, _ll, C.c_int, C.c_int, C.c_int,
+                                                _vp, _cp, _sz]),
+    "fastpls_hip_server_poll": (C.c_int, [_vp, _ll, C.c_int, _ip, _dp, _cp, _sz]),
+    "fastpls_hip_server_release": (C.c_int, [_vp, _ll, _cp, _sz]),
+    "fastpls_hip_server_export": (C.c_int, [_vp, _ll, C.c_int, _vp, _sz, _cp, _sz]),
+    "fastpls_hip_server_stats": (C.c_int, [_vp, _dp, C.c_int, _cp, _sz]),
+    "fastpls_hip_server_destroy": (None, [_vp]),
     "fastpls_hip_device_info": (C.c_int, [_cp, _sz]),
     "fastpls_hip_gemm": (C.c_int, [_cvp, _cvp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _cp, _sz]),
     "fastpls_hip_simpls_create": (_vp, [_cvp, _cvp, _ip, C.c_int, C.c_int, C.c_int, C.c_int,
@@ -565,3 +576,140 @@ def plssvd_cv_classification(x, labels, folds, classes, prefixes, precision=64, 
     return _cv_classification("fastpls_hip_plssvd_cv_classification", x, labels, folds, classes,
                               prefixes, precision, scaling, lda, oversample, power, seed,
                               store_predictions, store_scores)
+
+
+SERVER_STATS = ("completed", "failed", "predict_p50_ms", "predict_p99_ms", "fit_p50_ms",
+                "fit_p99_ms", "completed_per_s", "peak_bytes", "budget_bytes", "peak_concurrent",
+                "predict_batches", "mean_batch_rows", "max_batch_requests", "span_s",
+                "bytes_in_use", "resident_models")
+
+
+class Server:
+    """fastpls_hip_server_*: continuous batching of fit and predict requests
+    under an HBM budget. Arrays handed to a request are kept alive here until
+    the request is polled as finished."""
+
+    QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, 3
+
+    def __init__(self, budget_bytes: float, workers: int = 4, max_batch_rows: int = 8192):
+        e = _err()
+        h = load_library().fastpls_hip_server_create(float(budget_bytes), int(workers),
+                                                     int(max_batch_rows), e, 1024)
+        if not h:
+            _raise(e)
+        self._h = h
+        self._keep = {}
+        self._meta = {}
+
+    def submit_fit(self, x, y=None, labels=None, classes=None, method=SIMPLS, precision=64,
+                   components=2, scaling=2, oversample=32, power=5, seed=1, device=None):
+        """Host arrays, or device=(x_ptr, y_ptr or labels_ptr, n, p, q) for
+        device-resident inputs. Returns the request id, which is also the id
+        of the fitted model."""
+        e = _err()
+        lib = load_library()
+        if device is not None:
+            xp, rp, n, p, q = device
+            yp, lp = (rp, None) if labels is None else (None, rp)
+            keep = ()
+            rid = lib.fastpls_hip_server_submit_fit(
+                self._h, int(method), C.c_void_p(xp), C.c_void_p(yp) if yp else None,
+                C.cast(C.c_void_p(lp), _ip) if lp else None, int(precision), 1, int(n), int(p),
+                int(q), int(components), int(scaling), int(oversample), int(power), int(seed),
+                e, 1024)
+        else:
+            dt = _dtype(precision)
+            xa = _fmat(x, dt)
+            n, p = xa.shape
+            if labels is not None:
+                la = np.ascontiguousarray(labels, dtype=np.int32)
+                ya, q = None, int(classes)
+            else:
+                ya = _fmat(np.asarray(y).reshape(n, -1), dt)
+                la, q = None, ya.shape[1]
+            keep = (xa, ya, la)
+            rid = lib.fastpls_hip_server_submit_fit(
+                self._h, int(method), _ptr(xa), _ptr(ya), _iptr(la), int(precision), 0, n, p, q,
+                int(components), int(scaling), int(oversample), int(power), int(seed), e, 1024)
+        if rid == 0:
+            _raise(e)
+        self._keep[rid] = keep
+        self._meta[rid] = (int(precision), int(q), int(components))
+        return rid
+
+    def submit_predict(self, model, x=None, prefix=None, device=None):
+        """Host rows x (rows x p), or device=(ptr, ld, rows) for device rows.
+        Returns (request id, output array filled when the request is done)."""
+        precision, q, components = self._meta[model]
+        prefix = components if prefix is None else int(prefix)
+        dt = _dtype(precision)
+        e = _err()
+        lib = load_library()
+        if device is not None:
+            ptr, ld, rows = device
+            out = np.zeros((int(rows), q), dtype=dt, order="F")
+            rid = lib.fastpls_hip_server_submit_predict(self._h, int(model), C.c_void_p(ptr),
+                                                        int(ld), 1, int(rows), prefix, _ptr(out),
+                                                        e, 1024)
+            keep = (out,)
+        else:
+            xa = _fmat(x, dt)
+            rows = xa.shape[0]
+            out = np.zeros((rows, q), dtype=dt, order="F")
+            rid = lib.fastpls_hip_server_submit_predict(self._h, int(model), _ptr(xa), rows, 0,
+                                                        rows, prefix, _ptr(out), e, 1024)
+            keep = (xa, out)
+        if rid == 0:
+            _raise(e)
+        self._keep[rid] = keep
+        return rid, out
+
+    def poll(self, request, wait=True):
+        """(state, latency ms); raises HipError for a failed request."""
+        state = C.c_int(0)
+        lat = C.c_double(0.0)
+        e = _err()
+        if load_library().fastpls_hip_server_poll(self._h, int(request), int(bool(wait)),
+                                                  C.byref(state), C.byref(lat), e, 1024) != 0:
+            _raise(e)
+        if state.value in (self.DONE, self.FAILED):
+            self._keep.pop(request, None)
+            if state.value == self.FAILED:
+                _raise(e)
+        return state.value, lat.value
+
+    def release(self, model):
+        e = _err()
+        if load_library().fastpls_hip_server_release(self._h, int(model), e, 1024) != 0:
+            _raise(e)
+
+    def export(self, model, field, shape):
+        precision = self._meta[model][0]
+        out = np.zeros(shape, dtype=_dtype(precision), order="F")
+        e = _err()
+        f = FIELDS[field] if isinstance(field, str) else int(field)
+        if load_library().fastpls_hip_server_export(self._h, int(model), f, _ptr(out), out.size,
+                                                    e, 1024) != 0:
+            _raise(e)
+        return out
+
+    def stats(self):
+        out = np.zeros(len(SERVER_STATS))
+        e = _err()
+        if load_library().fastpls_hip_server_stats(self._h, _dptr(out), len(SERVER_STATS), e,
+                                                   1024) != 0:
+            _raise(e)
+        return dict(zip(SERVER_STATS, out.tolist()))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().fastpls_hip_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

@@ -231,6 +231,53 @@ int fastpls_hip_plssvd_cv_classification(const void* predictors, const int* labe
                                          int* status, double* metrics, char* error,
                                          size_t error_capacity);
 
+/* Request server (no counterpart in the reference, which has no serving loop;
+ * each request runs the resident handle API of src/cuda_resident_api.cuh):
+ * continuous batching of fit and predict requests on one GPU under an HBM
+ * budget (bytes), with `workers` worker threads and at most max_batch_rows
+ * rows per batched predict. Fits are admitted when their device-byte
+ * estimate fits the budget left by running fits and resident models; a fit
+ * whose estimate exceeds the whole budget is rejected at submission. Fitted
+ * models stay resident (compacted) until released; the model id is the fit
+ * request id. Predicts queued for the same model and prefix run as one
+ * product over their concatenated rows; results are bitwise the same alone
+ * and in a mix. Inputs and outputs must stay valid until the request is
+ * polled as finished; a finished request can be polled once. Destroying the
+ * server waits for running work and drops queued requests. */
+void* fastpls_hip_server_create(double hbm_budget_bytes, int workers, int max_batch_rows,
+                                char* error, size_t error_capacity);
+/* Returns the request id (> 0), 0 on error. Arguments as fastpls_hip_fit_device
+ * (device_inputs != 0) or fastpls_hip_simpls_create / _plssvd_create. */
+long long fastpls_hip_server_submit_fit(void* server, int method, const void* x, const void* y,
+                                        const int* labels, int precision, int device_inputs,
+                                        int n, int p, int q, int components, int scaling,
+                                        int oversample, int power, unsigned long long seed,
+                                        char* error, size_t error_capacity);
+/* rows x p predictors with leading dimension ldx (host, or device memory when
+ * device_inputs != 0); predictions rows x q (host) in the model's precision.
+ * Returns the request id (> 0), 0 on error. */
+long long fastpls_hip_server_submit_predict(void* server, long long model, const void* x,
+                                            long long ldx, int device_inputs, int rows,
+                                            int prefix, void* predictions, char* error,
+                                            size_t error_capacity);
+/* state: 0 queued, 1 running, 2 done, 3 failed (message in error; the call
+ * itself returns 0). wait != 0 blocks until the request finishes. */
+int fastpls_hip_server_poll(void* server, long long request, int wait, int* state,
+                            double* latency_ms, char* error, size_t error_capacity);
+int fastpls_hip_server_release(void* server, long long model, char* error,
+                               size_t error_capacity);
+/* Fields as fastpls_hip_export. */
+int fastpls_hip_server_export(void* server, long long model, int field, void* out,
+                              size_t elements, char* error, size_t error_capacity);
+/* out[0..15]: completed, failed, predict p50 ms, predict p99 ms, fit p50 ms,
+ * fit p99 ms, completed per second, peak bytes in use, budget bytes, peak
+ * concurrent batches, predict batches, mean rows per predict batch, most
+ * requests in one batch, seconds from first submission to last completion,
+ * bytes in use, resident models. */
+int fastpls_hip_server_stats(void* server, double* out, int count, char* error,
+                             size_t error_capacity);
+void fastpls_hip_server_destroy(void* server);
+
 /* Synthetic latent-factor data written straight into device memory
  * (Philox4x32-10 + Box-Muller): X (n x p, precision 32/64) = T P' + noise E,
  * Y (n x q, same precision) = T Q' + noise F, with factor column f scaled by
