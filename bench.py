@@ -31,6 +31,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# Hardware queues per process (HIP's default is 4): the serving section runs
+# many streams at once, and streams beyond the queue count share queues.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E bytes/s
 
@@ -228,6 +231,102 @@ def c3_batched(fp, seed=2026, details_out=None):
     }
 
 
+def c5_serving(fp, requests=10000, rate=500.0, seed=3, workers=32):
+    """Serving stream: `requests` requests with Poisson arrivals at `rate`/s
+    (seed 3) against the request server (HBM budget 200 GB, FP32 storage,
+    bf16x6 matrix-core products with FP32 accumulation). 80% predict 1..4096
+    rows against one of 16 resident SIMPLS models (p ~ U[1000, 20000],
+    q ~ U[1, 20], 10 components, fitted from 4096 latent-factor rows); 20%
+    fit a SIMPLS model (n log-U[1e3, 1e5], p log-U[1e2, 2e4], ncomp
+    U{2..25}, q U{1..20}) on one of 32 latent-factor data sets drawn from
+    that size distribution, and the fitted model is released once done.
+    Predict rows and fit data are resident in HBM before the stream starts
+    (device inputs); latency runs from submission to completion."""
+    rng = np.random.default_rng(seed)
+    keep = []
+    srv = fp.Server(200e9, workers=workers, max_batch_rows=8192)
+    models = []
+    for i in range(16):
+        p = int(rng.integers(1000, 20001)); q = int(rng.integers(1, 21)); n = 4096
+        xb = fp.DeviceBuffer(n * p * 4); yb = fp.DeviceBuffer(n * q * 4)
+        fp.latent_data_device(xb.ptr, yb.ptr, 32, n, p, q, factors=10, noise=0.1, seed=500 + i)
+        mid = srv.submit_fit(None, precision=32, components=10, oversample=10, power=2, seed=1,
+                             device=(xb.ptr, yb.ptr, n, p, q))
+        srv.poll(mid)
+        qb = fp.DeviceBuffer(4096 * p * 4); qy = fp.DeviceBuffer(4096 * q * 4)
+        fp.latent_data_device(qb.ptr, qy.ptr, 32, 4096, p, q, factors=10, noise=0.1, seed=900 + i)
+        keep += [xb, yb, qb, qy]
+        models.append((mid, qb))
+    pool = []
+    for d in range(32):
+        n = int(round(np.exp(rng.uniform(np.log(1e3), np.log(1e5)))))
+        p = int(round(np.exp(rng.uniform(np.log(1e2), np.log(2e4)))))
+        xb = fp.DeviceBuffer(n * p * 4); yb = fp.DeviceBuffer(n * 20 * 4)
+        fp.latent_data_device(xb.ptr, yb.ptr, 32, n, p, 20, factors=10, noise=0.1, seed=700 + d)
+        keep += [xb, yb]
+        pool.append((xb, yb, n, p))
+    data_gb = sum(b.nbytes for b in keep) / 1e9
+    log(f"[serving] 16 resident models and 32 fit data sets staged ({data_gb:.1f} GB)")
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, requests))
+    is_fit = rng.random(requests) < 0.2
+    subs = []
+    t0 = time.perf_counter()
+    for i in range(requests):
+        while True:
+            wait = arrivals[i] - (time.perf_counter() - t0)
+            if wait <= 0:
+                break
+            time.sleep(min(wait, 0.002))
+        if is_fit[i]:
+            xb, yb, n, p = pool[int(rng.integers(0, 32))]
+            A = min(int(rng.integers(2, 26)), p, n - 1)
+            q = int(rng.integers(1, 21))
+            rid = srv.submit_fit(None, precision=32, components=A, oversample=10, power=2,
+                                 seed=1 + i, device=(xb.ptr, yb.ptr, n, p, q))
+            subs.append((True, rid))
+        else:
+            mid, qb = models[int(rng.integers(0, 16))]
+            rows = int(rng.integers(1, 4097))
+            rid, _ = srv.submit_predict(mid, device=(qb.ptr, 4096, rows))
+            subs.append((False, rid))
+        if (i + 1) % 2000 == 0:
+            log(f"[serving] {i + 1}/{requests} submitted")
+    for fit, rid in subs:
+        srv.poll(rid)
+        if fit:
+            srv.release(rid)
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    for b in keep:
+        b.free()
+    log(f"[serving] {requests} requests in {wall:.2f} s: predict p50 {st['predict_p50_ms']:.2f} ms "
+        f"p99 {st['predict_p99_ms']:.2f} ms, fit p50 {st['fit_p50_ms']:.2f} ms "
+        f"p99 {st['fit_p99_ms']:.2f} ms")
+    return {
+        "workload": f"{requests} requests, Poisson {rate:g}/s (seed {seed}); 80% predict "
+                    "(1..4096 rows, 16 resident SIMPLS models, p~U[1e3,2e4], q~U[1,20]); 20% "
+                    "SIMPLS fit (n log-U[1e3,1e5], p log-U[1e2,2e4], ncomp U{2..25}, q U{1..20}, "
+                    "32 staged data sets); HBM budget 200 GB; FP32 storage, bf16x6 MFMA",
+        "offered_per_s": rate,
+        "completed_per_s": round(requests / wall, 2),
+        "predict_p50_ms": round(st["predict_p50_ms"], 3),
+        "predict_p99_ms": round(st["predict_p99_ms"], 3),
+        "fit_p50_ms": round(st["fit_p50_ms"], 3),
+        "fit_p99_ms": round(st["fit_p99_ms"], 3),
+        "failed": int(st["failed"]),
+        "peak_hbm_gb_admitted": round(st["peak_bytes"] / 1e9, 3),
+        "budget_gb": 200,
+        "peak_concurrent_batches": int(st["peak_concurrent"]),
+        "predict_batches": int(st["predict_batches"]),
+        "mean_rows_per_predict_batch": round(st["mean_batch_rows"], 1),
+        "max_requests_per_batch": int(st["max_batch_requests"]),
+        "fit_service_ms_mean": round(st["fit_service_ms"], 3),
+        "predict_service_ms_mean": round(st["predict_service_ms"], 3),
+        "workers": workers,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -243,6 +342,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=10000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -333,6 +434,11 @@ def main():
     if not args.no_batched:
         batched = c3_batched(fp)
 
+    # ------------------------------------------------------------ serving
+    serving = None
+    if not args.no_serving:
+        serving = c5_serving(fp, requests=args.serve_requests)
+
     # ------------------------------------------------------ CPU baseline
     cpu = None
     if not args.no_cpu:
@@ -367,6 +473,7 @@ def main():
         "cpu_baseline": cpu,
         **plain,
         "batched_simpls": batched,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

@@ -55,6 +55,7 @@ class Model final : public Handle {
     }
     A_ = components;
     FPLS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    for (Buf* b : all_bufs()) b->bind(stream_);
     try {
       Event start, stop;
       load_inputs(x, y, labels, device_inputs);
@@ -72,19 +73,12 @@ class Model final : public Handle {
       FPLS_HIP(hipEventElapsedTime(&ms, sc0_.e, sc1_.e));
       score_seconds_ = double(ms) * 1e-3;
     } catch (...) {
-      (void)hipStreamSynchronize(stream_);
-      (void)hipStreamDestroy(stream_);
-      stream_ = nullptr;
+      close_stream();
       throw;
     }
   }
 
-  ~Model() override {
-    if (stream_) {
-      (void)hipStreamSynchronize(stream_);
-      (void)hipStreamDestroy(stream_);
-    }
-  }
+  ~Model() override { close_stream(); }
 
   double last_fit_seconds() const override { return fit_seconds_; }
 
@@ -92,17 +86,7 @@ class Model final : public Handle {
 
   size_t device_bytes() const override {
     size_t total = 0;
-    for (const Buf* b : {&xown_, &yown_, &labels_dev_, &counts_, &Yc_, &H_, &ysum_, &my_, &mx_,
-                         &sx_, &colss_, &S_, &S0_, &G_, &R_, &Q_, &V_, &Tsc_, &Rs_, &sigma_,
-                         &gram_, &W_, &state_, &omega_, &steps_, &fitdev_, &simpls_scratch_,
-                         &cand_, &cand_scaled_, &t_raw_, &ts_part_, &ls_part_, &frag_, &part_,
-                         &tmp1_, &tmp2_, &tmp3_, &qr_gram_, &qr_rinv_, &qr_tmp_, &qr_fail_,
-                         &zero_ints_, &cp_part_, &cp_stats_, &usable_, &rows_, &scores_,
-                         &pred64_, &outT_, &top_, &metric_, &flags_, &obs_, &obs_labels_,
-                         &lda_gram_, &lda_sums_, &lda_linear_, &lda_constants_, &lda_ridge_,
-                         &lda_scratch_, &var_loadings_, &var_ss_})
-      total += b->bytes();
-    for (const Buf& b : svd_ws_) total += b.bytes();
+    for (const Buf* b : const_cast<Model*>(this)->all_bufs()) total += b->bytes();
     return total;
   }
 
@@ -277,6 +261,30 @@ class Model final : public Handle {
  private:
   int q_out() const { return q_; }
 
+  std::vector<Buf*> all_bufs() {
+    std::vector<Buf*> v = {&xown_, &yown_, &labels_dev_, &counts_, &Yc_, &H_, &ysum_, &my_, &mx_,
+                           &sx_, &colss_, &S_, &S0_, &G_, &R_, &Q_, &V_, &Tsc_, &Rs_, &sigma_,
+                           &gram_, &W_, &state_, &omega_, &steps_, &fitdev_, &simpls_scratch_,
+                           &cand_, &cand_scaled_, &t_raw_, &ts_part_, &ls_part_, &frag_, &part_,
+                           &tmp1_, &tmp2_, &tmp3_, &qr_gram_, &qr_rinv_, &qr_tmp_, &qr_fail_,
+                           &zero_ints_, &cp_part_, &cp_stats_, &usable_, &rows_, &scores_,
+                           &pred64_, &outT_, &top_, &metric_, &flags_, &obs_, &obs_labels_,
+                           &lda_gram_, &lda_sums_, &lda_linear_, &lda_constants_, &lda_ridge_,
+                           &lda_scratch_, &var_loadings_, &var_ss_};
+    for (Buf& b : svd_ws_) v.push_back(&b);
+    return v;
+  }
+
+  // Gives every device block back in stream order, then destroys the stream.
+  void close_stream() noexcept {
+    if (!stream_) return;
+    for (Buf* b : all_bufs()) b->release();
+    (void)hipStreamSynchronize(stream_);
+    pool_forget_stream(stream_);
+    (void)hipStreamDestroy(stream_);
+    stream_ = nullptr;
+  }
+
   // --------------------------------------------------------- data load
   void load_inputs(const T* x, const T* y, const int* labels, bool device_inputs) {
     const size_t np = (size_t)n_ * p_;
@@ -331,8 +339,7 @@ class Model final : public Handle {
 
   void fill_zero_int(int* ptr, int count) {
     // A kernel (not hipMemsetAsync) so the stream stays graph-capturable.
-    std::vector<int> zeros(count, 0);
-    FPLS_HIP(hipMemcpy(ptr, zeros.data(), sizeof(int) * count, hipMemcpyHostToDevice));
+    launch_fill_int(ptr, count, 0, stream_);
   }
 
   // --------------------------------------------------------- products
@@ -446,11 +453,16 @@ class Model final : public Handle {
       G_.alloc(sizeof(double) * (size_t)q_ * q_);
       tproduct<double>(S_.d(), p_, q_, nullptr, S_.d(), p_, q_, G_.d());
     }
-    omega_.alloc(sizeof(double) * pool.size());
-    FPLS_HIP(hipMemcpy(omega_.d(), pool.data(), sizeof(double) * pool.size(), hipMemcpyHostToDevice));
-    steps_.alloc(sizeof(RefreshStep) * steps.size());
-    FPLS_HIP(hipMemcpy(steps_.as<RefreshStep>(), steps.data(), sizeof(RefreshStep) * steps.size(),
-                       hipMemcpyHostToDevice));
+    // host copies are members: they outlive the asynchronous uploads
+    omega_host_ = std::move(pool);
+    steps_host_ = std::move(steps);
+    omega_.alloc(sizeof(double) * omega_host_.size());
+    FPLS_HIP(hipMemcpyAsync(omega_.d(), omega_host_.data(), sizeof(double) * omega_host_.size(),
+                            hipMemcpyHostToDevice, stream_));
+    steps_.alloc(sizeof(RefreshStep) * steps_host_.size());
+    FPLS_HIP(hipMemcpyAsync(steps_.as<RefreshStep>(), steps_host_.data(),
+                            sizeof(RefreshStep) * steps_host_.size(), hipMemcpyHostToDevice, stream_));
+    const std::vector<RefreshStep>& steps_ref = steps_host_;
     simpls_scratch_.alloc(sizeof(double) * (size_t)simpls_scratch_doubles(p_, q_, wmax));
     cand_.alloc(sizeof(double) * (size_t)p_ * bmax);
     cand_scaled_.alloc(sizeof(double) * (size_t)p_ * bmax);
@@ -464,7 +476,7 @@ class Model final : public Handle {
     f.n = n_; f.p = p_; f.q = q_; f.A = A_;
     f.power = power_; f.use_right_gram = right_gram ? 1 : 0;
     f.reorth = sizeof(T) == 4 ? 1 : 0;  // src/r_api.cpp:1416: two passes for float
-    f.nsteps = int(steps.size()); f.bmax = bmax; f.ts_splits = xs; f.ls_splits = ls;
+    f.nsteps = int(steps_ref.size()); f.bmax = bmax; f.ts_splits = xs; f.ls_splits = ls;
     f.ls_slab = (long long)p_ * bmax;
     f.steps = steps_.as<RefreshStep>(); f.omega = omega_.d();
     f.S = S_.d(); f.S0 = S0_.d(); f.G = right_gram ? G_.d() : nullptr; f.sx = sx_.d();
@@ -472,12 +484,14 @@ class Model final : public Handle {
     f.ts_part = ts_part_.d(); f.ls_part = ls_part_.d();
     f.R = R_.d(); f.Q = Q_.d(); f.V = V_.d(); f.T = Tsc_.d();
     f.scratch = simpls_scratch_.d(); f.state = state_.as<int>();
+    fitdev_host_ = f;
     fitdev_.alloc(sizeof(SimplsFitDev));
-    FPLS_HIP(hipMemcpy(fitdev_.as<SimplsFitDev>(), &f, sizeof(f), hipMemcpyHostToDevice));
+    FPLS_HIP(hipMemcpyAsync(fitdev_.as<SimplsFitDev>(), &fitdev_host_, sizeof(f), hipMemcpyHostToDevice,
+                            stream_));
 
     FPLS_HIP(hipEventRecord(sc0_.e, stream_));
-    for (size_t s = 0; s < steps.size(); ++s) {
-      const int b = steps[s].b;
+    for (size_t s = 0; s < steps_ref.size(); ++s) {
+      const int b = steps_ref[s].b;
       launch_simpls_refresh(fitdev_.as<SimplsFitDev>(), 1, int(s), stream_);
       // scores of the candidate block: X_s C = (X - 1 mᵀ)(C / s)
       XwTask<T> xw{};
@@ -587,9 +601,8 @@ class Model final : public Handle {
   void fill_zero_int_async(int* ptr, int count) {
     zero_ints_.reserve(sizeof(int) * 64);
     if (count > 64) throw std::invalid_argument("zero fill too large");
-    static const int zeros[64] = {0};
     if (!zero_ready_) {
-      FPLS_HIP(hipMemcpy(zero_ints_.as<int>(), zeros, sizeof(zeros), hipMemcpyHostToDevice));
+      launch_fill_int(zero_ints_.as<int>(), 64, 0, stream_);
       zero_ready_ = true;
     }
     FPLS_HIP(hipMemcpyAsync(ptr, zero_ints_.as<int>(), sizeof(int) * count,
@@ -683,7 +696,8 @@ class Model final : public Handle {
       throw std::runtime_error("fastPLS SIMPLS returned fewer components than requested");
     if (method_ == kPlssvd && !usable_.empty()) {
       int got = 0;
-      FPLS_HIP(hipMemcpy(&got, usable_.as<int>(), sizeof(int), hipMemcpyDeviceToHost));
+      FPLS_HIP(hipMemcpyAsync(&got, usable_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, stream_));
+      FPLS_HIP(hipStreamSynchronize(stream_));
       if (got < A_) throw std::runtime_error("fastPLS PLS-SVD returned fewer components than requested");
     }
   }
@@ -764,6 +778,7 @@ class Model final : public Handle {
     lda_ridge_.reserve(sizeof(double));
     lda_scratch_.reserve(sizeof(double) * (size_t)q_ * A_);
     Buf fl;
+    fl.bind(stream_);
     fl.alloc(sizeof(int));
     fill_zero_int(fl.as<int>(), 1);
     launch_lda_prepare(lda_gram_.d(), lda_sums_.d(), counts_.as<int>(), n_, A_, q_, k,
@@ -781,6 +796,7 @@ class Model final : public Handle {
     if (!var_ss_.empty()) return;
     if (X_ == nullptr || Tsc_.empty()) throw std::runtime_error("variance needs the training data; export it before compact");
     Buf cross, gram, seq;
+    for (Buf* b : {&cross, &gram, &seq}) b->bind(stream_);
     cross.alloc(sizeof(double) * (size_t)p_ * A_);
     tproduct<T>(X_, n_, p_, mx_.d(), Tsc_.d(), n_, A_, cross.d());
     launch_div_rows(cross.d(), sx_.d(), p_, A_, cross.d(), stream_);
@@ -812,6 +828,8 @@ class Model final : public Handle {
   // fit workspaces that outlive the kernels reading them (no mid-fit frees)
   Buf cp_part_, cp_stats_, usable_, svd_ws_[8];
   std::vector<double> omega_host_;
+  std::vector<RefreshStep> steps_host_;
+  SimplsFitDev fitdev_host_{};
   bool zero_ready_ = false;
   Buf rows_, scores_, pred64_, outT_, top_, metric_, flags_, obs_, obs_labels_;
   Buf lda_gram_, lda_sums_, lda_linear_, lda_constants_, lda_ridge_, lda_scratch_;

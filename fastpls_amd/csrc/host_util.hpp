@@ -9,28 +9,56 @@
 #include <cstdio>
 #include <exception>
 #include <random>
+#include <stdexcept>
 
 namespace fpls {
 
 // ------------------------------------------------------------- buffers
+// Device block cache shared by every handle of the process (pool.hip). A
+// block given back with the stream that last used it is handed out again at
+// once to that stream (stream order covers the earlier work) and to any
+// other stream once an event recorded at the give-back has completed, so
+// freeing never synchronises the device.
+void* pool_take(size_t bytes, hipStream_t stream, size_t* capacity);
+void pool_give(void* ptr, size_t capacity, hipStream_t stream) noexcept;
+// Called once `stream` has been drained and before it is destroyed: its
+// blocks become idle and no event recorded on it is queried again.
+void pool_forget_stream(hipStream_t stream) noexcept;
+
+// Device buffer. Unbound buffers own their memory directly (hipMalloc /
+// hipFree); buffers bound to a stream take it from the block cache and give
+// it back in that stream's order, so every use of a bound buffer must be
+// ordered on its stream.
 class Buf {
  public:
   Buf() = default;
   Buf(const Buf&) = delete;
   Buf& operator=(const Buf&) = delete;
   ~Buf() { release(); }
+  void bind(hipStream_t stream) {
+    if (ptr_) throw std::logic_error("bind a device buffer before its first allocation");
+    stream_ = stream;
+    bound_ = true;
+  }
   void alloc(size_t bytes) {
     release();
-    if (bytes) FPLS_HIP(hipMalloc(&ptr_, bytes));
+    if (bytes) {
+      if (bound_) ptr_ = pool_take(bytes, stream_, &cap_);
+      else FPLS_HIP(hipMalloc(&ptr_, bytes));
+    }
     bytes_ = bytes;
   }
   void reserve(size_t bytes) {
     if (bytes > bytes_) alloc(bytes);
   }
   void release() noexcept {
-    if (ptr_) (void)hipFree(ptr_);
+    if (ptr_) {
+      if (bound_) pool_give(ptr_, cap_, stream_);
+      else (void)hipFree(ptr_);
+    }
     ptr_ = nullptr;
     bytes_ = 0;
+    cap_ = 0;
   }
   template<class U> U* as() const { return static_cast<U*>(ptr_); }
   double* d() const { return static_cast<double*>(ptr_); }
@@ -40,6 +68,9 @@ class Buf {
  private:
   void* ptr_ = nullptr;
   size_t bytes_ = 0;
+  size_t cap_ = 0;
+  hipStream_t stream_ = nullptr;
+  bool bound_ = false;
 };
 
 // Gaussian sketch exactly as the reference draws it (simpls.hpp:244-249).

@@ -142,6 +142,10 @@ void launch_variance(const double* cross, const double* gram, const double* cols
 // p[0..count) = value (a kernel, so streams stay graph-capturable).
 void launch_fill_int(int* p, int count, int value, hipStream_t s);
 
+// Block copy: dst (rows x cols, leading dimension ldd) = src (leading dimension lds).
+template<class T>
+void launch_copy_block(const T* src, long long lds, long long rows, long long cols, T* dst,
+                       long long ldd, hipStream_t s);
 // Row gather: dst (count x cols) = src[idx, :] for a column-major src (n x cols).
 template<class T>
 void launch_gather_rows(const T* src, long long n, long long cols, const int* idx, long long count,

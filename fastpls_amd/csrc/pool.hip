@@ -1,0 +1,155 @@
+// Process-wide cache of device blocks for stream-bound buffers.
+//
+// Sizes are rounded up to one of four classes per power of two (at most 25%
+// slack). A given-back block keeps the stream that last used it and an event
+// recorded there; a later request of the same class takes it at once on that
+// stream, or on another stream once the event has completed. Blocks are
+// never returned to the device while the process runs, except that the
+// cache drops completed blocks (largest first) when what it holds idle
+// exceeds kIdleLimit.
+#include "host_util.hpp"
+
+#include <map>
+#include <mutex>
+#include <vector>
+
+namespace fpls {
+namespace {
+
+constexpr size_t kIdleLimit = size_t(48) << 30;
+
+struct Block {
+  void* ptr;
+  hipStream_t stream;   // null once the stream is drained and forgotten
+  hipEvent_t ready;     // null once the block is known to be idle
+};
+
+// Whether the work that last used a block has finished. hipEventQuery
+// reports "not ready" through the thread's last-error slot, which the launch
+// checks read, so the slot is cleared.
+bool idle_now(const Block& b) {
+  if (b.ready == nullptr) return true;
+  const hipError_t st = hipEventQuery(b.ready);
+  (void)hipGetLastError();
+  return st == hipSuccess;
+}
+
+struct Pool {
+  std::mutex mu;
+  std::map<size_t, std::vector<Block>> idle;   // by size class
+  std::vector<hipEvent_t> spare_events;
+  size_t idle_bytes = 0;
+};
+
+Pool& pool() {
+  static Pool* p = new Pool();   // lives until the process ends
+  return *p;
+}
+
+size_t size_class(size_t bytes) {
+  if (bytes <= 4096) return 4096;
+  size_t top = 1;
+  while ((top << 1) <= bytes) top <<= 1;   // largest power of two <= bytes
+  const size_t step = top / 4;
+  return (bytes + step - 1) / step * step;
+}
+
+// Drop completed idle blocks (largest first) until the idle total is under
+// the limit; called with the lock held.
+void trim(Pool& P) {
+  for (auto it = P.idle.rbegin(); it != P.idle.rend() && P.idle_bytes > kIdleLimit; ++it) {
+    auto& v = it->second;
+    for (size_t i = 0; i < v.size() && P.idle_bytes > kIdleLimit;) {
+      if (idle_now(v[i])) {
+        (void)hipFree(v[i].ptr);
+        if (v[i].ready) P.spare_events.push_back(v[i].ready);
+        P.idle_bytes -= it->first;
+        v.erase(v.begin() + long(i));
+      } else {
+        ++i;
+      }
+    }
+  }
+}
+
+}  // namespace
+
+void* pool_take(size_t bytes, hipStream_t stream, size_t* capacity) {
+  const size_t cls = size_class(bytes);
+  *capacity = cls;
+  Pool& P = pool();
+  {
+    std::lock_guard<std::mutex> lock(P.mu);
+    auto it = P.idle.find(cls);
+    if (it != P.idle.end()) {
+      auto& v = it->second;
+      for (size_t i = 0; i < v.size(); ++i) {
+        if ((v[i].stream != nullptr && v[i].stream == stream) || idle_now(v[i])) {
+          void* ptr = v[i].ptr;
+          if (v[i].ready) P.spare_events.push_back(v[i].ready);
+          v.erase(v.begin() + long(i));
+          P.idle_bytes -= cls;
+          return ptr;
+        }
+      }
+    }
+  }
+  void* ptr = nullptr;
+  hipError_t st = hipMalloc(&ptr, cls);
+  if (st != hipSuccess) {
+    // give idle memory back to the device and try once more
+    (void)hipGetLastError();
+    {
+      std::lock_guard<std::mutex> lock(P.mu);
+      for (auto& kv : P.idle)
+        for (Block& b : kv.second) {
+          if (b.ready) {
+            (void)hipEventSynchronize(b.ready);
+            P.spare_events.push_back(b.ready);
+          }
+          (void)hipFree(b.ptr);
+        }
+      P.idle.clear();
+      P.idle_bytes = 0;
+    }
+    FPLS_HIP(hipMalloc(&ptr, cls));
+  }
+  return ptr;
+}
+
+void pool_give(void* ptr, size_t capacity, hipStream_t stream) noexcept {
+  Pool& P = pool();
+  std::lock_guard<std::mutex> lock(P.mu);
+  hipEvent_t ev = nullptr;
+  if (!P.spare_events.empty()) {
+    ev = P.spare_events.back();
+    P.spare_events.pop_back();
+  } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+    (void)hipStreamSynchronize(stream);   // no event: the block is free once the stream drains
+    (void)hipFree(ptr);
+    return;
+  }
+  if (hipEventRecord(ev, stream) != hipSuccess) {
+    (void)hipStreamSynchronize(stream);
+    (void)hipFree(ptr);
+    P.spare_events.push_back(ev);
+    return;
+  }
+  P.idle[capacity].push_back(Block{ptr, stream, ev});
+  P.idle_bytes += capacity;
+  if (P.idle_bytes > kIdleLimit) trim(P);
+}
+
+void pool_forget_stream(hipStream_t stream) noexcept {
+  Pool& P = pool();
+  std::lock_guard<std::mutex> lock(P.mu);
+  for (auto& kv : P.idle)
+    for (Block& b : kv.second)
+      if (b.stream == stream) {
+        b.stream = nullptr;
+        if (b.ready) P.spare_events.push_back(b.ready);
+        b.ready = nullptr;
+      }
+}
+
+}  // namespace fpls

@@ -1,6 +1,8 @@
 // Preprocessing, prediction-side and data-generation kernels.
 #include "kernels.hpp"
 
+#include <algorithm>
+
 #include <cfloat>
 
 namespace fpls {
@@ -156,12 +158,21 @@ __global__ __launch_bounds__(kWG) void k_finalize_cross(
 template<class T>
 __global__ __launch_bounds__(kWG) void k_gather_rows(const T* src, long long n, long long cols,
                                                      const int* idx, long long count, T* dst) {
-  const long long total = count * cols;
-  for (long long e = (long long)blockIdx.x * kWG + threadIdx.x; e < total;
-       e += (long long)gridDim.x * kWG) {
-    const long long i = e % count, j = e / count;
-    dst[e] = src[(long long)idx[i] + j * n];
-  }
+  // columns over grid y, rows over threads (no index division)
+  for (long long j = blockIdx.y; j < cols; j += gridDim.y)
+    for (long long i = (long long)blockIdx.x * kWG + threadIdx.x; i < count;
+         i += (long long)gridDim.x * kWG)
+      dst[i + j * count] = src[(long long)idx[i] + j * n];
+}
+
+// dst (rows x cols, leading dimension ldd) = src (leading dimension lds).
+template<class T>
+__global__ __launch_bounds__(kWG) void k_copy_block(const T* src, long long lds, long long rows,
+                                                    long long cols, T* dst, long long ldd) {
+  for (long long j = blockIdx.y; j < cols; j += gridDim.y)
+    for (long long i = (long long)blockIdx.x * kWG + threadIdx.x; i < rows;
+         i += (long long)gridDim.x * kWG)
+      dst[i + j * ldd] = src[i + j * lds];
 }
 
 __global__ __launch_bounds__(kWG) void k_div_rows(const double* in, const double* d,
@@ -400,10 +411,28 @@ template<class T>
 void launch_gather_rows(const T* src, long long n, long long cols, const int* idx,
                         long long count, T* dst, hipStream_t s) {
   if (count <= 0 || cols <= 0) return;
-  hipLaunchKernelGGL((k_gather_rows<T>), dim3(grid_for(count * cols)), dim3(kWG), 0, s, src, n,
-                     cols, idx, count, dst);
+  if (count <= 0 || cols <= 0) return;
+  const long long bx = std::min<long long>((count + kWG - 1) / kWG, 64);
+  const long long by = std::min<long long>(cols, 65535);
+  hipLaunchKernelGGL((k_gather_rows<T>), dim3(unsigned(bx), unsigned(by)), dim3(kWG), 0, s, src,
+                     n, cols, idx, count, dst);
   FPLS_LAUNCHED("k_gather_rows");
 }
+
+template<class T>
+void launch_copy_block(const T* src, long long lds, long long rows, long long cols, T* dst,
+                       long long ldd, hipStream_t s) {
+  if (rows <= 0 || cols <= 0) return;
+  const long long bx = std::min<long long>((rows + kWG - 1) / kWG, 64);
+  const long long by = std::min<long long>(cols, 65535);
+  hipLaunchKernelGGL((k_copy_block<T>), dim3(unsigned(bx), unsigned(by)), dim3(kWG), 0, s, src,
+                     lds, rows, cols, dst, ldd);
+  FPLS_LAUNCHED("k_copy_block");
+}
+template void launch_copy_block<float>(const float*, long long, long long, long long, float*,
+                                       long long, hipStream_t);
+template void launch_copy_block<double>(const double*, long long, long long, long long, double*,
+                                        long long, hipStream_t);
 template void launch_gather_rows<float>(const float*, long long, long long, const int*, long long,
                                         float*, hipStream_t);
 template void launch_gather_rows<double>(const double*, long long, long long, const int*,

@@ -18,6 +18,7 @@
 #include "../../include/fastpls_hip.h"
 #include "handle.hpp"
 #include "host_util.hpp"
+#include "kernels.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -59,7 +60,7 @@ struct Request {
   // outcome
   int state = kQueued;
   std::string error;
-  Clock::time_point submitted, done;
+  Clock::time_point submitted, started, done;
 };
 
 struct Resident {
@@ -213,14 +214,16 @@ class Server {
     };
     const double span = completed_ > 0
         ? std::chrono::duration<double>(last_done_ - first_submit_).count() : 0.0;
-    const double v[16] = {double(completed_), double(failed_), pct(lat_predict_, 0.5),
+    const double v[18] = {double(completed_), double(failed_), pct(lat_predict_, 0.5),
                           pct(lat_predict_, 0.99), pct(lat_fit_, 0.5), pct(lat_fit_, 0.99),
                           span > 0.0 ? double(completed_) / span : 0.0, double(peak_in_use_),
                           double(budget_), double(peak_running_), double(batches_),
                           batches_ ? double(batch_rows_) / double(batches_) : 0.0,
                           double(max_batch_requests_), span, double(in_use_),
-                          double(models_.size())};
-    for (int i = 0; i < count && i < 16; ++i) out[i] = v[i];
+                          double(models_.size()),
+                          lat_fit_.empty() ? 0.0 : service_fit_ms_ / double(lat_fit_.size()),
+                          lat_predict_.empty() ? 0.0 : service_predict_ms_ / double(lat_predict_.size())};
+    for (int i = 0; i < count && i < 18; ++i) out[i] = v[i];
   }
 
  private:
@@ -287,6 +290,8 @@ class Server {
       ++completed_;
       (r->kind == kFitRequest ? lat_fit_ : lat_predict_)
           .push_back(std::chrono::duration<double, std::milli>(r->done - r->submitted).count());
+      (r->kind == kFitRequest ? service_fit_ms_ : service_predict_ms_) +=
+          std::chrono::duration<double, std::milli>(r->done - r->started).count();
     } else {
       ++failed_;
     }
@@ -295,22 +300,30 @@ class Server {
 
   void worker(int w) {
     (void)hipSetDevice(device_);
+    hipStream_t ws = nullptr;    // staging copies of this worker
+    if (hipStreamCreateWithFlags(&ws, hipStreamNonBlocking) != hipSuccess) ws = nullptr;
     Buf staging;                 // concatenated predict rows on the device
+    Buf upload;                  // one request's host rows
     std::vector<char> host_out;  // concatenated predictions
     for (;;) {
       std::vector<Request*> work;
       {
         std::unique_lock<std::mutex> lock(mu_);
         cv_.wait(lock, [&] { return stop_ || pick(work); });
-        if (work.empty()) return;  // stopping
-        for (Request* r : work) r->state = kRunning;
+        if (work.empty()) {  // stopping
+          lock.unlock();
+          if (ws) (void)hipStreamDestroy(ws);
+          return;
+        }
+        const Clock::time_point now = Clock::now();
+        for (Request* r : work) { r->state = kRunning; r->started = now; }
         ++running_;
         peak_running_ = std::max(peak_running_, running_);
         peak_in_use_ = std::max(peak_in_use_, in_use_);
       }
       done_cv_.notify_all();
       if (work[0]->kind == kFitRequest) run_fit(work[0]);
-      else run_predict(work, staging, host_out);
+      else run_predict(work, ws, staging, upload, host_out);
       {
         std::lock_guard<std::mutex> lock(mu_);
         --running_;
@@ -348,7 +361,8 @@ class Server {
     finish(r, error);
   }
 
-  void run_predict(std::vector<Request*>& work, Buf& staging, std::vector<char>& host_out) {
+  void run_predict(std::vector<Request*>& work, hipStream_t ws, Buf& staging, Buf& upload,
+                   std::vector<char>& host_out) {
     Handle* h = nullptr;
     int p = 0, q = 0, precision = 64;
     {
@@ -362,15 +376,31 @@ class Server {
     for (Request* r : work) total += r->rows;
     std::string error;
     try {
+      if (!ws) throw std::runtime_error("server worker has no stream");
       staging.reserve(s * (size_t)total * p);
-      // column-major concatenation: request i fills rows [off_i, off_i + rows_i)
+      // column-major concatenation: request i fills rows [off_i, off_i + rows_i);
+      // host rows go up in one contiguous copy, then a device block copy
+      // places them
       int off = 0;
       for (Request* r : work) {
-        FPLS_HIP(hipMemcpy2D(staging.as<char>() + s * (size_t)off, s * (size_t)total, r->x,
-                             s * (size_t)r->ldx, s * (size_t)r->rows, (size_t)p,
-                             r->device_inputs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+        const void* src = r->x;
+        long long lds = r->ldx;
+        if (!r->device_inputs) {
+          upload.reserve(s * (size_t)r->ldx * p);
+          FPLS_HIP(hipMemcpyAsync(upload.as<void>(), r->x, s * (size_t)r->ldx * p,
+                                  hipMemcpyHostToDevice, ws));
+          src = upload.as<void>();
+        }
+        if (precision == 32)
+          launch_copy_block<float>(static_cast<const float*>(src), lds, r->rows, p,
+                                   staging.as<float>() + off, total, ws);
+        else
+          launch_copy_block<double>(static_cast<const double*>(src), lds, r->rows, p,
+                                    staging.as<double>() + off, total, ws);
+        if (!r->device_inputs) FPLS_HIP(hipStreamSynchronize(ws));  // upload is reused
         off += r->rows;
       }
+      FPLS_HIP(hipStreamSynchronize(ws));
       host_out.resize(s * (size_t)total * q);
       const int prefix = work[0]->prefix;
       h->predict_path(staging.as<void>(), total, &prefix, 1, false, host_out.data(), true);
@@ -408,6 +438,7 @@ class Server {
   bool any_submitted_ = false;
   Clock::time_point first_submit_, last_done_;
   std::vector<double> lat_fit_, lat_predict_;
+  double service_fit_ms_ = 0.0, service_predict_ms_ = 0.0;
   std::vector<std::thread> threads_;
 };
 

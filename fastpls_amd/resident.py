@@ -581,7 +581,7 @@ def plssvd_cv_classification(x, labels, folds, classes, prefixes, precision=64, 
 SERVER_STATS = ("completed", "failed", "predict_p50_ms", "predict_p99_ms", "fit_p50_ms",
                 "fit_p99_ms", "completed_per_s", "peak_bytes", "budget_bytes", "peak_concurrent",
                 "predict_batches", "mean_batch_rows", "max_batch_requests", "span_s",
-                "bytes_in_use", "resident_models")
+                "bytes_in_use", "resident_models", "fit_service_ms", "predict_service_ms")
 
 
 class Server:

@@ -273,7 +273,8 @@ int fastpls_hip_server_export(void* server, long long model, int field, void* ou
  * fit p99 ms, completed per second, peak bytes in use, budget bytes, peak
  * concurrent batches, predict batches, mean rows per predict batch, most
  * requests in one batch, seconds from first submission to last completion,
- * bytes in use, resident models. */
+ * bytes in use, resident models, mean fit service ms (start to completion),
+ * mean predict service ms (of the batch it ran in). */
 int fastpls_hip_server_stats(void* server, double* out, int count, char* error,
                              size_t error_capacity);
 void fastpls_hip_server_destroy(void* server);
