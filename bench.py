@@ -231,7 +231,7 @@ def c3_batched(fp, seed=2026, details_out=None):
     }
 
 
-def c5_serving(fp, requests=10000, rate=500.0, seed=3, workers=32):
+def c5_serving(fp, requests=10000, rate=500.0, seed=3, workers=32, max_fits=0):
     """Serving stream: `requests` requests with Poisson arrivals at `rate`/s
     (seed 3) against the request server (HBM budget 200 GB, FP32 storage,
     bf16x6 matrix-core products with FP32 accumulation). 80% predict 1..4096
@@ -244,7 +244,7 @@ def c5_serving(fp, requests=10000, rate=500.0, seed=3, workers=32):
     (device inputs); latency runs from submission to completion."""
     rng = np.random.default_rng(seed)
     keep = []
-    srv = fp.Server(200e9, workers=workers, max_batch_rows=8192)
+    srv = fp.Server(200e9, workers=workers, max_batch_rows=8192, max_concurrent_fits=max_fits)
     models = []
     for i in range(16):
         p = int(rng.integers(1000, 20001)); q = int(rng.integers(1, 21)); n = 4096
@@ -324,6 +324,7 @@ def c5_serving(fp, requests=10000, rate=500.0, seed=3, workers=32):
         "fit_service_ms_mean": round(st["fit_service_ms"], 3),
         "predict_service_ms_mean": round(st["predict_service_ms"], 3),
         "workers": workers,
+        "max_concurrent_fits": max_fits if max_fits > 0 else max(1, workers // 2),
     }
 
 

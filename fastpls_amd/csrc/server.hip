@@ -3,8 +3,9 @@
 //
 // Worker threads take requests from one FIFO queue as device memory and
 // workers free up: a fit is admitted when its device-byte estimate fits the
-// budget left by running fits and resident models (a fit that does not fit
-// may be passed by later requests that do); a predict runs as soon as its
+// budget left by running fits and resident models and fewer than the fit
+// cap are running, so predicts always find workers (a fit that has to wait
+// may be passed by later requests that can run); a predict runs as soon as its
 // model is idle, together with every predict queued behind it for the same
 // model and prefix (up to a row cap), as one product over the concatenated
 // rows. Each model owns a stream, so fits and predicts on different models
@@ -89,8 +90,9 @@ size_t fit_estimate(int method, int precision, bool device_inputs, long long n, 
 
 class Server {
  public:
-  Server(double budget, int workers, int max_rows)
-      : budget_(size_t(budget)), max_rows_(max_rows) {
+  Server(double budget, int workers, int max_rows, int max_fits)
+      : budget_(size_t(budget)), max_rows_(max_rows),
+        max_fits_(max_fits > 0 ? max_fits : std::max(1, workers / 2)) {
     if (!(budget > 0.0)) throw std::invalid_argument("server HBM budget must be positive");
     if (workers < 1 || workers > 64) throw std::invalid_argument("server workers must lie in 1..64");
     if (max_rows < 1) throw std::invalid_argument("server batch row cap must be positive");
@@ -249,8 +251,9 @@ class Server {
     for (size_t i = 0; i < queue_.size() && i < kScan; ++i) {
       Request* r = queue_[i];
       if (r->kind == kFitRequest) {
-        if (in_use_ + r->bytes > budget_) continue;
+        if (running_fits_ >= max_fits_ || in_use_ + r->bytes > budget_) continue;
         in_use_ += r->bytes;
+        ++running_fits_;
         work.push_back(r);
         queue_.erase(queue_.begin() + long(i));
         return true;
@@ -349,6 +352,7 @@ class Server {
     }
     std::lock_guard<std::mutex> lock(mu_);
     in_use_ -= r->bytes;
+    --running_fits_;
     if (h) {
       Resident m;
       m.handle = std::move(h);
@@ -430,6 +434,8 @@ class Server {
   std::unordered_map<long long, Resident> models_;
   size_t budget_ = 0, in_use_ = 0, peak_in_use_ = 0;
   int max_rows_ = 0;
+  int max_fits_ = 1;
+  int running_fits_ = 0;
   int device_ = 0;
   long long next_id_ = 1;
   bool stop_ = false;
@@ -450,10 +456,11 @@ using fpls::Server;
 extern "C" {
 
 void* fastpls_hip_server_create(double hbm_budget_bytes, int workers, int max_batch_rows,
-                                char* error, size_t error_capacity) {
+                                int max_concurrent_fits, char* error, size_t error_capacity) {
   Server* s = nullptr;
-  fpls::guarded(error, error_capacity,
-                [&] { s = new Server(hbm_budget_bytes, workers, max_batch_rows); });
+  fpls::guarded(error, error_capacity, [&] {
+    s = new Server(hbm_budget_bytes, workers, max_batch_rows, max_concurrent_fits);
+  });
   return s;
 }
 

@@ -34,7 +34,7 @@ _CV_CLASSIFICATION = (C.c_int, [_cvp, _ip, _ip, C.c_int, C.c_int, C.c_int, C.c_i
                                 _vp, _ip, _dp, _cp, _sz])
 
 SIGNATURES = {
-    "fastpls_hip_server_create": (_vp, [C.c_double, C.c_int, C.c_int, _cp, _sz]),
+    "fastpls_hip_server_create": (_vp, [C.c_double, C.c_int, C.c_int, C.c_int, _cp, _sz]),
     "fastpls_hip_server_submit_fit": (_ll, [_vp, C.c_int, _cvp, _cvp, _ip, C.c_int, C.c_int,
                                             C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                             C.c_int, _ull, _cp, _sz]),
@@ -591,10 +591,12 @@ class Server:
 
     QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, 3
 
-    def __init__(self, budget_bytes: float, workers: int = 4, max_batch_rows: int = 8192):
+    def __init__(self, budget_bytes: float, workers: int = 4, max_batch_rows: int = 8192,
+                 max_concurrent_fits: int = 0):
         e = _err()
         h = load_library().fastpls_hip_server_create(float(budget_bytes), int(workers),
-                                                     int(max_batch_rows), e, 1024)
+                                                     int(max_batch_rows),
+                                                     int(max_concurrent_fits), e, 1024)
         if not h:
             _raise(e)
         self._h = h

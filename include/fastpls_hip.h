@@ -234,18 +234,22 @@ int fastpls_hip_plssvd_cv_classification(const void* predictors, const int* labe
 /* Request server (no counterpart in the reference, which has no serving loop;
  * each request runs the resident handle API of src/cuda_resident_api.cuh):
  * continuous batching of fit and predict requests on one GPU under an HBM
- * budget (bytes), with `workers` worker threads and at most max_batch_rows
- * rows per batched predict. Fits are admitted when their device-byte
- * estimate fits the budget left by running fits and resident models; a fit
+ * budget (bytes), with `workers` worker threads, at most max_batch_rows
+ * rows per batched predict and at most max_concurrent_fits fits running at
+ * once (<= 0: half the workers), so predicts always find workers. Fits are
+ * admitted when their device-byte estimate fits the budget left by running
+ * fits and resident models; a fit
  * whose estimate exceeds the whole budget is rejected at submission. Fitted
  * models stay resident (compacted) until released; the model id is the fit
  * request id. Predicts queued for the same model and prefix run as one
  * product over their concatenated rows; results are bitwise the same alone
  * and in a mix. Inputs and outputs must stay valid until the request is
  * polled as finished; a finished request can be polled once. Destroying the
- * server waits for running work and drops queued requests. */
+ * server waits for running work and drops queued requests. Serving
+ * processes should allow at least as many hardware queues as they run
+ * concurrent batches (environment GPU_MAX_HW_QUEUES, read when HIP starts). */
 void* fastpls_hip_server_create(double hbm_budget_bytes, int workers, int max_batch_rows,
-                                char* error, size_t error_capacity);
+                                int max_concurrent_fits, char* error, size_t error_capacity);
 /* Returns the request id (> 0), 0 on error. Arguments as fastpls_hip_fit_device
  * (device_inputs != 0) or fastpls_hip_simpls_create / _plssvd_create. */
 long long fastpls_hip_server_submit_fit(void* server, int method, const void* x, const void* y,
