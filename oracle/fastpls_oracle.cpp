@@ -351,6 +351,342 @@ Triplets randomized_svd(const Mat& A, int retained, int oversample, int power,
   return finalize_sample(A, sample, target, left_only);
 }
 
+// ------------------------------------------------------- operator rSVD
+// operator_rsvd.hpp:64-168 (finalize_operator_sample): eigen route when the
+// projected block is at least 4 rows and not taller than wide, with the
+// squared rank tolerance and no unit floor, accepted only when it yields
+// all `retained` directions; otherwise the dense SVD of the block.
+Triplets finalize_operator(const Mat& basis, const Mat& projected, size_t retained,
+                           bool left_only) {
+  Mat small_left;
+  std::vector<double> sv;
+  Mat small_vt;
+  bool completed = false;
+  if (projected.r >= 4 && projected.r <= projected.c) {
+    Mat gram = mul(projected, false, projected, true);
+    std::vector<double> ev;
+    if (symmetric_eigen(gram, ev)) {
+      const double largest = ev.empty() ? 0.0 : ev.back();
+      const double rel = std::numeric_limits<double>::epsilon() *
+        double(std::max(projected.r, projected.c));
+      const double tol = largest > 0.0 ? rel * rel * largest : 0.0;
+      size_t usable = 0;
+      for (size_t idx = ev.size(); idx > 0; --idx) {
+        if (ev[idx - 1] <= tol || usable == retained) break;
+        ++usable;
+      }
+      if (usable == retained) {
+        small_left.resize(projected.r, usable);
+        sv.resize(usable);
+        for (size_t c = 0; c < usable; ++c) {
+          const size_t src = ev.size() - 1 - c;
+          sv[c] = std::sqrt(std::max(ev[src], 0.0));
+          for (size_t i = 0; i < projected.r; ++i) small_left(i, c) = gram(i, src);
+        }
+        if (!left_only) {
+          small_vt = mul(small_left, true, projected, false);
+          for (size_t i = 0; i < usable; ++i) {
+            const double inv = 1.0 / sv[i];
+            for (size_t j = 0; j < small_vt.c; ++j) small_vt(i, j) *= inv;
+          }
+        }
+        completed = true;
+      }
+    }
+  }
+  if (!completed && !svd_economy(projected, left_only, small_left, sv, small_vt))
+    throw std::runtime_error("oracle operator rSVD reduced decomposition failed");
+  retained = std::min({retained, small_left.c, sv.size()});
+  if (retained == 0) throw std::runtime_error("oracle operator rSVD returned no usable directions");
+  Mat kept(small_left.r, retained);
+  for (size_t j = 0; j < retained; ++j) for (size_t i = 0; i < small_left.r; ++i) kept(i, j) = small_left(i, j);
+  Triplets out;
+  out.U = mul(basis, false, kept, false);
+  out.s.assign(sv.begin(), sv.begin() + retained);
+  if (!left_only) {
+    out.Vt.resize(retained, projected.c);
+    for (size_t j = 0; j < projected.c; ++j) for (size_t i = 0; i < retained; ++i) out.Vt(i, j) = small_vt(i, j);
+  }
+  return out;
+}
+
+// operator_rsvd.hpp:204-272 with the explicit operator A (operators.hpp:16-56).
+Triplets operator_svd(const Mat& A, int retained, int oversample, int power, unsigned int seed,
+                      bool left_only) {
+  const size_t maximum = std::min(A.r, A.c);
+  const size_t target = std::min(maximum, size_t(std::max(retained, 1)));
+  if (target == 0) return {};
+  const size_t width = std::min(maximum, target + size_t(std::max(oversample, 0)));
+  Mat omega(A.c, width);
+  gaussian(seed, omega.d.size(), omega.data());
+  Mat sample = mul(A, false, omega, false);
+  Mat fb;
+  for (int it = 0; it < std::max(power, 0); ++it) {
+    if (!qr_economy(sample, fb)) throw std::runtime_error("oracle operator rSVD forward QR failed");
+    Mat rev = mul(A, true, fb, false);
+    Mat rb;
+    if (!qr_economy(rev, rb)) throw std::runtime_error("oracle operator rSVD reverse QR failed");
+    sample = mul(A, false, rb, false);
+  }
+  if (!qr_economy(sample, fb)) throw std::runtime_error("oracle operator rSVD sketch QR failed");
+  Mat rev = mul(A, true, fb, false);           // n x width
+  Mat projected(rev.c, rev.r);                 // width x n
+  for (size_t j = 0; j < rev.c; ++j) for (size_t i = 0; i < rev.r; ++i) projected(j, i) = rev(i, j);
+  return finalize_operator(fb, projected, target, left_only);
+}
+
+// ----------------------------------------------------- audited rSVD
+struct Audit {
+  bool certified = false, fallback = false;
+  int attempts = 0, oversample = 0, power = 0;
+  unsigned int seed = 0;
+  double subspace = 0.0, singular = 0.0, residual = 0.0, omitted = 0.0;
+};
+
+bool finite_triplets(const Triplets& t) {
+  if (t.U.c == 0 || t.s.empty() || t.Vt.r == 0) return false;
+  for (double v : t.U.d) if (!std::isfinite(v)) return false;
+  for (double v : t.s) if (!std::isfinite(v)) return false;
+  for (double v : t.Vt.d) if (!std::isfinite(v)) return false;
+  return true;
+}
+
+// audited_rsvd.hpp:97-148
+double triplet_residual(const Mat& A, const Triplets& t) {
+  const size_t rank = std::min({t.U.c, t.Vt.r, t.s.size()});
+  if (rank == 0 || t.Vt.c != A.c || t.U.r != A.r) return std::numeric_limits<double>::infinity();
+  Mat right(A.c, rank);
+  for (size_t c = 0; c < rank; ++c) for (size_t i = 0; i < A.c; ++i) right(i, c) = t.Vt(c, i);
+  Mat forward = mul(A, false, right, false);
+  Mat reverse = mul(A, true, t.U, false);
+  const long double scale = std::max(std::fabs((long double)t.s.front()),
+                                     (long double)std::numeric_limits<double>::epsilon());
+  long double worst = 0.0L;
+  for (size_t c = 0; c < rank; ++c) {
+    long double f2 = 0.0L, r2 = 0.0L;
+    const long double sv = t.s[c];
+    for (size_t i = 0; i < A.r; ++i) {
+      const long double d = (long double)forward(i, c) - sv * (long double)t.U(i, c);
+      f2 += d * d;
+    }
+    for (size_t i = 0; i < A.c; ++i) {
+      const long double d = (long double)reverse(i, c) - sv * (long double)right(i, c);
+      r2 += d * d;
+    }
+    worst = std::max(worst, std::max(std::sqrt(f2), std::sqrt(r2)) / scale);
+  }
+  return double(worst);
+}
+
+double col_norm(const Mat& v) {
+  long double s = 0.0L;
+  for (size_t i = 0; i < v.r; ++i) s += (long double)v(i, 0) * (long double)v(i, 0);
+  return double(std::sqrt(s));
+}
+
+bool normalize_col(Mat& v) {
+  const double nrm = col_norm(v);
+  if (!std::isfinite(nrm) || nrm <= std::numeric_limits<double>::epsilon()) return false;
+  for (size_t i = 0; i < v.r; ++i) v(i, 0) /= nrm;
+  return true;
+}
+
+void project_away(Mat& v, const Mat& basis, size_t cols) {
+  cols = std::min(cols, basis.c);
+  for (size_t c = 0; c < cols; ++c) {
+    long double coef = 0.0L;
+    for (size_t i = 0; i < v.r; ++i) coef += (long double)basis(i, c) * (long double)v(i, 0);
+    const double cv = double(coef);
+    for (size_t i = 0; i < v.r; ++i) v(i, 0) -= cv * basis(i, c);
+  }
+}
+
+// audited_rsvd.hpp:150-183
+double omitted_ratio(const Mat& A, const Triplets& t, size_t retained, unsigned int seed) {
+  retained = std::min(retained, t.U.c);
+  if (retained == 0 || t.s.size() < retained) return std::numeric_limits<double>::infinity();
+  const double boundary = std::max(std::fabs(t.s[retained - 1]),
+                                   std::numeric_limits<double>::epsilon());
+  std::mt19937 generator(seed);
+  std::normal_distribution<double> normal(0.0, 1.0);
+  double largest = 0.0;
+  for (int probe = 0; probe < 3; ++probe) {
+    Mat right(A.c, 1);
+    for (size_t i = 0; i < right.r; ++i) right(i, 0) = normal(generator);
+    if (!normalize_col(right)) continue;
+    Mat left;
+    for (int it = 0; it < 2; ++it) {
+      left = mul(A, false, right, false);
+      project_away(left, t.U, retained);
+      if (!normalize_col(left)) break;
+      right = mul(A, true, left, false);
+      if (!normalize_col(right)) break;
+    }
+    left = mul(A, false, right, false);
+    project_away(left, t.U, retained);
+    largest = std::max(largest, col_norm(left));
+  }
+  return largest / boundary;
+}
+
+// audited_rsvd.hpp:185-205
+bool audit_triplets(const Mat& A, const Triplets& t, size_t retained, unsigned int seed,
+                    double& residual, double& omitted) {
+  if (!finite_triplets(t) || t.U.c < retained || t.s.size() < retained || t.Vt.r < retained)
+    return false;
+  residual = triplet_residual(A, t);
+  omitted = omitted_ratio(A, t, retained, seed + 32452843U);
+  if (t.s.size() > retained) {
+    const double boundary = std::max(std::fabs(t.s[retained - 1]),
+                                     std::numeric_limits<double>::epsilon());
+    omitted = std::max(omitted, std::fabs(t.s[retained] / boundary));
+  }
+  return std::isfinite(residual) && residual <= 1e-2 && std::isfinite(omitted) && omitted <= 1.01;
+}
+
+// audited_rsvd.hpp:207-252
+bool consensus(const Mat& A, const Triplets& lhs, const Triplets& rhs, double& subspace,
+               double& singular, double& residual) {
+  if (!finite_triplets(lhs) || !finite_triplets(rhs) || lhs.U.c != rhs.U.c ||
+      lhs.s.size() != rhs.s.size())
+    return false;
+  const size_t rank = lhs.U.c;
+  if (rank == 0) return false;
+  long double overlap = 0.0L;
+  for (size_t a = 0; a < rank; ++a)
+    for (size_t b = 0; b < rank; ++b) {
+      long double inner = 0.0L;
+      for (size_t i = 0; i < lhs.U.r; ++i) inner += (long double)lhs.U(i, a) * (long double)rhs.U(i, b);
+      overlap += inner * inner;
+    }
+  overlap = std::min((long double)rank, overlap);
+  subspace = double(std::sqrt(std::max(0.0L, ((long double)rank - overlap) / (long double)rank)));
+  const long double scale = std::max(std::fabs((long double)rhs.s.front()),
+                                     (long double)std::numeric_limits<double>::epsilon());
+  long double diff = 0.0L;
+  for (size_t i = 0; i < lhs.s.size(); ++i)
+    diff = std::max(diff, std::fabs((long double)lhs.s[i] - (long double)rhs.s[i]));
+  singular = double(diff / scale);
+  residual = triplet_residual(A, rhs);
+  return subspace <= 1e-3 && singular <= 1e-5 && residual <= 1e-6;
+}
+
+// audited_rsvd.hpp:254-278
+void trim_triplets(Triplets& t, size_t retained, bool left_only) {
+  retained = std::min({retained, t.U.c, t.s.size()});
+  Mat left(t.U.r, retained);
+  for (size_t j = 0; j < retained; ++j) for (size_t i = 0; i < t.U.r; ++i) left(i, j) = t.U(i, j);
+  t.U = left;
+  t.s.resize(retained);
+  if (left_only) { t.Vt = Mat(); return; }
+  Mat right(retained, t.Vt.c);
+  for (size_t j = 0; j < t.Vt.c; ++j) for (size_t i = 0; i < retained; ++i) right(i, j) = t.Vt(i, j);
+  t.Vt = right;
+}
+
+// audited_rsvd.hpp:298-414
+Triplets audited_rsvd(const Mat& A, int retained_components, int oversample, int power,
+                      unsigned int seed, bool left_only, Audit& audit) {
+  const size_t maximum = std::min(A.r, A.c);
+  const size_t retained = std::min(maximum, size_t(std::max(retained_components, 1)));
+  if (retained == 0) throw std::invalid_argument("oracle rSVD requires a nonempty matrix");
+  if (maximum < 6) {
+    Triplets out;
+    if (!svd_economy(A, false, out.U, out.s, out.Vt)) throw std::runtime_error("oracle dense SVD recovery failed");
+    trim_triplets(out, retained, false);
+    audit = Audit();
+    audit.certified = true;
+    audit.fallback = true;
+    audit.oversample = std::max(oversample, 0);
+    audit.power = std::max(power, 0);
+    audit.seed = seed;
+    return out;
+  }
+  const int audit_rank = int(std::min(maximum, retained + 1));
+  int cur_os = oversample, cur_pw = power;
+  unsigned int cur_seed = seed;
+  Triplets previous, accepted;
+  double residual = std::numeric_limits<double>::infinity();
+  double omitted = std::numeric_limits<double>::infinity();
+  double subspace = 0.0, singular = 0.0;
+  bool certified = false;
+  int attempts = 0;
+  const int oversamples[] = {std::max(oversample, 0), std::max(oversample, 32), std::max(oversample, 48)};
+  const int powers[] = {std::max(power, 0), std::max(power, 3), std::max(power, 4)};
+  const unsigned int seeds[] = {seed, seed + 104729U, seed + 209759U};
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    cur_os = oversamples[attempt];
+    cur_pw = powers[attempt];
+    cur_seed = seeds[attempt];
+    accepted = operator_svd(A, audit_rank, cur_os, cur_pw, cur_seed, false);
+    ++attempts;
+    const bool checked = audit_triplets(A, accepted, retained, cur_seed, residual, omitted);
+    bool agreed = false;
+    if (attempt > 0) agreed = consensus(A, previous, accepted, subspace, singular, residual);
+    certified = checked || agreed;
+    if (certified) break;
+    previous = accepted;
+  }
+  for (int recovery = 0; !certified && recovery < 4; ++recovery) {
+    const int width = std::min<int>(int(maximum), audit_rank + std::max(cur_os, 0));
+    const int remaining = int(maximum) - width;
+    const int enlarged = std::min<int>(int(maximum), std::max(width + std::min(32, remaining),
+                                                              width + std::min(width, remaining)));
+    cur_os = enlarged - audit_rank;
+    cur_pw = std::max(cur_pw, 6) + 2 * recovery;
+    cur_seed += 104729U * unsigned(recovery + 1);
+    accepted = operator_svd(A, audit_rank, cur_os, cur_pw, cur_seed, false);
+    ++attempts;
+    certified = audit_triplets(A, accepted, retained, cur_seed, residual, omitted);
+  }
+  if (!certified)
+    throw std::runtime_error("rSVD recovery did not meet numerical tolerances within its "
+                             "workspace/iteration budget; no unchecked result was returned");
+  trim_triplets(accepted, retained, left_only);
+  audit.certified = true;
+  audit.fallback = false;
+  audit.attempts = attempts;
+  audit.oversample = cur_os;
+  audit.power = cur_pw;
+  audit.seed = cur_seed;
+  audit.subspace = subspace;
+  audit.singular = singular;
+  audit.residual = residual;
+  audit.omitted = omitted;
+  return accepted;
+}
+
+// ------------------------------------------------------------ fastcor
+// src/r_api.cpp:4054-4080 (normalized_units): per unit (a row when by_row,
+// else a column), centre by the mean over its features and scale to unit
+// norm, in long double; a unit with a non-finite value or zero norm is NaN.
+// Returns units x features.
+Mat normalized_units(const Mat& X, bool by_row) {
+  const size_t units = by_row ? X.r : X.c, features = by_row ? X.c : X.r;
+  Mat out(units, features);
+  for (size_t u = 0; u < units; ++u) {
+    long double mean = 0.0L;
+    bool finite = true;
+    for (size_t f = 0; f < features; ++f) {
+      const double v = by_row ? X(u, f) : X(f, u);
+      if (!std::isfinite(v)) finite = false;
+      mean += v;
+    }
+    mean /= (long double)features;
+    long double ss = 0.0L;
+    for (size_t f = 0; f < features; ++f) {
+      const long double v = (by_row ? X(u, f) : X(f, u)) - mean;
+      ss += v * v;
+    }
+    const long double nrm = std::sqrt(ss);
+    for (size_t f = 0; f < features; ++f) {
+      const long double v = (by_row ? X(u, f) : X(f, u)) - mean;
+      out(u, f) = finite && nrm > 0.0L ? double(v / nrm) : std::numeric_limits<double>::quiet_NaN();
+    }
+  }
+  return out;
+}
+
 // ------------------------------------------------------------ preprocessing
 struct Prepared {
   Mat S;                      // p x q cross-covariance
@@ -1046,6 +1382,57 @@ int orc_rsvd(const double* a, int m, int n, int k, int oversample, int power,
     std::copy(t.s.begin(), t.s.end(), s);
     copy_out(t.Vt, Vt);
     *got = int(t.s.size());
+  });
+}
+
+// Audited operator rSVD of an explicit matrix (audited_rsvd.hpp:298-414).
+// U m*k, s k, Vt k*n (untouched when left_only). audit[10]: certified,
+// deterministic fallback, attempts, effective oversample, power, seed,
+// subspace error, singular value error, triplet residual, omitted ratio.
+int orc_audited_rsvd(const double* a, int m, int n, int k, int oversample, int power,
+                     unsigned int seed, int left_only, double* U, double* s, double* Vt,
+                     int* got, double* audit) {
+  return guarded([&] {
+    Audit au;
+    Triplets t = audited_rsvd(wrap(a, m, n), k, oversample, power, seed, left_only != 0, au);
+    copy_out(t.U, U);
+    std::copy(t.s.begin(), t.s.end(), s);
+    if (!left_only) copy_out(t.Vt, Vt);
+    *got = int(t.s.size());
+    const double v[10] = {double(au.certified), double(au.fallback), double(au.attempts),
+                          double(au.oversample), double(au.power), double(au.seed), au.subspace,
+                          au.singular, au.residual, au.omitted};
+    std::copy(v, v + 10, audit);
+  });
+}
+
+// fastcor_values (src/r_api.cpp:4082-4142): out = L Rᵀ of the normalised
+// units (units_a x units_b), or the per-unit dot products when diagonal.
+// b null correlates a with itself (and ignores diagonal).
+int orc_fastcor(const double* a, int ar, int ac, const double* b, int br, int bc, int by_row,
+                int diagonal, double* out) {
+  return guarded([&] {
+    const Mat L = normalized_units(wrap(a, ar, ac), by_row != 0);
+    Mat R;
+    const Mat* right = &L;
+    bool diag = diagonal != 0;
+    if (b) {
+      R = normalized_units(wrap(b, br, bc), by_row != 0);
+      if (L.c != R.c) throw std::invalid_argument("a and b must contain the same number of correlated features");
+      right = &R;
+    } else {
+      diag = false;
+    }
+    if (diag) {
+      if (L.r != right->r) throw std::invalid_argument("diag = TRUE requires matching numbers of rows or columns");
+      for (size_t u = 0; u < L.r; ++u) {
+        long double sum = 0.0L;
+        for (size_t f = 0; f < L.c; ++f) sum += (long double)L(u, f) * (*right)(u, f);
+        out[u] = double(sum);
+      }
+      return;
+    }
+    copy_out(mul(L, false, *right, true), out);
   });
 }
 

@@ -49,6 +49,26 @@ def _labels(labels):
     return np.ascontiguousarray(np.asarray(labels, dtype=np.int32))
 
 
+
+AUDIT_FIELDS = ("certified", "deterministic_fallback", "attempts", "effective_oversample",
+                "effective_power", "effective_seed", "subspace_error", "singular_value_error",
+                "triplet_residual", "omitted_direction_ratio")
+
+
+def _audited(lib_fn, check, a, k, oversample, power, seed, left_only):
+    a = _f64(a); m, n = a.shape
+    kk = min(k, m, n)
+    U = np.zeros((m, kk), order="F"); s = np.zeros(kk); Vt = np.zeros((kk, n), order="F")
+    got = C.c_int(0)
+    audit = np.zeros(10)
+    check(lib_fn(_d(a), m, n, k, oversample, power, C.c_uint(seed), int(bool(left_only)),
+                 _d(U), _d(s), _d(Vt), C.byref(got), _d(audit)))
+    g = got.value
+    out = dict(U=U[:, :g], s=s[:g], Vt=None if left_only else np.asfortranarray(Vt[:g]))
+    out["audit"] = dict(zip(AUDIT_FIELDS, audit.tolist()))
+    return out
+
+
 class Oracle:
     """FP64 restatement of the reference algorithms (column-major)."""
 
@@ -135,6 +155,27 @@ class Oracle:
                                       _d(U), _d(s), _d(Vt), C.byref(got)))
         g = got.value
         return dict(U=U[:, :g], s=s[:g], Vt=np.asfortranarray(Vt[:g]))
+
+    def audited_rsvd(self, a, k, oversample=32, power=5, seed=1, left_only=False):
+        """fastsvd: audited operator rSVD (audited_rsvd.hpp:298-414)."""
+        return _audited(self.lib.orc_audited_rsvd, self._check, a, k, oversample, power, seed,
+                        left_only)
+
+    def fastcor(self, a, b=None, by_row=False, diagonal=False):
+        """fastcor_values (src/r_api.cpp:4082-4142)."""
+        a = _f64(a)
+        bb = None if b is None else _f64(b)
+        ua = a.shape[0] if by_row else a.shape[1]
+        if bb is None:
+            ub, diagonal = ua, False
+        else:
+            ub = bb.shape[0] if by_row else bb.shape[1]
+        out = np.zeros(ua) if diagonal else np.zeros((ua, ub), order="F")
+        self._check(self.lib.orc_fastcor(_d(a), a.shape[0], a.shape[1], _d(bb),
+                                         0 if bb is None else bb.shape[0],
+                                         0 if bb is None else bb.shape[1], int(bool(by_row)),
+                                         int(bool(diagonal)), _d(out)))
+        return out
 
     def plssvd_fit(self, x, y=None, labels=None, classes=None, ncomp=2, scaling=2,
                    oversample=32, power=5, seed=1):
@@ -300,6 +341,10 @@ class Reference:
                                       _d(U), _d(s), _d(Vt), C.byref(got)))
         g = got.value
         return dict(U=U[:, :g], s=s[:g], Vt=np.asfortranarray(Vt[:g]))
+
+    def audited_rsvd(self, a, k, oversample=32, power=5, seed=1, left_only=False):
+        return _audited(self.lib.ref_audited_rsvd, self._check, a, k, oversample, power, seed,
+                        left_only)
 
     def lda(self, T, labels, classes, prefix, test_scores):
         T = _f64(T); n, a = T.shape

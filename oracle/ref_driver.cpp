@@ -10,6 +10,7 @@
 // in explicitly by this driver.
 
 #include <fastpls/core.hpp>
+#include <fastpls/core/audited_rsvd.hpp>
 
 #include "reference_backend.hpp"
 
@@ -168,6 +169,34 @@ int ref_rsvd(const double* a, int m, int n, int k, int oversample, int power,
     std::copy(t.singular_values.begin(), t.singular_values.end(), s);
     store(t.Vt, Vt);
     *got = int(t.singular_values.size());
+  });
+}
+
+// Audited operator rSVD of an explicit matrix (audited_rsvd.hpp:298-414);
+// outputs as orc_audited_rsvd.
+int ref_audited_rsvd(const double* a, int m, int n, int k, int oversample, int power,
+                     unsigned int seed, int left_only, double* U, double* s, double* Vt,
+                     int* got, double* audit) {
+  return guarded([&] {
+    ReferenceBackend<double> backend;
+    Matrix<double> A = load(a, m, n);
+    fastpls::core::ExplicitOperator<double, ReferenceBackend<double>> input(
+        ConstMatrixView<double>(A.view()), backend);
+    fastpls::core::RsvdControls controls;
+    controls.oversample = oversample; controls.power = power; controls.seed = seed;
+    controls.left_only = left_only != 0;
+    auto r = fastpls::core::audited_operator_rsvd<double>(input, k, controls, backend);
+    store(r.decomposition.U, U);
+    std::copy(r.decomposition.singular_values.begin(), r.decomposition.singular_values.end(), s);
+    if (!left_only) store(r.decomposition.Vt, Vt);
+    *got = int(r.decomposition.singular_values.size());
+    const auto& au = r.audit;
+    const double v[10] = {double(au.certified), double(au.deterministic_fallback),
+                          double(au.attempts), double(au.effective_oversample),
+                          double(au.effective_power), double(au.effective_seed),
+                          au.subspace_error, au.singular_value_error, au.triplet_residual,
+                          au.omitted_direction_ratio};
+    std::copy(v, v + 10, audit);
   });
 }
 

@@ -18,7 +18,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
-from oracle.oracle import Reference, build, class_data, latent_data  # noqa: E402
+from oracle.oracle import AUDIT_FIELDS, Reference, build, class_data, latent_data  # noqa: E402
 
 
 def digest(a: np.ndarray) -> str:
@@ -71,9 +71,38 @@ def plssvd_case(ref, name, X, Y=None, labels=None, classes=None, ncomp=3, oversa
     save(name, **arrays)
 
 
+def audited_cases(ref):
+    """fastsvd (audited operator rSVD): certified on the first attempt, after
+    three attempts, the exact route below six columns, a wide matrix and a
+    left-only call."""
+    rng = np.random.default_rng(51)
+    cases = [
+        ("audited_lowrank", latent_data(300, 120, 1, factors=8, noise=0.05, seed=3)[0], 5, 10, 2,
+         7, False),
+        ("audited_flat", rng.standard_normal((200, 90)), 5, 10, 2, 7, False),
+        ("audited_tiny", rng.standard_normal((5, 40)), 3, 32, 5, 7, False),
+        ("audited_wide", latent_data(60, 400, 1, factors=6, noise=0.1, seed=4)[0], 8, 0, 0, 9,
+         False),
+        ("audited_left", latent_data(150, 80, 1, factors=5, noise=0.1, seed=5)[0], 4, 6, 1, 3,
+         True),
+    ]
+    for name, A, k, os_, pw, seed, left in cases:
+        r = ref.audited_rsvd(A, k, oversample=os_, power=pw, seed=seed, left_only=left)
+        arrays = dict(A=A, U=r["U"], s=r["s"], k=np.int64(k), oversample=np.int64(os_),
+                      power=np.int64(pw), seed=np.int64(seed), left_only=np.int64(left),
+                      audit=np.array([r["audit"][f] for f in AUDIT_FIELDS]))
+        if not left:
+            arrays["Vt"] = r["Vt"]
+        save(name, **arrays)
+
+
 def main():
     build(reference=True)
     ref = Reference()
+    if len(sys.argv) > 1 and sys.argv[1] == "audited":
+        audited_cases(ref)
+        return
+    audited_cases(ref)
 
     # SIMPLS, dense responses, explicit width = q (direction independent of the sketch)
     X, Y = latent_data(64, 20, 3, factors=5, noise=0.1, seed=11)

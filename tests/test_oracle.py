@@ -84,6 +84,66 @@ def test_rsvd_matches_reference_core(oracle):
     assert rel(align_columns(r["Vt"].T, g["Vt"].T), g["Vt"].T) < 1e-10
 
 
+AUDIT_INTS = ("certified", "deterministic_fallback", "attempts", "effective_oversample",
+              "effective_power", "effective_seed")
+
+
+@pytest.mark.parametrize("name", ["audited_lowrank", "audited_flat", "audited_tiny",
+                                  "audited_wide", "audited_left"])
+def test_audited_rsvd_matches_reference_core(oracle, name):
+    """fastsvd: the same attempt/recovery decisions and the same triplets as
+    the compiled reference core (audited_rsvd.hpp:298-414)."""
+    from oracle.oracle import AUDIT_FIELDS
+    g = load_golden(name)
+    left = bool(g["left_only"])
+    r = oracle.audited_rsvd(g["A"], int(g["k"]), oversample=int(g["oversample"]),
+                            power=int(g["power"]), seed=int(g["seed"]), left_only=left)
+    gaudit = dict(zip(AUDIT_FIELDS, g["audit"].tolist()))
+    for f in AUDIT_INTS:
+        assert r["audit"][f] == gaudit[f], f
+    np.testing.assert_allclose(r["s"], g["s"], rtol=1e-12)
+    assert rel(r["U"] @ r["U"].T, g["U"] @ g["U"].T) < 1e-10
+    if not left:
+        assert rel(align_columns(r["Vt"].T, g["Vt"].T), g["Vt"].T) < 1e-10
+    for f in ("triplet_residual", "omitted_direction_ratio"):
+        assert r["audit"][f] == pytest.approx(gaudit[f], rel=1e-6, abs=1e-12), f
+
+
+def test_audited_rsvd_closed_forms(oracle):
+    """Exact low-rank input: the audited triplets are the top singular
+    triplets of the dense SVD; the exact route below six columns equals it."""
+    rng = np.random.default_rng(61)
+    A = rng.standard_normal((80, 6)) @ rng.standard_normal((6, 50))
+    r = oracle.audited_rsvd(A, 4, oversample=4, power=1, seed=2)
+    U, s, Vt = np.linalg.svd(A, full_matrices=False)
+    np.testing.assert_allclose(r["s"], s[:4], rtol=1e-10)
+    assert rel(np.abs(r["U"].T @ U[:, :4]), np.eye(4)) < 1e-8
+    assert r["audit"]["certified"] == 1 and r["audit"]["attempts"] == 1
+    B = rng.standard_normal((5, 30))
+    t = oracle.audited_rsvd(B, 5)
+    np.testing.assert_allclose(t["s"], np.linalg.svd(B, compute_uv=False), rtol=1e-12)
+    assert t["audit"]["deterministic_fallback"] == 1
+
+
+def test_fastcor_closed_forms(oracle):
+    """fastcor (src/r_api.cpp:4054-4142) is the Pearson correlation of units:
+    numpy corrcoef for columns and rows, its diagonal, and NaN for a constant
+    or non-finite unit."""
+    rng = np.random.default_rng(62)
+    a = rng.standard_normal((40, 6)); b = rng.standard_normal((40, 8))
+    ref = np.corrcoef(a.T, b.T)[:6, 6:]
+    np.testing.assert_allclose(oracle.fastcor(a, b), ref, atol=1e-14)
+    np.testing.assert_allclose(oracle.fastcor(a.T.copy(), b.T.copy(), by_row=True), ref, atol=1e-14)
+    np.testing.assert_allclose(oracle.fastcor(a), np.corrcoef(a.T), atol=1e-14)
+    c = rng.standard_normal((40, 6))
+    d = oracle.fastcor(a, c, diagonal=True)
+    np.testing.assert_allclose(d, [np.corrcoef(a[:, j], c[:, j])[0, 1] for j in range(6)],
+                               atol=1e-14)
+    bad = a.copy(); bad[:, 2] = 3.0; bad[5, 4] = np.nan
+    out = oracle.fastcor(bad, b)
+    assert np.all(np.isnan(out[[2, 4]])) and np.all(np.isfinite(out[[0, 1, 3, 5]]))
+
+
 def test_lda_matches_reference_core(oracle):
     g = load_golden("lda_small")
     out = oracle.lda(g["T"], g["labels"], int(g["classes"]), int(g["prefix"]), g["test"])
