@@ -3,12 +3,14 @@
 The compute path is the C ABI library ``lib/libfastpls_hip.so`` (declared in
 ``include/fastpls_hip.h``); this package is its Python binding.
 """
-from .resident import (FIELDS, PLSSVD, SIMPLS, DeviceBuffer, HipError, ResidentModel, Server,
-                       SimplsBatch, device_info, gemm, latent_data_device, load_library,
+from .resident import (AUDIT_FIELDS, FIELDS, PLSSVD, SIMPLS, DeviceBuffer, HipError,
+                       ResidentModel, Server, SimplsBatch, device_info, fastcor, fastsvd, gemm,
+                       latent_data_device, load_library,
                        plssvd_cv_classification, plssvd_cv_regression, simpls_cv_classification,
                        simpls_cv_regression)
 
-__all__ = ["FIELDS", "PLSSVD", "SIMPLS", "DeviceBuffer", "HipError", "ResidentModel", "Server",
-           "SimplsBatch", "device_info", "gemm", "latent_data_device", "load_library",
+__all__ = ["AUDIT_FIELDS", "FIELDS", "PLSSVD", "SIMPLS", "DeviceBuffer", "HipError",
+           "ResidentModel", "Server", "SimplsBatch", "device_info", "fastcor", "fastsvd", "gemm",
+           "latent_data_device", "load_library",
            "plssvd_cv_classification", "plssvd_cv_regression", "simpls_cv_classification",
            "simpls_cv_regression"]

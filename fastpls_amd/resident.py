@@ -86,6 +86,10 @@ SIGNATURES = {
     "fastpls_hip_plssvd_cv_regression": _CV_REGRESSION,
     "fastpls_hip_simpls_cv_classification": _CV_CLASSIFICATION,
     "fastpls_hip_plssvd_cv_classification": _CV_CLASSIFICATION,
+    "fastpls_hip_rsvd": (C.c_int, [_cvp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   C.c_int, C.c_uint, C.c_int, _dp, _dp, _dp, _ip, _dp, _cp, _sz]),
+    "fastpls_hip_fastcor": (C.c_int, [_cvp, C.c_int, C.c_int, _cvp, C.c_int, C.c_int, C.c_int,
+                                      C.c_int, C.c_int, C.c_int, _dp, _cp, _sz]),
 }
 
 _LIB = None
@@ -715,3 +719,55 @@ class Server:
         except Exception:
             pass
 
+
+# ---------------------------------------------------------------- fastsvd / fastcor
+AUDIT_FIELDS = ("certified", "deterministic_fallback", "attempts", "effective_oversample",
+                "effective_power", "effective_seed", "subspace_error", "singular_value_error",
+                "triplet_residual", "omitted_direction_ratio")
+
+
+def fastsvd(x, ncomp, oversample=32, power=5, seed=1, left_only=False, precision=64):
+    """fastsvd(x, ncomp, oversample, power, seed) (R/main.R:8181-8204): the
+    audited operator rSVD of x on the device (audited_rsvd.hpp:298-414).
+    Returns dict(U (m x k), s (k), Vt (k x n, None when left_only), audit)."""
+    lib = load_library()
+    a = _fmat(x, _dtype(precision))
+    m, n = a.shape
+    k = min(int(ncomp), m, n)
+    U = np.zeros((m, max(k, 1)), order="F")
+    s = np.zeros(max(k, 1))
+    Vt = None if left_only else np.zeros((max(k, 1), n), order="F")
+    got = C.c_int(0)
+    audit = np.zeros(10)
+    e = _err()
+    if lib.fastpls_hip_rsvd(_ptr(a), precision, 0, m, n, int(ncomp), int(oversample), int(power),
+                            C.c_uint(int(seed) & 0xFFFFFFFF), int(bool(left_only)), _dptr(U),
+                            _dptr(s), _dptr(Vt), C.byref(got), _dptr(audit), e, 1024) != 0:
+        _raise(e)
+    g = got.value
+    return dict(U=U[:, :g], s=s[:g], Vt=None if left_only else np.asfortranarray(Vt[:g]),
+                audit=dict(zip(AUDIT_FIELDS, audit.tolist())))
+
+
+def fastcor(a, b=None, by_row=True, diagonal=True, precision=64):
+    """fastcor(a, b, byrow = TRUE, diag = TRUE) (R/main.R:12799,
+    src/r_api.cpp:4054-4166): Pearson correlations of the units (rows when
+    by_row, else columns) of a with those of b, or only the matching pairs
+    when diagonal; without b, the full correlation of a's units."""
+    lib = load_library()
+    dt = _dtype(precision)
+    aa = _fmat(a, dt)
+    bb = None if b is None else _fmat(b, dt)
+    ua = aa.shape[0] if by_row else aa.shape[1]
+    if bb is None:
+        ub, diagonal = ua, False
+    else:
+        ub = bb.shape[0] if by_row else bb.shape[1]
+    out = np.zeros(ua) if diagonal else np.zeros((ua, ub), order="F")
+    e = _err()
+    if lib.fastpls_hip_fastcor(_ptr(aa), aa.shape[0], aa.shape[1], _ptr(bb),
+                               0 if bb is None else bb.shape[0], 0 if bb is None else bb.shape[1],
+                               precision, 0, int(bool(by_row)), int(bool(diagonal)), _dptr(out),
+                               e, 1024) != 0:
+        _raise(e)
+    return out

@@ -283,6 +283,35 @@ int fastpls_hip_server_stats(void* server, double* out, int count, char* error,
                              size_t error_capacity);
 void fastpls_hip_server_destroy(void* server);
 
+/* fastsvd: replaces _fastPLS_fastsvd_core_cpp / _fastPLS_fastsvd_float32_core_cpp
+ * (src/r_api.cpp:2521-2600, reached from fastsvd in R/main.R:8181-8204), the
+ * audited operator rSVD of an explicit matrix
+ * (inst/include/fastpls/core/audited_rsvd.hpp:298-414). x is m x n,
+ * column-major, precision 32 or 64; with device = 1, x, U, s and Vt are
+ * device pointers, otherwise host pointers. Writes U (m x kept), s (kept),
+ * Vt (kept x n, untouched when left_only), *got = kept = min(k, m, n), and
+ * audit[10] = {certified, deterministic_fallback, attempts,
+ * effective_oversample, effective_power, effective_seed, subspace_error,
+ * singular_value_error, triplet_residual, omitted_direction_ratio}. Fails
+ * (returns 1, no result) when no attempt or recovery is certified, with the
+ * reference's message, and when a sketch would exceed 512 columns. */
+int fastpls_hip_rsvd(const void* x, int precision, int device, int m, int n, int k,
+                     int oversample, int power, unsigned int seed, int left_only, double* U,
+                     double* s, double* Vt, int* got, double* audit, char* error,
+                     size_t error_capacity);
+
+/* fastcor: replaces _fastPLS_fastcor_core_cpp (src/r_api.cpp:4144-4166) and
+ * fastcor_values / normalized_units (src/r_api.cpp:4054-4142). Units are the
+ * columns of a (a_rows x a_cols, column-major), or its rows when by_row; each
+ * is centred, scaled to unit norm and rounded to the input precision (NaN
+ * for a unit with a non-finite value or zero norm). out = L Rᵀ (units_a x
+ * units_b, column-major FP64) of the units of a and b (a with itself when b
+ * is NULL, which ignores diagonal), or the units_a matching-unit dot products
+ * when diagonal. With device = 1, a, b and out are device pointers. */
+int fastpls_hip_fastcor(const void* a, int a_rows, int a_cols, const void* b, int b_rows,
+                        int b_cols, int precision, int device, int by_row, int diagonal,
+                        double* out, char* error, size_t error_capacity);
+
 /* Synthetic latent-factor data written straight into device memory
  * (Philox4x32-10 + Box-Muller): X (n x p, precision 32/64) = T P' + noise E,
  * Y (n x q, same precision) = T Q' + noise F, with factor column f scaled by

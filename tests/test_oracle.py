@@ -109,6 +109,31 @@ def test_audited_rsvd_matches_reference_core(oracle, name):
         assert r["audit"][f] == pytest.approx(gaudit[f], rel=1e-6, abs=1e-12), f
 
 
+def test_audited_rsvd_recovery_slow_spectrum(oracle):
+    """tests/testthat/test-rsvd-native-recovery.R: σ = 1 - 0.005 k, no
+    oversampling, no power steps: certified only after recovery widenings
+    (more than three attempts, oversample above 48, power at least 6, a new
+    seed), σ to 2e-5 and triplet residuals at 2e-12."""
+    rng = np.random.default_rng(614)
+    U0 = np.linalg.qr(rng.standard_normal((160, 100)))[0]
+    V0 = np.linalg.qr(rng.standard_normal((125, 100)))[0]
+    sv = 1 - 0.005 * np.arange(100)
+    A = (U0 * sv) @ V0.T
+    for seed in (1, 7):
+        r = oracle.audited_rsvd(A, 6, oversample=0, power=0, seed=seed)
+        a = r["audit"]
+        assert a["certified"] == 1 and a["deterministic_fallback"] == 0
+        assert a["attempts"] > 3 and a["effective_oversample"] > 48 and a["effective_power"] >= 6
+        assert a["effective_seed"] != seed
+        assert a["triplet_residual"] <= 1e-2 and a["omitted_direction_ratio"] <= 1.01
+        np.testing.assert_allclose(r["s"], sv[:6], rtol=2e-5)
+        V = r["Vt"].T
+        res = [max(np.linalg.norm(A @ V[:, j] - r["s"][j] * r["U"][:, j]),
+                   np.linalg.norm(A.T @ r["U"][:, j] - r["s"][j] * V[:, j])) / r["s"][0]
+               for j in range(6)]
+        assert max(res) <= 2e-12
+
+
 def test_audited_rsvd_closed_forms(oracle):
     """Exact low-rank input: the audited triplets are the top singular
     triplets of the dense SVD; the exact route below six columns equals it."""
