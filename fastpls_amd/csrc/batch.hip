@@ -72,7 +72,7 @@ class SimplsBatch {
     if (groups_.empty() || fits_.empty()) throw std::invalid_argument("empty batch");
     if (scaling < 1 || scaling > 3) throw std::invalid_argument("scaling must be 1, 2 or 3");
     if (oversample < 0 || power < 0) throw std::invalid_argument("oversample and power must be non-negative");
-    FPLS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    stream_ = stream_take();
     try {
       plan();
       Event start, stop;
@@ -85,18 +85,14 @@ class SimplsBatch {
       FPLS_HIP(hipEventElapsedTime(&ms, start.e, stop.e));
       seconds_ = double(ms) * 1e-3;
     } catch (...) {
-      (void)hipStreamSynchronize(stream_);
-      (void)hipStreamDestroy(stream_);
+      stream_give(stream_);
       stream_ = nullptr;
       throw;
     }
   }
 
   ~SimplsBatch() {
-    if (stream_) {
-      (void)hipStreamSynchronize(stream_);
-      (void)hipStreamDestroy(stream_);
-    }
+    if (stream_) stream_give(stream_);
   }
 
   int count() const { return int(fits_.size()); }
@@ -652,11 +648,10 @@ void simpls_cv_regression(const T* predictors, const T* responses, const int* fo
   std::vector<std::vector<int>> train_rows(fold_count);
   for (int f = 0; f < fold_count; ++f) train_rows[f] = split.training(f, n);
   const long double total_ss = cv_total_ss(responses, n, q);
-  hipStream_t stream = nullptr;
-  FPLS_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  hipStream_t stream = stream_take();
   struct StreamGuard {
     hipStream_t s;
-    ~StreamGuard() { if (s) { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); } }
+    ~StreamGuard() { stream_give(s); }
   } guard_stream{stream};
   Buf xfull, yfull, idx;
   xfull.alloc(sizeof(T) * (size_t)n * p);

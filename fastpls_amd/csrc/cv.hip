@@ -17,14 +17,11 @@
 namespace fpls {
 namespace {
 
-// Stream owned by a CV call; drained and destroyed on every exit path.
+// Stream borrowed by a CV call; drained and given back on every exit path.
 struct CvStream {
   hipStream_t s = nullptr;
-  CvStream() { FPLS_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
-  ~CvStream() {
-    (void)hipStreamSynchronize(s);
-    (void)hipStreamDestroy(s);
-  }
+  CvStream() : s(stream_take()) {}
+  ~CvStream() { stream_give(s); }
 };
 
 // Device row gathers of one fold into buffers sized for the largest fold.

@@ -55,11 +55,12 @@ class Model final : public Handle, private StreamOps {
         throw std::invalid_argument("classification PLS-SVD components must be below class count");
     }
     A_ = components;
-    FPLS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    stream_ = stream_take();
     for (Buf* b : all_bufs()) b->bind(stream_);
     try {
       Event start, stop;
       load_inputs(x, y, labels, device_inputs);
+      if (method_ == kPlssvd) stage_plssvd_sketch();
       FPLS_HIP(hipEventRecord(start.e, stream_));
       prepare_crossprod();
       if (method_ == kSimpls) fit_simpls(); else fit_plssvd();
@@ -276,13 +277,12 @@ class Model final : public Handle, private StreamOps {
     return v;
   }
 
-  // Gives every device block back in stream order, then destroys the stream.
+  // Gives every device block back in stream order, then the drained stream
+  // back to the stream cache.
   void close_stream() noexcept {
     if (!stream_) return;
     for (Buf* b : all_bufs()) b->release();
-    (void)hipStreamSynchronize(stream_);
-    pool_forget_stream(stream_);
-    (void)hipStreamDestroy(stream_);
+    stream_give(stream_);
     stream_ = nullptr;
   }
 
@@ -513,7 +513,14 @@ class Model final : public Handle, private StreamOps {
     part_.reserve(sizeof(double) * (size_t)need);
   }
 
-  void randomized_svd(int k, int w, int* usable, int* flags) {
+  // The Gaussian sketch of the randomized PLS-SVD (rsvd.hpp:205-210) goes to
+  // tmp1_ before the first pass over X is queued: a copy from pageable host
+  // memory returns only once the stream has reached it, so issued later it
+  // would hold the host for the length of that pass.
+  void stage_plssvd_sketch() {
+    const int maxdim = std::min(p_, q_);
+    const int w = std::min(maxdim, A_ + oversample_);
+    if (w >= maxdim) return;
     std::vector<double>& om = omega_host_;
     om.resize((size_t)q_ * w);
     gaussian(unsigned(seed_), om.size(), om.data());
@@ -521,6 +528,9 @@ class Model final : public Handle, private StreamOps {
     // om is a member: it outlives the asynchronous copy
     FPLS_HIP(hipMemcpyAsync(tmp1_.d(), om.data(), sizeof(double) * om.size(), hipMemcpyHostToDevice,
                             stream_));
+  }
+
+  void randomized_svd(int k, int w, int* usable, int* flags) {
     Buf &sample = svd_ws_[0], &basis = svd_ws_[1], &rev = svd_ws_[2], &rbasis = svd_ws_[3],
         &P = svd_ws_[4], &Up = svd_ws_[5], &Vt = svd_ws_[6], &scratch = svd_ws_[7];
     sample.alloc(sizeof(double) * (size_t)p_ * w);
@@ -760,8 +770,7 @@ namespace {
 
 template<class T>
 void host_gemm(const T* left, const T* right, int rows, int inner, int cols, T* out) {
-  hipStream_t s = nullptr;
-  FPLS_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  hipStream_t s = stream_take();
   Buf a, b, c, c64, part, wf;
   try {
     a.alloc(sizeof(T) * (size_t)rows * inner);
@@ -793,11 +802,10 @@ void host_gemm(const T* left, const T* right, int rows, int inner, int cols, T* 
     FPLS_HIP(hipMemcpyAsync(out, c.as<T>(), sizeof(T) * (size_t)rows * cols, hipMemcpyDeviceToHost, s));
     FPLS_HIP(hipStreamSynchronize(s));
   } catch (...) {
-    (void)hipStreamSynchronize(s);
-    (void)hipStreamDestroy(s);
+    stream_give(s);
     throw;
   }
-  (void)hipStreamDestroy(s);
+  stream_give(s);
 }
 
 }  // namespace
@@ -1019,8 +1027,11 @@ int fastpls_hip_latent_data(void* x_device, void* y_device, int precision, long 
     fpls::require_device();
     if (!x_device || !y_device || n < 1 || p < 1 || q < 1 || factors < 1)
       throw std::invalid_argument("invalid synthetic data request");
-    hipStream_t s = nullptr;
-    FPLS_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipStream_t s = fpls::stream_take();
+    struct Give {
+      hipStream_t s;
+      ~Give() { fpls::stream_give(s); }
+    } give{s};
     fpls::Buf scratch, y64;
     scratch.alloc(sizeof(double) * (size_t)(n + p + q) * factors);
     y64.alloc(sizeof(double) * (size_t)n * q);
@@ -1034,11 +1045,9 @@ int fastpls_hip_latent_data(void* x_device, void* y_device, int precision, long 
                                        s);
       fpls::launch_convert<double>(y64.d(), static_cast<double*>(y_device), n * q, s);
     } else {
-      (void)hipStreamDestroy(s);
       throw std::invalid_argument("precision must be 32 or 64");
     }
     FPLS_HIP(hipStreamSynchronize(s));
-    (void)hipStreamDestroy(s);
   });
 }
 

@@ -25,6 +25,12 @@ void pool_give(void* ptr, size_t capacity, hipStream_t stream) noexcept;
 // blocks become idle and no event recorded on it is queried again.
 void pool_forget_stream(hipStream_t stream) noexcept;
 
+// A non-blocking stream of default priority from the process-wide stream
+// cache (created when none is idle), and its return: stream_give drains the
+// stream and keeps it for the next owner.
+hipStream_t stream_take();
+void stream_give(hipStream_t stream) noexcept;
+
 // Device buffer. Unbound buffers own their memory directly (hipMalloc /
 // hipFree); buffers bound to a stream take it from the block cache and give
 // it back in that stream's order, so every use of a bound buffer must be
@@ -92,7 +98,7 @@ inline int splits_for(long long tiles_per_split, long long extent, long long min
 // enough work-groups to fill the device, and no split longer than the FP32
 // accumulation bound of the matrix-core kernel.
 inline int xtw_splits(long long m, long long k, int N) {
-  int s = splits_for(xtw_tiles(k, N), m, 256);
+  int s = splits_for(xtw_tiles(k, N), m, 128);
   if (N >= kMfmaMinN) {
     const long long need = (m + kXtwSplitRows - 1) / kXtwSplitRows;
     if (need > s) s = int(std::min<long long>(need, 4096));

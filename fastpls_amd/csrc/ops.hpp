@@ -16,14 +16,11 @@ namespace fpls {
 class StreamOps {
  protected:
   // Operand workspace of at least `bytes` (null when no launch needs one).
-  // Growing it first drains the stream, so no queued kernel still reads the
-  // old buffer.
+  // frag_ is bound to the stream, so the block it replaces when it grows is
+  // handed out again only in stream order: no drain is needed.
   void* frag(size_t bytes) {
     if (bytes == 0) return nullptr;
-    if (bytes > frag_.bytes()) {
-      FPLS_HIP(hipStreamSynchronize(stream_));
-      frag_.alloc(bytes);
-    }
+    frag_.reserve(bytes);
     return frag_.as<void>();
   }
 

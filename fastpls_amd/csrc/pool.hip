@@ -1,4 +1,5 @@
-// Process-wide cache of device blocks for stream-bound buffers.
+// Process-wide caches of device blocks for stream-bound buffers and of
+// streams.
 //
 // Sizes are rounded up to one of four classes per power of two (at most 25%
 // slack). A given-back block keeps the stream that last used it and an event
@@ -138,6 +139,57 @@ void pool_give(void* ptr, size_t capacity, hipStream_t stream) noexcept {
   P.idle[capacity].push_back(Block{ptr, stream, ev});
   P.idle_bytes += capacity;
   if (P.idle_bytes > kIdleLimit) trim(P);
+}
+
+// ----------------------------------------------------------- streams
+// Creating a stream sets up a hardware queue on this runtime (milliseconds),
+// so handles borrow non-blocking streams from this cache and give them back
+// drained. Streams keep their identity across owners, so cached blocks
+// tagged with a given-back stream stay valid.
+namespace {
+
+constexpr size_t kIdleStreams = 64;
+
+struct StreamCache {
+  std::mutex mu;
+  std::vector<hipStream_t> idle;
+};
+
+StreamCache& streams() {
+  static StreamCache* c = new StreamCache();   // lives until the process ends
+  return *c;
+}
+
+}  // namespace
+
+hipStream_t stream_take() {
+  StreamCache& C = streams();
+  {
+    std::lock_guard<std::mutex> lock(C.mu);
+    if (!C.idle.empty()) {
+      hipStream_t s = C.idle.back();
+      C.idle.pop_back();
+      return s;
+    }
+  }
+  hipStream_t s = nullptr;
+  FPLS_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  return s;
+}
+
+void stream_give(hipStream_t stream) noexcept {
+  if (!stream) return;
+  (void)hipStreamSynchronize(stream);
+  StreamCache& C = streams();
+  {
+    std::lock_guard<std::mutex> lock(C.mu);
+    if (C.idle.size() < kIdleStreams) {
+      C.idle.push_back(stream);
+      return;
+    }
+  }
+  pool_forget_stream(stream);
+  (void)hipStreamDestroy(stream);
 }
 
 void pool_forget_stream(hipStream_t stream) noexcept {

@@ -28,6 +28,7 @@ struct WgScratch {
   int pa[kMaxPairs];
   int pb[kMaxPairs];
   double red[kWaves];
+  double part[kWG];   // partial sums of the Householder dots
 };
 
 // Scratch layout of one SIMPLS fit (doubles).
@@ -83,23 +84,23 @@ __global__ __launch_bounds__(kWG) void k_simpls_refresh(const SimplsFitDev* fits
     wg::copy(q, w, omega, q, Z, q);
     for (int it = 0; it < f.power; ++it) {
       wg::gemm(q, w, q, 1.0, f.G, q, false, Z, q, false, 0.0, Y2, q);
-      ok = wg::householder_qr(Y2, q, q, w, Z, q, sh.tau, sh.dots, sh.red) && ok;
+      ok = wg::householder_qr(Y2, q, q, w, Z, q, sh.tau, sh.dots, sh.red, sh.part) && ok;
     }
     if (f.power == 0) {
       wg::copy(q, w, Z, q, Y2, q);
-      ok = wg::householder_qr(Y2, q, q, w, Z, q, sh.tau, sh.dots, sh.red) && ok;
+      ok = wg::householder_qr(Y2, q, q, w, Z, q, sh.tau, sh.dots, sh.red, sh.part) && ok;
     }
     wg::gemm(p, w, q, 1.0, f.S, p, false, Z, q, false, 0.0, sample, p);
   } else {
     wg::gemm(p, w, q, 1.0, f.S, p, false, omega, q, false, 0.0, sample, p);
     for (int it = 0; it < f.power; ++it) {
-      ok = wg::householder_qr(sample, p, p, w, Qb, p, sh.tau, sh.dots, sh.red) && ok;
+      ok = wg::householder_qr(sample, p, p, w, Qb, p, sh.tau, sh.dots, sh.red, sh.part) && ok;
       wg::gemm_tn_long(q, w, p, f.S, p, Qb, p, Y2, q);
-      ok = wg::householder_qr(Y2, q, q, w, Z, q, sh.tau, sh.dots, sh.red) && ok;
+      ok = wg::householder_qr(Y2, q, q, w, Z, q, sh.tau, sh.dots, sh.red, sh.part) && ok;
       wg::gemm(p, w, q, 1.0, f.S, p, false, Z, q, false, 0.0, sample, p);
     }
   }
-  ok = wg::householder_qr(sample, p, p, w, Qb, p, sh.tau, sh.dots, sh.red) && ok;
+  ok = wg::householder_qr(sample, p, p, w, Qb, p, sh.tau, sh.dots, sh.red, sh.part) && ok;
   wg::gemm_tn_long(w, q, p, Qb, p, f.S, p, Bm, w);
   wg::gemm(w, w, q, 1.0, Bm, w, false, Bm, w, true, 0.0, Gm, w);
   ok = wg::jacobi_eigen(Gm, Ve, w, ev, sh.cs, sh.sn, sh.pa, sh.pb, sh.red, vtmp) && ok;
@@ -214,10 +215,10 @@ __global__ __launch_bounds__(NT) void k_qr(double* A, int m, int n, double* Q, i
   if ((long long)m * n <= 6144) {
     for (int e = threadIdx.x; e < m * n; e += NT) la[e] = A[e];
     __syncthreads();
-    ok = wg::householder_qr<NT>(la, m, m, n, lq, m, sh.tau, sh.dots, sh.red);
+    ok = wg::householder_qr<NT>(la, m, m, n, lq, m, sh.tau, sh.dots, sh.red, sh.part);
     for (int e = threadIdx.x; e < m * n; e += NT) Q[e] = lq[e];
   } else {
-    ok = wg::householder_qr<NT>(A, m, m, n, Q, m, sh.tau, sh.dots, sh.red);
+    ok = wg::householder_qr<NT>(A, m, m, n, Q, m, sh.tau, sh.dots, sh.red, sh.part);
   }
   if (!ok && threadIdx.x == 0) atomicOr(flags, kFlagDecomposition);
 }
@@ -232,11 +233,10 @@ __global__ __launch_bounds__(kWG) void k_chol_inv(const double* G, int w, double
                                                   int* fail) {
   __shared__ double Lm[64 * 65];
   __shared__ double Ri[64 * 65];
-  __shared__ int bad;
   __shared__ double dmax;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
   for (int e = tid; e < w * w; e += kWG) Lm[(e % w) + (e / w) * 65] = G[e];
-  if (tid == 0) bad = (*fail != 0) ? 1 : 0;
+  const bool failed_before = *fail != 0;
   __syncthreads();
   if (tid < 64) {
     double d = tid < w ? Lm[tid + tid * 65] : 0.0;
@@ -244,22 +244,21 @@ __global__ __launch_bounds__(kWG) void k_chol_inv(const double* G, int w, double
     if (tid == 0) dmax = d;
   }
   __syncthreads();
-  // right-looking Cholesky in LDS, one column per step (lower triangle)
-  for (int j = 0; j < w; ++j) {
-    if (tid == 0 && !bad) {
-      const double v = Lm[j + j * 65];
-      if (!isfinite(v) || !(v > 1e-13 * dmax)) bad = 1;
-      else Lm[j + j * 65] = sqrt(v);
+  // Right-looking Cholesky on the unscaled columns: step j subtracts
+  // a_ij a_cj / a_jj from the trailing lower triangle; the columns are scaled
+  // by 1/sqrt(a_jj) afterwards. Every thread reads the same pivot, so the
+  // failure test is uniform and one barrier per step suffices.
+  bool bad = failed_before;
+  for (int j = 0; j < w && !bad; ++j) {
+    const double v = Lm[j + j * 65];
+    if (!isfinite(v) || !(v > 1e-13 * dmax)) {
+      bad = true;
+      break;
     }
-    __syncthreads();
-    if (bad) break;
-    const double djj = Lm[j + j * 65];
-    for (int i = j + 1 + tid; i < w; i += kWG) Lm[i + j * 65] /= djj;
-    __syncthreads();
-    const int r = w - j - 1;
-    for (int e = tid; e < r * r; e += kWG) {
-      const int i = j + 1 + e % r, c = j + 1 + e / r;
-      if (c <= i) Lm[i + c * 65] -= Lm[i + j * 65] * Lm[c + j * 65];
+    const double inv = 1.0 / v;
+    for (int i = j + 1 + ty; i < w; i += 16) {
+      const double lij = Lm[i + j * 65] * inv;
+      for (int c = j + 1 + tx; c <= i; c += 16) Lm[i + c * 65] -= lij * Lm[c + j * 65];
     }
     __syncthreads();
   }
@@ -268,20 +267,32 @@ __global__ __launch_bounds__(kWG) void k_chol_inv(const double* G, int w, double
     for (int e = tid; e < w * w; e += kWG) Rinv[e] = (e % w) == (e / w) ? 1.0 : 0.0;
     return;
   }
-  // Rinv = L^{-T}: column c solves Lᵀ x = e_c (upper triangular, x_i = 0 for i > c)
-  for (int c = tid; c < w; c += kWG) {
-    for (int i = w - 1; i >= 0; --i) {
-      double v = 0.0;
-      if (i <= c) {
-        v = i == c ? 1.0 : 0.0;
-        for (int l = i + 1; l <= c; ++l) v -= Lm[l + i * 65] * Ri[l + c * 65];
-        v /= Lm[i + i * 65];
-      }
-      Ri[i + c * 65] = v;
-    }
+  for (int c = ty; c < w; c += 16) {
+    const double d = sqrt(Lm[c + c * 65]);
+    for (int i = c + 1 + tx; i < w; i += 16) Lm[i + c * 65] /= d;
   }
   __syncthreads();
-  for (int e = tid; e < w * w; e += kWG) Rinv[e] = Ri[(e % w) + (e / w) * 65];
+  for (int c = tid; c < w; c += kWG) Lm[c + c * 65] = sqrt(Lm[c + c * 65]);
+  // Rinv = L^{-T} (upper triangular) by a sweep over the rows from the
+  // bottom: at step k row k of X holds Lᵀ-eliminated values; rows i < k take
+  // off l_ki x_kc / l_kk. Rows are divided by their pivots at the end.
+  for (int e = tid; e < w * w; e += kWG) {
+    const int i = e % w, c = e / w;
+    Ri[i + c * 65] = i == c ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int k = w - 1; k > 0; --k) {
+    const double inv = 1.0 / Lm[k + k * 65];
+    for (int c = k + tx; c < w; c += 16) {
+      const double xk = Ri[k + c * 65] * inv;
+      for (int i = ty; i < k; i += 16) Ri[i + c * 65] -= Lm[k + i * 65] * xk;
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < w * w; e += kWG) {
+    const int i = e % w, c = e / w;
+    Rinv[e] = i <= c ? Ri[i + c * 65] / Lm[i + i * 65] : 0.0;
+  }
 }
 
 // Householder recomputation of Q when CholQR2 failed; a no-op otherwise.
@@ -289,7 +300,7 @@ __global__ __launch_bounds__(kWG) void k_qr_fallback(double* A, int m, int n, do
                                                      const int* fail, int* flags) {
   if (*fail == 0) return;
   __shared__ WgScratch sh;
-  const bool ok = wg::householder_qr(A, m, m, n, Q, m, sh.tau, sh.dots, sh.red);
+  const bool ok = wg::householder_qr(A, m, m, n, Q, m, sh.tau, sh.dots, sh.red, sh.part);
   if (!ok && threadIdx.x == 0) atomicOr(flags, kFlagDecomposition);
 }
 

@@ -384,7 +384,7 @@ template<class T>
 class AuditedSvd : private StreamOps {
  public:
   AuditedSvd(const T* x, bool device, int m, int n) : m_(m), n_(n) {
-    FPLS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    stream_ = stream_take();
     for (Buf* b : bufs()) b->bind(stream_);
     try {
       if (device) {
@@ -519,9 +519,7 @@ class AuditedSvd : private StreamOps {
   void close() noexcept {
     if (!stream_) return;
     for (Buf* b : bufs()) b->release();
-    (void)hipStreamSynchronize(stream_);
-    pool_forget_stream(stream_);
-    (void)hipStreamDestroy(stream_);
+    stream_give(stream_);
     stream_ = nullptr;
   }
 
@@ -757,7 +755,7 @@ template<class T>
 class Correlation : private StreamOps {
  public:
   Correlation() {
-    FPLS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    stream_ = stream_take();
     for (Buf* b : bufs()) b->bind(stream_);
   }
   ~Correlation() { close(); }
@@ -814,9 +812,7 @@ class Correlation : private StreamOps {
   void close() noexcept {
     if (!stream_) return;
     for (Buf* b : bufs()) b->release();
-    (void)hipStreamSynchronize(stream_);
-    pool_forget_stream(stream_);
-    (void)hipStreamDestroy(stream_);
+    stream_give(stream_);
     stream_ = nullptr;
   }
 

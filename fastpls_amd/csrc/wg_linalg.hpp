@@ -63,12 +63,51 @@ __device__ inline double norm(const double* v, long n, double* red) {
   return sqrt(fmax(wg_sum_n<NT>(s, red), 0.0));
 }
 
+// d_c = Σ_{i=j..m-1} v_i B[i, c] for the columns c in [c0, n), with the
+// Householder vector v of column j of V (v_j = 1, v_i = V[i, j] below).
+// G threads share a column (G a power of two, G · (n - c0) <= NT): each sums
+// a strided subset of the rows and the G partials are added in LDS, so no
+// step waits on a chain of cross-lane shuffles. `part` is NT doubles of LDS.
+template<int NT = kWG>
+__device__ inline void reflector_dots(const double* V, long ldv, const double* B, long ldb,
+                                      int m, int j, int c0, int n, double* dots,
+                                      double* part) {
+  const int tid = threadIdx.x;
+  for (int base = c0; base < n; base += NT) {
+    const int cols = n - base < NT ? n - base : NT;
+    int lg = 0;
+    while ((2 << lg) * cols <= NT && lg < 6) ++lg;
+    const int G = 1 << lg;
+    const int c = base + (tid >> lg), g = tid & (G - 1);
+    double acc = 0.0;
+    if (tid < G * cols) {
+      const double* b = B + (long)c * ldb;
+      const double* v = V + (long)j * ldv;
+      int i = j + g;
+      if (g == 0) {
+        acc = b[j];
+        i += G;
+      }
+      for (; i < m; i += G) acc = fma(v[i], b[i], acc);
+    }
+    part[tid] = acc;
+    __syncthreads();
+    if (tid < cols) {
+      double s = 0.0;
+      for (int t = 0; t < G; ++t) s += part[(tid << lg) + t];
+      dots[base + tid] = s;
+    }
+    __syncthreads();
+  }
+}
+
 // Householder economy QR of A (m x n, m >= n), overwritten by the reflectors;
-// the explicit Q (m x n) goes to Q. tau and dots are LDS arrays of n doubles.
-// Returns false on a non-finite column norm.
+// the explicit Q (m x n) goes to Q. tau and dots are LDS arrays of n doubles,
+// part NT doubles of LDS. Returns false on a non-finite column norm.
 template<int NT = kWG>
 __device__ inline bool householder_qr(double* A, long lda, int m, int n, double* Q,
-                                      long ldq, double* tau, double* dots, double* red) {
+                                      long ldq, double* tau, double* dots, double* red,
+                                      double* part) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   bool finite = true;
   for (int j = 0; j < n; ++j) {
@@ -93,16 +132,12 @@ __device__ inline bool householder_qr(double* A, long lda, int m, int n, double*
     if (tid == 0) tau[j] = tj;
     __syncthreads();
     if (tj != 0.0 && j + 1 < n) {
-      for (int c = j + 1 + wave; c < n; c += (NT / 64)) {
-        double acc = 0.0;
-        for (int i = j + 1 + lane; i < m; i += 64) acc = fma(A[i + j * lda], A[i + c * lda], acc);
-        acc = wave_sum(acc);
-        if (lane == 0) dots[c] = (acc + A[j + c * lda]) * tj;
-      }
-      __syncthreads();
+      // dots of the trailing columns with the reflector (its unit head is
+      // implied: the stored diagonal is beta)
+      reflector_dots<NT>(A, lda, A, lda, m, j, j + 1, n, dots, part);
       // one wave per column, lanes over the rows (no index division)
       for (int c = j + 1 + wave; c < n; c += (NT / 64)) {
-        const double d = dots[c];
+        const double d = dots[c] * tj;
         for (int i = j + lane; i < m; i += 64) {
           const double v = i == j ? 1.0 : A[i + j * lda];
           A[i + c * lda] -= d * v;
@@ -117,15 +152,9 @@ __device__ inline bool householder_qr(double* A, long lda, int m, int n, double*
   for (int j = n - 1; j >= 0; --j) {
     const double tj = tau[j];
     if (tj == 0.0) continue;
+    reflector_dots<NT>(A, lda, Q, ldq, m, j, j, n, dots, part);
     for (int c = j + wave; c < n; c += (NT / 64)) {
-      double acc = 0.0;
-      for (int i = j + 1 + lane; i < m; i += 64) acc = fma(A[i + j * lda], Q[i + c * ldq], acc);
-      acc = wave_sum(acc);
-      if (lane == 0) dots[c] = (acc + Q[j + c * ldq]) * tj;
-    }
-    __syncthreads();
-    for (int c = j + wave; c < n; c += (NT / 64)) {
-      const double d = dots[c];
+      const double d = dots[c] * tj;
       for (int i = j + lane; i < m; i += 64) {
         const double v = i == j ? 1.0 : A[i + j * lda];
         Q[i + c * ldq] -= d * v;
