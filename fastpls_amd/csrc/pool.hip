@@ -25,22 +25,26 @@ struct Block {
   hipEvent_t ready;     // null once the block is known to be idle
 };
 
-// Whether the work that last used a block has finished. hipEventQuery
-// reports "not ready" through the thread's last-error slot, which the launch
-// checks read, so the slot is cleared.
-bool idle_now(const Block& b) {
-  if (b.ready == nullptr) return true;
-  const hipError_t st = hipEventQuery(b.ready);
-  (void)hipGetLastError();
-  return st == hipSuccess;
-}
-
 struct Pool {
   std::mutex mu;
   std::map<size_t, std::vector<Block>> idle;   // by size class
   std::vector<hipEvent_t> spare_events;
   size_t idle_bytes = 0;
 };
+
+// Whether the work that last used a block has finished; a block seen idle
+// once drops its event, so it is never queried again. hipEventQuery reports
+// "not ready" through the thread's last-error slot, which the launch checks
+// read, so the slot is cleared. Called with the lock held.
+bool idle_now(Pool& P, Block& b) {
+  if (b.ready == nullptr) return true;
+  const hipError_t st = hipEventQuery(b.ready);
+  (void)hipGetLastError();
+  if (st != hipSuccess) return false;
+  P.spare_events.push_back(b.ready);
+  b.ready = nullptr;
+  return true;
+}
 
 Pool& pool() {
   static Pool* p = new Pool();   // lives until the process ends
@@ -61,9 +65,8 @@ void trim(Pool& P) {
   for (auto it = P.idle.rbegin(); it != P.idle.rend() && P.idle_bytes > kIdleLimit; ++it) {
     auto& v = it->second;
     for (size_t i = 0; i < v.size() && P.idle_bytes > kIdleLimit;) {
-      if (idle_now(v[i])) {
+      if (idle_now(P, v[i])) {
         (void)hipFree(v[i].ptr);
-        if (v[i].ready) P.spare_events.push_back(v[i].ready);
         P.idle_bytes -= it->first;
         v.erase(v.begin() + long(i));
       } else {
@@ -85,7 +88,7 @@ void* pool_take(size_t bytes, hipStream_t stream, size_t* capacity) {
     if (it != P.idle.end()) {
       auto& v = it->second;
       for (size_t i = 0; i < v.size(); ++i) {
-        if ((v[i].stream != nullptr && v[i].stream == stream) || idle_now(v[i])) {
+        if ((v[i].stream != nullptr && v[i].stream == stream) || idle_now(P, v[i])) {
           void* ptr = v[i].ptr;
           if (v[i].ready) P.spare_events.push_back(v[i].ready);
           v.erase(v.begin() + long(i));
@@ -180,6 +183,7 @@ hipStream_t stream_take() {
 void stream_give(hipStream_t stream) noexcept {
   if (!stream) return;
   (void)hipStreamSynchronize(stream);
+  pool_forget_stream(stream);   // its blocks are idle now: no event to query
   StreamCache& C = streams();
   {
     std::lock_guard<std::mutex> lock(C.mu);
@@ -188,7 +192,6 @@ void stream_give(hipStream_t stream) noexcept {
       return;
     }
   }
-  pool_forget_stream(stream);
   (void)hipStreamDestroy(stream);
 }
 
