@@ -273,7 +273,7 @@ int fastpls_hip_server_release(void* server, long long model, char* error,
 /* Fields as fastpls_hip_export. */
 int fastpls_hip_server_export(void* server, long long model, int field, void* out,
                               size_t elements, char* error, size_t error_capacity);
-/* out[0..15]: completed, failed, predict p50 ms, predict p99 ms, fit p50 ms,
+/* out[0..17]: completed, failed, predict p50 ms, predict p99 ms, fit p50 ms,
  * fit p99 ms, completed per second, peak bytes in use, budget bytes, peak
  * concurrent batches, predict batches, mean rows per predict batch, most
  * requests in one batch, seconds from first submission to last completion,
