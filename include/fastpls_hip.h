@@ -15,7 +15,9 @@
  *   - every call is synchronous; calls on one handle must not overlap.
  *
  * Precision 32 stores X as FP32 on the device and runs every product over X
- * with FP64 accumulation; the cross-covariance, the randomized SVD, SIMPLS
+ * on the matrix cores in bf16x6 split precision (FP32-accurate products,
+ * FP32 sums over one 32- to 64-element step folded into FP64 accumulators);
+ * the cross-covariance, the randomized SVD, SIMPLS
  * deflation/orthogonalisation, PLS-SVD solves and LDA run in FP64 for both
  * precisions. Gaussian sketches are drawn on the host with
  * std::mt19937(seed) and std::normal_distribution<double>, the same stream
