@@ -244,3 +244,47 @@ def test_errors_are_reported_not_fallen_back(hip):
         fit(hip, X, labels=lab, classes=3, components=50)
     with pytest.raises(fp.HipError, match="components must be below"):
         fit(hip, X, labels=lab, classes=3, method=1, components=3)
+
+
+def test_plsda_full_size(hip):
+    """PLS-DA at full size: n=50000, p=10000, K=20 balanced classes whose means
+    are N(0, 0.5² I) in a 15-dimensional latent space, X = Z Lᵀ + E, 15
+    SIMPLS components and LDA; 10000 held-out rows. FP32 storage against the
+    FP64 path on the same values (both held to the oracle at 10000 x 2500 in
+    test_simpls_block_routing_classification): coefficients within 1e-5,
+    discriminant scores within 1e-5, held-out labels identical except where
+    the FP64 top-2 margin is below 1e-6."""
+    fp = hip[0]
+    n, p, K, A, nt = 50000, 10000, 20, 15, 10000
+    rng = np.random.default_rng(2)
+    labels = (np.arange(n) % K + 1).astype(np.int32)
+    rng.shuffle(labels)
+    means = 0.5 * rng.standard_normal((K, 15))
+    L = rng.standard_normal((p, 15))
+
+    def draw(lab, r):
+        X = (means[lab - 1] @ L.T).astype(np.float32)
+        X += r.standard_normal(X.shape, dtype=np.float32)
+        return np.asfortranarray(X)
+
+    X = draw(labels, rng)
+    rt = np.random.default_rng([2, 2])
+    labt = (np.arange(nt) % K + 1).astype(np.int32)
+    Xt = draw(labt, rt)
+    m32 = fp.ResidentModel.fit(X, labels=labels, classes=K, precision=32, components=A, seed=1)
+    m64 = fp.ResidentModel.fit(X.astype(np.float64), labels=labels, classes=K, precision=64,
+                               components=A, seed=1)
+    assert m32.controls()["block"] == m64.controls()["block"] == 15
+    for k in (1, 8, 15):
+        assert rel(hip_coef(m32, k), hip_coef(m64, k)) < 1e-5, k
+    d32 = m32.predict_path(Xt, [A], lda=True)[:, :, 0].astype(np.float64)
+    d64 = m64.predict_path(Xt.astype(np.float64), [A], lda=True)[:, :, 0]
+    assert rel(d32, d64) < 1e-5
+    l32 = m32.classify_path(Xt, [A], lda=True, top=1)[:, 0, 0]
+    l64 = m64.classify_path(Xt.astype(np.float64), [A], lda=True, top=1)[:, 0, 0]
+    s = np.sort(d64, axis=1)
+    ok = (l32 == l64) | (s[:, -1] - s[:, -2] < 1e-6)
+    assert ok.all(), np.flatnonzero(~ok)[:10]
+    assert (l64 == labt).mean() > 0.5   # the classes are separable at this spread
+    m32.close()
+    m64.close()
