@@ -55,12 +55,18 @@ class Model final : public Handle, private StreamOps {
         throw std::invalid_argument("classification PLS-SVD components must be below class count");
     }
     A_ = components;
+    // R/main.R:3454-3463: no cross-covariance is formed once p·q·8 bytes
+    // exceed 32 MiB (dense responses); above 512 MiB of entries SIMPLS takes
+    // the rank-one direction route (src/r_api.cpp:1480-1493).
+    implicit_ = !classification_ && double(p) * double(q) * 8.0 / (1024.0 * 1024.0) > 32.0;
+    rank_one_ = implicit_ && method_ == kSimpls &&
+                double(p) * double(q) * double(sizeof(T)) > 512.0 * 1024.0 * 1024.0;
     stream_ = stream_take();
     for (Buf* b : all_bufs()) b->bind(stream_);
     try {
       Event start, stop;
       load_inputs(x, y, labels, device_inputs);
-      if (method_ == kPlssvd) stage_plssvd_sketch();
+      if (method_ == kPlssvd && !implicit_) stage_plssvd_sketch();
       FPLS_HIP(hipEventRecord(start.e, stream_));
       prepare_crossprod();
       if (method_ == kSimpls) fit_simpls(); else fit_plssvd();
@@ -240,7 +246,7 @@ class Model final : public Handle, private StreamOps {
     *power = power_;
     *block = method_ == kSimpls ? block_ : A_;
     *block_limit = 0;
-    *implicit = 0;
+    *implicit = implicit_ ? 1 : 0;
     *cache = 0;
   }
 
@@ -253,9 +259,11 @@ class Model final : public Handle, private StreamOps {
     for (Buf* b : {&xown_, &yown_, &Yc_, &S_, &S0_, &G_, &V_, &Tsc_, &simpls_scratch_, &cand_,
                    &cand_scaled_, &t_raw_, &ts_part_, &ls_part_, &omega_, &steps_, &fitdev_,
                    &part_, &tmp1_, &tmp2_, &tmp3_, &H_, &labels_dev_, &frag_, &cp_part_,
-                   &cp_stats_, &qr_gram_, &qr_rinv_, &qr_tmp_, &qr_fail_})
+                   &cp_stats_, &qr_gram_, &qr_rinv_, &qr_tmp_, &qr_fail_, &qraw_, &op_sig_,
+                   &op_tmp_, &op_scaled_, &op_coef_, &op_corr_, &op_pb_, &op_rev_, &op_sample_})
       b->release();
     for (Buf& b : svd_ws_) b.release();
+    for (Buf& b : opw_) b.release();
     X_ = nullptr;
     compacted_ = true;
   }
@@ -267,12 +275,14 @@ class Model final : public Handle, private StreamOps {
     std::vector<Buf*> v = {&xown_, &yown_, &labels_dev_, &counts_, &Yc_, &H_, &ysum_, &my_, &mx_,
                            &sx_, &colss_, &S_, &S0_, &G_, &R_, &Q_, &V_, &Tsc_, &Rs_, &sigma_,
                            &gram_, &W_, &state_, &omega_, &steps_, &fitdev_, &simpls_scratch_,
-                           &cand_, &cand_scaled_, &t_raw_, &ts_part_, &ls_part_, &frag_, &part_,
-                           &tmp1_, &tmp2_, &tmp3_, &qr_gram_, &qr_rinv_, &qr_tmp_, &qr_fail_,
-                           &zero_ints_, &cp_part_, &cp_stats_, &usable_, &rows_, &scores_,
+                           &cand_, &cand_scaled_, &t_raw_, &ts_part_, &ls_part_, &tmp1_,
+                           &tmp2_, &tmp3_, &cp_part_, &cp_stats_, &usable_, &rows_, &scores_,
                            &pred64_, &outT_, &top_, &metric_, &flags_, &obs_, &obs_labels_,
                            &lda_gram_, &lda_sums_, &lda_linear_, &lda_constants_, &lda_ridge_,
-                           &lda_scratch_, &var_loadings_, &var_ss_};
+                           &lda_scratch_, &var_loadings_, &var_ss_, &qraw_, &op_sig_,
+                           &op_tmp_, &op_scaled_, &op_coef_, &op_corr_, &op_pb_, &op_rev_,
+                           &op_sample_};
+    for (Buf* b : ops_bufs()) v.push_back(b);
     for (Buf& b : svd_ws_) v.push_back(&b);
     return v;
   }
@@ -328,34 +338,112 @@ class Model final : public Handle, private StreamOps {
 
   // ----------------------------------------------------- preprocessing
   // supervised.hpp:27-124 / classification.hpp:86-184 in one pass over X.
+  // On the matrix-free route only the column statistics are kept: the pass
+  // runs against the first response column and its one-column product is
+  // dropped.
   void prepare_crossprod() {
+    const int qc = implicit_ ? 1 : q_;
     mx_.alloc(sizeof(double) * p_);
     sx_.alloc(sizeof(double) * p_);
     colss_.alloc(sizeof(double) * p_);
-    S_.alloc(sizeof(double) * (size_t)p_ * q_);
-    S0_.alloc(sizeof(double) * (size_t)p_ * q_);
-    const int splits = xtw_splits(n_, p_, std::min(q_, 64));
+    S_.alloc(sizeof(double) * (size_t)p_ * qc);
+    S0_.alloc(sizeof(double) * (size_t)p_ * qc);
+    const int splits = xtw_splits(n_, p_, std::min(qc, 64));
     Buf& s_part = cp_part_;
     Buf& stats = cp_stats_;
-    s_part.alloc(sizeof(double) * (size_t)splits * p_ * q_);
+    s_part.alloc(sizeof(double) * (size_t)splits * p_ * qc);
     stats.alloc(sizeof(double) * (size_t)splits * 2 * p_);
-    void* wf = frag(xtw_frag_bytes<T>(n_, std::min(q_, 64)));
+    void* wf = frag(xtw_frag_bytes<T>(n_, std::min(qc, 64)));
     FPLS_HIP(hipEventRecord(cp0_.e, stream_));
-    for (int c0 = 0; c0 < q_; c0 += 64) {
-      const int nb = std::min(64, q_ - c0);
+    for (int c0 = 0; c0 < qc; c0 += 64) {
+      const int nb = std::min(64, qc - c0);
       XtwTask<T> t{};
       t.A = X_; t.m = n_; t.k = p_; t.center = nullptr;
       t.W = Yc_.d() + (long long)c0 * n_; t.ldw = n_; t.N = nb; t.mode = kShiftStats;
       t.frag = wf;
       t.splits = splits;
-      t.out = s_part.d() + (long long)c0 * p_; t.ldo = p_; t.slab = (long long)p_ * q_;
+      t.out = s_part.d() + (long long)c0 * p_; t.ldo = p_; t.slab = (long long)p_ * qc;
       t.stats = c0 == 0 ? stats.d() : nullptr;
       launch_xtw_single<T>(t, stream_);
     }
     FPLS_HIP(hipEventRecord(cp1_.e, stream_));
-    launch_finalize_crossprod<T>(X_, n_, p_, q_, scaling_, stats.d(), s_part.d(), splits,
+    launch_finalize_crossprod<T>(X_, n_, p_, qc, scaling_, stats.d(), s_part.d(), splits,
                                  ysum_.d(), mx_.d(), sx_.d(), colss_.d(), S_.d(), S0_.d(),
                                  state_.as<int>() + 2, stream_);
+    if (implicit_) {
+      S_.release();
+      S0_.release();
+    }
+  }
+
+  // ------------------------------------------------- matrix-free route
+  // operators.hpp:58-156 (CenteredCrosscovOperator): S Z = X_sᵀ (Ỹ Z) for Z
+  // (q x w) and Sᵀ B = Ỹᵀ (X_s B) for B (p x w), with X_s = (X - 1 mᵀ)/s
+  // applied through the centred products and a row scaling.
+  void crosscov_fwd(const double* Z, int w, double* out) {
+    op_tmp_.reserve(sizeof(double) * (size_t)n_ * w);
+    product<double>(Yc_.d(), n_, q_, nullptr, Z, q_, w, op_tmp_.d());
+    tproduct<T>(X_, n_, p_, mx_.d(), op_tmp_.d(), n_, w, out);
+    launch_div_rows(out, sx_.d(), p_, w, out, stream_);
+  }
+
+  void crosscov_adj(const double* B, int w, double* out) {
+    op_scaled_.reserve(sizeof(double) * (size_t)p_ * w);
+    launch_div_rows(B, sx_.d(), p_, w, op_scaled_.d(), stream_);
+    op_tmp_.reserve(sizeof(double) * (size_t)n_ * w);
+    product<T>(X_, n_, p_, mx_.d(), op_scaled_.d(), p_, w, op_tmp_.d());
+    tproduct<double>(Yc_.d(), n_, q_, nullptr, op_tmp_.d(), n_, w, out);
+  }
+
+  // M (p x w) -= V (Vᵀ M) over the first `used` deflation directions (one
+  // classical Gram-Schmidt pass, operators.hpp:158-301).
+  void project_out(double* M, int w, int used) {
+    if (used <= 0) return;
+    op_coef_.reserve(sizeof(double) * (size_t)used * w);
+    op_corr_.reserve(sizeof(double) * (size_t)p_ * w);
+    tproduct<double>(V_.d(), p_, used, nullptr, M, p_, w, op_coef_.d());
+    product<double>(V_.d(), p_, used, nullptr, op_coef_.d(), used, w, op_corr_.d());
+    launch_axpy(-1.0, op_corr_.d(), M, (long long)p_ * w, stream_);
+  }
+
+  // (I - V Vᵀ) S Z and Sᵀ (I - V Vᵀ) B (ProjectedOperator::multiply).
+  void projected_fwd(const double* Z, int w, double* out, int used) {
+    crosscov_fwd(Z, w, out);
+    project_out(out, w, used);
+  }
+
+  void projected_adj(const double* B, int w, double* out, int used) {
+    op_pb_.reserve(sizeof(double) * (size_t)p_ * w);
+    FPLS_HIP(hipMemcpyAsync(op_pb_.d(), B, sizeof(double) * (size_t)p_ * w,
+                            hipMemcpyDeviceToDevice, stream_));
+    project_out(op_pb_.d(), w, used);
+    crosscov_adj(op_pb_.d(), w, out);
+  }
+
+  // operator_rsvd.hpp:175-202 (randomized_dominant_operator_direction): a
+  // p-vector from the host stream of `seed`, stabilised (two projections),
+  // normalised, then max(power, 1) rounds of Sᵀ, normalise, S, stabilise,
+  // normalise. `normals` holds the draw; a direction that cannot be
+  // normalised stops the fit (state[1]).
+  void dominant_direction(const double* normals, int used, double* dir) {
+    int* stop = state_.as<int>() + 1;
+    op_rev_.reserve(sizeof(double) * (size_t)q_);
+    op_sample_.reserve(sizeof(double) * (size_t)p_);
+    FPLS_HIP(hipMemcpyAsync(dir, normals, sizeof(double) * (size_t)p_, hipMemcpyDeviceToDevice,
+                            stream_));
+    project_out(dir, 1, used);
+    project_out(dir, 1, used);
+    launch_unit_columns(dir, p_, 1, stop, stream_);
+    for (int it = 0; it < std::max(power_, 1); ++it) {
+      projected_adj(dir, 1, op_rev_.d(), used);
+      launch_unit_columns(op_rev_.d(), q_, 1, stop, stream_);
+      projected_fwd(op_rev_.d(), 1, op_sample_.d(), used);
+      project_out(op_sample_.d(), 1, used);
+      project_out(op_sample_.d(), 1, used);
+      launch_unit_columns(op_sample_.d(), p_, 1, stop, stream_);
+      FPLS_HIP(hipMemcpyAsync(dir, op_sample_.d(), sizeof(double) * (size_t)p_,
+                              hipMemcpyDeviceToDevice, stream_));
+    }
   }
 
   // ------------------------------------------------------------ SIMPLS
@@ -363,8 +451,9 @@ class Model final : public Handle, private StreamOps {
   void fit_simpls() {
     const int maximum = std::min({A_, p_, std::max(n_ - 1, 1)});
     A_ = maximum;
-    block_ = candidate_block_size(maximum, p_, q_, classification_, n_, 64);
-    const bool right_gram = q_ <= p_ && q_ <= 512;
+    // the rank-one route refreshes one direction at a time
+    block_ = rank_one_ ? 1 : candidate_block_size(maximum, p_, q_, classification_, n_, 64);
+    const bool right_gram = !implicit_ && q_ <= p_ && q_ <= 512;
     const int maxdim = std::min(p_, q_);
     std::vector<RefreshStep> steps;
     std::vector<double> pool;
@@ -374,9 +463,12 @@ class Model final : public Handle, private StreamOps {
       const int w = std::min(maxdim, b + oversample_);
       RefreshStep st{};
       st.a = a; st.b = b; st.w = w; st.omega_off = (long long)pool.size();
+      // sketches: q x w for the explicit refresh; the rank-one route draws
+      // a p-vector per step; the operator rSVD draws its own (same stream)
+      const size_t count = !implicit_ ? (size_t)q_ * w : rank_one_ ? (size_t)p_ : 0;
       const size_t base = pool.size();
-      pool.resize(base + (size_t)q_ * w);
-      gaussian(unsigned(seed_) + unsigned(a), (size_t)q_ * w, pool.data() + base);
+      pool.resize(base + count);
+      if (count) gaussian(unsigned(seed_) + unsigned(a), count, pool.data() + base);
       steps.push_back(st);
       wmax = std::max(wmax, w);
       bmax = std::max(bmax, b);
@@ -395,8 +487,10 @@ class Model final : public Handle, private StreamOps {
     omega_host_ = std::move(pool);
     steps_host_ = std::move(steps);
     omega_.alloc(sizeof(double) * omega_host_.size());
-    FPLS_HIP(hipMemcpyAsync(omega_.d(), omega_host_.data(), sizeof(double) * omega_host_.size(),
-                            hipMemcpyHostToDevice, stream_));
+    if (!omega_host_.empty())
+      FPLS_HIP(hipMemcpyAsync(omega_.d(), omega_host_.data(),
+                              sizeof(double) * omega_host_.size(), hipMemcpyHostToDevice,
+                              stream_));
     steps_.alloc(sizeof(RefreshStep) * steps_host_.size());
     FPLS_HIP(hipMemcpyAsync(steps_.as<RefreshStep>(), steps_host_.data(),
                             sizeof(RefreshStep) * steps_host_.size(), hipMemcpyHostToDevice, stream_));
@@ -413,11 +507,19 @@ class Model final : public Handle, private StreamOps {
     SimplsFitDev f{};
     f.n = n_; f.p = p_; f.q = q_; f.A = A_;
     f.power = power_; f.use_right_gram = right_gram ? 1 : 0;
-    f.reorth = sizeof(T) == 4 ? 1 : 0;  // src/r_api.cpp:1416: two passes for float
+    // two passes for float (src/r_api.cpp:1416) and on the rank-one route
+    // (src/r_api.cpp:1491)
+    f.reorth = (sizeof(T) == 4 || rank_one_) ? 1 : 0;
     f.nsteps = int(steps_ref.size()); f.bmax = bmax; f.ts_splits = xs; f.ls_splits = ls;
     f.ls_slab = (long long)p_ * bmax;
     f.steps = steps_.as<RefreshStep>(); f.omega = omega_.d();
-    f.S = S_.d(); f.S0 = S0_.d(); f.G = right_gram ? G_.d() : nullptr; f.sx = sx_.d();
+    f.S = implicit_ ? nullptr : S_.d(); f.S0 = implicit_ ? nullptr : S0_.d();
+    f.G = right_gram ? G_.d() : nullptr; f.sx = sx_.d();
+    if (implicit_) {
+      qraw_.alloc(sizeof(double) * (size_t)q_ * bmax);
+      op_sig_.alloc(sizeof(double) * bmax);
+      f.qraw = qraw_.d();
+    }
     f.cand = cand_.d(); f.cand_scaled = cand_scaled_.d(); f.t_raw = t_raw_.d();
     f.ts_part = ts_part_.d(); f.ls_part = ls_part_.d();
     f.R = R_.d(); f.Q = Q_.d(); f.V = V_.d(); f.T = Tsc_.d();
@@ -430,7 +532,22 @@ class Model final : public Handle, private StreamOps {
     FPLS_HIP(hipEventRecord(sc0_.e, stream_));
     for (size_t s = 0; s < steps_ref.size(); ++s) {
       const int b = steps_ref[s].b;
-      launch_simpls_refresh(fitdev_.as<SimplsFitDev>(), 1, int(s), stream_);
+      if (!implicit_) {
+        launch_simpls_refresh(fitdev_.as<SimplsFitDev>(), 1, int(s), stream_);
+      } else {
+        // simpls.hpp:824-858: candidates from the projected operator
+        const int used = steps_ref[s].a;
+        if (rank_one_) {
+          dominant_direction(omega_.d() + steps_ref[s].omega_off, used, cand_.d());
+        } else {
+          operator_rsvd(
+              [&](const double* Z, int w, double* out) { projected_fwd(Z, w, out, used); },
+              [&](const double* B, int w, double* out) { projected_adj(B, w, out, used); },
+              p_, q_, b, oversample_, power_, unsigned(seed_) + unsigned(used), true, cand_.d(),
+              op_sig_.d(), nullptr, state_.as<int>() + 2);
+        }
+        launch_div_rows(cand_.d(), sx_.d(), p_, b, cand_scaled_.d(), stream_);
+      }
       // scores of the candidate block: X_s C = (X - 1 mᵀ)(C / s)
       XwTask<T> xw{};
       xw.A = X_; xw.m = n_; xw.k = p_; xw.center = mx_.d(); xw.W = cand_scaled_.d(); xw.ldw = p_;
@@ -445,6 +562,8 @@ class Model final : public Handle, private StreamOps {
       xt.slab = (long long)p_ * bmax;
       xt.frag = frag(xtw_frag_bytes<T>(n_, b));
       launch_xtw_single<T>(xt, stream_);
+      // response loadings without S0: Ỹᵀ t (scaled by 1/|t| in the update)
+      if (implicit_) tproduct<double>(Yc_.d(), n_, q_, nullptr, t_raw_.d(), n_, b, qraw_.d());
       launch_simpls_update(fitdev_.as<SimplsFitDev>(), 1, int(s), stream_);
     }
     FPLS_HIP(hipEventRecord(sc1_.e, stream_));
@@ -461,12 +580,21 @@ class Model final : public Handle, private StreamOps {
     sigma_.alloc(sizeof(double) * k);
     usable_.alloc(sizeof(int));
     int* flags = state_.as<int>() + 2;
-    reserve_plssvd_parts(k, width);
-    if (width >= maxdim) {
-      dense_svd(k, flags);
+    if (implicit_) {
+      // plssvd.hpp:222-248: the operator rSVD of S = X_sᵀ Ỹ without S
+      operator_rsvd([&](const double* Z, int w, double* out) { crosscov_fwd(Z, w, out); },
+                    [&](const double* B, int w, double* out) { crosscov_adj(B, w, out); }, p_,
+                    q_, k, oversample_, power_, unsigned(seed_), false, R_.d(), sigma_.d(),
+                    Q_.d(), flags);
       launch_fill_int(usable_.as<int>(), 1, k, stream_);
     } else {
-      randomized_svd(k, width, usable_.as<int>(), flags);
+      reserve_plssvd_parts(k, width);
+      if (width >= maxdim) {
+        dense_svd(k, flags);
+        launch_fill_int(usable_.as<int>(), 1, k, stream_);
+      } else {
+        randomized_svd(k, width, usable_.as<int>(), flags);
+      }
     }
     // (the usable component count is checked with the other fit flags)
     // scores T = X_s R, Gram TᵀT, prefix weights
@@ -747,6 +875,9 @@ class Model final : public Handle, private StreamOps {
   Buf var_loadings_, var_ss_;
   int lda_prefix_ = 0;
   int scores_rows_ = 0;
+  bool implicit_ = false, rank_one_ = false;
+  // matrix-free route workspaces
+  Buf qraw_, op_sig_, op_tmp_, op_scaled_, op_coef_, op_corr_, op_pb_, op_rev_, op_sample_;
 };
 
 }  // namespace

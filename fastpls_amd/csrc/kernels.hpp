@@ -39,6 +39,9 @@ struct SimplsFitDev {
   double* T;               // n x A
   double* scratch;         // refresh / update scratch
   int* state;              // [0] completed, [1] stopped, [2] error flags
+  // matrix-free route (no S): Ỹᵀ t_raw (q x bmax) of the candidate block;
+  // S, S0 and G are null and the update neither deflates S nor reads S0
+  const double* qraw;
 };
 
 // Error flags written to state[2].
@@ -73,6 +76,14 @@ void launch_rsvd_finalize(const double* P, int w, int q, int k, double* Up,
                           int* flags, hipStream_t s);
 // Dense economy SVD of a small square or tall matrix M (m x n, m >= n), the
 // first k triplets: U (m x k), sigma (k), V (n x k, right vectors as columns).
+// Eigen route of the operator rSVD's finalisation (operator_rsvd.hpp:64-168)
+// on the projected Gram (w x w) of a block w wide and `cols` long: writes the
+// top-r eigenvectors (left, w x r), the same divided by σ (left_scaled), σ (r)
+// and status[0] = 1, or status[0] = 0 when the dense route has to run.
+// scratch: 3 w² + w doubles (used when w > 64).
+void launch_operator_finalize(const double* gram, int w, int cols, int r, double* left,
+                              double* left_scaled, double* sigma, double* scratch, int* status,
+                              hipStream_t s);
 void launch_small_svd(const double* M, int m, int n, int k, double* U, double* sigma,
                       double* V, double* scratch, int* flags, hipStream_t s);
 
@@ -141,6 +152,11 @@ void launch_variance(const double* cross, const double* gram, const double* cols
 
 // p[0..count) = value (a kernel, so streams stay graph-capturable).
 void launch_fill_int(int* p, int count, int value, hipStream_t s);
+// y += alpha x over `count` doubles.
+void launch_axpy(double alpha, const double* x, double* y, long long count, hipStream_t s);
+// Columns of v (rows x cols) scaled to unit norm; a column that cannot be
+// (norm non-finite or <= eps) is left as it is and sets *stop to 1.
+void launch_unit_columns(double* v, long long rows, int cols, int* stop, hipStream_t s);
 
 // Block copy: dst (rows x cols, leading dimension ldd) = src (leading dimension lds).
 template<class T>

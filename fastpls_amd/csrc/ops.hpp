@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 namespace fpls {
@@ -109,8 +110,97 @@ class StreamOps {
                             hipMemcpyDeviceToDevice, stream_));
   }
 
+  // ----------------------------------------------------- operator rSVD
+  // operator_rsvd.hpp:204-272 (randomized_operator_svd) for an operator A
+  // (rows x cols) known through fwd(Z, w, out): out (rows x w) = A Z for Z
+  // (cols x w), and adj(B, w, out): out (cols x w) = Aᵀ B for B (rows x w).
+  // The Gaussian sketch is the host stream of `seed`. Writes U (rows x r),
+  // s (r) and, unless left_only, V (cols x r), the right vectors. The eigen
+  // route of the finalisation (squared rank tolerance, all r directions
+  // required) runs on the projected Gram; otherwise the dense route factors
+  // Pᵀ = Qr R and takes the SVD of R. One host synchronisation decides the
+  // route. QR failures set kFlagDecomposition in *flags.
+  template<class Fwd, class Adj>
+  void operator_rsvd(Fwd&& fwd, Adj&& adj, long long rows, long long cols, int r,
+                     int oversample, int power, unsigned int seed, bool left_only, double* U,
+                     double* s, double* V, int* flags) {
+    const long long maximum = std::min(rows, cols);
+    const int w = int(std::min<long long>(maximum, (long long)r + std::max(oversample, 0)));
+    if (r < 1 || r > w) throw std::invalid_argument("operator rSVD rank outside the sketch");
+    if (w > kOpMaxW)
+      throw std::runtime_error("sketch width " + std::to_string(w) +
+                               " exceeds the supported 512 columns");
+    Buf &omega = opw_[0], &sample = opw_[1], &fb = opw_[2], &rev = opw_[3], &rb = opw_[4],
+        &gram = opw_[5], &left = opw_[6], &lscaled = opw_[7], &scratch = opw_[8],
+        &status = opw_[9], &rcopy = opw_[10], &rm = opw_[11], &su = opw_[12], &ss = opw_[13],
+        &sv = opw_[14];
+    // the host sketch of the previous call must have been copied before it
+    // is overwritten: calls that took no route decision leave a wait here
+    if (op_upload_pending_) FPLS_HIP(hipStreamSynchronize(stream_));
+    op_upload_pending_ = true;
+    op_omega_host_.resize((size_t)cols * w);
+    gaussian(seed, op_omega_host_.size(), op_omega_host_.data());
+    omega.reserve(sizeof(double) * op_omega_host_.size());
+    FPLS_HIP(hipMemcpyAsync(omega.d(), op_omega_host_.data(),
+                            sizeof(double) * op_omega_host_.size(), hipMemcpyHostToDevice,
+                            stream_));
+    sample.reserve(sizeof(double) * (size_t)rows * w);
+    fb.reserve(sizeof(double) * (size_t)rows * w);
+    rev.reserve(sizeof(double) * (size_t)cols * w);
+    rb.reserve(sizeof(double) * (size_t)cols * w);
+    fwd(omega.d(), w, sample.d());
+    for (int it = 0; it < power; ++it) {
+      qr_tall(sample.d(), int(rows), w, fb.d(), flags);
+      adj(fb.d(), w, rev.d());
+      qr_tall(rev.d(), int(cols), w, rb.d(), flags);
+      fwd(rb.d(), w, sample.d());
+    }
+    qr_tall(sample.d(), int(rows), w, fb.d(), flags);
+    adj(fb.d(), w, rev.d());  // Pᵀ (cols x w)
+    bool eigen = false;
+    if (w >= 4) {
+      gram.reserve(sizeof(double) * (size_t)w * w);
+      left.reserve(sizeof(double) * (size_t)w * r);
+      lscaled.reserve(sizeof(double) * (size_t)w * r);
+      scratch.reserve(sizeof(double) * (3 * (size_t)w * w + w + 64));
+      status.reserve(sizeof(int));
+      tproduct<double>(rev.d(), cols, w, nullptr, rev.d(), cols, w, gram.d());
+      launch_operator_finalize(gram.d(), w, int(cols), r, left.d(), lscaled.d(), s,
+                               scratch.d(), status.as<int>(), stream_);
+      int st = 0;
+      FPLS_HIP(hipMemcpyAsync(&st, status.as<int>(), sizeof(int), hipMemcpyDeviceToHost,
+                              stream_));
+      FPLS_HIP(hipStreamSynchronize(stream_));
+      op_upload_pending_ = false;
+      eigen = st == 1;
+    }
+    if (eigen) {
+      product<double>(fb.d(), rows, w, nullptr, left.d(), w, r, U);
+      if (!left_only) product<double>(rev.d(), cols, w, nullptr, lscaled.d(), w, r, V);
+      return;
+    }
+    // dense route: Pᵀ = Qr R, R = Ur σ Vrᵀ  =>  P = Vr σ (Qr Ur)ᵀ
+    rcopy.reserve(sizeof(double) * (size_t)cols * w);
+    rm.reserve(sizeof(double) * (size_t)w * w);
+    su.reserve(sizeof(double) * (size_t)w * w);
+    ss.reserve(sizeof(double) * (size_t)w);
+    sv.reserve(sizeof(double) * (size_t)w * w);
+    scratch.reserve(sizeof(double) * (3 * (size_t)w * w + w + 64));
+    FPLS_HIP(hipMemcpyAsync(rcopy.d(), rev.d(), sizeof(double) * (size_t)cols * w,
+                            hipMemcpyDeviceToDevice, stream_));
+    qr_tall(rcopy.d(), int(cols), w, rb.d(), flags);
+    tproduct<double>(rb.d(), cols, w, nullptr, rev.d(), cols, w, rm.d());
+    launch_small_svd(rm.d(), w, w, w, su.d(), ss.d(), sv.d(), scratch.d(), flags, stream_);
+    product<double>(fb.d(), rows, w, nullptr, sv.d(), w, r, U);
+    if (!left_only) product<double>(rb.d(), cols, w, nullptr, su.d(), w, r, V);
+    FPLS_HIP(hipMemcpyAsync(s, ss.d(), sizeof(double) * r, hipMemcpyDeviceToDevice, stream_));
+  }
+
   std::vector<Buf*> ops_bufs() {
-    return {&frag_, &part_, &qr_gram_, &qr_rinv_, &qr_tmp_, &qr_fail_, &zero_ints_};
+    std::vector<Buf*> v = {&frag_, &part_, &qr_gram_, &qr_rinv_, &qr_tmp_, &qr_fail_,
+                           &zero_ints_};
+    for (Buf& b : opw_) v.push_back(&b);
+    return v;
   }
 
   hipStream_t stream_ = nullptr;
@@ -119,6 +209,10 @@ class StreamOps {
   bool zero_ready_ = false;
   Buf frag_;  // matrix-core operand workspace of the streaming products
   Buf part_, qr_gram_, qr_rinv_, qr_tmp_, qr_fail_, zero_ints_;
+  static constexpr int kOpMaxW = 512;   // widest operator sketch (one-work-group QR / SVD)
+  Buf opw_[15];                          // operator rSVD workspaces
+  std::vector<double> op_omega_host_;
+  bool op_upload_pending_ = false;
 };
 
 }  // namespace fpls

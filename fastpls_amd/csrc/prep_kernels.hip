@@ -70,6 +70,29 @@ __global__ __launch_bounds__(kWG) void k_fill_int(int* p, int count, int value) 
   for (int i = blockIdx.x * kWG + threadIdx.x; i < count; i += gridDim.x * kWG) p[i] = value;
 }
 
+__global__ __launch_bounds__(kWG) void k_axpy(double alpha, const double* x, double* y,
+                                              long long count) {
+  for (long long i = (long long)blockIdx.x * kWG + threadIdx.x; i < count;
+       i += (long long)gridDim.x * kWG)
+    y[i] = fma(alpha, x[i], y[i]);
+}
+
+// Each column (one work-group) scaled to unit norm; a column whose norm is
+// non-finite or at most eps stays as it is and sets *stop
+// (operator_rsvd.hpp:37-61, normalize_column).
+__global__ __launch_bounds__(kWG) void k_unit_columns(double* v, long long rows, int* stop) {
+  __shared__ double red[kWaves];
+  double* c = v + (long long)blockIdx.x * rows;
+  double ss = 0.0;
+  for (long long i = threadIdx.x; i < rows; i += kWG) ss = fma(c[i], c[i], ss);
+  const double nrm = sqrt(wg_sum(ss, red));
+  if (!isfinite(nrm) || !(nrm > DBL_EPSILON)) {
+    if (threadIdx.x == 0) atomicOr(stop, 1);
+    return;
+  }
+  for (long long i = threadIdx.x; i < rows; i += kWG) c[i] /= nrm;
+}
+
 // classification.hpp:106-123: class counts (integer atomics: order-free).
 __global__ __launch_bounds__(kWG) void k_label_counts(const int* labels, long long n, int K,
                                                       int* counts, int* flags) {
@@ -373,6 +396,18 @@ void launch_center_response(const T* Y, long long n, int q, double* mean, double
   hipLaunchKernelGGL(k_center_fold, dim3((q + kWG - 1) / kWG), dim3(kWG), 0, s, part,
                      part + (long long)q * kCenterBlocks, n, q, mean, ysum);
   FPLS_LAUNCHED("k_center_fold");
+}
+
+void launch_axpy(double alpha, const double* x, double* y, long long count, hipStream_t s) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_axpy, dim3(grid_for(count)), dim3(kWG), 0, s, alpha, x, y, count);
+  FPLS_LAUNCHED("k_axpy");
+}
+
+void launch_unit_columns(double* v, long long rows, int cols, int* stop, hipStream_t s) {
+  if (cols <= 0 || rows <= 0) return;
+  hipLaunchKernelGGL(k_unit_columns, dim3(unsigned(cols)), dim3(kWG), 0, s, v, rows, stop);
+  FPLS_LAUNCHED("k_unit_columns");
 }
 
 void launch_fill_int(int* p, int count, int value, hipStream_t s) {

@@ -154,8 +154,14 @@ __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits,
       Vc[i] = s / f.sx[i] * inv;
     }
     __syncthreads();
-    // response loadings q_a = S0ᵀ r
-    wg::gemm_tn_long(q, 1, p, f.S0, p, Rc, p, f.Q + (long long)comp * q, q);
+    // response loadings q_a = S0ᵀ r (= Ỹᵀ t on the matrix-free route)
+    if (f.qraw) {
+      for (int c = threadIdx.x; c < q; c += kWG)
+        f.Q[c + (long long)comp * q] = f.qraw[c + (long long)j * q] * inv;
+      __syncthreads();
+    } else {
+      wg::gemm_tn_long(q, 1, p, f.S0, p, Rc, p, f.Q + (long long)comp * q, q);
+    }
     // v = loading - V (Vᵀ loading), once (twice when re-orthogonalising)
     if (comp > 0) {
       for (int pass = 0; pass < (f.reorth ? 2 : 1); ++pass) {
@@ -177,13 +183,16 @@ __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits,
     const double vinv = 1.0 / vn;
     for (int i = threadIdx.x; i < p; i += kWG) Vc[i] *= vinv;
     __syncthreads();
-    // deflation row = vᵀ S ; S -= v row
-    wg::gemm_tn_long(q, 1, p, f.S, p, Vc, p, row, q);
-    for (long long e = threadIdx.x; e < (long long)p * q; e += kWG) {
-      const int i = int(e % p), c = int(e / p);
-      f.S[e] -= Vc[i] * row[c];
+    // deflation row = vᵀ S ; S -= v row (the matrix-free route keeps v in V
+    // and projects it out of every product instead)
+    if (f.S) {
+      wg::gemm_tn_long(q, 1, p, f.S, p, Vc, p, row, q);
+      for (long long e = threadIdx.x; e < (long long)p * q; e += kWG) {
+        const int i = int(e % p), c = int(e / p);
+        f.S[e] -= Vc[i] * row[c];
+      }
     }
-    if (f.use_right_gram) {
+    if (f.S && f.use_right_gram) {
       for (int e = threadIdx.x; e < q * q; e += kWG) {
         const int i = e % q, c = e / q;
         f.G[e] -= row[i] * row[c];
@@ -402,6 +411,56 @@ __global__ __launch_bounds__(NT) void k_rsvd_finalize(const double* P, int w, in
     __syncthreads();
     if (threadIdx.x == 0) *usable_out = ret;
   }
+}
+
+// ------------------------------------------------ operator finalisation
+// Eigen route of the operator finalisation (operator_rsvd.hpp:64-168): the
+// projected Gram G = P Pᵀ (w x w) of P = Qᵀ A (w x cols) is decomposed, the
+// rank tolerance is (eps max(w, cols))² λmax with no unit floor, and the route
+// is taken only when it yields all r directions. Writes the top-r
+// eigenvectors (left), the same divided by σ (left_scaled), σ, and
+// status[0] = 1 when taken, 0 when the dense route has to run.
+__global__ __launch_bounds__(kWG) void k_operator_finalize(const double* gram, int w, int cols,
+                                                           int r, double* left,
+                                                           double* left_scaled, double* sigma,
+                                                           double* scratch, int* status) {
+  __shared__ WgScratch sh;
+  __shared__ double lg[64 * 64], lv[64 * 64], lt[64 * 64], lev[64];
+  __shared__ int usable_sh;
+  const int tid = threadIdx.x;
+  const bool in_lds = w <= 64;
+  double* Gm = in_lds ? lg : scratch;
+  double* Ve = in_lds ? lv : scratch + (long)w * w;
+  double* vt = in_lds ? lt : scratch + 2L * w * w;
+  double* ev = in_lds ? lev : scratch + 3L * w * w;
+  for (int e = tid; e < w * w; e += kWG) Gm[e] = gram[e];
+  __syncthreads();
+  int done = 0;
+  if (wg::jacobi_eigen(Gm, Ve, w, ev, sh.cs, sh.sn, sh.pa, sh.pb, sh.red, vt)) {
+    if (tid == 0) {
+      const double largest = ev[w - 1];
+      const double rel = DBL_EPSILON * double(w > cols ? w : cols);
+      const double tol = largest > 0.0 ? rel * rel * largest : 0.0;
+      int u = 0;
+      for (int idx = w; idx > 0; --idx) {
+        if (ev[idx - 1] <= tol || u == r) break;
+        ++u;
+      }
+      usable_sh = u;
+    }
+    __syncthreads();
+    if (usable_sh == r) {
+      for (int e = tid; e < w * r; e += kWG) {
+        const int i = e % w, c = e / w;
+        const double v = Ve[i + (w - 1 - c) * w];
+        left[e] = v;
+        left_scaled[e] = v / sqrt(fmax(ev[w - 1 - c], 0.0));
+      }
+      for (int c = tid; c < r; c += kWG) sigma[c] = sqrt(fmax(ev[w - 1 - c], 0.0));
+      done = 1;
+    }
+  }
+  if (tid == 0) status[0] = done;
 }
 
 // Dense SVD of M (m x n, m >= n), first k triplets.
@@ -709,6 +768,15 @@ void launch_rsvd_finalize(const double* P, int w, int q, int k, double* Up, doub
   hipLaunchKernelGGL((k_rsvd_finalize<kWG>), dim3(1), dim3(kWG), 0, s, P, w, q, k, Up, sigma,
                      Vt, usable, scratch, flags);
   FPLS_LAUNCHED("k_rsvd_finalize");
+}
+
+void launch_operator_finalize(const double* gram, int w, int cols, int r, double* left,
+                              double* left_scaled, double* sigma, double* scratch, int* status,
+                              hipStream_t s) {
+  if (w < 1 || w > kMaxW || r < 1 || r > w) throw std::invalid_argument("operator finalisation shape outside the supported range");
+  hipLaunchKernelGGL(k_operator_finalize, dim3(1), dim3(kWG), 0, s, gram, w, cols, r, left,
+                     left_scaled, sigma, scratch, status);
+  FPLS_LAUNCHED("k_operator_finalize");
 }
 
 void launch_small_svd(const double* M, int m, int n, int k, double* U, double* sigma,

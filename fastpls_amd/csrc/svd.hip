@@ -31,68 +31,11 @@ constexpr int kAuditChunks = 64;  // row chunks of the residual reduction
 constexpr int kProbes = 3;        // omitted-direction probes (audited_rsvd.hpp:159)
 constexpr int kRecord = 5;        // residual, finite, omitted, subspace, singular
 
-struct EigLds {
-  double cs[kSvdMaxW / 2 + 1];
-  double sn[kSvdMaxW / 2 + 1];
-  int pa[kSvdMaxW / 2 + 1];
-  int pb[kSvdMaxW / 2 + 1];
-  double red[kWaves];
-};
-
 // std::max(a, b) as the reference applies it: b only when a < b, so a NaN
 // candidate never replaces the running value.
 __device__ inline double keep_max(double a, double b) { return a < b ? b : a; }
 
 // ------------------------------------------------------------ kernels
-// Eigen route of the operator finalisation (operator_rsvd.hpp:64-168): the
-// projected Gram G = P Pᵀ (w x w) of P = Qᵀ A (w x cols) is decomposed, the
-// rank tolerance is (eps max(w, cols))² λmax with no unit floor, and the route
-// is taken only when it yields all r directions. Writes the top-r
-// eigenvectors (left), the same divided by σ (left_scaled), σ, and
-// status[0] = 1 when taken, 0 when the dense route has to run.
-__global__ __launch_bounds__(kWG) void k_operator_finalize(const double* gram, int w, int cols,
-                                                           int r, double* left,
-                                                           double* left_scaled, double* sigma,
-                                                           double* scratch, int* status) {
-  __shared__ EigLds sh;
-  __shared__ double lg[64 * 64], lv[64 * 64], lt[64 * 64], lev[64];
-  __shared__ int usable_sh;
-  const int tid = threadIdx.x;
-  const bool in_lds = w <= 64;
-  double* Gm = in_lds ? lg : scratch;
-  double* Ve = in_lds ? lv : scratch + (long)w * w;
-  double* vt = in_lds ? lt : scratch + 2L * w * w;
-  double* ev = in_lds ? lev : scratch + 3L * w * w;
-  for (int e = tid; e < w * w; e += kWG) Gm[e] = gram[e];
-  __syncthreads();
-  int done = 0;
-  if (wg::jacobi_eigen(Gm, Ve, w, ev, sh.cs, sh.sn, sh.pa, sh.pb, sh.red, vt)) {
-    if (tid == 0) {
-      const double largest = ev[w - 1];
-      const double rel = DBL_EPSILON * double(w > cols ? w : cols);
-      const double tol = largest > 0.0 ? rel * rel * largest : 0.0;
-      int u = 0;
-      for (int idx = w; idx > 0; --idx) {
-        if (ev[idx - 1] <= tol || u == r) break;
-        ++u;
-      }
-      usable_sh = u;
-    }
-    __syncthreads();
-    if (usable_sh == r) {
-      for (int e = tid; e < w * r; e += kWG) {
-        const int i = e % w, c = e / w;
-        const double v = Ve[i + (w - 1 - c) * w];
-        left[e] = v;
-        left_scaled[e] = v / sqrt(fmax(ev[w - 1 - c], 0.0));
-      }
-      for (int c = tid; c < r; c += kWG) sigma[c] = sqrt(fmax(ev[w - 1 - c], 0.0));
-      done = 1;
-    }
-  }
-  if (tid == 0) status[0] = done;
-}
-
 // Sums of squares of F - U diag(σ) (side 0: F = A V, m rows) and of
 // R - V diag(σ) (side 1: R = Aᵀ U, n rows) per (row chunk, column), and the
 // count of non-finite entries of U and V (audited_rsvd.hpp:97-148).
@@ -505,9 +448,8 @@ class AuditedSvd : private StreamOps {
 
   std::vector<Buf*> bufs() {
     std::vector<Buf*> v = ops_bufs();
-    for (Buf* b : {&xown_, &flags_, &rec_, &omega_, &sample_, &fb_, &rev_, &rb_, &gram_, &left_,
-                   &left_scaled_, &fin_scratch_, &status_, &rcopy_, &qr_r_, &small_u_, &small_s_,
-                   &small_v_, &fwd_, &rev2_, &part_sums_, &bad_sums_, &probe_right_, &probe_raw_,
+    for (Buf* b : {&xown_, &flags_, &rec_, &rb_, &fin_scratch_, &rcopy_, &qr_r_, &small_u_,
+                   &small_s_, &small_v_, &fwd_, &rev2_, &part_sums_, &bad_sums_, &probe_right_, &probe_raw_,
                    &probe_left_, &probe_coef_, &probe_proj_, &probe_state_, &probe_norms_,
                    &cross_, &vt_out_, &wide_, &wide_t_})
       v.push_back(b);
@@ -544,73 +486,15 @@ class AuditedSvd : private StreamOps {
   // operator_rsvd.hpp:204-272 with the explicit operator X (operators.hpp:16-56):
   // triplets of rank r into slot `slot`.
   void operator_svd(int slot, int r, int oversample, int power, unsigned int seed) {
-    const int maximum = std::min(m_, n_);
-    const int w = std::min(maximum, r + std::max(oversample, 0));
-    if (w > kSvdMaxW)
-      throw std::runtime_error("fastsvd sketch width " + std::to_string(w) +
-                               " exceeds the supported 512 columns");
-    int* flags = flags_.as<int>();
-    omega_host_.resize((size_t)n_ * w);
-    gaussian(seed, omega_host_.size(), omega_host_.data());
-    omega_.reserve(sizeof(double) * omega_host_.size());
-    // a member vector: it outlives the asynchronous copy (the next resize
-    // follows a synchronisation of the stream)
-    FPLS_HIP(hipMemcpyAsync(omega_.d(), omega_host_.data(), sizeof(double) * omega_host_.size(),
-                            hipMemcpyHostToDevice, stream_));
-    sample_.reserve(sizeof(double) * (size_t)m_ * w);
-    fb_.reserve(sizeof(double) * (size_t)m_ * w);
-    rev_.reserve(sizeof(double) * (size_t)n_ * w);
-    rb_.reserve(sizeof(double) * (size_t)n_ * w);
-    product<T>(X_, m_, n_, nullptr, omega_.d(), n_, w, sample_.d());
-    for (int it = 0; it < power; ++it) {
-      qr_tall(sample_.d(), m_, w, fb_.d(), flags);
-      tproduct<T>(X_, m_, n_, nullptr, fb_.d(), m_, w, rev_.d());
-      qr_tall(rev_.d(), n_, w, rb_.d(), flags);
-      product<T>(X_, m_, n_, nullptr, rb_.d(), n_, w, sample_.d());
-    }
-    qr_tall(sample_.d(), m_, w, fb_.d(), flags);
-    tproduct<T>(X_, m_, n_, nullptr, fb_.d(), m_, w, rev_.d());  // Pᵀ (n x w)
     reserve_triplets(slot, r);
-    bool eigen = false;
-    if (w >= 4) {
-      gram_.reserve(sizeof(double) * (size_t)w * w);
-      left_.reserve(sizeof(double) * (size_t)w * r);
-      left_scaled_.reserve(sizeof(double) * (size_t)w * r);
-      fin_scratch_.reserve(sizeof(double) * (3 * (size_t)w * w + w));
-      status_.reserve(sizeof(int));
-      tproduct<double>(rev_.d(), n_, w, nullptr, rev_.d(), n_, w, gram_.d());
-      hipLaunchKernelGGL(k_operator_finalize, dim3(1), dim3(kWG), 0, stream_, gram_.d(), w, n_, r,
-                         left_.d(), left_scaled_.d(), s_of(slot), fin_scratch_.d(),
-                         status_.as<int>());
-      FPLS_LAUNCHED("k_operator_finalize");
-      int st = 0;
-      FPLS_HIP(hipMemcpyAsync(&st, status_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, stream_));
-      FPLS_HIP(hipStreamSynchronize(stream_));
-      eigen = st == 1;
-    }
-    if (eigen) {
-      product<double>(fb_.d(), m_, w, nullptr, left_.d(), w, r, U_of(slot));
-      product<double>(rev_.d(), n_, w, nullptr, left_scaled_.d(), w, r, V_of(slot));
-      return;
-    }
-    // dense route: Pᵀ = Qr R, R = Ur σ Vrᵀ  =>  P = Vr σ (Qr Ur)ᵀ
-    rcopy_.reserve(sizeof(double) * (size_t)n_ * w);
-    qr_r_.reserve(sizeof(double) * (size_t)w * w);
-    small_u_.reserve(sizeof(double) * (size_t)w * w);
-    small_s_.reserve(sizeof(double) * (size_t)w);
-    small_v_.reserve(sizeof(double) * (size_t)w * w);
-    fin_scratch_.reserve(sizeof(double) * (3 * (size_t)w * w + w + 64));
-    FPLS_HIP(hipMemcpyAsync(rcopy_.d(), rev_.d(), sizeof(double) * (size_t)n_ * w,
-                            hipMemcpyDeviceToDevice, stream_));
-    rb_.reserve(sizeof(double) * (size_t)n_ * w);
-    qr_tall(rcopy_.d(), n_, w, rb_.d(), flags);
-    tproduct<double>(rb_.d(), n_, w, nullptr, rev_.d(), n_, w, qr_r_.d());
-    launch_small_svd(qr_r_.d(), w, w, w, small_u_.d(), small_s_.d(), small_v_.d(),
-                     fin_scratch_.d(), flags, stream_);
-    product<double>(fb_.d(), m_, w, nullptr, small_v_.d(), w, r, U_of(slot));
-    product<double>(rb_.d(), n_, w, nullptr, small_u_.d(), w, r, V_of(slot));
-    FPLS_HIP(hipMemcpyAsync(s_of(slot), small_s_.d(), sizeof(double) * r,
-                            hipMemcpyDeviceToDevice, stream_));
+    auto fwd = [&](const double* Z, int w, double* out) {
+      product<T>(X_, m_, n_, nullptr, Z, n_, w, out);
+    };
+    auto adj = [&](const double* B, int w, double* out) {
+      tproduct<T>(X_, m_, n_, nullptr, B, m_, w, out);
+    };
+    operator_rsvd(fwd, adj, m_, n_, r, oversample, power, seed, false, U_of(slot), s_of(slot),
+                  V_of(slot), flags_.as<int>());
   }
 
   // audited_rsvd.hpp:185-205: residual partials and the three probes.
@@ -741,9 +625,9 @@ class AuditedSvd : private StreamOps {
 
   int m_, n_;
   const T* X_ = nullptr;
-  std::vector<double> omega_host_, probe_host_;
-  Buf xown_, flags_, rec_, omega_, sample_, fb_, rev_, rb_, gram_, left_, left_scaled_;
-  Buf fin_scratch_, status_, rcopy_, qr_r_, small_u_, small_s_, small_v_;
+  std::vector<double> probe_host_;
+  Buf xown_, flags_, rec_, rb_;
+  Buf fin_scratch_, rcopy_, qr_r_, small_u_, small_s_, small_v_;  // exact route
   Buf fwd_, rev2_, part_sums_, bad_sums_;
   Buf probe_right_, probe_raw_, probe_left_, probe_coef_, probe_proj_, probe_state_, probe_norms_;
   Buf cross_, vt_out_, wide_, wide_t_;

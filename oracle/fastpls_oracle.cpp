@@ -19,6 +19,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <numeric>
 #include <random>
@@ -410,29 +411,50 @@ Triplets finalize_operator(const Mat& basis, const Mat& projected, size_t retain
   return out;
 }
 
-// operator_rsvd.hpp:204-272 with the explicit operator A (operators.hpp:16-56).
-Triplets operator_svd(const Mat& A, int retained, int oversample, int power, unsigned int seed,
-                      bool left_only) {
-  const size_t maximum = std::min(A.r, A.c);
+// A linear operator (rows x cols) known only through its products A·Z and
+// Aᵀ·B (the operator interface of operators.hpp).
+struct LinOp {
+  size_t rows = 0, cols = 0;
+  std::function<Mat(const Mat&)> fwd, adj;
+};
+
+LinOp explicit_op(const Mat& A) {
+  LinOp op;
+  op.rows = A.r; op.cols = A.c;
+  op.fwd = [&A](const Mat& Z) { return mul(A, false, Z, false); };
+  op.adj = [&A](const Mat& B) { return mul(A, true, B, false); };
+  return op;
+}
+
+// operator_rsvd.hpp:204-272 (randomized_operator_svd) on any operator.
+Triplets operator_svd_op(const LinOp& A, int retained, int oversample, int power,
+                         unsigned int seed, bool left_only) {
+  const size_t maximum = std::min(A.rows, A.cols);
   const size_t target = std::min(maximum, size_t(std::max(retained, 1)));
   if (target == 0) return {};
   const size_t width = std::min(maximum, target + size_t(std::max(oversample, 0)));
-  Mat omega(A.c, width);
+  Mat omega(A.cols, width);
   gaussian(seed, omega.d.size(), omega.data());
-  Mat sample = mul(A, false, omega, false);
+  Mat sample = A.fwd(omega);
   Mat fb;
   for (int it = 0; it < std::max(power, 0); ++it) {
     if (!qr_economy(sample, fb)) throw std::runtime_error("oracle operator rSVD forward QR failed");
-    Mat rev = mul(A, true, fb, false);
+    Mat rev = A.adj(fb);
     Mat rb;
     if (!qr_economy(rev, rb)) throw std::runtime_error("oracle operator rSVD reverse QR failed");
-    sample = mul(A, false, rb, false);
+    sample = A.fwd(rb);
   }
   if (!qr_economy(sample, fb)) throw std::runtime_error("oracle operator rSVD sketch QR failed");
-  Mat rev = mul(A, true, fb, false);           // n x width
-  Mat projected(rev.c, rev.r);                 // width x n
+  Mat rev = A.adj(fb);                         // cols x width
+  Mat projected(rev.c, rev.r);                 // width x cols
   for (size_t j = 0; j < rev.c; ++j) for (size_t i = 0; i < rev.r; ++i) projected(j, i) = rev(i, j);
   return finalize_operator(fb, projected, target, left_only);
+}
+
+// The explicit operator A (operators.hpp:16-56).
+Triplets operator_svd(const Mat& A, int retained, int oversample, int power, unsigned int seed,
+                      bool left_only) {
+  return operator_svd_op(explicit_op(A), retained, oversample, power, seed, left_only);
 }
 
 // ----------------------------------------------------- audited rSVD
@@ -716,7 +738,9 @@ void column_stats(const Mat& X, int scaling, std::vector<double>& mx,
 }
 
 // supervised.hpp:27-124
-Prepared prepare_dense(const Mat& X, const Mat& Y, int scaling) {
+// crossprod = false skips S (the matrix-free route needs only the statistics
+// and the standardised predictors).
+Prepared prepare_dense(const Mat& X, const Mat& Y, int scaling, bool crossprod = true) {
   if (X.r != Y.r || X.r == 0 || X.c == 0 || Y.c == 0)
     throw std::invalid_argument("oracle dense cross-product dimensions are invalid");
   Prepared out;
@@ -738,6 +762,7 @@ Prepared prepare_dense(const Mat& X, const Mat& Y, int scaling) {
     }
     colsum[j] = s;
   }
+  if (!crossprod) return out;
   out.S = mul(out.Xs, true, Y, false);
   for (size_t c = 0; c < q; ++c)
     for (size_t j = 0; j < p; ++j) out.S(j, c) -= colsum[j] * out.my[c];
@@ -1105,6 +1130,9 @@ struct PlssvdModel {
   std::vector<Mat> latent;
 };
 
+PlssvdModel assemble_plssvd(const Mat& X, size_t p, size_t q, const std::vector<int>& comps,
+                            size_t retained, const Triplets& dec);
+
 // plssvd.hpp:57-166, 197-220
 PlssvdModel fit_plssvd(const Mat& X, const Mat& S, const std::vector<int>& comps,
                        int oversample, int power, unsigned int seed) {
@@ -1115,7 +1143,14 @@ PlssvdModel fit_plssvd(const Mat& X, const Mat& S, const std::vector<int>& comps
     if (c < 1 || size_t(c) > bound) throw std::invalid_argument("oracle PLS-SVD component count exceeds cross-covariance rank");
     retained = std::max(retained, size_t(c));
   }
-  Triplets dec = randomized_svd(S, int(retained), oversample, power, seed, false);
+  return assemble_plssvd(X, p, q, comps, retained,
+                         randomized_svd(S, int(retained), oversample, power, seed, false));
+}
+
+// plssvd.hpp:166-220 (assemble_plssvd_model): R, Q, T = X R and the prefix
+// weights from the decomposition of the cross-covariance.
+PlssvdModel assemble_plssvd(const Mat& X, size_t p, size_t q, const std::vector<int>& comps,
+                            size_t retained, const Triplets& dec) {
   const size_t done = std::min({dec.U.c, dec.Vt.r, dec.s.size()});
   if (done < retained) throw std::runtime_error("oracle PLS-SVD returned fewer components than requested");
   PlssvdModel m;
@@ -1142,6 +1177,226 @@ PlssvdModel fit_plssvd(const Mat& X, const Mat& S, const std::vector<int>& comps
     m.latent.push_back(lat);
   }
   return m;
+}
+
+// ------------------------------------------------- matrix-free path
+// operators.hpp:58-156 (CenteredCrosscovOperator): S = Xsᵀ (Y - 1 μᵀ) applied
+// without forming S or the centred responses.
+LinOp centered_crosscov(const Mat& Xs, const Mat& Y, const std::vector<double>& mu) {
+  LinOp op;
+  op.rows = Xs.c; op.cols = Y.c;
+  op.fwd = [&Xs, &Y, &mu](const Mat& Z) {
+    Mat inter = mul(Y, false, Z, false);
+    for (size_t c = 0; c < Z.c; ++c) {
+      double offset = 0.0;
+      for (size_t r = 0; r < Z.r; ++r) offset += mu[r] * Z(r, c);
+      for (size_t i = 0; i < inter.r; ++i) inter(i, c) -= offset;
+    }
+    return mul(Xs, true, inter, false);
+  };
+  op.adj = [&Xs, &Y, &mu](const Mat& B) {
+    Mat inter = mul(Xs, false, B, false);
+    Mat out = mul(Y, true, inter, false);
+    for (size_t c = 0; c < B.c; ++c) {
+      double sum = 0.0;
+      for (size_t i = 0; i < inter.r; ++i) sum += inter(i, c);
+      for (size_t r = 0; r < out.r; ++r) out(r, c) -= mu[r] * sum;
+    }
+    return out;
+  };
+  return op;
+}
+
+// operators.hpp:158-301 (ProjectedOperator): (I - V Vᵀ) S with the first
+// `used` columns of V; one classical Gram-Schmidt pass per product.
+void project_out(const Mat& V, size_t used, Mat& M) {
+  if (used == 0) return;
+  for (size_t c = 0; c < M.c; ++c) {
+    std::vector<double> coef(used, 0.0);
+    for (size_t j = 0; j < used; ++j)
+      for (size_t i = 0; i < M.r; ++i) coef[j] += V(i, j) * M(i, c);
+    std::vector<double> corr(M.r, 0.0);
+    for (size_t j = 0; j < used; ++j)
+      for (size_t i = 0; i < M.r; ++i) corr[i] += V(i, j) * coef[j];
+    for (size_t i = 0; i < M.r; ++i) M(i, c) -= corr[i];
+  }
+}
+
+LinOp projected_op(const LinOp& base, const Mat& V, const size_t& used) {
+  LinOp op;
+  op.rows = base.rows; op.cols = base.cols;
+  op.fwd = [&base, &V, &used](const Mat& Z) {
+    Mat out = base.fwd(Z);
+    project_out(V, used, out);
+    return out;
+  };
+  op.adj = [&base, &V, &used](const Mat& B) {
+    Mat pb = B;
+    project_out(V, used, pb);
+    return base.adj(pb);
+  };
+  return op;
+}
+
+// operator_rsvd.hpp:37-61 (normalize_column) with a long double norm.
+bool normalize_unit(Mat& v) {
+  long double s = 0.0L;
+  for (size_t i = 0; i < v.r; ++i) s += (long double)v(i, 0) * (long double)v(i, 0);
+  const double nrm = double(std::sqrt(s));
+  if (!std::isfinite(nrm) || nrm <= std::numeric_limits<double>::epsilon()) return false;
+  for (size_t i = 0; i < v.r; ++i) v(i, 0) /= nrm;
+  return true;
+}
+
+// operator_rsvd.hpp:175-202 (randomized_dominant_operator_direction) on the
+// projected operator; stabilising is two projection passes
+// (operators.hpp:262-285).
+bool dominant_direction(const LinOp& projected, const Mat& V, size_t used, int power,
+                        unsigned int seed, Mat& direction) {
+  direction.resize(projected.rows, 1);
+  std::mt19937 generator(seed);
+  std::normal_distribution<double> normal(0.0, 1.0);
+  for (size_t i = 0; i < direction.r; ++i) direction(i, 0) = normal(generator);
+  project_out(V, used, direction);
+  project_out(V, used, direction);
+  if (!normalize_unit(direction)) return false;
+  for (int it = 0; it < std::max(power, 1); ++it) {
+    Mat reverse = projected.adj(direction);
+    if (!normalize_unit(reverse)) return false;
+    Mat sample = projected.fwd(reverse);
+    project_out(V, used, sample);
+    project_out(V, used, sample);
+    if (!normalize_unit(sample)) return false;
+    direction = sample;
+  }
+  return true;
+}
+
+// simpls.hpp:768-1047 (fit_simpls_operator) with the centred cross-covariance
+// operator and its projection; rank_one selects the dominant-direction route
+// taken for a cross-covariance above 512 MiB (src/r_api.cpp:1480-1493).
+SimplsModel fit_simpls_operator(const Mat& Xs, const Mat& Y, const std::vector<double>& mu,
+                                const SimplsControls& ctl, bool rank_one) {
+  const size_t n = Xs.r, p = Xs.c, q = Y.c;
+  if (ctl.components == 0) throw std::invalid_argument("oracle implicit SIMPLS needs components");
+  const size_t maximum = std::min({ctl.components, p, std::max<size_t>(n - 1, 1)});
+  SimplsModel model;
+  model.R.resize(p, maximum); model.Q.resize(q, maximum); model.V.resize(p, maximum);
+  const bool retain_scores = ctl.store_scores || ctl.reorth;
+  if (retain_scores) model.T.resize(n, maximum);
+  Mat XtX;
+  if (ctl.cache) XtX = mul(Xs, true, Xs, false);
+  const LinOp initial = centered_crosscov(Xs, Y, mu);
+  size_t component = 0;
+  const LinOp projected = projected_op(initial, model.V, component);
+  while (component < maximum) {
+    const size_t block = std::min({std::max<size_t>(ctl.block, 1), maximum - component, std::min(p, q)});
+    const unsigned int seed = ctl.seed + unsigned(component);
+    Mat candidates;
+    if (block == 1 && rank_one) {
+      if (!dominant_direction(projected, model.V, component, ctl.power, seed, candidates)) break;
+    } else {
+      candidates = operator_svd_op(projected, int(block), ctl.oversample, ctl.power, seed, true).U;
+    }
+    const size_t available = std::min(block, candidates.c);
+    if (available == 0) break;
+    Mat cand_scores, cand_loadings;
+    if (ctl.batch_geometry && !ctl.cache) {
+      cand_scores = mul(Xs, false, candidates, false);
+      cand_loadings = mul(Xs, true, cand_scores, false);
+    }
+    bool stopped = false;
+    for (size_t cand = 0; cand < available && component < maximum; ++cand, ++component) {
+      std::vector<double> r(p), t(n, 0.0), w(p);
+      for (size_t i = 0; i < p; ++i) r[i] = candidates(i, cand);
+      if (ctl.cache) {
+        for (size_t i = 0; i < p; ++i) {
+          double s2 = 0;
+          for (size_t l = 0; l < p; ++l) s2 += XtX(i, l) * r[l];
+          w[i] = s2;
+        }
+        double ns = 0;
+        for (size_t i = 0; i < p; ++i) ns += r[i] * w[i];
+        if (!std::isfinite(ns) || ns <= 0) { stopped = true; break; }
+        const double sn = std::sqrt(ns);
+        for (double& x : r) x /= sn;
+        for (double& x : w) x /= sn;
+        if (retain_scores)
+          for (size_t l = 0; l < p; ++l) for (size_t i = 0; i < n; ++i) t[i] += Xs(i, l) * r[l];
+      } else {
+        if (ctl.batch_geometry) {
+          for (size_t i = 0; i < n; ++i) t[i] = cand_scores(i, cand);
+        } else {
+          for (size_t l = 0; l < p; ++l) for (size_t i = 0; i < n; ++i) t[i] += Xs(i, l) * r[l];
+        }
+        const bool reorthed = ctl.reorth && component > 0;
+        if (reorthed) remove_score_basis(model.T, model.R, component, t, r);
+        const double tn = norm2(t);
+        if (!std::isfinite(tn) || tn <= 0) { stopped = true; break; }
+        for (double& x : t) x /= tn;
+        for (double& x : r) x /= tn;
+        if (ctl.batch_geometry && !reorthed) {
+          for (size_t i = 0; i < p; ++i) w[i] = cand_loadings(i, cand) / tn;
+        } else {
+          for (size_t j = 0; j < p; ++j) {
+            double s2 = 0;
+            for (size_t i = 0; i < n; ++i) s2 += Xs(i, j) * t[i];
+            w[j] = s2;
+          }
+        }
+      }
+      Mat rm(p, 1);
+      for (size_t i = 0; i < p; ++i) rm(i, 0) = r[i];
+      const Mat qa = initial.adj(rm);          // response loadings Sᵀ r
+      std::vector<double> v = w;
+      remove_basis(model.V, component, v, ctl.reorth);
+      const double vn = norm2(v);
+      if (!std::isfinite(vn) || vn <= 0) { stopped = true; break; }
+      for (double& x : v) x /= vn;
+      for (size_t i = 0; i < p; ++i) { model.R(i, component) = r[i]; model.V(i, component) = v[i]; }
+      for (size_t c = 0; c < q; ++c) model.Q(c, component) = qa(c, 0);
+      if (retain_scores) for (size_t i = 0; i < n; ++i) model.T(i, component) = t[i];
+      model.completed = component + 1;
+    }
+    if (stopped) break;
+  }
+  return model;
+}
+
+// src/r_api.cpp:1466-1493: the implicit route's controls, with the rank-one
+// override for a cross-covariance above 512 MiB of `elem` bytes per entry.
+SimplsControls operator_controls(size_t n, size_t p, size_t q, size_t A, int oversample,
+                                 int power, unsigned int seed, size_t elem, bool& rank_one) {
+  SimplsControls c = simpls_controls(n, p, q, A, false, oversample, power, seed);
+  rank_one = (long double)p * (long double)q * (long double)elem > 512.0L * 1024.0L * 1024.0L;
+  if (rank_one) {
+    c.block = 1;
+    c.batch_geometry = false;
+    c.reorth = true;
+  }
+  return c;
+}
+
+// R/main.R:3454-3463 (.should_use_xprod_default): the CPU path forms no
+// cross-covariance once p·q·8 bytes exceed 32 MiB.
+bool implicit_route(size_t p, size_t q) {
+  return (double)p * (double)q * 8.0 / (1024.0 * 1024.0) > 32.0;
+}
+
+// plssvd.hpp:222-248 (fit_plssvd_operator).
+PlssvdModel fit_plssvd_operator(const Mat& Xs, const Mat& Y, const std::vector<double>& mu,
+                                const std::vector<int>& comps, int oversample, int power,
+                                unsigned int seed) {
+  const size_t p = Xs.c, q = Y.c;
+  const size_t bound = std::min(p, q);
+  size_t retained = 0;
+  for (int c : comps) {
+    if (c < 1 || size_t(c) > bound) throw std::invalid_argument("oracle implicit PLS-SVD component count exceeds cross-covariance rank");
+    retained = std::max(retained, size_t(c));
+  }
+  const LinOp crosscov = centered_crosscov(Xs, Y, mu);
+  return assemble_plssvd(Xs, p, q, comps, retained,
+                         operator_svd_op(crosscov, int(retained), oversample, power, seed, false));
 }
 
 // ------------------------------------------------------------ LDA
@@ -1460,6 +1715,83 @@ int orc_plssvd_fit(const double* x, const double* y, const int* labels, int n,
     std::copy(pr.mx.begin(), pr.mx.end(), mx);
     std::copy(pr.sx.begin(), pr.sx.end(), sx);
     std::copy(pr.my.begin(), pr.my.end(), my);
+  });
+}
+
+// Matrix-free SIMPLS (simpls.hpp:768-1047) after dense preprocessing, with
+// the implicit route's controls for entries of `elem` bytes (8 or 4).
+// Outputs as orc_simpls_fit; route[0] = 1 when the rank-one route ran.
+int orc_simpls_operator_fit(const double* x, const double* y, int n, int p, int q, int ncomp,
+                            int scaling, int oversample, int power, unsigned int seed, int elem,
+                            double* R, double* Q, double* V, double* T, double* mx, double* sx,
+                            double* my, int* completed, int* route) {
+  return guarded([&] {
+    const Mat X = wrap(x, n, p);
+    const Mat Y = wrap(y, n, q);
+    Prepared pr = prepare_dense(X, Y, scaling, false);
+    const size_t cap = std::min<size_t>(size_t(p), std::max<size_t>(size_t(n) - 1, 1));
+    const size_t A = std::max<size_t>(1, std::min<size_t>(size_t(ncomp), cap));
+    bool rank_one = false;
+    SimplsControls ctl = operator_controls(n, p, q, A, oversample, power, seed, size_t(elem),
+                                           rank_one);
+    SimplsModel m = fit_simpls_operator(pr.Xs, Y, pr.my, ctl, rank_one);
+    if (m.completed < A) throw std::runtime_error("oracle implicit SIMPLS returned fewer components than requested");
+    copy_out(m.R, R); copy_out(m.Q, Q); copy_out(m.V, V);
+    if (T) copy_out(m.T, T);
+    std::copy(pr.mx.begin(), pr.mx.end(), mx);
+    std::copy(pr.sx.begin(), pr.sx.end(), sx);
+    std::copy(pr.my.begin(), pr.my.end(), my);
+    *completed = int(m.completed);
+    if (route) route[0] = rank_one ? 1 : 0;
+  });
+}
+
+// Matrix-free PLS-SVD (plssvd.hpp:222-248) after dense preprocessing, for
+// prefixes 1..A; outputs as orc_plssvd_fit.
+int orc_plssvd_operator_fit(const double* x, const double* y, int n, int p, int q, int A,
+                            int scaling, int oversample, int power, unsigned int seed,
+                            double* R, double* Q, double* T, double* sigma, double* W,
+                            double* mx, double* sx, double* my) {
+  return guarded([&] {
+    const Mat X = wrap(x, n, p);
+    const Mat Y = wrap(y, n, q);
+    Prepared pr = prepare_dense(X, Y, scaling, false);
+    std::vector<int> comps(A);
+    std::iota(comps.begin(), comps.end(), 1);
+    PlssvdModel m = fit_plssvd_operator(pr.Xs, Y, pr.my, comps, oversample, power, seed);
+    copy_out(m.R, R); copy_out(m.Q, Q);
+    if (T) copy_out(m.T, T);
+    std::copy(m.s.begin(), m.s.end(), sigma);
+    if (W) {
+      for (int k = 1; k <= A; ++k)
+        std::memcpy(W + size_t(k - 1) * A * q, m.W[k - 1].data(), sizeof(double) * k * q);
+    }
+    std::copy(pr.mx.begin(), pr.mx.end(), mx);
+    std::copy(pr.sx.begin(), pr.sx.end(), sx);
+    std::copy(pr.my.begin(), pr.my.end(), my);
+  });
+}
+
+int orc_implicit_route(int p, int q) { return implicit_route(size_t(p), size_t(q)) ? 1 : 0; }
+
+// Matrix-free SIMPLS after dense preprocessing with explicit controls (the
+// counterpart of ref_simpls_operator_fit, for the golden cases).
+int orc_simpls_operator_core(const double* x, const double* y, int n, int p, int q, int A,
+                             int scaling, int oversample, int power, unsigned int seed,
+                             int block, int cache, int reorth, int batch_geometry, int rank_one,
+                             double* R, double* Q, double* V, double* T, int* completed) {
+  return guarded([&] {
+    const Mat X = wrap(x, n, p);
+    const Mat Y = wrap(y, n, q);
+    Prepared pr = prepare_dense(X, Y, scaling, false);
+    SimplsControls ctl;
+    ctl.components = size_t(A); ctl.block = size_t(block);
+    ctl.cache = cache != 0; ctl.reorth = reorth != 0; ctl.batch_geometry = batch_geometry != 0;
+    ctl.oversample = oversample; ctl.power = power; ctl.seed = seed;
+    SimplsModel m = fit_simpls_operator(pr.Xs, Y, pr.my, ctl, rank_one != 0);
+    copy_out(m.R, R); copy_out(m.Q, Q); copy_out(m.V, V);
+    if (T) copy_out(m.T, T);
+    *completed = int(m.completed);
   });
 }
 

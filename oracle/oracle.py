@@ -198,6 +198,58 @@ class Oracle:
                    for k in range(A)]
         return dict(R=R, Q=Q, T=T, sigma=s, W=weights, mx=mx, sx=sx, my=my)
 
+    def implicit_route(self, p, q):
+        """R/main.R:3454-3463: the CPU path is matrix-free once p·q·8 > 32 MiB."""
+        return bool(self.lib.orc_implicit_route(p, q))
+
+    def simpls_operator_fit(self, x, y, ncomp=2, scaling=2, oversample=32, power=5, seed=1,
+                            elem=8):
+        """Matrix-free SIMPLS (simpls.hpp:768-1047) with the implicit route's
+        controls for `elem`-byte entries (src/r_api.cpp:1466-1493)."""
+        x = _f64(x); yy = _f64(y)
+        n, p = x.shape; q = yy.shape[1]
+        A = max(1, min(ncomp, p, max(n - 1, 1)))
+        R = np.zeros((p, A), order="F"); Q = np.zeros((q, A), order="F")
+        V = np.zeros((p, A), order="F"); T = np.zeros((n, A), order="F")
+        mx = np.empty(p); sx = np.empty(p); my = np.empty(q)
+        done = C.c_int(0)
+        route = np.zeros(1, dtype=np.int32)
+        self._check(self.lib.orc_simpls_operator_fit(
+            _d(x), _d(yy), n, p, q, ncomp, scaling, oversample, power, C.c_uint(seed), elem,
+            _d(R), _d(Q), _d(V), _d(T), _d(mx), _d(sx), _d(my), C.byref(done), _i(route)))
+        return dict(R=R, Q=Q, V=V, T=T, mx=mx, sx=sx, my=my, completed=done.value,
+                    rank_one=int(route[0]))
+
+    def simpls_operator_core(self, x, y, ncomp=2, scaling=2, oversample=32, power=5, seed=1,
+                             block=1, cache=0, reorth=0, batch_geometry=0, rank_one=0):
+        """Matrix-free SIMPLS with explicit controls."""
+        x = _f64(x); yy = _f64(y)
+        n, p = x.shape; q = yy.shape[1]
+        A = ncomp
+        R = np.zeros((p, A), order="F"); Q = np.zeros((q, A), order="F")
+        V = np.zeros((p, A), order="F"); T = np.zeros((n, A), order="F")
+        done = C.c_int(0)
+        self._check(self.lib.orc_simpls_operator_core(
+            _d(x), _d(yy), n, p, q, A, scaling, oversample, power, C.c_uint(seed), block, cache,
+            reorth, batch_geometry, rank_one, _d(R), _d(Q), _d(V), _d(T), C.byref(done)))
+        return dict(R=R, Q=Q, V=V, T=T, completed=done.value)
+
+    def plssvd_operator_fit(self, x, y, ncomp=2, scaling=2, oversample=32, power=5, seed=1):
+        """Matrix-free PLS-SVD (plssvd.hpp:222-248)."""
+        x = _f64(x); yy = _f64(y)
+        n, p = x.shape; q = yy.shape[1]
+        A = ncomp
+        R = np.zeros((p, A), order="F"); Q = np.zeros((q, A), order="F")
+        T = np.zeros((n, A), order="F"); s = np.zeros(A)
+        W = np.zeros(A * A * q)
+        mx = np.empty(p); sx = np.empty(p); my = np.empty(q)
+        self._check(self.lib.orc_plssvd_operator_fit(
+            _d(x), _d(yy), n, p, q, A, scaling, oversample, power, C.c_uint(seed), _d(R),
+            _d(Q), _d(T), _d(s), _d(W), _d(mx), _d(sx), _d(my)))
+        weights = [W[k * A * q:k * A * q + (k + 1) * q].reshape((k + 1, q), order="F")
+                   for k in range(A)]
+        return dict(R=R, Q=Q, T=T, sigma=s, W=weights, mx=mx, sx=sx, my=my)
+
     def plssvd_core(self, xs, S, comps, oversample=32, power=5, seed=1):
         xs = _f64(xs); S = _f64(S)
         n, p = xs.shape; q = S.shape[1]
@@ -328,6 +380,36 @@ class Reference:
         self._check(self.lib.ref_plssvd_fit(_d(x), _d(yy), _i(lab), n, p, q, A, scaling,
                                             oversample, power, C.c_uint(seed), _d(R), _d(Q),
                                             _d(T), _d(s), _d(W), _d(mx), _d(sx), _d(my)))
+        weights = [W[k * A * q:k * A * q + (k + 1) * q].reshape((k + 1, q), order="F")
+                   for k in range(A)]
+        return dict(R=R, Q=Q, T=T, sigma=s, W=weights, mx=mx, sx=sx, my=my)
+
+    def simpls_operator_fit(self, x, y, ncomp=2, scaling=2, oversample=32, power=5, seed=1,
+                            block=1, cache=0, reorth=0, batch_geometry=0, rank_one=0):
+        x = _f64(x); yy = _f64(y)
+        n, p = x.shape; q = yy.shape[1]
+        A = ncomp
+        R = np.zeros((p, A), order="F"); Q = np.zeros((q, A), order="F")
+        V = np.zeros((p, A), order="F"); T = np.zeros((n, A), order="F")
+        mx = np.empty(p); sx = np.empty(p); my = np.empty(q)
+        done = C.c_int(0)
+        self._check(self.lib.ref_simpls_operator_fit(
+            _d(x), _d(yy), n, p, q, A, scaling, oversample, power, C.c_uint(seed), block, cache,
+            reorth, batch_geometry, rank_one, _d(R), _d(Q), _d(V), _d(T), _d(mx), _d(sx),
+            _d(my), C.byref(done)))
+        return dict(R=R, Q=Q, V=V, T=T, mx=mx, sx=sx, my=my, completed=done.value)
+
+    def plssvd_operator_fit(self, x, y, ncomp=2, scaling=2, oversample=32, power=5, seed=1):
+        x = _f64(x); yy = _f64(y)
+        n, p = x.shape; q = yy.shape[1]
+        A = ncomp
+        R = np.zeros((p, A), order="F"); Q = np.zeros((q, A), order="F")
+        T = np.zeros((n, A), order="F"); s = np.zeros(A)
+        W = np.zeros(A * A * q)
+        mx = np.empty(p); sx = np.empty(p); my = np.empty(q)
+        self._check(self.lib.ref_plssvd_operator_fit(
+            _d(x), _d(yy), n, p, q, A, scaling, oversample, power, C.c_uint(seed), _d(R), _d(Q),
+            _d(T), _d(s), _d(W), _d(mx), _d(sx), _d(my)))
         weights = [W[k * A * q:k * A * q + (k + 1) * q].reshape((k + 1, q), order="F")
                    for k in range(A)]
         return dict(R=R, Q=Q, T=T, sigma=s, W=weights, mx=mx, sx=sx, my=my)

@@ -156,6 +156,89 @@ int ref_plssvd_fit(const double* x, const double* y, const int* labels, int n,
   });
 }
 
+// Dense preprocessing, then the matrix-free SIMPLS of simpls.hpp:768-1047
+// with the reference's centred cross-covariance and projected operators
+// (operators.hpp:58-301) and the given controls. Outputs as ref_simpls_fit.
+int ref_simpls_operator_fit(const double* x, const double* y, int n, int p, int q, int A,
+                            int scaling, int oversample, int power, unsigned int seed,
+                            int block, int cache, int reorth, int batch_geometry, int rank_one,
+                            double* R, double* Q, double* V, double* T, double* mx, double* sx,
+                            double* my, int* completed) {
+  return guarded([&] {
+    ReferenceBackend<double> backend;
+    Matrix<double> X = load(x, n, p);
+    Matrix<double> Y = load(y, n, q);
+    auto pr = fastpls::core::prepare_scaled_dense_crossprod(
+      X.view(), ConstMatrixView<double>(Y.view()),
+      static_cast<PredictorScaling>(scaling), backend);
+    fastpls::core::SimplsControls controls;
+    controls.components = std::size_t(A);
+    controls.maximum_block = std::size_t(block);
+    controls.cache_predictor_crossprod = cache != 0;
+    controls.batch_candidate_geometry = batch_geometry != 0;
+    controls.reorthogonalize = reorth != 0;
+    controls.rank_one_operator_direction = rank_one != 0;
+    controls.store_scores = true;
+    controls.rsvd.oversample = oversample;
+    controls.rsvd.power = power;
+    controls.rsvd.seed = seed;
+    using Crosscov = fastpls::core::CenteredCrosscovOperator<double, ReferenceBackend<double>>;
+    Crosscov initial(ConstMatrixView<double>(X.view()), ConstMatrixView<double>(Y.view()),
+                     pr.response_mean.data(), pr.response_mean.size(), backend);
+    fastpls::core::ProjectedOperator<double, Crosscov, ReferenceBackend<double>> projected(
+      initial, controls.components, backend);
+    fastpls::core::SimplsWorkspace<double> workspace;
+    fastpls::core::OperatorRsvdWorkspace<double> rsvd_workspace;
+    auto model = fastpls::core::fit_simpls_operator<double>(
+      ConstMatrixView<double>(X.view()), initial, projected, controls, backend, workspace,
+      rsvd_workspace);
+    store(model.weights, R); store(model.response_loadings, Q);
+    store(model.deflation_basis, V); store(model.scores, T);
+    std::copy(pr.predictor_center.begin(), pr.predictor_center.end(), mx);
+    std::copy(pr.predictor_scale.begin(), pr.predictor_scale.end(), sx);
+    std::copy(pr.response_mean.begin(), pr.response_mean.end(), my);
+    *completed = int(model.completed_components);
+  });
+}
+
+// Dense preprocessing, then the matrix-free PLS-SVD of plssvd.hpp:222-248;
+// outputs as ref_plssvd_fit.
+int ref_plssvd_operator_fit(const double* x, const double* y, int n, int p, int q, int A,
+                            int scaling, int oversample, int power, unsigned int seed,
+                            double* R, double* Q, double* T, double* sigma, double* W,
+                            double* mx, double* sx, double* my) {
+  return guarded([&] {
+    ReferenceBackend<double> backend;
+    Matrix<double> X = load(x, n, p);
+    Matrix<double> Y = load(y, n, q);
+    auto pr = fastpls::core::prepare_scaled_dense_crossprod(
+      X.view(), ConstMatrixView<double>(Y.view()),
+      static_cast<PredictorScaling>(scaling), backend);
+    using Crosscov = fastpls::core::CenteredCrosscovOperator<double, ReferenceBackend<double>>;
+    Crosscov crosscov(ConstMatrixView<double>(X.view()), ConstMatrixView<double>(Y.view()),
+                      pr.response_mean.data(), pr.response_mean.size(), backend);
+    std::vector<int> comps(A);
+    for (int k = 0; k < A; ++k) comps[k] = k + 1;
+    fastpls::core::PlssvdControls controls;
+    controls.rsvd.oversample = oversample;
+    controls.rsvd.power = power;
+    controls.rsvd.seed = seed;
+    fastpls::core::OperatorRsvdWorkspace<double> workspace;
+    auto model = fastpls::core::fit_plssvd_operator<double>(
+      ConstMatrixView<double>(X.view()), crosscov, comps.data(), comps.size(), controls,
+      backend, workspace);
+    store(model.weights, R); store(model.response_loadings, Q);
+    store(model.scores, T);
+    std::copy(model.singular_values.begin(), model.singular_values.end(), sigma);
+    for (int k = 1; k <= A; ++k)
+      std::memcpy(W + std::size_t(k - 1) * A * q, model.prediction_weights[k - 1].data(),
+                  sizeof(double) * k * q);
+    std::copy(pr.predictor_center.begin(), pr.predictor_center.end(), mx);
+    std::copy(pr.predictor_scale.begin(), pr.predictor_scale.end(), sx);
+    std::copy(pr.response_mean.begin(), pr.response_mean.end(), my);
+  });
+}
+
 // Explicit randomized SVD of a dense matrix (rsvd.hpp:185-237).
 int ref_rsvd(const double* a, int m, int n, int k, int oversample, int power,
              unsigned int seed, double* U, double* s, double* Vt, int* got) {

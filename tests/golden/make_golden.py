@@ -96,13 +96,41 @@ def audited_cases(ref):
         save(name, **arrays)
 
 
+def operator_cases(ref):
+    """Matrix-free SIMPLS and PLS-SVD (simpls.hpp:768-1047, plssvd.hpp:222-248)
+    with the reference's centred cross-covariance and projected operators:
+    the block route (block 3, batched geometry), the rank-one direction route
+    of a cross-covariance above 512 MiB (block 1, re-orthogonalisation), and
+    PLS-SVD."""
+    X, Y = latent_data(200, 60, 12, factors=8, noise=0.1, seed=3)
+    for name, kw in (("simpls_operator_block", dict(block=3, batch_geometry=1, rank_one=0)),
+                     ("simpls_operator_rank1", dict(block=1, reorth=1, rank_one=1))):
+        m = ref.simpls_operator_fit(X, Y, ncomp=6, oversample=4, power=3, seed=5, **kw)
+        save(name, X=X, Y=Y, R=m["R"], Q=m["Q"], V=m["V"], T=m["T"], mx=m["mx"], sx=m["sx"],
+             my=m["my"], completed=np.int64(m["completed"]), ncomp=np.int64(6),
+             oversample=np.int64(4), power=np.int64(3), seed=np.int64(5),
+             **{k: np.int64(v) for k, v in kw.items()})
+    X, Y = latent_data(150, 40, 25, factors=6, noise=0.1, seed=4)
+    m = ref.plssvd_operator_fit(X, Y, ncomp=4, oversample=3, power=2, seed=7)
+    arrays = dict(X=X, Y=Y, R=m["R"], Q=m["Q"], T=m["T"], sigma=m["sigma"], mx=m["mx"],
+                  sx=m["sx"], my=m["my"], ncomp=np.int64(4), oversample=np.int64(3),
+                  power=np.int64(2), seed=np.int64(7))
+    for k, w in enumerate(m["W"]):
+        arrays[f"W{k + 1}"] = w
+    save("plssvd_operator", **arrays)
+
+
 def main():
     build(reference=True)
     ref = Reference()
     if len(sys.argv) > 1 and sys.argv[1] == "audited":
         audited_cases(ref)
         return
+    if len(sys.argv) > 1 and sys.argv[1] == "operator":
+        operator_cases(ref)
+        return
     audited_cases(ref)
+    operator_cases(ref)
 
     # SIMPLS, dense responses, explicit width = q (direction independent of the sketch)
     X, Y = latent_data(64, 20, 3, factors=5, noise=0.1, seed=11)

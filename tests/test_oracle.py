@@ -360,3 +360,63 @@ def test_crossprod_cache_rule_cases(oracle):
     assert (c["block"], c["cache"], c["batch_geometry"]) == (15, 0, 1)
     c = oracle.controls(2000, 1000, 3, 10)
     assert (c["block"], c["cache"], c["batch_geometry"]) == (1, 0, 0)
+
+
+# ------------------------------------------------------ matrix-free path
+@pytest.mark.parametrize("name", ["simpls_operator_block", "simpls_operator_rank1"])
+def test_simpls_operator_matches_reference_core(oracle, name):
+    """Matrix-free SIMPLS (simpls.hpp:768-1047) against the compiled reference
+    core with its centred cross-covariance / projected operators: the block
+    route (operator rSVD of the projected operator) and the rank-one
+    dominant-direction route (operator_rsvd.hpp:175-202)."""
+    g = load_golden(name)
+    kw = {k: int(g[k]) for k in ("block", "rank_one") if k in g}
+    kw.update({k: int(g[k]) for k in ("batch_geometry", "reorth") if k in g})
+    m = oracle.simpls_operator_core(g["X"], g["Y"], ncomp=int(g["ncomp"]),
+                                    oversample=int(g["oversample"]), power=int(g["power"]),
+                                    seed=int(g["seed"]), **kw)
+    assert m["completed"] == int(g["completed"])
+    for k in (1, 3, 6):
+        assert rel(m["R"][:, :k] @ m["Q"][:, :k].T, g["R"][:, :k] @ g["Q"][:, :k].T) < 1e-10, k
+    assert rel(np.abs(m["T"]), np.abs(g["T"])) < 1e-10
+    assert rel(np.abs(m["V"]), np.abs(g["V"])) < 1e-10
+
+
+def test_plssvd_operator_matches_reference_core(oracle):
+    g = load_golden("plssvd_operator")
+    m = oracle.plssvd_operator_fit(g["X"], g["Y"], ncomp=int(g["ncomp"]),
+                                   oversample=int(g["oversample"]), power=int(g["power"]),
+                                   seed=int(g["seed"]))
+    np.testing.assert_allclose(m["sigma"], g["sigma"], rtol=1e-12)
+    for k in range(1, int(g["ncomp"]) + 1):
+        assert rel(m["R"][:, :k] @ m["W"][k - 1], g["R"][:, :k] @ g[f"W{k}"]) < 1e-10, k
+
+
+def test_operator_route_equals_explicit_route(oracle):
+    """Below the rank-one threshold the matrix-free route computes the same
+    subspaces as the explicit one (the operator rSVD of (I - VVᵀ)S with the
+    same sketch spans (SSᵀ)^power S Ω as the right-Gram refresh), so
+    coefficients agree to rounding; the rank-one route does not (its power
+    iteration is not converged at small power) and converges to it as the
+    power grows. The 32 MiB routing rule (R/main.R:3454-3463)."""
+    X, Y = latent_data(300, 80, 15, factors=8, noise=0.1, seed=8)
+    for ncomp, os_, pw in ((6, 4, 2), (8, 10, 1)):
+        e = oracle.simpls_fit(X, Y, ncomp=ncomp, oversample=os_, power=pw, seed=3)
+        o = oracle.simpls_operator_fit(X, Y, ncomp=ncomp, oversample=os_, power=pw, seed=3)
+        assert o["rank_one"] == 0
+        assert rel(o["R"] @ o["Q"].T, e["R"] @ e["Q"].T) < 1e-10
+        pe = oracle.plssvd_fit(X, Y, ncomp=5, oversample=os_, power=pw, seed=3)
+        po = oracle.plssvd_operator_fit(X, Y, ncomp=5, oversample=os_, power=pw, seed=3)
+        np.testing.assert_allclose(po["sigma"], pe["sigma"], rtol=1e-10)
+        assert rel(po["R"] @ po["W"][4], pe["R"] @ pe["W"][4]) < 1e-10
+    # a decaying spectrum, so that the power iteration settles within a few steps
+    X, Y = latent_data(300, 80, 15, factors=8, noise=0.1, seed=8, decay=0.5)
+    errs = []
+    for pw in (1, 4, 12):
+        o = oracle.simpls_operator_fit(X, Y, ncomp=4, oversample=4, power=pw, seed=3,
+                                       elem=1 << 30)   # forces the rank-one route
+        e = oracle.simpls_fit(X, Y, ncomp=4, oversample=4, power=pw, seed=3)
+        assert o["rank_one"] == 1
+        errs.append(rel(o["R"] @ o["Q"].T, e["R"] @ e["Q"].T))
+    assert errs[0] > errs[1] > errs[2] and errs[2] < 1e-3, errs
+    assert not oracle.implicit_route(20000, 209) and oracle.implicit_route(20000, 210)
