@@ -66,6 +66,10 @@ void launch_qr(double* A, int m, int n, double* Q, int* flags, hipStream_t s);
 // when the factorisation is unusable; the fallback recomputes Q by
 // Householder from A only when *fail is set.
 void launch_chol_inv(const double* G, int w, double* Rinv, int* fail, hipStream_t s);
+// Whole CholQR2 (Householder when a Cholesky factor is unusable) of a small
+// A (m x w, m >= w, w <= 32, m·w <= 4096) in one work-group; A is not modified.
+bool small_qr_fits(int m, int w);
+void launch_cholqr2_small(const double* A, int m, int w, double* Q, int* flags, hipStream_t s);
 void launch_qr_fallback(double* A, int m, int n, double* Q, const int* fail, int* flags,
                         hipStream_t s);
 // Finalize a sketch: P (w x q) = Qbᵀ S is given; writes Up (w x k), sigma (k),

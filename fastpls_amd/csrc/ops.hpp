@@ -79,8 +79,14 @@ class StreamOps {
   // Tall-skinny QR: CholQR2 in FP64 (two Gram / Cholesky / triangular-apply
   // rounds through the multi-work-group products), with a device-side
   // Householder fallback when a Cholesky factor is unusable. A is destroyed
-  // only by the fallback; Q receives the orthonormal basis.
+  // only by the fallback; Q receives the orthonormal basis. Small matrices
+  // (w <= 32, m·w <= 4096) run the whole CholQR2 in one work-group; other
+  // shapes with m < 4w or w > 64 take the Householder QR directly.
   void qr_tall(double* A, int m, int w, double* Q, int* flags) {
+    if (small_qr_fits(m, w)) {
+      launch_cholqr2_small(A, m, w, Q, flags, stream_);
+      return;
+    }
     if (w > 64 || m < 4 * w) {
       launch_qr(A, m, w, Q, flags, stream_);
       return;

@@ -237,7 +237,22 @@ __global__ __launch_bounds__(NT) void k_qr(double* A, int m, int n, double* Q, i
 // factor, so that Q = A Rinv with R = Lᵀ. A pivot that is non-finite or
 // below 1e-13 of the largest diagonal entry marks the factorisation as
 // failed (*fail = 1) and leaves Rinv as the identity; the Householder
-// fallback then recomputes Q.
+// fallback then recomputes Q. A factorisation already marked failed is
+// skipped. Up to 32 columns one wave factors the Gram in registers
+// (wave_chol_inv); wider Grams take the work-group LDS form below.
+__global__ __launch_bounds__(64) void k_chol_inv_wave(const double* G, int w, double* Rinv,
+                                                      int* fail) {
+  constexpr int W = 32;
+  __shared__ double lc[W * W], xb[W];
+  const bool failed_before = *fail != 0;
+  bool ok = false;
+  if (!failed_before) ok = wg::wave_chol_inv<W>(G, w, w, Rinv, w, lc, xb);
+  if (!ok) {
+    if (threadIdx.x == 0) *fail = 1;
+    for (int e = threadIdx.x; e < w * w; e += 64) Rinv[e] = (e % w) == (e / w) ? 1.0 : 0.0;
+  }
+}
+
 __global__ __launch_bounds__(kWG) void k_chol_inv(const double* G, int w, double* Rinv,
                                                   int* fail) {
   __shared__ double Lm[64 * 65];
@@ -302,6 +317,74 @@ __global__ __launch_bounds__(kWG) void k_chol_inv(const double* G, int w, double
     const int i = e % w, c = e / w;
     Rinv[e] = i <= c ? Ri[i + c * 65] / Lm[i + i * 65] : 0.0;
   }
+}
+
+// Whole CholQR2 of a small A (m x w, m >= w, m·w <= kSmallQr, w <= 32) in one
+// work-group: A and the first-pass basis stay in LDS, wave 0 factors each
+// Gram. When a Cholesky factor is unusable the Householder QR of the original
+// A (kept in LDS) gives Q instead, as in the multi-kernel route.
+constexpr int kSmallQr = 4096;
+
+__device__ inline void lds_gram(const double* X, int m, int w, double* G, int ldg) {
+  for (int e = threadIdx.x; e < w * w; e += kWG) {
+    const int i = e % w, j = e / w;
+    const double* xi = X + (long)i * m;
+    const double* xj = X + (long)j * m;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int l = 0;
+    for (; l + 4 <= m; l += 4) {
+      s0 = fma(xi[l], xj[l], s0);
+      s1 = fma(xi[l + 1], xj[l + 1], s1);
+      s2 = fma(xi[l + 2], xj[l + 2], s2);
+      s3 = fma(xi[l + 3], xj[l + 3], s3);
+    }
+    for (; l < m; ++l) s0 = fma(xi[l], xj[l], s0);
+    G[i + j * ldg] = (s0 + s1) + (s2 + s3);
+  }
+}
+
+// Y (m x w, ld m) = X Rinv for an upper-triangular Rinv (ld ldr).
+__device__ inline void lds_apply_upper(const double* X, int m, int w, const double* Rinv,
+                                       int ldr, double* Y) {
+  for (int e = threadIdx.x; e < m * w; e += kWG) {
+    const int l = e % m, c = e / m;
+    double acc = 0.0;
+    for (int k = 0; k <= c; ++k) acc = fma(X[l + (long)k * m], Rinv[k + c * ldr], acc);
+    Y[e] = acc;
+  }
+}
+
+__global__ __launch_bounds__(kWG) void k_cholqr2_small(const double* A, int m, int w, double* Q,
+                                                       int* flags) {
+  constexpr int W = 32;
+  __shared__ double la[kSmallQr], lq[kSmallQr];
+  __shared__ double gm[W * W], ri[W * W], lc[W * W], xb[W];
+  __shared__ WgScratch sh;
+  __shared__ int ok_sh;
+  // Gram of X in gm, then wave 0 factors it into ri; true when usable
+  auto factor = [&](const double* X) __attribute__((always_inline)) -> bool {
+    lds_gram(X, m, w, gm, W);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      const bool good = wg::wave_chol_inv<W>(gm, W, w, ri, W, lc, xb);
+      if (threadIdx.x == 0) ok_sh = good ? 1 : 0;
+    }
+    __syncthreads();
+    return ok_sh != 0;
+  };
+  for (int e = threadIdx.x; e < m * w; e += kWG) la[e] = A[e];
+  __syncthreads();
+  if (factor(la)) {
+    lds_apply_upper(la, m, w, ri, W, lq);
+    __syncthreads();
+    if (factor(lq)) {
+      lds_apply_upper(lq, m, w, ri, W, Q);
+      return;
+    }
+  }
+  const bool fine = wg::householder_qr<kWG>(la, m, m, w, lq, m, sh.tau, sh.dots, sh.red, sh.part);
+  for (int e = threadIdx.x; e < m * w; e += kWG) Q[e] = lq[e];
+  if (!fine && threadIdx.x == 0) atomicOr(flags, kFlagDecomposition);
 }
 
 // Householder recomputation of Q when CholQR2 failed; a no-op otherwise.
@@ -749,9 +832,18 @@ void launch_qr(double* A, int m, int n, double* Q, int* flags, hipStream_t s) {
 }
 
 void launch_chol_inv(const double* G, int w, double* Rinv, int* fail, hipStream_t s) {
-  if (w > 64) throw std::invalid_argument("CholQR2 handles at most 64 columns");
-  hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(kWG), 0, s, G, w, Rinv, fail);
+  if (w < 1 || w > 64) throw std::invalid_argument("CholQR2 handles 1 to 64 columns");
+  if (w <= 32) hipLaunchKernelGGL(k_chol_inv_wave, dim3(1), dim3(64), 0, s, G, w, Rinv, fail);
+  else hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(kWG), 0, s, G, w, Rinv, fail);
   FPLS_LAUNCHED("k_chol_inv");
+}
+
+bool small_qr_fits(int m, int w) { return w >= 1 && w <= 32 && m >= w && (long long)m * w <= kSmallQr; }
+
+void launch_cholqr2_small(const double* A, int m, int w, double* Q, int* flags, hipStream_t s) {
+  if (!small_qr_fits(m, w)) throw std::invalid_argument("small CholQR2 shape outside the supported range");
+  hipLaunchKernelGGL(k_cholqr2_small, dim3(1), dim3(kWG), 0, s, A, m, w, Q, flags);
+  FPLS_LAUNCHED("k_cholqr2_small");
 }
 
 void launch_qr_fallback(double* A, int m, int n, double* Q, const int* fail, int* flags,

@@ -165,6 +165,88 @@ __device__ inline bool householder_qr(double* A, long lda, int m, int n, double*
   return finite;
 }
 
+// ------------------------------------------------------ one-wave routines
+// Value of v in lane l (l uniform).
+__device__ inline double readlane_d(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(b & 0xffffffffLL), l));
+  const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(b >> 32), l));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Orders the LDS accesses of one wave: stores issued before it are seen by
+// the loads of every lane after it. A wave's LDS operations execute in issue
+// order, so only compiler reordering has to be prevented.
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Cholesky factorisation of the symmetric w x w matrix G (w <= W = 32) by
+// ONE wave, and the inverse of its upper factor: G = Rᵀ R, Rinv = R⁻¹ (upper
+// triangular, leading dimension ldr), so that A R⁻¹ has orthonormal columns
+// when G = AᵀA. The factorisation is G = L D Lᵀ (unit L, right-looking on
+// unscaled columns), R = D^½ Lᵀ and R⁻¹ = L⁻ᵀ D^-½. Lane i keeps row i of the
+// trailing matrix and row i of L⁻¹ in registers; step j eliminates column j
+// from both (Gauss-Jordan on [G | I]), broadcasting column j of G and row j
+// of L⁻¹ through LDS. A pivot that is non-finite or not above 1e-13 of the
+// largest diagonal entry of G makes it return false (uniformly) without
+// writing Rinv. lc is W·W doubles of LDS, xb W doubles; G may be in LDS or
+// global memory and is read before anything is written.
+template<int W>
+__device__ __forceinline__ bool wave_chol_inv(const double* G, long ldg, int w, double* Rinv,
+                                              long ldr, double* lc, double* xb) {
+  static_assert(W == 32, "register rows of 32 entries");
+  const int lane = threadIdx.x & 63;
+  double a[W], b[W];
+#pragma unroll
+  for (int c = 0; c < W; ++c) {
+    a[c] = (lane < w && c < w) ? G[lane + c * ldg] : 0.0;
+    b[c] = c == lane ? 1.0 : 0.0;
+  }
+  double dg = lane < w ? G[lane + (long)lane * ldg] : 0.0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) dg = fmax(dg, __shfl_xor(dg, off, 64));
+  const double floor = 1e-13 * dg;
+  wave_lds_sync();
+  // dn: this lane's diagonal entry after the previous step, formed from its
+  // own registers (the same operation the trailing update applies), so the
+  // next pivot does not wait for the LDS exchange.
+  double dn = a[0], mine = 1.0;
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    if (j >= w || !ok) continue;
+    const double d = readlane_d(dn, j);
+    if (!isfinite(d) || !(d > floor)) {
+      ok = false;
+      continue;
+    }
+    if (lane == j) mine = d;
+    const double l = a[j] * (1.0 / d);
+    const double lb = lane > j ? l : 0.0;  // rows below the pivot only
+    if (lane < W) lc[j * W + lane] = a[j];
+    if (lane == j) {
+#pragma unroll
+      for (int c = 0; c <= j; ++c) xb[c] = b[c];
+    }
+    if (j + 1 < W) dn = fma(-l, a[j], a[j + 1]);
+    wave_lds_sync();
+#pragma unroll
+    for (int c = j + 1; c < W; ++c) a[c] = fma(-l, lc[j * W + c], a[c]);
+#pragma unroll
+    for (int c = 0; c <= j; ++c) b[c] = fma(-lb, xb[c], b[c]);
+  }
+  if (!ok) return false;
+  // lane i holds row i of L⁻¹: column i of R⁻¹ is that row times d_i^-½
+  const double rs = 1.0 / sqrt(mine);
+#pragma unroll
+  for (int r = 0; r < W; ++r)
+    if (r < w && lane < w) Rinv[r + lane * ldr] = b[r] * rs;
+  return true;
+}
+
 // Pair (a, b) of round r for k-th slot of the round-robin (circle) schedule
 // over m (even) players.
 template<int NT = kWG>
@@ -175,10 +257,67 @@ __device__ inline void rr_pair(int m, int r, int k, int& a, int& b) {
   if (a > b) { const int t = a; a = b; b = t; }
 }
 
+// 1/x and 1/sqrt(x) for positive normal x: the hardware estimate refined by
+// two Newton steps (a few ulp, shorter dependency chain than IEEE division
+// and square root).
+__device__ __forceinline__ double rcp_refined(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+__device__ __forceinline__ double rsq_refined(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  double e = fma(-(h * y), y, 0.5);
+  y = fma(y, e, y);
+  e = fma(-(h * y), y, 0.5);
+  return fma(y, e, y);
+}
+
+// Rotation (c, s) that annihilates a_pq of the symmetric 2 x 2 block
+// [[a_pp, a_pq], [a_pq, a_qq]]: t = sign(θ) / (|θ| + sqrt(1 + θ²)) with
+// θ = (a_qq − a_pp) / (2 a_pq), written as sign(θ)|e| / (|d| + sqrt(d² + e²))
+// for d = a_qq − a_pp, e = 2 a_pq scaled by a power of two (no overflow or
+// underflow); c = 1 / sqrt(1 + t²), s = t c. Returns c = 1, s = 0 when a_pq is
+// zero.
+__device__ __forceinline__ void jacobi_rotation(double app, double aqq, double apq, double& c,
+                                                double& s) {
+  c = 1.0;
+  s = 0.0;
+  if (apq == 0.0) return;
+  double d = aqq - app, e = 2.0 * apq;
+  const int k = ilogb(fmax(fabs(d), fabs(e)));
+  d = ldexp(d, -k);
+  e = ldexp(e, -k);
+  const double h = fma(d, d, e * e);
+  const double root = h * rsq_refined(h);
+  const double sg = (d == 0.0 || (d > 0.0) == (e > 0.0)) ? 1.0 : -1.0;
+  const double t = sg * fabs(e) * rcp_refined(fabs(d) + root);
+  c = rsq_refined(fma(t, t, 1.0));
+  s = t * c;
+}
+
+// Player at position pos of round r of the round-robin (circle) schedule
+// over m (even) players: position 0 is fixed, the others rotate.
+__device__ __forceinline__ int rr_at(int pos, int r, int m) {
+  if (pos == 0) return 0;
+  int x = pos - 1 + r;
+  if (x >= m - 1) x -= m - 1;
+  return x + 1;
+}
+
 // Parallel cyclic Jacobi eigen-decomposition of the symmetric n x n matrix A
 // (leading dimension n). On return V holds the eigenvectors as columns and
 // w the eigenvalues, ascending (the dsyevd contract). cs/sn/pa/pb are LDS
 // scratch of n/2+1 entries each. Returns false on non-finite input.
+// Each round of the round-robin schedule rotates n/2 disjoint planes: one
+// thread per plane forms its rotation, then one pass applies Jᵀ A J block by
+// block (2 x 2 blocks of the row and column planes, the upper ones mirrored,
+// so A stays exactly symmetric) together with V J. Up to n = 64 (with 256
+// threads) every thread keeps its block and V assignments in registers.
 template<int NT = kWG>
 __device__ inline bool jacobi_eigen(double* A, double* V, int n, double* w, double* cs,
                                     double* sn, int* pa, int* pb, double* red,
@@ -190,6 +329,56 @@ __device__ inline bool jacobi_eigen(double* A, double* V, int n, double* w, doub
   if (wg_sum_n<NT>(fin, red) != 0.0) return false;
   const int m = n + (n & 1);
   const int half = m / 2;
+  // per-thread assignments: blocks (k <= l) and V row-plane items (l, i)
+  constexpr int kMaxB = (32 * 33 / 2 + NT - 1) / NT, kMaxV = (32 * 64 + NT - 1) / NT;
+  const bool cached = n <= 64;
+  int bk[kMaxB], bl[kMaxB], vl[kMaxV], vi[kMaxV];
+  int nbk = 0, nvi = 0;
+  if (cached) {
+    int k = 0, rem = tid;
+#pragma unroll
+    for (int u = 0; u < kMaxB; ++u) {
+      while (k < half && rem >= half - k) { rem -= half - k; ++k; }
+      bk[u] = k < half ? k : -1;
+      bl[u] = k + rem;
+      if (k < half) nbk = u + 1;
+      rem += NT;
+    }
+#pragma unroll
+    for (int u = 0; u < kMaxV; ++u) {
+      const int e = tid + u * NT;
+      vl[u] = e / (n > 0 ? n : 1);
+      vi[u] = e % (n > 0 ? n : 1);
+      if (e < half * n) nvi = u + 1;
+    }
+  }
+  auto block = [&](int k, int l) {
+    const int ak = pa[k], bkk = pb[k], al = pa[l], bll = pb[l];
+    const double ck = cs[k], sk = sn[k], cl = cs[l], sl = sn[l];
+    const double x00 = A[ak + al * n];
+    const double x01 = bll >= 0 ? A[ak + bll * n] : 0.0;
+    const double x10 = bkk >= 0 ? A[bkk + al * n] : 0.0;
+    const double x11 = (bkk >= 0 && bll >= 0) ? A[bkk + bll * n] : 0.0;
+    // columns: (x0, x1) -> (c x0 - s x1, s x0 + c x1), then rows
+    const double y00 = cl * x00 - sl * x01, y01 = sl * x00 + cl * x01;
+    const double y10 = cl * x10 - sl * x11, y11 = sl * x10 + cl * x11;
+    const double z00 = ck * y00 - sk * y10, z10 = sk * y00 + ck * y10;
+    const double z01 = ck * y01 - sk * y11, z11 = sk * y01 + ck * y11;
+    A[ak + al * n] = z00;
+    A[al + ak * n] = z00;
+    if (bll >= 0) { A[ak + bll * n] = z01; A[bll + ak * n] = z01; }
+    if (bkk >= 0) { A[bkk + al * n] = z10; A[al + bkk * n] = z10; }
+    if (bkk >= 0 && bll >= 0) { A[bkk + bll * n] = z11; A[bll + bkk * n] = z11; }
+  };
+  auto vrot = [&](int l, int i) {
+    const int a = pa[l], b = pb[l];
+    if (b < 0) return;
+    const double c = cs[l], s = sn[l];
+    if (s == 0.0) return;
+    const double va = V[i + a * n], vb = V[i + b * n];
+    V[i + a * n] = c * va - s * vb;
+    V[i + b * n] = s * va + c * vb;
+  };
   for (int sweep = 0; sweep < 40 && n > 1; ++sweep) {
     double off = 0.0, dg = 0.0;
     for (int e = tid; e < n * n; e += NT) {
@@ -202,46 +391,27 @@ __device__ inline bool jacobi_eigen(double* A, double* V, int n, double* w, doub
     if (off == 0.0 || off <= 1e-34 * dg) break;
     for (int r = 0; r < m - 1; ++r) {
       if (tid < half) {
-        int a, b;
-        rr_pair(m, r, tid, a, b);
+        int a = rr_at(tid, r, m), b = rr_at(m - 1 - tid, r, m);
+        if (a > b) { const int t = a; a = b; b = t; }
         double c = 1.0, s = 0.0;
-        if (b < n) {
-          const double apq = A[a + b * n];
-          if (apq != 0.0) {
-            const double theta = (A[b + b * n] - A[a + a * n]) / (2.0 * apq);
-            const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(1.0 + theta * theta));
-            c = 1.0 / sqrt(1.0 + t * t);
-            s = t * c;
-          }
-        } else {
-          b = -1;
-        }
+        if (b < n) jacobi_rotation(A[a + a * n], A[b + b * n], A[a + b * n], c, s);
+        else b = -1;
         cs[tid] = c; sn[tid] = s; pa[tid] = a; pb[tid] = b;
       }
       __syncthreads();
-      // columns of A and V
-      for (int e = tid; e < half * n; e += NT) {
-        const int k = int(e / n), i = int(e % n);
-        const int a = pa[k], b = pb[k];
-        if (b < 0 || sn[k] == 0.0) continue;
-        const double c = cs[k], s = sn[k];
-        const double xa = A[i + a * n], xb = A[i + b * n];
-        A[i + a * n] = c * xa - s * xb;
-        A[i + b * n] = s * xa + c * xb;
-        const double va = V[i + a * n], vb = V[i + b * n];
-        V[i + a * n] = c * va - s * vb;
-        V[i + b * n] = s * va + c * vb;
-      }
-      __syncthreads();
-      // rows of A
-      for (int e = tid; e < half * n; e += NT) {
-        const int k = int(e / n), i = int(e % n);
-        const int a = pa[k], b = pb[k];
-        if (b < 0 || sn[k] == 0.0) continue;
-        const double c = cs[k], s = sn[k];
-        const double xa = A[a + i * n], xb = A[b + i * n];
-        A[a + i * n] = c * xa - s * xb;
-        A[b + i * n] = s * xa + c * xb;
+      if (cached) {
+#pragma unroll
+        for (int u = 0; u < kMaxB; ++u)
+          if (u < nbk) block(bk[u], bl[u]);
+#pragma unroll
+        for (int u = 0; u < kMaxV; ++u)
+          if (u < nvi) vrot(vl[u], vi[u]);
+      } else {
+        for (int e = tid; e < half * half; e += NT) {
+          const int k = e % half, l = e / half;
+          if (k <= l) block(k, l);
+        }
+        for (int e = tid; e < half * n; e += NT) vrot(int(e / n), int(e % n));
       }
       __syncthreads();
     }
