@@ -348,7 +348,7 @@ class Model final : public Handle, private StreamOps {
     colss_.alloc(sizeof(double) * p_);
     S_.alloc(sizeof(double) * (size_t)p_ * qc);
     S0_.alloc(sizeof(double) * (size_t)p_ * qc);
-    const int splits = xtw_splits(n_, p_, std::min(qc, 64));
+    const int splits = xtw_splits(n_, p_, std::min(qc, 64), sizeof(T) == 4);
     Buf& s_part = cp_part_;
     Buf& stats = cp_stats_;
     s_part.alloc(sizeof(double) * (size_t)splits * p_ * qc);
@@ -499,8 +499,8 @@ class Model final : public Handle, private StreamOps {
     cand_.alloc(sizeof(double) * (size_t)p_ * bmax);
     cand_scaled_.alloc(sizeof(double) * (size_t)p_ * bmax);
     t_raw_.alloc(sizeof(double) * (size_t)n_ * bmax);
-    const int xs = xw_splits(n_, p_, bmax);
-    const int ls = xtw_splits(n_, p_, bmax);
+    const int xs = xw_splits(n_, p_, bmax, sizeof(T) == 4);
+    const int ls = xtw_splits(n_, p_, bmax, sizeof(T) == 4);
     ts_part_.alloc(sizeof(double) * (size_t)xs * n_ * bmax);
     ls_part_.alloc(sizeof(double) * (size_t)ls * p_ * bmax);
 
@@ -603,7 +603,7 @@ class Model final : public Handle, private StreamOps {
     Tsc_.alloc(sizeof(double) * (size_t)n_ * k);
     {
       // scores T = (X - 1 mᵀ)(R / s): the second and last pass over X
-      const int splits = xw_splits(n_, p_, k);
+      const int splits = xw_splits(n_, p_, k, sizeof(T) == 4);
       part_.reserve(sizeof(double) * (size_t)splits * n_ * k);
       XwTask<T> t{};
       t.A = X_; t.m = n_; t.k = p_; t.center = mx_.d(); t.W = Rs_.d(); t.ldw = p_; t.N = k;

@@ -94,10 +94,53 @@ inline int splits_for(long long tiles_per_split, long long extent, long long min
   return int(std::min<long long>(s, 256));
 }
 
-// Row splits of an xtw pass over an m x k matrix with N right-hand columns:
-// enough work-groups to fill the device, and no split longer than the FP32
-// accumulation bound of the matrix-core kernel.
-inline int xtw_splits(long long m, long long k, int N) {
+// Compute units of the current device, queried once per process.
+inline int device_cus() {
+  static const int cus = [] {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+      return v;
+    return 256;
+  }();
+  return cus;
+}
+
+// Split count of a bf16x6 matrix-core pass (one work-group per CU at a time).
+// The pass runs in ceil(tiles·s / CUs) rounds of work-groups, each covering
+// 1/s of the streamed extent, and every split adds one FP64 partial slab that
+// is written and summed again. Picks the s in [1, min(256, extent / quantum)]
+// minimising
+//   stream_bytes · (CUs / tiles) · ceil(tiles·s / CUs) / s + 2 s · slab_bytes,
+// the smallest on ties (measured on MI355X: the PLS-SVD preprocessing pass
+// over a 1e5 x 2e4 FP32 X runs in 1.94 ms at 13 splits and 2.23 ms at 25).
+inline int balanced_splits(long long tiles, long long extent, long long quantum,
+                           double stream_bytes, double slab_bytes) {
+  const long long cus = device_cus();
+  const long long cap = std::max<long long>(1, std::min<long long>(256, extent / quantum));
+  const long long t = std::max<long long>(tiles, 1);
+  int best = 1;
+  double best_cost = 0.0;
+  for (long long s = 1; s <= cap; ++s) {
+    const double rounds = double((t * s + cus - 1) / cus);
+    const double cost = stream_bytes * double(cus) / double(t) * rounds / double(s) +
+                        2.0 * double(s) * slab_bytes;
+    if (s == 1 || cost < best_cost * (1.0 - 1e-9)) {
+      best_cost = cost;
+      best = int(s);
+    }
+  }
+  return best;
+}
+
+// Row splits of an xtw pass over an m x k matrix with N right-hand columns.
+// FP32 A on the matrix cores (bf6) balances the rounds of work-groups against
+// the partial slabs; otherwise enough work-groups to fill the device, and for
+// the FP64 matrix-core kernel no split longer than kXtwSplitRows.
+inline int xtw_splits(long long m, long long k, int N, bool bf6 = false) {
+  if (bf6 && N >= kMfmaMinN)
+    return balanced_splits(xtw_tiles(k, N), m, 128, 4.0 * double(m) * double(k),
+                           8.0 * double(k) * double(std::min(N, 64)));
   int s = splits_for(xtw_tiles(k, N), m, 128);
   if (N >= kMfmaMinN) {
     const long long need = (m + kXtwSplitRows - 1) / kXtwSplitRows;
@@ -106,10 +149,11 @@ inline int xtw_splits(long long m, long long k, int N) {
   return s;
 }
 
-// Column splits of an xw pass (m x k times k x N): enough work-groups to fill
-// the device, and no split wider than the FP32 accumulation bound of the
-// matrix-core kernel.
-inline int xw_splits(long long m, long long k, int N) {
+// Column splits of an xw pass (m x k times k x N), chosen as for xtw.
+inline int xw_splits(long long m, long long k, int N, bool bf6 = false) {
+  if (bf6 && N >= kMfmaMinN)
+    return balanced_splits(xw_row_tiles(m), k, 64, 4.0 * double(m) * double(k),
+                           8.0 * double(m) * double(std::min(N, 64)));
   int s = splits_for(xw_row_tiles(m), k, 32);
   if (N >= kMfmaMinN) {
     const long long need = (k + kXwSplitCols - 1) / kXwSplitCols;

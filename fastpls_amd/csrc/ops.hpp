@@ -37,7 +37,7 @@ class StreamOps {
                long long ldw, int N, double* out) {
     for (int c0 = 0; c0 < N; c0 += 64) {
       const int nb = std::min(64, N - c0);
-      const int splits = invariant_ ? xw_splits_by_depth(k) : xw_splits(m, k, nb);
+      const int splits = invariant_ ? xw_splits_by_depth(k) : xw_splits(m, k, nb, sizeof(U) == 4);
       XwTask<U> t{};
       t.A = A; t.m = m; t.k = k; t.center = center; t.W = W + (long long)c0 * ldw; t.ldw = ldw;
       t.N = nb; t.splits = splits; t.frag = frag(xw_frag_bytes<U>(k, nb));
@@ -59,7 +59,7 @@ class StreamOps {
                 long long ldw, int N, double* out) {
     for (int c0 = 0; c0 < N; c0 += 64) {
       const int nb = std::min(64, N - c0);
-      const int splits = xtw_splits(m, k, nb);
+      const int splits = xtw_splits(m, k, nb, sizeof(U) == 4);
       XtwTask<U> t{};
       t.A = A; t.m = m; t.k = k; t.center = center; t.W = W + (long long)c0 * ldw; t.ldw = ldw;
       t.N = nb; t.mode = center ? kCentered : kPlain; t.splits = splits;

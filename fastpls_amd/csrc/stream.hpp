@@ -85,8 +85,8 @@ constexpr int kMfmaMinN = 8;
 __host__ __device__ inline long long xtw_tiles(long long k, int maxn) {
   return maxn >= kMfmaMinN ? (k + 127) / 128 : (k + 63) / 64;
 }
-// Longest row split of the matrix-core xtw and longest column split of the
-// matrix-core xw (their FP32 accumulators run over one split).
+// Longest row split of an FP64 matrix-core xtw and longest column split of an
+// FP64 matrix-core xw, and of every split of the batched launches.
 constexpr long long kXtwSplitRows = 4096;
 constexpr long long kXwSplitCols = 4096;
 

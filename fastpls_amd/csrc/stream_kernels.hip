@@ -642,9 +642,9 @@ __global__ __launch_bounds__(kBf6Threads) void k_xtw_bf6(const XtwTask<float>* t
 // those columns as the A fragment of output rows {2r' + e}. The W fragments
 // (already split, W's own column order) and the FP32 centre of a step are
 // staged in LDS (double-buffered, shared by the eight waves); the next
-// step's loads are issued before the current step's arithmetic. Each
-// sub-step's products run in a fresh FP32 chain folded into FP64
-// accumulators.
+// step's loads are issued before the current step's arithmetic. The products
+// of two consecutive steps (four 32-column sub-steps) run in one FP32 chain
+// per output block, folded into FP64 accumulators.
 template<int NB>
 __global__ __launch_bounds__(kBf6Threads) void k_xw_bf6(const XwTask<float>* tasks, int count,
                                                         XwTask<float> single) {
@@ -733,7 +733,10 @@ __global__ __launch_bounds__(kBf6Threads) void k_xw_bf6(const XwTask<float>* tas
 #pragma unroll
     for (int nb = 0; nb < NBLK; ++nb) acc[e][nb] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-  auto compute = [&](const float (&x)[2][8][2], int b) {
+  // FP32 chains run over both sub-steps of two consecutive steps (buffers 0
+  // then 1) and are folded into the FP64 accumulators after the second
+  f32x4 cc[2][NBLK];
+  auto compute = [&](const float (&x)[2][8][2], int b, bool start, bool finish) {
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
       float c[8];
@@ -750,7 +753,10 @@ __global__ __launch_bounds__(kBf6Threads) void k_xw_bf6(const XwTask<float>* tas
           const u32x4 bh = wl[b][nb * 16 + m16][4 * u + g];
           const u32x4 bm = wl[b][NB + nb * 16 + m16][4 * u + g];
           const u32x4 bl = wl[b][2 * NB + nb * 16 + m16][4 * u + g];
-          fold(acc[e][nb], mma6(a, bh, bm, bl, f32x4{0.f, 0.f, 0.f, 0.f}));
+          const f32x4 c0 = (start && u == 0) ? f32x4{0.f, 0.f, 0.f, 0.f} : cc[e][nb];
+          const f32x4 c = mma6(a, bh, bm, bl, c0);
+          if (finish && u == 1) fold(acc[e][nb], c);
+          else cc[e][nb] = c;
         }
         __builtin_amdgcn_sched_barrier(0);
       }
@@ -778,17 +784,17 @@ __global__ __launch_bounds__(kBf6Threads) void k_xw_bf6(const XwTask<float>* tas
     for (; s + 1 < nsteps; s += 2) {
       load_w(wr, cr, j0 + COLS * (s + 1));
       load_any(xb, j0 + COLS * (s + 1));
-      compute(xa, 0);
+      compute(xa, 0, true, false);
       store_w(wr, cr, 1);
       __syncthreads();
       const long long nx = j0 + COLS * (s + 2 < nsteps ? s + 2 : s + 1);
       load_w(wr, cr, nx);
       load_any(xa, nx);
-      compute(xb, 1);
+      compute(xb, 1, false, true);
       store_w(wr, cr, 0);
       __syncthreads();
     }
-    if (s < nsteps) compute(xa, 0);
+    if (s < nsteps) compute(xa, 0, true, true);
   }
 
   double FPLS_GLOBAL* o = gpw(t.out) + (long long)split * t.slab;
