@@ -56,6 +56,49 @@ __device__ inline void set_flag(int* state, int flag) {
   if (threadIdx.x == 0) atomicOr(&state[2], flag);
 }
 
+// C (m x n) = A B for a tall A (m x k) and a small B (k x n): B staged in LDS
+// (`stage`, 2 kStageDoubles doubles) and one thread per row of C when n <= 32
+// and k n fits, else the generic work-group product.
+__device__ inline void tall_times_small(long m, int n, int k, const double* A, long lda,
+                                        const double* B, long ldb, double* C, long ldc,
+                                        double* stage) {
+  constexpr int kCap = 2 * wg::kStageDoubles;
+  if (n <= 16 && k * 16 <= kCap) {
+    wg::stage_rows<16>(B, ldb, k, n, stage);
+    wg::gemm_rows_lds<16>(m, n, k, A, lda, stage, C, ldc);
+  } else if (n <= 32 && k * 32 <= kCap) {
+    wg::stage_rows<32>(B, ldb, k, n, stage);
+    wg::gemm_rows_lds<32>(m, n, k, A, lda, stage, C, ldc);
+  } else {
+    wg::gemm(int(m), n, k, 1.0, A, lda, false, B, ldb, false, 0.0, C, ldc);
+  }
+}
+
+// C (m x n) = Aᵀ B for A (k x m), B (k x n) with a long k: on the matrix
+// cores when m, n <= 64, one wave per output otherwise.
+__device__ inline void tall_tn(int m, int n, long k, const double* A, long lda, const double* B,
+                               long ldb, double* C, long ldc, double* stage) {
+  if (m <= 64 && n <= 64) wg::gemm_tn_mfma(m, n, k, A, lda, B, ldb, C, ldc, stage);
+  else wg::gemm_tn_long(m, n, int(k), A, lda, B, ldb, C, ldc);
+}
+
+// LDS of the one-work-group tall CholQR2.
+struct TallQrLds {
+  double stage[2 * wg::kStageDoubles];
+  double gm[32 * 32], ri[32 * 32], lc[32 * 32], xb[32];
+  int ok;
+};
+
+// Orthonormal basis Qb of a tall sample (p x w): CholQR2 by one work-group
+// when w <= 32, Householder when a Cholesky factor is unusable or w > 32.
+// Returns false on a non-finite column norm (Householder).
+__device__ inline bool tall_basis(double* sample, int p, int w, double* Qb, TallQrLds& t,
+                                  WgScratch& sh) {
+  if (w <= 32 && wg::cholqr2_tall(sample, p, w, Qb, t.stage, t.gm, t.ri, t.lc, t.xb, &t.ok))
+    return true;
+  return wg::householder_qr(sample, p, p, w, Qb, p, sh.tau, sh.dots, sh.red, sh.part);
+}
+
 // ----------------------------------------------------------- SIMPLS refresh
 // simpls.hpp:226-368 (refresh_directions), right-Gram and plain branches.
 __global__ __launch_bounds__(kWG) void k_simpls_refresh(const SimplsFitDev* fits, int count,
@@ -66,6 +109,7 @@ __global__ __launch_bounds__(kWG) void k_simpls_refresh(const SimplsFitDev* fits
   const RefreshStep st = f.steps[step];
   const int p = f.p, q = f.q, w = st.w, b = st.b;
   __shared__ WgScratch sh;
+  __shared__ TallQrLds tq;
   const RefreshLayout L(p, q, w, f.A);
   double* base = f.scratch;
   double* Z = base + L.Z;
@@ -90,28 +134,30 @@ __global__ __launch_bounds__(kWG) void k_simpls_refresh(const SimplsFitDev* fits
       wg::copy(q, w, Z, q, Y2, q);
       ok = wg::householder_qr(Y2, q, q, w, Z, q, sh.tau, sh.dots, sh.red, sh.part) && ok;
     }
-    wg::gemm(p, w, q, 1.0, f.S, p, false, Z, q, false, 0.0, sample, p);
+    tall_times_small(p, w, q, f.S, p, Z, q, sample, p, tq.stage);
   } else {
-    wg::gemm(p, w, q, 1.0, f.S, p, false, omega, q, false, 0.0, sample, p);
+    tall_times_small(p, w, q, f.S, p, omega, q, sample, p, tq.stage);
     for (int it = 0; it < f.power; ++it) {
-      ok = wg::householder_qr(sample, p, p, w, Qb, p, sh.tau, sh.dots, sh.red, sh.part) && ok;
-      wg::gemm_tn_long(q, w, p, f.S, p, Qb, p, Y2, q);
+      ok = tall_basis(sample, p, w, Qb, tq, sh) && ok;
+      tall_tn(q, w, p, f.S, p, Qb, p, Y2, q, tq.stage);
       ok = wg::householder_qr(Y2, q, q, w, Z, q, sh.tau, sh.dots, sh.red, sh.part) && ok;
-      wg::gemm(p, w, q, 1.0, f.S, p, false, Z, q, false, 0.0, sample, p);
+      tall_times_small(p, w, q, f.S, p, Z, q, sample, p, tq.stage);
     }
   }
-  ok = wg::householder_qr(sample, p, p, w, Qb, p, sh.tau, sh.dots, sh.red, sh.part) && ok;
-  wg::gemm_tn_long(w, q, p, Qb, p, f.S, p, Bm, w);
+  ok = tall_basis(sample, p, w, Qb, tq, sh) && ok;
+  tall_tn(w, q, p, Qb, p, f.S, p, Bm, w, tq.stage);
   wg::gemm(w, w, q, 1.0, Bm, w, false, Bm, w, true, 0.0, Gm, w);
   ok = wg::jacobi_eigen(Gm, Ve, w, ev, sh.cs, sh.sn, sh.pa, sh.pb, sh.red, vtmp) && ok;
   // candidates = Qb * (top-b eigenvectors, largest first)
-  for (long long e = threadIdx.x; e < (long long)p * b; e += kWG) {
-    const int i = int(e % p), c = int(e / p);
+  for (int c = 0; c < b; ++c) {
     const double* v = Ve + (long long)(w - 1 - c) * w;
-    double acc = 0.0;
-    for (int l = 0; l < w; ++l) acc = fma(Qb[i + (long long)l * p], v[l], acc);
-    f.cand[i + (long long)c * p] = acc;
-    f.cand_scaled[i + (long long)c * p] = acc / f.sx[i];
+    for (int i = threadIdx.x; i < p; i += kWG) {
+      double acc = 0.0;
+#pragma unroll 8
+      for (int l = 0; l < w; ++l) acc = fma(Qb[i + (long long)l * p], v[l], acc);
+      f.cand[i + (long long)c * p] = acc;
+      f.cand_scaled[i + (long long)c * p] = acc / f.sx[i];
+    }
   }
   if (!ok) {
     set_flag(f.state, kFlagDecomposition);
@@ -168,6 +214,7 @@ __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits,
         wg::gemm_tn_long(comp, 1, p, f.V, p, Vc, p, proj, comp);
         for (int i = threadIdx.x; i < p; i += kWG) {
           double s = 0.0;
+#pragma unroll 8
           for (int l = 0; l < comp; ++l) s = fma(f.V[i + (long long)l * p], proj[l], s);
           Vc[i] -= s;
         }
@@ -187,9 +234,10 @@ __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits,
     // and projects it out of every product instead)
     if (f.S) {
       wg::gemm_tn_long(q, 1, p, f.S, p, Vc, p, row, q);
-      for (long long e = threadIdx.x; e < (long long)p * q; e += kWG) {
-        const int i = int(e % p), c = int(e / p);
-        f.S[e] -= Vc[i] * row[c];
+      for (int i = threadIdx.x; i < p; i += kWG) {
+        const double v = Vc[i];
+#pragma unroll 4
+        for (int c = 0; c < q; ++c) f.S[i + (long long)c * p] -= v * row[c];
       }
     }
     if (f.S && f.use_right_gram) {

@@ -31,17 +31,201 @@ __device__ inline void gemm(int m, int n, int k, double alpha, const double* A,
 }
 
 // C(m x n) = Aᵀ B with A (k x m), B (k x n) and a long inner dimension k:
-// one wave per output, lanes stride the inner dimension (coalesced).
+// one wave per output, lanes stride the inner dimension (coalesced), four
+// independent partial sums so that eight loads per lane are in flight.
 template<int NT = kWG>
-__device__ inline void gemm_tn_long(int m, int n, int k, const double* A, long lda,
+__device__ __attribute__((noinline)) void gemm_tn_long(int m, int n, int k, const double* A, long lda,
                                     const double* B, long ldb, double* C, long ldc) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int e = wave; e < m * n; e += (NT / 64)) {
     const int i = e % m, j = e / m;
-    double acc = 0.0;
-    for (int l = lane; l < k; l += 64) acc = fma(A[l + i * lda], B[l + (long)j * ldb], acc);
-    acc = wave_sum(acc);
+    const double* a = A + (long)i * lda;
+    const double* b = B + (long)j * ldb;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int l = lane;
+    for (; l + 192 < k; l += 256) {
+      s0 = fma(a[l], b[l], s0);
+      s1 = fma(a[l + 64], b[l + 64], s1);
+      s2 = fma(a[l + 128], b[l + 128], s2);
+      s3 = fma(a[l + 192], b[l + 192], s3);
+    }
+    for (; l < k; l += 64) s0 = fma(a[l], b[l], s0);
+    const double acc = wave_sum((s0 + s1) + (s2 + s3));
     if (lane == 0) C[i + j * ldc] = acc;
+  }
+  __syncthreads();
+}
+
+// C (m x n, ldc) = Aᵀ B for A (k x m, lda) and B (k x n, ldb), m, n <= 64,
+// with a long inner dimension k, on the FP64 matrix cores by one work-group
+// of 256 threads. Blocks of rb rows of A and B (rb a power of two, rb (m + n)
+// <= kStageElems) are staged in LDS, double-buffered through registers; wave
+// w multiplies rows [w rb/4, (w + 1) rb/4) of each block into 16 x 16 output
+// tiles (16x16x4 FP64 MFMA: lane l supplies column l % 16 of the tile at row
+// l / 16 of each 4-row step; item r of lane l holds tile entry (l/16 + 4r,
+// l % 16)). The wave partials are added in wave order (fixed order). stage:
+// 2 kStageDoubles doubles of LDS. Ends with a barrier.
+constexpr int kStageElems = 4096;
+constexpr int kStageDoubles = kStageElems + 128;
+typedef double wg_f64x4 __attribute__((ext_vector_type(4)));
+__device__ __attribute__((noinline)) void gemm_tn_mfma(int m, int n, long k, const double* A,
+                                                       long lda, const double* B, long ldb,
+                                                       double* C, long ldc, double* stage) {
+  constexpr int NT = 256;
+  constexpr int kPer = kStageElems / NT;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m16 = lane & 15, g = lane >> 4;
+  const int cols = m + n;
+  int lg = 4;
+  while ((2 << lg) * cols <= kStageElems && lg < 8) ++lg;
+  const int rb = 1 << lg, ld = rb + 1, wrows = rb / 4;
+  const int mt_n = (m + 15) / 16, nt_n = (n + 15) / 16;
+  wg_f64x4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = wg_f64x4{0.0, 0.0, 0.0, 0.0};
+  double reg[kPer];
+  // branch-free loads: out-of-range elements read a valid address (the last
+  // row or column) and are replaced by zero
+  auto load = [&](long r0) {
+#pragma unroll
+    for (int s = 0; s < kPer; ++s) {
+      const int e = tid + s * NT;
+      const int r = e & (rb - 1), col = e >> lg;
+      const int cc = col < cols ? col : cols - 1;
+      const long rr = r0 + r < k ? r0 + r : k - 1;
+      const double* src = cc < m ? A + (long)cc * lda : B + (long)(cc - m) * ldb;
+      double v = ((const double FPLS_GLOBAL*)src)[rr];
+      asm volatile("" : "+v"(v));  // keep the load unconditional (not sunk into a branch)
+      reg[s] = (col < cols && r0 + r < k) ? v : 0.0;
+    }
+  };
+  auto store = [&](double* buf) {
+#pragma unroll
+    for (int s = 0; s < kPer; ++s) {
+      const int e = tid + s * NT;
+      const int r = e & (rb - 1), col = e >> lg;
+      if (col < cols) buf[col * ld + r] = reg[s];
+    }
+  };
+  auto compute = [&](const double* buf) {
+    const int r0 = wave * wrows + g;
+    for (int st = 0; st < wrows; st += 4) {
+      double a[4], b[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int ca = t * 16 + m16, cb = t * 16 + m16;
+        a[t] = (t < mt_n && ca < m) ? buf[ca * ld + r0 + st] : 0.0;
+        b[t] = (t < nt_n && cb < n) ? buf[(m + cb) * ld + r0 + st] : 0.0;
+      }
+#pragma unroll
+      for (int ta = 0; ta < 4; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < 4; ++tb)
+          if (ta < mt_n && tb < nt_n)
+            acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta], b[tb], acc[ta][tb], 0, 0, 0);
+    }
+  };
+  const long nblk = (k + rb - 1) / rb;
+  if (nblk > 0) {
+    load(0);
+    store(stage);
+    __syncthreads();
+    for (long blk = 0; blk < nblk; ++blk) {
+      double* cur = stage + (blk & 1) * kStageDoubles;
+      double* nxt = stage + ((blk + 1) & 1) * kStageDoubles;
+      if (blk + 1 < nblk) load((blk + 1) * rb);
+      compute(cur);
+      if (blk + 1 < nblk) store(nxt);
+      __syncthreads();
+    }
+  }
+  // wave partials in wave order: red[row + 64 col]
+  double* red = stage;
+  for (int w2 = 0; w2 < 4; ++w2) {
+    if (wave == w2) {
+#pragma unroll
+      for (int ta = 0; ta < 4; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < 4; ++tb) {
+          if (ta >= mt_n || tb >= nt_n) continue;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = ta * 16 + g + 4 * r, col = tb * 16 + m16;
+            if (w2 == 0) red[row + 64 * col] = acc[ta][tb][r];
+            else red[row + 64 * col] += acc[ta][tb][r];
+          }
+        }
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < m * n; e += NT) {
+    const int row = e % m, col = e / m;
+    C[row + (long)col * ldc] = red[row + 64 * col];
+  }
+  __syncthreads();
+}
+
+// Bs (k x NMAX, row-major: Bs[l NMAX + j]) = B (k x n, ldb) padded with zero
+// columns, for gemm_rows_lds. Ends with a barrier.
+template<int NMAX, int NT = kWG>
+__device__ inline void stage_rows(const double* B, long ldb, int k, int n, double* Bs) {
+  for (int e = threadIdx.x; e < k * NMAX; e += NT) {
+    const int l = e / NMAX, j = e % NMAX;
+    Bs[e] = j < n ? B[l + (long)j * ldb] : 0.0;
+  }
+  __syncthreads();
+}
+
+// C (m x n, ldc) = A B for a tall A (m x k, lda) in global memory and B
+// staged by stage_rows in LDS (n <= NMAX): one thread per row of C (two rows
+// per pass), B read as LDS broadcasts, no data-dependent branches.
+// A row is read in full before it is written, so C may be A itself (ldc ==
+// lda, n <= k). Ends with a barrier.
+template<int NMAX, int NT = kWG>
+__device__ __attribute__((noinline)) void gemm_rows_lds(long m, int n, int k, const double* A,
+                                                        long lda, const double* Bs, double* C,
+                                                        long ldc) {
+  // two rows per thread and pass (the second clamped to a valid row and not
+  // written when past m), so twice the loads are in flight
+  const double FPLS_GLOBAL* Ag = (const double FPLS_GLOBAL*)A;
+  for (long i0 = threadIdx.x; i0 < m; i0 += 2 * NT) {
+    const long i1 = i0 + NT < m ? i0 + NT : i0;
+    double acc0[NMAX], acc1[NMAX];
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) { acc0[j] = 0.0; acc1[j] = 0.0; }
+    int l = 0;
+    for (; l + 4 <= k; l += 4) {
+      double a0[4], a1[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a0[u] = Ag[i0 + (long)(l + u) * lda];
+        a1[u] = Ag[i1 + (long)(l + u) * lda];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int j = 0; j < NMAX; ++j) {
+          const double bv = Bs[(l + u) * NMAX + j];
+          acc0[j] = fma(a0[u], bv, acc0[j]);
+          acc1[j] = fma(a1[u], bv, acc1[j]);
+        }
+    }
+    for (; l < k; ++l) {
+      const double a0 = Ag[i0 + (long)l * lda], a1 = Ag[i1 + (long)l * lda];
+#pragma unroll
+      for (int j = 0; j < NMAX; ++j) {
+        acc0[j] = fma(a0, Bs[l * NMAX + j], acc0[j]);
+        acc1[j] = fma(a1, Bs[l * NMAX + j], acc1[j]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j)
+      if (j < n) {
+        C[i0 + (long)j * ldc] = acc0[j];
+        if (i0 + NT < m) C[i1 + (long)j * ldc] = acc1[j];
+      }
   }
   __syncthreads();
 }
@@ -58,9 +242,16 @@ __device__ inline void copy(int m, int n, const double* A, long lda, double* B, 
 // Euclidean norm of a vector (all threads receive it).
 template<int NT = kWG>
 __device__ inline double norm(const double* v, long n, double* red) {
-  double s = 0.0;
-  for (long i = threadIdx.x; i < n; i += NT) s = fma(v[i], v[i], s);
-  return sqrt(fmax(wg_sum_n<NT>(s, red), 0.0));
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  long i = threadIdx.x;
+  for (; i + 3 * NT < n; i += 4 * NT) {
+    s0 = fma(v[i], v[i], s0);
+    s1 = fma(v[i + NT], v[i + NT], s1);
+    s2 = fma(v[i + 2 * NT], v[i + 2 * NT], s2);
+    s3 = fma(v[i + 3 * NT], v[i + 3 * NT], s3);
+  }
+  for (; i < n; i += NT) s0 = fma(v[i], v[i], s0);
+  return sqrt(fmax(wg_sum_n<NT>((s0 + s1) + (s2 + s3), red), 0.0));
 }
 
 // d_c = Σ_{i=j..m-1} v_i B[i, c] for the columns c in [c0, n), with the
@@ -105,7 +296,7 @@ __device__ inline void reflector_dots(const double* V, long ldv, const double* B
 // the explicit Q (m x n) goes to Q. tau and dots are LDS arrays of n doubles,
 // part NT doubles of LDS. Returns false on a non-finite column norm.
 template<int NT = kWG>
-__device__ inline bool householder_qr(double* A, long lda, int m, int n, double* Q,
+__device__ __attribute__((noinline)) bool householder_qr(double* A, long lda, int m, int n, double* Q,
                                       long ldq, double* tau, double* dots, double* red,
                                       double* part) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -247,6 +438,31 @@ __device__ __forceinline__ bool wave_chol_inv(const double* G, long ldg, int w, 
   return true;
 }
 
+// CholQR2 of a tall A (m x w, leading dimension m, w <= 32) in global memory
+// by one work-group of 256 threads, into Q (m x w, leading dimension m, not
+// aliasing A): each Gram on the matrix cores (gemm_tn_mfma), its factor by one
+// wave (wave_chol_inv), the rows transformed by gemm_rows_lds (the second
+// pass in place). Returns false, with A untouched, when a Cholesky factor is
+// unusable. LDS: stage (2 kStageDoubles), gm, ri, lc (32·32 each), xb (32),
+// okflag.
+__device__ __attribute__((noinline)) bool cholqr2_tall(const double* A, long m, int w, double* Q,
+                                                       double* stage, double* gm, double* ri,
+                                                       double* lc, double* xb, int* okflag) {
+  for (int pass = 0; pass < 2; ++pass) {
+    const double* X = pass == 0 ? A : Q;
+    gemm_tn_mfma(w, w, m, X, m, X, m, gm, 32, stage);
+    if (threadIdx.x < 64) {
+      const bool good = wave_chol_inv<32>(gm, 32, w, ri, 32, lc, xb);
+      if (threadIdx.x == 0) *okflag = good ? 1 : 0;
+    }
+    __syncthreads();
+    if (*okflag == 0) return false;
+    stage_rows<32, kWG>(ri, 32, w, w, gm);
+    gemm_rows_lds<32, kWG>(m, w, w, X, m, gm, Q, m);
+  }
+  return true;
+}
+
 // Pair (a, b) of round r for k-th slot of the round-robin (circle) schedule
 // over m (even) players.
 template<int NT = kWG>
@@ -319,7 +535,7 @@ __device__ __forceinline__ int rr_at(int pos, int r, int m) {
 // so A stays exactly symmetric) together with V J. Up to n = 64 (with 256
 // threads) every thread keeps its block and V assignments in registers.
 template<int NT = kWG>
-__device__ inline bool jacobi_eigen(double* A, double* V, int n, double* w, double* cs,
+__device__ __attribute__((noinline)) bool jacobi_eigen(double* A, double* V, int n, double* w, double* cs,
                                     double* sn, int* pa, int* pb, double* red,
                                     double* vtmp /* n x n scratch */) {
   const int tid = threadIdx.x;
@@ -447,7 +663,7 @@ __device__ inline bool jacobi_eigen(double* A, double* V, int n, double* w, doub
 // On return: s (n, descending), U (m x n) left vectors, Vm (n x n) right
 // vectors as columns. W is destroyed. cs/sn/pa/pb: LDS, n/2+1 entries.
 template<int NT = kWG>
-__device__ inline bool jacobi_svd(double* W, int m, int n, double* U, double* Vm,
+__device__ __attribute__((noinline)) bool jacobi_svd(double* W, int m, int n, double* U, double* Vm,
                                   double* s, double* cs, double* sn, int* pa, int* pb,
                                   double* red, double* sc /* 3*(n/2+1) LDS */) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
