@@ -78,7 +78,7 @@ __device__ inline void tall_times_small(long m, int n, int k, const double* A, l
 // cores when m, n <= 64, one wave per output otherwise.
 __device__ inline void tall_tn(int m, int n, long k, const double* A, long lda, const double* B,
                                long ldb, double* C, long ldc, double* stage) {
-  if (m <= 64 && n <= 64) wg::gemm_tn_mfma(m, n, k, A, lda, B, ldb, C, ldc, stage);
+  if (m <= 64 && n <= 64) wg::gemm_tn_fast(m, n, k, A, lda, B, ldb, C, ldc, stage);
   else wg::gemm_tn_long(m, n, int(k), A, lda, B, ldb, C, ldc);
 }
 

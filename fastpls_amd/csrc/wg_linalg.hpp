@@ -167,6 +167,127 @@ __device__ __attribute__((noinline)) void gemm_tn_mfma(int m, int n, long k, con
   __syncthreads();
 }
 
+// C (m x n, ldc) = Aᵀ B as gemm_tn_mfma, without staging: the 4-row steps of
+// k are dealt to the four waves in turn, each lane loads its MFMA operands
+// straight from global memory (column l % 16 of a tile at row l / 16 of the
+// step; U steps in flight), TM x TN 16 x 16 tiles cover m <= 16 TM, n <= 16 TN.
+// The wave partials are added in wave order through `red` (4096 doubles of
+// LDS). Ends with a barrier.
+template<int TM, int TN, int U, bool SYM>
+__device__ __attribute__((noinline)) void gemm_tn_direct(int m, int n, long k, const double* A,
+                                                         long lda, const double* B, long ldb,
+                                                         double* C, long ldc, double* red) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m16 = lane & 15, g = lane >> 4;
+  const double FPLS_GLOBAL* pa[TM];
+  const double FPLS_GLOBAL* pb[TN];
+  bool va[TM], vb[TN];
+#pragma unroll
+  for (int t = 0; t < TM; ++t) {
+    const int c = t * 16 + m16;
+    va[t] = c < m;
+    pa[t] = (const double FPLS_GLOBAL*)(A + (long)(va[t] ? c : 0) * lda);
+  }
+#pragma unroll
+  for (int t = 0; t < TN; ++t) {
+    const int c = t * 16 + m16;
+    vb[t] = c < n;
+    pb[t] = (const double FPLS_GLOBAL*)(B + (long)(vb[t] ? c : 0) * ldb);
+  }
+  wg_f64x4 acc[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b) acc[a][b] = wg_f64x4{0.0, 0.0, 0.0, 0.0};
+  // wave w owns the 4-row steps w, w + 4, ...; an iteration covers U of them
+  // and the operands of the next iteration are loaded before the matrix
+  // products of the current one (two register sets)
+  const long nsteps = (k + 3) / 4;
+  double av[2][U][TM], bv[2][U][SYM ? 1 : TN];
+  auto load = [&](int set, long s0) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long row = 4 * (s0 + 4L * u) + g;
+      const bool ok = row < k;
+      const long rr = ok ? row : 0;
+#pragma unroll
+      for (int t = 0; t < TM; ++t) {
+        double v = pa[t][rr];
+        asm volatile("" : "+v"(v));
+        av[set][u][t] = (ok && va[t]) ? v : 0.0;
+      }
+      if constexpr (!SYM) {
+#pragma unroll
+        for (int t = 0; t < TN; ++t) {
+          double v = pb[t][rr];
+          asm volatile("" : "+v"(v));
+          bv[set][u][t] = (ok && vb[t]) ? v : 0.0;
+        }
+      }
+    }
+  };
+  auto multiply = [&](int set) {
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b) {
+          const double bb = SYM ? av[set][u][b] : bv[set][u][SYM ? 0 : b];
+          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[set][u][a], bb, acc[a][b], 0, 0, 0);
+        }
+  };
+  const long stride = 4L * U;
+  long s0 = wave;
+  if (s0 < nsteps) load(0, s0);
+  for (; s0 < nsteps; s0 += 2 * stride) {
+    if (s0 + stride < nsteps) load(1, s0 + stride);
+    multiply(0);
+    if (s0 + stride >= nsteps) break;
+    if (s0 + 2 * stride < nsteps) load(0, s0 + 2 * stride);
+    multiply(1);
+  }
+  for (int w2 = 0; w2 < 4; ++w2) {
+    if (wave == w2) {
+#pragma unroll
+      for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = a * 16 + g + 4 * r, col = b * 16 + m16;
+            if (w2 == 0) red[row + 64 * col] = acc[a][b][r];
+            else red[row + 64 * col] += acc[a][b][r];
+          }
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < m * n; e += 256) {
+    const int row = e % m, col = e / m;
+    C[row + (long)col * ldc] = red[row + 64 * col];
+  }
+  __syncthreads();
+}
+
+// Dispatch of gemm_tn_direct by tile counts (m, n <= 64).
+// A Gram (A == B, m == n) loads each operand once.
+__device__ inline void gemm_tn_fast(int m, int n, long k, const double* A, long lda,
+                                    const double* B, long ldb, double* C, long ldc, double* red) {
+  const int tm = (m + 15) / 16, tn = (n + 15) / 16;
+  const bool sym = A == B && lda == ldb && m == n;
+  if (sym) {
+    if (tm <= 1) gemm_tn_direct<1, 1, 16, true>(m, n, k, A, lda, B, ldb, C, ldc, red);
+    else if (tm <= 2) gemm_tn_direct<2, 2, 8, true>(m, n, k, A, lda, B, ldb, C, ldc, red);
+    else gemm_tn_direct<4, 4, 4, true>(m, n, k, A, lda, B, ldb, C, ldc, red);
+  } else if (tm <= 1 && tn <= 1) {
+    gemm_tn_direct<1, 1, 16, false>(m, n, k, A, lda, B, ldb, C, ldc, red);
+  } else if (tm <= 2 && tn <= 2) {
+    gemm_tn_direct<2, 2, 8, false>(m, n, k, A, lda, B, ldb, C, ldc, red);
+  } else {
+    gemm_tn_direct<4, 4, 4, false>(m, n, k, A, lda, B, ldb, C, ldc, red);
+  }
+}
+
 // Bs (k x NMAX, row-major: Bs[l NMAX + j]) = B (k x n, ldb) padded with zero
 // columns, for gemm_rows_lds. Ends with a barrier.
 template<int NMAX, int NT = kWG>
@@ -450,7 +571,7 @@ __device__ __attribute__((noinline)) bool cholqr2_tall(const double* A, long m, 
                                                        double* lc, double* xb, int* okflag) {
   for (int pass = 0; pass < 2; ++pass) {
     const double* X = pass == 0 ? A : Q;
-    gemm_tn_mfma(w, w, m, X, m, X, m, gm, 32, stage);
+    gemm_tn_fast(w, w, m, X, m, X, m, gm, 32, stage);
     if (threadIdx.x < 64) {
       const bool good = wave_chol_inv<32>(gm, 32, w, ri, 32, lc, xb);
       if (threadIdx.x == 0) *okflag = good ? 1 : 0;
