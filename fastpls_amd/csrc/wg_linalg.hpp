@@ -56,237 +56,9 @@ __device__ __attribute__((noinline)) void gemm_tn_long(int m, int n, int k, cons
   __syncthreads();
 }
 
-// C (m x n, ldc) = Aᵀ B for A (k x m, lda) and B (k x n, ldb), m, n <= 64,
-// with a long inner dimension k, on the FP64 matrix cores by one work-group
-// of 256 threads. Blocks of rb rows of A and B (rb a power of two, rb (m + n)
-// <= kStageElems) are staged in LDS, double-buffered through registers; wave
-// w multiplies rows [w rb/4, (w + 1) rb/4) of each block into 16 x 16 output
-// tiles (16x16x4 FP64 MFMA: lane l supplies column l % 16 of the tile at row
-// l / 16 of each 4-row step; item r of lane l holds tile entry (l/16 + 4r,
-// l % 16)). The wave partials are added in wave order (fixed order). stage:
-// 2 kStageDoubles doubles of LDS. Ends with a barrier.
 constexpr int kStageElems = 4096;
 constexpr int kStageDoubles = kStageElems + 128;
 typedef double wg_f64x4 __attribute__((ext_vector_type(4)));
-__device__ __attribute__((noinline)) void gemm_tn_mfma(int m, int n, long k, const double* A,
-                                                       long lda, const double* B, long ldb,
-                                                       double* C, long ldc, double* stage) {
-  constexpr int NT = 256;
-  constexpr int kPer = kStageElems / NT;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int m16 = lane & 15, g = lane >> 4;
-  const int cols = m + n;
-  int lg = 4;
-  while ((2 << lg) * cols <= kStageElems && lg < 8) ++lg;
-  const int rb = 1 << lg, ld = rb + 1, wrows = rb / 4;
-  const int mt_n = (m + 15) / 16, nt_n = (n + 15) / 16;
-  wg_f64x4 acc[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = wg_f64x4{0.0, 0.0, 0.0, 0.0};
-  double reg[kPer];
-  // branch-free loads: out-of-range elements read a valid address (the last
-  // row or column) and are replaced by zero
-  auto load = [&](long r0) {
-#pragma unroll
-    for (int s = 0; s < kPer; ++s) {
-      const int e = tid + s * NT;
-      const int r = e & (rb - 1), col = e >> lg;
-      const int cc = col < cols ? col : cols - 1;
-      const long rr = r0 + r < k ? r0 + r : k - 1;
-      const double* src = cc < m ? A + (long)cc * lda : B + (long)(cc - m) * ldb;
-      double v = ((const double FPLS_GLOBAL*)src)[rr];
-      asm volatile("" : "+v"(v));  // keep the load unconditional (not sunk into a branch)
-      reg[s] = (col < cols && r0 + r < k) ? v : 0.0;
-    }
-  };
-  auto store = [&](double* buf) {
-#pragma unroll
-    for (int s = 0; s < kPer; ++s) {
-      const int e = tid + s * NT;
-      const int r = e & (rb - 1), col = e >> lg;
-      if (col < cols) buf[col * ld + r] = reg[s];
-    }
-  };
-  auto compute = [&](const double* buf) {
-    const int r0 = wave * wrows + g;
-    for (int st = 0; st < wrows; st += 4) {
-      double a[4], b[4];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int ca = t * 16 + m16, cb = t * 16 + m16;
-        a[t] = (t < mt_n && ca < m) ? buf[ca * ld + r0 + st] : 0.0;
-        b[t] = (t < nt_n && cb < n) ? buf[(m + cb) * ld + r0 + st] : 0.0;
-      }
-#pragma unroll
-      for (int ta = 0; ta < 4; ++ta)
-#pragma unroll
-        for (int tb = 0; tb < 4; ++tb)
-          if (ta < mt_n && tb < nt_n)
-            acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta], b[tb], acc[ta][tb], 0, 0, 0);
-    }
-  };
-  const long nblk = (k + rb - 1) / rb;
-  if (nblk > 0) {
-    load(0);
-    store(stage);
-    __syncthreads();
-    for (long blk = 0; blk < nblk; ++blk) {
-      double* cur = stage + (blk & 1) * kStageDoubles;
-      double* nxt = stage + ((blk + 1) & 1) * kStageDoubles;
-      if (blk + 1 < nblk) load((blk + 1) * rb);
-      compute(cur);
-      if (blk + 1 < nblk) store(nxt);
-      __syncthreads();
-    }
-  }
-  // wave partials in wave order: red[row + 64 col]
-  double* red = stage;
-  for (int w2 = 0; w2 < 4; ++w2) {
-    if (wave == w2) {
-#pragma unroll
-      for (int ta = 0; ta < 4; ++ta)
-#pragma unroll
-        for (int tb = 0; tb < 4; ++tb) {
-          if (ta >= mt_n || tb >= nt_n) continue;
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int row = ta * 16 + g + 4 * r, col = tb * 16 + m16;
-            if (w2 == 0) red[row + 64 * col] = acc[ta][tb][r];
-            else red[row + 64 * col] += acc[ta][tb][r];
-          }
-        }
-    }
-    __syncthreads();
-  }
-  for (int e = tid; e < m * n; e += NT) {
-    const int row = e % m, col = e / m;
-    C[row + (long)col * ldc] = red[row + 64 * col];
-  }
-  __syncthreads();
-}
-
-// C (m x n, ldc) = Aᵀ B as gemm_tn_mfma, without staging: the 4-row steps of
-// k are dealt to the four waves in turn, each lane loads its MFMA operands
-// straight from global memory (column l % 16 of a tile at row l / 16 of the
-// step; U steps in flight), TM x TN 16 x 16 tiles cover m <= 16 TM, n <= 16 TN.
-// The wave partials are added in wave order through `red` (4096 doubles of
-// LDS). Ends with a barrier.
-template<int TM, int TN, int U, bool SYM>
-__device__ __attribute__((noinline)) void gemm_tn_direct(int m, int n, long k, const double* A,
-                                                         long lda, const double* B, long ldb,
-                                                         double* C, long ldc, double* red) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int m16 = lane & 15, g = lane >> 4;
-  const double FPLS_GLOBAL* pa[TM];
-  const double FPLS_GLOBAL* pb[TN];
-  bool va[TM], vb[TN];
-#pragma unroll
-  for (int t = 0; t < TM; ++t) {
-    const int c = t * 16 + m16;
-    va[t] = c < m;
-    pa[t] = (const double FPLS_GLOBAL*)(A + (long)(va[t] ? c : 0) * lda);
-  }
-#pragma unroll
-  for (int t = 0; t < TN; ++t) {
-    const int c = t * 16 + m16;
-    vb[t] = c < n;
-    pb[t] = (const double FPLS_GLOBAL*)(B + (long)(vb[t] ? c : 0) * ldb);
-  }
-  wg_f64x4 acc[TM][TN];
-#pragma unroll
-  for (int a = 0; a < TM; ++a)
-#pragma unroll
-    for (int b = 0; b < TN; ++b) acc[a][b] = wg_f64x4{0.0, 0.0, 0.0, 0.0};
-  // wave w owns the 4-row steps w, w + 4, ...; an iteration covers U of them
-  // and the operands of the next iteration are loaded before the matrix
-  // products of the current one (two register sets)
-  const long nsteps = (k + 3) / 4;
-  double av[2][U][TM], bv[2][U][SYM ? 1 : TN];
-  auto load = [&](int set, long s0) {
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const long row = 4 * (s0 + 4L * u) + g;
-      const bool ok = row < k;
-      const long rr = ok ? row : 0;
-#pragma unroll
-      for (int t = 0; t < TM; ++t) {
-        double v = pa[t][rr];
-        asm volatile("" : "+v"(v));
-        av[set][u][t] = (ok && va[t]) ? v : 0.0;
-      }
-      if constexpr (!SYM) {
-#pragma unroll
-        for (int t = 0; t < TN; ++t) {
-          double v = pb[t][rr];
-          asm volatile("" : "+v"(v));
-          bv[set][u][t] = (ok && vb[t]) ? v : 0.0;
-        }
-      }
-    }
-  };
-  auto multiply = [&](int set) {
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-#pragma unroll
-      for (int a = 0; a < TM; ++a)
-#pragma unroll
-        for (int b = 0; b < TN; ++b) {
-          const double bb = SYM ? av[set][u][b] : bv[set][u][SYM ? 0 : b];
-          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[set][u][a], bb, acc[a][b], 0, 0, 0);
-        }
-  };
-  const long stride = 4L * U;
-  long s0 = wave;
-  if (s0 < nsteps) load(0, s0);
-  for (; s0 < nsteps; s0 += 2 * stride) {
-    if (s0 + stride < nsteps) load(1, s0 + stride);
-    multiply(0);
-    if (s0 + stride >= nsteps) break;
-    if (s0 + 2 * stride < nsteps) load(0, s0 + 2 * stride);
-    multiply(1);
-  }
-  for (int w2 = 0; w2 < 4; ++w2) {
-    if (wave == w2) {
-#pragma unroll
-      for (int a = 0; a < TM; ++a)
-#pragma unroll
-        for (int b = 0; b < TN; ++b)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int row = a * 16 + g + 4 * r, col = b * 16 + m16;
-            if (w2 == 0) red[row + 64 * col] = acc[a][b][r];
-            else red[row + 64 * col] += acc[a][b][r];
-          }
-    }
-    __syncthreads();
-  }
-  for (int e = tid; e < m * n; e += 256) {
-    const int row = e % m, col = e / m;
-    C[row + (long)col * ldc] = red[row + 64 * col];
-  }
-  __syncthreads();
-}
-
-// Dispatch of gemm_tn_direct by tile counts (m, n <= 64).
-// A Gram (A == B, m == n) loads each operand once.
-__device__ inline void gemm_tn_fast(int m, int n, long k, const double* A, long lda,
-                                    const double* B, long ldb, double* C, long ldc, double* red) {
-  const int tm = (m + 15) / 16, tn = (n + 15) / 16;
-  const bool sym = A == B && lda == ldb && m == n;
-  if (sym) {
-    if (tm <= 1) gemm_tn_direct<1, 1, 16, true>(m, n, k, A, lda, B, ldb, C, ldc, red);
-    else if (tm <= 2) gemm_tn_direct<2, 2, 8, true>(m, n, k, A, lda, B, ldb, C, ldc, red);
-    else gemm_tn_direct<4, 4, 4, true>(m, n, k, A, lda, B, ldb, C, ldc, red);
-  } else if (tm <= 1 && tn <= 1) {
-    gemm_tn_direct<1, 1, 16, false>(m, n, k, A, lda, B, ldb, C, ldc, red);
-  } else if (tm <= 2 && tn <= 2) {
-    gemm_tn_direct<2, 2, 8, false>(m, n, k, A, lda, B, ldb, C, ldc, red);
-  } else {
-    gemm_tn_direct<4, 4, 4, false>(m, n, k, A, lda, B, ldb, C, ldc, red);
-  }
-}
 
 // Bs (k x NMAX, row-major: Bs[l NMAX + j]) = B (k x n, ldb) padded with zero
 // columns, for gemm_rows_lds. Ends with a barrier.
@@ -561,7 +333,7 @@ __device__ __forceinline__ bool wave_chol_inv(const double* G, long ldg, int w, 
 
 // CholQR2 of a tall A (m x w, leading dimension m, w <= 32) in global memory
 // by one work-group of 256 threads, into Q (m x w, leading dimension m, not
-// aliasing A): each Gram on the matrix cores (gemm_tn_mfma), its factor by one
+// aliasing A): each Gram on the matrix cores (gemm_tn_fast), its factor by one
 // wave (wave_chol_inv), the rows transformed by gemm_rows_lds (the second
 // pass in place). Returns false, with A untouched, when a Cholesky factor is
 // unusable. LDS: stage (2 kStageDoubles), gm, ri, lc (32·32 each), xb (32),
