@@ -56,6 +56,25 @@ __device__ inline void set_flag(int* state, int flag) {
   if (threadIdx.x == 0) atomicOr(&state[2], flag);
 }
 
+// dst[i] = src[i] * factor for i < len by one work-group, four elements per
+// thread loaded before any is stored (dst may alias src), so the loads of a
+// pass are in flight together.
+__device__ inline void scale_copy(double* dst, const double* src, long long len, double factor) {
+  for (long long i0 = threadIdx.x; i0 < len; i0 += 4 * kWG) {
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long long i = i0 + u * kWG;
+      v[u] = i < len ? src[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long long i = i0 + u * kWG;
+      if (i < len) dst[i] = v[u] * factor;
+    }
+  }
+}
+
 // C (m x n) = A B for a tall A (m x k) and a small B (k x n): B staged in LDS
 // (`stage`, 2 kStageDoubles doubles) and one thread per row of C when n <= 32
 // and k n fits, else the generic work-group product.
@@ -175,9 +194,16 @@ __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits,
   const RefreshStep st = f.steps[step];
   const int n = f.n, p = f.p, q = f.q;
   __shared__ WgScratch sh;
+  __shared__ double red[4 * 64];
   const RefreshLayout L(p, q, st.w, f.A);
   double* proj = f.scratch + L.proj;
   double* row = f.scratch + L.row;
+  // out (m) = Xᵀ v for X (p x m): one thread per row of X when m <= 64
+  auto dots = [&](int m, const double* X, const double* v, double* out) {
+    if (m <= 16) wg::gemm_tv_rows<16>(m, p, X, p, v, out, red);
+    else if (m <= 64) wg::gemm_tv_rows<64>(m, p, X, p, v, out, red);
+    else wg::gemm_tn_long(m, 1, p, X, p, v, p, out, m);
+  };
 
   for (int j = 0; j < st.b; ++j) {
     const int comp = st.a + j;
@@ -191,13 +217,28 @@ __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits,
     }
     const double inv = 1.0 / tn;
     double* Tc = f.T + (long long)comp * n;
-    for (long long i = threadIdx.x; i < n; i += kWG) Tc[i] = t[i] * inv;
+    scale_copy(Tc, t, n, inv);
     double* Rc = f.R + (long long)comp * p;
     double* Vc = f.V + (long long)comp * p;
-    for (int i = threadIdx.x; i < p; i += kWG) {
-      Rc[i] = f.cand[i + (long long)j * p] * inv;
-      const double s = sum_strided(f.ls_part + i + (long long)j * p, f.ls_slab, f.ls_splits);
-      Vc[i] = s / f.sx[i] * inv;
+    scale_copy(Rc, f.cand + (long long)j * p, p, inv);
+    {
+      // loading sums over the row splits: four elements per thread in flight
+      const double* part = f.ls_part + (long long)j * p;
+      for (int i0 = threadIdx.x; i0 < p; i0 += 4 * kWG) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int sp = 0; sp < f.ls_splits; ++sp) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * kWG;
+            if (i < p) acc[u] += part[i + (long long)sp * f.ls_slab];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int i = i0 + u * kWG;
+          if (i < p) Vc[i] = acc[u] / f.sx[i] * inv;
+        }
+      }
     }
     __syncthreads();
     // response loadings q_a = S0ᵀ r (= Ỹᵀ t on the matrix-free route)
@@ -206,17 +247,23 @@ __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits,
         f.Q[c + (long long)comp * q] = f.qraw[c + (long long)j * q] * inv;
       __syncthreads();
     } else {
-      wg::gemm_tn_long(q, 1, p, f.S0, p, Rc, p, f.Q + (long long)comp * q, q);
+      dots(q, f.S0, Rc, f.Q + (long long)comp * q);
     }
     // v = loading - V (Vᵀ loading), once (twice when re-orthogonalising)
     if (comp > 0) {
       for (int pass = 0; pass < (f.reorth ? 2 : 1); ++pass) {
-        wg::gemm_tn_long(comp, 1, p, f.V, p, Vc, p, proj, comp);
-        for (int i = threadIdx.x; i < p; i += kWG) {
-          double s = 0.0;
+        dots(comp, f.V, Vc, proj);
+        for (int i0 = threadIdx.x; i0 < p; i0 += 2 * kWG) {
+          const int i1 = i0 + kWG < p ? i0 + kWG : i0;
+          double s0 = 0.0, s1 = 0.0;
 #pragma unroll 8
-          for (int l = 0; l < comp; ++l) s = fma(f.V[i + (long long)l * p], proj[l], s);
-          Vc[i] -= s;
+          for (int l = 0; l < comp; ++l) {
+            const double pl = proj[l];
+            s0 = fma(f.V[i0 + (long long)l * p], pl, s0);
+            s1 = fma(f.V[i1 + (long long)l * p], pl, s1);
+          }
+          Vc[i0] -= s0;
+          if (i0 + kWG < p) Vc[i1] -= s1;
         }
         __syncthreads();
       }
@@ -228,16 +275,24 @@ __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits,
       return;
     }
     const double vinv = 1.0 / vn;
-    for (int i = threadIdx.x; i < p; i += kWG) Vc[i] *= vinv;
+    scale_copy(Vc, Vc, p, vinv);
     __syncthreads();
     // deflation row = vᵀ S ; S -= v row (the matrix-free route keeps v in V
     // and projects it out of every product instead)
     if (f.S) {
-      wg::gemm_tn_long(q, 1, p, f.S, p, Vc, p, row, q);
+      dots(q, f.S, Vc, row);
+      // each row's columns are loaded (16 at a time) before any is stored
       for (int i = threadIdx.x; i < p; i += kWG) {
         const double v = Vc[i];
-#pragma unroll 4
-        for (int c = 0; c < q; ++c) f.S[i + (long long)c * p] -= v * row[c];
+        for (int c0 = 0; c0 < q; c0 += 16) {
+          double sv[16];
+#pragma unroll
+          for (int u = 0; u < 16; ++u)
+            if (c0 + u < q) sv[u] = f.S[i + (long long)(c0 + u) * p];
+#pragma unroll
+          for (int u = 0; u < 16; ++u)
+            if (c0 + u < q) f.S[i + (long long)(c0 + u) * p] = sv[u] - v * row[c0 + u];
+        }
       }
     }
     if (f.S && f.use_right_gram) {

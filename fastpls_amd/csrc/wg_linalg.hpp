@@ -60,6 +60,176 @@ constexpr int kStageElems = 4096;
 constexpr int kStageDoubles = kStageElems + 128;
 typedef double wg_f64x4 __attribute__((ext_vector_type(4)));
 
+// out (m) = Aᵀ b for a tall A (k x m, lda) and b (k), m <= MMAX, by one
+// work-group of 256 threads: one thread per row of A (two rows per pass, b
+// read once per row), m partial sums per thread, then per-wave sums and the
+// four wave sums added in wave order (fixed order) through `red` (4 MMAX
+// doubles of LDS). Ends with a barrier.
+template<int MMAX>
+__device__ __attribute__((noinline)) void gemm_tv_rows(int m, long k, const double* A, long lda,
+                                                       const double* b, double* out,
+                                                       double* red) {
+  constexpr int NT = 256;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const double FPLS_GLOBAL* Ag = (const double FPLS_GLOBAL*)A;
+  const double FPLS_GLOBAL* bg = (const double FPLS_GLOBAL*)b;
+  double acc[MMAX];
+#pragma unroll
+  for (int i = 0; i < MMAX; ++i) acc[i] = 0.0;
+  for (long l0 = tid; l0 < k; l0 += 2 * NT) {
+    const bool two = l0 + NT < k;
+    const long l1 = two ? l0 + NT : l0;
+    const double b0 = bg[l0];
+    const double b1 = two ? bg[l1] : 0.0;
+    double a0[MMAX], a1[MMAX];
+#pragma unroll
+    for (int i = 0; i < MMAX; ++i) {
+      if (i < m) {
+        a0[i] = Ag[l0 + (long)i * lda];
+        a1[i] = Ag[l1 + (long)i * lda];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < MMAX; ++i)
+      if (i < m) acc[i] = fma(a1[i], b1, fma(a0[i], b0, acc[i]));
+  }
+#pragma unroll
+  for (int i = 0; i < MMAX; ++i) {
+    if (i < m) {
+      const double v = wave_sum(acc[i]);
+      if (lane == 0) red[wave * MMAX + i] = v;
+    }
+  }
+  __syncthreads();
+  if (tid < m) out[tid] = ((red[tid] + red[MMAX + tid]) + red[2 * MMAX + tid]) + red[3 * MMAX + tid];
+  __syncthreads();
+}
+
+// C (m x n, ldc) = Aᵀ B for A (k x m, lda) and B (k x n, ldb), m, n <= 64,
+// with a long inner dimension k, on the FP64 matrix cores by one work-group of
+// 256 threads (16x16x4 FP64 MFMA: lane l supplies column l % 16 of a 16 x 16
+// tile at row l / 16 of each 4-row step; item r of lane l holds tile entry
+// (l/16 + 4r, l % 16)). The 4-row steps of k are dealt to the four waves in
+// turn and the operands are loaded straight from global memory, U steps at a
+// time, the next U while the current ones multiply; TM x TN tiles cover
+// m <= 16 TM, n <= 16 TN; SYM (A == B) loads one operand stream. The wave
+// partials are added in wave order (fixed order) through `red` (4096 doubles
+// of LDS). Ends with a barrier.
+template<int TM, int TN, int U, bool SYM>
+__device__ __attribute__((noinline)) void gemm_tn_direct(int m, int n, long k, const double* A,
+                                                         long lda, const double* B, long ldb,
+                                                         double* C, long ldc, double* red) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m16 = lane & 15, g = lane >> 4;
+  const double FPLS_GLOBAL* pa[TM];
+  const double FPLS_GLOBAL* pb[TN];
+  bool va[TM], vb[TN];
+#pragma unroll
+  for (int t = 0; t < TM; ++t) {
+    const int c = t * 16 + m16;
+    va[t] = c < m;
+    pa[t] = (const double FPLS_GLOBAL*)(A + (long)(va[t] ? c : 0) * lda);
+  }
+#pragma unroll
+  for (int t = 0; t < TN; ++t) {
+    const int c = t * 16 + m16;
+    vb[t] = c < n;
+    pb[t] = (const double FPLS_GLOBAL*)(B + (long)(vb[t] ? c : 0) * ldb);
+  }
+  wg_f64x4 acc[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b) acc[a][b] = wg_f64x4{0.0, 0.0, 0.0, 0.0};
+  // wave w owns the 4-row steps w, w + 4, ...; an iteration covers U of them
+  // and the operands of the next iteration are loaded before the matrix
+  // products of the current one (two register sets)
+  const long nsteps = (k + 3) / 4;
+  double av[2][U][TM], bv[2][U][SYM ? 1 : TN];
+  auto load = [&](int set, long s0) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long row = 4 * (s0 + 4L * u) + g;
+      const bool ok = row < k;
+      const long rr = ok ? row : 0;
+#pragma unroll
+      for (int t = 0; t < TM; ++t) {
+        double v = pa[t][rr];
+        asm volatile("" : "+v"(v));
+        av[set][u][t] = (ok && va[t]) ? v : 0.0;
+      }
+      if constexpr (!SYM) {
+#pragma unroll
+        for (int t = 0; t < TN; ++t) {
+          double v = pb[t][rr];
+          asm volatile("" : "+v"(v));
+          bv[set][u][t] = (ok && vb[t]) ? v : 0.0;
+        }
+      }
+    }
+  };
+  auto multiply = [&](int set) {
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b) {
+          const double bb = SYM ? av[set][u][b] : bv[set][u][SYM ? 0 : b];
+          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[set][u][a], bb, acc[a][b], 0, 0, 0);
+        }
+  };
+  const long stride = 4L * U;
+  long s0 = wave;
+  if (s0 < nsteps) load(0, s0);
+  for (; s0 < nsteps; s0 += 2 * stride) {
+    if (s0 + stride < nsteps) load(1, s0 + stride);
+    multiply(0);
+    if (s0 + stride >= nsteps) break;
+    if (s0 + 2 * stride < nsteps) load(0, s0 + 2 * stride);
+    multiply(1);
+  }
+  for (int w2 = 0; w2 < 4; ++w2) {
+    if (wave == w2) {
+#pragma unroll
+      for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = a * 16 + g + 4 * r, col = b * 16 + m16;
+            if (w2 == 0) red[row + 64 * col] = acc[a][b][r];
+            else red[row + 64 * col] += acc[a][b][r];
+          }
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < m * n; e += 256) {
+    const int row = e % m, col = e / m;
+    C[row + (long)col * ldc] = red[row + 64 * col];
+  }
+  __syncthreads();
+}
+
+// Dispatch of gemm_tn_direct by tile counts (m, n <= 64).
+// A Gram (A == B, m == n) loads each operand once.
+__device__ inline void gemm_tn_fast(int m, int n, long k, const double* A, long lda,
+                                    const double* B, long ldb, double* C, long ldc, double* red) {
+  const int tm = (m + 15) / 16, tn = (n + 15) / 16;
+  const bool sym = A == B && lda == ldb && m == n;
+  if (sym) {
+    if (tm <= 1) gemm_tn_direct<1, 1, 16, true>(m, n, k, A, lda, B, ldb, C, ldc, red);
+    else if (tm <= 2) gemm_tn_direct<2, 2, 8, true>(m, n, k, A, lda, B, ldb, C, ldc, red);
+    else gemm_tn_direct<4, 4, 4, true>(m, n, k, A, lda, B, ldb, C, ldc, red);
+  } else if (tm <= 1 && tn <= 1) {
+    gemm_tn_direct<1, 1, 16, false>(m, n, k, A, lda, B, ldb, C, ldc, red);
+  } else if (tm <= 2 && tn <= 2) {
+    gemm_tn_direct<2, 2, 8, false>(m, n, k, A, lda, B, ldb, C, ldc, red);
+  } else {
+    gemm_tn_direct<4, 4, 4, false>(m, n, k, A, lda, B, ldb, C, ldc, red);
+  }
+}
+
 // Bs (k x NMAX, row-major: Bs[l NMAX + j]) = B (k x n, ldb) padded with zero
 // columns, for gemm_rows_lds. Ends with a barrier.
 template<int NMAX, int NT = kWG>
