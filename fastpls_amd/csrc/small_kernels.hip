@@ -226,7 +226,22 @@ __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits,
       const double* part = f.ls_part + (long long)j * p;
       for (int i0 = threadIdx.x; i0 < p; i0 += 4 * kWG) {
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int sp = 0; sp < f.ls_splits; ++sp) {
+        int sp = 0;
+        for (; sp + 8 <= f.ls_splits; sp += 8) {
+          double v[8][4];
+#pragma unroll
+          for (int e = 0; e < 8; ++e)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const int i = i0 + u * kWG;
+              v[e][u] = i < p ? part[i + (long long)(sp + e) * f.ls_slab] : 0.0;
+            }
+#pragma unroll
+          for (int e = 0; e < 8; ++e)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc[u] += v[e][u];
+        }
+        for (; sp < f.ls_splits; ++sp) {
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             const int i = i0 + u * kWG;
@@ -281,17 +296,27 @@ __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits,
     // and projects it out of every product instead)
     if (f.S) {
       dots(q, f.S, Vc, row);
-      // each row's columns are loaded (16 at a time) before any is stored
-      for (int i = threadIdx.x; i < p; i += kWG) {
-        const double v = Vc[i];
+      // two rows per thread, each row's columns loaded (16 at a time) before
+      // any is stored
+      for (int i0 = threadIdx.x; i0 < p; i0 += 2 * kWG) {
+        const bool two = i0 + kWG < p;
+        const int i1 = two ? i0 + kWG : i0;
+        const double v0 = Vc[i0], v1 = Vc[i1];
         for (int c0 = 0; c0 < q; c0 += 16) {
-          double sv[16];
+          double s0[16], s1[16];
 #pragma unroll
           for (int u = 0; u < 16; ++u)
-            if (c0 + u < q) sv[u] = f.S[i + (long long)(c0 + u) * p];
+            if (c0 + u < q) {
+              s0[u] = f.S[i0 + (long long)(c0 + u) * p];
+              s1[u] = f.S[i1 + (long long)(c0 + u) * p];
+            }
 #pragma unroll
           for (int u = 0; u < 16; ++u)
-            if (c0 + u < q) f.S[i + (long long)(c0 + u) * p] = sv[u] - v * row[c0 + u];
+            if (c0 + u < q) {
+              const double r = row[c0 + u];
+              f.S[i0 + (long long)(c0 + u) * p] = s0[u] - v0 * r;
+              if (two) f.S[i1 + (long long)(c0 + u) * p] = s1[u] - v1 * r;
+            }
         }
       }
     }
