@@ -118,6 +118,88 @@ __device__ inline bool tall_basis(double* sample, int p, int w, double* Qb, Tall
   return wg::householder_qr(sample, p, p, w, Qb, p, sh.tau, sh.dots, sh.red, sh.part);
 }
 
+// Whole CholQR2 of a small A (m x w, m >= w, m·w <= kSmallQr, w <= 32) by one
+// work-group: A and the first-pass basis stay in LDS, wave 0 factors each
+// Gram. When a Cholesky factor is unusable the Householder QR of the original
+// A (kept in LDS) gives Q instead, as in the multi-kernel route.
+constexpr int kSmallQr = 4096;
+
+__device__ inline void lds_gram(const double* X, int m, int w, double* G, int ldg) {
+  for (int e = threadIdx.x; e < w * w; e += kWG) {
+    const int i = e % w, j = e / w;
+    const double* xi = X + (long)i * m;
+    const double* xj = X + (long)j * m;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int l = 0;
+    for (; l + 4 <= m; l += 4) {
+      s0 = fma(xi[l], xj[l], s0);
+      s1 = fma(xi[l + 1], xj[l + 1], s1);
+      s2 = fma(xi[l + 2], xj[l + 2], s2);
+      s3 = fma(xi[l + 3], xj[l + 3], s3);
+    }
+    for (; l < m; ++l) s0 = fma(xi[l], xj[l], s0);
+    G[i + j * ldg] = (s0 + s1) + (s2 + s3);
+  }
+}
+
+// Y (m x w, ld m) = X Rinv for an upper-triangular Rinv (ld ldr).
+__device__ inline void lds_apply_upper(const double* X, int m, int w, const double* Rinv,
+                                       int ldr, double* Y) {
+  for (int e = threadIdx.x; e < m * w; e += kWG) {
+    const int l = e % m, c = e / m;
+    double acc = 0.0;
+    for (int k = 0; k <= c; ++k) acc = fma(X[l + (long)k * m], Rinv[k + c * ldr], acc);
+    Y[e] = acc;
+  }
+}
+
+// Q (m x w, ld m, global) = the orthonormal factor of A (m x w, ld m,
+// global, not modified). la, lq: kSmallQr doubles of LDS each; gm, ri, lc:
+// 32·32, xb 32 doubles, okflag one int of LDS. Returns false on a non-finite
+// column norm (Householder route). Ends with a barrier.
+__device__ __attribute__((noinline)) bool small_qr_lds(const double* A, int m, int w, double* Q,
+                                                       double* la, double* lq, double* gm,
+                                                       double* ri, double* lc, double* xb,
+                                                       int* okflag, WgScratch& sh) {
+  constexpr int W = 32;
+  auto factor = [&](const double* X) __attribute__((always_inline)) -> bool {
+    lds_gram(X, m, w, gm, W);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      const bool good = wg::wave_chol_inv<W>(gm, W, w, ri, W, lc, xb);
+      if (threadIdx.x == 0) *okflag = good ? 1 : 0;
+    }
+    __syncthreads();
+    return *okflag != 0;
+  };
+  for (int e = threadIdx.x; e < m * w; e += kWG) la[e] = A[e];
+  __syncthreads();
+  if (factor(la)) {
+    lds_apply_upper(la, m, w, ri, W, lq);
+    __syncthreads();
+    if (factor(lq)) {
+      lds_apply_upper(lq, m, w, ri, W, Q);
+      __syncthreads();
+      return true;
+    }
+  }
+  const bool fine = wg::householder_qr<kWG>(la, m, m, w, lq, m, sh.tau, sh.dots, sh.red, sh.part);
+  for (int e = threadIdx.x; e < m * w; e += kWG) Q[e] = lq[e];
+  __syncthreads();
+  return fine;
+}
+
+__global__ __launch_bounds__(kWG) void k_cholqr2_small(const double* A, int m, int w, double* Q,
+                                                       int* flags) {
+  constexpr int W = 32;
+  __shared__ double la[kSmallQr], lq[kSmallQr];
+  __shared__ double gm[W * W], ri[W * W], lc[W * W], xb[W];
+  __shared__ WgScratch sh;
+  __shared__ int ok_sh;
+  const bool fine = small_qr_lds(A, m, w, Q, la, lq, gm, ri, lc, xb, &ok_sh, sh);
+  if (!fine && threadIdx.x == 0) atomicOr(flags, kFlagDecomposition);
+}
+
 // ----------------------------------------------------------- SIMPLS refresh
 // simpls.hpp:226-368 (refresh_directions), right-Gram and plain branches.
 __global__ __launch_bounds__(kWG) void k_simpls_refresh(const SimplsFitDev* fits, int count,
@@ -143,15 +225,22 @@ __global__ __launch_bounds__(kWG) void k_simpls_refresh(const SimplsFitDev* fits
   const double* omega = f.omega + st.omega_off;
   bool ok = true;
 
+  // right-hand (q x w) QRs: whole CholQR2 in LDS when small, else Householder
+  auto qside_qr = [&](double* Y, double* Zo) -> bool {
+    if (w <= 32 && q * w <= kSmallQr)
+      return small_qr_lds(Y, q, w, Zo, tq.stage, tq.stage + kSmallQr, tq.gm, tq.ri, tq.lc, tq.xb,
+                          &tq.ok, sh);
+    return wg::householder_qr(Y, q, q, w, Zo, q, sh.tau, sh.dots, sh.red, sh.part);
+  };
   if (f.use_right_gram) {
     wg::copy(q, w, omega, q, Z, q);
     for (int it = 0; it < f.power; ++it) {
       wg::gemm(q, w, q, 1.0, f.G, q, false, Z, q, false, 0.0, Y2, q);
-      ok = wg::householder_qr(Y2, q, q, w, Z, q, sh.tau, sh.dots, sh.red, sh.part) && ok;
+      ok = qside_qr(Y2, Z) && ok;
     }
     if (f.power == 0) {
       wg::copy(q, w, Z, q, Y2, q);
-      ok = wg::householder_qr(Y2, q, q, w, Z, q, sh.tau, sh.dots, sh.red, sh.part) && ok;
+      ok = qside_qr(Y2, Z) && ok;
     }
     tall_times_small(p, w, q, f.S, p, Z, q, sample, p, tq.stage);
   } else {
@@ -159,17 +248,23 @@ __global__ __launch_bounds__(kWG) void k_simpls_refresh(const SimplsFitDev* fits
     for (int it = 0; it < f.power; ++it) {
       ok = tall_basis(sample, p, w, Qb, tq, sh) && ok;
       tall_tn(q, w, p, f.S, p, Qb, p, Y2, q, tq.stage);
-      ok = wg::householder_qr(Y2, q, q, w, Z, q, sh.tau, sh.dots, sh.red, sh.part) && ok;
+      ok = qside_qr(Y2, Z) && ok;
       tall_times_small(p, w, q, f.S, p, Z, q, sample, p, tq.stage);
     }
   }
   ok = tall_basis(sample, p, w, Qb, tq, sh) && ok;
   tall_tn(w, q, p, Qb, p, f.S, p, Bm, w, tq.stage);
-  wg::gemm(w, w, q, 1.0, Bm, w, false, Bm, w, true, 0.0, Gm, w);
-  ok = wg::jacobi_eigen(Gm, Ve, w, ev, sh.cs, sh.sn, sh.pa, sh.pb, sh.red, vtmp) && ok;
+  // eigen of the projected Gram, in LDS when it fits
+  const bool eig_lds = 3 * w * w + w <= 2 * wg::kStageDoubles;
+  double* gE = eig_lds ? tq.stage : Gm;
+  double* vE = eig_lds ? tq.stage + w * w : Ve;
+  double* tE = eig_lds ? tq.stage + 2 * w * w : vtmp;
+  double* eE = eig_lds ? tq.stage + 3 * w * w : ev;
+  wg::gemm(w, w, q, 1.0, Bm, w, false, Bm, w, true, 0.0, gE, w);
+  ok = wg::jacobi_eigen(gE, vE, w, eE, sh.cs, sh.sn, sh.pa, sh.pb, sh.red, tE) && ok;
   // candidates = Qb * (top-b eigenvectors, largest first)
   for (int c = 0; c < b; ++c) {
-    const double* v = Ve + (long long)(w - 1 - c) * w;
+    const double* v = vE + (long long)(w - 1 - c) * w;
     for (int i = threadIdx.x; i < p; i += kWG) {
       double acc = 0.0;
 #pragma unroll 8
@@ -445,74 +540,6 @@ __global__ __launch_bounds__(kWG) void k_chol_inv(const double* G, int w, double
     const int i = e % w, c = e / w;
     Rinv[e] = i <= c ? Ri[i + c * 65] / Lm[i + i * 65] : 0.0;
   }
-}
-
-// Whole CholQR2 of a small A (m x w, m >= w, m·w <= kSmallQr, w <= 32) in one
-// work-group: A and the first-pass basis stay in LDS, wave 0 factors each
-// Gram. When a Cholesky factor is unusable the Householder QR of the original
-// A (kept in LDS) gives Q instead, as in the multi-kernel route.
-constexpr int kSmallQr = 4096;
-
-__device__ inline void lds_gram(const double* X, int m, int w, double* G, int ldg) {
-  for (int e = threadIdx.x; e < w * w; e += kWG) {
-    const int i = e % w, j = e / w;
-    const double* xi = X + (long)i * m;
-    const double* xj = X + (long)j * m;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int l = 0;
-    for (; l + 4 <= m; l += 4) {
-      s0 = fma(xi[l], xj[l], s0);
-      s1 = fma(xi[l + 1], xj[l + 1], s1);
-      s2 = fma(xi[l + 2], xj[l + 2], s2);
-      s3 = fma(xi[l + 3], xj[l + 3], s3);
-    }
-    for (; l < m; ++l) s0 = fma(xi[l], xj[l], s0);
-    G[i + j * ldg] = (s0 + s1) + (s2 + s3);
-  }
-}
-
-// Y (m x w, ld m) = X Rinv for an upper-triangular Rinv (ld ldr).
-__device__ inline void lds_apply_upper(const double* X, int m, int w, const double* Rinv,
-                                       int ldr, double* Y) {
-  for (int e = threadIdx.x; e < m * w; e += kWG) {
-    const int l = e % m, c = e / m;
-    double acc = 0.0;
-    for (int k = 0; k <= c; ++k) acc = fma(X[l + (long)k * m], Rinv[k + c * ldr], acc);
-    Y[e] = acc;
-  }
-}
-
-__global__ __launch_bounds__(kWG) void k_cholqr2_small(const double* A, int m, int w, double* Q,
-                                                       int* flags) {
-  constexpr int W = 32;
-  __shared__ double la[kSmallQr], lq[kSmallQr];
-  __shared__ double gm[W * W], ri[W * W], lc[W * W], xb[W];
-  __shared__ WgScratch sh;
-  __shared__ int ok_sh;
-  // Gram of X in gm, then wave 0 factors it into ri; true when usable
-  auto factor = [&](const double* X) __attribute__((always_inline)) -> bool {
-    lds_gram(X, m, w, gm, W);
-    __syncthreads();
-    if (threadIdx.x < 64) {
-      const bool good = wg::wave_chol_inv<W>(gm, W, w, ri, W, lc, xb);
-      if (threadIdx.x == 0) ok_sh = good ? 1 : 0;
-    }
-    __syncthreads();
-    return ok_sh != 0;
-  };
-  for (int e = threadIdx.x; e < m * w; e += kWG) la[e] = A[e];
-  __syncthreads();
-  if (factor(la)) {
-    lds_apply_upper(la, m, w, ri, W, lq);
-    __syncthreads();
-    if (factor(lq)) {
-      lds_apply_upper(lq, m, w, ri, W, Q);
-      return;
-    }
-  }
-  const bool fine = wg::householder_qr<kWG>(la, m, m, w, lq, m, sh.tau, sh.dots, sh.red, sh.part);
-  for (int e = threadIdx.x; e < m * w; e += kWG) Q[e] = lq[e];
-  if (!fine && threadIdx.x == 0) atomicOr(flags, kFlagDecomposition);
 }
 
 // Householder recomputation of Q when CholQR2 failed; a no-op otherwise.
