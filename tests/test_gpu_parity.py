@@ -92,6 +92,37 @@ def test_simpls_c1_workload(hip, oracle):
         m.close()
 
 
+@pytest.mark.parametrize("precision", [64, 32])
+def test_simpls_midsize_refresh(hip, oracle, precision):
+    """p=5000, q=14, 8 components (block 1): the refresh's tall CholQR2 of the
+    5000 x 11 sketch (matrix-core Grams), the right-hand CholQR2 and the
+    projected-Gram eigen in LDS, the row-parallel update, against the oracle
+    on the same values."""
+    X, Y = latent_data(1500, 5000, 14, factors=12, noise=0.1, seed=21, decay=0.8)
+    dt = np.float64 if precision == 64 else np.float32
+    Xp, Yp = X.astype(dt), Y.astype(dt)
+    m = fit(hip, Xp, Yp, precision=precision, components=8, oversample=10, power=2, seed=3)
+    ref = oracle.simpls_fit(Xp.astype(np.float64), Yp.astype(np.float64), ncomp=8, oversample=10,
+                            power=2, seed=3)
+    for k in (1, 4, 8):
+        assert rel(hip_coef(m, k), coef(ref, k)) < TOL[precision], k
+    m.close()
+
+
+def test_simpls_rank_deficient_response(hip, oracle):
+    """Y of rank 2 in 12 columns: the refresh sketches (w = 11) have rank 2, so
+    both CholQR2 factorisations break down and the Householder QRs take over
+    (right-hand in LDS, tall in global memory); the two leading components
+    against the oracle."""
+    X, Y0 = latent_data(800, 3000, 2, factors=6, noise=0.1, seed=22)
+    Y = Y0 @ np.random.default_rng(22).standard_normal((2, 12))
+    m = fit(hip, X, Y, components=2, oversample=10, power=2, seed=5)
+    ref = oracle.simpls_fit(X, Y, ncomp=2, oversample=10, power=2, seed=5)
+    for k in (1, 2):
+        assert rel(hip_coef(m, k), coef(ref, k)) < 1e-8, k
+    m.close()
+
+
 def test_simpls_block_routing_classification(hip, oracle):
     """n*p*K >= 5e8 routes classification to a candidate block (simpls.hpp:43-62)."""
     X, lab = class_data(10000, 2500, 20, latent=15, seed=2)
