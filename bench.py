@@ -346,6 +346,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=10000)
     args = ap.parse_args()
+    if args.steps < 1 or args.plain_steps < 1:
+        ap.error("--steps and --plain-steps need at least one fit (use --no-plain to skip the FP64 line)")
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(independent fits would shard one scheduler per GPU).", file=sys.stderr)

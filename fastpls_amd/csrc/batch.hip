@@ -62,6 +62,11 @@ class SimplsBatch {
   // right-hand chunk of the per-step score pass (FP32 matrix-core launches
   // take at most 32 columns)
   static constexpr int kXwChunk = sizeof(T) == 4 ? 32 : 64;
+  // longest row split of the cross-product and loading passes and longest
+  // column split of the score pass (both within the matrix-core kernels'
+  // limits, kXtwSplitRows and kXwSplitCols)
+  static constexpr long long kStepRows = 2048;
+  static constexpr long long kStepCols = 1024;
 
  public:
   SimplsBatch(const std::vector<GroupIn>& groups, const std::vector<FitIn>& fits, int scaling,
@@ -168,10 +173,9 @@ class SimplsBatch {
   struct Fit {
     int q, A, bmax, wmax, nsteps, off, boff;
     bool right_gram;
-    double *my, *G, *R, *Q, *V, *Tsc, *Rs, *scratch, *center;
+    double *my, *G, *R, *Q, *V, *Tsc, *Rs, *scratch;
     int* state;
     int* counts;
-    T* yperm;
     int* lperm;
     RefreshStep* steps;
     double* omega;
@@ -249,11 +253,14 @@ class SimplsBatch {
     for (Group& G : gplan_) {
       long long s = (2048 + tiles1 - 1) / std::max<long long>(tiles1, 1);
       s = std::max<long long>(1, std::min<long long>(s, G.n / 256));
-      s = std::max<long long>(s, (G.n + kXtwSplitRows - 1) / kXtwSplitRows);
+      // every work-group of a pass streams at most kStepRows rows (the
+      // cross-product and loading passes) or kStepCols columns (the score
+      // pass) of its group, so work-groups of all groups are alike in size and
+      // a launch over groups of very different shapes has no long tail
+      s = std::max<long long>(s, (G.n + kStepRows - 1) / kStepRows);
       G.splits = int(std::min<long long>(s, 4096));
-      const bool mfma = std::min(G.bcat, 64) >= kMfmaMinN;
-      G.ls_splits = mfma ? int((G.n + kXtwSplitRows - 1) / kXtwSplitRows) : 1;
-      G.xs_splits = (mfma && sizeof(T) == 4) ? int((G.p + kXwSplitCols - 1) / kXwSplitCols) : 1;
+      G.ls_splits = int(std::min<long long>(4096, (G.n + kStepRows - 1) / kStepRows));
+      G.xs_splits = int(std::min<long long>(4096, (G.p + kStepCols - 1) / kStepCols));
     }
     // arena layout
     constexpr int kGo = 16, kFo = 15;
@@ -275,7 +282,7 @@ class SimplsBatch {
       o[10] = arena_.add(8 * p * B);                   // cand_scaled
       o[11] = arena_.add(8 * n * B);                   // t_raw
       o[12] = arena_.add(8 * (size_t)G.ls_splits * p * B);  // loads (split slabs)
-      o[13] = arena_.add(sizeof(int) * 4);             // flags
+      o[13] = up_.add(sizeof(int) * 4);                // flags (uploaded zero)
       o[14] = G.xs_splits > 1 ? arena_.add(8 * (size_t)G.xs_splits * n * B) : 0;  // score slabs
       // operand workspaces: one per right-hand chunk of each pass
       const size_t xw_chunks = (B + kXwChunk - 1) / kXwChunk, xt_chunks = (B + 63) / 64;
@@ -298,16 +305,22 @@ class SimplsBatch {
       o[5] = arena_.add(8 * n * A);                               // T
       o[6] = arena_.add(8 * p * A);                               // Rs
       o[7] = arena_.add(8 * (size_t)simpls_scratch_doubles(G.p, F.q, F.wmax));
-      o[8] = arena_.add(sizeof(int) * 4);                         // state
+      o[8] = up_.add(sizeof(int) * 4);                            // state (uploaded zero)
       o[9] = arena_.add(sizeof(int) * q);                         // counts
-      o[10] = fits_[f].perm ? arena_.add(sizeof(T) * n * q) : 0;  // permuted responses
-      o[11] = fits_[f].perm ? arena_.add(sizeof(int) * n) : 0;    // permuted labels
-      o[12] = arena_.add(sizeof(RefreshStep) * F.host_steps.size());
-      o[13] = arena_.add(8 * F.host_omega.size());
-      o[14] = arena_.add(8 * (size_t)kCenterScratch * q);        // centring scratch
+      o[11] = fits_[f].perm && fits_[f].labels ? arena_.add(sizeof(int) * n) : 0;  // permuted labels
+      o[12] = up_.add(sizeof(RefreshStep) * F.host_steps.size());  // uploaded
+      o[13] = up_.add(8 * F.host_omega.size());                    // uploaded
     }
-    devfits_off_ = arena_.add(sizeof(SimplsFitDev) * plan_.size());
+    devfits_off_ = up_.add(sizeof(SimplsFitDev) * plan_.size());
+    center_count_ = 0;
+    for (size_t f = 0; f < plan_.size(); ++f)
+      if (!fits_[f].labels) center_count_ += plan_[f].q;
+    const size_t center_off = up_.add(sizeof(CenterColumn<T>) * (size_t)center_count_);
     arena_.allocate();
+    up_.allocate();
+    // host image of the uploaded pieces (states and flags start at zero):
+    // one copy for the whole batch
+    std::vector<char> image(up_.bytes(), 0);
     std::vector<char*> frag_cursor(gplan_.size());
     for (size_t g = 0; g < gplan_.size(); ++g) {
       Group& G = gplan_[g];
@@ -318,7 +331,7 @@ class SimplsBatch {
       G.colss = arena_.at<double>(o[6]); G.stats = arena_.at<double>(o[7]);
       G.spart = arena_.at<double>(o[8]); G.cand = arena_.at<double>(o[9]);
       G.cand_scaled = arena_.at<double>(o[10]); G.t_raw = arena_.at<double>(o[11]);
-      G.loads = arena_.at<double>(o[12]); G.flags = arena_.at<int>(o[13]);
+      G.loads = arena_.at<double>(o[12]); G.flags = up_.at<int>(o[13]);
       G.tpart = G.xs_splits > 1 ? arena_.at<double>(o[14]) : nullptr;
       frag_cursor[g] = arena_.at<char>(o[15]);
     }
@@ -337,16 +350,12 @@ class SimplsBatch {
       F.R = arena_.at<double>(o[2]); F.Q = arena_.at<double>(o[3]);
       F.V = arena_.at<double>(o[4]); F.Tsc = arena_.at<double>(o[5]);
       F.Rs = arena_.at<double>(o[6]); F.scratch = arena_.at<double>(o[7]);
-      F.state = arena_.at<int>(o[8]); F.counts = arena_.at<int>(o[9]);
-      F.yperm = fits_[f].perm ? arena_.at<T>(o[10]) : nullptr;
-      F.lperm = fits_[f].perm ? arena_.at<int>(o[11]) : nullptr;
-      F.steps = arena_.at<RefreshStep>(o[12]);
-      F.omega = arena_.at<double>(o[13]);
-      F.center = arena_.at<double>(o[14]);
-      FPLS_HIP(hipMemcpy(F.steps, F.host_steps.data(), sizeof(RefreshStep) * F.host_steps.size(),
-                         hipMemcpyHostToDevice));
-      FPLS_HIP(hipMemcpy(F.omega, F.host_omega.data(), sizeof(double) * F.host_omega.size(),
-                         hipMemcpyHostToDevice));
+      F.state = up_.at<int>(o[8]); F.counts = arena_.at<int>(o[9]);
+      F.lperm = fits_[f].perm && fits_[f].labels ? arena_.at<int>(o[11]) : nullptr;
+      F.steps = up_.at<RefreshStep>(o[12]);
+      F.omega = up_.at<double>(o[13]);
+      std::memcpy(image.data() + o[12], F.host_steps.data(), sizeof(RefreshStep) * F.host_steps.size());
+      std::memcpy(image.data() + o[13], F.host_omega.data(), sizeof(double) * F.host_omega.size());
       SimplsFitDev& d = dev[f];
       d.n = G.n; d.p = G.p; d.q = F.q; d.A = F.A;
       d.power = power_; d.use_right_gram = F.right_gram ? 1 : 0;
@@ -365,12 +374,26 @@ class SimplsBatch {
       d.R = F.R; d.Q = F.Q; d.V = F.V; d.T = F.Tsc;
       d.scratch = F.scratch; d.state = F.state;
     }
-    devfits_ = arena_.at<SimplsFitDev>(devfits_off_);
-    FPLS_HIP(hipMemcpy(devfits_, dev.data(), sizeof(SimplsFitDev) * dev.size(), hipMemcpyHostToDevice));
-    // zero states and flags
-    std::vector<int> zeros(4, 0);
-    for (Fit& F : plan_) FPLS_HIP(hipMemcpy(F.state, zeros.data(), sizeof(int) * 4, hipMemcpyHostToDevice));
-    for (Group& G : gplan_) FPLS_HIP(hipMemcpy(G.flags, zeros.data(), sizeof(int) * 4, hipMemcpyHostToDevice));
+    devfits_ = up_.at<SimplsFitDev>(devfits_off_);
+    std::memcpy(image.data() + devfits_off_, dev.data(), sizeof(SimplsFitDev) * dev.size());
+    // response columns to centre: column c of fit f goes to column off + c of
+    // its group's centred block
+    center_list_ = up_.at<CenterColumn<T>>(center_off);
+    CenterColumn<T>* cl = reinterpret_cast<CenterColumn<T>*>(image.data() + center_off);
+    for (size_t f = 0; f < plan_.size(); ++f) {
+      if (fits_[f].labels) continue;
+      const Fit& F = plan_[f];
+      const Group& G = gplan_[fits_[f].group];
+      for (int c = 0; c < F.q; ++c, ++cl) {
+        cl->y = static_cast<const T*>(fits_[f].y) + (size_t)c * G.n;
+        cl->perm = fits_[f].perm;
+        cl->n = G.n;
+        cl->yc = G.Yc + (size_t)(F.off + c) * G.n;
+        cl->mean = F.my + c;
+        cl->ysum = G.ysum + F.off + c;
+      }
+    }
+    FPLS_HIP(hipMemcpy(up_.at<char>(0), image.data(), image.size(), hipMemcpyHostToDevice));
     // task lists of the per-step passes (identical for every step), in group
     // order; prefix bookkeeping for the active groups of each step
     std::vector<XwTask<T>> xw;
@@ -498,15 +521,10 @@ class SimplsBatch {
         }
         launch_label_response(lab, G.n, F.q, F.counts, F.my, yc, nullptr, G.ysum + F.off,
                               F.state + 2, stream_);
-      } else {
-        const T* y = static_cast<const T*>(in.y);
-        if (in.perm) {
-          launch_gather_rows<T>(y, G.n, F.q, in.perm, G.n, F.yperm, stream_);
-          y = F.yperm;
-        }
-        launch_center_response<T>(y, G.n, F.q, F.my, yc, G.ysum + F.off, F.center, stream_);
       }
     }
+    // every response column of every fit: one launch
+    launch_center_columns<T>(center_list_, center_count_, stream_);
     // one launch: column statistics + cross-products of every group
     launch_xtw<T>(pre_dev_.as<XtwTask<T>>(), int(pre_.size()), pre_tiles_, pre_maxn_, stream_);
     for (Group& G : gplan_)
@@ -584,6 +602,9 @@ class SimplsBatch {
   int scaling_, oversample_, power_, reorth_;
   int max_steps_ = 0;
   Arena arena_;
+  Arena up_;  // pieces filled from the host: schedules, sketches, fit descriptors
+  CenterColumn<T>* center_list_ = nullptr;
+  int center_count_ = 0;
   size_t devfits_off_ = 0;
   SimplsFitDev* devfits_ = nullptr;
   Buf xw_tasks_, xt_tasks_, sums_, pre_dev_, proj_frag_;

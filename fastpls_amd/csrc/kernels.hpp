@@ -104,6 +104,20 @@ constexpr int kCenterScratch = 128;
 template<class T>
 void launch_center_response(const T* Y, long long n, int q, double* mean, double* Yc,
                             double* ysum, double* scratch, hipStream_t s);
+// The same for many response columns in one launch (one work-group per
+// column): column `y` (n values, read through `perm` when it is not null) ->
+// *mean, yc (n doubles) and *ysum.
+template<class T>
+struct CenterColumn {
+  const T* y;
+  const int* perm;
+  long long n;
+  double* yc;
+  double* mean;
+  double* ysum;
+};
+template<class T>
+void launch_center_columns(const CenterColumn<T>* device_list, int count, hipStream_t s);
 // Labels (one-based) -> class counts, priors, centred one-hot Yc, plain
 // one-hot H (may be null) and ysum; invalid labels / empty classes set flags.
 void launch_label_response(const int* labels, long long n, int K, int* counts,

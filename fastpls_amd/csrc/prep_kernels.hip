@@ -66,6 +66,30 @@ __global__ __launch_bounds__(kWG) void k_center_fold(const double* part, const d
   ysum[c] = cs;
 }
 
+// One response column per work-group: the mean, then the centred copy and
+// its sum (a second read of the column, from cache).
+template<class T>
+__global__ __launch_bounds__(kWG) void k_center_columns(const CenterColumn<T>* list) {
+  __shared__ double red[kWaves];
+  const CenterColumn<T> c = list[blockIdx.x];
+  double s = 0.0;
+#pragma unroll 4
+  for (long long i = threadIdx.x; i < c.n; i += kWG) s += double(c.y[c.perm ? c.perm[i] : i]);
+  const double mu = wg_sum(s, red) / double(c.n);
+  double cs = 0.0;
+#pragma unroll 4
+  for (long long i = threadIdx.x; i < c.n; i += kWG) {
+    const double v = double(c.y[c.perm ? c.perm[i] : i]) - mu;
+    c.yc[i] = v;
+    cs += v;
+  }
+  cs = wg_sum(cs, red);
+  if (threadIdx.x == 0) {
+    *c.mean = mu;
+    *c.ysum = cs;
+  }
+}
+
 __global__ __launch_bounds__(kWG) void k_fill_int(int* p, int count, int value) {
   for (int i = blockIdx.x * kWG + threadIdx.x; i < count; i += gridDim.x * kWG) p[i] = value;
 }
@@ -398,6 +422,13 @@ void launch_center_response(const T* Y, long long n, int q, double* mean, double
   FPLS_LAUNCHED("k_center_fold");
 }
 
+template<class T>
+void launch_center_columns(const CenterColumn<T>* device_list, int count, hipStream_t s) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL((k_center_columns<T>), dim3(unsigned(count)), dim3(kWG), 0, s, device_list);
+  FPLS_LAUNCHED("k_center_columns");
+}
+
 void launch_axpy(double alpha, const double* x, double* y, long long count, hipStream_t s) {
   if (count <= 0) return;
   hipLaunchKernelGGL(k_axpy, dim3(grid_for(count)), dim3(kWG), 0, s, alpha, x, y, count);
@@ -540,6 +571,7 @@ void launch_convert(const double* in, T* out, long long count, hipStream_t s) {
 #define FPLS_PREP_INSTANCES(T)                                                              \
   template void launch_center_response<T>(const T*, long long, int, double*, double*,        \
                                           double*, double*, hipStream_t);                    \
+  template void launch_center_columns<T>(const CenterColumn<T>*, int, hipStream_t);          \
   template void launch_finalize_crossprod<T>(const T*, long long, int, int, int,             \
                                              const double*, const double*, int,              \
                                              const double*, double*, double*, double*,       \
