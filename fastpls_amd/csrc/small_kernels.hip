@@ -82,7 +82,10 @@ __device__ inline void tall_times_small(long m, int n, int k, const double* A, l
                                         const double* B, long ldb, double* C, long ldc,
                                         double* stage) {
   constexpr int kCap = 2 * wg::kStageDoubles;
-  if (n <= 16 && k * 16 <= kCap) {
+  if (n <= 8 && k * 8 <= kCap) {
+    wg::stage_rows<8>(B, ldb, k, n, stage);
+    wg::gemm_rows_lds<8>(m, n, k, A, lda, stage, C, ldc);
+  } else if (n <= 16 && k * 16 <= kCap) {
     wg::stage_rows<16>(B, ldb, k, n, stage);
     wg::gemm_rows_lds<16>(m, n, k, A, lda, stage, C, ldc);
   } else if (n <= 32 && k * 32 <= kCap) {

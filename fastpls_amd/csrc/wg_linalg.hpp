@@ -141,16 +141,20 @@ __device__ __attribute__((noinline)) void gemm_tn_direct(int m, int n, long k, c
   for (int a = 0; a < TM; ++a)
 #pragma unroll
     for (int b = 0; b < TN; ++b) acc[a][b] = wg_f64x4{0.0, 0.0, 0.0, 0.0};
-  // wave w owns the 4-row steps w, w + 4, ...; an iteration covers U of them
-  // and the operands of the next iteration are loaded before the matrix
-  // products of the current one (two register sets)
+  // wave w owns a contiguous quarter of the 4-row steps (so its loads use
+  // whole cache lines); an iteration covers U consecutive steps and the
+  // operands of the next iteration are loaded before the matrix products of
+  // the current one (two register sets)
   const long nsteps = (k + 3) / 4;
+  const long quarter = (nsteps + 3) / 4;
+  const long sb = wave * quarter;
+  const long se = sb + quarter < nsteps ? sb + quarter : nsteps;
   double av[2][U][TM], bv[2][U][SYM ? 1 : TN];
   auto load = [&](int set, long s0) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const long row = 4 * (s0 + 4L * u) + g;
-      const bool ok = row < k;
+      const long row = 4 * (s0 + u) + g;
+      const bool ok = s0 + u < se && row < k;
       const long rr = ok ? row : 0;
 #pragma unroll
       for (int t = 0; t < TM; ++t) {
@@ -179,14 +183,13 @@ __device__ __attribute__((noinline)) void gemm_tn_direct(int m, int n, long k, c
           acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[set][u][a], bb, acc[a][b], 0, 0, 0);
         }
   };
-  const long stride = 4L * U;
-  long s0 = wave;
-  if (s0 < nsteps) load(0, s0);
-  for (; s0 < nsteps; s0 += 2 * stride) {
-    if (s0 + stride < nsteps) load(1, s0 + stride);
+  long s0 = sb;
+  if (s0 < se) load(0, s0);
+  for (; s0 < se; s0 += 2 * U) {
+    if (s0 + U < se) load(1, s0 + U);
     multiply(0);
-    if (s0 + stride >= nsteps) break;
-    if (s0 + 2 * stride < nsteps) load(0, s0 + 2 * stride);
+    if (s0 + U >= se) break;
+    if (s0 + 2 * U < se) load(0, s0 + 2 * U);
     multiply(1);
   }
   for (int w2 = 0; w2 < 4; ++w2) {
@@ -218,11 +221,11 @@ __device__ inline void gemm_tn_fast(int m, int n, long k, const double* A, long 
   const int tm = (m + 15) / 16, tn = (n + 15) / 16;
   const bool sym = A == B && lda == ldb && m == n;
   if (sym) {
-    if (tm <= 1) gemm_tn_direct<1, 1, 16, true>(m, n, k, A, lda, B, ldb, C, ldc, red);
+    if (tm <= 1) gemm_tn_direct<1, 1, 32, true>(m, n, k, A, lda, B, ldb, C, ldc, red);
     else if (tm <= 2) gemm_tn_direct<2, 2, 8, true>(m, n, k, A, lda, B, ldb, C, ldc, red);
     else gemm_tn_direct<4, 4, 4, true>(m, n, k, A, lda, B, ldb, C, ldc, red);
   } else if (tm <= 1 && tn <= 1) {
-    gemm_tn_direct<1, 1, 16, false>(m, n, k, A, lda, B, ldb, C, ldc, red);
+    gemm_tn_direct<1, 1, 24, false>(m, n, k, A, lda, B, ldb, C, ldc, red);
   } else if (tm <= 2 && tn <= 2) {
     gemm_tn_direct<2, 2, 8, false>(m, n, k, A, lda, B, ldb, C, ldc, red);
   } else {
@@ -242,52 +245,60 @@ __device__ inline void stage_rows(const double* B, long ldb, int k, int n, doubl
 }
 
 // C (m x n, ldc) = A B for a tall A (m x k, lda) in global memory and B
-// staged by stage_rows in LDS (n <= NMAX): one thread per row of C (two rows
-// per pass), B read as LDS broadcasts, no data-dependent branches.
-// A row is read in full before it is written, so C may be A itself (ldc ==
-// lda, n <= k). Ends with a barrier.
+// staged by stage_rows in LDS (n <= NMAX): one thread per row of C (R rows
+// per pass: 4 for NMAX <= 16, else 2), B read as LDS broadcasts, no
+// data-dependent branches. A row is read in full before it is written, so C
+// may be A itself (ldc == lda, n <= k). Ends with a barrier.
 template<int NMAX, int NT = kWG>
 __device__ __attribute__((noinline)) void gemm_rows_lds(long m, int n, int k, const double* A,
                                                         long lda, const double* Bs, double* C,
                                                         long ldc) {
-  // two rows per thread and pass (the second clamped to a valid row and not
-  // written when past m), so twice the loads are in flight
+  // rows past m are clamped to the pass's first row and not written, so
+  // every row's loads are in flight together
+  constexpr int R = NMAX <= 16 ? 4 : 2;
   const double FPLS_GLOBAL* Ag = (const double FPLS_GLOBAL*)A;
-  for (long i0 = threadIdx.x; i0 < m; i0 += 2 * NT) {
-    const long i1 = i0 + NT < m ? i0 + NT : i0;
-    double acc0[NMAX], acc1[NMAX];
+  for (long i0 = threadIdx.x; i0 < m; i0 += R * NT) {
+    long ir[R];
 #pragma unroll
-    for (int j = 0; j < NMAX; ++j) { acc0[j] = 0.0; acc1[j] = 0.0; }
+    for (int r = 0; r < R; ++r) ir[r] = i0 + r * NT < m ? i0 + r * NT : i0;
+    double acc[R][NMAX];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int j = 0; j < NMAX; ++j) acc[r][j] = 0.0;
     int l = 0;
     for (; l + 4 <= k; l += 4) {
-      double a0[4], a1[4];
+      double a[R][4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        a0[u] = Ag[i0 + (long)(l + u) * lda];
-        a1[u] = Ag[i1 + (long)(l + u) * lda];
-      }
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[r][u] = Ag[ir[r] + (long)(l + u) * lda];
 #pragma unroll
       for (int u = 0; u < 4; ++u)
 #pragma unroll
         for (int j = 0; j < NMAX; ++j) {
           const double bv = Bs[(l + u) * NMAX + j];
-          acc0[j] = fma(a0[u], bv, acc0[j]);
-          acc1[j] = fma(a1[u], bv, acc1[j]);
+#pragma unroll
+          for (int r = 0; r < R; ++r) acc[r][j] = fma(a[r][u], bv, acc[r][j]);
         }
     }
     for (; l < k; ++l) {
-      const double a0 = Ag[i0 + (long)l * lda], a1 = Ag[i1 + (long)l * lda];
+      double a[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) a[r] = Ag[ir[r] + (long)l * lda];
 #pragma unroll
       for (int j = 0; j < NMAX; ++j) {
-        acc0[j] = fma(a0, Bs[l * NMAX + j], acc0[j]);
-        acc1[j] = fma(a1, Bs[l * NMAX + j], acc1[j]);
+        const double bv = Bs[l * NMAX + j];
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r][j] = fma(a[r], bv, acc[r][j]);
       }
     }
 #pragma unroll
     for (int j = 0; j < NMAX; ++j)
       if (j < n) {
-        C[i0 + (long)j * ldc] = acc0[j];
-        if (i0 + NT < m) C[i1 + (long)j * ldc] = acc1[j];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (r == 0 || i0 + r * NT < m) C[ir[r] + (long)j * ldc] = acc[r][j];
       }
   }
   __syncthreads();
@@ -520,8 +531,17 @@ __device__ __attribute__((noinline)) bool cholqr2_tall(const double* A, long m, 
     }
     __syncthreads();
     if (*okflag == 0) return false;
-    stage_rows<32, kWG>(ri, 32, w, w, gm);
-    gemm_rows_lds<32, kWG>(m, w, w, X, m, gm, Q, m);
+    // Q = X R⁻¹ with the narrowest staged width that holds w columns
+    if (w <= 8) {
+      stage_rows<8, kWG>(ri, 32, w, w, gm);
+      gemm_rows_lds<8, kWG>(m, w, w, X, m, gm, Q, m);
+    } else if (w <= 16) {
+      stage_rows<16, kWG>(ri, 32, w, w, gm);
+      gemm_rows_lds<16, kWG>(m, w, w, X, m, gm, Q, m);
+    } else {
+      stage_rows<32, kWG>(ri, 32, w, w, gm);
+      gemm_rows_lds<32, kWG>(m, w, w, X, m, gm, Q, m);
+    }
   }
   return true;
 }
