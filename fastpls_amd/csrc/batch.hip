@@ -21,6 +21,7 @@
 #include <limits>
 #include <memory>
 #include <stdexcept>
+#include <thread>
 #include <vector>
 
 namespace fpls {
@@ -187,6 +188,31 @@ class SimplsBatch {
     if (f < 0 || f >= count()) throw std::invalid_argument("fit index outside the batch");
   }
 
+  // The Gaussian sketch of every refresh step (simpls.hpp:244-249, one seed
+  // per fit and step), drawn on up to 8 host threads: each draw is
+  // independent, so the values do not depend on the thread count.
+  void draw_sketches() {
+    const size_t nf = plan_.size();
+    auto work = [&](size_t first, size_t stride) {
+      for (size_t f = first; f < nf; f += stride)
+        for (const RefreshStep& st : plan_[f].host_steps)
+          gaussian(unsigned(fits_[f].seed) + unsigned(st.a), (size_t)plan_[f].q * st.w,
+                   plan_[f].host_omega.data() + st.omega_off);
+    };
+    const size_t hw = std::max<size_t>(1, std::min<size_t>(8, std::thread::hardware_concurrency()));
+    const size_t nt = std::min(hw, nf);
+    std::vector<std::thread> pool;
+    size_t started = 1;
+    try {
+      for (; started < nt; ++started) pool.emplace_back(work, started, nt);
+    } catch (...) {
+      // fewer threads than asked for: this thread draws the rest
+    }
+    work(0, nt);
+    for (std::thread& t : pool) t.join();
+    for (size_t t = started; t < nt; ++t) work(t, nt);
+  }
+
   // Sizes, schedules and one arena for everything.
   void plan() {
     gplan_.resize(groups_.size());
@@ -220,7 +246,6 @@ class SimplsBatch {
         st.a = a; st.b = b; st.w = w; st.omega_off = (long long)F.host_omega.size();
         const size_t base = F.host_omega.size();
         F.host_omega.resize(base + (size_t)F.q * w);
-        gaussian(unsigned(in.seed) + unsigned(a), (size_t)F.q * w, F.host_omega.data() + base);
         F.host_steps.push_back(st);
         F.bmax = std::max(F.bmax, b);
         F.wmax = std::max(F.wmax, w);
@@ -235,6 +260,7 @@ class SimplsBatch {
       G.max_steps = std::max(G.max_steps, F.nsteps);
       max_steps_ = std::max(max_steps_, F.nsteps);
     }
+    draw_sketches();
     // groups in decreasing order of component steps: the groups still active
     // at step s are a prefix of this order
     order_.resize(gplan_.size());
@@ -262,7 +288,8 @@ class SimplsBatch {
       G.ls_splits = int(std::min<long long>(4096, (G.n + kStepRows - 1) / kStepRows));
       G.xs_splits = int(std::min<long long>(4096, (G.p + kStepCols - 1) / kStepCols));
     }
-    // arena layout
+    // arena layout; the status words (zero at upload) lead the uploaded arena
+    status_off_ = up_.add(sizeof(int) * 4 * (gplan_.size() + plan_.size()));
     constexpr int kGo = 16, kFo = 15;
     std::vector<size_t> go(gplan_.size() * kGo), fo(plan_.size() * kFo);
     for (size_t g = 0; g < gplan_.size(); ++g) {
@@ -282,7 +309,7 @@ class SimplsBatch {
       o[10] = arena_.add(8 * p * B);                   // cand_scaled
       o[11] = arena_.add(8 * n * B);                   // t_raw
       o[12] = arena_.add(8 * (size_t)G.ls_splits * p * B);  // loads (split slabs)
-      o[13] = up_.add(sizeof(int) * 4);                // flags (uploaded zero)
+      o[13] = 0;                                       // flags: in the status block
       o[14] = G.xs_splits > 1 ? arena_.add(8 * (size_t)G.xs_splits * n * B) : 0;  // score slabs
       // operand workspaces: one per right-hand chunk of each pass
       const size_t xw_chunks = (B + kXwChunk - 1) / kXwChunk, xt_chunks = (B + 63) / 64;
@@ -305,7 +332,7 @@ class SimplsBatch {
       o[5] = arena_.add(8 * n * A);                               // T
       o[6] = arena_.add(8 * p * A);                               // Rs
       o[7] = arena_.add(8 * (size_t)simpls_scratch_doubles(G.p, F.q, F.wmax));
-      o[8] = up_.add(sizeof(int) * 4);                            // state (uploaded zero)
+      o[8] = 0;                                                   // state: in the status block
       o[9] = arena_.add(sizeof(int) * q);                         // counts
       o[11] = fits_[f].perm && fits_[f].labels ? arena_.add(sizeof(int) * n) : 0;  // permuted labels
       o[12] = up_.add(sizeof(RefreshStep) * F.host_steps.size());  // uploaded
@@ -331,7 +358,7 @@ class SimplsBatch {
       G.colss = arena_.at<double>(o[6]); G.stats = arena_.at<double>(o[7]);
       G.spart = arena_.at<double>(o[8]); G.cand = arena_.at<double>(o[9]);
       G.cand_scaled = arena_.at<double>(o[10]); G.t_raw = arena_.at<double>(o[11]);
-      G.loads = arena_.at<double>(o[12]); G.flags = up_.at<int>(o[13]);
+      G.loads = arena_.at<double>(o[12]); G.flags = up_.at<int>(status_off_) + 4 * g;
       G.tpart = G.xs_splits > 1 ? arena_.at<double>(o[14]) : nullptr;
       frag_cursor[g] = arena_.at<char>(o[15]);
     }
@@ -350,7 +377,7 @@ class SimplsBatch {
       F.R = arena_.at<double>(o[2]); F.Q = arena_.at<double>(o[3]);
       F.V = arena_.at<double>(o[4]); F.Tsc = arena_.at<double>(o[5]);
       F.Rs = arena_.at<double>(o[6]); F.scratch = arena_.at<double>(o[7]);
-      F.state = up_.at<int>(o[8]); F.counts = arena_.at<int>(o[9]);
+      F.state = up_.at<int>(status_off_) + 4 * (gplan_.size() + f); F.counts = arena_.at<int>(o[9]);
       F.lperm = fits_[f].perm && fits_[f].labels ? arena_.at<int>(o[11]) : nullptr;
       F.steps = up_.at<RefreshStep>(o[12]);
       F.omega = up_.at<double>(o[13]);
@@ -576,15 +603,17 @@ class SimplsBatch {
   }
 
   void collect() {
-    completed_.assign(plan_.size(), 0);
-    flags_.assign(plan_.size(), 0);
-    for (size_t f = 0; f < plan_.size(); ++f) {
-      int st[4];
-      int gf[4];
-      FPLS_HIP(hipMemcpy(st, plan_[f].state, sizeof(st), hipMemcpyDeviceToHost));
-      FPLS_HIP(hipMemcpy(gf, gplan_[fits_[f].group].flags, sizeof(gf), hipMemcpyDeviceToHost));
+    // group flags then fit states, one copy
+    const size_t ng = gplan_.size(), nf = plan_.size();
+    std::vector<int> status(4 * (ng + nf));
+    FPLS_HIP(hipMemcpy(status.data(), up_.at<int>(status_off_), sizeof(int) * status.size(),
+                       hipMemcpyDeviceToHost));
+    completed_.assign(nf, 0);
+    flags_.assign(nf, 0);
+    for (size_t f = 0; f < nf; ++f) {
+      const int* st = &status[4 * (ng + f)];
       completed_[f] = st[0];
-      flags_[f] = st[2] | gf[0];
+      flags_[f] = st[2] | status[4 * fits_[f].group];
     }
   }
 
@@ -606,6 +635,7 @@ class SimplsBatch {
   CenterColumn<T>* center_list_ = nullptr;
   int center_count_ = 0;
   size_t devfits_off_ = 0;
+  size_t status_off_ = 0;  // int[4] flags per group, then int[4] state per fit
   SimplsFitDev* devfits_ = nullptr;
   Buf xw_tasks_, xt_tasks_, sums_, pre_dev_, proj_frag_;
   std::vector<std::unique_ptr<Buf>> extra_xt_;
