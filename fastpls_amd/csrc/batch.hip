@@ -343,6 +343,7 @@ class SimplsBatch {
     for (size_t f = 0; f < plan_.size(); ++f)
       if (!fits_[f].labels) center_count_ += plan_[f].q;
     const size_t center_off = up_.add(sizeof(CenterColumn<T>) * (size_t)center_count_);
+    const size_t div_off = up_.add(sizeof(DivRows) * plan_.size());
     arena_.allocate();
     up_.allocate();
     // host image of the uploaded pieces (states and flags start at zero):
@@ -403,6 +404,16 @@ class SimplsBatch {
     }
     devfits_ = up_.at<SimplsFitDev>(devfits_off_);
     std::memcpy(image.data() + devfits_off_, dev.data(), sizeof(SimplsFitDev) * dev.size());
+    // R / sx (the coefficients on the original predictor scale) of every fit
+    div_list_ = up_.at<DivRows>(div_off);
+    div_max_ = 0;
+    DivRows* dl = reinterpret_cast<DivRows*>(image.data() + div_off);
+    for (size_t f = 0; f < plan_.size(); ++f) {
+      const Fit& F = plan_[f];
+      const Group& G = gplan_[fits_[f].group];
+      dl[f] = DivRows{F.R, G.sx, (long long)G.p, F.A, F.Rs};
+      div_max_ = std::max(div_max_, (long long)G.p * F.A);
+    }
     // response columns to centre: column c of fit f goes to column off + c of
     // its group's centred block
     center_list_ = up_.at<CenterColumn<T>>(center_off);
@@ -595,11 +606,8 @@ class SimplsBatch {
       launch_xtw<T>(xl, L.xt_count, L.xt_tiles, L.xt_maxn, stream_);
       launch_simpls_update(devfits_, nf, s, stream_);
     }
-    for (size_t f = 0; f < plan_.size(); ++f) {
-      const Fit& F = plan_[f];
-      const Group& G = gplan_[fits_[f].group];
-      launch_div_rows(F.R, G.sx, G.p, F.A, F.Rs, stream_);
-    }
+    // R / sx of every fit: one launch
+    launch_div_rows_list(div_list_, int(plan_.size()), div_max_, stream_);
   }
 
   void collect() {
@@ -633,6 +641,8 @@ class SimplsBatch {
   Arena arena_;
   Arena up_;  // pieces filled from the host: schedules, sketches, fit descriptors
   CenterColumn<T>* center_list_ = nullptr;
+  DivRows* div_list_ = nullptr;
+  long long div_max_ = 0;
   int center_count_ = 0;
   size_t devfits_off_ = 0;
   size_t status_off_ = 0;  // int[4] flags per group, then int[4] state per fit

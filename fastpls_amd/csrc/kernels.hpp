@@ -134,6 +134,17 @@ void launch_finalize_crossprod(const T* X, long long n, int p, int q, int scalin
 // out[i] = in[i] / d[i % rows]  (scale rows of a rows x cols matrix)
 void launch_div_rows(const double* in, const double* d, long long rows, int cols,
                      double* out, hipStream_t s);
+// The same for a device list of independent problems, one launch
+// (work-groups over grid x, problems over grid y).
+struct DivRows {
+  const double* in;
+  const double* d;
+  long long rows;
+  int cols;
+  double* out;
+};
+void launch_div_rows_list(const DivRows* device_list, int count, long long max_elements,
+                          hipStream_t s);
 
 // ---- prediction / classification helpers
 // out (rows x q, element T and/or FP64) = Tsc[:, :k] (ld rows) * M + bias,

@@ -231,6 +231,15 @@ __global__ __launch_bounds__(kWG) void k_div_rows(const double* in, const double
   }
 }
 
+__global__ __launch_bounds__(kWG) void k_div_rows_list(const DivRows* list) {
+  const DivRows t = list[blockIdx.y];
+  const long long total = t.rows * t.cols;
+  for (long long e = (long long)blockIdx.x * kWG + threadIdx.x; e < total;
+       e += (long long)gridDim.x * kWG) {
+    t.out[e] = t.in[e] / t.d[e % t.rows];
+  }
+}
+
 // ----------------------------------------------------------- prediction
 // out = Tsc[:, :k] * M + bias with M(l, c) = Mr[l * rs + c * cs], one thread
 // per output element.
@@ -511,6 +520,17 @@ void launch_div_rows(const double* in, const double* d, long long rows, int cols
   hipLaunchKernelGGL(k_div_rows, dim3(grid_for(rows * cols)), dim3(kWG), 0, s, in, d, rows,
                      cols, out);
   FPLS_LAUNCHED("k_div_rows");
+}
+
+void launch_div_rows_list(const DivRows* device_list, int count, long long max_elements,
+                          hipStream_t s) {
+  if (count <= 0 || max_elements <= 0) return;
+  const unsigned bx = std::min<unsigned>(grid_for(max_elements), 64);
+  for (int c0 = 0; c0 < count; c0 += 65535) {
+    const int c = std::min(65535, count - c0);
+    hipLaunchKernelGGL(k_div_rows_list, dim3(bx, unsigned(c)), dim3(kWG), 0, s, device_list + c0);
+    FPLS_LAUNCHED("k_div_rows_list");
+  }
 }
 
 template<class T>
