@@ -171,6 +171,27 @@ __global__ __launch_bounds__(kWG) void k_xtw(const XtwTask<T>* tasks, int count,
 #pragma unroll
     for (int b = 0; b < MN; ++b) acc[a][b] = 0.0;
 
+  // the A and W values of the next 32-row tile are loaded into registers
+  // while the current tile is multiplied out of LDS
+  constexpr int WPT = (32 * NB + kWG - 1) / kWG;
+  T xr[8];
+  double wr[WPT];
+  auto fetch = [&](long long i0) __attribute__((always_inline)) {
+    const long long row = i0 + lrow;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const long long col = col0 + lcol + 8 * e;
+      xr[e] = (row < r1 && col < t.k) ? A[row + col * t.m] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < WPT; ++u) {
+      const int e = tid + u * kWG;
+      const int r = e & 31, c = e >> 5;
+      const long long w = i0 + r;
+      wr[u] = (e < 32 * NB && w < r1 && c < t.N) ? W[w + (long long)c * t.ldw] : 0.0;
+    }
+  };
+  if (r0 < r1) fetch(r0);
   for (long long i0 = r0; i0 < r1; i0 += 32) {
     const long long row = i0 + lrow;
     const bool rv = row < r1;
@@ -180,17 +201,18 @@ __global__ __launch_bounds__(kWG) void k_xtw(const XtwTask<T>* tasks, int count,
       const long long col = col0 + lcol + 8 * e;
       double v = 0.0;
       if (rv && col < t.k) {
-        v = double(A[row + col * t.m]) - centre[e];
+        v = double(xr[e]) - centre[e];
         s1[e] += v;
         s2[e] = fma(v, v, s2[e]);
       }
       xs[lrow][lcol + 8 * e] = v;
     }
-    for (int e = tid; e < 32 * NB; e += kWG) {
-      const int r = e & 31, c = e >> 5;
-      const long long wr = i0 + r;
-      wsh[r][c] = (wr < r1 && c < t.N) ? W[wr + (long long)c * t.ldw] : 0.0;
+#pragma unroll
+    for (int u = 0; u < WPT; ++u) {
+      const int e = tid + u * kWG;
+      if (e < 32 * NB) wsh[e & 31][e >> 5] = wr[u];
     }
+    if (i0 + 32 < r1) fetch(i0 + 32);
     __syncthreads();
 #pragma unroll 4
     for (int r = 0; r < 32; ++r) {
