@@ -60,9 +60,13 @@ struct FitIn {
 
 template<class T>
 class SimplsBatch {
-  // right-hand chunk of the per-step score pass (FP32 matrix-core launches
-  // take at most 32 columns)
-  static constexpr int kXwChunk = sizeof(T) == 4 ? 32 : 64;
+  // Every product over X runs in FP64 (FP32-stored X is widened as it is
+  // loaded): SIMPLS amplifies the rounding of these products by up to ~1e3 in
+  // its later directions, so FP32-accurate (bf16x6) sums would not hold B to
+  // 1e-5 of the FP64 fit.
+  static constexpr bool kF64 = true;
+  // right-hand chunk of the per-step score pass
+  static constexpr int kXwChunk = 64;
   // longest row split of the cross-product and loading passes and longest
   // column split of the score pass (both within the matrix-core kernels'
   // limits, kXtwSplitRows and kXwSplitCols)
@@ -131,7 +135,7 @@ class SimplsBatch {
     t.A = x; t.m = rows; t.k = G.p; t.center = G.mx; t.W = F.Rs; t.ldw = G.p; t.N = prefix;
     const int splits = xw_splits(rows, G.p, prefix);
     t.splits = splits; t.ldo = rows; t.slab = (long long)rows * prefix;
-    const size_t fb = xw_frag_bytes<T>(G.p, prefix);
+    const size_t fb = xw_frag_bytes<T>(G.p, prefix, kF64);
     if (fb > proj_frag_.bytes()) {
       FPLS_HIP(hipStreamSynchronize(stream_));
       proj_frag_.alloc(fb);
@@ -139,7 +143,7 @@ class SimplsBatch {
     t.frag = fb ? proj_frag_.as<void>() : nullptr;
     if (splits == 1) {
       t.out = scores;
-      launch_xw_single<T>(t, stream_);
+      launch_xw_single<T>(t, stream_, kF64);
     } else {
       const size_t pb = sizeof(double) * (size_t)splits * rows * prefix;
       if (pb > part.bytes()) {
@@ -147,7 +151,7 @@ class SimplsBatch {
         part.alloc(pb);
       }
       t.out = part.d();
-      launch_xw_single<T>(t, stream_);
+      launch_xw_single<T>(t, stream_, kF64);
       launch_sum_slabs(part.d(), splits, (long long)rows * prefix, (long long)rows * prefix,
                        scores, stream_);
     }
@@ -314,9 +318,9 @@ class SimplsBatch {
       // operand workspaces: one per right-hand chunk of each pass
       const size_t xw_chunks = (B + kXwChunk - 1) / kXwChunk, xt_chunks = (B + 63) / 64;
       const size_t pre_chunks = (Q + 63) / 64;
-      o[15] = arena_.add(xw_chunks * xw_frag_bytes<T>(G.p, maxn_xw) +
-                         xt_chunks * xtw_frag_bytes<T>(G.n, maxn_step) +
-                         pre_chunks * xtw_frag_bytes<T>(G.n, maxn_pre) +
+      o[15] = arena_.add(xw_chunks * xw_frag_bytes<T>(G.p, maxn_xw, kF64) +
+                         xt_chunks * xtw_frag_bytes<T>(G.n, maxn_step, kF64) +
+                         pre_chunks * xtw_frag_bytes<T>(G.n, maxn_pre, kF64) +
                          256 * (xw_chunks + xt_chunks + pre_chunks));
     }
     for (size_t f = 0; f < plan_.size(); ++f) {
@@ -454,7 +458,7 @@ class SimplsBatch {
           a.out = G.t_raw + (size_t)c0 * G.n; a.slab = 0;
         }
         a.ldo = G.n;
-        a.frag = take_frag(g, xw_frag_bytes<T>(G.p, maxn_xw));
+        a.frag = take_frag(g, xw_frag_bytes<T>(G.p, maxn_xw, kF64));
         xw.push_back(a);
         xw_mn[gi + 1] = std::max(xw_mn[gi + 1], nb);
       }
@@ -464,7 +468,7 @@ class SimplsBatch {
         b.A = G.X; b.m = G.n; b.k = G.p; b.center = G.mx; b.mode = kCentered;
         b.W = G.t_raw + (size_t)c0 * G.n; b.ldw = G.n; b.N = nb; b.splits = G.ls_splits;
         b.out = G.loads + (size_t)c0 * G.p; b.ldo = G.p; b.slab = (long long)G.p * G.bcat;
-        b.frag = take_frag(g, xtw_frag_bytes<T>(G.n, maxn_step));
+        b.frag = take_frag(g, xtw_frag_bytes<T>(G.n, maxn_step, kF64));
         xt.push_back(b);
         xt_mn[gi + 1] = std::max(xt_mn[gi + 1], nb);
       }
@@ -482,8 +486,8 @@ class SimplsBatch {
       sum_end[gi + 1] = int(sums.size());
     }
     int mn = 0;
-    const long long xw_total = pack_xw(xw.data(), int(xw.size()), &mn);
-    const long long xt_total = pack_xtw(xt.data(), int(xt.size()), &mn);
+    const long long xw_total = pack_xw(xw.data(), int(xw.size()), &mn, kF64);
+    const long long xt_total = pack_xtw(xt.data(), int(xt.size()), &mn, kF64);
     // tiles of a prefix: tile0 of the first task after it. The xtw tile count
     // depends on the launch width, so packs of each prefix are recomputed.
     steps_.assign(order_.size() + 1, StepLaunch{});
@@ -509,7 +513,7 @@ class SimplsBatch {
       if (all_mfma != pre_mfma) {
         std::vector<XtwTask<T>> sub(xt.begin(), xt.begin() + L.xt_count);
         int m2 = 0;
-        L.xt_tiles = pack_xtw(sub.data(), int(sub.size()), &m2);
+        L.xt_tiles = pack_xtw(sub.data(), int(sub.size()), &m2, kF64);
         extra_xt_.push_back(std::make_unique<Buf>());
         extra_xt_.back()->alloc(sizeof(XtwTask<T>) * sub.size());
         FPLS_HIP(hipMemcpy(extra_xt_.back()->as<XtwTask<T>>(), sub.data(),
@@ -535,11 +539,11 @@ class SimplsBatch {
         t.mode = kShiftStats; t.splits = G.splits;
         t.out = G.spart + (size_t)c0 * G.p; t.ldo = G.p; t.slab = (long long)G.p * G.qcat;
         t.stats = c0 == 0 ? G.stats : nullptr;
-        t.frag = take_frag(g, xtw_frag_bytes<T>(G.n, maxn_pre));
+        t.frag = take_frag(g, xtw_frag_bytes<T>(G.n, maxn_pre, kF64));
         pre_.push_back(t);
       }
     }
-    pre_tiles_ = pack_xtw(pre_.data(), int(pre_.size()), &pre_maxn_);
+    pre_tiles_ = pack_xtw(pre_.data(), int(pre_.size()), &pre_maxn_, kF64);
     pre_dev_.alloc(sizeof(XtwTask<T>) * pre_.size());
     FPLS_HIP(hipMemcpy(pre_dev_.as<XtwTask<T>>(), pre_.data(), sizeof(XtwTask<T>) * pre_.size(), hipMemcpyHostToDevice));
   }
@@ -564,7 +568,8 @@ class SimplsBatch {
     // every response column of every fit: one launch
     launch_center_columns<T>(center_list_, center_count_, stream_);
     // one launch: column statistics + cross-products of every group
-    launch_xtw<T>(pre_dev_.as<XtwTask<T>>(), int(pre_.size()), pre_tiles_, pre_maxn_, stream_);
+    launch_xtw<T>(pre_dev_.as<XtwTask<T>>(), int(pre_.size()), pre_tiles_, pre_maxn_, stream_,
+                  kF64);
     for (Group& G : gplan_)
       launch_finalize_crossprod<T>(G.X, G.n, G.p, G.qcat, scaling_, G.stats, G.spart, G.splits,
                                    G.ysum, G.mx, G.sx, G.colss, G.S, G.S0, G.flags, stream_);
@@ -599,11 +604,11 @@ class SimplsBatch {
       while (na < int(order_.size()) && gplan_[order_[na]].max_steps > s) ++na;
       const StepLaunch& L = steps_[na];
       launch_simpls_refresh(devfits_, nf, s, stream_);
-      launch_xw<T>(xw_tasks_.as<XwTask<T>>(), L.xw_count, L.xw_tiles, L.xw_maxn, stream_);
+      launch_xw<T>(xw_tasks_.as<XwTask<T>>(), L.xw_count, L.xw_tiles, L.xw_maxn, stream_, kF64);
       if (L.sum_count > 0) launch_sum_slabs_list(sums_.as<SlabSum>(), L.sum_count, L.sum_max, stream_);
       const XtwTask<T>* xl = L.xt_list ? extra_xt_[L.xt_list - 1]->template as<XtwTask<T>>()
                                        : xt_tasks_.as<XtwTask<T>>();
-      launch_xtw<T>(xl, L.xt_count, L.xt_tiles, L.xt_maxn, stream_);
+      launch_xtw<T>(xl, L.xt_count, L.xt_tiles, L.xt_maxn, stream_, kF64);
       launch_simpls_update(devfits_, nf, s, stream_);
     }
     // R / sx of every fit: one launch

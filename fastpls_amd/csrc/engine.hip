@@ -61,6 +61,11 @@ class Model final : public Handle, private StreamOps {
     implicit_ = !classification_ && double(p) * double(q) * 8.0 / (1024.0 * 1024.0) > 32.0;
     rank_one_ = implicit_ && method_ == kSimpls &&
                 double(p) * double(q) * double(sizeof(T)) > 512.0 * 1024.0 * 1024.0;
+    // SIMPLS amplifies the rounding of the products over X (the later
+    // directions of its component recursion move by up to ~1e3 times a
+    // relative perturbation of S), so FP32-stored X is widened to FP64 for
+    // every SIMPLS product; PLS-SVD keeps bf16x6 (FP32-accurate) products.
+    f64_products_ = method_ == kSimpls;
     stream_ = stream_take();
     for (Buf* b : all_bufs()) b->bind(stream_);
     try {
@@ -348,12 +353,13 @@ class Model final : public Handle, private StreamOps {
     colss_.alloc(sizeof(double) * p_);
     S_.alloc(sizeof(double) * (size_t)p_ * qc);
     S0_.alloc(sizeof(double) * (size_t)p_ * qc);
-    const int splits = xtw_splits(n_, p_, std::min(qc, 64), sizeof(T) == 4);
+    const bool f64 = f64_products_;
+    const int splits = xtw_splits(n_, p_, std::min(qc, 64), sizeof(T) == 4 && !f64);
     Buf& s_part = cp_part_;
     Buf& stats = cp_stats_;
     s_part.alloc(sizeof(double) * (size_t)splits * p_ * qc);
     stats.alloc(sizeof(double) * (size_t)splits * 2 * p_);
-    void* wf = frag(xtw_frag_bytes<T>(n_, std::min(qc, 64)));
+    void* wf = frag(xtw_frag_bytes<T>(n_, std::min(qc, 64), f64));
     FPLS_HIP(hipEventRecord(cp0_.e, stream_));
     for (int c0 = 0; c0 < qc; c0 += 64) {
       const int nb = std::min(64, qc - c0);
@@ -364,7 +370,7 @@ class Model final : public Handle, private StreamOps {
       t.splits = splits;
       t.out = s_part.d() + (long long)c0 * p_; t.ldo = p_; t.slab = (long long)p_ * qc;
       t.stats = c0 == 0 ? stats.d() : nullptr;
-      launch_xtw_single<T>(t, stream_);
+      launch_xtw_single<T>(t, stream_, f64);
     }
     FPLS_HIP(hipEventRecord(cp1_.e, stream_));
     launch_finalize_crossprod<T>(X_, n_, p_, qc, scaling_, stats.d(), s_part.d(), splits,
@@ -499,8 +505,9 @@ class Model final : public Handle, private StreamOps {
     cand_.alloc(sizeof(double) * (size_t)p_ * bmax);
     cand_scaled_.alloc(sizeof(double) * (size_t)p_ * bmax);
     t_raw_.alloc(sizeof(double) * (size_t)n_ * bmax);
-    const int xs = xw_splits(n_, p_, bmax, sizeof(T) == 4);
-    const int ls = xtw_splits(n_, p_, bmax, sizeof(T) == 4);
+    const bool f64 = f64_products_;
+    const int xs = xw_splits(n_, p_, bmax, sizeof(T) == 4 && !f64);
+    const int ls = xtw_splits(n_, p_, bmax, sizeof(T) == 4 && !f64);
     ts_part_.alloc(sizeof(double) * (size_t)xs * n_ * bmax);
     ls_part_.alloc(sizeof(double) * (size_t)ls * p_ * bmax);
 
@@ -552,16 +559,16 @@ class Model final : public Handle, private StreamOps {
       XwTask<T> xw{};
       xw.A = X_; xw.m = n_; xw.k = p_; xw.center = mx_.d(); xw.W = cand_scaled_.d(); xw.ldw = p_;
       xw.N = b; xw.splits = xs; xw.out = ts_part_.d(); xw.ldo = n_; xw.slab = (long long)n_ * b;
-      xw.frag = frag(xw_frag_bytes<T>(p_, b));
-      launch_xw_single<T>(xw, stream_);
+      xw.frag = frag(xw_frag_bytes<T>(p_, b, f64));
+      launch_xw_single<T>(xw, stream_, f64);
       launch_sum_slabs(ts_part_.d(), xs, (long long)n_ * b, (long long)n_ * b, t_raw_.d(), stream_);
       // loadings: (X - 1 mᵀ)ᵀ t  (divided by s in the update)
       XtwTask<T> xt{};
       xt.A = X_; xt.m = n_; xt.k = p_; xt.center = mx_.d(); xt.W = t_raw_.d(); xt.ldw = n_;
       xt.N = b; xt.mode = kCentered; xt.splits = ls; xt.out = ls_part_.d(); xt.ldo = p_;
       xt.slab = (long long)p_ * bmax;
-      xt.frag = frag(xtw_frag_bytes<T>(n_, b));
-      launch_xtw_single<T>(xt, stream_);
+      xt.frag = frag(xtw_frag_bytes<T>(n_, b, f64));
+      launch_xtw_single<T>(xt, stream_, f64);
       // response loadings without S0: Ỹᵀ t (scaled by 1/|t| in the update)
       if (implicit_) tproduct<double>(Yc_.d(), n_, q_, nullptr, t_raw_.d(), n_, b, qraw_.d());
       launch_simpls_update(fitdev_.as<SimplsFitDev>(), 1, int(s), stream_);

@@ -37,17 +37,19 @@ class StreamOps {
                long long ldw, int N, double* out) {
     for (int c0 = 0; c0 < N; c0 += 64) {
       const int nb = std::min(64, N - c0);
-      const int splits = invariant_ ? xw_splits_by_depth(k) : xw_splits(m, k, nb, sizeof(U) == 4);
+      const bool f64 = f64_products_;
+      const int splits = invariant_ ? xw_splits_by_depth(k)
+                                    : xw_splits(m, k, nb, sizeof(U) == 4 && !f64);
       XwTask<U> t{};
       t.A = A; t.m = m; t.k = k; t.center = center; t.W = W + (long long)c0 * ldw; t.ldw = ldw;
-      t.N = nb; t.splits = splits; t.frag = frag(xw_frag_bytes<U>(k, nb));
+      t.N = nb; t.splits = splits; t.frag = frag(xw_frag_bytes<U>(k, nb, f64));
       if (splits == 1) {
         t.out = out + (long long)c0 * m; t.ldo = m; t.slab = 0;
-        launch_xw_single<U>(t, stream_);
+        launch_xw_single<U>(t, stream_, f64);
       } else {
         part_.reserve(sizeof(double) * (size_t)splits * m * nb);
         t.out = part_.d(); t.ldo = m; t.slab = m * nb;
-        launch_xw_single<U>(t, stream_);
+        launch_xw_single<U>(t, stream_, f64);
         launch_sum_slabs(part_.d(), splits, m * nb, m * nb, out + (long long)c0 * m, stream_);
       }
     }
@@ -59,18 +61,19 @@ class StreamOps {
                 long long ldw, int N, double* out) {
     for (int c0 = 0; c0 < N; c0 += 64) {
       const int nb = std::min(64, N - c0);
-      const int splits = xtw_splits(m, k, nb, sizeof(U) == 4);
+      const bool f64 = f64_products_;
+      const int splits = xtw_splits(m, k, nb, sizeof(U) == 4 && !f64);
       XtwTask<U> t{};
       t.A = A; t.m = m; t.k = k; t.center = center; t.W = W + (long long)c0 * ldw; t.ldw = ldw;
       t.N = nb; t.mode = center ? kCentered : kPlain; t.splits = splits;
-      t.frag = frag(xtw_frag_bytes<U>(m, nb));
+      t.frag = frag(xtw_frag_bytes<U>(m, nb, f64));
       if (splits == 1) {
         t.out = out + (long long)c0 * k; t.ldo = k; t.slab = 0;
-        launch_xtw_single<U>(t, stream_);
+        launch_xtw_single<U>(t, stream_, f64);
       } else {
         part_.reserve(sizeof(double) * (size_t)splits * k * nb);
         t.out = part_.d(); t.ldo = k; t.slab = k * nb;
-        launch_xtw_single<U>(t, stream_);
+        launch_xtw_single<U>(t, stream_, f64);
         launch_sum_slabs(part_.d(), splits, k * nb, k * nb, out + (long long)c0 * k, stream_);
       }
     }
@@ -212,6 +215,9 @@ class StreamOps {
   hipStream_t stream_ = nullptr;
   // batch-invariant query splits: xw splits depend on the depth only
   bool invariant_ = false;
+  // FP32 operands of product / tproduct run on the FP64 matrix cores
+  // (widened on load) instead of in bf16x6 split precision
+  bool f64_products_ = false;
   bool zero_ready_ = false;
   Buf frag_;  // matrix-core operand workspace of the streaming products
   Buf part_, qr_gram_, qr_rinv_, qr_tmp_, qr_fail_, zero_ints_;

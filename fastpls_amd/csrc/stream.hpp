@@ -59,21 +59,27 @@ struct XtwTask {
 // host task array, checks it and returns the grid size; the caller uploads
 // the packed array once and launches with the device copy, so launches carry
 // no host-to-device traffic and can be captured in a graph.
+//
+// `f64` selects the products for FP32 A on the matrix cores: false runs them
+// in bf16x6 split precision (FP32 sums per 32- to 64-long step), true widens
+// A to FP64 as it is loaded and runs them on the FP64 matrix cores (FP64
+// products and sums throughout; no operand workspace). FP64 A always takes
+// the FP64 route.
 template<class T>
-long long pack_xw(XwTask<T>* tasks, int count, int* maxn);
+long long pack_xw(XwTask<T>* tasks, int count, int* maxn, bool f64 = false);
 template<class T>
-long long pack_xtw(XtwTask<T>* tasks, int count, int* maxn);
+long long pack_xtw(XtwTask<T>* tasks, int count, int* maxn, bool f64 = false);
 template<class T>
 void launch_xw(const XwTask<T>* device_tasks, int count, long long tiles, int maxn,
-               hipStream_t stream);
+               hipStream_t stream, bool f64 = false);
 template<class T>
 void launch_xtw(const XtwTask<T>* device_tasks, int count, long long tiles, int maxn,
-                hipStream_t stream);
+                hipStream_t stream, bool f64 = false);
 // One task passed by value as a kernel argument (no device task array).
 template<class T>
-void launch_xw_single(XwTask<T> task, hipStream_t stream);
+void launch_xw_single(XwTask<T> task, hipStream_t stream, bool f64 = false);
 template<class T>
-void launch_xtw_single(XtwTask<T> task, hipStream_t stream);
+void launch_xtw_single(XtwTask<T> task, hipStream_t stream, bool f64 = false);
 
 // Number of 256-row tiles of an xw task and of 64-column tiles of xtw.
 __host__ __device__ inline long long xw_row_tiles(long long m) { return (m + 255) / 256; }
@@ -103,14 +109,14 @@ inline int mfma_width(int maxn) { return maxn <= 16 ? 16 : maxn <= 32 ? 32 : max
 //   xtw: [hi, mid, lo][NB][m rounded up to 64], rows permuted within 64-row tiles
 constexpr int kSplitTerms = 3;
 template<class T>
-inline size_t xw_frag_bytes(long long k, int maxn) {
-  if (sizeof(T) != 4 || maxn < kMfmaMinN) return 0;
+inline size_t xw_frag_bytes(long long k, int maxn, bool f64 = false) {
+  if (sizeof(T) != 4 || f64 || maxn < kMfmaMinN) return 0;
   const size_t k64 = size_t((k + 63) / 64 * 64);
   return k64 * (size_t)mfma_width(maxn) * 2 * kSplitTerms + k64 * 4;
 }
 template<class T>
-inline size_t xtw_frag_bytes(long long m, int maxn) {
-  if (sizeof(T) != 4 || maxn < kMfmaMinN) return 0;
+inline size_t xtw_frag_bytes(long long m, int maxn, bool f64 = false) {
+  if (sizeof(T) != 4 || f64 || maxn < kMfmaMinN) return 0;
   const size_t m64 = size_t((m + 63) / 64 * 64);
   return m64 * (size_t)mfma_width(maxn) * 2 * kSplitTerms;
 }

@@ -27,6 +27,21 @@ __device__ __forceinline__ const T FPLS_GLOBAL* gp(const T* p) { return (const T
 template<class T>
 __device__ __forceinline__ T FPLS_GLOBAL* gpw(T* p) { return (T FPLS_GLOBAL*)p; }
 
+// Four consecutive elements widened to FP64, read with 16-byte loads that need
+// only element alignment (gfx950 global loads accept dword-aligned 16-byte
+// accesses), so a column of any length is read with vector loads.
+typedef float float4_ea __attribute__((ext_vector_type(4), aligned(4)));
+typedef double double2_ea __attribute__((ext_vector_type(2), aligned(8)));
+__device__ __forceinline__ void load4_wide(const float FPLS_GLOBAL* p, double* out) {
+  const float4_ea a = *(const float4_ea FPLS_GLOBAL*)p;
+  out[0] = a.x; out[1] = a.y; out[2] = a.z; out[3] = a.w;
+}
+__device__ __forceinline__ void load4_wide(const double FPLS_GLOBAL* p, double* out) {
+  const double2_ea a = *(const double2_ea FPLS_GLOBAL*)p;
+  const double2_ea b = *(const double2_ea FPLS_GLOBAL*)(p + 2);
+  out[0] = a.x; out[1] = a.y; out[2] = b.x; out[3] = b.y;
+}
+
 // The task of this work-group: the single by-value task, or a binary search
 // of the flattened tile space of a device task list.
 template<class Task>
@@ -838,13 +853,15 @@ __global__ __launch_bounds__(kBf6Threads) void k_xw_bf6(const XwTask<float>* tas
 // Same tiling as the bf16x6 kernel (128 columns, 64-row steps, lane rows
 // 16v + 4g + e); MFMA call (v, e) pairs row 16v + 4g + e of A with the same
 // row of W, which is staged in LDS as FP64. The FP64 matrix cores bound this
-// kernel, so A is single-buffered.
-template<int NB>
-__global__ __launch_bounds__(kWG) void k_xtw_f64(const XtwTask<double>* tasks, int count,
-                                                 XtwTask<double> single) {
+// kernel, so A is single-buffered. A is FP64, or FP32 widened to FP64 as it
+// is loaded (every product and sum is then FP64: the accurate route for
+// FP32-stored X).
+template<int NB, class TX>
+__global__ __launch_bounds__(kWG) void k_xtw_f64(const XtwTask<TX>* tasks, int count,
+                                                 XtwTask<TX> single) {
   constexpr int NBLK = NB / 16;
   constexpr int LD = 64 + 2;
-  const XtwTask<double> t = load_task(tasks, count, single);
+  const XtwTask<TX> t = load_task(tasks, count, single);
   const long long ctiles = (t.k + kXtwCols - 1) / kXtwCols;
   const long long total = ctiles * t.splits;
   long long local = (long long)blockIdx.x - t.tile0;
@@ -859,11 +876,10 @@ __global__ __launch_bounds__(kWG) void k_xtw_f64(const XtwTask<double>* tasks, i
   __shared__ double ws[NB][LD];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m16 = lane & 15, g = lane >> 4;
-  const bool vec = (t.m & 1) == 0 && (((unsigned long long)t.A) & 15) == 0;
   const bool stats = t.mode == kShiftStats;
   const double FPLS_GLOBAL* W = gp(t.W);
 
-  const double FPLS_GLOBAL* xcol[2];
+  const TX FPLS_GLOBAL* xcol[2];
   double centre[2];
   bool cvalid[2];
 #pragma unroll
@@ -874,7 +890,7 @@ __global__ __launch_bounds__(kWG) void k_xtw_f64(const XtwTask<double>* tasks, i
     double c = 0.0;
     if (cvalid[mb]) {
       if (t.mode == kCentered && t.center != nullptr) c = gp(t.center)[col];
-      else if (stats) c = xcol[mb][0];
+      else if (stats) c = double(xcol[mb][0]);
     }
     centre[mb] = c;
   }
@@ -888,23 +904,19 @@ __global__ __launch_bounds__(kWG) void k_xtw_f64(const XtwTask<double>* tasks, i
 
   for (long long i0 = r0; i0 < r1; i0 += 64) {
     double x[2][16];
-    const bool full = vec && i0 + 64 <= r1;
+    const bool full = i0 + 64 <= r1;
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb) {
       if (full) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const double2 a = *(const double2 FPLS_GLOBAL*)(xcol[mb] + i0 + 16 * v + 4 * g);
-          const double2 b = *(const double2 FPLS_GLOBAL*)(xcol[mb] + i0 + 16 * v + 4 * g + 2);
-          x[mb][4 * v] = a.x; x[mb][4 * v + 1] = a.y; x[mb][4 * v + 2] = b.x; x[mb][4 * v + 3] = b.y;
-        }
+        for (int v = 0; v < 4; ++v) load4_wide(xcol[mb] + i0 + 16 * v + 4 * g, &x[mb][4 * v]);
       } else {
 #pragma unroll
         for (int v = 0; v < 4; ++v)
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             const long long row = i0 + 16 * v + 4 * g + e;
-            x[mb][4 * v + e] = (cvalid[mb] && row < r1) ? xcol[mb][row] : centre[mb];
+            x[mb][4 * v + e] = (cvalid[mb] && row < r1) ? double(xcol[mb][row]) : centre[mb];
           }
       }
     }
@@ -967,12 +979,13 @@ __global__ __launch_bounds__(kWG) void k_xtw_f64(const XtwTask<double>* tasks, i
 // ------------------------------------------------------- xw, FP64 MFMA
 // Same tiling as the bf16x6 kernel (256 rows, 32-column steps, lane rows
 // 4r..4r+3 of columns 8g..8g+7); MFMA call (e, j) pairs column 8g + j of row
-// 4r + e with row 8g + j of W, read straight from W's columns in L2.
-template<int NB>
-__global__ __launch_bounds__(kWG) void k_xw_f64(const XwTask<double>* tasks, int count,
-                                                XwTask<double> single) {
+// 4r + e with row 8g + j of W, read straight from W's columns in L2. A is
+// FP64, or FP32 widened to FP64 as it is loaded.
+template<int NB, class TX>
+__global__ __launch_bounds__(kWG) void k_xw_f64(const XwTask<TX>* tasks, int count,
+                                                XwTask<TX> single) {
   constexpr int NBLK = NB / 16;
-  const XwTask<double> t = load_task(tasks, count, single);
+  const XwTask<TX> t = load_task(tasks, count, single);
   const long long local = (long long)blockIdx.x - t.tile0;
   const long long rtiles = xw_row_tiles(t.m);
   if (local >= rtiles * t.splits) return;
@@ -984,9 +997,8 @@ __global__ __launch_bounds__(kWG) void k_xw_f64(const XwTask<double>* tasks, int
   const long long per = split_extent(t.k, t.splits, 32);
   const long long j0 = split * per;
   const long long j1 = j0 + per < t.k ? j0 + per : t.k;
-  const bool vec = (t.m & 1) == 0 && (((unsigned long long)t.A) & 15) == 0;
   const bool rows_full = i0w + 64 <= t.m;
-  const double FPLS_GLOBAL* xr = gp(t.A) + (rowb < t.m ? rowb : 0);
+  const TX FPLS_GLOBAL* xr = gp(t.A) + (rowb < t.m ? rowb : 0);
   const double FPLS_GLOBAL* W = gp(t.W);
   const double FPLS_GLOBAL* C = t.center ? gp(t.center) : nullptr;
 
@@ -1000,19 +1012,15 @@ __global__ __launch_bounds__(kWG) void k_xw_f64(const XwTask<double>* tasks, int
     const long long cb = jc + 8 * g;
     const bool full = jc + 32 <= j1;
     double x[8][4], c[8];
-    if (vec && rows_full && full) {
+    if (rows_full && full) {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const double2 a = *(const double2 FPLS_GLOBAL*)(xr + (cb + j) * t.m);
-        const double2 b = *(const double2 FPLS_GLOBAL*)(xr + (cb + j) * t.m + 2);
-        x[j][0] = a.x; x[j][1] = a.y; x[j][2] = b.x; x[j][3] = b.y;
-      }
+      for (int j = 0; j < 8; ++j) load4_wide(xr + (cb + j) * t.m, x[j]);
     } else {
 #pragma unroll
       for (int j = 0; j < 8; ++j)
 #pragma unroll
         for (int e = 0; e < 4; ++e)
-          x[j][e] = (cb + j < j1 && rowb + e < t.m) ? xr[(cb + j) * t.m + e] : 0.0;
+          x[j][e] = (cb + j < j1 && rowb + e < t.m) ? double(xr[(cb + j) * t.m + e]) : 0.0;
     }
 #pragma unroll
     for (int j = 0; j < 8; ++j) c[j] = (C != nullptr && cb + j < j1) ? C[cb + j] : 0.0;
@@ -1088,25 +1096,28 @@ void with_width(int maxn, Fn&& fn) {
 
 template<class T>
 void dispatch_xw(const XwTask<T>* d, int count, const XwTask<T>& single, long long tiles,
-                 int maxn, hipStream_t stream) {
+                 int maxn, hipStream_t stream, bool f64) {
   if (tiles <= 0) return;
   if (tiles > 0x7fffffffLL) throw std::invalid_argument("xw grid too large");
   const dim3 grid((unsigned)tiles), block((unsigned)kWG);
   if (maxn >= kMfmaMinN) {
-    if (sizeof(T) == 4 && maxn > kXwBf6MaxN)
+    const bool bf6 = sizeof(T) == 4 && !f64;
+    if (bf6 && maxn > kXwBf6MaxN)
       throw std::invalid_argument("FP32 matrix-core xw launches take at most 32 right-hand columns");
     with_width(maxn, [&](auto w) {
       constexpr int NB = decltype(w)::value;
       if constexpr (sizeof(T) == 4 && NB <= kXwBf6MaxN) {
-        hipLaunchKernelGGL((k_xw_split<NB>), dim3(kSplitBlocks, unsigned(d ? count : 1)), block, 0,
-                           stream, d, single);
-        FPLS_LAUNCHED("k_xw_split");
-        hipLaunchKernelGGL((k_xw_bf6<NB>), grid, dim3(kBf6Threads), 0, stream, d, count, single);
-        FPLS_LAUNCHED("k_xw_bf6");
-      } else if constexpr (sizeof(T) == 8) {
-        hipLaunchKernelGGL((k_xw_f64<NB>), grid, block, 0, stream, d, count, single);
-        FPLS_LAUNCHED("k_xw_f64");
+        if (bf6) {
+          hipLaunchKernelGGL((k_xw_split<NB>), dim3(kSplitBlocks, unsigned(d ? count : 1)), block,
+                             0, stream, d, single);
+          FPLS_LAUNCHED("k_xw_split");
+          hipLaunchKernelGGL((k_xw_bf6<NB>), grid, dim3(kBf6Threads), 0, stream, d, count, single);
+          FPLS_LAUNCHED("k_xw_bf6");
+          return;
+        }
       }
+      hipLaunchKernelGGL((k_xw_f64<NB, T>), grid, block, 0, stream, d, count, single);
+      FPLS_LAUNCHED("k_xw_f64");
     });
     return;
   }
@@ -1118,7 +1129,7 @@ void dispatch_xw(const XwTask<T>* d, int count, const XwTask<T>& single, long lo
 
 template<class T>
 void dispatch_xtw(const XtwTask<T>* d, int count, const XtwTask<T>& single, long long tiles,
-                  int maxn, hipStream_t stream) {
+                  int maxn, hipStream_t stream, bool f64) {
   if (tiles <= 0) return;
   if (tiles > 0x7fffffffLL) throw std::invalid_argument("xtw grid too large");
   const dim3 grid((unsigned)tiles), block((unsigned)kWG);
@@ -1126,15 +1137,17 @@ void dispatch_xtw(const XtwTask<T>* d, int count, const XtwTask<T>& single, long
     with_width(maxn, [&](auto w) {
       constexpr int NB = decltype(w)::value;
       if constexpr (sizeof(T) == 4) {
-        hipLaunchKernelGGL((k_xtw_split<NB>), dim3(kSplitBlocks, unsigned(d ? count : 1)), block, 0,
-                           stream, d, single);
-        FPLS_LAUNCHED("k_xtw_split");
-        hipLaunchKernelGGL((k_xtw_bf6<NB>), grid, dim3(kBf6Threads), 0, stream, d, count, single);
-        FPLS_LAUNCHED("k_xtw_bf6");
-      } else {
-        hipLaunchKernelGGL((k_xtw_f64<NB>), grid, block, 0, stream, d, count, single);
-        FPLS_LAUNCHED("k_xtw_f64");
+        if (!f64) {
+          hipLaunchKernelGGL((k_xtw_split<NB>), dim3(kSplitBlocks, unsigned(d ? count : 1)), block,
+                             0, stream, d, single);
+          FPLS_LAUNCHED("k_xtw_split");
+          hipLaunchKernelGGL((k_xtw_bf6<NB>), grid, dim3(kBf6Threads), 0, stream, d, count, single);
+          FPLS_LAUNCHED("k_xtw_bf6");
+          return;
+        }
       }
+      hipLaunchKernelGGL((k_xtw_f64<NB, T>), grid, block, 0, stream, d, count, single);
+      FPLS_LAUNCHED("k_xtw_f64");
     });
     return;
   }
@@ -1146,16 +1159,16 @@ void dispatch_xtw(const XtwTask<T>* d, int count, const XtwTask<T>& single, long
 }  // namespace
 
 template<class T>
-long long pack_xw(XwTask<T>* tasks, int count, int* maxn) {
+long long pack_xw(XwTask<T>* tasks, int count, int* maxn, bool f64) {
   long long tiles = 0;
   int mn = 0;
   for (int i = 0; i < count; ++i) mn = std::max(mn, tasks[i].N);
   for (int i = 0; i < count; ++i) {
     if (tasks[i].N < 1 || tasks[i].N > 64) throw std::invalid_argument("xw supports 1..64 right-hand columns");
     if (tasks[i].splits < 1) throw std::invalid_argument("xw needs at least one split");
-    if (xw_frag_bytes<T>(tasks[i].k, mn) > 0 && tasks[i].frag == nullptr)
+    if (xw_frag_bytes<T>(tasks[i].k, mn, f64) > 0 && tasks[i].frag == nullptr)
       throw std::invalid_argument("xw task without its operand workspace");
-    if (xw_frag_bytes<T>(tasks[i].k, mn) > 0 && tasks[i].m > kMfmaMaxRows)
+    if (xw_frag_bytes<T>(tasks[i].k, mn, f64) > 0 && tasks[i].m > kMfmaMaxRows)
       throw std::invalid_argument("matrix-core products support at most 2^24 rows");
     tasks[i].tile0 = tiles;
     tiles += xw_row_tiles(tasks[i].m) * tasks[i].splits;
@@ -1165,7 +1178,7 @@ long long pack_xw(XwTask<T>* tasks, int count, int* maxn) {
 }
 
 template<class T>
-long long pack_xtw(XtwTask<T>* tasks, int count, int* maxn) {
+long long pack_xtw(XtwTask<T>* tasks, int count, int* maxn, bool f64) {
   int mn = 0;
   for (int i = 0; i < count; ++i) {
     if (tasks[i].N < 1 || tasks[i].N > 64) throw std::invalid_argument("xtw supports 1..64 right-hand columns");
@@ -1174,9 +1187,9 @@ long long pack_xtw(XtwTask<T>* tasks, int count, int* maxn) {
   }
   long long tiles = 0;
   for (int i = 0; i < count; ++i) {
-    if (xtw_frag_bytes<T>(tasks[i].m, mn) > 0 && tasks[i].frag == nullptr)
+    if (xtw_frag_bytes<T>(tasks[i].m, mn, f64) > 0 && tasks[i].frag == nullptr)
       throw std::invalid_argument("xtw task without its operand workspace");
-    if (xtw_frag_bytes<T>(tasks[i].m, mn) > 0 && tasks[i].m > kMfmaMaxRows)
+    if (xtw_frag_bytes<T>(tasks[i].m, mn, f64) > 0 && tasks[i].m > kMfmaMaxRows)
       throw std::invalid_argument("matrix-core products support at most 2^24 rows");
     tasks[i].tile0 = tiles;
     tiles += xtw_tiles(tasks[i].k, mn) * tasks[i].splits;
@@ -1187,42 +1200,42 @@ long long pack_xtw(XtwTask<T>* tasks, int count, int* maxn) {
 
 template<class T>
 void launch_xw(const XwTask<T>* device_tasks, int count, long long tiles, int maxn,
-               hipStream_t stream) {
+               hipStream_t stream, bool f64) {
   if (count <= 0) return;
   if (count > 65535) throw std::invalid_argument("xw task list longer than 65535");
-  dispatch_xw<T>(device_tasks, count, XwTask<T>{}, tiles, maxn, stream);
+  dispatch_xw<T>(device_tasks, count, XwTask<T>{}, tiles, maxn, stream, f64);
 }
 
 template<class T>
 void launch_xtw(const XtwTask<T>* device_tasks, int count, long long tiles, int maxn,
-                hipStream_t stream) {
+                hipStream_t stream, bool f64) {
   if (count <= 0) return;
   if (count > 65535) throw std::invalid_argument("xtw task list longer than 65535");
-  dispatch_xtw<T>(device_tasks, count, XtwTask<T>{}, tiles, maxn, stream);
+  dispatch_xtw<T>(device_tasks, count, XtwTask<T>{}, tiles, maxn, stream, f64);
 }
 
 template<class T>
-void launch_xw_single(XwTask<T> task, hipStream_t stream) {
-  // FP32 matrix-core launches take at most 32 right-hand columns: wider
-  // tasks run as consecutive 32-column pieces (same stream, so the pieces
-  // may share the operand workspace)
-  const int step = (sizeof(T) == 4 && task.N >= kMfmaMinN) ? kXwBf6MaxN : task.N;
+void launch_xw_single(XwTask<T> task, hipStream_t stream, bool f64) {
+  // FP32 bf16x6 launches take at most 32 right-hand columns: wider tasks run
+  // as consecutive 32-column pieces (same stream, so the pieces may share the
+  // operand workspace)
+  const int step = (sizeof(T) == 4 && !f64 && task.N >= kMfmaMinN) ? kXwBf6MaxN : task.N;
   for (int c0 = 0; c0 < task.N; c0 += step) {
     XwTask<T> piece = task;
     piece.N = std::min(step, task.N - c0);
     piece.W = task.W + (long long)c0 * task.ldw;
     piece.out = task.out + (long long)c0 * task.ldo;
     int maxn = 0;
-    const long long tiles = pack_xw(&piece, 1, &maxn);
-    dispatch_xw<T>(nullptr, 1, piece, tiles, maxn, stream);
+    const long long tiles = pack_xw(&piece, 1, &maxn, f64);
+    dispatch_xw<T>(nullptr, 1, piece, tiles, maxn, stream, f64);
   }
 }
 
 template<class T>
-void launch_xtw_single(XtwTask<T> task, hipStream_t stream) {
+void launch_xtw_single(XtwTask<T> task, hipStream_t stream, bool f64) {
   int maxn = 0;
-  const long long tiles = pack_xtw(&task, 1, &maxn);
-  dispatch_xtw<T>(nullptr, 1, task, tiles, maxn, stream);
+  const long long tiles = pack_xtw(&task, 1, &maxn, f64);
+  dispatch_xtw<T>(nullptr, 1, task, tiles, maxn, stream, f64);
 }
 
 void launch_sum_slabs(const double* part, int splits, long long stride,
@@ -1246,13 +1259,13 @@ void launch_sum_slabs_list(const SlabSum* device_list, int count, long long max_
   FPLS_LAUNCHED("k_sum_slabs_list");
 }
 
-#define FPLS_STREAM_INSTANCES(T)                                                        \
-  template long long pack_xw<T>(XwTask<T>*, int, int*);                                  \
-  template long long pack_xtw<T>(XtwTask<T>*, int, int*);                                \
-  template void launch_xw<T>(const XwTask<T>*, int, long long, int, hipStream_t);       \
-  template void launch_xtw<T>(const XtwTask<T>*, int, long long, int, hipStream_t);      \
-  template void launch_xw_single<T>(XwTask<T>, hipStream_t);                             \
-  template void launch_xtw_single<T>(XtwTask<T>, hipStream_t);
+#define FPLS_STREAM_INSTANCES(T)                                                          \
+  template long long pack_xw<T>(XwTask<T>*, int, int*, bool);                              \
+  template long long pack_xtw<T>(XtwTask<T>*, int, int*, bool);                            \
+  template void launch_xw<T>(const XwTask<T>*, int, long long, int, hipStream_t, bool);   \
+  template void launch_xtw<T>(const XtwTask<T>*, int, long long, int, hipStream_t, bool);  \
+  template void launch_xw_single<T>(XwTask<T>, hipStream_t, bool);                         \
+  template void launch_xtw_single<T>(XtwTask<T>, hipStream_t, bool);
 FPLS_STREAM_INSTANCES(float)
 FPLS_STREAM_INSTANCES(double)
 #undef FPLS_STREAM_INSTANCES
