@@ -1,18 +1,28 @@
 """Benchmark of the HIP backend on one MI355X.
 
-Workload (default): PLS-SVD through the randomized SVD of XᵀY,
-n=100000, p=20000, q=50, 20 components, oversample 10, 2 power iterations,
-rSVD seed 1; latent-factor data (30 factors, column norms 0.8^j, noise 0.05,
-data seed 0) generated directly in HBM. X is FP32 (8 GB) for the mixed path
-(bf16x6 matrix-core products, FP32 accumulation, FP64 small math) and FP64
-(16 GB) for the plain path (FP64 throughout). A step is one complete fit
-(preprocessing, cross-product, randomized SVD, scores, prefix weights) with
-inputs already resident in HBM; value = fits per second of the mixed path.
+Headline (value): batched SIMPLS throughput, fits per second, on the 256-fit
+set (2 data sets x 5-fold CV x 20 row permutations of the responses + 56
+uneven singletons; per data set n ~ U[500, 2e4], p ~ U[100, 5e3],
+q ~ U[1, 10], ncomp ~ U[2, 30]; latent-factor data, 10 factors, noise 0.1;
+oversample 10, power 2). A step is one batch of all 256 fits with X and Y
+already resident in HBM. The mixed path stores X and Y in FP32 and runs
+every SIMPLS product in FP64 (FP32 widened on load); the plain path stores
+FP64. Mixed and plain are compared on identical values (B and fitted values
+at every prefix of every fit).
 
-Accuracy of mixed against plain is reported on identical inputs (the plain
-path's FP64 buffers hold the FP32 values): Ŷ on the first 4096 training rows,
-and B through 4096 random probes x = mu + sigma z (z ~ N(0, I)):
-|Ŷ32(x) - Ŷ64(x)| / |Ŷ64(x) - mu_y| (Frobenius norms).
+Sections beside the headline, each with its own roofline and CPU baseline:
+  c1  plain SIMPLS 2000 x 1000, q 3, 10 components (FP64; single-fit latency
+      and 64 concurrent fits);
+  c2  PLS-SVD through the randomized SVD of XᵀY, n 1e5, p 2e4, q 50, k 20
+      (X FP32, 8 GB; bf16x6 matrix-core products);
+  c4  PLS-DA: SIMPLS scores + LDA, n 5e4, p 1e4, 20 classes, 15 components,
+      10 000 held-out rows classified;
+  c5  serving: 10 000 requests, Poisson 500/s, 80% predict / 20% fit.
+
+CPU baselines run the FP64 oracle (a restatement of the reference CPU core,
+pinned to it by the tests) with its products on numpy's bundled OpenBLAS
+dgemm, at 1 thread (the reference default: OPENBLAS_NUM_THREADS unset) and
+at the host's core share, on the GPU box's host.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 `--calibrate-copy MIB` instead times one device-to-device copy of MIB MiB
@@ -65,35 +75,81 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def run_fits(fp, xbuf, ybuf, n, p, q, precision, k, steps, warmup, label):
-    lat, phases = [], []
-    model = None
-    for i in range(warmup + steps):
-        t0 = time.perf_counter()
-        m = fp.ResidentModel.fit_device(xbuf.ptr, ybuf.ptr, None, n, p, q, method=fp.PLSSVD,
-                                        precision=precision, components=k, scaling=2,
-                                        oversample=10, power=2, seed=1)
-        t1 = time.perf_counter()
-        ph = m.fit_phases()
-        if i >= warmup:
-            lat.append(t1 - t0)
-            phases.append(ph)
-        log(f"[{label}] fit {i + 1}/{warmup + steps}: wall {1e3 * (t1 - t0):.2f} ms, "
-            f"device {1e3 * ph['fit']:.2f} ms, cross-product pass {1e3 * ph['crossprod_pass']:.3f} ms, "
-            f"score pass {1e3 * ph['score_pass']:.3f} ms")
-        if model is not None:
-            model.close()
-        model = m
-    return model, np.array(lat), phases
+def pct(values, q):
+    return round(1e3 * float(np.percentile(np.asarray(values), q)), 3)
 
 
-def cpu_baseline(rows, p, q, k):
-    from oracle.oracle import Oracle, latent_data
-    orc = Oracle()
-    X, Y = latent_data(rows, p, q, factors=30, noise=0.05, seed=0, decay=0.8)
-    t0 = time.perf_counter()
-    orc.plssvd_fit(X, Y, ncomp=k, oversample=10, power=2, seed=1)
-    return time.perf_counter() - t0
+def roof(bytes_alg, seconds, kernel, traffic=None, **extra):
+    achieved = bytes_alg / seconds
+    out = {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
+           "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
+           "algorithmic_bytes": bytes_alg, "kernel": kernel, "kernel_ms": round(1e3 * seconds, 4)}
+    out.update(extra)
+    return out
+
+
+def profile_traffic(name, kernel, workload_prefix):
+    """HBM bytes per launch from a committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary, when it was taken on this kernel and workload."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    tr = json.load(open(path))
+    if tr.get("kernel") == kernel and tr.get("workload", "").startswith(workload_prefix):
+        return tr["traffic_bytes"]
+    return None
+
+
+# ------------------------------------------------------------- CPU baseline
+def host_cpu():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    return model, os.cpu_count(), min(share, len(os.sched_getaffinity(0)))
+
+
+class CpuBaseline:
+    """The FP64 oracle with its products on numpy's bundled OpenBLAS."""
+
+    def __init__(self):
+        from oracle.oracle import Oracle
+        self.o = Oracle()
+        self.model, self.nproc, self.share = host_cpu()
+        self.blas = None
+
+    def threads(self, t):
+        self.blas = self.o.use_blas(t)
+
+    def close(self):
+        self.o.use_blas(None)
+
+    def run(self, fns, budget_s, min_runs=1, max_runs=None):
+        """Times fns in order (cycling) until budget_s has passed and at least
+        min_runs ran, at most max_runs (default: every fn once, or min_runs);
+        returns the list of seconds."""
+        cap = max_runs if max_runs is not None else max(min_runs, len(fns))
+        out = []
+        t_all = time.perf_counter()
+        while len(out) < cap:
+            t0 = time.perf_counter()
+            fns[len(out) % len(fns)]()
+            out.append(time.perf_counter() - t0)
+            if len(out) >= min_runs and time.perf_counter() - t_all >= budget_s:
+                break
+        return out
+
+    def record(self, value, unit, threads, kind, sample, extra=None):
+        rec = {"value": value, "unit": unit, "cores": threads, "kind": kind, "sample": sample,
+               "cpu_model": self.model, "nproc": self.nproc, "blas": self.blas}
+        if extra:
+            rec.update(extra)
+        return rec
 
 
 def calibrate_copy(fp, mib):
@@ -107,17 +163,23 @@ def calibrate_copy(fp, mib):
           flush=True)
 
 
-def c3_batched(fp, seed=2026, details_out=None):
-    """Batched SIMPLS: 256 fits = 2 datasets x 5-fold CV x 20 row permutations
-    of the training responses (the fits of one fold share X) + 56 uneven
-    singletons; per dataset n ~ U[500, 20000], p ~ U[100, 5000],
-    q ~ U[1, 10], ncomp ~ U[2, 30]; latent-factor data (10 factors, noise
-    0.1); per-fit rSVD seed 1000 + i. Times the whole batch in FP32 (mixed)
-    and in FP64 (plain, the same values) and, as the baseline, the plain FP64
-    fits one at a time through the single-fit entry (device-resident
-    inputs). Inputs are staged before the timed regions."""
+# --------------------------------------------------------------- C3 (head)
+C3_WORKLOAD = ("batched_simpls 256 fits: 2 x (5-fold CV x 20 response-row permutations) + 56 "
+               "singletons; n~U[500,2e4], p~U[100,5e3], q~U[1,10], ncomp~U[2,30]; latent-factor "
+               "data (10 factors, noise 0.1); oversample 10, power 2; X, Y FP32 in HBM")
+# phase -> kernel of that phase in the 256-fit batch (the narrow groups have
+# one right-hand column, the shared-X groups 20)
+C3_KERNELS = {"crossprod": "k_xtw / k_xtw_f64 (column statistics + cross-products, FP64 products)",
+              "score_valu": "k_xw<float,1> (score pass t = X_s r, FP64 VALU)",
+              "score_mfma": "k_xw_f64<32,float> (score pass of shared-X groups, FP64 MFMA)",
+              "loading_valu": "k_xtw<float,4> (loading pass w = X_s' t, FP64 VALU)",
+              "loading_mfma": "k_xtw_f64<32,float> (loading pass of shared-X groups, FP64 MFMA)"}
+
+
+def c3_stage(fp, sample_count, seed=2026):
     rng = np.random.default_rng(seed)
-    keep, groups32, groups64, fits32, fits64, seq = [], [], [], [], [], []
+    sample = set(np.random.default_rng(7).choice(256, sample_count, replace=False).tolist())
+    keep, groups32, groups64, fits32, fits64, seq, host = [], [], [], [], [], [], {}
     for d in range(58):
         n = int(rng.integers(500, 20001)); p = int(rng.integers(100, 5001))
         q = int(rng.integers(1, 11)); A = int(rng.integers(2, 31))
@@ -125,7 +187,6 @@ def c3_batched(fp, seed=2026, details_out=None):
         fp.latent_data_device(xb.ptr, yb.ptr, 32, n, p, q, factors=10, noise=0.1, seed=100 + d)
         X = xb.download((n, p), np.float32); Y = yb.download((n, q), np.float32)
         xb.free(); yb.free()
-        parts = []
         if d < 2:
             folds = np.arange(n) % 5
             rng.shuffle(folds)
@@ -151,86 +212,403 @@ def c3_batched(fp, seed=2026, details_out=None):
                 fits32.append(dict(base, y=bufs[2].ptr))
                 fits64.append(dict(base, y=bufs[3].ptr))
                 yseq = fp.DeviceBuffer.from_array(ys.astype(np.float64)); keep.append(yseq)
-                seq.append((bufs[1].ptr, yseq.ptr, ng, p, q, A, 1000 + i, nperm > 0))
+                seq.append((bufs[1].ptr, yseq.ptr, ng, p, q, A, 1000 + i))
+                if i in sample:
+                    host[i] = (Xg, np.asfortranarray(ys), A, 1000 + i)
     assert len(fits32) == 256
-    log(f"[batched] 256 fits over {len(groups32)} shared-X groups staged")
+    return groups32, groups64, fits32, fits64, seq, keep, host
 
-    def run_batch(precision, groups, fits, reorth=-1):
+
+def c3_batched(fp, steps, warmup, cpu, cpu_budget):
+    groups32, groups64, fits32, fits64, seq, keep, host = c3_stage(fp, 32)
+    log(f"[c3] 256 fits over {len(groups32)} shared-X groups staged")
+
+    def run_batch(precision, groups, fits):
         t0 = time.perf_counter()
-        b = fp.SimplsBatch(precision, groups, fits, scaling=2, oversample=10, power=2,
-                           reorth=reorth)
+        b = fp.SimplsBatch(precision, groups, fits, scaling=2, oversample=10, power=2)
         wall = time.perf_counter() - t0
         done, flags, dev = b.status()
         assert all(f == 0 for f in flags) and all(d == f["components"] for d, f in zip(done, fits))
         return b, wall, dev
 
-    # warm-up, then timed batches
-    b32, _, _ = run_batch(32, groups32, fits32); b32.close()
-    b32, wall32, dev32 = run_batch(32, groups32, fits32)
-    log(f"[batched] mixed: {1e3 * wall32:.1f} ms wall, {1e3 * dev32:.1f} ms device")
+    for _ in range(warmup):
+        run_batch(32, groups32, fits32)[0].close()
+    walls, devs = [], []
+    b32 = None
+    for s in range(steps):
+        if b32 is not None:
+            b32.close()
+        b32, wall, dev = run_batch(32, groups32, fits32)
+        walls.append(wall); devs.append(dev)
+        log(f"[c3] mixed batch {s + 1}/{steps}: {1e3 * wall:.1f} ms wall, {1e3 * dev:.1f} ms device")
     b64, wall64, dev64 = run_batch(64, groups64, fits64)
-    log(f"[batched] plain: {1e3 * wall64:.1f} ms wall, {1e3 * dev64:.1f} ms device")
-    # mixed against plain on the same values, for every fit (the permuted
-    # refits included) at every prefix k <= ncomp: fitted values T_k Q_kᵀ and
-    # coefficients B_k = R_k Q_kᵀ (relative Frobenius errors)
-    err = dict(fitted=0.0, b=0.0, fitted_rank=0.0, b_rank=0.0)
-    details = []
+    log(f"[c3] plain batch: {1e3 * wall64:.1f} ms wall, {1e3 * dev64:.1f} ms device")
+    # mixed against plain on identical values, every fit, every prefix
+    err = dict(fitted=0.0, b=0.0)
     for i, f in enumerate(fits32):
-        T32, Q32 = b32.export(i, "T").astype(np.float64), b32.export(i, "Q").astype(np.float64)
-        R32 = b32.export(i, "R").astype(np.float64)
-        T64, Q64, R64 = b64.export(i, "T"), b64.export(i, "Q"), b64.export(i, "R")
-        A = f["components"]
-        e_fit = [rel(T32[:, :k] @ Q32[:, :k].T, T64[:, :k] @ Q64[:, :k].T) for k in range(1, A + 1)]
-        e_b = [rel(R32[:, :k] @ Q32[:, :k].T, R64[:, :k] @ Q64[:, :k].T) for k in range(1, A + 1)]
-        kr = min(A, 10)
-        e = dict(fitted=max(e_fit), b=max(e_b), fitted_rank=e_fit[kr - 1], b_rank=e_b[kr - 1])
-        for key, v in e.items():
-            err[key] = max(err[key], v)
-        g = groups32[f["group"]]
-        details.append(dict(fit=i, n=g[1], p=g[2], q=f["q"], A=A, permuted="perm" in f, **e))
+        # the FP64 values each fit holds (an FP32 export would round R and Q,
+        # and B = R Qᵀ of an ill-conditioned fit magnifies that rounding)
+        T32, Q32, R32 = (b32.export(i, k, f64=True) for k in ("T", "Q", "R"))
+        T64, Q64, R64 = (b64.export(i, k, f64=True) for k in ("T", "Q", "R"))
+        for k in range(1, f["components"] + 1):
+            err["fitted"] = max(err["fitted"], rel(T32[:, :k] @ Q32[:, :k].T, T64[:, :k] @ Q64[:, :k].T))
+            err["b"] = max(err["b"], rel(R32[:, :k] @ Q32[:, :k].T, R64[:, :k] @ Q64[:, :k].T))
     b32.close(); b64.close()
-    if details_out is not None:
-        details_out.extend(details)
-    # baseline: plain FP64, one fit at a time
+    # phases of one more mixed batch, with events around every launch
+    os.environ["FASTPLS_BENCH_PHASE_TIMING"] = "1"
+    try:
+        bt, wall_t, _ = run_batch(32, groups32, fits32)
+        ph = bt.phases()
+        bt.close()
+    finally:
+        del os.environ["FASTPLS_BENCH_PHASE_TIMING"]
+    # plain FP64, one fit at a time through the single-fit entry
     lat = []
-    for (xp, yp, n, p, q, A, sd, _) in seq:
+    for (xp, yp, n, p, q, A, sd) in seq:
         t0 = time.perf_counter()
         m = fp.ResidentModel.fit_device(xp, yp, None, n, p, q, method=fp.SIMPLS, precision=64,
                                         components=A, scaling=2, oversample=10, power=2, seed=sd)
         lat.append(time.perf_counter() - t0)
         m.close()
     lat = np.array(lat)
-    log(f"[batched] one at a time (plain FP64): {1e3 * lat.sum():.1f} ms")
+    log(f"[c3] one at a time (plain FP64): {1e3 * lat.sum():.1f} ms")
     for b in keep:
         b.free()
+
+    xpass = {k: ph[k] for k in C3_KERNELS if ph[k]["launches"]}
+    dom = max(xpass, key=lambda k: xpass[k]["seconds"])
+    d = xpass[dom]
+    per_launch_bytes = d["bytes"] / d["launches"]
+    per_launch_s = d["seconds"] / d["launches"]
+    kname = C3_KERNELS[dom].split(" ")[0]
+    roofline = roof(per_launch_bytes, per_launch_s, C3_KERNELS[dom],
+                    traffic=profile_traffic("simpls_batch_c3_traffic.json", kname, "batched_simpls"),
+                    launches=d["launches"],
+                    note="per-launch averages over the batch's launches of this kernel "
+                         "(algorithmic bytes: X, the right-hand side and the output once each)",
+                    phases_ms={k: round(1e3 * v["seconds"], 3) for k, v in ph.items()
+                               if isinstance(v, dict)},
+                    phases_frac={k: round(v["bytes"] / v["seconds"] / HBM_PEAK, 4)
+                                 for k, v in xpass.items()},
+                    batch_device_ms_with_events=round(1e3 * ph["seconds"], 3),
+                    batch_bytes_streamed=sum(v["bytes"] for v in xpass.values()),
+                    batch_frac_of_hbm=round(sum(v["bytes"] for v in xpass.values()) /
+                                            ph["seconds"] / HBM_PEAK, 4))
+
+    cpu_rec = None
+    if cpu is not None:
+        order = sorted(host)
+        data = [(np.asfortranarray(host[i][0], dtype=np.float64),
+                 np.asfortranarray(host[i][1], dtype=np.float64), host[i][2], host[i][3])
+                for i in order]
+
+        def make(X, Y, A, sd):
+            return lambda: cpu.o.simpls_fit(X, Y, ncomp=A, scaling=2, oversample=10, power=2,
+                                            seed=sd)
+        fns = [make(*t) for t in data]
+        cpu.threads(1)
+        t1 = cpu.run(fns, cpu_budget, min_runs=3)
+        cpu.threads(cpu.share)
+        ta = cpu.run(fns[:len(t1)], 0.0, min_runs=len(t1))
+        cpu_rec = cpu.record(round(len(t1) / sum(t1), 4), "fits/s", 1, "port",
+                             f"the first {len(t1)} of 32 fits drawn from the 256 (seeded), each "
+                             f"fitted once in FP64 (prepare + SIMPLS), {sum(t1):.1f} s in total",
+                             {"value_all_cores": round(len(ta) / sum(ta), 4),
+                              "cores_all": cpu.share})
+        log(f"[c3] CPU oracle + OpenBLAS: {cpu_rec['value']} fits/s (1 thread), "
+            f"{cpu_rec['value_all_cores']} fits/s ({cpu.share} threads)")
+    wall = float(np.sum(walls))
     return {
-        "workload": "256 SIMPLS fits: 2 x (5-fold CV x 20 response-row permutations) + 56 "
-                    "singletons; n~U[500,2e4], p~U[100,5e3], q~U[1,10], ncomp~U[2,30]; "
-                    "oversample 10, power 2",
-        "batched_mixed_fits_per_s": round(256 / wall32, 2),
-        "batched_mixed_wall_ms": round(1e3 * wall32, 2),
-        "batched_plain_fits_per_s": round(256 / wall64, 2),
+        "value": round(256 * steps / wall, 2),
+        "ms_per_step": round(1e3 * wall / steps, 3),
+        "p50_ms": pct(walls, 50), "p99_ms": pct(walls, 99),
+        "device_ms_per_step": round(1e3 * float(np.mean(devs)), 3),
+        "plain_batched_fits_per_s": round(256 / wall64, 2),
+        "plain_batched_ms": round(1e3 * wall64, 3),
         "sequential_plain_fits_per_s": round(256 / float(lat.sum()), 2),
-        "sequential_plain_p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
-        "sequential_plain_p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
-        "speedup_batched_mixed_vs_sequential_plain": round(float(lat.sum()) / wall32, 2),
-        "speedup_batched_mixed_vs_batched_plain": round(wall64 / wall32, 2),
-        "max_rel_err_fitted_mixed_vs_plain_at_rank": err["fitted_rank"],
-        "max_rel_err_B_mixed_vs_plain_at_rank": err["b_rank"],
-        "max_rel_err_fitted_mixed_vs_plain_all_components": err["fitted"],
-        "max_rel_err_B_mixed_vs_plain_all_components": err["b"],
+        "sequential_plain_p50_ms": pct(lat, 50), "sequential_plain_p99_ms": pct(lat, 99),
+        "speedup_batched_mixed_vs_sequential_plain": round(float(lat.sum()) / (wall / steps), 2),
+        "speedup_batched_mixed_vs_batched_plain": round(wall64 / (wall / steps), 3),
+        "max_rel_err_fitted_mixed_vs_plain_all_fits_all_prefixes": err["fitted"],
+        "max_rel_err_B_mixed_vs_plain_all_fits_all_prefixes": err["b"],
         "accuracy_gate_1e-5_met": bool(max(err.values()) <= 1e-5),
-        "accuracy_note": "maxima over all 256 fits (permuted refits included) and every prefix "
-                         "k <= ncomp ('at rank': k = min(ncomp, 10), the data's latent rank); "
-                         "relative Frobenius errors of T_k Q_k' and R_k Q_k' against the plain "
-                         "FP64 batch on identical values",
+        "roofline": roofline,
+        "cpu_baseline": cpu_rec,
     }
 
 
+# ------------------------------------------------------------------- C1
+def c1_plain(fp, steps, cpu):
+    from oracle.oracle import latent_data
+    n, p, q, A = 2000, 1000, 3, 10
+    X, Y = latent_data(n, p, q, factors=10, noise=0.1, seed=0)
+    res = {"workload": "simpls n=2000 p=1000 q=3 ncomp=10, latent-factor data (10 factors, "
+                       "noise 0.1, seed 0), autoscaling, oversample 32, power 5, rSVD seed 1"}
+    models = {}
+    for prec, dt in ((64, np.float64), (32, np.float32)):
+        xb, yb = fp.DeviceBuffer.from_array(X.astype(dt)), fp.DeviceBuffer.from_array(Y.astype(dt))
+        lat, dev = [], []
+        for i in range(steps + 2):
+            t0 = time.perf_counter()
+            m = fp.ResidentModel.fit_device(xb.ptr, yb.ptr, None, n, p, q, method=fp.SIMPLS,
+                                            precision=prec, components=A, scaling=2,
+                                            oversample=32, power=5, seed=1)
+            wall = time.perf_counter() - t0
+            if i >= 2:
+                lat.append(wall); dev.append(m.fit_phases()["fit"])
+            if i < steps + 1:
+                m.close()
+        models[prec] = m
+        xb.free(); yb.free()
+        key = "plain" if prec == 64 else "mixed"
+        res[f"{key}_fits_per_s"] = round(len(lat) / float(np.sum(lat)), 2)
+        res[f"{key}_p50_ms"] = pct(lat, 50)
+        res[f"{key}_p99_ms"] = pct(lat, 99)
+        res[f"{key}_device_ms"] = round(1e3 * float(np.median(dev)), 3)
+        if prec == 64:
+            bytes_alg = (1 + A) * n * p * 8.0
+            res["roofline"] = roof(bytes_alg, float(np.median(dev)),
+                                   "whole fit (X passes + per-component small kernels)",
+                                   note="(1+A) n p 8 algorithmic bytes over the fit's device time: "
+                                        "X (16 MB) sits in the 256 MB MALL and the fit is bound by "
+                                        "the latency of its 10 dependent component steps")
+    # 64 concurrent C1-shaped fits (64 data sets) in one batch
+    bufs, groups, fits = [], [], []
+    for d in range(64):
+        Xd, Yd = latent_data(n, p, q, factors=10, noise=0.1, seed=d)
+        xb, yb = fp.DeviceBuffer.from_array(Xd), fp.DeviceBuffer.from_array(Yd)
+        bufs += [xb, yb]
+        groups.append((xb.ptr, n, p))
+        fits.append(dict(group=d, q=q, components=A, seed=1, y=yb.ptr))
+    fp.SimplsBatch(64, groups, fits, scaling=2, oversample=32, power=5).close()
+    t0 = time.perf_counter()
+    b = fp.SimplsBatch(64, groups, fits, scaling=2, oversample=32, power=5)
+    bw = time.perf_counter() - t0
+    b.close()
+    for bb in bufs:
+        bb.free()
+    res["plain_batched_64_fits_per_s"] = round(64 / bw, 2)
+    # accuracy: plain against the oracle, mixed against plain (B, fitted values at k = 10)
+    o = None
+    if cpu is not None:
+        o = cpu.o.simpls_fit(X, Y, ncomp=A, scaling=2, oversample=32, power=5, seed=1)
+    m64, m32 = models[64], models[32]
+    B64 = m64.coefficients(A)
+    res["rel_err_B_mixed_vs_plain"] = rel(m32.coefficients(A), B64)
+    if o is not None:
+        res["rel_err_B_plain_vs_oracle"] = rel(B64, o["R"] @ o["Q"].T)
+        Xs = (X - o["mx"]) / o["sx"]
+        res["rel_err_fitted_plain_vs_oracle"] = rel(Xs @ B64, Xs @ (o["R"] @ o["Q"].T))
+    m64.close(); m32.close()
+    if cpu is not None:
+        fn = [lambda: cpu.o.simpls_fit(X, Y, ncomp=A, scaling=2, oversample=32, power=5, seed=1)]
+        cpu.threads(1)
+        t1 = cpu.run(fn, 3.0, min_runs=5, max_runs=50)
+        cpu.threads(cpu.share)
+        ta = cpu.run(fn, 1.5, min_runs=5, max_runs=50)
+        res["cpu_baseline"] = cpu.record(round(1.0 / float(np.median(t1)), 3), "fits/s", 1, "port",
+                                         f"the full fit, median of {len(t1)} runs",
+                                         {"value_all_cores": round(1.0 / float(np.median(ta)), 3),
+                                          "cores_all": cpu.share})
+        log(f"[c1] CPU {res['cpu_baseline']['value']} fits/s; GPU plain {res['plain_fits_per_s']}, "
+            f"mixed {res['mixed_fits_per_s']}, 64 batched {res['plain_batched_64_fits_per_s']}")
+    return res
+
+
+# ------------------------------------------------------------------- C2
+def c2_plssvd(fp, steps, warmup, plain_steps, cpu, cpu_rows, cpu_budget):
+    n, p, q, k = 100000, 20000, 50, 20
+    rows = 4096
+    res = {"workload": f"plssvd_rsvd n={n} p={p} q={q} k={k} oversample=10 power=2 (30 latent "
+                       "factors, column norms 0.8^j, noise 0.05; X fp32 8.0 GB)"}
+
+    def run_fits(xb, yb, precision, count, wu, label):
+        lat, phases, model = [], [], None
+        for i in range(wu + count):
+            t0 = time.perf_counter()
+            m = fp.ResidentModel.fit_device(xb.ptr, yb.ptr, None, n, p, q, method=fp.PLSSVD,
+                                            precision=precision, components=k, scaling=2,
+                                            oversample=10, power=2, seed=1)
+            t1 = time.perf_counter()
+            ph = m.fit_phases()
+            if i >= wu:
+                lat.append(t1 - t0); phases.append(ph)
+            log(f"[c2 {label}] fit {i + 1}/{wu + count}: wall {1e3 * (t1 - t0):.2f} ms, "
+                f"device {1e3 * ph['fit']:.2f} ms, cross-product pass "
+                f"{1e3 * ph['crossprod_pass']:.3f} ms, score pass {1e3 * ph['score_pass']:.3f} ms")
+            if model is not None:
+                model.close()
+            model = m
+        return model, np.array(lat), phases
+
+    x32, y32 = fp.DeviceBuffer(n * p * 4), fp.DeviceBuffer(n * q * 4)
+    fp.latent_data_device(x32.ptr, y32.ptr, 32, n, p, q, factors=30, decay=0.8, noise=0.05, seed=0)
+    m32, lat32, ph32 = run_fits(x32, y32, 32, steps, warmup, "mixed")
+    xr32 = gather_rows(x32, n, p, rows, np.float32)
+    x32.free(); y32.free()
+    res["value"] = round(len(lat32) / float(lat32.sum()), 4)
+    res["unit"] = "fits/s"
+    res["p50_ms"], res["p99_ms"] = pct(lat32, 50), pct(lat32, 99)
+    res["device_ms_per_fit"] = round(1e3 * float(np.mean([ph["fit"] for ph in ph32])), 3)
+    x64, y64 = fp.DeviceBuffer(n * p * 8), fp.DeviceBuffer(n * q * 8)
+    fp.latent_data_device(x64.ptr, y64.ptr, 64, n, p, q, factors=30, decay=0.8, noise=0.05,
+                          seed=0, round32=True)
+    m64, lat64, _ = run_fits(x64, y64, 64, plain_steps, 1, "plain")
+    x64.free(); y64.free()
+    yhat32 = m32.predict_path(xr32, [k]).astype(np.float64)[:, :, 0]
+    yhat64 = m64.predict_path(xr32.astype(np.float64), [k])[:, :, 0]
+    mu, sd, my = m64.export("mX"), m64.export("vX"), m64.export("mY")
+    z = np.random.default_rng(7).standard_normal((rows, p))
+    probe = np.asfortranarray(mu[None, :] + sd[None, :] * z)
+    b32 = m32.predict_path(probe.astype(np.float32), [k]).astype(np.float64)[:, :, 0]
+    b64 = m64.predict_path(probe, [k])[:, :, 0]
+    s32, s64 = m32.export("R"), m64.export("R")
+    sig_u = np.linalg.norm(s64.astype(np.float64) - s32.astype(np.float64) @
+                           (s32.astype(np.float64).T @ s64), 2)
+    res.update({
+        "plain_fits_per_s": round(plain_steps / float(lat64.sum()), 4),
+        "plain_ms_per_fit": round(1e3 * float(np.median(lat64)), 3),
+        "speedup_mixed_vs_plain": round(float(np.median(lat64)) / float(np.median(lat32)), 3),
+        "rel_err_yhat_mixed_vs_plain": rel(yhat32, yhat64),
+        "rel_err_B_probe_mixed_vs_plain": float(np.linalg.norm(b32 - b64) /
+                                                np.linalg.norm(b64 - my[None, :])),
+        "subspace_angle_R_mixed_vs_plain": float(sig_u),
+    })
+    m32.close(); m64.close()
+    cp = np.array([ph["crossprod_pass"] for ph in ph32])
+    sc = np.array([ph["score_pass"] for ph in ph32])
+    cp_bytes = float(n) * p * 4 + float(n) * q * 4
+    sc_bytes = float(n) * p * 4 + float(n) * k * 8
+    res["roofline"] = roof(cp_bytes, float(np.mean(cp)),
+                           "k_xtw_bf6<64> (column stats + X^T Y, one pass over X, bf16x6 MFMA)",
+                           traffic=profile_traffic("plssvd_c2_traffic.json", "k_xtw_bf6<64>",
+                                                   f"plssvd_rsvd n={n} p={p} q={q} k={k}"),
+                           score_pass_kernel="k_xw_bf6<32> (T = X_s R, bf16x6 MFMA)",
+                           score_pass_ms=round(1e3 * float(np.mean(sc)), 4),
+                           score_pass_frac=round(sc_bytes / float(np.mean(sc)) / HBM_PEAK, 4),
+                           fit_frac_two_passes=round((cp_bytes + sc_bytes) /
+                                                     float(np.mean([ph["fit"] for ph in ph32])) /
+                                                     HBM_PEAK, 4))
+    if cpu is not None:
+        from oracle.oracle import latent_data
+        Xc, Yc = latent_data(cpu_rows, p, q, factors=30, noise=0.05, seed=0, decay=0.8)
+        fn = [lambda: cpu.o.plssvd_fit(Xc, Yc, ncomp=k, oversample=10, power=2, seed=1)]
+        cpu.threads(1)
+        t1 = cpu.run(fn, cpu_budget, min_runs=1, max_runs=3)
+        cpu.threads(cpu.share)
+        ta = cpu.run(fn, cpu_budget / 3, min_runs=3, max_runs=5)
+        scale = n / cpu_rows
+        res["cpu_baseline"] = cpu.record(
+            round(1.0 / (float(np.median(t1)) * scale), 6), "fits/s", 1, "port",
+            f"FP64 PLS-SVD fit on {cpu_rows} of {n} rows (same p, q, k, oversample, power), "
+            f"median {float(np.median(t1)):.2f} s of {len(t1)} run(s), scaled linearly in n",
+            {"value_all_cores": round(1.0 / (float(np.median(ta)) * scale), 6),
+             "cores_all": cpu.share})
+        log(f"[c2] CPU {res['cpu_baseline']}")
+    return res
+
+
+# ------------------------------------------------------------------- C4
+def class_split(n, ntest, p, K, latent=15, spread=0.5, seed=2):
+    """Balanced PLS-DA data: class means spread * N(0, I) in a latent space
+    embedded through a p x latent loading, plus N(0, 1) noise; training and
+    held-out rows share the means and the loading."""
+    rng = np.random.default_rng(seed)
+    means = spread * rng.standard_normal((K, latent))
+    L = rng.standard_normal((p, latent))
+
+    def draw(rows):
+        lab = (np.arange(rows) % K) + 1
+        rng.shuffle(lab)
+        X = means[lab - 1] @ L.T + rng.standard_normal((rows, p))
+        return np.asfortranarray(X), lab.astype(np.int32)
+    Xtr, ltr = draw(n)
+    Xte, lte = draw(ntest)
+    return Xtr, ltr, Xte, lte
+
+
+def c4_plsda(fp, steps, cpu):
+    n, p, K, A, nt = 50000, 10000, 20, 15, 10000
+    t0 = time.perf_counter()
+    Xtr, ltr, Xte, lte = class_split(n, nt, p, K)
+    log(f"[c4] data generated in {time.perf_counter() - t0:.1f} s")
+    res = {"workload": f"plsda n={n} p={p} K={K} ncomp={A} (balanced, 15-dim latent class means "
+                       f"N(0, 0.5^2 I), noise N(0,1), seed 2) + LDA classification of {nt} "
+                       "held-out rows"}
+    out = {}
+    for prec, dt in ((32, np.float32), (64, np.float64)):
+        xb = fp.DeviceBuffer.from_array(Xtr.astype(dt))
+        lb = fp.DeviceBuffer.from_array(ltr)
+        Xq = np.asfortranarray(Xte.astype(dt))
+        lat, dev, proj = [], [], []
+        for i in range(steps + 1):
+            t0 = time.perf_counter()
+            m = fp.ResidentModel.fit_device(xb.ptr, None, lb.ptr, n, p, K, method=fp.SIMPLS,
+                                            precision=prec, components=A, scaling=2,
+                                            oversample=32, power=5, seed=1)
+            lab = m.classify_path(Xq, [A], lda=True)[:, 0, 0]
+            wall = time.perf_counter() - t0
+            ph = m.fit_phases()
+            if i >= 1:
+                lat.append(wall); dev.append(ph["fit"]); proj.append(ph["projection"])
+            if i < steps:
+                m.close()
+        disc = m.predict_path(Xq, [A], lda=True)[:, :, 0].astype(np.float64)
+        out[prec] = (lab, disc, m)
+        xb.free(); lb.free()
+        key = "mixed" if prec == 32 else "plain"
+        res[f"{key}_fit_predict_per_s"] = round(len(lat) / float(np.sum(lat)), 3)
+        res[f"{key}_p50_ms"] = pct(lat, 50)
+        res[f"{key}_device_fit_ms"] = round(1e3 * float(np.median(dev)), 3)
+        res[f"{key}_device_predict_projection_ms"] = round(1e3 * float(np.median(proj)), 3)
+        if prec == 32:
+            bytes_alg = 2.0 * n * p * 4 + float(nt) * p * 4
+            res["roofline"] = roof(bytes_alg, float(np.median(dev)) + float(np.median(proj)),
+                                   "fit (class sums + block refresh + 2 X passes) + held-out "
+                                   "projection", note="2 n p 4 + n_test p 4 algorithmic bytes over "
+                                   "the fit's and the projection's device time")
+
+    def mismatches(a, ref_lab, ref_disc):
+        top2 = np.sort(ref_disc, axis=1)[:, -2:]
+        margin = top2[:, 1] - top2[:, 0]
+        bad = (a != ref_lab) & (margin >= 1e-6)
+        return int(bad.sum()), int(((a != ref_lab) & (margin < 1e-6)).sum())
+
+    lab32, disc32, m32 = out[32]
+    lab64, disc64, m64 = out[64]
+    res["label_mismatch_mixed_vs_plain"], res["near_tie_mixed_vs_plain"] = \
+        mismatches(lab32, lab64, disc64)
+    res["test_accuracy_mixed"] = round(float(np.mean(lab32 == lte)), 4)
+    m32.close(); m64.close()
+    if cpu is not None:
+        Xd = np.asfortranarray(Xtr, dtype=np.float64)
+
+        def fit_predict():
+            o = cpu.o.simpls_fit(Xd, labels=ltr, classes=K, ncomp=A, scaling=2, oversample=32,
+                                 power=5, seed=1)
+            ts = ((Xte - o["mx"]) / o["sx"]) @ o["R"]
+            return cpu.o.lda(o["T"], ltr, K, A, np.asfortranarray(ts))
+        cpu.threads(cpu.share)
+        ta = cpu.run([fit_predict], 0.0, min_runs=1)
+        lda = fit_predict()
+        res["label_mismatch_plain_vs_oracle"], res["near_tie_plain_vs_oracle"] = \
+            mismatches(lab64, lda["labels"], lda["disc"])
+        cpu.threads(1)
+        t1 = cpu.run([fit_predict], 0.0, min_runs=1)
+        res["cpu_baseline"] = cpu.record(round(1.0 / t1[0], 4), "fits+predicts/s", 1, "port",
+                                         "the full fit + held-out LDA prediction, one run",
+                                         {"value_all_cores": round(1.0 / ta[0], 4),
+                                          "cores_all": cpu.share})
+        log(f"[c4] CPU {res['cpu_baseline']}; labels vs oracle "
+            f"{res['label_mismatch_plain_vs_oracle']} mismatches")
+    return res
+
+
+# ------------------------------------------------------------------- C5
 def c5_serving(fp, requests=10000, rate=500.0, seed=3, workers=32, max_fits=0):
     """Serving stream: `requests` requests with Poisson arrivals at `rate`/s
     (seed 3) against the request server (HBM budget 200 GB, FP32 storage,
-    bf16x6 matrix-core products with FP32 accumulation). 80% predict 1..4096
+    FP64 products for these SIMPLS models). 80% predict 1..4096
     rows against one of 16 resident SIMPLS models (p ~ U[1000, 20000],
     q ~ U[1, 20], 10 components, fitted from 4096 latent-factor rows); 20%
     fit a SIMPLS model (n log-U[1e3, 1e5], p log-U[1e2, 2e4], ncomp
@@ -303,7 +681,7 @@ def c5_serving(fp, requests=10000, rate=500.0, seed=3, workers=32, max_fits=0):
         "workload": f"{requests} requests, Poisson {rate:g}/s (seed {seed}); 80% predict "
                     "(1..4096 rows, 16 resident SIMPLS models, p~U[1e3,2e4], q~U[1,20]); 20% "
                     "SIMPLS fit (n log-U[1e3,1e5], p log-U[1e2,2e4], ncomp U{2..25}, q U{1..20}, "
-                    "32 staged data sets); HBM budget 200 GB; FP32 storage, bf16x6 MFMA",
+                    "32 staged data sets); HBM budget 200 GB; FP32 storage",
         "offered_per_s": rate,
         "completed_per_s": round(requests / wall, 2),
         "predict_p50_ms": round(st["predict_p50_ms"], 3),
@@ -324,26 +702,46 @@ def c5_serving(fp, requests=10000, rate=500.0, seed=3, workers=32, max_fits=0):
     }
 
 
+
+def predict_roofline(fp, rows=4096, p=20000, q=20, A=10, reps=5):
+    """The prediction product of a resident FP32 SIMPLS model (the kernel a
+    predict request runs): scores = X_s R_k over `rows` x p query rows."""
+    xb, yb = fp.DeviceBuffer(rows * p * 4), fp.DeviceBuffer(rows * q * 4)
+    fp.latent_data_device(xb.ptr, yb.ptr, 32, rows, p, q, factors=10, noise=0.1, seed=901)
+    m = fp.ResidentModel.fit_device(xb.ptr, yb.ptr, None, rows, p, q, method=fp.SIMPLS,
+                                    precision=32, components=A, scaling=2, oversample=10,
+                                    power=2, seed=1)
+    Xq = xb.download((rows, p), np.float32)
+    t, nbytes = [], 0.0
+    for i in range(reps + 1):
+        m.predict_path(Xq, [A])
+        ph = m.fit_phases()
+        if i:
+            t.append(ph["projection"]); nbytes = ph["projection_bytes"]
+    m.close(); xb.free(); yb.free()
+    return roof(nbytes, float(np.mean(t)),
+                f"k_xw_f64<16,float> (predict scores X_s R_k, {rows} x {p} rows, k = {A}, FP64 MFMA)",
+                note="HIP events around the prediction product on the model's stream; the "
+                     "host-to-device copy of the rows is outside the interval")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=5, help="timed 256-fit batches (headline)")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--fit-steps", type=int, default=10, help="timed fits of the c1/c2 sections")
     ap.add_argument("--plain-steps", type=int, default=3)
-    ap.add_argument("--n", type=int, default=100000)
-    ap.add_argument("--p", type=int, default=20000)
-    ap.add_argument("--q", type=int, default=50)
-    ap.add_argument("--k", type=int, default=20)
-    ap.add_argument("--cpu-rows", type=int, default=6000)
+    ap.add_argument("--cpu-rows", type=int, default=20000, help="rows of the c2 CPU sample")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds per 1-thread CPU run")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB")
-    ap.add_argument("--no-batched", action="store_true")
-    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--sections", default="c1,c2,c4,c5",
+                    help="comma list of sections beside the headline (c1, c2, c4, c5)")
     ap.add_argument("--serve-requests", type=int, default=10000)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
-    if args.steps < 1 or args.plain_steps < 1:
-        ap.error("--steps and --plain-steps need at least one fit (use --no-plain to skip the FP64 line)")
+    if args.steps < 1 or args.fit_steps < 1 or args.plain_steps < 1:
+        ap.error("--steps, --fit-steps and --plain-steps need at least one")
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(independent fits would shard one scheduler per GPU).", file=sys.stderr)
@@ -355,124 +753,48 @@ def main():
         return
     info = fp.device_info()
     log(f"device: {info}")
-    n, p, q, k = args.n, args.p, args.q, args.k
-    rows = min(4096, n)
+    cpu = None if args.no_cpu else CpuBaseline()
+    if cpu is not None:
+        log(f"host CPU: {cpu.model}, nproc {cpu.nproc}, core share {cpu.share}")
+    sections = [s for s in args.sections.split(",") if s]
 
-    # ---------------------------------------------------- mixed (FP32 X)
-    log("generating FP32 latent-factor data in HBM")
-    x32 = fp.DeviceBuffer(n * p * 4)
-    y32 = fp.DeviceBuffer(n * q * 4)
-    fp.latent_data_device(x32.ptr, y32.ptr, 32, n, p, q, factors=30, decay=0.8, noise=0.05, seed=0)
-    m32, lat32, ph32 = run_fits(fp, x32, y32, n, p, q, 32, k, args.steps, args.warmup, "mixed")
-    wall32 = float(lat32.sum())
-    fps = args.steps / wall32
-    xr32 = gather_rows(x32, n, p, rows, np.float32)
-    x32.free(); y32.free()
-
-    # ---------------------------------------------------- plain (FP64 X)
-    plain = {}
-    if not args.no_plain:
-        # the same data as the mixed path (FP32 values held in FP64), so the
-        # comparison measures the mixed path's arithmetic alone
-        log("generating the same latent-factor data in FP64 storage")
-        x64 = fp.DeviceBuffer(n * p * 8)
-        y64 = fp.DeviceBuffer(n * q * 8)
-        fp.latent_data_device(x64.ptr, y64.ptr, 64, n, p, q, factors=30, decay=0.8, noise=0.05,
-                              seed=0, round32=True)
-        m64, lat64, ph64 = run_fits(fp, x64, y64, n, p, q, 64, k, args.plain_steps, 1, "plain")
-        x64.free(); y64.free()
-        yhat32 = m32.predict_path(xr32, [k]).astype(np.float64)[:, :, 0]
-        yhat64 = m64.predict_path(xr32.astype(np.float64), [k])[:, :, 0]
-        mu, sd, my = m64.export("mX"), m64.export("vX"), m64.export("mY")
-        z = np.random.default_rng(7).standard_normal((rows, p))
-        probe = np.asfortranarray(mu[None, :] + sd[None, :] * z)
-        b32 = m32.predict_path(probe.astype(np.float32), [k]).astype(np.float64)[:, :, 0]
-        b64 = m64.predict_path(probe, [k])[:, :, 0]
-        plain = {
-            "plain_fits_per_s": round(args.plain_steps / float(lat64.sum()), 4),
-            "plain_ms_per_fit": round(1e3 * float(np.median(lat64)), 3),
-            "speedup_mixed_vs_plain": round(float(np.median(lat64)) / float(np.median(lat32)), 3),
-            "rel_err_yhat_mixed_vs_plain": rel(yhat32, yhat64),
-            "rel_err_B_probe_mixed_vs_plain": float(np.linalg.norm(b32 - b64) /
-                                                    np.linalg.norm(b64 - my[None, :])),
-        }
-        m64.close()
-    m32.close()
-
-    # ------------------------------------------------- roofline (X passes)
-    # Each event interval spans the operand split kernel (tens of µs) and the
-    # matrix-core pass that reads X once: algorithmic bytes = n p 4 (X) +
-    # n q 4 (Y) for the cross-product pass and n p 4 + n k 8 for the scores.
-    cp = np.array([ph["crossprod_pass"] for ph in ph32])
-    sc = np.array([ph["score_pass"] for ph in ph32])
-    cp_bytes = float(n) * p * 4 + float(n) * q * 4
-    sc_bytes = float(n) * p * 4 + float(n) * k * 8
-    t_cp, t_sc = float(np.mean(cp)), float(np.mean(sc))
-    achieved = cp_bytes / t_cp
-    # HBM bytes per launch of the same kernel from the committed rocprofv3
-    # FETCH_SIZE / WRITE_SIZE runs (scaled by a copy of known size), when
-    # they were taken at this workload
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "plssvd_c2_traffic.json")
-    if os.path.exists(tpath):
-        tr = json.load(open(tpath))
-        if (tr.get("kernel") == "k_xtw_bf6<64>" and
-                tr.get("workload", "").startswith(f"plssvd_rsvd n={n} p={p} q={q} k={k}")):
-            traffic = tr["traffic_bytes"]
-    roofline = {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
-                "algorithmic_bytes": cp_bytes,
-                "kernel": "k_xtw_bf6<64> (column stats + X^T Y, one pass over X, bf16x6 MFMA)",
-                "kernel_ms": round(1e3 * t_cp, 4),
-                "score_pass_kernel": "k_xw_bf6<32> (T = X_s R, bf16x6 MFMA)",
-                "score_pass_ms": round(1e3 * t_sc, 4),
-                "score_pass_frac": round(sc_bytes / t_sc / HBM_PEAK, 4)}
-
-    # ----------------------------------------------- batched SIMPLS fits
-    batched = None
-    if not args.no_batched:
-        batched = c3_batched(fp)
-
-    # ------------------------------------------------------------ serving
-    serving = None
-    if not args.no_serving:
-        serving = c5_serving(fp, requests=args.serve_requests)
-
-    # ------------------------------------------------------ CPU baseline
-    cpu = None
-    if not args.no_cpu:
-        log(f"CPU oracle on a {args.cpu_rows}-row sample (1 thread)")
-        dt = cpu_baseline(args.cpu_rows, p, q, k)
-        t_full = dt * n / args.cpu_rows
-        cpu = {"value": round(1.0 / t_full, 6), "unit": "fits/s", "cores": 1, "kind": "port",
-               "sample": f"FP64 oracle PLS-SVD fit on {args.cpu_rows} of {n} rows (same p, q, k, "
-                         f"oversample, power), {dt:.2f} s, scaled linearly in n"}
+    head = c3_batched(fp, args.steps, args.warmup, cpu, args.cpu_budget)
+    extra = {}
+    if "c1" in sections:
+        extra["c1"] = c1_plain(fp, args.fit_steps, cpu)
+    if "c2" in sections:
+        extra["c2"] = c2_plssvd(fp, args.fit_steps, 2, args.plain_steps, cpu, args.cpu_rows,
+                                args.cpu_budget)
+    if "c4" in sections:
+        extra["c4"] = c4_plsda(fp, 3, cpu)
+    if "c5" in sections:
+        c5 = c5_serving(fp, requests=args.serve_requests)
+        c5["roofline"] = predict_roofline(fp)
+        extra["c5"] = c5
+    if cpu is not None:
+        cpu.close()
 
     out = {
         "metric": "SIMPLS fits/s + p50/p99 latency, mixed vs plain FP64 GPU at rel err <=1e-5",
-        "value": round(fps, 4),
+        "value": head["value"],
         "unit": "fits/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * wall32 / args.steps, 3),
-        "p50_ms": round(1e3 * float(np.percentile(lat32, 50)), 3),
-        "p99_ms": round(1e3 * float(np.percentile(lat32, 99)), 3),
-        "device_ms_per_fit": round(1e3 * float(np.mean([ph["fit"] for ph in ph32])), 3),
+        "ms_per_step": head["ms_per_step"],
+        "p50_ms": head["p50_ms"],
+        "p99_ms": head["p99_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 storage; bf16x6 MFMA with f32 accumulation; f64 small math",
+        "dtype": "f32 storage; f64 products and small math (SIMPLS)",
         "data": "synthetic latent-factor data generated in HBM (Philox4x32-10)",
-        "config": {"workload": f"plssvd_rsvd n={n} p={p} q={q} k={k} oversample=10 power=2 "
-                               f"(X fp32 {float(n) * p * 4 / 1e9:.1f} GB)",
-                   "n": n, "p": p, "q": q, "components": k},
+        "config": {"workload": C3_WORKLOAD, "fits_per_step": 256, "step": "one batch of 256 fits"},
         "device": info,
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        **plain,
-        "batched_simpls": batched,
-        "serving": serving,
+        "roofline": head.pop("roofline"),
+        "cpu_baseline": head.pop("cpu_baseline"),
+        "batched_simpls": head,
+        **extra,
     }
     print(json.dumps(out), flush=True)
 

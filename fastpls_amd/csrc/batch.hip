@@ -16,6 +16,7 @@
 #include "stream.hpp"
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <limits>
@@ -67,6 +68,10 @@ class SimplsBatch {
   static constexpr bool kF64 = true;
   // right-hand chunk of the per-step score pass
   static constexpr int kXwChunk = 64;
+  // Launch classes by right-hand width: 0 = VALU (below kMfmaMinN columns),
+  // 1..4 = matrix-core launches compiled for 16, 32, 48, 64 columns.
+  static constexpr int kWidthClasses = 5;
+  static int width_class(int N) { return N < kMfmaMinN ? 0 : mfma_width(N) / 16; }
   // longest row split of the cross-product and loading passes and longest
   // column split of the score pass (both within the matrix-core kernels'
   // limits, kXtwSplitRows and kXwSplitCols)
@@ -82,6 +87,10 @@ class SimplsBatch {
     if (groups_.empty() || fits_.empty()) throw std::invalid_argument("empty batch");
     if (scaling < 1 || scaling > 3) throw std::invalid_argument("scaling must be 1, 2 or 3");
     if (oversample < 0 || power < 0) throw std::invalid_argument("oversample and power must be non-negative");
+    // per-phase device times with events around every launch, as the
+    // reference's FASTPLS_BENCH_PHASE_TIMING switch (src/r_api.cpp:1182-1184)
+    const char* pt = std::getenv("FASTPLS_BENCH_PHASE_TIMING");
+    timing_ = pt != nullptr && std::strcmp(pt, "1") == 0;
     stream_ = stream_take();
     try {
       plan();
@@ -94,7 +103,13 @@ class SimplsBatch {
       float ms = 0.f;
       FPLS_HIP(hipEventElapsedTime(&ms, start.e, stop.e));
       seconds_ = double(ms) * 1e-3;
+      collect_phases();
     } catch (...) {
+      for (TimedLaunch& t : timed_) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+      }
+      timed_.clear();
       stream_give(stream_);
       stream_ = nullptr;
       throw;
@@ -107,11 +122,28 @@ class SimplsBatch {
 
   int count() const { return int(fits_.size()); }
   double seconds() const { return seconds_; }
+
+  // Phases of the batch: 0 column statistics + cross-products, 1 refresh,
+  // 2 / 3 score passes (VALU / matrix-core launches), 4 score slab sums,
+  // 5 / 6 loading passes (VALU / matrix-core), 7 update. out[0] = device
+  // seconds of the batch; phase i fills out[1 + 3i] seconds, out[2 + 3i]
+  // algorithmic bytes (X, the right-hand side and the output once each) and
+  // out[3 + 3i] launches. Zero unless FASTPLS_BENCH_PHASE_TIMING=1 was set
+  // when the batch was created.
+  static constexpr int kPhases = 8;
+  void phases(double* out, int count) const {
+    for (int i = 0; i < count; ++i) out[i] = 0.0;
+    if (count > 0) out[0] = seconds_;
+    for (int ph = 0; ph < kPhases; ++ph)
+      for (int j = 0; j < 3; ++j)
+        if (1 + 3 * ph + j < count) out[1 + 3 * ph + j] = phase_[ph][j];
+  }
   int completed(int f) const { return completed_[f]; }
   int flags(int f) const { return flags_[f]; }
 
-  // Fields as fastpls_hip_export: 0 R, 1 Q, 2 T, 3 x means, 4 x scales, 5 y means.
-  void export_field(int f, int field, void* out, size_t elements) {
+  // Fields as fastpls_hip_export: 0 R, 1 Q, 2 T, 3 x means, 4 x scales, 5 y
+  // means; in the batch precision, or as the FP64 values the fit holds (f64).
+  void export_field(int f, int field, void* out, size_t elements, bool f64 = false) {
     check_fit(f);
     const Fit& F = plan_[f];
     const Group& G = gplan_[fits_[f].group];
@@ -121,6 +153,10 @@ class SimplsBatch {
       throw std::invalid_argument("invalid batch field output size");
     const double* src = field == 0 ? F.R : field == 1 ? F.Q : field == 2 ? F.Tsc
                       : field == 3 ? G.mx : field == 4 ? G.sx : F.my;
+    if (f64) {
+      FPLS_HIP(hipMemcpy(out, src, sizeof(double) * elements, hipMemcpyDeviceToHost));
+      return;
+    }
     std::vector<double> host(elements);
     FPLS_HIP(hipMemcpy(host.data(), src, sizeof(double) * elements, hipMemcpyDeviceToHost));
     T* o = static_cast<T*>(out);
@@ -437,18 +473,33 @@ class SimplsBatch {
     }
     FPLS_HIP(hipMemcpy(up_.at<char>(0), image.data(), image.size(), hipMemcpyHostToDevice));
     // task lists of the per-step passes (identical for every step), in group
-    // order; prefix bookkeeping for the active groups of each step
-    std::vector<XwTask<T>> xw;
-    std::vector<XtwTask<T>> xt;
+    // order; prefix bookkeeping for the active groups of each step. Tasks are
+    // kept in width classes, each launched on its own: narrow right-hand sides
+    // (below kMfmaMinN columns, e.g. a singleton fit's one candidate) on the
+    // VALU kernels, wider ones (groups of refits sharing X) on the matrix
+    // cores at the width they need, so no task pays for another's columns.
+    std::vector<XwTask<T>> xw[kWidthClasses];
+    std::vector<XtwTask<T>> xt[kWidthClasses];
     std::vector<SlabSum> sums;
-    std::vector<int> xw_end(order_.size() + 1, 0), xt_end(order_.size() + 1, 0),
-        sum_end(order_.size() + 1, 0), xw_mn(order_.size() + 1, 0), xt_mn(order_.size() + 1, 0);
-    std::vector<long long> sum_max(order_.size() + 1, 0);
-    for (size_t gi = 0; gi < order_.size(); ++gi) {
+    const size_t ng = order_.size();
+    std::vector<int> xw_end[kWidthClasses], xt_end[kWidthClasses], xw_mn[kWidthClasses],
+        xt_mn[kWidthClasses];
+    for (int c = 0; c < kWidthClasses; ++c) {
+      xw_end[c].assign(ng + 1, 0); xt_end[c].assign(ng + 1, 0);
+      xw_mn[c].assign(ng + 1, 0); xt_mn[c].assign(ng + 1, 0);
+    }
+    std::vector<int> sum_end(ng + 1, 0);
+    std::vector<long long> sum_max(ng + 1, 0);
+    for (size_t gi = 0; gi < ng; ++gi) {
       const int g = order_[gi];
       Group& G = gplan_[g];
+      for (int c = 0; c < kWidthClasses; ++c) {
+        xw_mn[c][gi + 1] = xw_mn[c][gi];
+        xt_mn[c][gi + 1] = xt_mn[c][gi];
+      }
       for (int c0 = 0; c0 < G.bcat; c0 += kXwChunk) {
         const int nb = std::min(kXwChunk, G.bcat - c0);
+        const int cls = width_class(nb);
         XwTask<T> a{};
         a.A = G.X; a.m = G.n; a.k = G.p; a.center = G.mx;
         a.W = G.cand_scaled + (size_t)c0 * G.p; a.ldw = G.p; a.N = nb; a.splits = G.xs_splits;
@@ -459,18 +510,19 @@ class SimplsBatch {
         }
         a.ldo = G.n;
         a.frag = take_frag(g, xw_frag_bytes<T>(G.p, maxn_xw, kF64));
-        xw.push_back(a);
-        xw_mn[gi + 1] = std::max(xw_mn[gi + 1], nb);
+        xw[cls].push_back(a);
+        xw_mn[cls][gi + 1] = std::max(xw_mn[cls][gi + 1], nb);
       }
       for (int c0 = 0; c0 < G.bcat; c0 += 64) {
         const int nb = std::min(64, G.bcat - c0);
+        const int cls = width_class(nb);
         XtwTask<T> b{};
         b.A = G.X; b.m = G.n; b.k = G.p; b.center = G.mx; b.mode = kCentered;
         b.W = G.t_raw + (size_t)c0 * G.n; b.ldw = G.n; b.N = nb; b.splits = G.ls_splits;
         b.out = G.loads + (size_t)c0 * G.p; b.ldo = G.p; b.slab = (long long)G.p * G.bcat;
         b.frag = take_frag(g, xtw_frag_bytes<T>(G.n, maxn_step, kF64));
-        xt.push_back(b);
-        xt_mn[gi + 1] = std::max(xt_mn[gi + 1], nb);
+        xt[cls].push_back(b);
+        xt_mn[cls][gi + 1] = std::max(xt_mn[cls][gi + 1], nb);
       }
       if (G.xs_splits > 1) {
         const long long cnt = (long long)G.n * G.bcat;
@@ -479,57 +531,61 @@ class SimplsBatch {
       } else {
         sum_max[gi + 1] = sum_max[gi];
       }
-      xw_mn[gi + 1] = std::max(xw_mn[gi + 1], xw_mn[gi]);
-      xt_mn[gi + 1] = std::max(xt_mn[gi + 1], xt_mn[gi]);
-      xw_end[gi + 1] = int(xw.size());
-      xt_end[gi + 1] = int(xt.size());
+      for (int c = 0; c < kWidthClasses; ++c) {
+        xw_end[c][gi + 1] = int(xw[c].size());
+        xt_end[c][gi + 1] = int(xt[c].size());
+      }
       sum_end[gi + 1] = int(sums.size());
     }
-    int mn = 0;
-    const long long xw_total = pack_xw(xw.data(), int(xw.size()), &mn, kF64);
-    const long long xt_total = pack_xtw(xt.data(), int(xt.size()), &mn, kF64);
-    // tiles of a prefix: tile0 of the first task after it. The xtw tile count
-    // depends on the launch width, so packs of each prefix are recomputed.
-    steps_.assign(order_.size() + 1, StepLaunch{});
-    for (size_t na = 1; na <= order_.size(); ++na) {
-      StepLaunch& L = steps_[na];
-      L.xw_count = xw_end[na];
-      L.xt_count = xt_end[na];
-      L.sum_count = sum_end[na];
-      L.sum_max = sum_max[na];
-      L.xw_maxn = xw_mn[na];
-      L.xt_maxn = xt_mn[na];
-      L.xw_tiles = L.xw_count < int(xw.size()) ? xw[L.xw_count].tile0 : xw_total;
-      L.xt_tiles = L.xt_count < int(xt.size()) ? xt[L.xt_count].tile0 : xt_total;
-    }
-    // xtw tiles per task depend on the widest task of the launch (128 vs 64
-    // columns per work-group); the prefixes above assumed the widest over all
-    // tasks, so a prefix whose own width selects the other kernel repacks
-    for (size_t na = 1; na <= order_.size(); ++na) {
-      StepLaunch& L = steps_[na];
-      const bool all_mfma = mn >= kMfmaMinN;
-      const bool pre_mfma = L.xt_maxn >= kMfmaMinN;
-      L.xt_list = 0;
-      if (all_mfma != pre_mfma) {
-        std::vector<XtwTask<T>> sub(xt.begin(), xt.begin() + L.xt_count);
-        int m2 = 0;
-        L.xt_tiles = pack_xtw(sub.data(), int(sub.size()), &m2, kF64);
-        extra_xt_.push_back(std::make_unique<Buf>());
-        extra_xt_.back()->alloc(sizeof(XtwTask<T>) * sub.size());
-        FPLS_HIP(hipMemcpy(extra_xt_.back()->as<XtwTask<T>>(), sub.data(),
-                           sizeof(XtwTask<T>) * sub.size(), hipMemcpyHostToDevice));
-        L.xt_list = int(extra_xt_.size());
+    // tiles of a prefix: tile0 of the first task after it (within a class
+    // every task has the same kernel, so its tile count is prefix-independent)
+    steps_.assign(ng + 1, StepLaunch{});
+    for (int c = 0; c < kWidthClasses; ++c) {
+      int mn = 0;
+      const long long xw_total = pack_xw(xw[c].data(), int(xw[c].size()), &mn, kF64);
+      const long long xt_total = pack_xtw(xt[c].data(), int(xt[c].size()), &mn, kF64);
+      // algorithmic bytes of each prefix: A, W and the output once each
+      std::vector<double> xwb(xw[c].size() + 1, 0.0), xtb(xt[c].size() + 1, 0.0);
+      for (size_t i = 0; i < xw[c].size(); ++i) {
+        const XwTask<T>& a = xw[c][i];
+        xwb[i + 1] = xwb[i] + double(a.m) * a.k * sizeof(T) + 8.0 * a.k * a.N + 8.0 * a.m * a.N;
+      }
+      for (size_t i = 0; i < xt[c].size(); ++i) {
+        const XtwTask<T>& b = xt[c][i];
+        xtb[i + 1] = xtb[i] + double(b.m) * b.k * sizeof(T) + 8.0 * b.m * b.N + 8.0 * b.k * b.N;
+      }
+      for (size_t na = 1; na <= ng; ++na) {
+        auto& P = steps_[na].pass[c];
+        P.xw_bytes = xwb[xw_end[c][na]];
+        P.xt_bytes = xtb[xt_end[c][na]];
+        P.xw_count = xw_end[c][na];
+        P.xt_count = xt_end[c][na];
+        P.xw_maxn = xw_mn[c][na];
+        P.xt_maxn = xt_mn[c][na];
+        P.xw_tiles = P.xw_count < int(xw[c].size()) ? xw[c][P.xw_count].tile0 : xw_total;
+        P.xt_tiles = P.xt_count < int(xt[c].size()) ? xt[c][P.xt_count].tile0 : xt_total;
+      }
+      if (!xw[c].empty()) {
+        xw_tasks_[c].alloc(sizeof(XwTask<T>) * xw[c].size());
+        FPLS_HIP(hipMemcpy(xw_tasks_[c].as<XwTask<T>>(), xw[c].data(),
+                           sizeof(XwTask<T>) * xw[c].size(), hipMemcpyHostToDevice));
+      }
+      if (!xt[c].empty()) {
+        xt_tasks_[c].alloc(sizeof(XtwTask<T>) * xt[c].size());
+        FPLS_HIP(hipMemcpy(xt_tasks_[c].as<XtwTask<T>>(), xt[c].data(),
+                           sizeof(XtwTask<T>) * xt[c].size(), hipMemcpyHostToDevice));
       }
     }
-    xw_tasks_.alloc(sizeof(XwTask<T>) * xw.size());
-    xt_tasks_.alloc(sizeof(XtwTask<T>) * xt.size());
-    FPLS_HIP(hipMemcpy(xw_tasks_.as<XwTask<T>>(), xw.data(), sizeof(XwTask<T>) * xw.size(), hipMemcpyHostToDevice));
-    FPLS_HIP(hipMemcpy(xt_tasks_.as<XtwTask<T>>(), xt.data(), sizeof(XtwTask<T>) * xt.size(), hipMemcpyHostToDevice));
+    for (size_t na = 1; na <= ng; ++na) {
+      steps_[na].sum_count = sum_end[na];
+      steps_[na].sum_max = sum_max[na];
+    }
     if (!sums.empty()) {
       sums_.alloc(sizeof(SlabSum) * sums.size());
       FPLS_HIP(hipMemcpy(sums_.as<SlabSum>(), sums.data(), sizeof(SlabSum) * sums.size(), hipMemcpyHostToDevice));
     }
-    // preprocessing task list: column statistics + cross-products of every group
+    // preprocessing task lists: column statistics + cross-products of every
+    // group, one launch per width class
     for (const int g : order_) {
       Group& G = gplan_[g];
       for (int c0 = 0; c0 < G.qcat; c0 += 64) {
@@ -540,12 +596,18 @@ class SimplsBatch {
         t.out = G.spart + (size_t)c0 * G.p; t.ldo = G.p; t.slab = (long long)G.p * G.qcat;
         t.stats = c0 == 0 ? G.stats : nullptr;
         t.frag = take_frag(g, xtw_frag_bytes<T>(G.n, maxn_pre, kF64));
-        pre_.push_back(t);
+        pre_[width_class(nb)].push_back(t);
       }
     }
-    pre_tiles_ = pack_xtw(pre_.data(), int(pre_.size()), &pre_maxn_, kF64);
-    pre_dev_.alloc(sizeof(XtwTask<T>) * pre_.size());
-    FPLS_HIP(hipMemcpy(pre_dev_.as<XtwTask<T>>(), pre_.data(), sizeof(XtwTask<T>) * pre_.size(), hipMemcpyHostToDevice));
+    for (int c = 0; c < kWidthClasses; ++c) {
+      if (pre_[c].empty()) continue;
+      for (const XtwTask<T>& t : pre_[c])
+        pre_bytes_[c] += double(t.m) * t.k * sizeof(T) + 8.0 * t.m * t.N + 8.0 * t.k * t.N;
+      pre_tiles_[c] = pack_xtw(pre_[c].data(), int(pre_[c].size()), &pre_maxn_[c], kF64);
+      pre_dev_[c].alloc(sizeof(XtwTask<T>) * pre_[c].size());
+      FPLS_HIP(hipMemcpy(pre_dev_[c].as<XtwTask<T>>(), pre_[c].data(),
+                         sizeof(XtwTask<T>) * pre_[c].size(), hipMemcpyHostToDevice));
+    }
   }
 
   void run() {
@@ -568,8 +630,12 @@ class SimplsBatch {
     // every response column of every fit: one launch
     launch_center_columns<T>(center_list_, center_count_, stream_);
     // one launch: column statistics + cross-products of every group
-    launch_xtw<T>(pre_dev_.as<XtwTask<T>>(), int(pre_.size()), pre_tiles_, pre_maxn_, stream_,
-                  kF64);
+    for (int c = 0; c < kWidthClasses; ++c)
+      if (!pre_[c].empty())
+        timed(0, pre_bytes_[c], [&] {
+          launch_xtw<T>(pre_dev_[c].as<XtwTask<T>>(), int(pre_[c].size()), pre_tiles_[c],
+                        pre_maxn_[c], stream_, kF64);
+        });
     for (Group& G : gplan_)
       launch_finalize_crossprod<T>(G.X, G.n, G.p, G.qcat, scaling_, G.stats, G.spart, G.splits,
                                    G.ysum, G.mx, G.sx, G.colss, G.S, G.S0, G.flags, stream_);
@@ -603,16 +669,60 @@ class SimplsBatch {
       int na = 0;
       while (na < int(order_.size()) && gplan_[order_[na]].max_steps > s) ++na;
       const StepLaunch& L = steps_[na];
-      launch_simpls_refresh(devfits_, nf, s, stream_);
-      launch_xw<T>(xw_tasks_.as<XwTask<T>>(), L.xw_count, L.xw_tiles, L.xw_maxn, stream_, kF64);
-      if (L.sum_count > 0) launch_sum_slabs_list(sums_.as<SlabSum>(), L.sum_count, L.sum_max, stream_);
-      const XtwTask<T>* xl = L.xt_list ? extra_xt_[L.xt_list - 1]->template as<XtwTask<T>>()
-                                       : xt_tasks_.as<XtwTask<T>>();
-      launch_xtw<T>(xl, L.xt_count, L.xt_tiles, L.xt_maxn, stream_, kF64);
-      launch_simpls_update(devfits_, nf, s, stream_);
+      timed(1, 0.0, [&] { launch_simpls_refresh(devfits_, nf, s, stream_); });
+      for (int c = 0; c < kWidthClasses; ++c) {
+        const auto& P = L.pass[c];
+        if (P.xw_count > 0)
+          timed(c == 0 ? 2 : 3, P.xw_bytes, [&] {
+            launch_xw<T>(xw_tasks_[c].as<XwTask<T>>(), P.xw_count, P.xw_tiles, P.xw_maxn,
+                         stream_, kF64);
+          });
+      }
+      if (L.sum_count > 0)
+        timed(4, 0.0, [&] {
+          launch_sum_slabs_list(sums_.as<SlabSum>(), L.sum_count, L.sum_max, stream_);
+        });
+      for (int c = 0; c < kWidthClasses; ++c) {
+        const auto& P = L.pass[c];
+        if (P.xt_count > 0)
+          timed(c == 0 ? 5 : 6, P.xt_bytes, [&] {
+            launch_xtw<T>(xt_tasks_[c].as<XtwTask<T>>(), P.xt_count, P.xt_tiles, P.xt_maxn,
+                          stream_, kF64);
+          });
+      }
+      timed(7, 0.0, [&] { launch_simpls_update(devfits_, nf, s, stream_); });
     }
     // R / sx of every fit: one launch
     launch_div_rows_list(div_list_, int(plan_.size()), div_max_, stream_);
+  }
+
+  // Runs `launch`; with phase timing on, between two events on the stream.
+  template<class F>
+  void timed(int phase, double bytes, F&& launch) {
+    if (!timing_) {
+      launch();
+      return;
+    }
+    TimedLaunch t{phase, bytes, nullptr, nullptr};
+    FPLS_HIP(hipEventCreate(&t.a));
+    timed_.push_back(t);
+    FPLS_HIP(hipEventCreate(&timed_.back().b));
+    FPLS_HIP(hipEventRecord(timed_.back().a, stream_));
+    launch();
+    FPLS_HIP(hipEventRecord(timed_.back().b, stream_));
+  }
+
+  void collect_phases() {
+    for (TimedLaunch& t : timed_) {
+      float ms = 0.f;
+      FPLS_HIP(hipEventElapsedTime(&ms, t.a, t.b));
+      phase_[t.phase][0] += double(ms) * 1e-3;
+      phase_[t.phase][1] += t.bytes;
+      phase_[t.phase][2] += 1.0;
+      (void)hipEventDestroy(t.a);
+      (void)hipEventDestroy(t.b);
+    }
+    timed_.clear();
   }
 
   void collect() {
@@ -631,8 +741,13 @@ class SimplsBatch {
   }
 
   struct StepLaunch {
-    int xw_count = 0, xt_count = 0, sum_count = 0, xw_maxn = 0, xt_maxn = 0, xt_list = 0;
-    long long xw_tiles = 0, xt_tiles = 0, sum_max = 0;
+    struct Pass {  // one width class of the score and loading passes
+      int xw_count = 0, xt_count = 0, xw_maxn = 0, xt_maxn = 0;
+      long long xw_tiles = 0, xt_tiles = 0;
+      double xw_bytes = 0.0, xt_bytes = 0.0;  // algorithmic bytes of the launch
+    } pass[kWidthClasses];
+    int sum_count = 0;
+    long long sum_max = 0;
   };
 
   std::vector<GroupIn> groups_;
@@ -652,11 +767,20 @@ class SimplsBatch {
   size_t devfits_off_ = 0;
   size_t status_off_ = 0;  // int[4] flags per group, then int[4] state per fit
   SimplsFitDev* devfits_ = nullptr;
-  Buf xw_tasks_, xt_tasks_, sums_, pre_dev_, proj_frag_;
-  std::vector<std::unique_ptr<Buf>> extra_xt_;
-  std::vector<XtwTask<T>> pre_;
-  long long pre_tiles_ = 0;
-  int pre_maxn_ = 0;
+  Buf xw_tasks_[kWidthClasses], xt_tasks_[kWidthClasses], sums_, proj_frag_;
+  Buf pre_dev_[kWidthClasses];
+  std::vector<XtwTask<T>> pre_[kWidthClasses];
+  long long pre_tiles_[kWidthClasses] = {};
+  int pre_maxn_[kWidthClasses] = {};
+  double pre_bytes_[kWidthClasses] = {};
+  struct TimedLaunch {
+    int phase;
+    double bytes;
+    hipEvent_t a, b;
+  };
+  bool timing_ = false;
+  std::vector<TimedLaunch> timed_;
+  double phase_[kPhases][3] = {};
   hipStream_t stream_ = nullptr;
   double seconds_ = 0.0;
   std::vector<int> completed_, flags_;
@@ -669,7 +793,8 @@ class BatchHandle {
   virtual double seconds() const = 0;
   virtual int completed(int f) const = 0;
   virtual int flags(int f) const = 0;
-  virtual void export_field(int f, int field, void* out, size_t elements) = 0;
+  virtual void export_field(int f, int field, void* out, size_t elements, bool f64) = 0;
+  virtual void phases(double* out, int count) const = 0;
 };
 
 template<class T>
@@ -680,9 +805,10 @@ class TypedBatch final : public BatchHandle {
   double seconds() const override { return b_->seconds(); }
   int completed(int f) const override { return b_->completed(f); }
   int flags(int f) const override { return b_->flags(f); }
-  void export_field(int f, int field, void* out, size_t elements) override {
-    b_->export_field(f, field, out, elements);
+  void export_field(int f, int field, void* out, size_t elements, bool f64) override {
+    b_->export_field(f, field, out, elements, f64);
   }
+  void phases(double* out, int count) const override { b_->phases(out, count); }
 
  private:
   std::unique_ptr<SimplsBatch<T>> b_;
@@ -853,11 +979,27 @@ int fastpls_hip_batch_status(void* batch, int* completed, int* flags, double* se
   });
 }
 
+int fastpls_hip_batch_phases(void* batch, double* out, int count, char* error,
+                             size_t error_capacity) {
+  return fpls::guarded(error, error_capacity, [&] {
+    if (!batch || !out || count < 1) throw std::invalid_argument("invalid batch phase request");
+    static_cast<BatchHandle*>(batch)->phases(out, count);
+  });
+}
+
 int fastpls_hip_batch_export(void* batch, int fit, int field, void* out, size_t elements,
                              char* error, size_t error_capacity) {
   return fpls::guarded(error, error_capacity, [&] {
     if (!batch) throw std::invalid_argument("null batch");
-    static_cast<BatchHandle*>(batch)->export_field(fit, field, out, elements);
+    static_cast<BatchHandle*>(batch)->export_field(fit, field, out, elements, false);
+  });
+}
+
+int fastpls_hip_batch_export64(void* batch, int fit, int field, double* out, size_t elements,
+                             char* error, size_t error_capacity) {
+  return fpls::guarded(error, error_capacity, [&] {
+    if (!batch) throw std::invalid_argument("null batch");
+    static_cast<BatchHandle*>(batch)->export_field(fit, field, out, elements, true);
   });
 }
 

@@ -104,9 +104,14 @@ class Model final : public Handle, private StreamOps {
   }
 
   void phases(double* out, int count) const override {
-    const double v[4] = {fit_seconds_, crossprod_seconds_, score_seconds_,
-                         double(n_) * double(p_) * double(sizeof(T))};
-    for (int i = 0; i < count && i < 4; ++i) out[i] = v[i];
+    // the last query's projection: its events were passed before the query
+    // returned (every query synchronises its stream)
+    float ms = 0.f;
+    if (proj_timed_) FPLS_HIP(hipEventElapsedTime(&ms, pr0_.e, pr1_.e));
+    const double v[6] = {fit_seconds_, crossprod_seconds_, score_seconds_,
+                         double(n_) * double(p_) * double(sizeof(T)), double(ms) * 1e-3,
+                         proj_bytes_};
+    for (int i = 0; i < count && i < 6; ++i) out[i] = v[i];
   }
 
   // ------------------------------------------------------------ queries
@@ -773,7 +778,12 @@ class Model final : public Handle, private StreamOps {
   // scores (rows x prefix) = ((x - m) / s) R[:, :prefix]
   void project_rows(const T* x, int rows, int prefix) {
     scores_.reserve(sizeof(double) * (size_t)rows * prefix);
+    FPLS_HIP(hipEventRecord(pr0_.e, stream_));
     product<T>(x, rows, p_, mx_.d(), Rs_.d(), p_, prefix, scores_.d());
+    FPLS_HIP(hipEventRecord(pr1_.e, stream_));
+    proj_timed_ = true;
+    // algorithmic bytes: the query rows, R / s and the scores once each
+    proj_bytes_ = double(rows) * p_ * sizeof(T) + 8.0 * p_ * prefix + 8.0 * rows * prefix;
     scores_rows_ = rows;
   }
 
@@ -866,7 +876,9 @@ class Model final : public Handle, private StreamOps {
   int block_ = 1;
   bool compacted_ = false;
   double fit_seconds_ = 0.0, crossprod_seconds_ = 0.0, score_seconds_ = 0.0;
-  Event cp0_, cp1_, sc0_, sc1_;
+  Event cp0_, cp1_, sc0_, sc1_, pr0_, pr1_;
+  bool proj_timed_ = false;
+  double proj_bytes_ = 0.0;
   const T* X_ = nullptr;
   Buf xown_, yown_, labels_dev_, counts_, Yc_, H_, ysum_, my_, mx_, sx_, colss_, S_, S0_, G_;
   Buf R_, Q_, V_, Tsc_, Rs_, sigma_, gram_, W_, state_;

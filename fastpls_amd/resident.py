@@ -81,6 +81,8 @@ SIGNATURES = {
                                               C.c_int, C.c_int, C.c_int, _cp, _sz]),
     "fastpls_hip_batch_status": (C.c_int, [_vp, _ip, _ip, _dp, _cp, _sz]),
     "fastpls_hip_batch_export": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _sz, _cp, _sz]),
+    "fastpls_hip_batch_phases": (C.c_int, [_vp, _vp, C.c_int, _cp, _sz]),
+    "fastpls_hip_batch_export64": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _sz, _cp, _sz]),
     "fastpls_hip_batch_destroy": (None, [_vp]),
     "fastpls_hip_simpls_cv_regression": _CV_REGRESSION,
     "fastpls_hip_plssvd_cv_regression": _CV_REGRESSION,
@@ -349,12 +351,12 @@ class ResidentModel:
         return v.value
 
     def fit_phases(self):
-        out = (C.c_double * 4)()
+        out = (C.c_double * 6)()
         e = _err()
-        if load_library().fastpls_hip_fit_phases(self._h, out, 4, e, 1024) != 0:
+        if load_library().fastpls_hip_fit_phases(self._h, out, 6, e, 1024) != 0:
             _raise(e)
         return {"fit": out[0], "crossprod_pass": out[1], "score_pass": out[2],
-                "x_bytes": out[3]}
+                "x_bytes": out[3], "projection": out[4], "projection_bytes": out[5]}
 
     def coefficients(self, prefix):
         """B for one prefix (p x q) in standardized units: R_k Q_kᵀ (SIMPLS)."""
@@ -475,15 +477,34 @@ class SimplsBatch:
             _raise(e)
         return list(done), list(flags), sec.value
 
-    def export(self, fit, field):
+    PHASES = ("crossprod", "refresh", "score_valu", "score_mfma", "score_sums", "loading_valu",
+              "loading_mfma", "update")
+
+    def phases(self):
+        """Device-time phases (set FASTPLS_BENCH_PHASE_TIMING=1 before creating
+        the batch): {"seconds": s, phase: {"seconds", "bytes", "launches"}}."""
+        out = np.zeros(1 + 3 * len(self.PHASES))
+        e = _err()
+        if load_library().fastpls_hip_batch_phases(self._h, _ptr(out), out.size, e, 1024) != 0:
+            _raise(e)
+        res = {"seconds": float(out[0])}
+        for i, name in enumerate(self.PHASES):
+            res[name] = {"seconds": float(out[1 + 3 * i]), "bytes": float(out[2 + 3 * i]),
+                         "launches": int(out[3 + 3 * i])}
+        return res
+
+    def export(self, fit, field, f64=False):
+        """Field of one fit in the batch precision, or (f64=True) the FP64
+        values the fit holds."""
         f = BATCH_FIELDS[field] if isinstance(field, str) else int(field)
         n, p = self.groups[self.fits[fit]["group"]]
         q, a = self.fits[fit]["q"], self.fits[fit]["components"]
         shape = {0: (p, a), 1: (q, a), 2: (n, a), 3: (p,), 4: (p,), 5: (q,)}[f]
-        out = np.empty(shape, dtype=self.dtype, order="F")
+        out = np.empty(shape, dtype=np.float64 if f64 else self.dtype, order="F")
         e = _err()
-        if load_library().fastpls_hip_batch_export(self._h, int(fit), f, _ptr(out), out.size, e,
-                                                   1024) != 0:
+        fn = (load_library().fastpls_hip_batch_export64 if f64
+              else load_library().fastpls_hip_batch_export)
+        if fn(self._h, int(fit), f, _ptr(out), out.size, e, 1024) != 0:
             _raise(e)
         return out
 

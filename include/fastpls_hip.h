@@ -14,10 +14,13 @@
  *     message is written to (error, error_capacity) with snprintf semantics;
  *   - every call is synchronous; calls on one handle must not overlap.
  *
- * Precision 32 stores X as FP32 on the device and runs every product over X
- * on the matrix cores in bf16x6 split precision (FP32-accurate products,
- * FP32 sums over one 32- to 64-element step folded into FP64 accumulators);
- * the cross-covariance, the randomized SVD, SIMPLS
+ * Precision 32 stores X as FP32 on the device. SIMPLS (fits, batches, CV and
+ * their predictions) widens it to FP64 as it is loaded and computes every
+ * product over X in FP64: its component recursion magnifies the rounding of
+ * these products by up to ~1e3. PLS-SVD, fastsvd and fastcor run their
+ * products over FP32 X on the matrix cores in bf16x6 split precision
+ * (FP32-accurate products, FP32 sums over one 32- to 64-element step folded
+ * into FP64 accumulators). The cross-covariance, the randomized SVD, SIMPLS
  * deflation/orthogonalisation, PLS-SVD solves and LDA run in FP64 for both
  * precisions. Gaussian sketches are drawn on the host with
  * std::mt19937(seed) and std::normal_distribution<double>, the same stream
@@ -144,7 +147,9 @@ int fastpls_hip_memcpy(void* dst, const void* src, size_t bytes, int kind, char*
 /* Device-time breakdown of the last fit from events on the handle's stream
  * (no reference counterpart): out[0] fit seconds, out[1] the one-pass
  * statistics + cross-product kernel over X, out[2] the PLS-SVD score pass
- * over X (SIMPLS: the whole component loop), out[3] bytes of X. */
+ * over X (SIMPLS: the whole component loop), out[3] bytes of X, out[4]
+ * seconds of the projection of the last query's rows (predict / classify /
+ * project), out[5] that projection's algorithmic bytes. */
 int fastpls_hip_fit_phases(void* model, double* out, int count, char* error,
                            size_t error_capacity);
 
@@ -175,10 +180,26 @@ void* fastpls_hip_simpls_batch_create(int precision, int group_count, const void
  * seconds of the whole batch. */
 int fastpls_hip_batch_status(void* batch, int* completed, int* flags, double* seconds,
                              char* error, size_t error_capacity);
+/* Device-time phases of the batch (no reference counterpart), measured with
+ * events around every launch when the environment variable
+ * FASTPLS_BENCH_PHASE_TIMING=1 was set at creation (the reference's switch,
+ * src/r_api.cpp:1182-1184), zero otherwise: out[0] batch device seconds;
+ * phase i (0 statistics + cross-products, 1 refresh, 2 / 3 score passes on the
+ * VALU / matrix cores, 4 score slab sums, 5 / 6 loading passes on the VALU /
+ * matrix cores, 7 update) fills out[1 + 3i] seconds, out[2 + 3i] algorithmic
+ * bytes and out[3 + 3i] launches. count <= 25 entries are written. */
+int fastpls_hip_batch_phases(void* batch, double* out, int count, char* error,
+                             size_t error_capacity);
 /* Fields 0 R (p*A), 1 Q (q*A), 2 T (n*A), 3 x means (p), 4 x scales (p),
  * 5 y means (q) of fit `fit`, in the batch precision. */
 int fastpls_hip_batch_export(void* batch, int fit, int field, void* out, size_t elements,
                              char* error, size_t error_capacity);
+/* The same fields as the FP64 values the fit holds, whatever the batch
+ * precision (no reference counterpart; for accuracy audits: a precision-32
+ * export rounds R, Q and T to FP32, and B = R Qᵀ of an ill-conditioned fit
+ * magnifies that rounding). */
+int fastpls_hip_batch_export64(void* batch, int fit, int field, double* out, size_t elements,
+                               char* error, size_t error_capacity);
 void fastpls_hip_batch_destroy(void* batch);
 
 /* fastpls_resident_simpls_cv_regression (src/cuda_resident_api.h:20-26):

@@ -15,6 +15,7 @@
 // Labels at this boundary are one-based, as in src/cuda_resident_api.h:10.
 
 #include <algorithm>
+#include <dlfcn.h>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
@@ -44,9 +45,28 @@ struct Mat {
   const double* data() const { return d.data(); }
 };
 
+// Optional BLAS dgemm (Fortran interface with 64-bit integers), bound at run
+// time by orc_use_blas for CPU-baseline timing only: the reference's CPU
+// adapter runs every product through BLAS dgemm (src/core_cpu_backend.cpp:441-619).
+// Unset (the default, and always in the pinning tests) the plain loops below run.
+typedef void (*DgemmFn)(const char*, const char*, const int64_t*, const int64_t*,
+                        const int64_t*, const double*, const double*, const int64_t*,
+                        const double*, const int64_t*, const double*, double*,
+                        const int64_t*, size_t, size_t);
+DgemmFn g_dgemm = nullptr;
+
 // C(m x n) = op(A) * op(B); op(A) is m x k. Plain loops in contiguous order.
 void gemm(const double* A, size_t lda, bool ta, const double* B, size_t ldb,
           bool tb, size_t m, size_t n, size_t k, double* C, size_t ldc) {
+  if (g_dgemm && m && n) {
+    const int64_t M = int64_t(m), N = int64_t(n), K = int64_t(k);
+    const int64_t LA = int64_t(std::max<size_t>(lda, 1)), LB = int64_t(std::max<size_t>(ldb, 1));
+    const int64_t LC = int64_t(std::max<size_t>(ldc, 1));
+    const double one = 1.0, zero = 0.0;
+    g_dgemm(ta ? "T" : "N", tb ? "T" : "N", &M, &N, &K, &one, A, &LA, B, &LB, &zero, C, &LC,
+            1, 1);
+    return;
+  }
   for (size_t j = 0; j < n; ++j) {
     double* cj = C + j * ldc;
     if (ta) {
@@ -991,14 +1011,12 @@ SimplsModel fit_simpls(const Mat& X, const Mat& S0, const SimplsControls& ctl) {
         const double sn = std::sqrt(ns);
         for (double& x : r) x /= sn;
         for (double& x : w) x /= sn;
-        if (retain_scores) {
-          for (size_t l = 0; l < p; ++l) for (size_t i = 0; i < n; ++i) t[i] += X(i, l) * r[l];
-        }
+        if (retain_scores) gemm(X.data(), n, false, r.data(), p, false, n, 1, p, t.data(), n);
       } else {
         if (ctl.batch_geometry) {
           for (size_t i = 0; i < n; ++i) t[i] = cand_scores(i, cand);
         } else {
-          for (size_t l = 0; l < p; ++l) for (size_t i = 0; i < n; ++i) t[i] += X(i, l) * r[l];
+          gemm(X.data(), n, false, r.data(), p, false, n, 1, p, t.data(), n);
         }
         const bool reorthed = ctl.reorth && component > 0;
         if (reorthed) remove_score_basis(model.T, model.R, component, t, r);
@@ -1009,11 +1027,7 @@ SimplsModel fit_simpls(const Mat& X, const Mat& S0, const SimplsControls& ctl) {
         if (ctl.batch_geometry && !reorthed) {
           for (size_t i = 0; i < p; ++i) w[i] = cand_loadings(i, cand) / tn;
         } else {
-          for (size_t j = 0; j < p; ++j) {
-            double s = 0;
-            for (size_t i = 0; i < n; ++i) s += X(i, j) * t[i];
-            w[j] = s;
-          }
+          gemm(X.data(), n, true, t.data(), n, false, p, 1, n, w.data(), p);
         }
       }
       std::vector<double> qa(q);
@@ -1543,6 +1557,26 @@ void copy_out(const Mat& m, double* out) {
 }  // namespace
 
 extern "C" {
+
+// Binds the products to the BLAS dgemm exported as `symbol` by the shared
+// library at `path` (CPU-baseline timing), with `threads` BLAS threads set
+// through `threads_symbol` when both are given; path NULL unbinds.
+int orc_use_blas(const char* path, const char* symbol, const char* threads_symbol,
+                 int threads) {
+  if (!path) { g_dgemm = nullptr; return 0; }
+  void* h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+  if (!h) { g_error = dlerror(); return 1; }
+  DgemmFn f = reinterpret_cast<DgemmFn>(dlsym(h, symbol));
+  if (!f) { g_error = std::string("missing BLAS symbol ") + symbol; return 1; }
+  if (threads_symbol && threads > 0) {
+    typedef void (*SetThreads)(int);
+    SetThreads st = reinterpret_cast<SetThreads>(dlsym(h, threads_symbol));
+    if (!st) { g_error = std::string("missing BLAS symbol ") + threads_symbol; return 1; }
+    st(threads);
+  }
+  g_dgemm = f;
+  return 0;
+}
 
 const char* orc_last_error() { return g_error.c_str(); }
 

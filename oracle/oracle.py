@@ -82,6 +82,21 @@ class Oracle:
         if status != 0:
             raise RuntimeError(self.lib.orc_last_error().decode())
 
+    def use_blas(self, threads: int | None = 1) -> str:
+        """Route the oracle's products through numpy's bundled OpenBLAS dgemm
+        (the reference CPU adapter's BLAS, for CPU-baseline timing) with
+        `threads` BLAS threads; ``threads=None`` returns to the plain loops.
+        Returns a description of the library bound."""
+        if threads is None:
+            self._check(self.lib.orc_use_blas(None, None, None, 0))
+            return "plain loops"
+        path = openblas_path()
+        self._check(self.lib.orc_use_blas(path.encode(), b"scipy_dgemm_64_",
+                                          b"scipy_openblas_set_num_threads64_", int(threads)))
+        cfg = C.CDLL(path).scipy_openblas_get_config64_
+        cfg.restype = C.c_char_p
+        return f"{os.path.basename(path)} ({cfg().decode()}), {threads} thread(s)"
+
     def gaussian(self, seed: int, count: int) -> np.ndarray:
         out = np.empty(count, dtype=np.float64)
         self.lib.orc_gaussian(C.c_uint(seed), C.c_size_t(count), _d(out))
@@ -438,6 +453,16 @@ class Reference:
         self._check(self.lib.ref_lda(_d(T), n, a, _i(lab), classes, prefix, _d(ts), rows,
                                      _d(disc), _i(pred), C.byref(ridge)))
         return dict(disc=disc, labels=pred, ridge=ridge.value)
+
+
+def openblas_path() -> str:
+    """numpy's bundled OpenBLAS (scipy_openblas64_, 64-bit integer interface)."""
+    import glob
+    libs = os.path.join(os.path.dirname(os.path.dirname(np.__file__)), "numpy.libs")
+    found = sorted(glob.glob(os.path.join(libs, "libscipy_openblas64_*.so")))
+    if not found:
+        raise RuntimeError(f"no bundled OpenBLAS under {libs}")
+    return found[0]
 
 
 def latent_data(n, p, q, factors=10, noise=0.1, seed=0, decay=None, dtype=np.float64):

@@ -420,3 +420,28 @@ def test_operator_route_equals_explicit_route(oracle):
         errs.append(rel(o["R"] @ o["Q"].T, e["R"] @ e["Q"].T))
     assert errs[0] > errs[1] > errs[2] and errs[2] < 1e-3, errs
     assert not oracle.implicit_route(20000, 209) and oracle.implicit_route(20000, 210)
+
+
+def test_blas_bound_oracle_matches_plain_loops(oracle):
+    """The CPU baseline times the oracle with its products on numpy's bundled
+    OpenBLAS dgemm (the reference CPU adapter's BLAS). Bound that way it must
+    give the plain-loop oracle's fit (only the summation order differs):
+    SIMPLS within the latent rank and PLS-SVD, relative 1e-9."""
+    from oracle.oracle import latent_data
+    X, Y = latent_data(700, 150, 3, factors=6, noise=0.1, seed=12)
+    plain_s = oracle.simpls_fit(X, Y, ncomp=6, oversample=10, power=2, seed=3)
+    plain_p = oracle.plssvd_fit(X, Y, ncomp=3, oversample=10, power=2, seed=3)
+    try:
+        desc = oracle.use_blas(2)
+        assert "OpenBLAS" in desc
+        blas_s = oracle.simpls_fit(X, Y, ncomp=6, oversample=10, power=2, seed=3)
+        blas_p = oracle.plssvd_fit(X, Y, ncomp=3, oversample=10, power=2, seed=3)
+    finally:
+        oracle.use_blas(None)
+    for k in range(1, 7):
+        b0 = plain_s["R"][:, :k] @ plain_s["Q"][:, :k].T
+        b1 = blas_s["R"][:, :k] @ blas_s["Q"][:, :k].T
+        assert np.linalg.norm(b1 - b0) / np.linalg.norm(b0) < 1e-9, k
+    for key in ("R", "Q", "T"):
+        a, b = plain_p[key], blas_p[key]
+        assert np.linalg.norm(a - b) / np.linalg.norm(a) < 1e-9, key
