@@ -14,7 +14,7 @@ LIB = os.path.join(ROOT, "fastpls_amd", "lib", "libfastpls_hip.so")
 def declared():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(fastpls_hip_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(fastpls_hip_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_header_declares_the_reference_surface():
@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
         b.build()
     out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True,
                          check=True).stdout
-    exported = set(re.findall(r"\bT (fastpls_hip_[a-z_]+)", out))
+    exported = set(re.findall(r"\bT (fastpls_hip_[a-z0-9_]+)", out))
     missing = [n for n in declared() if n not in exported]
     assert not missing, missing
 
