@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -27,6 +28,13 @@
 
 namespace fpls {
 namespace {
+
+// Scratch of the no-allocation prediction entry: one array per GPU (each
+// device holds its own copy until the process ends). Host code must never
+// pass `scratch` itself to a kernel: it is a host-side stand-in; the device
+// address comes from hipGetSymbolAddress (noalloc_scratch below).
+constexpr size_t kNoAllocScratchDoubles = size_t(1) << 22;  // 32 MiB
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
 
 template<class T>
 class Model final : public Handle, private StreamOps {
@@ -209,6 +217,56 @@ class Model final : public Handle, private StreamOps {
       FPLS_HIP(hipMemcpyAsync(sums + (size_t)j * 3 * q_, metric_.d(), sizeof(double) * 3 * q_,
                               hipMemcpyDeviceToHost, stream_));
     }
+    FPLS_HIP(hipStreamSynchronize(stream_));
+  }
+
+  void predict_scratch(const void* x, int rows, int prefix, void* out, double* scratch,
+                       size_t capacity) override {
+    if (!x || !out || !scratch || rows < 1 || prefix < 1 || prefix > A_)
+      throw std::invalid_argument("invalid resident prediction request");
+    const bool f64 = f64_products_;
+    const int nb = std::min(prefix, 64);
+    const int splits = invariant_ ? xw_splits_by_depth(p_)
+                                  : xw_splits(rows, p_, nb, sizeof(T) == 4 && !f64);
+    // layout (doubles): scores rows x prefix | FP64 predictions rows x q |
+    // split slabs of one 64-column chunk | operand workspace
+    const size_t r = size_t(rows);
+    const size_t n_scores = r * prefix, n_pred = r * q_;
+    const size_t n_slabs = splits > 1 ? size_t(splits) * r * nb : 0;
+    const size_t n_frag = (xw_frag_bytes<T>(p_, nb, f64) + 255) / 256 * 32;
+    const size_t need = n_scores + n_pred + n_slabs + n_frag;
+    if (need > capacity)
+      throw std::invalid_argument("prediction needs " + std::to_string(need) +
+                                  " scratch doubles; the no-allocation entry holds " +
+                                  std::to_string(capacity));
+    double* scores = scratch;
+    double* pred = scores + n_scores;
+    double* slabs = pred + n_pred;
+    void* fragp = n_frag ? static_cast<void*>(slabs + n_slabs) : nullptr;
+    const T* xr = static_cast<const T*>(x);
+    for (int c0 = 0; c0 < prefix; c0 += 64) {
+      const int w = std::min(64, prefix - c0);
+      XwTask<T> t{};
+      t.A = xr; t.m = rows; t.k = p_; t.center = mx_.d(); t.W = Rs_.d() + (size_t)c0 * p_;
+      t.ldw = p_; t.N = w; t.splits = splits; t.frag = fragp; t.ldo = rows;
+      if (splits == 1) {
+        t.out = scores + (size_t)c0 * r; t.slab = 0;
+        launch_xw_single<T>(t, stream_, f64);
+      } else {
+        t.out = slabs; t.slab = (long long)r * w;
+        launch_xw_single<T>(t, stream_, f64);
+        launch_sum_slabs(slabs, splits, (long long)r * w, (long long)r * w, scores + (size_t)c0 * r,
+                         stream_);
+      }
+    }
+    if (method_ == kSimpls) {
+      launch_apply_small<T>(scores, rows, prefix, Q_.d(), q_, 1, q_, my_.d(), nullptr, pred, stream_);
+    } else {
+      const double* Wk = W_.d() + (size_t)(prefix - 1) * A_ * q_;
+      launch_apply_small<T>(scores, rows, prefix, Wk, 1, prefix, q_, my_.d(), nullptr, pred,
+                            stream_);
+    }
+    launch_convert<T>(pred, static_cast<T*>(out), (long long)r * q_, stream_);
     FPLS_HIP(hipStreamSynchronize(stream_));
   }
 
@@ -959,6 +1017,26 @@ void host_gemm(const T* left, const T* right, int rows, int inner, int cols, T* 
 }
 
 }  // namespace
+
+// Device address of `scratch` on the current device, looked up once per device.
+static double* noalloc_scratch() {
+  static std::mutex mu;
+  static double* addr[64] = {};
+  int dev = 0;
+  FPLS_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) throw std::runtime_error("device index outside 0..63");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      throw std::runtime_error(std::string("hipGetSymbolAddress(scratch) failed: ") +
+                               hipGetErrorString(e));
+    addr[dev] = static_cast<double*>(p);
+  }
+  return addr[dev];
+}
+
 }  // namespace fpls
 
 using fpls::Handle;
@@ -1050,6 +1128,17 @@ void* fastpls_hip_fit_device(int method, const void* x_device, const void* y_dev
                      components, scaling, oversample, power, seed);
   });
   return h;
+}
+
+
+int fastpls_hip_predict_noalloc(void* model, const void* x_device, int rows, int prefix,
+                                void* predictions_device, char* error, size_t error_capacity) {
+  return fpls::guarded(error, error_capacity, [&] {
+    if (!model) throw std::invalid_argument("null resident model");
+    static_cast<Handle*>(model)->predict_scratch(x_device, rows, prefix, predictions_device,
+                                                 fpls::noalloc_scratch(),
+                                                 fpls::kNoAllocScratchDoubles);
+  });
 }
 
 int fastpls_hip_predict_path(void* model, const void* x, int rows, const int* prefixes,

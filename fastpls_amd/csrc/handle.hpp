@@ -28,6 +28,11 @@ class Handle {
   virtual void heldout_regression(const void* x, const void* y, int rows, const int* prefixes,
                                   int count, double* predictions, double* sums) = 0;
   virtual void project(const void* x, int rows, int prefix, void* out) = 0;
+  // Responses of one prefix for device rows x into device `out` (rows x q, the
+  // model's precision) with every intermediate in the caller's `scratch`
+  // (device, `capacity` doubles): no device memory is allocated.
+  virtual void predict_scratch(const void* x, int rows, int prefix, void* out, double* scratch,
+                               size_t capacity) = 0;
   virtual void export_field(int field, void* out, size_t elements) = 0;
   virtual void controls(int* oversample, int* power, int* block, int* block_limit,
                         int* implicit, int* cache) = 0;

@@ -57,6 +57,7 @@ SIGNATURES = {
                                            C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _ull,
                                            C.c_int, C.c_double, C.c_int, C.c_double, _cp, _sz]),
     "fastpls_hip_predict_path": (C.c_int, [_vp, _cvp, C.c_int, _ip, C.c_int, C.c_int, _vp, _cp, _sz]),
+    "fastpls_hip_predict_noalloc": (C.c_int, [_vp, _cvp, C.c_int, C.c_int, _vp, _cp, _sz]),
     "fastpls_hip_destroy": (None, [_vp]),
     "fastpls_hip_export": (C.c_int, [_vp, C.c_int, _vp, _sz, _cp, _sz]),
     "fastpls_hip_classify_path": (C.c_int, [_vp, _cvp, C.c_int, _ip, C.c_int, C.c_int, C.c_int,
@@ -270,6 +271,16 @@ class ResidentModel:
 
     def predict(self, x, prefix, lda=False):
         return self.predict_path(x, [prefix], lda)[:, :, 0]
+
+    def predict_noalloc(self, x_device, rows, prefix, out_device):
+        """Responses of one prefix for device rows (pointer) into a device
+        output (pointer, rows x q, model precision), allocating nothing
+        (fastpls_hip_predict_noalloc)."""
+        e = _err()
+        if load_library().fastpls_hip_predict_noalloc(self._h, C.c_void_p(x_device), int(rows),
+                                                      int(prefix), C.c_void_p(out_device), e,
+                                                      1024) != 0:
+            _raise(e)
 
     def classify_path(self, x, prefixes, lda=False, top=1):
         xa = self._x(x)

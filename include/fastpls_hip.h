@@ -82,6 +82,19 @@ int fastpls_hip_predict_path(void* model, const void* x, int rows, const int* pr
                              int prefix_count, int lda, void* predictions, char* error,
                              size_t error_capacity);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): fastpls_hip_predict_path for one prefix, without LDA,
+ * that allocates nothing. x_device (rows x p) and predictions_device (rows x
+ * q, the model's precision) are DEVICE pointers. Every intermediate (split
+ * partials, scores, FP64 predictions, the bf16x6 operand workspace) lives in
+ * one 32 MiB device array of the library per GPU: requests needing more
+ * (about rows * (prefix * (splits + 1) + q) doubles) fail with an error.
+ * Calls on one GPU share that array, so calls that overlap on one GPU race;
+ * calls on one handle must not overlap anyway. Synchronous, like every call. */
+int fastpls_hip_predict_noalloc(void* model, const void* x_device, int rows, int prefix,
+                                void* predictions_device, char* error, size_t error_capacity);
+
 /* fastpls_resident_simpls_destroy (src/cuda_resident_api.h:60). */
 void fastpls_hip_destroy(void* model);
 
