@@ -91,6 +91,14 @@ SIGNATURES = {
     "fastpls_hip_plssvd_cv_classification": _CV_CLASSIFICATION,
     "fastpls_hip_rsvd": (C.c_int, [_cvp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                    C.c_int, C.c_uint, C.c_int, _dp, _dp, _dp, _ip, _dp, _cp, _sz]),
+    "fastpls_hip_core_workspace_create_f32": (_vp, [_cp, _sz]),
+    "fastpls_hip_core_workspace_destroy_f32": (None, [_vp]),
+    "fastpls_hip_core_gemm_into_f32": (C.c_int, [_vp, _cvp, _sz, _sz, _sz, _cvp, _sz, _sz, _sz,
+                                                 C.c_int, C.c_int, _vp, _sz, _sz, _sz, _cp, _sz]),
+    "fastpls_hip_core_self_gram_into_f32": (C.c_int, [_vp, _cvp, _sz, _sz, _sz, C.c_int, _vp,
+                                                      _sz, _sz, _sz, C.c_int, _cp, _sz]),
+    "fastpls_hip_core_gemm_f32": (C.c_int, [_cvp, _sz, _sz, _sz, _cvp, _sz, _sz, _sz, C.c_int,
+                                            C.c_int, _vp, _cp, _sz]),
     "fastpls_hip_fastcor": (C.c_int, [_cvp, C.c_int, C.c_int, _cvp, C.c_int, C.c_int, C.c_int,
                                       C.c_int, C.c_int, C.c_int, _dp, _cp, _sz]),
 }
@@ -801,5 +809,75 @@ def fastcor(a, b=None, by_row=True, diagonal=True, precision=64):
                                0 if bb is None else bb.shape[0], 0 if bb is None else bb.shape[1],
                                precision, 0, int(bool(by_row)), int(bool(diagonal)), _dptr(out),
                                e, 1024) != 0:
+        _raise(e)
+    return out
+
+
+def _view32(a):
+    """(pointer, rows, columns, leading dimension) of a 2-D float32 array that
+    is column-major with unit row stride (a column slice of a larger Fortran
+    array keeps its parent's leading dimension)."""
+    if a.dtype != np.float32 or a.ndim != 2:
+        raise ValueError("expected a column-major float32 matrix view")
+    if a.size == 0:
+        return _ptr(a), a.shape[0], a.shape[1], max(a.shape[0], 1)
+    if a.strides[0] != 4 and a.shape[0] > 1:
+        raise ValueError("expected a column-major float32 matrix view")
+    ld = a.strides[1] // 4 if a.shape[1] > 1 else max(a.shape[0], 1)
+    return _ptr(a), a.shape[0], a.shape[1], ld
+
+
+class CoreWorkspace:
+    """The FP32 core hooks (fastpls_hip_core_*_f32) over host matrix views."""
+
+    def __init__(self):
+        e = _err()
+        self._h = load_library().fastpls_hip_core_workspace_create_f32(e, 1024)
+        if not self._h:
+            _raise(e)
+
+    def gemm_into(self, left, right, transpose_left, transpose_right, out):
+        lp, lr, lc, ll = _view32(left)
+        rp, rr, rc, rl = _view32(right)
+        op, orows, ocols, old = _view32(out)
+        e = _err()
+        if load_library().fastpls_hip_core_gemm_into_f32(
+                self._h, lp, lr, lc, ll, rp, rr, rc, rl, int(bool(transpose_left)),
+                int(bool(transpose_right)), op, orows, ocols, old, e, 1024) != 0:
+            _raise(e)
+
+    def self_gram_into(self, a, transpose_input, out, full_output):
+        ap, ar, ac, al = _view32(a)
+        op, orows, ocols, old = _view32(out)
+        e = _err()
+        if load_library().fastpls_hip_core_self_gram_into_f32(
+                self._h, ap, ar, ac, al, int(bool(transpose_input)), op, orows, ocols, old,
+                int(bool(full_output)), e, 1024) != 0:
+            _raise(e)
+
+    def close(self):
+        if self._h:
+            load_library().fastpls_hip_core_workspace_destroy_f32(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def core_gemm_f32(left, right, transpose_left=False, transpose_right=False):
+    """cuda_core_gemm_f32 counterpart: op(left) op(right) as a new float32 array."""
+    lp, lr, lc, ll = _view32(left)
+    rp, rr, rc, rl = _view32(right)
+    rows = lc if transpose_left else lr
+    cols = rr if transpose_right else rc
+    out = np.zeros((rows, cols), dtype=np.float32, order="F")
+    e = _err()
+    if load_library().fastpls_hip_core_gemm_f32(lp, lr, lc, ll, rp, rr, rc, rl,
+                                                int(bool(transpose_left)),
+                                                int(bool(transpose_right)), _ptr(out), e,
+                                                1024) != 0:
         _raise(e)
     return out

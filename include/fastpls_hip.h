@@ -348,6 +348,48 @@ int fastpls_hip_fastcor(const void* a, int a_rows, int a_cols, const void* b, in
                         int b_cols, int precision, int device, int by_row, int diagonal,
                         double* out, char* error, size_t error_capacity);
 
+/* ------------------------------------------------- FP32 core hooks
+ * Counterparts of the reference's hybrid accelerator hooks for its float
+ * core (src/accelerator_core_backend.h:10-36: cuda_core_gemm_f32,
+ * cuda_core_workspace_create_f32 / _destroy_f32, cuda_core_gemm_into_f32,
+ * cuda_core_self_gram_into_f32; implemented in src/core_cuda_backend.cu:38-293
+ * and called by RoutedLinearAlgebraF32, src/r_api.cpp:421-660). The C++
+ * ConstMatrixView<float> / MatrixView<float> arguments become (pointer, rows,
+ * columns, leading dimension) of host column-major matrices; bool returns
+ * become 0 / 1 with the message in the error buffer (C++ exceptions never
+ * cross the ABI). No CPU fallback: without a device every call fails.
+ *
+ * The workspace owns one stream and device buffers that grow as needed and
+ * persist across calls. Products are FP32-accurate (bf16x6 matrix-core
+ * products or FP64 VALU sums, FP64 accumulation) and rounded once to FP32. */
+void* fastpls_hip_core_workspace_create_f32(char* error, size_t error_capacity);
+void fastpls_hip_core_workspace_destroy_f32(void* workspace);
+/* output (output_rows x output_columns, leading dimension output_ld) =
+ * op(left) op(right), op = transpose when the flag is set. A product with an
+ * empty dimension succeeds without touching output, as the reference's. */
+int fastpls_hip_core_gemm_into_f32(void* workspace, const float* left, size_t left_rows,
+                                   size_t left_columns, size_t left_ld, const float* right,
+                                   size_t right_rows, size_t right_columns, size_t right_ld,
+                                   int transpose_left, int transpose_right, float* output,
+                                   size_t output_rows, size_t output_columns, size_t output_ld,
+                                   char* error, size_t error_capacity);
+/* output (d x d) = input inputᵀ (transpose_input 0, d = rows) or inputᵀ input
+ * (d = columns): the lower triangle, with the strict upper triangle zero, or
+ * mirrored from the lower one when full_output != 0 (cublasSsyrk lower +
+ * mirror_lower_to_upper in the reference). */
+int fastpls_hip_core_self_gram_into_f32(void* workspace, const float* input, size_t rows,
+                                        size_t columns, size_t ld, int transpose_input,
+                                        float* output, size_t output_rows, size_t output_columns,
+                                        size_t output_ld, int full_output, char* error,
+                                        size_t error_capacity);
+/* cuda_core_gemm_f32: op(left) op(right) into output (contiguous, leading
+ * dimension = its rows), with a workspace of its own for the one call. */
+int fastpls_hip_core_gemm_f32(const float* left, size_t left_rows, size_t left_columns,
+                              size_t left_ld, const float* right, size_t right_rows,
+                              size_t right_columns, size_t right_ld, int transpose_left,
+                              int transpose_right, float* output, char* error,
+                              size_t error_capacity);
+
 /* Synthetic latent-factor data written straight into device memory
  * (Philox4x32-10 + Box-Muller): X (n x p, precision 32/64) = T P' + noise E,
  * Y (n x q, same precision) = T Q' + noise F, with factor column f scaled by
