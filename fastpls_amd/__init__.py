@@ -5,14 +5,14 @@ The compute path is the C ABI library ``lib/libfastpls_hip.so`` (declared in
 """
 from .resident import (AUDIT_FIELDS, FIELDS, PLSSVD, SIMPLS, CoreWorkspace, DeviceBuffer, HipError,
                        ResidentModel, Server, SimplsBatch, core_gemm_f32, device_info, fastcor,
-                       fastsvd, gemm,
+                       fastsvd, gemm, lda_moments, plssvd_weights,
                        latent_data_device, load_library,
                        plssvd_cv_classification, plssvd_cv_regression, simpls_cv_classification,
                        simpls_cv_regression)
 
 __all__ = ["AUDIT_FIELDS", "FIELDS", "PLSSVD", "SIMPLS", "CoreWorkspace", "DeviceBuffer", "HipError",
            "ResidentModel", "Server", "SimplsBatch", "core_gemm_f32", "device_info", "fastcor",
-           "fastsvd", "gemm",
+           "fastsvd", "gemm", "lda_moments", "plssvd_weights",
            "latent_data_device", "load_library",
            "plssvd_cv_classification", "plssvd_cv_regression", "simpls_cv_classification",
            "simpls_cv_regression"]

@@ -210,6 +210,7 @@ class SimplsBatch {
     double *cand, *cand_scaled, *t_raw, *tpart, *loads;
     int splits = 1, ls_splits = 1, xs_splits = 1;
     int* flags;
+    int* col_flags;  // per response column: offset of its fit's flag word in the status block
   };
   struct Fit {
     int q, A, bmax, wmax, nsteps, off, boff;
@@ -384,6 +385,9 @@ class SimplsBatch {
       if (!fits_[f].labels) center_count_ += plan_[f].q;
     const size_t center_off = up_.add(sizeof(CenterColumn<T>) * (size_t)center_count_);
     const size_t div_off = up_.add(sizeof(DivRows) * plan_.size());
+    std::vector<size_t> colflag_off(gplan_.size());
+    for (size_t g = 0; g < gplan_.size(); ++g)
+      colflag_off[g] = up_.add(sizeof(int) * std::max(gplan_[g].qcat, 1));
     arena_.allocate();
     up_.allocate();
     // host image of the uploaded pieces (states and flags start at zero):
@@ -400,6 +404,7 @@ class SimplsBatch {
       G.spart = arena_.at<double>(o[8]); G.cand = arena_.at<double>(o[9]);
       G.cand_scaled = arena_.at<double>(o[10]); G.t_raw = arena_.at<double>(o[11]);
       G.loads = arena_.at<double>(o[12]); G.flags = up_.at<int>(status_off_) + 4 * g;
+      G.col_flags = up_.at<int>(colflag_off[g]);
       G.tpart = G.xs_splits > 1 ? arena_.at<double>(o[14]) : nullptr;
       frag_cursor[g] = arena_.at<char>(o[15]);
     }
@@ -441,6 +446,13 @@ class SimplsBatch {
       d.ls_part = G.loads + (size_t)F.boff * G.p;
       d.R = F.R; d.Q = F.Q; d.V = F.V; d.T = F.Tsc;
       d.scratch = F.scratch; d.state = F.state;
+    }
+    // a non-finite cross-product column flags the fit that owns it
+    status_base_ = up_.at<int>(status_off_);
+    for (size_t f = 0; f < plan_.size(); ++f) {
+      const Fit& F = plan_[f];
+      int* cf = reinterpret_cast<int*>(image.data() + colflag_off[fits_[f].group]);
+      for (int c = 0; c < F.q; ++c) cf[F.off + c] = int(4 * (gplan_.size() + f) + 2);
     }
     devfits_ = up_.at<SimplsFitDev>(devfits_off_);
     std::memcpy(image.data() + devfits_off_, dev.data(), sizeof(SimplsFitDev) * dev.size());
@@ -638,7 +650,8 @@ class SimplsBatch {
         });
     for (Group& G : gplan_)
       launch_finalize_crossprod<T>(G.X, G.n, G.p, G.qcat, scaling_, G.stats, G.spart, G.splits,
-                                   G.ysum, G.mx, G.sx, G.colss, G.S, G.S0, G.flags, stream_);
+                                   G.ysum, G.mx, G.sx, G.colss, G.S, G.S0, G.flags, stream_,
+                                   status_base_, G.col_flags);
     // right Grams SᵀS of every fit: one launch
     std::vector<XtwTask<double>> gram;
     for (size_t f = 0; f < plan_.size(); ++f) {
@@ -764,6 +777,7 @@ class SimplsBatch {
   DivRows* div_list_ = nullptr;
   long long div_max_ = 0;
   int center_count_ = 0;
+  int* status_base_ = nullptr;
   size_t devfits_off_ = 0;
   size_t status_off_ = 0;  // int[4] flags per group, then int[4] state per fit
   SimplsFitDev* devfits_ = nullptr;

@@ -124,13 +124,17 @@ void launch_label_response(const int* labels, long long n, int K, int* counts,
                            double* priors, double* Yc, double* H, double* ysum,
                            int* flags, hipStream_t s);
 // From the stats/cross-product partials of a kShiftStats pass compute mean,
-// scale and S (p x q), and copy S into S0 when S0 is non-null.
+// scale and S (p x q), and copy S into S0 when S0 is non-null. Non-finite
+// statistics set `flags`; a non-finite entry of S column c sets `flags`, or
+// flags[cross_offsets[c]] when a device array of per-column offsets is given
+// (columns of a batch belong to different fits).
 template<class T>
 void launch_finalize_crossprod(const T* X, long long n, int p, int q, int scaling,
                                const double* stats_part, const double* s_part, int splits,
                                const double* ysum, double* mean, double* scale,
                                double* colss, double* S, double* S0, int* flags,
-                               hipStream_t s);
+                               hipStream_t s, int* cross_flags = nullptr,
+                               const int* cross_offsets = nullptr);
 // out[i] = in[i] / d[i % rows]  (scale rows of a rows x cols matrix)
 void launch_div_rows(const double* in, const double* d, long long rows, int cols,
                      double* out, hipStream_t s);

@@ -189,10 +189,11 @@ template<class T>
 __global__ __launch_bounds__(kWG) void k_finalize_cross(
     const T* X, long long n, int p, int q, const double* s_part, int splits,
     const double* ysum, const double* mean, const double* scale, double* S, double* S0,
-    int* flags) {
+    int* flags, const int* offsets) {
   const int j = blockIdx.x * kWG + threadIdx.x;
   const int c = blockIdx.y;
   if (j >= p) return;
+  if (offsets) flags += offsets[c];
   const double delta = double(X[(long long)j * n]) - mean[j];
   const double acc = sum_strided(s_part + j + (long long)c * p, (long long)p * q, splits);
   const double v = (acc + delta * ysum[c]) / scale[j];
@@ -472,13 +473,14 @@ void launch_finalize_crossprod(const T* X, long long n, int p, int q, int scalin
                                const double* stats_part, const double* s_part, int splits,
                                const double* ysum, double* mean, double* scale,
                                double* colss, double* S, double* S0, int* flags,
-                               hipStream_t s) {
+                               hipStream_t s, int* cross_flags, const int* cross_offsets) {
   const unsigned blocks = unsigned((p + kWG - 1) / kWG);
   hipLaunchKernelGGL((k_finalize_stats<T>), dim3(blocks), dim3(kWG), 0, s, X, n, p, scaling,
                      stats_part, splits, mean, scale, colss, flags);
   if (q > 0)
     hipLaunchKernelGGL((k_finalize_cross<T>), dim3(blocks, q), dim3(kWG), 0, s, X, n, p, q, s_part,
-                       splits, ysum, mean, scale, S, S0, flags);
+                       splits, ysum, mean, scale, S, S0,
+                       cross_offsets ? cross_flags : flags, cross_offsets);
   FPLS_LAUNCHED("k_finalize_cross");
 }
 
@@ -595,7 +597,8 @@ void launch_convert(const double* in, T* out, long long count, hipStream_t s) {
   template void launch_finalize_crossprod<T>(const T*, long long, int, int, int,             \
                                              const double*, const double*, int,              \
                                              const double*, double*, double*, double*,       \
-                                             double*, double*, int*, hipStream_t);           \
+                                             double*, double*, int*, hipStream_t, int*,      \
+                                             const int*);                                    \
   template void launch_apply_small<T>(const double*, long long, int, const double*,          \
                                       long long, long long, int, const double*, T*, double*, \
                                       hipStream_t);                                          \

@@ -724,7 +724,7 @@ __global__ __launch_bounds__(kWG) void k_small_svd(const double* M, int m, int n
 // ------------------------------------------------------ PLS-SVD weights
 // plssvd.hpp:138-164 with linalg.hpp:57-177: prefix k (work-group k - 1)
 // solves G_k L = diag(sigma_1..k) for the leading k x k block of TᵀT by
-// Cholesky (linalg.hpp:4-47), or by pivoted elimination (linalg.hpp:51-114)
+// Cholesky (linalg.hpp:57-100), or by pivoted elimination (linalg.hpp:104-154)
 // when a pivot is not positive or the solution is not finite, and writes
 // W_k = L Q_kᵀ (k x q, leading dimension k). Prefixes are independent.
 __global__ __launch_bounds__(kWG) void k_plssvd_weights(const double* G, const double* sigma,
@@ -788,7 +788,7 @@ __global__ __launch_bounds__(kWG) void k_plssvd_weights(const double* G, const d
     solved = __syncthreads_and(good ? 1 : 0) != 0;
   }
   if (!solved) {
-    // pivoted elimination (linalg.hpp:51-114), serial
+    // pivoted elimination (linalg.hpp:104-154, pivoted_solve), serial
     if (tid == 0) {
       double scale = 0.0;
       for (int j = 0; j < k; ++j)

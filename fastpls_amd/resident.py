@@ -91,6 +91,9 @@ SIGNATURES = {
     "fastpls_hip_plssvd_cv_classification": _CV_CLASSIFICATION,
     "fastpls_hip_rsvd": (C.c_int, [_cvp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                    C.c_int, C.c_uint, C.c_int, _dp, _dp, _dp, _ip, _dp, _cp, _sz]),
+    "fastpls_hip_lda_moments": (C.c_int, [_dp, C.c_int, _dp, _ip, C.c_int, _ll, C.c_int, _dp,
+                                          _dp, _dp, _cp, _sz]),
+    "fastpls_hip_plssvd_weights": (C.c_int, [_dp, _dp, _dp, C.c_int, C.c_int, _dp, _cp, _sz]),
     "fastpls_hip_core_workspace_create_f32": (_vp, [_cp, _sz]),
     "fastpls_hip_core_workspace_destroy_f32": (None, [_vp]),
     "fastpls_hip_core_gemm_into_f32": (C.c_int, [_vp, _cvp, _sz, _sz, _sz, _cvp, _sz, _sz, _sz,
@@ -881,3 +884,35 @@ def core_gemm_f32(left, right, transpose_left=False, transpose_right=False):
                                                 1024) != 0:
         _raise(e)
     return out
+
+
+def lda_moments(gram, class_sums, counts, n, prefix):
+    """LDA from score moments on the device (fastpls_hip_lda_moments):
+    returns dict(linear=K x prefix, constants=K, ridge)."""
+    g = np.asfortranarray(gram, dtype=np.float64)
+    cs = np.asfortranarray(class_sums, dtype=np.float64)
+    cnt = np.ascontiguousarray(counts, dtype=np.int32)
+    a, K = g.shape[0], cs.shape[0]
+    lin = np.zeros((K, prefix), order="F"); con = np.zeros(K); ridge = np.zeros(1)
+    e = _err()
+    if load_library().fastpls_hip_lda_moments(_dptr(g), a, _dptr(cs), _iptr(cnt), K, int(n),
+                                              int(prefix), _dptr(lin), _dptr(con), _dptr(ridge),
+                                              e, 1024) != 0:
+        _raise(e)
+    return dict(linear=lin, constants=con, ridge=float(ridge[0]))
+
+
+def plssvd_weights(gram, sigma, Q):
+    """PLS-SVD prefix weights on the device (fastpls_hip_plssvd_weights):
+    list of W_k (k x q) for k = 1..components."""
+    g = np.asfortranarray(gram, dtype=np.float64)
+    s = np.ascontiguousarray(sigma, dtype=np.float64)
+    q_ = np.asfortranarray(Q, dtype=np.float64)
+    A, q = g.shape[0], q_.shape[0]
+    W = np.zeros(A * A * q)
+    e = _err()
+    if load_library().fastpls_hip_plssvd_weights(_dptr(g), _dptr(s), _dptr(q_), A, q, _dptr(W), e,
+                                                 1024) != 0:
+        _raise(e)
+    return [W[(k - 1) * A * q:(k - 1) * A * q + k * q].reshape((k, q), order="F")
+            for k in range(1, A + 1)]

@@ -348,6 +348,31 @@ int fastpls_hip_fastcor(const void* a, int a_rows, int a_cols, const void* b, in
                         int b_cols, int precision, int device, int by_row, int diagonal,
                         double* out, char* error, size_t error_capacity);
 
+/* ------------------------------------------------ small operators
+ * LDA from score moments (no ABI counterpart in the reference; its core's
+ * train_lda_prefixes_from_moments + finalize_lda_prefixes, lda.hpp:226-346,
+ * and its resident CUDA LDA, src/cuda_resident_lda.cuh:41-121, which works
+ * from device moments): gram (a x a) = TᵀT of the training scores,
+ * class_sums (classes x a) the per-class score sums, counts the class sizes
+ * (summing to n). For the first `prefix` components writes linear (classes x
+ * prefix, column-major), constants (classes) and the ridge chosen from
+ * {1e-8, 1e-6, 1e-5, 1e-4, 1e-3, 1e-2} x trace(pooled) / prefix; fails when
+ * every ridge leaves the pooled covariance without a Cholesky factor.
+ * prefix <= 64. */
+int fastpls_hip_lda_moments(const double* gram, int a, const double* class_sums,
+                            const int* counts, int classes, long long n, int prefix,
+                            double* linear, double* constants, double* ridge, char* error,
+                            size_t error_capacity);
+/* PLS-SVD prefix weights (no ABI counterpart; plssvd.hpp:138-164): for k =
+ * 1..components, L_k solves G_k L = diag(sigma_1..k) for the leading k x k
+ * block of the score Gram (Cholesky, pivoted elimination when it fails,
+ * linalg.hpp:57-177) and W_k = L_k Q_kᵀ (k x q, leading dimension k) is
+ * written at weights + (k - 1) * components * q. Q is q x components.
+ * components <= 64; fails when a prefix has no solution. */
+int fastpls_hip_plssvd_weights(const double* gram, const double* sigma, const double* Q,
+                               int components, int q, double* weights, char* error,
+                               size_t error_capacity);
+
 /* ------------------------------------------------- FP32 core hooks
  * Counterparts of the reference's hybrid accelerator hooks for its float
  * core (src/accelerator_core_backend.h:10-36: cuda_core_gemm_f32,

@@ -1455,7 +1455,10 @@ bool lda_chol_solve(const Mat& pooled, const Mat& means, double ridge, Mat& line
   return true;
 }
 
-// lda.hpp:86-223
+std::vector<Lda> finalize_lda(const Mat& pooled, const Mat& means, const std::vector<double>& counts,
+                              size_t n, const std::vector<int>& prefixes);
+
+// lda.hpp:156-223 (train_lda_prefixes)
 std::vector<Lda> train_lda(const Mat& T, const int* labels1, size_t K,
                            const std::vector<int>& prefixes) {
   const size_t n = T.r;
@@ -1487,6 +1490,50 @@ std::vector<Lda> train_lda(const Mat& T, const int* labels1, size_t K,
       pooled(r2, c2) = v; pooled(c2, r2) = v;
     }
   }
+  return finalize_lda(pooled, means, counts, n, prefixes);
+}
+
+// lda.hpp:226-300 (train_lda_prefixes_from_moments): class means and the
+// pooled covariance from the score Gram and the per-class score sums.
+std::vector<Lda> train_lda_moments(const Mat& gram, const Mat& sums, const std::vector<double>& counts,
+                                   size_t n, const std::vector<int>& prefixes) {
+  const size_t K = sums.r;
+  size_t maximum = 0;
+  for (int pfx : prefixes) {
+    if (pfx < 1 || size_t(pfx) > gram.c) throw std::invalid_argument("oracle LDA prefix outside moment dimension");
+    maximum = std::max(maximum, size_t(pfx));
+  }
+  Mat means(K, maximum);
+  double total = 0;
+  for (size_t c = 0; c < K; ++c) {
+    if (!std::isfinite(counts[c]) || counts[c] <= 0) throw std::invalid_argument("oracle LDA empty class");
+    total += counts[c];
+    for (size_t j = 0; j < maximum; ++j) {
+      if (!std::isfinite(sums(c, j))) throw std::invalid_argument("oracle LDA moments must be finite");
+      means(c, j) = sums(c, j) / counts[c];
+    }
+  }
+  if (std::abs(total - double(n)) > 1e-8 * std::max(1.0, double(n)))
+    throw std::invalid_argument("oracle LDA class counts do not sum to n");
+  Mat pooled(maximum, maximum);
+  const double den = double(std::max<size_t>(1, n > K ? n - K : 1));
+  for (size_t c2 = 0; c2 < maximum; ++c2) {
+    for (size_t r2 = 0; r2 <= c2; ++r2) {
+      double v = gram(r2, c2);
+      if (!std::isfinite(v)) throw std::invalid_argument("oracle LDA moments must be finite");
+      for (size_t c = 0; c < K; ++c) v -= counts[c] * means(c, r2) * means(c, c2);
+      v /= den;
+      pooled(r2, c2) = v; pooled(c2, r2) = v;
+    }
+  }
+  return finalize_lda(pooled, means, counts, n, prefixes);
+}
+
+// lda.hpp:86-153 (finalize_lda_prefixes): per prefix, the ridge ladder over
+// the pooled covariance, Cholesky solves for the linear weights, constants.
+std::vector<Lda> finalize_lda(const Mat& pooled, const Mat& means, const std::vector<double>& counts,
+                              size_t n, const std::vector<int>& prefixes) {
+  const size_t K = means.r;
   static const double grid[] = {1e-8, 1e-6, 1e-5, 1e-4, 1e-3, 1e-2};
   std::vector<Lda> models;
   for (int pfx : prefixes) {
@@ -1864,6 +1911,49 @@ int orc_simpls_predict(const double* x, int rows, int p, int q, int A,
     Mat Yh = mul(Tk, false, Qk, true);
     for (int c = 0; c < q; ++c) for (int i = 0; i < rows; ++i) Yh(i, c) += my[c];
     copy_out(Yh, out);
+  });
+}
+
+// LDA from score moments (lda.hpp:226-346): linear K x prefix, constants K,
+// the ridge and its relative level.
+int orc_lda_moments(const double* gram, int a, const double* sums, const int* counts, int K,
+                    long long n, int prefix, double* linear, double* constants, double* ridge,
+                    double* relative) {
+  return guarded([&] {
+    std::vector<double> cnt(counts, counts + K);
+    auto models = train_lda_moments(wrap(gram, a, a), wrap(sums, K, a), cnt, size_t(n), {prefix});
+    const Lda& m = models[0];
+    std::copy(m.linear.d.begin(), m.linear.d.end(), linear);
+    std::copy(m.constants.begin(), m.constants.end(), constants);
+    *ridge = m.ridge;
+    *relative = m.relative;
+  });
+}
+
+// PLS-SVD prefix weights (plssvd.hpp:138-164): W_k = solve(G_k, diag(sigma))
+// Q_kᵀ at W + (k - 1) A q (k x q); route[k - 1] = 0 Cholesky, 1 pivoted.
+int orc_plssvd_weights(const double* G, const double* sigma, const double* Q, int A, int q,
+                       double* W, int* route) {
+  return guarded([&] {
+    const Mat g = wrap(G, A, A), qm = wrap(Q, q, A);
+    for (int k = 1; k <= A; ++k) {
+      Mat gk(k, k), dg(k, k), lat;
+      for (int j = 0; j < k; ++j) {
+        dg(j, j) = sigma[j];
+        for (int i = 0; i < k; ++i) gk(i, j) = g(i, j);
+      }
+      if (cholesky_solve(gk, dg, lat)) {
+        route[k - 1] = 0;
+      } else if (pivoted_solve(gk, dg, lat)) {
+        route[k - 1] = 1;
+      } else {
+        throw std::runtime_error("oracle PLS-SVD latent solve failed");
+      }
+      Mat qk(q, k);
+      for (int j = 0; j < k; ++j) for (int i = 0; i < q; ++i) qk(i, j) = qm(i, j);
+      const Mat wk = mul(lat, false, qk, true);
+      std::copy(wk.d.begin(), wk.d.end(), W + size_t(k - 1) * A * q);
+    }
   });
 }
 

@@ -55,6 +55,25 @@ AUDIT_FIELDS = ("certified", "deterministic_fallback", "attempts", "effective_ov
                 "triplet_residual", "omitted_direction_ratio")
 
 
+def _lda_moments(fn, check, gram, class_sums, counts, cptr, n, prefix):
+    g = _f64(gram); cs = _f64(class_sums)
+    K, a = cs.shape
+    lin = np.zeros((K, prefix), order="F"); con = np.zeros(K)
+    ridge = C.c_double(0); rel = C.c_double(0)
+    check(fn(_d(g), a, _d(cs), cptr(counts), K, C.c_longlong(int(n)), int(prefix), _d(lin),
+             _d(con), C.byref(ridge), C.byref(rel)))
+    return dict(linear=lin, constants=con, ridge=ridge.value, relative=rel.value)
+
+
+def _plssvd_weights(fn, check, gram, sigma, Q):
+    g = _f64(gram); s = np.ascontiguousarray(sigma, dtype=np.float64); q_ = _f64(Q)
+    A, q = g.shape[0], q_.shape[0]
+    W = np.zeros(A * A * q); route = np.zeros(A, dtype=np.int32)
+    check(fn(_d(g), _d(s), _d(q_), A, q, _d(W), _i(route)))
+    return dict(W=[W[(k - 1) * A * q:(k - 1) * A * q + k * q].reshape((k, q), order="F")
+                   for k in range(1, A + 1)], route=route)
+
+
 def _audited(lib_fn, check, a, k, oversample, power, seed, left_only):
     a = _f64(a); m, n = a.shape
     kk = min(k, m, n)
@@ -300,6 +319,15 @@ class Oracle:
                                      _d(disc), _i(pred), C.byref(ridge)))
         return dict(disc=disc, labels=pred, ridge=ridge.value)
 
+    def lda_moments(self, gram, class_sums, counts, n, prefix):
+        """LDA from score moments (lda.hpp:226-346)."""
+        return _lda_moments(self.lib.orc_lda_moments, self._check, gram, class_sums,
+                            np.ascontiguousarray(counts, dtype=np.int32), _i, n, prefix)
+
+    def plssvd_weights(self, gram, sigma, Q):
+        """PLS-SVD prefix weights (plssvd.hpp:138-164) and the solve route per prefix."""
+        return _plssvd_weights(self.lib.orc_plssvd_weights, self._check, gram, sigma, Q)
+
     def topk(self, values, top):
         v = _f64(values); rows, classes = v.shape
         out = np.zeros((rows, top), dtype=np.int32, order="F")
@@ -442,6 +470,13 @@ class Reference:
     def audited_rsvd(self, a, k, oversample=32, power=5, seed=1, left_only=False):
         return _audited(self.lib.ref_audited_rsvd, self._check, a, k, oversample, power, seed,
                         left_only)
+
+    def lda_moments(self, gram, class_sums, counts, n, prefix):
+        return _lda_moments(self.lib.ref_lda_moments, self._check, gram, class_sums,
+                            np.ascontiguousarray(counts, dtype=np.float64), _d, n, prefix)
+
+    def plssvd_weights(self, gram, sigma, Q):
+        return _plssvd_weights(self.lib.ref_plssvd_weights, self._check, gram, sigma, Q)
 
     def lda(self, T, labels, classes, prefix, test_scores):
         T = _f64(T); n, a = T.shape

@@ -303,4 +303,53 @@ int ref_lda(const double* T, int n, int a, const int* labels, int K, int prefix,
   });
 }
 
+// LDA from score moments: the reference's train_lda_prefixes_from_moments
+// (lda.hpp:226-300) with finalize_lda_prefixes. linear K x prefix.
+int ref_lda_moments(const double* gram, int a, const double* sums, const double* counts, int K,
+                    long long n, int prefix, double* linear, double* constants, double* ridge,
+                    double* relative) {
+  return guarded([&] {
+    Matrix<double> g = load(gram, a, a);
+    Matrix<double> cs = load(sums, K, a);
+    const int prefixes[] = {prefix};
+    auto models = fastpls::core::train_lda_prefixes_from_moments(
+      ConstMatrixView<double>(g.view()), ConstMatrixView<double>(cs.view()), counts,
+      std::size_t(K), std::size_t(n), prefixes, 1);
+    store(models[0].linear, linear);
+    std::copy(models[0].constants.begin(), models[0].constants.end(), constants);
+    *ridge = models[0].ridge;
+    *relative = models[0].relative_ridge;
+  });
+}
+
+// PLS-SVD prefix weights as plssvd.hpp:138-164 composes them, with the
+// reference's own solvers (linalg.hpp:57-177): route[k - 1] = 0 when its
+// Cholesky solve succeeds, 1 when the pivoted solve is needed.
+int ref_plssvd_weights(const double* G, const double* sigma, const double* Q, int A, int q,
+                       double* W, int* route) {
+  return guarded([&] {
+    for (int k = 1; k <= A; ++k) {
+      Matrix<double> gk(k, k), dg(k, k), lat;
+      for (int j = 0; j < k; ++j) {
+        dg(j, j) = sigma[j];
+        for (int i = 0; i < k; ++i) gk(i, j) = G[i + j * A];
+      }
+      if (fastpls::core::cholesky_solve(ConstMatrixView<double>(gk.view()),
+                                        ConstMatrixView<double>(dg.view()), lat)) {
+        route[k - 1] = 0;
+      } else if (fastpls::core::pivoted_solve(ConstMatrixView<double>(gk.view()),
+                                              ConstMatrixView<double>(dg.view()), lat)) {
+        route[k - 1] = 1;
+      } else {
+        throw std::runtime_error("fastPLS PLS-SVD latent solve failed");
+      }
+      Matrix<double> wk(k, q);
+      ConstMatrixView<double> qk(Q, q, k, q);
+      fastpls::core::reference_gemm(ConstMatrixView<double>(lat.view()), qk, false, true,
+                                    wk.view());
+      store(wk, W + std::size_t(k - 1) * A * q);
+    }
+  });
+}
+
 }  // extern "C"

@@ -33,15 +33,17 @@ def save(name, **arrays):
 
 def simpls_case(ref, name, X, Y=None, labels=None, classes=None, ncomp=5, oversample=32,
                 power=5, seed=1, block=1, cache=0, batch_geometry=0, store_inputs=True,
-                recipe=None):
+                recipe=None, scaling=2):
     m = ref.simpls_fit(X, Y, labels=labels, classes=classes, ncomp=ncomp, oversample=oversample,
                        power=power, seed=seed, block=block, cache=cache,
-                       batch_geometry=batch_geometry)
+                       batch_geometry=batch_geometry, scaling=scaling)
     arrays = dict(R=m["R"], Q=m["Q"], V=m["V"], T=m["T"], mx=m["mx"], sx=m["sx"], my=m["my"],
                   completed=np.int64(m["completed"]), ncomp=np.int64(ncomp),
                   oversample=np.int64(oversample), power=np.int64(power), seed=np.int64(seed),
                   block=np.int64(block), cache=np.int64(cache),
                   batch_geometry=np.int64(batch_geometry), x_sha256=digest(X))
+    if scaling != 2:
+        arrays["scaling"] = np.int64(scaling)
     if store_inputs:
         arrays["X"] = X
         if Y is not None:
@@ -55,12 +57,14 @@ def simpls_case(ref, name, X, Y=None, labels=None, classes=None, ncomp=5, oversa
 
 
 def plssvd_case(ref, name, X, Y=None, labels=None, classes=None, ncomp=3, oversample=32,
-                power=5, seed=1):
+                power=5, seed=1, scaling=2):
     m = ref.plssvd_fit(X, Y, labels=labels, classes=classes, ncomp=ncomp, oversample=oversample,
-                       power=power, seed=seed)
+                       power=power, seed=seed, scaling=scaling)
     arrays = dict(R=m["R"], Q=m["Q"], T=m["T"], sigma=m["sigma"], mx=m["mx"], sx=m["sx"],
                   my=m["my"], ncomp=np.int64(ncomp), oversample=np.int64(oversample),
                   power=np.int64(power), seed=np.int64(seed), X=X)
+    if scaling != 2:
+        arrays["scaling"] = np.int64(scaling)
     for k, w in enumerate(m["W"]):
         arrays[f"W{k + 1}"] = w
     if Y is not None:
@@ -120,6 +124,72 @@ def operator_cases(ref):
     save("plssvd_operator", **arrays)
 
 
+def edge_cases(ref):
+    """Scaling 1 (centring, R's default, R/main.R:9746-9748) and 3 (none) for
+    SIMPLS (dense and labels) and PLS-SVD; the LDA ridge ladder past 1e-8 and
+    its exhaustion (lda.hpp:86-131), and the PLS-SVD prefix solve's pivoted
+    fallback and its failure (plssvd.hpp:138-164, linalg.hpp:57-177), from
+    exactly constructed moments."""
+    X, Y = latent_data(150, 40, 3, factors=5, noise=0.1, seed=61)
+    for sc in (1, 3):
+        simpls_case(ref, f"simpls_scaling{sc}", X, Y, ncomp=5, seed=7, scaling=sc)
+    X, lab = class_data(300, 25, 4, latent=4, seed=62)
+    simpls_case(ref, "simpls_labels_scaling1", X, labels=lab, classes=4, ncomp=3, oversample=1,
+                power=2, seed=5, scaling=1)
+    X, Y = latent_data(120, 30, 8, factors=6, noise=0.1, seed=63)
+    for sc in (1, 3):
+        plssvd_case(ref, f"plssvd_scaling{sc}", X, Y, ncomp=3, oversample=2, power=2, seed=9,
+                    scaling=sc)
+
+    # LDA from moments: a pooled covariance with eigenvalues (2, 1, -6e-6)
+    # (rotated) needs the 1e-5 rung at prefix 3; one with eigenvalue -0.5
+    # fails at every rung
+    rng = np.random.default_rng(64)
+    Qr, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    counts = np.array([4, 5, 3], dtype=np.int32)
+    n = int(counts.sum())
+    means = rng.standard_normal((3, 3))
+    sums = means * counts[:, None]
+
+    def gram_of(eig):
+        pooled = Qr @ np.diag(eig) @ Qr.T
+        return pooled * (n - 3) + sum(counts[c] * np.outer(means[c], means[c]) for c in range(3))
+    arrays = dict(sums=sums, counts=counts, n=np.int64(n))
+    arrays["gram"] = gram_of([2.0, 1.0, -6e-6])
+    arrays["gram_fail"] = gram_of([2.0, 1.0, -0.5])
+    for k in (1, 2, 3):
+        m = ref.lda_moments(arrays["gram"], sums, counts, n, k)
+        arrays[f"linear{k}"] = m["linear"]
+        arrays[f"constants{k}"] = m["constants"]
+        arrays[f"ridge{k}"] = np.float64(m["ridge"])
+        arrays[f"relative{k}"] = np.float64(m["relative"])
+    try:
+        ref.lda_moments(arrays["gram_fail"], sums, counts, n, 3)
+        raise SystemExit("the failing LDA case was solved")
+    except RuntimeError:
+        pass
+    save("lda_ladder", **arrays)
+
+    # PLS-SVD prefix weights: prefix 1 takes Cholesky, prefixes 2-4 the
+    # pivoted solve (leading 2 x 2 block [[2, 3], [3, 2]] is indefinite); a
+    # singular Gram fails
+    G = np.array([[2.0, 3.0, 0.5, 0.1], [3.0, 2.0, 0.2, 0.3], [0.5, 0.2, 1.5, 0.4],
+                  [0.1, 0.3, 0.4, 1.2]])
+    sigma = np.array([5.0, 4.0, 3.0, 2.0])
+    Qm = rng.standard_normal((6, 4))
+    m = ref.plssvd_weights(G, sigma, Qm)
+    arrays = dict(G=G, sigma=sigma, Q=Qm, route=m["route"])
+    for k, w in enumerate(m["W"]):
+        arrays[f"W{k + 1}"] = w
+    arrays["G_singular"] = np.array([[1.0, 1.0], [1.0, 1.0]])
+    try:
+        ref.plssvd_weights(arrays["G_singular"], sigma[:2], Qm[:, :2])
+        raise SystemExit("the singular PLS-SVD case was solved")
+    except RuntimeError:
+        pass
+    save("plssvd_pivoted", **arrays)
+
+
 def main():
     build(reference=True)
     ref = Reference()
@@ -129,8 +199,12 @@ def main():
     if len(sys.argv) > 1 and sys.argv[1] == "operator":
         operator_cases(ref)
         return
+    if len(sys.argv) > 1 and sys.argv[1] == "edges":
+        edge_cases(ref)
+        return
     audited_cases(ref)
     operator_cases(ref)
+    edge_cases(ref)
 
     # SIMPLS, dense responses, explicit width = q (direction independent of the sketch)
     X, Y = latent_data(64, 20, 3, factors=5, noise=0.1, seed=11)

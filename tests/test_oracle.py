@@ -445,3 +445,70 @@ def test_blas_bound_oracle_matches_plain_loops(oracle):
     for key in ("R", "Q", "T"):
         a, b = plain_p[key], blas_p[key]
         assert np.linalg.norm(a - b) / np.linalg.norm(a) < 1e-9, key
+
+
+# ------------------------------------------------- scaling 1 / 3, edge rules
+@pytest.mark.parametrize("name", ["simpls_scaling1", "simpls_scaling3", "simpls_labels_scaling1"])
+def test_oracle_simpls_scaling_goldens(oracle, name):
+    """Scaling 1 (centring, R's default) and 3 (none) against the compiled
+    reference core (supervised.hpp:63-124, classification.hpp:86-184)."""
+    g = load_golden(name)
+    kw = dict(ncomp=int(g["ncomp"]), scaling=int(g["scaling"]), oversample=int(g["oversample"]),
+              power=int(g["power"]), seed=int(g["seed"]))
+    if "labels" in g:
+        o = oracle.simpls_fit(g["X"], labels=g["labels"], classes=int(g["classes"]), **kw)
+    else:
+        o = oracle.simpls_fit(g["X"], g["Y"], **kw)
+    for k in range(1, int(g["ncomp"]) + 1):
+        B0 = g["R"][:, :k] @ g["Q"][:, :k].T
+        assert rel(o["R"][:, :k] @ o["Q"][:, :k].T, B0) < 1e-10, k
+    np.testing.assert_allclose(o["mx"], g["mx"], rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(o["sx"], g["sx"], rtol=1e-12)
+    if int(g["scaling"]) != 2:
+        assert np.all(g["sx"] == 1.0)
+    if int(g["scaling"]) == 3:
+        assert np.all(g["mx"] == 0.0)
+
+
+@pytest.mark.parametrize("name", ["plssvd_scaling1", "plssvd_scaling3"])
+def test_oracle_plssvd_scaling_goldens(oracle, name):
+    g = load_golden(name)
+    o = oracle.plssvd_fit(g["X"], g["Y"], ncomp=int(g["ncomp"]), scaling=int(g["scaling"]),
+                          oversample=int(g["oversample"]), power=int(g["power"]),
+                          seed=int(g["seed"]))
+    assert rel(align_columns(o["R"], g["R"]), g["R"]) < 1e-10
+    np.testing.assert_allclose(o["sigma"], g["sigma"], rtol=1e-12)
+    for k in range(1, int(g["ncomp"]) + 1):
+        # B_k = R_k W_k does not depend on the components' signs
+        assert rel(o["R"][:, :k] @ o["W"][k - 1], g["R"][:, :k] @ g[f"W{k}"]) < 1e-10, k
+    np.testing.assert_allclose(o["mx"], g["mx"], rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(o["sx"], g["sx"], rtol=1e-12)
+
+
+def test_oracle_lda_ladder_golden(oracle):
+    """The LDA ridge ladder past 1e-8 and its exhaustion (lda.hpp:86-131),
+    from moments whose pooled covariance has a negative eigenvalue of
+    -6e-6 (the 1e-5 rung) or -0.5 (every rung fails)."""
+    g = load_golden("lda_ladder")
+    for k in (1, 2, 3):
+        o = oracle.lda_moments(g["gram"], g["sums"], g["counts"], int(g["n"]), k)
+        assert o["relative"] == float(g[f"relative{k}"]), k
+        np.testing.assert_allclose(o["linear"], g[f"linear{k}"], rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(o["constants"], g[f"constants{k}"], rtol=1e-12, atol=1e-12)
+    assert float(g["relative3"]) == 1e-5
+    with pytest.raises(RuntimeError, match="every ridge"):
+        oracle.lda_moments(g["gram_fail"], g["sums"], g["counts"], int(g["n"]), 3)
+
+
+def test_oracle_plssvd_pivoted_golden(oracle):
+    """The PLS-SVD prefix solve: Cholesky for prefix 1, the pivoted solve for
+    the indefinite prefixes 2-4 (linalg.hpp:104-177), failure for a singular
+    Gram."""
+    g = load_golden("plssvd_pivoted")
+    o = oracle.plssvd_weights(g["G"], g["sigma"], g["Q"])
+    np.testing.assert_array_equal(o["route"], g["route"])
+    assert list(g["route"]) == [0, 1, 1, 1]
+    for k in range(1, 5):
+        np.testing.assert_allclose(o["W"][k - 1], g[f"W{k}"], rtol=1e-12, atol=1e-12)
+    with pytest.raises(RuntimeError, match="latent solve failed"):
+        oracle.plssvd_weights(g["G_singular"], g["sigma"][:2], g["Q"][:, :2])
