@@ -41,9 +41,6 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-# Hardware queues per process (HIP's default is 4): the serving section runs
-# many streams at once, and streams beyond the queue count share queues.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E bytes/s
 
@@ -698,7 +695,10 @@ def c5_serving(fp, requests=10000, rate=500.0, seed=3, workers=32, max_fits=0):
         "fit_service_ms_mean": round(st["fit_service_ms"], 3),
         "predict_service_ms_mean": round(st["predict_service_ms"], 3),
         "workers": workers,
-        "max_concurrent_fits": max_fits if max_fits > 0 else max(1, workers // 2),
+        "fit_slots": min(max_fits if max_fits > 0 else 2, 4,
+                         int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) - 2),
+        "query_streams": 2,
+        "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4),
     }
 
 

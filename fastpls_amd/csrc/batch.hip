@@ -433,7 +433,10 @@ class SimplsBatch {
       d.n = G.n; d.p = G.p; d.q = F.q; d.A = F.A;
       d.power = power_; d.use_right_gram = F.right_gram ? 1 : 0;
       // src/r_api.cpp:1416: two passes for float, one for double, unless forced
-      d.reorth = reorth_ >= 0 ? reorth_ : (sizeof(T) == 4 ? 1 : 0);
+      // one Gram-Schmidt pass unless forced: FP32-stored fits compute in FP64
+      // and take the FP64 routing (src/r_api.cpp:1416 doubles the passes
+      // only to offset float arithmetic)
+      d.reorth = reorth_ >= 0 ? reorth_ : 0;
       d.nsteps = F.nsteps; d.bmax = F.bmax; d.ts_splits = 1; d.ls_splits = G.ls_splits;
       d.ls_slab = (long long)G.p * G.bcat;
       d.steps = F.steps; d.omega = F.omega;

@@ -43,7 +43,7 @@ class Model final : public Handle, private StreamOps {
 
   Model(int method, const T* x, const T* y, const int* labels, bool device_inputs, int n,
         int p, int q, int components, int scaling, int oversample, int power,
-        unsigned long long seed)
+        unsigned long long seed, hipStream_t stream = nullptr)
       : method_(method), n_(n), p_(p), q_(q), scaling_(scaling), oversample_(oversample),
         power_(power), seed_(seed), classification_(labels != nullptr) {
     if ((y == nullptr) == (labels == nullptr))
@@ -74,7 +74,8 @@ class Model final : public Handle, private StreamOps {
     // relative perturbation of S), so FP32-stored X is widened to FP64 for
     // every SIMPLS product; PLS-SVD keeps bf16x6 (FP32-accurate) products.
     f64_products_ = method_ == kSimpls;
-    stream_ = stream_take();
+    owns_stream_ = stream == nullptr;
+    stream_ = owns_stream_ ? stream_take() : stream;
     for (Buf* b : all_bufs()) b->bind(stream_);
     try {
       Event start, stop;
@@ -360,9 +361,22 @@ class Model final : public Handle, private StreamOps {
   void close_stream() noexcept {
     if (!stream_) return;
     for (Buf* b : all_bufs()) b->release();
-    stream_give(stream_);
+    if (owns_stream_) stream_give(stream_);
+    else (void)hipStreamSynchronize(stream_);
     stream_ = nullptr;
   }
+
+ public:
+  void attach_stream(hipStream_t stream) override {
+    if (!stream || stream == stream_) return;
+    FPLS_HIP(hipStreamSynchronize(stream_));
+    for (Buf* b : all_bufs()) b->rebind(stream);
+    if (owns_stream_) stream_give(stream_);
+    stream_ = stream;
+    owns_stream_ = false;
+  }
+
+ private:
 
   // --------------------------------------------------------- data load
   void load_inputs(const T* x, const T* y, const int* labels, bool device_inputs) {
@@ -577,9 +591,12 @@ class Model final : public Handle, private StreamOps {
     SimplsFitDev f{};
     f.n = n_; f.p = p_; f.q = q_; f.A = A_;
     f.power = power_; f.use_right_gram = right_gram ? 1 : 0;
-    // two passes for float (src/r_api.cpp:1416) and on the rank-one route
-    // (src/r_api.cpp:1491)
-    f.reorth = (sizeof(T) == 4 || rank_one_) ? 1 : 0;
+    // the rank-one route re-orthogonalises twice (src/r_api.cpp:1491); the
+    // reference's float core does too (src/r_api.cpp:1416) to offset float
+    // arithmetic, which this engine's FP32-stored fits do not have: their
+    // products and small math are FP64, so both precisions take the FP64
+    // routing (one pass) and give the same fit on the same values
+    f.reorth = rank_one_ ? 1 : 0;
     f.nsteps = int(steps_ref.size()); f.bmax = bmax; f.ts_splits = xs; f.ls_splits = ls;
     f.ls_slab = (long long)p_ * bmax;
     f.steps = steps_.as<RefreshStep>(); f.omega = omega_.d();
@@ -935,6 +952,7 @@ class Model final : public Handle, private StreamOps {
   bool compacted_ = false;
   double fit_seconds_ = 0.0, crossprod_seconds_ = 0.0, score_seconds_ = 0.0;
   Event cp0_, cp1_, sc0_, sc1_, pr0_, pr1_;
+  bool owns_stream_ = true;
   bool proj_timed_ = false;
   double proj_bytes_ = 0.0;
   const T* X_ = nullptr;
@@ -961,16 +979,16 @@ class Model final : public Handle, private StreamOps {
 
 Handle* create(int method, const void* x, const void* y, const int* labels, int precision,
                bool device, int n, int p, int q, int components, int scaling, int oversample,
-               int power, unsigned long long seed) {
+               int power, unsigned long long seed, hipStream_t stream) {
   require_device();
   if (precision == 32)
     return new Model<float>(method, static_cast<const float*>(x), static_cast<const float*>(y),
                             labels, device, n, p, q, components, scaling, oversample, power,
-                            seed);
+                            seed, stream);
   if (precision == 64)
     return new Model<double>(method, static_cast<const double*>(x),
                              static_cast<const double*>(y), labels, device, n, p, q, components,
-                             scaling, oversample, power, seed);
+                             scaling, oversample, power, seed, stream);
   throw std::invalid_argument("precision must be 32 or 64");
 }
 

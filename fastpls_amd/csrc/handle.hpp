@@ -3,6 +3,8 @@
 
 #include <cstddef>
 
+#include <hip/hip_runtime.h>
+
 namespace fpls {
 
 // One fitted model with its device buffers and stream
@@ -46,12 +48,19 @@ class Handle {
   // [0] fit, [1] cross-product pass over X, [2] score pass over X (PLS-SVD)
   // or the whole component loop (SIMPLS), [3] bytes of X, seconds from events.
   virtual void phases(double* out, int count) const = 0;
+  // Runs later calls on `stream`, owned by the caller (it outlives the handle
+  // or another attach): the current stream is drained first, and a stream
+  // the handle took from the stream cache goes back to it.
+  virtual void attach_stream(hipStream_t stream) = 0;
 };
 
 // method 0 = SIMPLS, 1 = PLS-SVD; precision 32 or 64. With `device_inputs`
-// x, y and labels are device arrays that must outlive the handle.
+// x, y and labels are device arrays that must outlive the handle. A non-null
+// `stream` (owned by the caller) carries the fit instead of a stream from the
+// stream cache.
 Handle* create(int method, const void* x, const void* y, const int* labels, int precision,
                bool device_inputs, int n, int p, int q, int components, int scaling,
-               int oversample, int power, unsigned long long seed);
+               int oversample, int power, unsigned long long seed,
+               hipStream_t stream = nullptr);
 
 }  // namespace fpls

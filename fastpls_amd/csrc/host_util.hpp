@@ -46,6 +46,11 @@ class Buf {
     stream_ = stream;
     bound_ = true;
   }
+  // Moves a bound buffer to another stream. The caller has synchronised the
+  // old stream, so no queued work uses the memory any more.
+  void rebind(hipStream_t stream) {
+    if (bound_) stream_ = stream;
+  }
   void alloc(size_t bytes) {
     release();
     if (bytes) {

@@ -8,11 +8,19 @@
 // may be passed by later requests that can run); a predict runs as soon as its
 // model is idle, together with every predict queued behind it for the same
 // model and prefix (up to a row cap), as one product over the concatenated
-// rows. Each model owns a stream, so fits and predicts on different models
-// overlap on the device. Fitted models stay resident (compacted) until
-// released. Query products use a depth-only split rule and every reduction
-// has a fixed order, so a request gives bitwise the same result alone and in
-// a mix. Latency is measured from submission to completion.
+// rows. Fitted models stay resident (compacted) until released. Query products
+// use a depth-only split rule and every reduction has a fixed order, so a
+// request gives bitwise the same result alone and in a mix. Latency is
+// measured from submission to completion.
+//
+// Streams: the server owns one stream per fit slot and a few query streams,
+// whatever the number of models, and never more than the device's hardware
+// queues can hold. Work on one hardware queue runs in order, so a stream that
+// shares a queue with a fit's stream waits behind the fit's kernels (hundreds
+// of milliseconds for a large fit), and queues beyond the hardware slots are
+// time-sliced; fits and queries therefore run on disjoint streams: a fit on
+// its slot's stream, every predict batch on a query stream its model is
+// attached to for the batch, with its rows staged on that same stream.
 //
 // The reference has no serving loop; its resident handle API
 // (src/cuda_resident_api.cuh:69-479) is what each request runs.
@@ -24,6 +32,7 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <memory>
@@ -53,6 +62,7 @@ struct Request {
   const int* labels = nullptr;
   bool device_inputs = false;
   size_t bytes = 0;  // admission estimate of a fit
+  int slot = -1;      // fit stream slot while running
   // predict
   long long model = 0;
   int rows = 0, prefix = 0;
@@ -92,7 +102,7 @@ class Server {
  public:
   Server(double budget, int workers, int max_rows, int max_fits)
       : budget_(size_t(budget)), max_rows_(max_rows),
-        max_fits_(max_fits > 0 ? max_fits : std::max(1, workers / 2)) {
+        max_fits_(max_fits > 0 ? std::min(max_fits, kMaxFitStreams) : 2) {
     if (!(budget > 0.0)) throw std::invalid_argument("server HBM budget must be positive");
     if (workers < 1 || workers > 64) throw std::invalid_argument("server workers must lie in 1..64");
     if (max_rows < 1) throw std::invalid_argument("server batch row cap must be positive");
@@ -100,6 +110,21 @@ class Server {
     int device = 0;
     FPLS_HIP(hipGetDevice(&device));
     device_ = device;
+    // fit and query streams together stay within the hardware queues HIP
+    // gives the process (GPU_MAX_HW_QUEUES, 4 unless set)
+    const int hwq = hardware_queues();
+    const int queries = std::max(1, std::min({kMaxQueryStreams, workers - 1, hwq - 1}));
+    max_fits_ = std::max(1, std::min(max_fits_, hwq - queries));
+    try {
+      for (int i = 0; i < max_fits_; ++i) {
+        fit_streams_.push_back(stream_take());
+        free_slots_.push_back(i);
+      }
+      for (int i = 0; i < queries; ++i) query_streams_.push_back(stream_take());
+    } catch (...) {
+      give_streams();
+      throw;
+    }
     for (int w = 0; w < workers; ++w) threads_.emplace_back([this, w] { worker(w); });
   }
 
@@ -110,6 +135,8 @@ class Server {
     }
     cv_.notify_all();
     for (auto& t : threads_) t.join();
+    models_.clear();   // resident handles drain their (server) streams
+    give_streams();
   }
 
   long long submit_fit(int method, const void* x, const void* y, const int* labels, int precision,
@@ -244,6 +271,22 @@ class Server {
     return id;
   }
 
+  static constexpr int kMaxFitStreams = 4;
+  static constexpr int kMaxQueryStreams = 2;
+
+  static int hardware_queues() {
+    const char* v = std::getenv("GPU_MAX_HW_QUEUES");
+    const int n = v ? std::atoi(v) : 0;
+    return n > 1 ? n : 4;
+  }
+
+  void give_streams() noexcept {
+    for (hipStream_t st : fit_streams_) stream_give(st);
+    for (hipStream_t st : query_streams_) stream_give(st);
+    fit_streams_.clear();
+    query_streams_.clear();
+  }
+
   // Under mu_: the next runnable work, FIFO with bounded bypass. Predicts on
   // an unknown model fail here.
   bool pick(std::vector<Request*>& work) {
@@ -254,6 +297,8 @@ class Server {
         if (running_fits_ >= max_fits_ || in_use_ + r->bytes > budget_) continue;
         in_use_ += r->bytes;
         ++running_fits_;
+        r->slot = free_slots_.back();
+        free_slots_.pop_back();
         work.push_back(r);
         queue_.erase(queue_.begin() + long(i));
         return true;
@@ -303,10 +348,13 @@ class Server {
 
   void worker(int w) {
     (void)hipSetDevice(device_);
-    hipStream_t ws = nullptr;    // staging copies of this worker
-    if (hipStreamCreateWithFlags(&ws, hipStreamNonBlocking) != hipSuccess) ws = nullptr;
+    // this worker's predict batches run on one query stream, where its
+    // staging buffers live too
+    hipStream_t qs = query_streams_[size_t(w) % query_streams_.size()];
     Buf staging;                 // concatenated predict rows on the device
     Buf upload;                  // one request's host rows
+    staging.bind(qs);
+    upload.bind(qs);
     std::vector<char> host_out;  // concatenated predictions
     for (;;) {
       std::vector<Request*> work;
@@ -315,7 +363,8 @@ class Server {
         cv_.wait(lock, [&] { return stop_ || pick(work); });
         if (work.empty()) {  // stopping
           lock.unlock();
-          if (ws) (void)hipStreamDestroy(ws);
+          staging.release();
+          upload.release();
           return;
         }
         const Clock::time_point now = Clock::now();
@@ -325,8 +374,8 @@ class Server {
         peak_in_use_ = std::max(peak_in_use_, in_use_);
       }
       done_cv_.notify_all();
-      if (work[0]->kind == kFitRequest) run_fit(work[0]);
-      else run_predict(work, ws, staging, upload, host_out);
+      if (work[0]->kind == kFitRequest) run_fit(work[0], qs);
+      else run_predict(work, qs, staging, upload, host_out);
       {
         std::lock_guard<std::mutex> lock(mu_);
         --running_;
@@ -336,15 +385,18 @@ class Server {
     }
   }
 
-  void run_fit(Request* r) {
+  void run_fit(Request* r, hipStream_t qs) {
     std::unique_ptr<Handle> h;
     std::string error;
     size_t resident = 0;
     try {
       h.reset(create(r->method, r->x, r->y, r->labels, r->precision, r->device_inputs, r->n,
-                     r->p, r->q, r->components, r->scaling, r->oversample, r->power, r->seed));
+                     r->p, r->q, r->components, r->scaling, r->oversample, r->power, r->seed,
+                     fit_streams_[size_t(r->slot)]));
       h->compact(false);
       h->set_batch_invariant(true);
+      // off the fit slot's stream before the slot is handed on
+      h->attach_stream(qs);
       resident = h->device_bytes();
     } catch (const std::exception& e) {
       error = e.what();
@@ -353,6 +405,7 @@ class Server {
     std::lock_guard<std::mutex> lock(mu_);
     in_use_ -= r->bytes;
     --running_fits_;
+    free_slots_.push_back(r->slot);
     if (h) {
       Resident m;
       m.handle = std::move(h);
@@ -365,7 +418,7 @@ class Server {
     finish(r, error);
   }
 
-  void run_predict(std::vector<Request*>& work, hipStream_t ws, Buf& staging, Buf& upload,
+  void run_predict(std::vector<Request*>& work, hipStream_t qs, Buf& staging, Buf& upload,
                    std::vector<char>& host_out) {
     Handle* h = nullptr;
     int p = 0, q = 0, precision = 64;
@@ -380,7 +433,8 @@ class Server {
     for (Request* r : work) total += r->rows;
     std::string error;
     try {
-      if (!ws) throw std::runtime_error("server worker has no stream");
+      // the model is busy for this batch: no other work runs on it
+      h->attach_stream(qs);
       staging.reserve(s * (size_t)total * p);
       // column-major concatenation: request i fills rows [off_i, off_i + rows_i);
       // host rows go up in one contiguous copy, then a device block copy
@@ -392,19 +446,18 @@ class Server {
         if (!r->device_inputs) {
           upload.reserve(s * (size_t)r->ldx * p);
           FPLS_HIP(hipMemcpyAsync(upload.as<void>(), r->x, s * (size_t)r->ldx * p,
-                                  hipMemcpyHostToDevice, ws));
+                                  hipMemcpyHostToDevice, qs));
           src = upload.as<void>();
         }
         if (precision == 32)
           launch_copy_block<float>(static_cast<const float*>(src), lds, r->rows, p,
-                                   staging.as<float>() + off, total, ws);
+                                   staging.as<float>() + off, total, qs);
         else
           launch_copy_block<double>(static_cast<const double*>(src), lds, r->rows, p,
-                                    staging.as<double>() + off, total, ws);
-        if (!r->device_inputs) FPLS_HIP(hipStreamSynchronize(ws));  // upload is reused
+                                    staging.as<double>() + off, total, qs);
+        if (!r->device_inputs) FPLS_HIP(hipStreamSynchronize(qs));  // upload is reused
         off += r->rows;
       }
-      FPLS_HIP(hipStreamSynchronize(ws));
       host_out.resize(s * (size_t)total * q);
       const int prefix = work[0]->prefix;
       h->predict_path(staging.as<void>(), total, &prefix, 1, false, host_out.data(), true);
@@ -436,6 +489,8 @@ class Server {
   int max_rows_ = 0;
   int max_fits_ = 1;
   int running_fits_ = 0;
+  std::vector<hipStream_t> fit_streams_, query_streams_;
+  std::vector<int> free_slots_;   // fit stream slots not in use
   int device_ = 0;
   long long next_id_ = 1;
   bool stop_ = false;

@@ -178,8 +178,9 @@ int fastpls_hip_fit_phases(void* model, double* out, int count, char* error,
  * seed + a as in simpls.hpp:555-556). Every phase of every fit is one launch
  * per step; the call returns when all fits are done. Arrays of pointers are
  * host arrays; y_device / labels_device / perm_device may be NULL as a whole.
- * reorth: -1 follows the reference routing (two Gram-Schmidt passes of the
- * loadings for precision 32, one for 64; src/r_api.cpp:1416), 0 / 1 force
+ * reorth: -1 follows the FP64 routing (one Gram-Schmidt pass of the
+ * loadings; precision 32 computes in FP64 too, so the reference's doubled
+ * passes for its float core, src/r_api.cpp:1416, do not apply), 0 / 1 force
  * one / two passes. */
 void* fastpls_hip_simpls_batch_create(int precision, int group_count, const void* const* x_device,
                                       const int* n, const int* p, int fit_count,
@@ -272,7 +273,12 @@ int fastpls_hip_plssvd_cv_classification(const void* predictors, const int* labe
  * continuous batching of fit and predict requests on one GPU under an HBM
  * budget (bytes), with `workers` worker threads, at most max_batch_rows
  * rows per batched predict and at most max_concurrent_fits fits running at
- * once (<= 0: half the workers), so predicts always find workers. Fits are
+ * once (<= 0: two; at most four), so predicts always find workers. Fits run
+ * on one stream per fit slot and predict batches on two query streams,
+ * whatever the number of resident models; fit slots and query streams
+ * together stay within the process's hardware queues (GPU_MAX_HW_QUEUES, 4
+ * unless set), so a predict never queues behind a fit on a shared hardware
+ * queue (the fit cap is lowered to fit them). Fits are
  * admitted when their device-byte estimate fits the budget left by running
  * fits and resident models; a fit
  * whose estimate exceeds the whole budget is rejected at submission. Fitted
@@ -281,9 +287,7 @@ int fastpls_hip_plssvd_cv_classification(const void* predictors, const int* labe
  * product over their concatenated rows; results are bitwise the same alone
  * and in a mix. Inputs and outputs must stay valid until the request is
  * polled as finished; a finished request can be polled once. Destroying the
- * server waits for running work and drops queued requests. Serving
- * processes should allow at least as many hardware queues as they run
- * concurrent batches (environment GPU_MAX_HW_QUEUES, read when HIP starts). */
+ * server waits for running work and drops queued requests. */
 void* fastpls_hip_server_create(double hbm_budget_bytes, int workers, int max_batch_rows,
                                 int max_concurrent_fits, char* error, size_t error_capacity);
 /* Returns the request id (> 0), 0 on error. Arguments as fastpls_hip_fit_device
