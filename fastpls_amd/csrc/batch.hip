@@ -77,6 +77,11 @@ class SimplsBatch {
   // limits, kXtwSplitRows and kXwSplitCols)
   static constexpr long long kStepRows = 2048;
   static constexpr long long kStepCols = 1024;
+  // row splits of the loading pass of groups whose per-step passes run on
+  // the FP64 matrix cores (shared-X refits): those work-groups are
+  // compute-bound, so more, shorter splits keep more of them in flight (the
+  // score pass measured no better with shorter column splits)
+  static constexpr long long kWideStepRows = 512;
 
  public:
   SimplsBatch(const std::vector<GroupIn>& groups, const std::vector<FitIn>& fits, int scaling,
@@ -326,7 +331,8 @@ class SimplsBatch {
       // a launch over groups of very different shapes has no long tail
       s = std::max<long long>(s, (G.n + kStepRows - 1) / kStepRows);
       G.splits = int(std::min<long long>(s, 4096));
-      G.ls_splits = int(std::min<long long>(4096, (G.n + kStepRows - 1) / kStepRows));
+      const long long rows = G.bcat >= kMfmaMinN ? kWideStepRows : kStepRows;
+      G.ls_splits = int(std::min<long long>(4096, (G.n + rows - 1) / rows));
       G.xs_splits = int(std::min<long long>(4096, (G.p + kStepCols - 1) / kStepCols));
     }
     // arena layout; the status words (zero at upload) lead the uploaded arena
@@ -611,14 +617,18 @@ class SimplsBatch {
         t.out = G.spart + (size_t)c0 * G.p; t.ldo = G.p; t.slab = (long long)G.p * G.qcat;
         t.stats = c0 == 0 ? G.stats : nullptr;
         t.frag = take_frag(g, xtw_frag_bytes<T>(G.n, maxn_pre, kF64));
-        pre_[width_class(nb)].push_back(t);
+        // several response columns ride the FP64 matrix cores even below
+        // kMfmaMinN (the VALU kernel for 2..7 columns is LDS-bound); one
+        // column has its own VALU kernel
+        pre_[nb <= 1 ? 0 : std::max(1, width_class(nb))].push_back(t);
       }
     }
     for (int c = 0; c < kWidthClasses; ++c) {
       if (pre_[c].empty()) continue;
       for (const XtwTask<T>& t : pre_[c])
         pre_bytes_[c] += double(t.m) * t.k * sizeof(T) + 8.0 * t.m * t.N + 8.0 * t.k * t.N;
-      pre_tiles_[c] = pack_xtw(pre_[c].data(), int(pre_[c].size()), &pre_maxn_[c], kF64);
+      pre_tiles_[c] = pack_xtw(pre_[c].data(), int(pre_[c].size()), &pre_maxn_[c], kF64,
+                               c == 0 ? 0 : kMfmaMinN);
       pre_dev_[c].alloc(sizeof(XtwTask<T>) * pre_[c].size());
       FPLS_HIP(hipMemcpy(pre_dev_[c].as<XtwTask<T>>(), pre_[c].data(),
                          sizeof(XtwTask<T>) * pre_[c].size(), hipMemcpyHostToDevice));
@@ -649,7 +659,7 @@ class SimplsBatch {
       if (!pre_[c].empty())
         timed(0, pre_bytes_[c], [&] {
           launch_xtw<T>(pre_dev_[c].as<XtwTask<T>>(), int(pre_[c].size()), pre_tiles_[c],
-                        pre_maxn_[c], stream_, kF64);
+                        pre_maxn_[c], stream_, kF64, /*stats=*/true);
         });
     for (Group& G : gplan_)
       launch_finalize_crossprod<T>(G.X, G.n, G.p, G.qcat, scaling_, G.stats, G.spart, G.splits,

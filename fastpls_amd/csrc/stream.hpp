@@ -64,17 +64,20 @@ struct XtwTask {
 // in bf16x6 split precision (FP32 sums per 32- to 64-long step), true widens
 // A to FP64 as it is loaded and runs them on the FP64 matrix cores (FP64
 // products and sums throughout; no operand workspace). FP64 A always takes
-// the FP64 route.
+// the FP64 route. `min_width` makes pack_xtw plan the list as if its widest
+// task had at least that many right-hand columns (a list of narrow tasks can
+// so be sent to the matrix cores). `stats` tells launch_xtw that the list's
+// tasks are kShiftStats tasks (a list is all of them or none).
 template<class T>
 long long pack_xw(XwTask<T>* tasks, int count, int* maxn, bool f64 = false);
 template<class T>
-long long pack_xtw(XtwTask<T>* tasks, int count, int* maxn, bool f64 = false);
+long long pack_xtw(XtwTask<T>* tasks, int count, int* maxn, bool f64 = false, int min_width = 0);
 template<class T>
 void launch_xw(const XwTask<T>* device_tasks, int count, long long tiles, int maxn,
                hipStream_t stream, bool f64 = false);
 template<class T>
 void launch_xtw(const XtwTask<T>* device_tasks, int count, long long tiles, int maxn,
-                hipStream_t stream, bool f64 = false);
+                hipStream_t stream, bool f64 = false, bool stats = false);
 // One task passed by value as a kernel argument (no device task array).
 template<class T>
 void launch_xw_single(XwTask<T> task, hipStream_t stream, bool f64 = false);

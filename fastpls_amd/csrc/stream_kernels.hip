@@ -41,6 +41,14 @@ __device__ __forceinline__ void load4_wide(const double FPLS_GLOBAL* p, double* 
   const double2_ea b = *(const double2_ea FPLS_GLOBAL*)(p + 2);
   out[0] = a.x; out[1] = a.y; out[2] = b.x; out[3] = b.y;
 }
+// The same four elements kept in their own type (widened where they are used).
+__device__ __forceinline__ void load4_raw(const float FPLS_GLOBAL* p, float* out) {
+  const float4_ea a = *(const float4_ea FPLS_GLOBAL*)p;
+  out[0] = a.x; out[1] = a.y; out[2] = a.z; out[3] = a.w;
+}
+__device__ __forceinline__ void load4_raw(const double FPLS_GLOBAL* p, double* out) {
+  load4_wide(p, out);
+}
 
 // The task of this work-group: the single by-value task, or a binary search
 // of the flattened tile space of a device task list.
@@ -265,6 +273,108 @@ __global__ __launch_bounds__(kWG) void k_xtw(const XtwTask<T>* tasks, int count,
         double FPLS_GLOBAL* st = gpw(t.stats) + (long long)split * 2 * t.k;
         st[col] = a1;
         st[t.k + col] = a2;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------- one right-hand column
+// xtw with N = 1 (a singleton fit's loading pass, or the statistics and
+// cross-product of a one-response fit). The LDS-staged k_xtw above widens
+// every element into LDS and reads it back once per output; with a single
+// right-hand column that LDS traffic, not HBM, bounds it. Here a work-group
+// of eight waves owns 64 columns (eight per wave) and one row split: a lane
+// owns four consecutive rows, reads them from each of its wave's eight
+// columns with one 16-byte load per column (all eight in flight together)
+// and the four right-hand values once, and keeps eight running dot products
+// (and, with kStats, the shifted sums) that the wave reduces at the end.
+// Columns past k read column k - 1 and are not written, so the loads stay
+// unconditional.
+template<class T, bool kStats>
+__global__ __launch_bounds__(512) void k_xtw1(const XtwTask<T>* tasks, int count,
+                                              XtwTask<T> single) {
+  constexpr int kCols = 8;  // columns per wave
+  const XtwTask<T> t = load_task(tasks, count, single);
+  const long long local = (long long)blockIdx.x - t.tile0;
+  const long long ctiles = xtw_col_tiles(t.k);
+  if (local >= ctiles * t.splits) return;
+  const int split = int(local / ctiles);
+  const long long col0 = (local % ctiles) * 64;
+  const long long per = (t.m + t.splits - 1) / t.splits;
+  const long long r0 = split * per;
+  const long long r1 = r0 + per < t.m ? r0 + per : t.m;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long jw = col0 + wave * kCols;  // this wave's first column
+  if (jw >= t.k) return;                     // no barrier below
+  const T FPLS_GLOBAL* A = gp(t.A);
+  const double FPLS_GLOBAL* W = gp(t.W);
+
+  const T FPLS_GLOBAL* col[kCols];
+  double cv[kCols], acc[kCols], s1[kCols], s2[kCols];
+#pragma unroll
+  for (int u = 0; u < kCols; ++u) {
+    const long long j = jw + u < t.k ? jw + u : t.k - 1;
+    col[u] = A + j * t.m;
+    double c = 0.0;
+    if (!kStats && t.mode == kCentered && t.center != nullptr) c = gp(t.center)[j];
+    if (kStats) c = double(col[u][0]);
+    cv[u] = c; acc[u] = 0.0; s1[u] = 0.0; s2[u] = 0.0;
+  }
+  for (long long i0 = r0; i0 < r1; i0 += 256) {
+    const long long row = i0 + 4 * lane;
+    if (row + 3 < r1) {
+      double w[4];
+      load4_wide(W + row, w);
+      T x[kCols][4];
+#pragma unroll
+      for (int u = 0; u < kCols; ++u) load4_raw(col[u] + row, x[u]);
+#pragma unroll
+      for (int u = 0; u < kCols; ++u)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const double d = double(x[u][e]) - cv[u];
+          acc[u] = fma(d, w[e], acc[u]);
+          if (kStats) {
+            s1[u] += d;
+            s2[u] = fma(d, d, s2[u]);
+          }
+        }
+    } else if (row < r1) {
+      // the split's last rows: rows past r1 contribute nothing
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if (row + e >= r1) break;
+        const double w = W[row + e];
+#pragma unroll
+        for (int u = 0; u < kCols; ++u) {
+          const double d = double(col[u][row + e]) - cv[u];
+          acc[u] = fma(d, w, acc[u]);
+          if (kStats) {
+            s1[u] += d;
+            s2[u] = fma(d, d, s2[u]);
+          }
+        }
+      }
+    }
+  }
+  double FPLS_GLOBAL* o = gpw(t.out) + (long long)split * t.slab;
+#pragma unroll
+  for (int u = 0; u < kCols; ++u) {
+    double a = acc[u], b1 = s1[u], b2 = s2[u];
+    for (int off = 32; off > 0; off >>= 1) {
+      a += __shfl_xor(a, off, 64);
+      if (kStats) {
+        b1 += __shfl_xor(b1, off, 64);
+        b2 += __shfl_xor(b2, off, 64);
+      }
+    }
+    const long long j = jw + u;
+    if (lane == 0 && j < t.k) {
+      o[j] = a;
+      if (kStats && t.stats != nullptr) {
+        double FPLS_GLOBAL* st = gpw(t.stats) + (long long)split * 2 * t.k;
+        st[j] = b1;
+        st[t.k + j] = b2;
       }
     }
   }
@@ -1129,7 +1239,7 @@ void dispatch_xw(const XwTask<T>* d, int count, const XwTask<T>& single, long lo
 
 template<class T>
 void dispatch_xtw(const XtwTask<T>* d, int count, const XtwTask<T>& single, long long tiles,
-                  int maxn, hipStream_t stream, bool f64) {
+                  int maxn, hipStream_t stream, bool f64, bool stats) {
   if (tiles <= 0) return;
   if (tiles > 0x7fffffffLL) throw std::invalid_argument("xtw grid too large");
   const dim3 grid((unsigned)tiles), block((unsigned)kWG);
@@ -1149,6 +1259,14 @@ void dispatch_xtw(const XtwTask<T>* d, int count, const XtwTask<T>& single, long
       hipLaunchKernelGGL((k_xtw_f64<NB, T>), grid, block, 0, stream, d, count, single);
       FPLS_LAUNCHED("k_xtw_f64");
     });
+    return;
+  }
+  if (maxn <= 1) {
+    // the statistics variant carries three accumulators per column; a task
+    // list is all statistics tasks or none (pack_xtw checks)
+    if (stats) hipLaunchKernelGGL((k_xtw1<T, true>), grid, dim3(512), 0, stream, d, count, single);
+    else hipLaunchKernelGGL((k_xtw1<T, false>), grid, dim3(512), 0, stream, d, count, single);
+    FPLS_LAUNCHED("k_xtw1");
     return;
   }
   if (maxn <= 4) hipLaunchKernelGGL((k_xtw<T, 4>), grid, block, 0, stream, d, count, single);
@@ -1178,10 +1296,12 @@ long long pack_xw(XwTask<T>* tasks, int count, int* maxn, bool f64) {
 }
 
 template<class T>
-long long pack_xtw(XtwTask<T>* tasks, int count, int* maxn, bool f64) {
-  int mn = 0;
+long long pack_xtw(XtwTask<T>* tasks, int count, int* maxn, bool f64, int min_width) {
+  int mn = min_width;
   for (int i = 0; i < count; ++i) {
     if (tasks[i].N < 1 || tasks[i].N > 64) throw std::invalid_argument("xtw supports 1..64 right-hand columns");
+    if ((tasks[i].mode == kShiftStats) != (tasks[0].mode == kShiftStats))
+      throw std::invalid_argument("an xtw task list mixes statistics and plain tasks");
     if (tasks[i].splits < 1) throw std::invalid_argument("xtw needs at least one split");
     mn = std::max(mn, tasks[i].N);
   }
@@ -1208,10 +1328,10 @@ void launch_xw(const XwTask<T>* device_tasks, int count, long long tiles, int ma
 
 template<class T>
 void launch_xtw(const XtwTask<T>* device_tasks, int count, long long tiles, int maxn,
-                hipStream_t stream, bool f64) {
+                hipStream_t stream, bool f64, bool stats) {
   if (count <= 0) return;
   if (count > 65535) throw std::invalid_argument("xtw task list longer than 65535");
-  dispatch_xtw<T>(device_tasks, count, XtwTask<T>{}, tiles, maxn, stream, f64);
+  dispatch_xtw<T>(device_tasks, count, XtwTask<T>{}, tiles, maxn, stream, f64, stats);
 }
 
 template<class T>
@@ -1235,7 +1355,7 @@ template<class T>
 void launch_xtw_single(XtwTask<T> task, hipStream_t stream, bool f64) {
   int maxn = 0;
   const long long tiles = pack_xtw(&task, 1, &maxn, f64);
-  dispatch_xtw<T>(nullptr, 1, task, tiles, maxn, stream, f64);
+  dispatch_xtw<T>(nullptr, 1, task, tiles, maxn, stream, f64, task.mode == kShiftStats);
 }
 
 void launch_sum_slabs(const double* part, int splits, long long stride,
@@ -1261,9 +1381,9 @@ void launch_sum_slabs_list(const SlabSum* device_list, int count, long long max_
 
 #define FPLS_STREAM_INSTANCES(T)                                                          \
   template long long pack_xw<T>(XwTask<T>*, int, int*, bool);                              \
-  template long long pack_xtw<T>(XtwTask<T>*, int, int*, bool);                            \
+  template long long pack_xtw<T>(XtwTask<T>*, int, int*, bool, int);                       \
   template void launch_xw<T>(const XwTask<T>*, int, long long, int, hipStream_t, bool);   \
-  template void launch_xtw<T>(const XtwTask<T>*, int, long long, int, hipStream_t, bool);  \
+  template void launch_xtw<T>(const XtwTask<T>*, int, long long, int, hipStream_t, bool, bool); \
   template void launch_xw_single<T>(XwTask<T>, hipStream_t, bool);                         \
   template void launch_xtw_single<T>(XtwTask<T>, hipStream_t, bool);
 FPLS_STREAM_INSTANCES(float)
