@@ -167,9 +167,9 @@ C3_WORKLOAD = ("batched_simpls 256 fits: 2 x (5-fold CV x 20 response-row permut
 # phase -> kernel of that phase in the 256-fit batch (the narrow groups have
 # one right-hand column, the shared-X groups 20)
 C3_KERNELS = {"crossprod": "k_xtw / k_xtw_f64 (column statistics + cross-products, FP64 products)",
-              "score_valu": "k_xw1<float> (score pass t = X_s r of one-column fits, FP64 VALU)",
+              "score_valu": "k_xw<float,1> (score pass t = X_s r of one-column fits, FP64 VALU)",
               "score_mfma": "k_xw_f64<32,float> (score pass of shared-X groups, FP64 MFMA)",
-              "loading_valu": "k_xtw1<float> (loading pass w = X_s' t of one-column fits, FP64 VALU)",
+              "loading_valu": "k_xtw1<float,false> (loading pass w = X_s' t of one-column fits, FP64 VALU)",
               "loading_mfma": "k_xtw_f64<32,float> (loading pass of shared-X groups, FP64 MFMA)"}
 
 

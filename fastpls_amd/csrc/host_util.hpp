@@ -30,6 +30,12 @@ void pool_forget_stream(hipStream_t stream) noexcept;
 // stream and keeps it for the next owner.
 hipStream_t stream_take();
 void stream_give(hipStream_t stream) noexcept;
+// A new stream, never one from the cache (the runtime binds a stream to one
+// of its hardware queues when the stream is created, and a cached stream
+// keeps the queue it got then), and the destruction of every idle cached
+// stream.
+hipStream_t stream_create();
+void stream_cache_drop() noexcept;
 
 // Device buffer. Unbound buffers own their memory directly (hipMalloc /
 // hipFree); buffers bound to a stream take it from the block cache and give

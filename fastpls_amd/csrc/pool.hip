@@ -175,9 +175,24 @@ hipStream_t stream_take() {
       return s;
     }
   }
+  return stream_create();
+}
+
+hipStream_t stream_create() {
   hipStream_t s = nullptr;
   FPLS_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   return s;
+}
+
+void stream_cache_drop() noexcept {
+  StreamCache& C = streams();
+  std::vector<hipStream_t> idle;
+  {
+    std::lock_guard<std::mutex> lock(C.mu);
+    idle.swap(C.idle);
+  }
+  // drained and forgotten by the pool when they were given back
+  for (hipStream_t s : idle) (void)hipStreamDestroy(s);
 }
 
 void stream_give(hipStream_t stream) noexcept {

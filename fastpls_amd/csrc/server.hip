@@ -115,15 +115,33 @@ class Server {
     const int hwq = hardware_queues();
     const int queries = std::max(1, std::min({kMaxQueryStreams, workers - 1, hwq - 1}));
     max_fits_ = std::max(1, std::min(max_fits_, hwq - queries));
+    // Each stream is bound to a hardware queue when it is created: once the
+    // GPU_MAX_HW_QUEUES queues exist, to the one the fewest live streams use
+    // (the first such queue on ties). Two streams on one queue run in order,
+    // so a query stream that shared a queue with a fit stream would wait
+    // behind the fit's kernels. Other live streams of the process (idle
+    // cached ones, those of open handles) leave the queues unevenly used, so
+    // the server creates a run of streams: the first ones even the usage
+    // out, after which consecutive streams land on consecutive queues, and
+    // the server keeps the last ones (which then share no queue) and
+    // destroys the rest, which never ran work. Idle cached streams are
+    // destroyed first, so the run is short.
+    stream_cache_drop();
+    const int keep = queries + max_fits_;
+    const int run = std::max(keep, 8 * hwq);
+    std::vector<hipStream_t> made;
     try {
-      for (int i = 0; i < max_fits_; ++i) {
-        fit_streams_.push_back(stream_take());
-        free_slots_.push_back(i);
-      }
-      for (int i = 0; i < queries; ++i) query_streams_.push_back(stream_take());
+      for (int i = 0; i < run; ++i) made.push_back(stream_create());
     } catch (...) {
-      give_streams();
+      for (hipStream_t st : made) (void)hipStreamDestroy(st);
       throw;
+    }
+    for (int i = 0; i < run - keep; ++i) (void)hipStreamDestroy(made[size_t(i)]);
+    made.erase(made.begin(), made.begin() + (run - keep));
+    for (int i = 0; i < queries; ++i) query_streams_.push_back(made[size_t(i)]);
+    for (int i = 0; i < max_fits_; ++i) {
+      fit_streams_.push_back(made[size_t(queries + i)]);
+      free_slots_.push_back(i);
     }
     for (int w = 0; w < workers; ++w) threads_.emplace_back([this, w] { worker(w); });
   }
