@@ -121,20 +121,19 @@ __device__ __attribute__((noinline)) void gemm_tn_direct(int m, int n, long k, c
                                                          double* C, long ldc, double* red) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int m16 = lane & 15, g = lane >> 4;
+  // columns past m (n) read column 0: output (i, j) depends on columns i
+  // and j only, and entries past m x n are not written
   const double FPLS_GLOBAL* pa[TM];
   const double FPLS_GLOBAL* pb[TN];
-  bool va[TM], vb[TN];
 #pragma unroll
   for (int t = 0; t < TM; ++t) {
     const int c = t * 16 + m16;
-    va[t] = c < m;
-    pa[t] = (const double FPLS_GLOBAL*)(A + (long)(va[t] ? c : 0) * lda);
+    pa[t] = (const double FPLS_GLOBAL*)(A + (long)(c < m ? c : 0) * lda);
   }
 #pragma unroll
   for (int t = 0; t < TN; ++t) {
     const int c = t * 16 + m16;
-    vb[t] = c < n;
-    pb[t] = (const double FPLS_GLOBAL*)(B + (long)(vb[t] ? c : 0) * ldb);
+    pb[t] = (const double FPLS_GLOBAL*)(B + (long)(c < n ? c : 0) * ldb);
   }
   wg_f64x4 acc[TM][TN];
 #pragma unroll
@@ -150,7 +149,24 @@ __device__ __attribute__((noinline)) void gemm_tn_direct(int m, int n, long k, c
   const long sb = wave * quarter;
   const long se = sb + quarter < nsteps ? sb + quarter : nsteps;
   double av[2][U][TM], bv[2][U][SYM ? 1 : TN];
+  // A block of U steps whose rows all lie in the wave's range and in k is
+  // loaded without conditions, so its loads stay in flight together while
+  // the other set multiplies; only a wave's last block masks rows (a masked
+  // load is waited for where it is selected).
   auto load = [&](int set, long s0) {
+    if (s0 + U <= se && 4 * (s0 + U) <= k) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const long row = 4 * (s0 + u) + g;
+#pragma unroll
+        for (int t = 0; t < TM; ++t) av[set][u][t] = pa[t][row];
+        if constexpr (!SYM) {
+#pragma unroll
+          for (int t = 0; t < TN; ++t) bv[set][u][t] = pb[t][row];
+        }
+      }
+      return;
+    }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const long row = 4 * (s0 + u) + g;
@@ -158,16 +174,14 @@ __device__ __attribute__((noinline)) void gemm_tn_direct(int m, int n, long k, c
       const long rr = ok ? row : 0;
 #pragma unroll
       for (int t = 0; t < TM; ++t) {
-        double v = pa[t][rr];
-        asm volatile("" : "+v"(v));
-        av[set][u][t] = (ok && va[t]) ? v : 0.0;
+        const double v = pa[t][rr];
+        av[set][u][t] = ok ? v : 0.0;
       }
       if constexpr (!SYM) {
 #pragma unroll
         for (int t = 0; t < TN; ++t) {
-          double v = pb[t][rr];
-          asm volatile("" : "+v"(v));
-          bv[set][u][t] = (ok && vb[t]) ? v : 0.0;
+          const double v = pb[t][rr];
+          bv[set][u][t] = ok ? v : 0.0;
         }
       }
     }
