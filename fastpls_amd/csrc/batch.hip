@@ -123,7 +123,52 @@ class SimplsBatch {
   }
 
   ~SimplsBatch() {
+    if (graph_) (void)hipGraphExecDestroy(graph_);
     if (stream_) stream_give(stream_);
+  }
+
+  // Executes every fit again on the current contents of the caller's device
+  // inputs (X, responses or labels, permutations: same pointers and shapes),
+  // as a permutation test refits with new permutations. The first call
+  // captures run() into a graph on the batch's stream; every call launches
+  // that graph, so a repeated execution costs one graph launch and no
+  // planning, upload or per-kernel launch on the host. With phase timing on,
+  // run() is launched eagerly instead (events are not captured).
+  void rerun() {
+    Event start, stop;
+    if (timing_) {
+      FPLS_HIP(hipEventRecord(start.e, stream_));
+      run();
+    } else {
+      if (!graph_) {
+        hipGraph_t g = nullptr;
+        FPLS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        try {
+          run();
+        } catch (...) {
+          hipGraph_t partial = nullptr;
+          (void)hipStreamEndCapture(stream_, &partial);
+          if (partial) (void)hipGraphDestroy(partial);
+          throw;
+        }
+        FPLS_HIP(hipStreamEndCapture(stream_, &g));
+        const hipError_t st = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        FPLS_HIP(st);
+      }
+      FPLS_HIP(hipEventRecord(start.e, stream_));
+      FPLS_HIP(hipGraphLaunch(graph_, stream_));
+    }
+    FPLS_HIP(hipEventRecord(stop.e, stream_));
+    FPLS_HIP(hipEventSynchronize(stop.e));
+    collect();
+    float ms = 0.f;
+    FPLS_HIP(hipEventElapsedTime(&ms, start.e, stop.e));
+    seconds_ = double(ms) * 1e-3;
+    if (timing_) {
+      for (auto& ph : phase_) ph[0] = ph[1] = ph[2] = 0.0;
+      collect_phases();
+    }
   }
 
   int count() const { return int(fits_.size()); }
@@ -636,9 +681,38 @@ class SimplsBatch {
       FPLS_HIP(hipMemcpy(pre_dev_[c].as<XtwTask<T>>(), pre_[c].data(),
                          sizeof(XtwTask<T>) * pre_[c].size(), hipMemcpyHostToDevice));
     }
+    // right Grams SᵀS of every fit that works in the response space: one
+    // task list, launched after the cross-products
+    std::vector<XtwTask<double>> gram;
+    for (size_t f = 0; f < plan_.size(); ++f) {
+      Fit& F = plan_[f];
+      if (!F.right_gram) continue;
+      Group& G = gplan_[fits_[f].group];
+      const double* S = G.S + (size_t)F.off * G.p;
+      for (int c0 = 0; c0 < F.q; c0 += 64) {
+        const int nb = std::min(64, F.q - c0);
+        XtwTask<double> t{};
+        t.A = S; t.m = G.p; t.k = F.q; t.W = S + (size_t)c0 * G.p; t.ldw = G.p; t.N = nb;
+        t.mode = kPlain; t.splits = 1; t.out = F.G + (size_t)c0 * F.q; t.ldo = F.q; t.slab = 0;
+        gram.push_back(t);
+      }
+    }
+    if (!gram.empty()) {
+      gram_count_ = int(gram.size());
+      gram_tiles_ = pack_xtw(gram.data(), gram_count_, &gram_maxn_);
+      gram_dev_.alloc(sizeof(XtwTask<double>) * gram.size());
+      FPLS_HIP(hipMemcpy(gram_dev_.as<XtwTask<double>>(), gram.data(),
+                         sizeof(XtwTask<double>) * gram.size(), hipMemcpyHostToDevice));
+    }
   }
 
+  // The whole execution as launches on stream_ (no host synchronisation, no
+  // memory set or copy), so it can be captured into a graph and replayed.
   void run() {
+    // group flags and fit states start at zero (a replay clears what the
+    // previous execution left; a fill kernel, since memset nodes are kept
+    // out of graphs)
+    launch_fill_int(status_base_, int(4 * (gplan_.size() + plan_.size())), 0, stream_);
     // responses of every fit into its columns of the group's centred block
     for (size_t f = 0; f < plan_.size(); ++f) {
       Fit& F = plan_[f];
@@ -669,28 +743,9 @@ class SimplsBatch {
                                    G.ysum, G.mx, G.sx, G.colss, G.S, G.S0, G.flags, stream_,
                                    status_base_, G.col_flags);
     // right Grams SᵀS of every fit: one launch
-    std::vector<XtwTask<double>> gram;
-    for (size_t f = 0; f < plan_.size(); ++f) {
-      Fit& F = plan_[f];
-      if (!F.right_gram) continue;
-      Group& G = gplan_[fits_[f].group];
-      const double* S = G.S + (size_t)F.off * G.p;
-      for (int c0 = 0; c0 < F.q; c0 += 64) {
-        const int nb = std::min(64, F.q - c0);
-        XtwTask<double> t{};
-        t.A = S; t.m = G.p; t.k = F.q; t.W = S + (size_t)c0 * G.p; t.ldw = G.p; t.N = nb;
-        t.mode = kPlain; t.splits = 1; t.out = F.G + (size_t)c0 * F.q; t.ldo = F.q; t.slab = 0;
-        gram.push_back(t);
-      }
-    }
-    Buf gram_dev;
-    if (!gram.empty()) {
-      int maxn = 0;
-      const long long gt = pack_xtw(gram.data(), int(gram.size()), &maxn);
-      gram_dev.alloc(sizeof(XtwTask<double>) * gram.size());
-      FPLS_HIP(hipMemcpy(gram_dev.as<XtwTask<double>>(), gram.data(), sizeof(XtwTask<double>) * gram.size(), hipMemcpyHostToDevice));
-      launch_xtw<double>(gram_dev.as<XtwTask<double>>(), int(gram.size()), gt, maxn, stream_);
-    }
+    if (gram_count_ > 0)
+      launch_xtw<double>(gram_dev_.as<XtwTask<double>>(), gram_count_, gram_tiles_, gram_maxn_,
+                         stream_);
     // component steps: refresh, score pass, loading pass, update over the
     // groups still active at the step (a prefix of the step order)
     const int nf = int(plan_.size());
@@ -800,6 +855,10 @@ class SimplsBatch {
   Buf xw_tasks_[kWidthClasses], xt_tasks_[kWidthClasses], sums_, proj_frag_;
   Buf pre_dev_[kWidthClasses];
   std::vector<XtwTask<T>> pre_[kWidthClasses];
+  Buf gram_dev_;
+  int gram_count_ = 0, gram_maxn_ = 0;
+  long long gram_tiles_ = 0;
+  hipGraphExec_t graph_ = nullptr;  // run() captured by the first rerun
   long long pre_tiles_[kWidthClasses] = {};
   int pre_maxn_[kWidthClasses] = {};
   double pre_bytes_[kWidthClasses] = {};
@@ -825,6 +884,7 @@ class BatchHandle {
   virtual int flags(int f) const = 0;
   virtual void export_field(int f, int field, void* out, size_t elements, bool f64) = 0;
   virtual void phases(double* out, int count) const = 0;
+  virtual void rerun() = 0;
 };
 
 template<class T>
@@ -839,6 +899,7 @@ class TypedBatch final : public BatchHandle {
     b_->export_field(f, field, out, elements, f64);
   }
   void phases(double* out, int count) const override { b_->phases(out, count); }
+  void rerun() override { b_->rerun(); }
 
  private:
   std::unique_ptr<SimplsBatch<T>> b_;
@@ -1030,6 +1091,13 @@ int fastpls_hip_batch_export64(void* batch, int fit, int field, double* out, siz
   return fpls::guarded(error, error_capacity, [&] {
     if (!batch) throw std::invalid_argument("null batch");
     static_cast<BatchHandle*>(batch)->export_field(fit, field, out, elements, true);
+  });
+}
+
+int fastpls_hip_batch_rerun(void* batch, char* error, size_t error_capacity) {
+  return fpls::guarded(error, error_capacity, [&] {
+    if (!batch) throw std::invalid_argument("null batch");
+    static_cast<BatchHandle*>(batch)->rerun();
   });
 }
 

@@ -84,6 +84,7 @@ SIGNATURES = {
     "fastpls_hip_batch_export": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _sz, _cp, _sz]),
     "fastpls_hip_batch_phases": (C.c_int, [_vp, _vp, C.c_int, _cp, _sz]),
     "fastpls_hip_batch_export64": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _sz, _cp, _sz]),
+    "fastpls_hip_batch_rerun": (C.c_int, [_vp, _cp, _sz]),
     "fastpls_hip_batch_destroy": (None, [_vp]),
     "fastpls_hip_simpls_cv_regression": _CV_REGRESSION,
     "fastpls_hip_plssvd_cv_regression": _CV_REGRESSION,
@@ -486,6 +487,13 @@ class SimplsBatch:
             C.cast(pm, C.c_void_p) if pm is not None else None, qs, cs, sd, int(scaling),
             int(oversample), int(power), int(reorth), e, 1024)
         if not self._h:
+            _raise(e)
+
+    def rerun(self):
+        """Execute every fit again on the current contents of the device inputs
+        (fastpls_hip_batch_rerun: a captured graph of the whole batch)."""
+        e = _err()
+        if load_library().fastpls_hip_batch_rerun(self._h, e, 1024) != 0:
             _raise(e)
 
     def status(self):

@@ -214,6 +214,16 @@ int fastpls_hip_batch_export(void* batch, int fit, int field, void* out, size_t 
  * magnifies that rounding). */
 int fastpls_hip_batch_export64(void* batch, int fit, int field, double* out, size_t elements,
                                char* error, size_t error_capacity);
+/* Executes every fit of the batch again on the current contents of the
+ * device inputs it was created with (same pointers and shapes: new response
+ * permutations, responses or X values written in place), as a permutation
+ * test's rounds of refits (no reference counterpart: the reference refits
+ * permutations on the CPU, R/main.R:9419-9450). The first call captures the
+ * batch's whole launch sequence (statistics, cross-products, the component
+ * loop) into a HIP graph and every call replays it: no planning, upload or
+ * per-kernel launch on the host. Status, seconds and exports then describe
+ * the new execution; the rSVD sketches are those of the creation seeds. */
+int fastpls_hip_batch_rerun(void* batch, char* error, size_t error_capacity);
 void fastpls_hip_batch_destroy(void* batch);
 
 /* fastpls_resident_simpls_cv_regression (src/cuda_resident_api.h:20-26):
