@@ -230,6 +230,15 @@ def c3_batched(fp, steps, warmup, cpu, cpu_budget):
 
     for _ in range(warmup):
         run_batch(32, groups32, fits32)[0].close()
+    # split lengths chosen by the autotuner on the workload itself; the timed
+    # batches below are planned with its choice (the process default)
+    t0 = time.perf_counter()
+    tb, _, _ = run_batch(32, groups32, fits32)
+    tune = tb.autotune(trials=1)
+    tune_wall = time.perf_counter() - t0
+    tb.close()
+    log(f"[c3] autotune: splits {tune['splits']}, {1e3 * tune['start_seconds']:.1f} -> "
+        f"{1e3 * tune['seconds']:.1f} ms device, {tune['plans']} plans in {tune_wall:.1f} s")
     walls, devs = [], []
     b32 = None
     for s in range(steps):
@@ -332,6 +341,10 @@ def c3_batched(fp, steps, warmup, cpu, cpu_budget):
         "ms_per_step": round(1e3 * wall / steps, 3),
         "p50_ms": pct(walls, 50), "p99_ms": pct(walls, 99),
         "device_ms_per_step": round(1e3 * float(np.mean(devs)), 3),
+        "autotune": {"splits_step_rows_cols_wide_rows_cols": list(tune["splits"]),
+                     "tuned_device_ms": round(1e3 * tune["seconds"], 3),
+                     "default_device_ms": round(1e3 * tune["start_seconds"], 3),
+                     "plans_run": tune["plans"], "tuning_wall_s": round(tune_wall, 2)},
         "graph_replay_fits_per_s": round(256 * steps / float(np.sum(rwalls)), 2),
         "graph_replay_ms_per_step": round(1e3 * float(np.mean(rwalls)), 3),
         "graph_replay_device_ms": round(1e3 * float(np.mean(rdevs)), 3),

@@ -21,6 +21,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <thread>
 #include <vector>
@@ -59,6 +60,80 @@ struct FitIn {
   unsigned long long seed;
 };
 
+// Split lengths of the batched passes: the longest row split of the
+// cross-product and loading passes and the longest column split of the score
+// pass, for groups on the VALU and for groups whose per-step passes run on
+// the FP64 matrix cores (shared-X refits: those work-groups are compute-
+// bound, so shorter splits keep more of them in flight). The built-in values
+// were measured on the 256-fit workload (shared-X score pass 8.0 -> 4.5 ms,
+// loading pass 5.5 -> 4.0 ms per batch at 256 / 512 against 1024 / 2048);
+// fastpls_hip_batch_autotune replaces them with the fastest of a candidate
+// set on the caller's own batch. Every length lies in [kMinSplit, kMaxSplit]
+// (the upper bound is the matrix-core kernels' limit, kXtwSplitRows and
+// kXwSplitCols).
+struct BatchSplits {
+  long long step_rows = 2048, step_cols = 1024, wide_rows = 512, wide_cols = 256;
+  long long& at(int i) {
+    return i == 0 ? step_rows : i == 1 ? step_cols : i == 2 ? wide_rows : wide_cols;
+  }
+  long long get(int i) const { return const_cast<BatchSplits*>(this)->at(i); }
+};
+constexpr long long kMinSplit = 128;
+constexpr long long kMaxSplit = 4096;
+static_assert(kMaxSplit <= kXtwSplitRows && kMaxSplit <= kXwSplitCols, "split limit");
+constexpr const char* kSplitsTag = "fastpls-hip-batch-splits-v1";
+
+bool valid_splits(const long long* v) {
+  for (int i = 0; i < 4; ++i)
+    if (v[i] < kMinSplit || v[i] > kMaxSplit) return false;
+  return true;
+}
+
+std::mutex& splits_mutex() {
+  static std::mutex m;
+  return m;
+}
+
+// The process-wide default for new batches: the built-in lengths, or those of
+// the file named by FASTPLS_HIP_TUNE_FILE (as an autotune wrote it) when it
+// exists and holds valid lengths, until an autotune or a set replaces them.
+// Called with splits_mutex held.
+BatchSplits& splits_state() {
+  static BatchSplits state = [] {
+    BatchSplits d;
+    const char* path = std::getenv("FASTPLS_HIP_TUNE_FILE");
+    if (path == nullptr) return d;
+    FILE* f = std::fopen(path, "r");
+    if (f == nullptr) return d;
+    char tag[64] = {};
+    long long v[4] = {};
+    if (std::fscanf(f, "%63s %lld %lld %lld %lld", tag, &v[0], &v[1], &v[2], &v[3]) == 5 &&
+        std::strcmp(tag, kSplitsTag) == 0 && valid_splits(v))
+      for (int i = 0; i < 4; ++i) d.at(i) = v[i];
+    std::fclose(f);
+    return d;
+  }();
+  return state;
+}
+
+BatchSplits default_splits() {
+  std::lock_guard<std::mutex> lock(splits_mutex());
+  return splits_state();
+}
+
+// Replaces the default; with FASTPLS_HIP_TUNE_FILE set, also writes it there.
+void set_default_splits(const BatchSplits& b) {
+  std::lock_guard<std::mutex> lock(splits_mutex());
+  splits_state() = b;
+  const char* path = std::getenv("FASTPLS_HIP_TUNE_FILE");
+  if (path == nullptr) return;
+  FILE* f = std::fopen(path, "w");
+  if (f == nullptr) throw std::runtime_error("cannot write the split file named by FASTPLS_HIP_TUNE_FILE");
+  std::fprintf(f, "%s %lld %lld %lld %lld\n", kSplitsTag, b.step_rows, b.step_cols, b.wide_rows,
+               b.wide_cols);
+  std::fclose(f);
+}
+
 template<class T>
 class SimplsBatch {
   // Every product over X runs in FP64 (FP32-stored X is widened as it is
@@ -72,23 +147,12 @@ class SimplsBatch {
   // 1..4 = matrix-core launches compiled for 16, 32, 48, 64 columns.
   static constexpr int kWidthClasses = 5;
   static int width_class(int N) { return N < kMfmaMinN ? 0 : mfma_width(N) / 16; }
-  // longest row split of the cross-product and loading passes and longest
-  // column split of the score pass (both within the matrix-core kernels'
-  // limits, kXtwSplitRows and kXwSplitCols)
-  static constexpr long long kStepRows = 2048;
-  static constexpr long long kStepCols = 1024;
-  // the same for groups whose per-step passes run on the FP64 matrix cores
-  // (shared-X refits): those work-groups are compute-bound, so more, shorter
-  // splits keep more of them in flight (measured on the 256-fit set: score
-  // pass 8.0 -> 4.5 ms, loading pass 5.5 -> 4.0 ms per batch)
-  static constexpr long long kWideStepRows = 512;
-  static constexpr long long kWideStepCols = 256;
 
  public:
   SimplsBatch(const std::vector<GroupIn>& groups, const std::vector<FitIn>& fits, int scaling,
-              int oversample, int power, int reorth)
+              int oversample, int power, int reorth, const BatchSplits& splits = default_splits())
       : groups_(groups), fits_(fits), scaling_(scaling), oversample_(oversample), power_(power),
-        reorth_(reorth) {
+        reorth_(reorth), splits_(splits) {
     if (reorth < -1 || reorth > 1) throw std::invalid_argument("reorth must be -1, 0 or 1");
     if (groups_.empty() || fits_.empty()) throw std::invalid_argument("empty batch");
     if (scaling < 1 || scaling > 3) throw std::invalid_argument("scaling must be 1, 2 or 3");
@@ -173,6 +237,11 @@ class SimplsBatch {
 
   int count() const { return int(fits_.size()); }
   double seconds() const { return seconds_; }
+  const BatchSplits& splits() const { return splits_; }
+  // whether some group's per-step passes run on the FP64 matrix cores
+  // (wide_rows / wide_cols apply) and whether some run on the VALU
+  bool has_wide() const { return has_wide_; }
+  bool has_narrow() const { return has_narrow_; }
 
   // Phases of the batch: 0 column statistics + cross-products, 1 refresh,
   // 2 / 3 score passes (VALU / matrix-core launches), 4 score slab sums,
@@ -371,15 +440,17 @@ class SimplsBatch {
     for (Group& G : gplan_) {
       long long s = (2048 + tiles1 - 1) / std::max<long long>(tiles1, 1);
       s = std::max<long long>(1, std::min<long long>(s, G.n / 256));
-      // every work-group of a pass streams at most kStepRows rows (the
-      // cross-product and loading passes) or kStepCols columns (the score
-      // pass) of its group, so work-groups of all groups are alike in size and
-      // a launch over groups of very different shapes has no long tail
-      s = std::max<long long>(s, (G.n + kStepRows - 1) / kStepRows);
+      // every work-group of a pass streams at most splits_.step_rows rows
+      // (the cross-product and loading passes) or splits_.step_cols columns
+      // (the score pass) of its group, so work-groups of all groups are alike
+      // in size and a launch over groups of very different shapes has no
+      // long tail
+      s = std::max<long long>(s, (G.n + splits_.step_rows - 1) / splits_.step_rows);
       G.splits = int(std::min<long long>(s, 4096));
       const bool wide = G.bcat >= kMfmaMinN;
-      const long long rows = wide ? kWideStepRows : kStepRows;
-      const long long cols = wide ? kWideStepCols : kStepCols;
+      (wide ? has_wide_ : has_narrow_) = true;
+      const long long rows = wide ? splits_.wide_rows : splits_.step_rows;
+      const long long cols = wide ? splits_.wide_cols : splits_.step_cols;
       G.ls_splits = int(std::min<long long>(4096, (G.n + rows - 1) / rows));
       G.xs_splits = int(std::min<long long>(4096, (G.p + cols - 1) / cols));
     }
@@ -859,6 +930,8 @@ class SimplsBatch {
   int gram_count_ = 0, gram_maxn_ = 0;
   long long gram_tiles_ = 0;
   hipGraphExec_t graph_ = nullptr;  // run() captured by the first rerun
+  BatchSplits splits_;
+  bool has_wide_ = false, has_narrow_ = false;
   long long pre_tiles_[kWidthClasses] = {};
   int pre_maxn_[kWidthClasses] = {};
   double pre_bytes_[kWidthClasses] = {};
@@ -885,12 +958,18 @@ class BatchHandle {
   virtual void export_field(int f, int field, void* out, size_t elements, bool f64) = 0;
   virtual void phases(double* out, int count) const = 0;
   virtual void rerun() = 0;
+  virtual void autotune(int trials, double* report, int count) = 0;
+  virtual BatchSplits splits() const = 0;
 };
 
 template<class T>
 class TypedBatch final : public BatchHandle {
  public:
-  explicit TypedBatch(std::unique_ptr<SimplsBatch<T>> b) : b_(std::move(b)) {}
+  TypedBatch(const std::vector<GroupIn>& groups, const std::vector<FitIn>& fits, int scaling,
+             int oversample, int power, int reorth)
+      : groups_(groups), fits_(fits), scaling_(scaling), oversample_(oversample), power_(power),
+        reorth_(reorth),
+        b_(std::make_unique<SimplsBatch<T>>(groups, fits, scaling, oversample, power, reorth)) {}
   int count() const override { return b_->count(); }
   double seconds() const override { return b_->seconds(); }
   int completed(int f) const override { return b_->completed(f); }
@@ -900,8 +979,67 @@ class TypedBatch final : public BatchHandle {
   }
   void phases(double* out, int count) const override { b_->phases(out, count); }
   void rerun() override { b_->rerun(); }
+  BatchSplits splits() const override { return b_->splits(); }
+
+  // Chooses the split lengths for this batch by measurement: starting from
+  // the batch's own lengths, each length in turn is tried at 1/4, 1/2, 2 and
+  // 4 times its current value (within [kMinSplit, kMaxSplit]; the wide
+  // lengths only when some group runs on the matrix cores, the others only
+  // when some runs on the VALU), re-planning and running the whole batch
+  // `trials` times per candidate and keeping a candidate that lowers the
+  // fastest device time by more than 1%. The batch then holds the fits of the
+  // chosen plan (the same fits; split sums differ in rounding only) and the
+  // choice becomes the process default for new batches. report: [0..3] the
+  // chosen lengths (step_rows, step_cols, wide_rows, wide_cols), [4] the
+  // chosen plan's device seconds, [5] the starting plan's, [6] plans run.
+  void autotune(int trials, double* report, int count) override {
+    if (trials < 1 || trials > 16) throw std::invalid_argument("autotune trials must lie in 1..16");
+    int runs = 0;
+    auto measure = [&](const BatchSplits& cfg, std::unique_ptr<SimplsBatch<T>>& keep) {
+      double best = std::numeric_limits<double>::infinity();
+      for (int t = 0; t < trials; ++t) {
+        auto b = std::make_unique<SimplsBatch<T>>(groups_, fits_, scaling_, oversample_, power_,
+                                                  reorth_, cfg);
+        ++runs;
+        if (b->seconds() < best) {
+          best = b->seconds();
+          keep = std::move(b);
+        }
+      }
+      return best;
+    };
+    BatchSplits best = b_->splits();
+    std::unique_ptr<SimplsBatch<T>> best_batch;
+    double best_s = measure(best, best_batch);
+    const double start_s = best_s;
+    for (int i = 0; i < 4; ++i) {
+      if ((i < 2 && !best_batch->has_narrow()) || (i >= 2 && !best_batch->has_wide())) continue;
+      const long long base = best.get(i);
+      for (const double f : {0.25, 0.5, 2.0, 4.0}) {
+        const long long v = (long long)(double(base) * f);
+        if (v < kMinSplit || v > kMaxSplit) continue;
+        BatchSplits cfg = best;
+        cfg.at(i) = v;
+        std::unique_ptr<SimplsBatch<T>> cand;
+        const double t = measure(cfg, cand);
+        if (t < best_s * 0.99) {
+          best = cfg;
+          best_s = t;
+          best_batch = std::move(cand);
+        }
+      }
+    }
+    b_ = std::move(best_batch);
+    set_default_splits(best);
+    const double out[7] = {double(best.step_rows), double(best.step_cols), double(best.wide_rows),
+                           double(best.wide_cols), best_s, start_s, double(runs)};
+    for (int i = 0; i < count && i < 7; ++i) report[i] = out[i];
+  }
 
  private:
+  std::vector<GroupIn> groups_;
+  std::vector<FitIn> fits_;
+  int scaling_, oversample_, power_, reorth_;
   std::unique_ptr<SimplsBatch<T>> b_;
 };
 
@@ -1048,9 +1186,9 @@ void* fastpls_hip_simpls_batch_create(int precision, int group_count, const void
       fits[f].seed = seeds[f];
     }
     if (precision == 32)
-      h = new fpls::TypedBatch<float>(std::make_unique<fpls::SimplsBatch<float>>(groups, fits, scaling, oversample, power, reorth));
+      h = new fpls::TypedBatch<float>(groups, fits, scaling, oversample, power, reorth);
     else if (precision == 64)
-      h = new fpls::TypedBatch<double>(std::make_unique<fpls::SimplsBatch<double>>(groups, fits, scaling, oversample, power, reorth));
+      h = new fpls::TypedBatch<double>(groups, fits, scaling, oversample, power, reorth);
     else
       throw std::invalid_argument("precision must be 32 or 64");
   });
@@ -1091,6 +1229,34 @@ int fastpls_hip_batch_export64(void* batch, int fit, int field, double* out, siz
   return fpls::guarded(error, error_capacity, [&] {
     if (!batch) throw std::invalid_argument("null batch");
     static_cast<BatchHandle*>(batch)->export_field(fit, field, out, elements, true);
+  });
+}
+
+int fastpls_hip_batch_autotune(void* batch, int trials, double* report, int report_count,
+                               char* error, size_t error_capacity) {
+  return fpls::guarded(error, error_capacity, [&] {
+    if (!batch || (report_count > 0 && !report)) throw std::invalid_argument("invalid autotune request");
+    static_cast<BatchHandle*>(batch)->autotune(trials, report, report_count);
+  });
+}
+
+int fastpls_hip_batch_splits(void* batch, long long* lengths, char* error,
+                             size_t error_capacity) {
+  return fpls::guarded(error, error_capacity, [&] {
+    if (!lengths) throw std::invalid_argument("null split output");
+    const fpls::BatchSplits b =
+        batch ? static_cast<BatchHandle*>(batch)->splits() : fpls::default_splits();
+    for (int i = 0; i < 4; ++i) lengths[i] = b.get(i);
+  });
+}
+
+int fastpls_hip_batch_set_splits(const long long* lengths, char* error, size_t error_capacity) {
+  return fpls::guarded(error, error_capacity, [&] {
+    if (!lengths || !fpls::valid_splits(lengths))
+      throw std::invalid_argument("split lengths must lie in 128..4096");
+    fpls::BatchSplits b;
+    for (int i = 0; i < 4; ++i) b.at(i) = lengths[i];
+    fpls::set_default_splits(b);
   });
 }
 

@@ -85,6 +85,9 @@ SIGNATURES = {
     "fastpls_hip_batch_phases": (C.c_int, [_vp, _vp, C.c_int, _cp, _sz]),
     "fastpls_hip_batch_export64": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _sz, _cp, _sz]),
     "fastpls_hip_batch_rerun": (C.c_int, [_vp, _cp, _sz]),
+    "fastpls_hip_batch_autotune": (C.c_int, [_vp, C.c_int, _dp, C.c_int, _cp, _sz]),
+    "fastpls_hip_batch_splits": (C.c_int, [_vp, C.POINTER(C.c_longlong), _cp, _sz]),
+    "fastpls_hip_batch_set_splits": (C.c_int, [C.POINTER(C.c_longlong), _cp, _sz]),
     "fastpls_hip_batch_destroy": (None, [_vp]),
     "fastpls_hip_simpls_cv_regression": _CV_REGRESSION,
     "fastpls_hip_plssvd_cv_regression": _CV_REGRESSION,
@@ -451,6 +454,24 @@ def latent_data_device(x_ptr, y_ptr, precision, n, p, q, factors=10, decay=1.0, 
 BATCH_FIELDS = {"R": 0, "Q": 1, "T": 2, "mX": 3, "vX": 4, "mY": 5}
 
 
+def batch_splits(handle=None):
+    """Split lengths of a batch, or the process default for new batches
+    (fastpls_hip_batch_splits)."""
+    out = (C.c_longlong * 4)()
+    e = _err()
+    if load_library().fastpls_hip_batch_splits(handle, out, e, 1024) != 0:
+        _raise(e)
+    return tuple(int(v) for v in out)
+
+
+def set_batch_splits(lengths):
+    """Replace the process default split lengths (fastpls_hip_batch_set_splits)."""
+    arr = (C.c_longlong * 4)(*[int(v) for v in lengths])
+    e = _err()
+    if load_library().fastpls_hip_batch_set_splits(arr, e, 1024) != 0:
+        _raise(e)
+
+
 class SimplsBatch:
     """Many SIMPLS fits in one grouped execution (fastpls_hip_simpls_batch_create).
 
@@ -495,6 +516,21 @@ class SimplsBatch:
         e = _err()
         if load_library().fastpls_hip_batch_rerun(self._h, e, 1024) != 0:
             _raise(e)
+
+    def autotune(self, trials=2):
+        """Choose the batch's split lengths by measurement
+        (fastpls_hip_batch_autotune): {"splits": (step_rows, step_cols,
+        wide_rows, wide_cols), "seconds", "start_seconds", "plans"}."""
+        rep = np.zeros(7)
+        e = _err()
+        if load_library().fastpls_hip_batch_autotune(self._h, int(trials), _dptr(rep), 7, e,
+                                                      1024) != 0:
+            _raise(e)
+        return {"splits": tuple(int(v) for v in rep[:4]), "seconds": float(rep[4]),
+                "start_seconds": float(rep[5]), "plans": int(rep[6])}
+
+    def splits(self):
+        return batch_splits(self._h)
 
     def status(self):
         F = len(self.fits)

@@ -224,6 +224,31 @@ int fastpls_hip_batch_export64(void* batch, int fit, int field, double* out, siz
  * per-kernel launch on the host. Status, seconds and exports then describe
  * the new execution; the rSVD sketches are those of the creation seeds. */
 int fastpls_hip_batch_rerun(void* batch, char* error, size_t error_capacity);
+/* Split lengths of the batched passes (no reference counterpart): [0] the
+ * longest row split of the cross-product and loading passes and [1] the
+ * longest column split of the score pass of groups on the VALU, [2] / [3] the
+ * same for groups whose per-step passes run on the FP64 matrix cores; each in
+ * 128..4096. New batches take the process default: built-in lengths measured
+ * on the 256-fit workload, or those of the file named by the environment
+ * variable FASTPLS_HIP_TUNE_FILE when it exists at the first batch, or the
+ * last autotune's or set's.
+ * fastpls_hip_batch_autotune chooses them by measurement on this batch:
+ * starting from its lengths, each length in turn is tried at 1/4, 1/2, 2 and
+ * 4 times its value (wide lengths only with matrix-core groups, the others
+ * only with VALU groups), the whole batch re-planned and run `trials` (1..16)
+ * times per candidate, and a candidate kept when it lowers the fastest device
+ * time by more than 1%. The batch then holds the fits of the chosen plan
+ * (split sums differ from the first plan's in rounding only), the choice
+ * becomes the process default and, with FASTPLS_HIP_TUNE_FILE set, is written
+ * there. report (report_count <= 7 entries): [0..3] the chosen lengths, [4]
+ * the chosen plan's device seconds, [5] the starting plan's, [6] plans run.
+ * fastpls_hip_batch_splits writes a batch's lengths (the process default for
+ * a NULL batch); fastpls_hip_batch_set_splits replaces the default. */
+int fastpls_hip_batch_autotune(void* batch, int trials, double* report, int report_count,
+                               char* error, size_t error_capacity);
+int fastpls_hip_batch_splits(void* batch, long long* lengths, char* error,
+                             size_t error_capacity);
+int fastpls_hip_batch_set_splits(const long long* lengths, char* error, size_t error_capacity);
 void fastpls_hip_batch_destroy(void* batch);
 
 /* fastpls_resident_simpls_cv_regression (src/cuda_resident_api.h:20-26):
