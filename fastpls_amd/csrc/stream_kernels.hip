@@ -95,9 +95,12 @@ __device__ __forceinline__ long long xcd_local(long long local, long long total)
 template<class T, int NB>
 __global__ __launch_bounds__(kWG) void k_xw(const XwTask<T>* tasks, int count, XwTask<T> single) {
   const XwTask<T> t = load_task(tasks, count, single);
-  const long long local = (long long)blockIdx.x - t.tile0;
+  long long local = (long long)blockIdx.x - t.tile0;
   const long long rtiles = xw_row_tiles(t.m);
   if (local >= rtiles * t.splits) return;
+  // neighbouring row tiles share the 128-byte line at their boundary
+  // (columns of any length start unaligned): on one XCD, one L2 fetches it
+  local = xcd_local(local, rtiles * t.splits);
   const int split = int(local / rtiles);
   const long long row = (local % rtiles) * kWG + threadIdx.x;
   const long long per = (t.k + t.splits - 1) / t.splits;
@@ -295,9 +298,10 @@ __global__ __launch_bounds__(512) void k_xtw1(const XtwTask<T>* tasks, int count
                                               XtwTask<T> single) {
   constexpr int kCols = 8;  // columns per wave
   const XtwTask<T> t = load_task(tasks, count, single);
-  const long long local = (long long)blockIdx.x - t.tile0;
+  long long local = (long long)blockIdx.x - t.tile0;
   const long long ctiles = xtw_col_tiles(t.k);
   if (local >= ctiles * t.splits) return;
+  local = xcd_local(local, ctiles * t.splits);
   const int split = int(local / ctiles);
   const long long col0 = (local % ctiles) * 64;
   const long long per = (t.m + t.splits - 1) / t.splits;
@@ -801,6 +805,8 @@ __global__ __launch_bounds__(kBf6Threads) void k_xw_bf6(const XwTask<float>* tas
   constexpr int WCH = 3 * NB * CPR;                            // W chunks of one step
   constexpr int WPT = (WCH + kBf6Threads - 1) / kBf6Threads;
   const XwTask<float> t = load_task(tasks, count, single);
+  // (no XCD renumbering here: measured slower on the 1e5 x 2e4 pass, whose
+  // 400000-byte columns start aligned)
   const long long local = (long long)blockIdx.x - t.tile0;
   const long long rtiles = xw_row_tiles(t.m);
   if (local >= rtiles * t.splits) return;
@@ -1096,9 +1102,12 @@ __global__ __launch_bounds__(kWG) void k_xw_f64(const XwTask<TX>* tasks, int cou
                                                 XwTask<TX> single) {
   constexpr int NBLK = NB / 16;
   const XwTask<TX> t = load_task(tasks, count, single);
-  const long long local = (long long)blockIdx.x - t.tile0;
+  long long local = (long long)blockIdx.x - t.tile0;
   const long long rtiles = xw_row_tiles(t.m);
   if (local >= rtiles * t.splits) return;
+  // neighbouring row tiles share the 128-byte line at their boundary
+  // (columns of any length start unaligned): on one XCD, one L2 fetches it
+  local = xcd_local(local, rtiles * t.splits);
   const int split = int(local / rtiles);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m16 = lane & 15, g = lane >> 4;
