@@ -1089,65 +1089,96 @@ void simpls_cv_regression(const T* predictors, const T* responses, const int* fo
   }
   idx.alloc(sizeof(int) * all_idx.size());
   FPLS_HIP(hipMemcpy(idx.as<int>(), all_idx.data(), sizeof(int) * all_idx.size(), hipMemcpyHostToDevice));
-  std::vector<std::unique_ptr<Buf>> xtr(fold_count), ytr(fold_count), xte(fold_count), yte(fold_count);
-  std::vector<GroupIn> groups;
-  std::vector<FitIn> fits;
-  std::vector<int> fold_of;
+  // Folds are fitted together in batches of as many folds as fit: a fold holds
+  // gathered copies of its training and held-out rows (n rows of X and Y in
+  // total) and its fit's device state. The reference fits one fold at a time
+  // (one training copy resident, src/cuda_resident_api.cuh:770-1023); here
+  // the folds of a sub-batch share every launch, and a sub-batch holds at
+  // most 2/5 of the free device memory (as the reference's query workspace
+  // rule, :259-297), down to one fold per sub-batch.
+  // FASTPLS_HIP_CV_FREE_BYTES, when set, stands for the free memory.
+  std::vector<int> usable;
   for (int f = 0; f < fold_count; ++f) {
-    const int ntr = int(train_rows[f].size()), nte = int(test_rows[f].size());
-    if (nte == 0) { status[f] = 2; continue; }
-    if (maxp > std::min(ntr - 1, p)) throw std::invalid_argument("HIP CV component count exceeds a fold rank");
-    xtr[f].reset(new Buf); ytr[f].reset(new Buf); xte[f].reset(new Buf); yte[f].reset(new Buf);
-    xtr[f]->alloc(sizeof(T) * (size_t)ntr * p);
-    ytr[f]->alloc(sizeof(T) * (size_t)ntr * q);
-    xte[f]->alloc(sizeof(T) * (size_t)nte * p);
-    yte[f]->alloc(sizeof(T) * (size_t)nte * q);
-    launch_gather_rows<T>(xfull.as<T>(), n, p, idx.as<int>() + tr_off[f], ntr, xtr[f]->as<T>(), stream);
-    launch_gather_rows<T>(yfull.as<T>(), n, q, idx.as<int>() + tr_off[f], ntr, ytr[f]->as<T>(), stream);
-    launch_gather_rows<T>(xfull.as<T>(), n, p, idx.as<int>() + te_off[f], nte, xte[f]->as<T>(), stream);
-    launch_gather_rows<T>(yfull.as<T>(), n, q, idx.as<int>() + te_off[f], nte, yte[f]->as<T>(), stream);
-    groups.push_back({xtr[f]->as<T>(), ntr, p});
-    FitIn fi{};
-    fi.group = int(groups.size()) - 1; fi.q = q; fi.A = maxp; fi.y = ytr[f]->as<T>();
-    fi.labels = nullptr; fi.perm = nullptr; fi.seed = seed;  // SIMPLS folds share the seed
-    fits.push_back(fi);
-    fold_of.push_back(f);
-  }
-  FPLS_HIP(hipStreamSynchronize(stream));
-  if (fits.empty()) throw std::invalid_argument("HIP CV has no non-empty fold");
-  SimplsBatch<T> batch(groups, fits, scaling, oversample, power, -1);
-  for (int i = 0; i < batch.count(); ++i)
-    if (batch.completed(i) < maxp || batch.flags(i))
-      throw std::runtime_error("resident SIMPLS failed: invalid labels, decomposition or component norm");
-  RegressionTally tally(prefix_count);
-  Buf scores, part, pred, sums, flag;
-  sums.alloc(sizeof(double) * 3 * q);
-  flag.alloc(sizeof(int));
-  hipStream_t bs = batch.stream();
-  for (int i = 0; i < batch.count(); ++i) {
-    const int f = fold_of[i];
-    const int nte = int(test_rows[f].size());
-    scores.reserve(sizeof(double) * (size_t)nte * maxp);
-    pred.reserve(sizeof(double) * (size_t)nte * q);
-    batch.project_device(i, xte[f]->as<T>(), nte, maxp, scores.d(), part);
-    std::vector<double> host_metric(3 * (size_t)q * prefix_count);
-    std::vector<double> host_pred(store_predictions ? (size_t)nte * q * prefix_count : 0);
-    for (int j = 0; j < prefix_count; ++j) {
-      batch.predict_from_scores(i, scores.d(), nte, prefixes[j], pred.d());
-      if (store_predictions)
-        FPLS_HIP(hipMemcpyAsync(host_pred.data() + (size_t)j * nte * q, pred.d(),
-                                sizeof(double) * (size_t)nte * q, hipMemcpyDeviceToHost, bs));
-      launch_fill_int(flag.as<int>(), 1, 0, bs);
-      launch_response_sums<T>(pred.d(), yte[f]->as<T>(), nullptr, batch.y_mean(i), nte, q,
-                              sums.d(), flag.as<int>(), bs);
-      FPLS_HIP(hipMemcpyAsync(host_metric.data() + (size_t)j * 3 * q, sums.d(),
-                              sizeof(double) * 3 * q, hipMemcpyDeviceToHost, bs));
-      FPLS_HIP(hipStreamSynchronize(bs));
+    if (test_rows[f].empty()) {
+      status[f] = 2;
+      continue;
     }
-    tally.add_fold(host_metric.data(), q, nte);
-    if (store_predictions)
-      scatter_predictions(host_pred.data(), test_rows[f], n, q, prefix_count, prediction_output);
-    status[f] = 1;
+    if (maxp > std::min(int(train_rows[f].size()) - 1, p))
+      throw std::invalid_argument("HIP CV component count exceeds a fold rank");
+    usable.push_back(f);
+  }
+  if (usable.empty()) throw std::invalid_argument("HIP CV has no non-empty fold");
+  const double fold_bytes =
+      1.25 * (double(sizeof(T)) * double(n) * double(p + q) +
+              8.0 * double(n) * double(2 * maxp + 2 * q + 4) +
+              8.0 * double(p) * double(4 * maxp + 3 * q + 96) + 8.0 * double(q) * double(q));
+  size_t free_b = 0, total_b = 0;
+  if (const char* v = std::getenv("FASTPLS_HIP_CV_FREE_BYTES"))
+    free_b = size_t(std::strtoull(v, nullptr, 10));
+  else
+    FPLS_HIP(hipMemGetInfo(&free_b, &total_b));
+  const int per_batch = int(std::max<double>(
+      1.0, std::min<double>(double(usable.size()), std::floor(double(free_b) * 0.4 / fold_bytes))));
+  RegressionTally tally(prefix_count);
+  for (size_t b0 = 0; b0 < usable.size(); b0 += size_t(per_batch)) {
+    const size_t b1 = std::min(usable.size(), b0 + size_t(per_batch));
+    std::vector<std::unique_ptr<Buf>> xtr, ytr, xte, yte;
+    std::vector<GroupIn> groups;
+    std::vector<FitIn> fits;
+    std::vector<int> fold_of;
+    for (size_t u = b0; u < b1; ++u) {
+      const int f = usable[u];
+      const int ntr = int(train_rows[f].size()), nte = int(test_rows[f].size());
+      for (auto* v : {&xtr, &ytr, &xte, &yte}) v->emplace_back(new Buf);
+      xtr.back()->alloc(sizeof(T) * (size_t)ntr * p);
+      ytr.back()->alloc(sizeof(T) * (size_t)ntr * q);
+      xte.back()->alloc(sizeof(T) * (size_t)nte * p);
+      yte.back()->alloc(sizeof(T) * (size_t)nte * q);
+      launch_gather_rows<T>(xfull.as<T>(), n, p, idx.as<int>() + tr_off[f], ntr, xtr.back()->as<T>(), stream);
+      launch_gather_rows<T>(yfull.as<T>(), n, q, idx.as<int>() + tr_off[f], ntr, ytr.back()->as<T>(), stream);
+      launch_gather_rows<T>(xfull.as<T>(), n, p, idx.as<int>() + te_off[f], nte, xte.back()->as<T>(), stream);
+      launch_gather_rows<T>(yfull.as<T>(), n, q, idx.as<int>() + te_off[f], nte, yte.back()->as<T>(), stream);
+      groups.push_back({xtr.back()->as<T>(), ntr, p});
+      FitIn fi{};
+      fi.group = int(groups.size()) - 1; fi.q = q; fi.A = maxp; fi.y = ytr.back()->as<T>();
+      fi.labels = nullptr; fi.perm = nullptr; fi.seed = seed;  // SIMPLS folds share the seed
+      fits.push_back(fi);
+      fold_of.push_back(f);
+    }
+    FPLS_HIP(hipStreamSynchronize(stream));
+    SimplsBatch<T> batch(groups, fits, scaling, oversample, power, -1);
+    for (int i = 0; i < batch.count(); ++i)
+      if (batch.completed(i) < maxp || batch.flags(i))
+        throw std::runtime_error("resident SIMPLS failed: invalid labels, decomposition or component norm");
+    Buf scores, part, pred, sums, flag;
+    sums.alloc(sizeof(double) * 3 * q);
+    flag.alloc(sizeof(int));
+    hipStream_t bs = batch.stream();
+    for (int i = 0; i < batch.count(); ++i) {
+      const int f = fold_of[i];
+      const int nte = int(test_rows[f].size());
+      scores.reserve(sizeof(double) * (size_t)nte * maxp);
+      pred.reserve(sizeof(double) * (size_t)nte * q);
+      batch.project_device(i, xte[i]->as<T>(), nte, maxp, scores.d(), part);
+      std::vector<double> host_metric(3 * (size_t)q * prefix_count);
+      std::vector<double> host_pred(store_predictions ? (size_t)nte * q * prefix_count : 0);
+      for (int j = 0; j < prefix_count; ++j) {
+        batch.predict_from_scores(i, scores.d(), nte, prefixes[j], pred.d());
+        if (store_predictions)
+          FPLS_HIP(hipMemcpyAsync(host_pred.data() + (size_t)j * nte * q, pred.d(),
+                                  sizeof(double) * (size_t)nte * q, hipMemcpyDeviceToHost, bs));
+        launch_fill_int(flag.as<int>(), 1, 0, bs);
+        launch_response_sums<T>(pred.d(), yte[i]->as<T>(), nullptr, batch.y_mean(i), nte, q,
+                                sums.d(), flag.as<int>(), bs);
+        FPLS_HIP(hipMemcpyAsync(host_metric.data() + (size_t)j * 3 * q, sums.d(),
+                                sizeof(double) * 3 * q, hipMemcpyDeviceToHost, bs));
+        FPLS_HIP(hipStreamSynchronize(bs));
+      }
+      tally.add_fold(host_metric.data(), q, nte);
+      if (store_predictions)
+        scatter_predictions(host_pred.data(), test_rows[f], n, q, prefix_count, prediction_output);
+      status[f] = 1;
+    }
   }
   tally.finish(total_ss, metric, metrics, q2_values, rmsd_values, observed_r2_values);
 }

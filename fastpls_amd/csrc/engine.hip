@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -130,10 +131,14 @@ class Model final : public Handle, private StreamOps {
     check_path(prefixes, count);
     if (lda && !classification_) throw std::invalid_argument("LDA requires class labels");
     const int maxp = prefixes[count - 1];
-    project_rows(load_rows(static_cast<const T*>(x), rows, device_rows), rows, maxp);
+    const T* xs = static_cast<const T*>(x);
     T* o = static_cast<T*>(out);
-    for (int j = 0; j < count; ++j) {
-      response_block(rows, prefixes[j], lda, o + (size_t)j * rows * q_out(), nullptr);
+    const int block = device_rows ? rows : query_block_rows(rows, query_row_bytes(maxp, 0));
+    for (int start = 0; start < rows; start += block) {
+      const int cnt = std::min(block, rows - start);
+      project_rows(device_rows ? xs : load_row_block(xs, rows, start, cnt), cnt, maxp);
+      for (int j = 0; j < count; ++j)
+        response_block(cnt, prefixes[j], lda, o + (size_t)j * rows * q_out() + start, rows);
     }
     FPLS_HIP(hipStreamSynchronize(stream_));
   }
@@ -145,21 +150,29 @@ class Model final : public Handle, private StreamOps {
       throw std::invalid_argument("invalid resident classification-path request");
     check_path(prefixes, count);
     const int maxp = prefixes[count - 1];
-    project_rows(load_rows(static_cast<const T*>(x), rows, device_rows), rows, maxp);
-    top_.reserve(sizeof(int) * (size_t)rows * top);
-    for (int j = 0; j < count; ++j) {
-      T* resp = responses ? static_cast<T*>(responses) + (size_t)j * rows * q_ : nullptr;
-      if (resp) response_block(rows, prefixes[j], false, resp, nullptr);
-      const double* constants = nullptr;
-      if (lda) {
-        response_block(rows, prefixes[j], true, nullptr, nullptr);
-        constants = nullptr;  // discriminants already include the constants
-      } else if (!resp) {
-        response_block(rows, prefixes[j], false, nullptr, nullptr);
+    const T* xs = static_cast<const T*>(x);
+    const int block = device_rows ? rows : query_block_rows(rows, query_row_bytes(maxp, top));
+    for (int start = 0; start < rows; start += block) {
+      const int cnt = std::min(block, rows - start);
+      project_rows(device_rows ? xs : load_row_block(xs, rows, start, cnt), cnt, maxp);
+      top_.reserve(sizeof(int) * (size_t)cnt * top);
+      for (int j = 0; j < count; ++j) {
+        T* resp = responses ? static_cast<T*>(responses) + (size_t)j * rows * q_ + start : nullptr;
+        if (resp) response_block(cnt, prefixes[j], false, resp, rows);
+        if (lda) {
+          // discriminants already include the class constants
+          response_block(cnt, prefixes[j], true, nullptr, rows);
+        } else if (!resp) {
+          response_block(cnt, prefixes[j], false, nullptr, rows);
+        }
+        launch_topk(pred64_.d(), cnt, q_, nullptr, top, top_.as<int>(), stream_);
+        // labels of the block: cnt x top into the rows x top output
+        FPLS_HIP(hipMemcpy2DAsync(labels + (size_t)j * rows * top + start, sizeof(int) * (size_t)rows,
+                                  top_.as<int>(), sizeof(int) * (size_t)cnt,
+                                  sizeof(int) * (size_t)cnt, top, hipMemcpyDeviceToHost, stream_));
+        // top_ is reused by the next prefix and block
+        FPLS_HIP(hipStreamSynchronize(stream_));
       }
-      launch_topk(pred64_.d(), rows, q_, constants, top, top_.as<int>(), stream_);
-      FPLS_HIP(hipMemcpyAsync(labels + (size_t)j * rows * top, top_.as<int>(),
-                              sizeof(int) * (size_t)rows * top, hipMemcpyDeviceToHost, stream_));
     }
     FPLS_HIP(hipStreamSynchronize(stream_));
   }
@@ -170,7 +183,7 @@ class Model final : public Handle, private StreamOps {
       throw std::invalid_argument("invalid resident response representation");
     if (!x || rows < 1 || prefix < 1 || prefix > A_) throw std::invalid_argument("invalid resident prediction input");
     project_rows(load_rows(static_cast<const T*>(x), rows), rows, prefix);
-    response_block(rows, prefix, false, nullptr, nullptr);
+    response_block(rows, prefix, false, nullptr, 0);
     metric_.reserve(sizeof(double) * 3 * q_);
     flags_.reserve(sizeof(int));
     fill_zero_int(flags_.as<int>(), 1);
@@ -208,7 +221,7 @@ class Model final : public Handle, private StreamOps {
     metric_.reserve(sizeof(double) * 3 * q_);
     flags_.reserve(sizeof(int));
     for (int j = 0; j < count; ++j) {
-      response_block(rows, prefixes[j], false, nullptr, nullptr);
+      response_block(rows, prefixes[j], false, nullptr, 0);
       if (predictions)
         FPLS_HIP(hipMemcpyAsync(predictions + (size_t)j * rows * q_, pred64_.d(),
                                 sizeof(double) * (size_t)rows * q_, hipMemcpyDeviceToHost, stream_));
@@ -850,6 +863,43 @@ class Model final : public Handle, private StreamOps {
     return rows_.as<T>();
   }
 
+  // Rows start .. start + count of the host rows x p matrix x (leading
+  // dimension rows) into the device row buffer (count x p).
+  const T* load_row_block(const T* x, int rows, int start, int count) {
+    if (start == 0 && count == rows) return load_rows(x, rows);
+    rows_.reserve(sizeof(T) * (size_t)count * p_);
+    FPLS_HIP(hipMemcpy2DAsync(rows_.as<T>(), sizeof(T) * (size_t)count, x + start,
+                              sizeof(T) * (size_t)rows, sizeof(T) * (size_t)count, p_,
+                              hipMemcpyHostToDevice, stream_));
+    return rows_.as<T>();
+  }
+
+  // Device bytes a query holds per host row: the row, its scores, FP64 and
+  // output responses and top-k labels.
+  size_t query_row_bytes(int prefix, int top) const {
+    return sizeof(T) * (size_t)p_ + sizeof(double) * size_t(prefix + q_) + sizeof(T) * (size_t)q_ +
+           sizeof(int) * (size_t)top;
+  }
+
+  // Rows per block of a query on host rows (src/cuda_resident_api.cuh:259-297,
+  // where the reference blocks classify; here predictions as well): when the
+  // query's workspace (rows x bytes_per_row) exceeds 2/5 of the free device
+  // memory, rows go through in blocks of max(1024, min(rows, 65536,
+  // max(1024 rows' bytes, 3/20 of free) / bytes_per_row)) rows, else all at
+  // once. FASTPLS_HIP_QUERY_FREE_BYTES, when set, stands for the free memory
+  // (so blocking can be exercised at small sizes).
+  int query_block_rows(int rows, size_t bytes_per_row) const {
+    size_t free_b = 0, total_b = 0;
+    if (const char* v = std::getenv("FASTPLS_HIP_QUERY_FREE_BYTES"))
+      free_b = size_t(std::strtoull(v, nullptr, 10));
+    else
+      FPLS_HIP(hipMemGetInfo(&free_b, &total_b));
+    if (size_t(rows) * bytes_per_row <= free_b / 5 * 2) return rows;
+    const size_t target = std::max(bytes_per_row * 1024, free_b / 20 * 3);
+    const long long fit = (long long)(target / bytes_per_row);
+    return int(std::max<long long>(1024, std::min<long long>({(long long)rows, 65536LL, fit})));
+  }
+
   // scores (rows x prefix) = ((x - m) / s) R[:, :prefix]
   void project_rows(const T* x, int rows, int prefix) {
     scores_.reserve(sizeof(double) * (size_t)rows * prefix);
@@ -863,8 +913,9 @@ class Model final : public Handle, private StreamOps {
   }
 
   // Responses (or LDA discriminants) of one prefix from scores_, FP64 into
-  // pred64_, converted copy to host `out` (rows x q) when non-null.
-  void response_block(int rows, int prefix, bool lda, T* out, void*) {
+  // pred64_, converted copy to host `out` (rows x q, leading dimension ldo;
+  // 0 = rows) when non-null.
+  void response_block(int rows, int prefix, bool lda, T* out, int ldo) {
     pred64_.reserve(sizeof(double) * (size_t)rows * q_);
     if (lda) {
       ensure_lda(prefix);
@@ -884,8 +935,13 @@ class Model final : public Handle, private StreamOps {
     if (out) {
       outT_.reserve(sizeof(T) * (size_t)rows * q_);
       launch_convert<T>(pred64_.d(), outT_.as<T>(), (long long)rows * q_, stream_);
-      FPLS_HIP(hipMemcpyAsync(out, outT_.as<T>(), sizeof(T) * (size_t)rows * q_,
-                              hipMemcpyDeviceToHost, stream_));
+      if (ldo <= 0 || ldo == rows)
+        FPLS_HIP(hipMemcpyAsync(out, outT_.as<T>(), sizeof(T) * (size_t)rows * q_,
+                                hipMemcpyDeviceToHost, stream_));
+      else
+        FPLS_HIP(hipMemcpy2DAsync(out, sizeof(T) * (size_t)ldo, outT_.as<T>(),
+                                  sizeof(T) * (size_t)rows, sizeof(T) * (size_t)rows, q_,
+                                  hipMemcpyDeviceToHost, stream_));
       // outT_ is reused by the next block: keep the copy ordered before it
       FPLS_HIP(hipStreamSynchronize(stream_));
     }
