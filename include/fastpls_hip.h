@@ -77,7 +77,12 @@ void* fastpls_hip_kernelpls_create(const void* x, const void* y, const int* labe
                                    size_t error_capacity);
 
 /* fastpls_resident_predict_path (src/cuda_resident_api.h:57-59): output is
- * rows x q x prefix_count; lda != 0 returns LDA discriminant scores. */
+ * rows x q x prefix_count; lda != 0 returns LDA discriminant scores.
+ * Queries on host rows (this entry and the classify entries) go through the
+ * device in row blocks when their workspace would exceed 2/5 of the free
+ * device memory, as the reference blocks classify
+ * (src/cuda_resident_api.cuh:259-297); the environment variable
+ * FASTPLS_HIP_QUERY_FREE_BYTES, when set, stands for the free memory. */
 int fastpls_hip_predict_path(void* model, const void* x, int rows, const int* prefixes,
                              int prefix_count, int lda, void* predictions, char* error,
                              size_t error_capacity);
@@ -253,8 +258,11 @@ void fastpls_hip_batch_destroy(void* batch);
 
 /* fastpls_resident_simpls_cv_regression (src/cuda_resident_api.h:20-26):
  * same arguments, outputs and metric ladder (2 observed R^2, 3 Q^2, 4 RMSD;
- * src/cuda_resident_api.cuh:772-1023). All folds are fitted in one batch;
- * SIMPLS folds share the seed, as in the reference. */
+ * src/cuda_resident_api.cuh:772-1023). The folds are fitted in batches of as
+ * many folds as fit in 2/5 of the free device memory (all folds in one batch
+ * when they fit, one at a time at worst; FASTPLS_HIP_CV_FREE_BYTES, when set,
+ * stands for the free memory); SIMPLS folds share the seed, as in the
+ * reference. */
 int fastpls_hip_simpls_cv_regression(const void* predictors, const void* responses,
                                      const int* folds, int precision, int n, int p, int q,
                                      const int* prefixes, int prefix_count, int scaling,
