@@ -44,7 +44,7 @@ class Model final : public Handle, private StreamOps {
 
   Model(int method, const T* x, const T* y, const int* labels, bool device_inputs, int n,
         int p, int q, int components, int scaling, int oversample, int power,
-        unsigned long long seed, hipStream_t stream = nullptr)
+        unsigned long long seed, hipStream_t stream = nullptr, bool retain_scores = true)
       : method_(method), n_(n), p_(p), q_(q), scaling_(scaling), oversample_(oversample),
         power_(power), seed_(seed), classification_(labels != nullptr) {
     if ((y == nullptr) == (labels == nullptr))
@@ -64,12 +64,26 @@ class Model final : public Handle, private StreamOps {
         throw std::invalid_argument("classification PLS-SVD components must be below class count");
     }
     A_ = components;
+    // training scores are dropped after the fit only for large
+    // classification fits (src/cuda_resident_simpls.cuh:213-216,
+    // src/cuda_resident_plssvd.cuh:166-169): they are needed by nothing else
+    // once the LDA moments are formed
+    {
+      const double score_bytes = double(n) * double(components) * double(sizeof(T));
+      const bool small = score_bytes <= 256.0 * 1024.0 * 1024.0 || n < 8 * p;
+      keep_scores_ = retain_scores || !classification_ || small ||
+                     (method_ == kSimpls ? components < 8 : 2 * components < p);
+    }
+    // src/r_api.cpp:1165-1176: SIMPLS takes its loadings from a cached X_sᵀX_s
+    // when A >= 20, p <= n, n >= 8p and p <= 512 (the CPU core's rule)
+    cache_gram_ = method_ == kSimpls && components >= 20 && p <= n && n >= 8 * p && p <= 512;
     // R/main.R:3454-3463: no cross-covariance is formed once p·q·8 bytes
     // exceed 32 MiB (dense responses); above 512 MiB of entries SIMPLS takes
     // the rank-one direction route (src/r_api.cpp:1480-1493).
     implicit_ = !classification_ && double(p) * double(q) * 8.0 / (1024.0 * 1024.0) > 32.0;
     rank_one_ = implicit_ && method_ == kSimpls &&
                 double(p) * double(q) * double(sizeof(T)) > 512.0 * 1024.0 * 1024.0;
+    cache_gram_ = cache_gram_ && !implicit_;
     // SIMPLS amplifies the rounding of the products over X (the later
     // directions of its component recursion move by up to ~1e3 times a
     // relative perturbation of S), so FP32-stored X is widened to FP64 for
@@ -86,6 +100,11 @@ class Model final : public Handle, private StreamOps {
       prepare_crossprod();
       if (method_ == kSimpls) fit_simpls(); else fit_plssvd();
       finish_fit();
+      if (!keep_scores_) {
+        ensure_lda_moments();  // classification only: its moments come from the scores
+        FPLS_HIP(hipStreamSynchronize(stream_));
+        Tsc_.release();
+      }
       FPLS_HIP(hipEventRecord(stop.e, stream_));
       check_state();
       float ms = 0.f;
@@ -304,7 +323,9 @@ class Model final : public Handle, private StreamOps {
       case 0: src = R_.d(); break;
       case 1: src = Q_.d(); break;
       case 2:
-        if (Tsc_.empty()) throw std::runtime_error("training scores were released by compact");
+        if (Tsc_.empty())
+          throw std::runtime_error(compacted_ ? "training scores were released by compact"
+                                              : "training scores were not retained");
         src = Tsc_.d();
         break;
       case 3: src = mx_.d(); break;
@@ -329,7 +350,7 @@ class Model final : public Handle, private StreamOps {
     *block = method_ == kSimpls ? block_ : A_;
     *block_limit = 0;
     *implicit = implicit_ ? 1 : 0;
-    *cache = 0;
+    *cache = cache_gram_ ? 1 : 0;
   }
 
   void compact(bool prepare_lda) override {
@@ -338,7 +359,7 @@ class Model final : public Handle, private StreamOps {
       ensure_lda_moments();
     }
     FPLS_HIP(hipStreamSynchronize(stream_));
-    for (Buf* b : {&xown_, &yown_, &Yc_, &S_, &S0_, &G_, &V_, &Tsc_, &simpls_scratch_, &cand_,
+    for (Buf* b : {&xown_, &yown_, &Yc_, &S_, &S0_, &G_, &V_, &Tsc_, &xgram_, &simpls_scratch_, &cand_,
                    &cand_scaled_, &t_raw_, &ts_part_, &ls_part_, &omega_, &steps_, &fitdev_,
                    &part_, &tmp1_, &tmp2_, &tmp3_, &H_, &labels_dev_, &frag_, &cp_part_,
                    &cp_stats_, &qr_gram_, &qr_rinv_, &qr_tmp_, &qr_fail_, &qraw_, &op_sig_,
@@ -355,7 +376,7 @@ class Model final : public Handle, private StreamOps {
 
   std::vector<Buf*> all_bufs() {
     std::vector<Buf*> v = {&xown_, &yown_, &labels_dev_, &counts_, &Yc_, &H_, &ysum_, &my_, &mx_,
-                           &sx_, &colss_, &S_, &S0_, &G_, &R_, &Q_, &V_, &Tsc_, &Rs_, &sigma_,
+                           &sx_, &colss_, &S_, &S0_, &G_, &R_, &Q_, &V_, &Tsc_, &xgram_, &Rs_, &sigma_,
                            &gram_, &W_, &state_, &omega_, &steps_, &fitdev_, &simpls_scratch_,
                            &cand_, &cand_scaled_, &t_raw_, &ts_part_, &ls_part_, &tmp1_,
                            &tmp2_, &tmp3_, &cp_part_, &cp_stats_, &usable_, &rows_, &scores_,
@@ -579,6 +600,20 @@ class Model final : public Handle, private StreamOps {
       G_.alloc(sizeof(double) * (size_t)q_ * q_);
       tproduct<double>(S_.d(), p_, q_, nullptr, S_.d(), p_, q_, G_.d());
     }
+    // the cached route's X_sᵀX_s (simpls.hpp:521-532): columns of X_s in FP64,
+    // 64 at a time, against the centred X, rows then divided by the scales
+    if (cache_gram_) {
+      xgram_.alloc(sizeof(double) * (size_t)p_ * p_);
+      Buf& chunk = tmp3_;
+      chunk.reserve(sizeof(double) * (size_t)n_ * std::min(64, p_));
+      for (int c0 = 0; c0 < p_; c0 += 64) {
+        const int w = std::min(64, p_ - c0);
+        launch_standardize_columns<T>(X_, n_, c0, w, mx_.d(), sx_.d(), chunk.d(), stream_);
+        double* out = xgram_.d() + (size_t)c0 * p_;
+        tproduct<T>(X_, n_, p_, mx_.d(), chunk.d(), n_, w, out);
+        launch_div_rows(out, sx_.d(), p_, w, out, stream_);
+      }
+    }
     // host copies are members: they outlive the asynchronous uploads
     omega_host_ = std::move(pool);
     steps_host_ = std::move(steps);
@@ -623,6 +658,7 @@ class Model final : public Handle, private StreamOps {
     f.cand = cand_.d(); f.cand_scaled = cand_scaled_.d(); f.t_raw = t_raw_.d();
     f.ts_part = ts_part_.d(); f.ls_part = ls_part_.d();
     f.R = R_.d(); f.Q = Q_.d(); f.V = V_.d(); f.T = Tsc_.d();
+    f.xgram = cache_gram_ ? xgram_.d() : nullptr;
     f.scratch = simpls_scratch_.d(); f.state = state_.as<int>();
     fitdev_host_ = f;
     fitdev_.alloc(sizeof(SimplsFitDev));
@@ -655,6 +691,11 @@ class Model final : public Handle, private StreamOps {
       xw.frag = frag(xw_frag_bytes<T>(p_, b, f64));
       launch_xw_single<T>(xw, stream_, f64);
       launch_sum_slabs(ts_part_.d(), xs, (long long)n_ * b, (long long)n_ * b, t_raw_.d(), stream_);
+      if (cache_gram_) {
+        // cached route: the loadings come from X_sᵀX_s in the update
+        launch_simpls_update(fitdev_.as<SimplsFitDev>(), 1, int(s), stream_);
+        continue;
+      }
       // loadings: (X - 1 mᵀ)ᵀ t  (divided by s in the update)
       XtwTask<T> xt{};
       xt.A = X_; xt.m = n_; xt.k = p_; xt.center = mx_.d(); xt.W = t_raw_.d(); xt.ldw = n_;
@@ -1027,6 +1068,9 @@ class Model final : public Handle, private StreamOps {
   int lda_prefix_ = 0;
   int scores_rows_ = 0;
   bool implicit_ = false, rank_one_ = false;
+  bool cache_gram_ = false;   // SIMPLS loadings from the cached X_sᵀX_s (xgram_)
+  bool keep_scores_ = true;   // training scores kept after the fit
+  Buf xgram_;
   // matrix-free route workspaces
   Buf qraw_, op_sig_, op_tmp_, op_scaled_, op_coef_, op_corr_, op_pb_, op_rev_, op_sample_;
 };
@@ -1035,16 +1079,16 @@ class Model final : public Handle, private StreamOps {
 
 Handle* create(int method, const void* x, const void* y, const int* labels, int precision,
                bool device, int n, int p, int q, int components, int scaling, int oversample,
-               int power, unsigned long long seed, hipStream_t stream) {
+               int power, unsigned long long seed, hipStream_t stream, bool retain_scores) {
   require_device();
   if (precision == 32)
     return new Model<float>(method, static_cast<const float*>(x), static_cast<const float*>(y),
                             labels, device, n, p, q, components, scaling, oversample, power,
-                            seed, stream);
+                            seed, stream, retain_scores);
   if (precision == 64)
     return new Model<double>(method, static_cast<const double*>(x),
                              static_cast<const double*>(y), labels, device, n, p, q, components,
-                             scaling, oversample, power, seed, stream);
+                             scaling, oversample, power, seed, stream, retain_scores);
   throw std::invalid_argument("precision must be 32 or 64");
 }
 
@@ -1155,11 +1199,12 @@ void* fastpls_hip_simpls_create(const void* x, const void* y, const int* labels,
                                 int n, int p, int q, int components, int scaling, int oversample,
                                 int power, int retain_scores, unsigned long long seed,
                                 char* error, size_t error_capacity) {
-  (void)retain_scores;
   Handle* h = nullptr;
   fpls::guarded(error, error_capacity, [&] {
+    if (retain_scores != 0 && retain_scores != 1)
+      throw std::invalid_argument("invalid resident score-retention request");
     h = fpls::create(0, x, y, labels, precision, false, n, p, q, components, scaling,
-                     oversample, power, seed);
+                     oversample, power, seed, nullptr, retain_scores == 1);
   });
   return h;
 }
@@ -1168,11 +1213,12 @@ void* fastpls_hip_plssvd_create(const void* x, const void* y, const int* labels,
                                 int n, int p, int q, int components, int scaling, int oversample,
                                 int power, int retain_scores, unsigned long long seed,
                                 char* error, size_t error_capacity) {
-  (void)retain_scores;
   Handle* h = nullptr;
   fpls::guarded(error, error_capacity, [&] {
+    if (retain_scores != 0 && retain_scores != 1)
+      throw std::invalid_argument("invalid resident score-retention request");
     h = fpls::create(1, x, y, labels, precision, false, n, p, q, components, scaling,
-                     oversample, power, seed);
+                     oversample, power, seed, nullptr, retain_scores == 1);
   });
   return h;
 }

@@ -57,10 +57,12 @@ class Handle {
 // method 0 = SIMPLS, 1 = PLS-SVD; precision 32 or 64. With `device_inputs`
 // x, y and labels are device arrays that must outlive the handle. A non-null
 // `stream` (owned by the caller) carries the fit instead of a stream from the
-// stream cache.
+// stream cache. `retain_scores` false lets a large classification fit drop
+// its training scores after the fit (the reference's rule,
+// src/cuda_resident_simpls.cuh:213-216, cuda_resident_plssvd.cuh:166-169).
 Handle* create(int method, const void* x, const void* y, const int* labels, int precision,
                bool device_inputs, int n, int p, int q, int components, int scaling,
                int oversample, int power, unsigned long long seed,
-               hipStream_t stream = nullptr);
+               hipStream_t stream = nullptr, bool retain_scores = true);
 
 }  // namespace fpls

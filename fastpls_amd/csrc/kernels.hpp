@@ -42,6 +42,10 @@ struct SimplsFitDev {
   // matrix-free route (no S): Ỹᵀ t_raw (q x bmax) of the candidate block;
   // S, S0 and G are null and the update neither deflates S nor reads S0
   const double* qraw;
+  // cached cross-product route (simpls.hpp:600-620): the p x p X_sᵀ X_s;
+  // the update forms the loading X_sᵀ X_s r and |t|² = rᵀ X_sᵀ X_s r from it
+  // (no loading pass; T, when not null, is t_raw / |t|)
+  const double* xgram;
 };
 
 // Error flags written to state[2].
@@ -135,6 +139,11 @@ void launch_finalize_crossprod(const T* X, long long n, int p, int q, int scalin
                                double* colss, double* S, double* S0, int* flags,
                                hipStream_t s, int* cross_flags = nullptr,
                                const int* cross_offsets = nullptr);
+// out (n x cols, FP64) = (X[:, c0 + c] - mean[c0 + c]) / scale[c0 + c]: columns
+// c0 .. c0 + cols of the standardised X (leading dimension n)
+template<class T>
+void launch_standardize_columns(const T* X, long long n, int c0, int cols, const double* mean,
+                                const double* scale, double* out, hipStream_t s);
 // out[i] = in[i] / d[i % rows]  (scale rows of a rows x cols matrix)
 void launch_div_rows(const double* in, const double* d, long long rows, int cols,
                      double* out, hipStream_t s);

@@ -232,6 +232,20 @@ __global__ __launch_bounds__(kWG) void k_div_rows(const double* in, const double
   }
 }
 
+// out (n x cols, FP64) = (X[:, c0 + c] - mean[c0 + c]) / scale[c0 + c]
+template<class T>
+__global__ __launch_bounds__(kWG) void k_standardize_columns(const T* X, long long n, int c0,
+                                                             int cols, const double* mean,
+                                                             const double* scale, double* out) {
+  const long long total = n * cols;
+  for (long long e = (long long)blockIdx.x * kWG + threadIdx.x; e < total;
+       e += (long long)gridDim.x * kWG) {
+    const long long c = e / n, i = e - c * n;
+    const long long j = c0 + c;
+    out[e] = (double(X[i + j * n]) - mean[j]) / scale[j];
+  }
+}
+
 __global__ __launch_bounds__(kWG) void k_div_rows_list(const DivRows* list) {
   const DivRows t = list[blockIdx.y];
   const long long total = t.rows * t.cols;
@@ -585,6 +599,15 @@ void launch_transpose(const double* in, long long rows, long long cols, double* 
 }
 
 template<class T>
+void launch_standardize_columns(const T* X, long long n, int c0, int cols, const double* mean,
+                                const double* scale, double* out, hipStream_t s) {
+  if (n <= 0 || cols <= 0) return;
+  hipLaunchKernelGGL((k_standardize_columns<T>), dim3(grid_for(n * cols)), dim3(kWG), 0, s, X, n,
+                     c0, cols, mean, scale, out);
+  FPLS_LAUNCHED("k_standardize_columns");
+}
+
+template<class T>
 void launch_convert(const double* in, T* out, long long count, hipStream_t s) {
   hipLaunchKernelGGL((k_convert<T>), dim3(grid_for(count)), dim3(kWG), 0, s, in, out, count);
   FPLS_LAUNCHED("k_convert");
@@ -606,7 +629,9 @@ void launch_convert(const double* in, T* out, long long count, hipStream_t s) {
                                         long long, int, double*, int*, hipStream_t);         \
   template void launch_latent_data<T>(T*, double*, long long, int, int, int, double, double, \
                                       unsigned long long, int, double*, hipStream_t);             \
-  template void launch_convert<T>(const double*, T*, long long, hipStream_t);
+  template void launch_convert<T>(const double*, T*, long long, hipStream_t);                \
+  template void launch_standardize_columns<T>(const T*, long long, int, int, const double*,  \
+                                              const double*, double*, hipStream_t);
 FPLS_PREP_INSTANCES(float)
 FPLS_PREP_INSTANCES(double)
 #undef FPLS_PREP_INSTANCES

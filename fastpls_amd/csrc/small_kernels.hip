@@ -307,17 +307,46 @@ __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits,
     const int comp = st.a + j;
     if (comp >= f.A) break;
     const double* t = f.t_raw + (long long)j * n;
+    double* Rc = f.R + (long long)comp * p;
+    double* Vc = f.V + (long long)comp * p;
+    double inv = 0.0;  // 1 / |t|
+    if (f.xgram) {
+      // cached route (simpls.hpp:600-620): loading w = G r, |t|² = rᵀ w,
+      // r and w divided by |t|
+      const double* r = f.cand + (long long)j * p;
+      for (int i = threadIdx.x; i < p; i += kWG) {
+        double s0 = 0.0, s1 = 0.0;
+        int l = 0;
+        for (; l + 2 <= p; l += 2) {
+          s0 = fma(f.xgram[i + (long long)l * p], r[l], s0);
+          s1 = fma(f.xgram[i + (long long)(l + 1) * p], r[l + 1], s1);
+        }
+        if (l < p) s0 = fma(f.xgram[i + (long long)l * p], r[l], s0);
+        Vc[i] = s0 + s1;
+      }
+      __syncthreads();
+      double part = 0.0;
+      for (int i = threadIdx.x; i < p; i += kWG) part = fma(r[i], Vc[i], part);
+      const double ns = wg_sum_n<kWG>(part, sh.red);
+      if (!(ns > 0.0) || !isfinite(ns)) {
+        set_flag(f.state, isfinite(ns) ? kFlagZeroNorm : kFlagNonFinite);
+        if (threadIdx.x == 0) f.state[1] = 1;
+        return;
+      }
+      inv = 1.0 / sqrt(ns);
+      if (f.T) scale_copy(f.T + (long long)comp * n, t, n, inv);
+      scale_copy(Rc, r, p, inv);
+      scale_copy(Vc, Vc, p, inv);
+      __syncthreads();
+    } else {
     const double tn = wg::norm(t, n, sh.red);
     if (!(tn > 0.0) || !isfinite(tn)) {
       set_flag(f.state, isfinite(tn) ? kFlagZeroNorm : kFlagNonFinite);
       if (threadIdx.x == 0) f.state[1] = 1;
       return;
     }
-    const double inv = 1.0 / tn;
-    double* Tc = f.T + (long long)comp * n;
-    scale_copy(Tc, t, n, inv);
-    double* Rc = f.R + (long long)comp * p;
-    double* Vc = f.V + (long long)comp * p;
+    inv = 1.0 / tn;
+    if (f.T) scale_copy(f.T + (long long)comp * n, t, n, inv);
     scale_copy(Rc, f.cand + (long long)j * p, p, inv);
     {
       // loading sums over the row splits: four elements per thread in flight
@@ -354,6 +383,7 @@ __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits,
       }
     }
     __syncthreads();
+    }
     // response loadings q_a = S0ᵀ r (= Ỹᵀ t on the matrix-free route)
     if (f.qraw) {
       for (int c = threadIdx.x; c < q; c += kWG)
