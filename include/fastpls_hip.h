@@ -47,14 +47,22 @@ int fastpls_hip_gemm(const void* left, const void* right, int precision, int row
                      size_t error_capacity);
 
 /* fastpls_resident_simpls_create (src/cuda_resident_api.h:40-43). Exactly one
- * of y (n x q) and labels (n, one-based, q classes) is non-NULL. */
+ * of y (n x q) and labels (n, one-based, q classes) is non-NULL.
+ * retain_scores (0 or 1): with 0 a classification fit with components >= 8,
+ * n >= 8p and n x components scores above 256 MiB forms its LDA moments and
+ * drops its training scores after the fit (src/cuda_resident_simpls.cuh:
+ * 213-216); every other fit keeps them. Fits with components >= 20, p <= n,
+ * n >= 8p and p <= 512 take their loadings from a cached X_sᵀX_s (the CPU
+ * core's rule, src/r_api.cpp:1165-1176). */
 void* fastpls_hip_simpls_create(const void* x, const void* y, const int* labels,
                                 int precision, int n, int p, int q, int components,
                                 int scaling, int oversample, int power,
                                 int retain_scores, unsigned long long seed,
                                 char* error, size_t error_capacity);
 
-/* fastpls_resident_plssvd_create (src/cuda_resident_api.h:44-47). */
+/* fastpls_resident_plssvd_create (src/cuda_resident_api.h:44-47).
+ * retain_scores as above, with the PLS-SVD rule (2 x components >= p instead
+ * of components >= 8; src/cuda_resident_plssvd.cuh:166-169). */
 void* fastpls_hip_plssvd_create(const void* x, const void* y, const int* labels,
                                 int precision, int n, int p, int q, int components,
                                 int scaling, int oversample, int power,
@@ -131,7 +139,8 @@ int fastpls_hip_response_sums(void* model, const void* x, const void* y, const i
 int fastpls_hip_project(void* model, const void* x, int rows, int prefix, void* out,
                         char* error, size_t error_capacity);
 
-/* fastpls_resident_controls (src/cuda_resident_api.h:73-75). */
+/* fastpls_resident_controls (src/cuda_resident_api.h:73-75).
+ * predictor_crossprod_cache is 1 when the fit took the cached X_sᵀX_s route. */
 int fastpls_hip_controls(void* model, int* oversample, int* power, int* block,
                          int* block_limit, int* implicit_operator,
                          int* predictor_crossprod_cache, char* error,
