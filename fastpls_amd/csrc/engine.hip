@@ -366,6 +366,7 @@ class Model final : public Handle, private StreamOps {
                    &op_tmp_, &op_scaled_, &op_coef_, &op_corr_, &op_pb_, &op_rev_, &op_sample_})
       b->release();
     for (Buf& b : svd_ws_) b.release();
+    for (Buf& b : wide_ws_) b.release();
     for (Buf& b : opw_) b.release();
     X_ = nullptr;
     compacted_ = true;
@@ -387,6 +388,7 @@ class Model final : public Handle, private StreamOps {
                            &op_sample_};
     for (Buf* b : ops_bufs()) v.push_back(b);
     for (Buf& b : svd_ws_) v.push_back(&b);
+    for (Buf& b : wide_ws_) v.push_back(&b);
     return v;
   }
 
@@ -633,6 +635,17 @@ class Model final : public Handle, private StreamOps {
     const bool f64 = f64_products_;
     const int xs = xw_splits(n_, p_, bmax, sizeof(T) == 4 && !f64);
     const int ls = xtw_splits(n_, p_, bmax, sizeof(T) == 4 && !f64);
+    // Fits with many predictors run their refresh and update as several
+    // launches whose products use the whole device: one work-group walking
+    // p-long vectors and p x q matrices streams at tens of GB/s (both took
+    // 1.4 ms per component at p = 20000, as long as a pass over an 8 GB X;
+    // the whole 1e5 x 2e4 x 25 fit 145 -> 85 ms).
+    // FASTPLS_HIP_WIDE_P, when set, replaces the p threshold.
+    {
+      long long threshold = kWideP;
+      if (const char* v = std::getenv("FASTPLS_HIP_WIDE_P")) threshold = std::atoll(v);
+      wide_ = !implicit_ && !rank_one_ && !cache_gram_ && p_ >= threshold;
+    }
     ts_part_.alloc(sizeof(double) * (size_t)xs * n_ * bmax);
     ls_part_.alloc(sizeof(double) * (size_t)ls * p_ * bmax);
 
@@ -668,7 +681,9 @@ class Model final : public Handle, private StreamOps {
     FPLS_HIP(hipEventRecord(sc0_.e, stream_));
     for (size_t s = 0; s < steps_ref.size(); ++s) {
       const int b = steps_ref[s].b;
-      if (!implicit_) {
+      if (wide_) {
+        wide_refresh(steps_ref[s], right_gram);
+      } else if (!implicit_) {
         launch_simpls_refresh(fitdev_.as<SimplsFitDev>(), 1, int(s), stream_);
       } else {
         // simpls.hpp:824-858: candidates from the projected operator
@@ -705,9 +720,100 @@ class Model final : public Handle, private StreamOps {
       launch_xtw_single<T>(xt, stream_, f64);
       // response loadings without S0: Ỹᵀ t (scaled by 1/|t| in the update)
       if (implicit_) tproduct<double>(Yc_.d(), n_, q_, nullptr, t_raw_.d(), n_, b, qraw_.d());
-      launch_simpls_update(fitdev_.as<SimplsFitDev>(), 1, int(s), stream_);
+      if (wide_)
+        wide_update(steps_ref[s], ls, (long long)p_ * bmax, right_gram);
+      else
+        launch_simpls_update(fitdev_.as<SimplsFitDev>(), 1, int(s), stream_);
     }
     FPLS_HIP(hipEventRecord(sc1_.e, stream_));
+  }
+
+  // refresh_directions (simpls.hpp:226-368) of a wide fit through the
+  // multi-work-group products: the q-side power iterations on the right Gram
+  // (or on S when the right Gram is not used), the sample S Z, its CholQR2
+  // basis Qb, Bm = Qbᵀ S, the top-b eigenvectors of Bm Bmᵀ (one work-group)
+  // and the candidates Qb v, divided by the scales for the score pass.
+  void wide_refresh(const RefreshStep& st, bool right_gram) {
+    const int w = st.w, b = st.b;
+    int* state = state_.as<int>();
+    int* flags = state + 2;
+    Buf &Z = wide_ws_[0], &Y2 = wide_ws_[1], &sample = wide_ws_[2], &basis = wide_ws_[3],
+        &Bm = wide_ws_[4], &vtop = wide_ws_[5], &scratch = wide_ws_[6];
+    Z.reserve(sizeof(double) * (size_t)q_ * w);
+    Y2.reserve(sizeof(double) * (size_t)q_ * w);
+    sample.reserve(sizeof(double) * (size_t)p_ * w);
+    basis.reserve(sizeof(double) * (size_t)p_ * w);
+    Bm.reserve(sizeof(double) * (size_t)w * q_);
+    vtop.reserve(sizeof(double) * (size_t)w * b);
+    scratch.reserve(sizeof(double) * ((size_t)3 * w * w + w));
+    const double* omega = omega_.d() + st.omega_off;
+    if (right_gram) {
+      FPLS_HIP(hipMemcpyAsync(Z.d(), omega, sizeof(double) * (size_t)q_ * w,
+                              hipMemcpyDeviceToDevice, stream_));
+      for (int it = 0; it < power_; ++it) {
+        product<double>(G_.d(), q_, q_, nullptr, Z.d(), q_, w, Y2.d());
+        qr_tall(Y2.d(), q_, w, Z.d(), flags);
+      }
+      if (power_ == 0) {
+        FPLS_HIP(hipMemcpyAsync(Y2.d(), Z.d(), sizeof(double) * (size_t)q_ * w,
+                                hipMemcpyDeviceToDevice, stream_));
+        qr_tall(Y2.d(), q_, w, Z.d(), flags);
+      }
+      product<double>(S_.d(), p_, q_, nullptr, Z.d(), q_, w, sample.d());
+    } else {
+      product<double>(S_.d(), p_, q_, nullptr, omega, q_, w, sample.d());
+      for (int it = 0; it < power_; ++it) {
+        qr_tall(sample.d(), p_, w, basis.d(), flags);
+        tproduct<double>(S_.d(), p_, q_, nullptr, basis.d(), p_, w, Y2.d());
+        qr_tall(Y2.d(), q_, w, Z.d(), flags);
+        product<double>(S_.d(), p_, q_, nullptr, Z.d(), q_, w, sample.d());
+      }
+    }
+    qr_tall(sample.d(), p_, w, basis.d(), flags);
+    tproduct<double>(basis.d(), p_, w, nullptr, S_.d(), p_, q_, Bm.d());
+    launch_refresh_gate(state, stream_);
+    launch_refresh_eigen(Bm.d(), w, q_, b, vtop.d(), scratch.d(), state, stream_);
+    product<double>(basis.d(), p_, w, nullptr, vtop.d(), w, b, cand_.d());
+    launch_div_rows(cand_.d(), sx_.d(), p_, b, cand_scaled_.d(), stream_);
+  }
+
+  // The component update (simpls.hpp:600-756, k_simpls_update's steps) of a
+  // wide fit, one candidate of the block at a time: |t|² and the rescaled
+  // T, R and loading; q = S0ᵀ r; the loading's projection on the earlier
+  // loadings removed (once, twice when re-orthogonalising) and normalised;
+  // the deflation row vᵀ S, S -= v row and the right-Gram downdate.
+  void wide_update(const RefreshStep& st, int splits, long long slab, bool right_gram) {
+    int* state = state_.as<int>();
+    Buf &scal = wide_ws_[7], &proj = wide_ws_[8], &row = wide_ws_[9], &dv = wide_ws_[10];
+    scal.reserve(sizeof(double) * 2);
+    proj.reserve(sizeof(double) * (size_t)std::max(A_, 1));
+    row.reserve(sizeof(double) * (size_t)q_);
+    dv.reserve(sizeof(double) * (size_t)p_);
+    for (int j = 0; j < st.b; ++j) {
+      const int comp = st.a + j;
+      if (comp >= A_) break;
+      const double* t = t_raw_.d() + (size_t)j * n_;
+      double* Rc = R_.d() + (size_t)comp * p_;
+      double* Vc = V_.d() + (size_t)comp * p_;
+      tproduct<double>(t, n_, 1, nullptr, t, n_, 1, scal.d());
+      launch_wide_component(n_, p_, t, cand_.d() + (size_t)j * p_, ls_part_.d() + (size_t)j * p_,
+                            splits, slab, sx_.d(), scal.d(),
+                            Tsc_.empty() ? nullptr : Tsc_.d() + (size_t)comp * n_, Rc, Vc, state,
+                            stream_);
+      tproduct<double>(S0_.d(), p_, q_, nullptr, Rc, p_, 1, Q_.d() + (size_t)comp * q_);
+      if (comp > 0) {
+        for (int pass = 0; pass < (fitdev_host_.reorth ? 2 : 1); ++pass) {
+          tproduct<double>(V_.d(), p_, comp, nullptr, Vc, p_, 1, proj.d());
+          product<double>(V_.d(), p_, comp, nullptr, proj.d(), comp, 1, dv.d());
+          launch_wide_subtract(Vc, dv.d(), p_, state, stream_);
+        }
+      }
+      tproduct<double>(Vc, p_, 1, nullptr, Vc, p_, 1, scal.d() + 1);
+      launch_wide_unit(Vc, p_, scal.d() + 1, state, stream_);
+      tproduct<double>(S_.d(), p_, q_, nullptr, Vc, p_, 1, row.d());
+      launch_wide_deflate(S_.d(), p_, q_, Vc, row.d(), state, stream_);
+      launch_wide_finish(right_gram ? G_.d() : nullptr, q_, row.d(), comp, state, stream_);
+    }
   }
 
   // ----------------------------------------------------------- PLS-SVD
@@ -1069,6 +1175,12 @@ class Model final : public Handle, private StreamOps {
   int scores_rows_ = 0;
   bool implicit_ = false, rank_one_ = false;
   bool cache_gram_ = false;   // SIMPLS loadings from the cached X_sᵀX_s (xgram_)
+  bool wide_ = false;         // multi-launch refresh and update (many predictors)
+  // measured crossover (FP32 fits, q 3..17): the one-work-group route wins
+  // at p = 1000 (4.1 against 5.0 ms), the multi-launch route at p = 2186
+  // (11.8 against 16.5 ms) and beyond
+  static constexpr long long kWideP = 1536;
+  Buf wide_ws_[11];
   bool keep_scores_ = true;   // training scores kept after the fit
   Buf xgram_;
   // matrix-free route workspaces

@@ -62,6 +62,22 @@ long long simpls_scratch_doubles(int p, int q, int wmax);
 
 void launch_simpls_refresh(const SimplsFitDev* fits, int count, int step, hipStream_t s);
 void launch_simpls_update(const SimplsFitDev* fits, int count, int step, hipStream_t s);
+// Steps of the multi-launch refresh and update of a wide single fit (the
+// products between them run through the xw / xtw passes; state is the fit's
+// [completed, stopped, flags]).
+void launch_refresh_eigen(const double* Bm, int w, int q, int b, double* vtop, double* scratch,
+                          int* state, hipStream_t s);
+void launch_refresh_gate(int* state, hipStream_t s);
+void launch_wide_component(long long n, int p, const double* t, const double* r,
+                           const double* part, int splits, long long slab, const double* sx,
+                           const double* ss, double* Tc, double* Rc, double* Vc, int* state,
+                           hipStream_t s);
+void launch_wide_subtract(double* v, const double* d, int p, const int* state, hipStream_t s);
+void launch_wide_unit(double* v, int p, const double* vv, int* state, hipStream_t s);
+void launch_wide_deflate(double* S, int p, int q, const double* v, const double* row,
+                         const int* state, hipStream_t s);
+void launch_wide_finish(double* G, int q, const double* row, int comp, int* state,
+                        hipStream_t s);
 
 // ---- single-work-group helpers for the PLS-SVD randomized SVD
 // Householder QR: A (m x n) destroyed, Q (m x n) written. flags |= decomposition.

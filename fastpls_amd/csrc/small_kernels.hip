@@ -469,6 +469,138 @@ __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits,
   }
 }
 
+// ------------------------------------------- SIMPLS steps of wide fits
+// The refresh and update of one SIMPLS fit whose p is large, as several
+// launches whose products run through the multi-work-group xw / xtw passes
+// (one work-group walking p-long vectors streams at tens of GB/s). These
+// kernels are the steps between those products; they read the fit's scalars
+// (|t|², |v|²) from device memory, so the host never waits, and do nothing
+// once the fit has stopped (state[1]).
+
+// Top-b eigenvectors (largest first) of Bm Bmᵀ for Bm (w x q): the eigen step
+// of refresh_directions (simpls.hpp:322-368). vtop: w x b. scratch: 3 w² + w
+// doubles (used when the problem does not fit in LDS).
+__global__ __launch_bounds__(kWG) void k_refresh_eigen(const double* Bm, int w, int q, int b,
+                                                       double* vtop, double* scratch,
+                                                       int* state) {
+  __shared__ WgScratch sh;
+  __shared__ double stage[2 * wg::kStageDoubles];
+  if (state[1] != 0) return;
+  const bool lds = 3 * w * w + w <= 2 * wg::kStageDoubles;
+  double* base = lds ? stage : scratch;
+  double* gE = base;
+  double* vE = base + w * w;
+  double* tE = base + 2 * w * w;
+  double* eE = base + 3 * w * w;
+  wg::gemm(w, w, q, 1.0, Bm, w, false, Bm, w, true, 0.0, gE, w);
+  const bool ok = wg::jacobi_eigen(gE, vE, w, eE, sh.cs, sh.sn, sh.pa, sh.pb, sh.red, tE);
+  for (int e = threadIdx.x; e < w * b; e += kWG) {
+    const int i = e % w, c = e / w;
+    vtop[e] = vE[i + (long long)(w - 1 - c) * w];
+  }
+  if (!ok) {
+    set_flag(state, kFlagDecomposition);
+    if (threadIdx.x == 0) state[1] = 1;
+  }
+}
+
+// A QR of the refresh flagged a decomposition failure: the fit stops.
+__global__ void k_refresh_gate(int* state) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && (state[2] & kFlagDecomposition) != 0) state[1] = 1;
+}
+
+// Component `comp` from its raw score t (n), candidate r (p) and loading
+// split partials: with inv = 1 / |t| (|t|² at *ss), T[:, comp] = t inv,
+// R[:, comp] = r inv, v = (sum of the splits) / sx · inv (simpls.hpp:600-660).
+// A zero or non-finite |t| stops the fit (k_simpls_update's rule).
+__global__ __launch_bounds__(kWG) void k_wide_component(long long n, int p, const double* t,
+                                                        const double* r, const double* part,
+                                                        int splits, long long slab,
+                                                        const double* sx, const double* ss,
+                                                        double* Tc, double* Rc, double* Vc,
+                                                        int* state) {
+  if (state[1] != 0) return;
+  const double tn = sqrt(*ss);
+  if (!(tn > 0.0) || !isfinite(tn)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      atomicOr(&state[2], isfinite(tn) ? kFlagZeroNorm : kFlagNonFinite);
+      state[1] = 1;
+    }
+    return;
+  }
+  const double inv = 1.0 / tn;
+  const long long stride = (long long)gridDim.x * kWG;
+  for (long long i = (long long)blockIdx.x * kWG + threadIdx.x; i < n; i += stride)
+    if (Tc) Tc[i] = t[i] * inv;
+  for (long long i = (long long)blockIdx.x * kWG + threadIdx.x; i < p; i += stride) {
+    double acc = 0.0;
+    for (int sp = 0; sp < splits; ++sp) acc += part[i + (long long)sp * slab];
+    Rc[i] = r[i] * inv;
+    Vc[i] = acc / sx[i] * inv;
+  }
+}
+
+// v -= d (the projection of v on the earlier loadings).
+__global__ __launch_bounds__(kWG) void k_wide_subtract(double* v, const double* d, int p,
+                                                       const int* state) {
+  if (state[1] != 0) return;
+  for (long long i = (long long)blockIdx.x * kWG + threadIdx.x; i < p;
+       i += (long long)gridDim.x * kWG)
+    v[i] -= d[i];
+}
+
+// v /= |v| (|v|² at *vv); a zero or non-finite norm stops the fit.
+__global__ __launch_bounds__(kWG) void k_wide_unit(double* v, int p, const double* vv,
+                                                   int* state) {
+  if (state[1] != 0) return;
+  const double vn = sqrt(*vv);
+  if (!(vn > 0.0) || !isfinite(vn)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      atomicOr(&state[2], isfinite(vn) ? kFlagZeroNorm : kFlagNonFinite);
+      state[1] = 1;
+    }
+    return;
+  }
+  const double inv = 1.0 / vn;
+  for (long long i = (long long)blockIdx.x * kWG + threadIdx.x; i < p;
+       i += (long long)gridDim.x * kWG)
+    v[i] *= inv;
+}
+
+// S (p x q) -= v rowᵀ.
+__global__ __launch_bounds__(kWG) void k_wide_deflate(double* S, int p, int q, const double* v,
+                                                      const double* row, const int* state) {
+  if (state[1] != 0) return;
+  const long long total = (long long)p * q;
+  for (long long e = (long long)blockIdx.x * kWG + threadIdx.x; e < total;
+       e += (long long)gridDim.x * kWG) {
+    const long long c = e / p, i = e - c * p;
+    S[e] -= v[i] * row[c];
+  }
+}
+
+// G (q x q) -= row rowᵀ, kept exactly symmetric, and component comp done.
+__global__ __launch_bounds__(kWG) void k_wide_finish(double* G, int q, const double* row,
+                                                     int comp, int* state) {
+  if (state[1] != 0) return;
+  if (G) {
+    for (int e = threadIdx.x; e < q * q; e += kWG) {
+      const int i = e % q, c = e / q;
+      G[e] -= row[i] * row[c];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < q * q; e += kWG) {
+      const int i = e % q, c = e / q;
+      if (i < c) {
+        const double v = 0.5 * (G[i + c * q] + G[c + i * q]);
+        G[i + c * q] = v;
+        G[c + i * q] = v;
+      }
+    }
+  }
+  if (threadIdx.x == 0) state[0] = comp + 1;
+}
+
 // --------------------------------------------------------------- tall QR
 // Householder QR; a matrix of up to 6144 doubles is factorised in LDS.
 template<int NT>
@@ -1011,6 +1143,56 @@ void launch_simpls_update(const SimplsFitDev* fits, int count, int step, hipStre
   if (count <= 0) return;
   hipLaunchKernelGGL(k_simpls_update, dim3(count), dim3(kWG), 0, s, fits, count, step);
   FPLS_LAUNCHED("k_simpls_update");
+}
+
+void launch_refresh_eigen(const double* Bm, int w, int q, int b, double* vtop, double* scratch,
+                          int* state, hipStream_t s) {
+  hipLaunchKernelGGL(k_refresh_eigen, dim3(1), dim3(kWG), 0, s, Bm, w, q, b, vtop, scratch,
+                     state);
+  FPLS_LAUNCHED("k_refresh_eigen");
+}
+
+void launch_refresh_gate(int* state, hipStream_t s) {
+  hipLaunchKernelGGL(k_refresh_gate, dim3(1), dim3(64), 0, s, state);
+  FPLS_LAUNCHED("k_refresh_gate");
+}
+
+namespace {
+unsigned wide_grid(long long count) {
+  return unsigned(std::max<long long>(1, std::min<long long>(1024, (count + kWG - 1) / kWG)));
+}
+}  // namespace
+
+void launch_wide_component(long long n, int p, const double* t, const double* r,
+                           const double* part, int splits, long long slab, const double* sx,
+                           const double* ss, double* Tc, double* Rc, double* Vc, int* state,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_wide_component, dim3(wide_grid(std::max<long long>(n, p))), dim3(kWG), 0,
+                     s, n, p, t, r, part, splits, slab, sx, ss, Tc, Rc, Vc, state);
+  FPLS_LAUNCHED("k_wide_component");
+}
+
+void launch_wide_subtract(double* v, const double* d, int p, const int* state, hipStream_t s) {
+  hipLaunchKernelGGL(k_wide_subtract, dim3(wide_grid(p)), dim3(kWG), 0, s, v, d, p, state);
+  FPLS_LAUNCHED("k_wide_subtract");
+}
+
+void launch_wide_unit(double* v, int p, const double* vv, int* state, hipStream_t s) {
+  hipLaunchKernelGGL(k_wide_unit, dim3(wide_grid(p)), dim3(kWG), 0, s, v, p, vv, state);
+  FPLS_LAUNCHED("k_wide_unit");
+}
+
+void launch_wide_deflate(double* S, int p, int q, const double* v, const double* row,
+                         const int* state, hipStream_t s) {
+  hipLaunchKernelGGL(k_wide_deflate, dim3(wide_grid((long long)p * q)), dim3(kWG), 0, s, S, p, q,
+                     v, row, state);
+  FPLS_LAUNCHED("k_wide_deflate");
+}
+
+void launch_wide_finish(double* G, int q, const double* row, int comp, int* state,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_wide_finish, dim3(1), dim3(kWG), 0, s, G, q, row, comp, state);
+  FPLS_LAUNCHED("k_wide_finish");
 }
 
 void launch_qr(double* A, int m, int n, double* Q, int* flags, hipStream_t s) {
