@@ -746,6 +746,18 @@ class Model final : public Handle, private StreamOps {
     Bm.reserve(sizeof(double) * (size_t)w * q_);
     vtop.reserve(sizeof(double) * (size_t)w * b);
     scratch.reserve(sizeof(double) * ((size_t)3 * w * w + w));
+    if (right_gram && w == q_) {
+      // a sketch as wide as the response count spans ℝ^q: the directions are
+      // the top-b left singular vectors of S, S v_c for the eigenvectors v_c
+      // of SᵀS formed from the current S (k_simpls_refresh's square-sketch
+      // step; the update divides each candidate by its score norm, so their
+      // scale does not matter)
+      tproduct<double>(S_.d(), p_, q_, nullptr, S_.d(), p_, q_, Bm.d());
+      launch_refresh_eigen(Bm.d(), w, q_, b, vtop.d(), scratch.d(), state, stream_, true);
+      product<double>(S_.d(), p_, q_, nullptr, vtop.d(), q_, b, cand_.d());
+      launch_div_rows(cand_.d(), sx_.d(), p_, b, cand_scaled_.d(), stream_);
+      return;
+    }
     const double* omega = omega_.d() + st.omega_off;
     if (right_gram) {
       FPLS_HIP(hipMemcpyAsync(Z.d(), omega, sizeof(double) * (size_t)q_ * w,

@@ -65,8 +65,9 @@ void launch_simpls_update(const SimplsFitDev* fits, int count, int step, hipStre
 // Steps of the multi-launch refresh and update of a wide single fit (the
 // products between them run through the xw / xtw passes; state is the fit's
 // [completed, stopped, flags]).
+// Top-b eigenvectors of Bm Bmᵀ (Bm w x q), or of Bm itself (w x w) with gram.
 void launch_refresh_eigen(const double* Bm, int w, int q, int b, double* vtop, double* scratch,
-                          int* state, hipStream_t s);
+                          int* state, hipStream_t s, bool gram = false);
 void launch_refresh_gate(int* state, hipStream_t s);
 void launch_wide_component(long long n, int p, const double* t, const double* r,
                            const double* part, int splits, long long slab, const double* sx,

@@ -204,6 +204,51 @@ __global__ __launch_bounds__(kWG) void k_cholqr2_small(const double* A, int m, i
 }
 
 // ----------------------------------------------------------- SIMPLS refresh
+// A sketch as wide as the response count (w == q, so q <= p) spans ℝ^q
+// whatever its values: every Z of refresh_directions (simpls.hpp:236-299) is
+// square and orthogonal, range(S Z) = range(S), and the directions it returns
+// (simpls.hpp:301-366) are the top-b left singular vectors of S. They are
+// formed here from the eigen-decomposition of SᵀS computed from the current S
+// (not from the deflated right Gram, whose rank-one downdates lose relative
+// accuracy as S shrinks): candidate c = S v_c / |S v_c| for the eigenvector
+// v_c of the c-th largest eigenvalue. Returns false, having written nothing
+// the sketch route needs, when the eigen-decomposition fails or a candidate
+// norm is zero or non-finite; the caller then runs the sketch route, which
+// reports such data as the reference does. Uniform across the work-group.
+__device__ bool square_sketch_refresh(const SimplsFitDev& f, int p, int q, int b, TallQrLds& tq,
+                                      WgScratch& sh) {
+  double* Gs = tq.gm;  // q x q, then the top-b eigenvectors (q x b)
+  double* Vs = tq.ri;
+  tall_tn(q, q, p, f.S, p, f.S, p, Gs, q, tq.stage);
+  if (!wg::jacobi_eigen(Gs, Vs, q, tq.xb, sh.cs, sh.sn, sh.pa, sh.pb, sh.red, tq.lc)) return false;
+  for (int e = threadIdx.x; e < q * b; e += kWG) {
+    const int l = e % q, c = e / q;
+    Gs[e] = Vs[l + (q - 1 - c) * q];
+  }
+  __syncthreads();
+  tall_times_small(p, b, q, f.S, p, Gs, q, f.cand, p, tq.stage);
+  for (int c = 0; c < b; ++c) {
+    const double* r = f.cand + (long long)c * p;
+    double part = 0.0;
+    for (int i = threadIdx.x; i < p; i += kWG) part = fma(r[i], r[i], part);
+    const double ss = wg_sum_n<kWG>(part, sh.red);
+    if (!isfinite(ss) || !(ss > 0.0)) return false;
+    if (threadIdx.x == 0) sh.dots[c] = 1.0 / sqrt(ss);
+  }
+  __syncthreads();
+  for (int c = 0; c < b; ++c) {
+    const double inv = sh.dots[c];
+    double* r = f.cand + (long long)c * p;
+    double* rs = f.cand_scaled + (long long)c * p;
+    for (int i = threadIdx.x; i < p; i += kWG) {
+      const double v = r[i] * inv;
+      r[i] = v;
+      rs[i] = v / f.sx[i];
+    }
+  }
+  return true;
+}
+
 // simpls.hpp:226-368 (refresh_directions), right-Gram and plain branches.
 __global__ __launch_bounds__(kWG) void k_simpls_refresh(const SimplsFitDev* fits, int count,
                                                         int step) {
@@ -227,6 +272,7 @@ __global__ __launch_bounds__(kWG) void k_simpls_refresh(const SimplsFitDev* fits
   double* ev = base + L.evals;
   const double* omega = f.omega + st.omega_off;
   bool ok = true;
+  if (w == q && q <= 32 && square_sketch_refresh(f, p, q, b, tq, sh)) return;
 
   // right-hand (q x w) QRs: whole CholQR2 in LDS when small, else Householder
   auto qside_qr = [&](double* Y, double* Zo) -> bool {
@@ -478,11 +524,12 @@ __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits,
 // once the fit has stopped (state[1]).
 
 // Top-b eigenvectors (largest first) of Bm Bmᵀ for Bm (w x q): the eigen step
-// of refresh_directions (simpls.hpp:322-368). vtop: w x b. scratch: 3 w² + w
+// of refresh_directions (simpls.hpp:322-368). With `gram` set, Bm is already
+// the symmetric w x w matrix to decompose. vtop: w x b. scratch: 3 w² + w
 // doubles (used when the problem does not fit in LDS).
 __global__ __launch_bounds__(kWG) void k_refresh_eigen(const double* Bm, int w, int q, int b,
                                                        double* vtop, double* scratch,
-                                                       int* state) {
+                                                       int* state, int gram) {
   __shared__ WgScratch sh;
   __shared__ double stage[2 * wg::kStageDoubles];
   if (state[1] != 0) return;
@@ -492,7 +539,8 @@ __global__ __launch_bounds__(kWG) void k_refresh_eigen(const double* Bm, int w, 
   double* vE = base + w * w;
   double* tE = base + 2 * w * w;
   double* eE = base + 3 * w * w;
-  wg::gemm(w, w, q, 1.0, Bm, w, false, Bm, w, true, 0.0, gE, w);
+  if (gram) wg::copy(w, w, Bm, w, gE, w);
+  else wg::gemm(w, w, q, 1.0, Bm, w, false, Bm, w, true, 0.0, gE, w);
   const bool ok = wg::jacobi_eigen(gE, vE, w, eE, sh.cs, sh.sn, sh.pa, sh.pb, sh.red, tE);
   for (int e = threadIdx.x; e < w * b; e += kWG) {
     const int i = e % w, c = e / w;
@@ -1146,9 +1194,9 @@ void launch_simpls_update(const SimplsFitDev* fits, int count, int step, hipStre
 }
 
 void launch_refresh_eigen(const double* Bm, int w, int q, int b, double* vtop, double* scratch,
-                          int* state, hipStream_t s) {
+                          int* state, hipStream_t s, bool gram) {
   hipLaunchKernelGGL(k_refresh_eigen, dim3(1), dim3(kWG), 0, s, Bm, w, q, b, vtop, scratch,
-                     state);
+                     state, gram ? 1 : 0);
   FPLS_LAUNCHED("k_refresh_eigen");
 }
 

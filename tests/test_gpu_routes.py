@@ -14,7 +14,19 @@
   above 256 MiB drops its training scores after the fit. Exporting T then
   fails with a message; LDA classification and predictions equal those of
   the same fit with its scores retained, bitwise.
+* The square-sketch refresh: when every refresh is as wide as the response
+  count (q <= block + oversample, so the sketch spans ℝ^q), the directions
+  are the top left singular vectors of S, taken from the eigen-decomposition
+  of SᵀS without the sketch, in the one-work-group refresh and in the
+  multi-launch refresh of wide fits (FASTPLS_HIP_WIDE_P moves the threshold).
+  Each fit is compared at every prefix with the oracle, which draws the sketch
+  and runs the power iterations and QRs (1e-9 at precision 64, 1e-5 at
+  precision 32 on the same FP32 values), and a fit with another seed is equal
+  bitwise (no Gaussian draw is used). The classification fit takes candidate
+  blocks of 6 (n p K >= 5e8) with K = 8 classes.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -76,3 +88,51 @@ def test_classification_scores_not_retained(hip):
                                  oversample=4, power=1, seed=2, retain_scores=False)
     assert small.export("T").shape == (2000, A)
     small.close()
+
+
+def _fit_route(fp, X, Y=None, wide=None, **kw):
+    os.environ["FASTPLS_HIP_WIDE_P"] = str(wide)
+    try:
+        if Y is None:
+            return fp.ResidentModel.fit(X, **kw)
+        return fp.ResidentModel.fit(X, Y, **kw)
+    finally:
+        os.environ.pop("FASTPLS_HIP_WIDE_P", None)
+
+
+def _prefixes_match(m, ref, A, tol):
+    R, Q, T = (m.export(k).astype(np.float64) for k in ("R", "Q", "T"))
+    for k in range(1, A + 1):
+        assert rel(R[:, :k] @ Q[:, :k].T, ref["R"][:, :k] @ ref["Q"][:, :k].T) < tol, k
+        assert rel(T[:, :k] @ Q[:, :k].T, ref["T"][:, :k] @ ref["Q"][:, :k].T) < tol, k
+
+
+@pytest.mark.parametrize("wide", [1, 10 ** 9])
+@pytest.mark.parametrize("precision", [64, 32])
+def test_square_sketch_refresh(hip, oracle, wide, precision):
+    fp = hip[0]
+    dt = np.float64 if precision == 64 else np.float32
+    # all 14 components within the 16 factors: past the factors the fit is
+    # ill-conditioned (the oracle differs from itself by ~1e-7 between seeds)
+    X, Y = latent_data(1500, 400, 5, factors=16, noise=0.1, seed=74)
+    X, Y = X.astype(dt), Y.astype(dt)
+    kw = dict(precision=precision, components=14, oversample=6, power=2)
+    m = _fit_route(fp, X, Y, wide=wide, seed=5, **kw)
+    o = oracle.simpls_fit(X.astype(np.float64), Y.astype(np.float64), ncomp=14, oversample=6,
+                          power=2, seed=5)
+    _prefixes_match(m, o, 14, TOL[precision])
+    other = _fit_route(fp, X, Y, wide=wide, seed=999, **kw)
+    np.testing.assert_array_equal(other.export("R"), m.export("R"))
+    m.close(); other.close()
+
+
+@pytest.mark.parametrize("wide", [1, 10 ** 9])
+def test_square_sketch_refresh_classification_blocks(hip, oracle, wide):
+    fp = hip[0]
+    X, lab = class_data(12000, 5300, 8, latent=7, seed=75, spread=1.5)
+    kw = dict(labels=lab, classes=8, precision=64, components=6, oversample=4, power=1)
+    m = _fit_route(fp, X, wide=wide, seed=7, **kw)
+    assert m.controls()["block"] == 6
+    o = oracle.simpls_fit(X, labels=lab, classes=8, ncomp=6, oversample=4, power=1, seed=7)
+    _prefixes_match(m, o, 6, TOL[64])
+    m.close()

@@ -278,6 +278,22 @@ def test_simpls_width_covers_column_space_equals_exact_svd_direction(oracle):
     assert abs(abs(r @ u) - 1.0) < 1e-12
 
 
+def test_simpls_square_sketch_independent_of_sketch(oracle):
+    # every refresh has width q (q <= block + oversample): the sketch spans
+    # ℝ^q, so the whole fit is independent of the Gaussian draws and of the
+    # power iterations (the premise of the square-sketch refresh); with a
+    # narrower sketch it is not (rounding differences reach ~1e-10 in the
+    # components past the data's 6 factors)
+    X, Y = latent_data(200, 40, 4, factors=6, noise=0.2, seed=11)
+    fits = [oracle.simpls_fit(X, Y, ncomp=12, oversample=o, power=pw, seed=sd)
+            for o, pw, sd in ((4, 2, 3), (4, 0, 77), (9, 5, 1234))]
+    for k in range(1, 13):
+        B = [f["R"][:, :k] @ f["Q"][:, :k].T for f in fits]
+        assert rel(B[1], B[0]) < 1e-8 and rel(B[2], B[0]) < 1e-8, k
+    narrow = [oracle.simpls_fit(X, Y, ncomp=3, oversample=0, power=0, seed=sd) for sd in (3, 4)]
+    assert rel(narrow[0]["R"] @ narrow[0]["Q"].T, narrow[1]["R"] @ narrow[1]["Q"].T) > 1e-6
+
+
 def test_topk_tie_rule(oracle):
     vals = np.array([[1.0, 3.0, 3.0, 2.0], [5.0, 5.0, 5.0, 5.0]])
     out = oracle.topk(vals, 3)
