@@ -56,20 +56,21 @@ __device__ inline void set_flag(int* state, int flag) {
   if (threadIdx.x == 0) atomicOr(&state[2], flag);
 }
 
-// dst[i] = src[i] * factor for i < len by one work-group, four elements per
-// thread loaded before any is stored (dst may alias src), so the loads of a
-// pass are in flight together.
+// dst[i] = src[i] * factor for i < len by one work-group, sixteen elements
+// per thread loaded before any is stored (dst may alias src), so the loads of
+// a pass are in flight together.
 __device__ inline void scale_copy(double* dst, const double* src, long long len, double factor) {
-  for (long long i0 = threadIdx.x; i0 < len; i0 += 4 * kWG) {
-    double v[4];
+  constexpr int U = 16;
+  for (long long i0 = threadIdx.x; i0 < len; i0 += U * kWG) {
+    double v[U];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const long long i = i0 + u * kWG;
-      v[u] = i < len ? src[i] : 0.0;
+    for (int u = 0; u < U; ++u) {
+      const long long i = i0 + (long long)u * kWG;
+      v[u] = src[i < len ? i : i0];
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const long long i = i0 + u * kWG;
+    for (int u = 0; u < U; ++u) {
+      const long long i = i0 + (long long)u * kWG;
       if (i < len) dst[i] = v[u] * factor;
     }
   }
@@ -328,6 +329,103 @@ __global__ __launch_bounds__(kWG) void k_simpls_refresh(const SimplsFitDev* fits
   }
 }
 
+// Streaming helpers of the update: one work-group walks p- or n-long vectors,
+// so each thread keeps several independent loads in flight (indices past the
+// end are clamped to a valid element and their values discarded).
+constexpr int kStreamU = 32;
+
+// Σ v[i]² over n doubles; every thread receives it.
+__device__ inline double sumsq_stream(const double* v, long long n, double* red) {
+  double s[kStreamU];
+#pragma unroll
+  for (int u = 0; u < kStreamU; ++u) s[u] = 0.0;
+  for (long long i0 = threadIdx.x; i0 < n; i0 += kStreamU * kWG) {
+    double x[kStreamU];
+#pragma unroll
+    for (int u = 0; u < kStreamU; ++u) {
+      const long long i = i0 + (long long)u * kWG;
+      x[u] = v[i < n ? i : i0];
+    }
+#pragma unroll
+    for (int u = 0; u < kStreamU; ++u)
+      if (i0 + (long long)u * kWG < n) s[u] = fma(x[u], x[u], s[u]);
+  }
+  double tot = 0.0;
+#pragma unroll
+  for (int u = 0; u < kStreamU; ++u) tot += s[u];
+  return wg_sum_n<kWG>(tot, red);
+}
+
+// One pass over the rows of the loading v (p, in place): v -= V[:, :comp] proj
+// (nothing when comp is 0; the sum over the earlier loadings in index order),
+// returning this thread's partial Σ v² of the result when `norm` is set. Four
+// rows per thread, sixteen loadings at a time.
+__device__ __forceinline__ double orth_pass(double* v, int p, const double* V, const double* proj,
+                                            int comp, bool norm) {
+  constexpr int L = 16, R = 4;
+  double ss = 0.0;
+  for (int i0 = threadIdx.x; i0 < p; i0 += R * kWG) {
+    int ir[R];
+    double x[R], s[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      ir[r] = i0 + r * kWG < p ? i0 + r * kWG : i0;
+      x[r] = v[ir[r]];
+      s[r] = 0.0;
+    }
+    for (int l0 = 0; l0 < comp; l0 += L) {
+      double a[R][L], pl[L];
+#pragma unroll
+      for (int u = 0; u < L; ++u) {
+        const int l = l0 + u < comp ? l0 + u : comp - 1;
+        pl[u] = l0 + u < comp ? proj[l] : 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[r][u] = V[ir[r] + (long long)l * p];
+      }
+#pragma unroll
+      for (int u = 0; u < L; ++u)
+#pragma unroll
+        for (int r = 0; r < R; ++r) s[r] = fma(a[r][u], pl[u], s[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const bool in = i0 + r * kWG < p;
+      if (comp > 0) {
+        x[r] -= s[r];
+        if (in) v[ir[r]] = x[r];
+      }
+      if (norm && in) ss = fma(x[r], x[r], ss);
+    }
+  }
+  return ss;
+}
+
+// S (p x q, in place) -= (v vs) (row rs)ᵀ element-wise: sixteen entries per
+// thread loaded (with their v and row values) before any is stored.
+__device__ __forceinline__ void deflate_elementwise(double* S, int p, int q, const double* v,
+                                                   double vs, const double* row) {
+  constexpr int U = 16;
+  const long long len = (long long)p * q;
+  for (long long e0 = threadIdx.x; e0 < len; e0 += U * kWG) {
+    double sv[U], vv[U], rr[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      long long e = e0 + (long long)u * kWG;
+      if (e >= len) e = e0;
+      const int c = int(e / p);
+      const int i = int(e - (long long)c * p);
+      sv[u] = S[e];
+      vv[u] = v[i];
+      rr[u] = row[c];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long e = e0 + (long long)u * kWG;
+      if (e < len) S[e] = sv[u] - (vv[u] * vs) * rr[u];
+    }
+  }
+}
+
 // ------------------------------------------------------------ SIMPLS update
 // simpls.hpp:588-756: per candidate of the block, in order.
 __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits, int count,
@@ -342,11 +440,22 @@ __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits,
   const RefreshLayout L(p, q, st.w, f.A);
   double* proj = f.scratch + L.proj;
   double* row = f.scratch + L.row;
-  // out (m) = Xᵀ v for X (p x m): one thread per row of X when m <= 64
+  // out (m) = Xᵀ v for X (p x m): one thread per row of X, 32 columns of X
+  // per pass (more partial sums per thread spill registers) when m <= 128
   auto dots = [&](int m, const double* X, const double* v, double* out) {
-    if (m <= 16) wg::gemm_tv_rows<16>(m, p, X, p, v, out, red);
-    else if (m <= 64) wg::gemm_tv_rows<64>(m, p, X, p, v, out, red);
-    else wg::gemm_tn_long(m, 1, p, X, p, v, p, out, m);
+    if (m <= 16) {
+      wg::gemm_tv_rows<16>(m, p, X, p, v, out, red);
+    } else if (m <= 128) {
+      for (int c0 = 0; c0 < m; c0 += 32)
+        wg::gemm_tv_rows<32>(m - c0 < 32 ? m - c0 : 32, p, X + (long long)c0 * p, p, v, out + c0,
+                             red);
+    } else {
+      wg::gemm_tn_long(m, 1, p, X, p, v, p, out, m);
+    }
+  };
+  auto stop = [&](double x) {
+    set_flag(f.state, isfinite(x) ? kFlagZeroNorm : kFlagNonFinite);
+    if (threadIdx.x == 0) f.state[1] = 1;
   };
 
   for (int j = 0; j < st.b; ++j) {
@@ -355,6 +464,7 @@ __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits,
     const double* t = f.t_raw + (long long)j * n;
     double* Rc = f.R + (long long)comp * p;
     double* Vc = f.V + (long long)comp * p;
+    double* Qc = f.Q + (long long)comp * q;
     double inv = 0.0;  // 1 / |t|
     if (f.xgram) {
       // cached route (simpls.hpp:600-620): loading w = G r, |t|² = rᵀ w,
@@ -374,126 +484,86 @@ __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits,
       double part = 0.0;
       for (int i = threadIdx.x; i < p; i += kWG) part = fma(r[i], Vc[i], part);
       const double ns = wg_sum_n<kWG>(part, sh.red);
-      if (!(ns > 0.0) || !isfinite(ns)) {
-        set_flag(f.state, isfinite(ns) ? kFlagZeroNorm : kFlagNonFinite);
-        if (threadIdx.x == 0) f.state[1] = 1;
-        return;
-      }
+      if (!(ns > 0.0) || !isfinite(ns)) return stop(ns);
       inv = 1.0 / sqrt(ns);
       if (f.T) scale_copy(f.T + (long long)comp * n, t, n, inv);
       scale_copy(Rc, r, p, inv);
       scale_copy(Vc, Vc, p, inv);
       __syncthreads();
     } else {
-    const double tn = wg::norm(t, n, sh.red);
-    if (!(tn > 0.0) || !isfinite(tn)) {
-      set_flag(f.state, isfinite(tn) ? kFlagZeroNorm : kFlagNonFinite);
-      if (threadIdx.x == 0) f.state[1] = 1;
-      return;
-    }
-    inv = 1.0 / tn;
-    if (f.T) scale_copy(f.T + (long long)comp * n, t, n, inv);
-    scale_copy(Rc, f.cand + (long long)j * p, p, inv);
-    {
-      // loading sums over the row splits: four elements per thread in flight
+      const double tn = sqrt(fmax(sumsq_stream(t, n, sh.red), 0.0));
+      if (!(tn > 0.0) || !isfinite(tn)) return stop(tn);
+      inv = 1.0 / tn;
+      if (f.T) scale_copy(f.T + (long long)comp * n, t, n, inv);
+      // one pass over the rows: R = r / |t| and loading = (sum of the
+      // row-split partials, in split order) / sx / |t|
+      const double* cand = f.cand + (long long)j * p;
       const double* part = f.ls_part + (long long)j * p;
+      const int splits = f.ls_splits;
       for (int i0 = threadIdx.x; i0 < p; i0 += 4 * kWG) {
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        int sp = 0;
-        for (; sp + 8 <= f.ls_splits; sp += 8) {
-          double v[8][4];
+        int ic[4];
 #pragma unroll
-          for (int e = 0; e < 8; ++e)
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              const int i = i0 + u * kWG;
-              v[e][u] = i < p ? part[i + (long long)(sp + e) * f.ls_slab] : 0.0;
-            }
-#pragma unroll
-          for (int e = 0; e < 8; ++e)
-#pragma unroll
-            for (int u = 0; u < 4; ++u) acc[u] += v[e][u];
-        }
-        for (; sp < f.ls_splits; ++sp) {
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const int i = i0 + u * kWG;
-            if (i < p) acc[u] += part[i + (long long)sp * f.ls_slab];
-          }
-        }
+        for (int u = 0; u < 4; ++u) ic[u] = i0 + u * kWG < p ? i0 + u * kWG : i0;
+        double acc[4] = {0.0, 0.0, 0.0, 0.0}, r[4], sx[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const int i = i0 + u * kWG;
-          if (i < p) Vc[i] = acc[u] / f.sx[i] * inv;
+          r[u] = cand[ic[u]];
+          sx[u] = f.sx[ic[u]];
         }
+        for (int sp0 = 0; sp0 < splits; sp0 += 8) {
+          double v[8][4];
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const long long off = (long long)(sp0 + e < splits ? sp0 + e : splits - 1) * f.ls_slab;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[e][u] = part[ic[u] + off];
+          }
+#pragma unroll
+          for (int e = 0; e < 8; ++e)
+            if (sp0 + e < splits)
+#pragma unroll
+              for (int u = 0; u < 4; ++u) acc[u] += v[e][u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (i0 + u * kWG < p) {
+            Vc[i0 + u * kWG] = acc[u] / sx[u] * inv;
+            Rc[i0 + u * kWG] = r[u] * inv;
+          }
       }
-    }
-    __syncthreads();
+      __syncthreads();
     }
     // response loadings q_a = S0ᵀ r (= Ỹᵀ t on the matrix-free route)
     if (f.qraw) {
-      for (int c = threadIdx.x; c < q; c += kWG)
-        f.Q[c + (long long)comp * q] = f.qraw[c + (long long)j * q] * inv;
+      for (int c = threadIdx.x; c < q; c += kWG) Qc[c] = f.qraw[c + (long long)j * q] * inv;
       __syncthreads();
     } else {
-      dots(q, f.S0, Rc, f.Q + (long long)comp * q);
+      dots(q, f.S0, Rc, Qc);
     }
-    // v = loading - V (Vᵀ loading), once (twice when re-orthogonalising)
-    if (comp > 0) {
-      for (int pass = 0; pass < (f.reorth ? 2 : 1); ++pass) {
-        dots(comp, f.V, Vc, proj);
-        for (int i0 = threadIdx.x; i0 < p; i0 += 2 * kWG) {
-          const int i1 = i0 + kWG < p ? i0 + kWG : i0;
-          double s0 = 0.0, s1 = 0.0;
-#pragma unroll 8
-          for (int l = 0; l < comp; ++l) {
-            const double pl = proj[l];
-            s0 = fma(f.V[i0 + (long long)l * p], pl, s0);
-            s1 = fma(f.V[i1 + (long long)l * p], pl, s1);
-          }
-          Vc[i0] -= s0;
-          if (i0 + kWG < p) Vc[i1] -= s1;
-        }
-        __syncthreads();
-      }
+    // v = loading - V (Vᵀ loading), once (twice when re-orthogonalising); the
+    // last pass also sums |v|²
+    const int passes = comp > 0 && f.reorth ? 2 : 1;
+    double ss = 0.0;
+    for (int pass = 0; pass < passes; ++pass) {
+      if (comp > 0) dots(comp, f.V, Vc, proj);
+      const bool last = pass == passes - 1;
+      ss = orth_pass(Vc, p, f.V, proj, comp, last);
+      if (!last) __syncthreads();
     }
-    const double vn = wg::norm(Vc, p, sh.red);
-    if (!(vn > 0.0) || !isfinite(vn)) {
-      set_flag(f.state, isfinite(vn) ? kFlagZeroNorm : kFlagNonFinite);
-      if (threadIdx.x == 0) f.state[1] = 1;
-      return;
-    }
+    const double vn = sqrt(fmax(wg_sum_n<kWG>(ss, sh.red), 0.0));
+    if (!(vn > 0.0) || !isfinite(vn)) return stop(vn);
     const double vinv = 1.0 / vn;
-    scale_copy(Vc, Vc, p, vinv);
-    __syncthreads();
     // deflation row = vᵀ S ; S -= v row (the matrix-free route keeps v in V
-    // and projects it out of every product instead)
+    // and projects it out of every product instead); the row is formed from
+    // the unit v as (Sᵀ v_raw) / |v_raw|
     if (f.S) {
       dots(q, f.S, Vc, row);
-      // two rows per thread, each row's columns loaded (16 at a time) before
-      // any is stored
-      for (int i0 = threadIdx.x; i0 < p; i0 += 2 * kWG) {
-        const bool two = i0 + kWG < p;
-        const int i1 = two ? i0 + kWG : i0;
-        const double v0 = Vc[i0], v1 = Vc[i1];
-        for (int c0 = 0; c0 < q; c0 += 16) {
-          double s0[16], s1[16];
-#pragma unroll
-          for (int u = 0; u < 16; ++u)
-            if (c0 + u < q) {
-              s0[u] = f.S[i0 + (long long)(c0 + u) * p];
-              s1[u] = f.S[i1 + (long long)(c0 + u) * p];
-            }
-#pragma unroll
-          for (int u = 0; u < 16; ++u)
-            if (c0 + u < q) {
-              const double r = row[c0 + u];
-              f.S[i0 + (long long)(c0 + u) * p] = s0[u] - v0 * r;
-              if (two) f.S[i1 + (long long)(c0 + u) * p] = s1[u] - v1 * r;
-            }
-        }
-      }
+      if (threadIdx.x < q) row[threadIdx.x] *= vinv;
+      __syncthreads();
+      deflate_elementwise(f.S, p, q, Vc, vinv, row);
+      __syncthreads();
     }
+    scale_copy(Vc, Vc, p, vinv);
     if (f.S && f.use_right_gram) {
       for (int e = threadIdx.x; e < q * q; e += kWG) {
         const int i = e % q, c = e / q;
