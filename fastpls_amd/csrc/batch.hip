@@ -149,8 +149,11 @@ class SimplsBatch {
   static int width_class(int N) { return N < kMfmaMinN ? 0 : mfma_width(N) / 16; }
 
  public:
+  // Plans the batch on `stream` (a stream of its own from the stream cache
+  // when null) and, unless `deferred`, executes it and waits for it.
   SimplsBatch(const std::vector<GroupIn>& groups, const std::vector<FitIn>& fits, int scaling,
-              int oversample, int power, int reorth, const BatchSplits& splits = default_splits())
+              int oversample, int power, int reorth, const BatchSplits& splits = default_splits(),
+              hipStream_t stream = nullptr, bool deferred = false)
       : groups_(groups), fits_(fits), scaling_(scaling), oversample_(oversample), power_(power),
         reorth_(reorth), splits_(splits) {
     if (reorth < -1 || reorth > 1) throw std::invalid_argument("reorth must be -1, 0 or 1");
@@ -161,26 +164,22 @@ class SimplsBatch {
     // reference's FASTPLS_BENCH_PHASE_TIMING switch (src/r_api.cpp:1182-1184)
     const char* pt = std::getenv("FASTPLS_BENCH_PHASE_TIMING");
     timing_ = pt != nullptr && std::strcmp(pt, "1") == 0;
-    stream_ = stream_take();
+    own_stream_ = stream == nullptr;
+    stream_ = own_stream_ ? stream_take() : stream;
     try {
       plan();
-      Event start, stop;
-      FPLS_HIP(hipEventRecord(start.e, stream_));
-      run();
-      FPLS_HIP(hipEventRecord(stop.e, stream_));
-      FPLS_HIP(hipEventSynchronize(stop.e));
-      collect();
-      float ms = 0.f;
-      FPLS_HIP(hipEventElapsedTime(&ms, start.e, stop.e));
-      seconds_ = double(ms) * 1e-3;
-      collect_phases();
+      if (!deferred) {
+        launch(nullptr, false);
+        finish();
+      }
     } catch (...) {
       for (TimedLaunch& t : timed_) {
         (void)hipEventDestroy(t.a);
         (void)hipEventDestroy(t.b);
       }
       timed_.clear();
-      stream_give(stream_);
+      if (own_stream_) stream_give(stream_);
+      else (void)hipStreamSynchronize(stream_);
       stream_ = nullptr;
       throw;
     }
@@ -188,51 +187,58 @@ class SimplsBatch {
 
   ~SimplsBatch() {
     if (graph_) (void)hipGraphExecDestroy(graph_);
-    if (stream_) stream_give(stream_);
+    if (stream_) {
+      if (own_stream_) stream_give(stream_);
+      else (void)hipStreamSynchronize(stream_);
+    }
   }
 
-  // Executes every fit again on the current contents of the caller's device
-  // inputs (X, responses or labels, permutations: same pointers and shapes),
-  // as a permutation test refits with new permutations. The first call
-  // captures run() into a graph on the batch's stream; every call launches
-  // that graph, so a repeated execution costs one graph launch and no
-  // planning, upload or per-kernel launch on the host. With phase timing on,
-  // run() is launched eagerly instead (events are not captured).
-  void rerun() {
-    Event start, stop;
-    if (timing_) {
-      FPLS_HIP(hipEventRecord(start.e, stream_));
-      run();
-    } else {
-      if (!graph_) {
-        hipGraph_t g = nullptr;
-        FPLS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-        try {
-          run();
-        } catch (...) {
-          hipGraph_t partial = nullptr;
-          (void)hipStreamEndCapture(stream_, &partial);
-          if (partial) (void)hipGraphDestroy(partial);
-          throw;
-        }
-        FPLS_HIP(hipStreamEndCapture(stream_, &g));
-        const hipError_t st = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        FPLS_HIP(st);
-      }
-      FPLS_HIP(hipEventRecord(start.e, stream_));
-      FPLS_HIP(hipGraphLaunch(graph_, stream_));
-    }
-    FPLS_HIP(hipEventRecord(stop.e, stream_));
-    FPLS_HIP(hipEventSynchronize(stop.e));
+  // Enqueues the whole execution on the batch's stream after `after` (an
+  // event recorded on another stream, or null), without waiting: run()'s
+  // launches, or with `graph` the graph run() was captured into on the
+  // batch's stream at the first such call (so a repeated execution costs one
+  // graph launch and no planning, upload or per-kernel launch on the host).
+  // With phase timing on, run() is launched eagerly (events are not
+  // captured).
+  void launch(hipEvent_t after, bool graph) {
+    const bool use_graph = graph && !timing_;
+    if (use_graph && !graph_) capture();
+    if (after) FPLS_HIP(hipStreamWaitEvent(stream_, after, 0));
+    FPLS_HIP(hipEventRecord(start_.e, stream_));
+    if (use_graph) FPLS_HIP(hipGraphLaunch(graph_, stream_));
+    else run();
+    FPLS_HIP(hipEventRecord(stop_.e, stream_));
+  }
+
+  // Waits for the execution launch() enqueued and reads its statuses and
+  // device time (and phase times with phase timing on).
+  void finish() {
+    FPLS_HIP(hipEventSynchronize(stop_.e));
     collect();
     float ms = 0.f;
-    FPLS_HIP(hipEventElapsedTime(&ms, start.e, stop.e));
+    FPLS_HIP(hipEventElapsedTime(&ms, start_.e, stop_.e));
     seconds_ = double(ms) * 1e-3;
     if (timing_) {
       for (auto& ph : phase_) ph[0] = ph[1] = ph[2] = 0.0;
       collect_phases();
     }
+  }
+
+  // Device seconds from `e` (recorded before launch()) to the end of the
+  // finished execution.
+  double seconds_since(hipEvent_t e) const {
+    float ms = 0.f;
+    FPLS_HIP(hipEventElapsedTime(&ms, e, stop_.e));
+    return double(ms) * 1e-3;
+  }
+
+  // Executes every fit again on the current contents of the caller's device
+  // inputs (X, responses or labels, permutations: same pointers and shapes),
+  // as a permutation test refits with new permutations, from the captured
+  // graph.
+  void rerun() {
+    launch(nullptr, true);
+    finish();
   }
 
   int count() const { return int(fits_.size()); }
@@ -943,7 +949,27 @@ class SimplsBatch {
   bool timing_ = false;
   std::vector<TimedLaunch> timed_;
   double phase_[kPhases][3] = {};
+  // run() captured into graph_ on the batch's stream
+  void capture() {
+    hipGraph_t g = nullptr;
+    FPLS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    try {
+      run();
+    } catch (...) {
+      hipGraph_t partial = nullptr;
+      (void)hipStreamEndCapture(stream_, &partial);
+      if (partial) (void)hipGraphDestroy(partial);
+      throw;
+    }
+    FPLS_HIP(hipStreamEndCapture(stream_, &g));
+    const hipError_t st = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    FPLS_HIP(st);
+  }
+
   hipStream_t stream_ = nullptr;
+  bool own_stream_ = true;
+  Event start_, stop_;   // around the last launch()
   double seconds_ = 0.0;
   std::vector<int> completed_, flags_;
 };
@@ -960,76 +986,111 @@ class BatchHandle {
   virtual void rerun() = 0;
   virtual void autotune(int trials, double* report, int count) = 0;
   virtual BatchSplits splits() const = 0;
+  virtual int lanes() const = 0;
 };
 
+// A batch split into lanes: the shared-X groups are dealt to 1..4 sub-batches
+// (largest estimated work first, each to the lane with the least so far),
+// and each lane runs on its own stream of a stream set, so the lanes share
+// no hardware queue and run at the same time. The per-fit kernels of a lane
+// (refresh, update: one work-group per fit, latency-bound) then overlap the
+// bandwidth-bound passes of the others instead of leaving the device idle
+// between passes. Every fit's arithmetic is the same as in one lane. The lane
+// count is 2 for batches of 32 fits or more over at least two groups, 1
+// otherwise, FASTPLS_HIP_BATCH_LANES when set, and 1 with phase timing on
+// (per-launch events of concurrent lanes would overlap).
 template<class T>
 class TypedBatch final : public BatchHandle {
  public:
   TypedBatch(const std::vector<GroupIn>& groups, const std::vector<FitIn>& fits, int scaling,
              int oversample, int power, int reorth)
       : groups_(groups), fits_(fits), scaling_(scaling), oversample_(oversample), power_(power),
-        reorth_(reorth),
-        b_(std::make_unique<SimplsBatch<T>>(groups, fits, scaling, oversample, power, reorth)) {}
-  int count() const override { return b_->count(); }
-  double seconds() const override { return b_->seconds(); }
-  int completed(int f) const override { return b_->completed(f); }
-  int flags(int f) const override { return b_->flags(f); }
-  void export_field(int f, int field, void* out, size_t elements, bool f64) override {
-    b_->export_field(f, field, out, elements, f64);
+        reorth_(reorth) {
+    if (groups_.empty() || fits_.empty()) throw std::invalid_argument("empty batch");
+    for (const FitIn& f : fits_)
+      if (f.group < 0 || f.group >= int(groups_.size()))
+        throw std::invalid_argument("fit refers to a missing group");
+    partition(lane_count());
+    plan_ = build(default_splits());
+    execute(*plan_, false);
   }
-  void phases(double* out, int count) const override { b_->phases(out, count); }
-  void rerun() override { b_->rerun(); }
-  BatchSplits splits() const override { return b_->splits(); }
+  int count() const override { return int(fits_.size()); }
+  double seconds() const override { return plan_->seconds; }
+  int completed(int f) const override { return lane(f).completed(local_[size_t(f)]); }
+  int flags(int f) const override { return lane(f).flags(local_[size_t(f)]); }
+  void export_field(int f, int field, void* out, size_t elements, bool f64) override {
+    if (f < 0 || f >= count()) throw std::invalid_argument("fit index outside the batch");
+    lane(f).export_field(local_[size_t(f)], field, out, elements, f64);
+  }
+  // Phase times of a one-lane batch (phase timing runs one lane).
+  void phases(double* out, int count) const override {
+    if (plan_->b.size() == 1) {
+      plan_->b[0]->phases(out, count);
+      return;
+    }
+    for (int i = 0; i < count; ++i) out[i] = 0.0;
+    if (count > 0) out[0] = plan_->seconds;
+  }
+  void rerun() override { execute(*plan_, true); }
+  BatchSplits splits() const override { return plan_->b[0]->splits(); }
+  int lanes() const override { return int(plan_->b.size()); }
 
   // Chooses the split lengths for this batch by measurement: starting from
   // the batch's own lengths, each length in turn is tried at 1/4, 1/2, 2 and
   // 4 times its current value (within [kMinSplit, kMaxSplit]; the wide
   // lengths only when some group runs on the matrix cores, the others only
   // when some runs on the VALU), re-planning and running the whole batch
-  // `trials` times per candidate and keeping a candidate that lowers the
-  // fastest device time by more than 1%. The batch then holds the fits of the
-  // chosen plan (the same fits; split sums differ in rounding only) and the
-  // choice becomes the process default for new batches. report: [0..3] the
-  // chosen lengths (step_rows, step_cols, wide_rows, wide_cols), [4] the
-  // chosen plan's device seconds, [5] the starting plan's, [6] plans run.
+  // (all its lanes) `trials` times per candidate and keeping a candidate that
+  // lowers the fastest device time by more than 1%. The batch then holds the
+  // fits of the chosen plan (the same fits; split sums differ in rounding
+  // only) and the choice becomes the process default for new batches.
+  // report: [0..3] the chosen lengths (step_rows, step_cols, wide_rows,
+  // wide_cols), [4] the chosen plan's device seconds, [5] the starting
+  // plan's, [6] plans run.
   void autotune(int trials, double* report, int count) override {
     if (trials < 1 || trials > 16) throw std::invalid_argument("autotune trials must lie in 1..16");
     int runs = 0;
-    auto measure = [&](const BatchSplits& cfg, std::unique_ptr<SimplsBatch<T>>& keep) {
+    auto measure = [&](const BatchSplits& cfg, std::unique_ptr<Plan>& keep) {
       double best = std::numeric_limits<double>::infinity();
       for (int t = 0; t < trials; ++t) {
-        auto b = std::make_unique<SimplsBatch<T>>(groups_, fits_, scaling_, oversample_, power_,
-                                                  reorth_, cfg);
+        std::unique_ptr<Plan> P = build(cfg);
+        execute(*P, false);
         ++runs;
-        if (b->seconds() < best) {
-          best = b->seconds();
-          keep = std::move(b);
+        if (P->seconds < best) {
+          best = P->seconds;
+          keep = std::move(P);
         }
       }
       return best;
     };
-    BatchSplits best = b_->splits();
-    std::unique_ptr<SimplsBatch<T>> best_batch;
-    double best_s = measure(best, best_batch);
+    BatchSplits best = plan_->b[0]->splits();
+    plan_.reset();
+    std::unique_ptr<Plan> best_plan;
+    double best_s = measure(best, best_plan);
     const double start_s = best_s;
+    bool narrow = false, wide = false;
+    for (const auto& b : best_plan->b) {
+      narrow = narrow || b->has_narrow();
+      wide = wide || b->has_wide();
+    }
     for (int i = 0; i < 4; ++i) {
-      if ((i < 2 && !best_batch->has_narrow()) || (i >= 2 && !best_batch->has_wide())) continue;
+      if ((i < 2 && !narrow) || (i >= 2 && !wide)) continue;
       const long long base = best.get(i);
       for (const double f : {0.25, 0.5, 2.0, 4.0}) {
         const long long v = (long long)(double(base) * f);
         if (v < kMinSplit || v > kMaxSplit) continue;
         BatchSplits cfg = best;
         cfg.at(i) = v;
-        std::unique_ptr<SimplsBatch<T>> cand;
+        std::unique_ptr<Plan> cand;
         const double t = measure(cfg, cand);
         if (t < best_s * 0.99) {
           best = cfg;
           best_s = t;
-          best_batch = std::move(cand);
+          best_plan = std::move(cand);
         }
       }
     }
-    b_ = std::move(best_batch);
+    plan_ = std::move(best_plan);
     set_default_splits(best);
     const double out[7] = {double(best.step_rows), double(best.step_cols), double(best.wide_rows),
                            double(best.wide_cols), best_s, start_s, double(runs)};
@@ -1037,10 +1098,126 @@ class TypedBatch final : public BatchHandle {
   }
 
  private:
+  // The lanes of one plan (split lengths) and their streams.
+  struct Plan {
+    std::vector<hipStream_t> streams;   // a stream set when there are several lanes
+    std::vector<std::unique_ptr<SimplsBatch<T>>> b;
+    double seconds = 0.0;
+    ~Plan() {
+      b.clear();   // each drains its stream before its buffers go
+      stream_set_give(streams);
+    }
+  };
+
+  int lane_count() const {
+    const char* pt = std::getenv("FASTPLS_BENCH_PHASE_TIMING");
+    if (pt != nullptr && std::strcmp(pt, "1") == 0) return 1;
+    int used = 0;
+    std::vector<char> has(groups_.size(), 0);
+    for (const FitIn& f : fits_)
+      if (!has[size_t(f.group)]) {
+        has[size_t(f.group)] = 1;
+        ++used;
+      }
+    int L = fits_.size() >= 32 ? 2 : 1;
+    if (const char* v = std::getenv("FASTPLS_HIP_BATCH_LANES")) {
+      L = std::atoi(v);
+      if (L < 1 || L > std::min(4, hardware_queues()))
+        throw std::invalid_argument("FASTPLS_HIP_BATCH_LANES must lie in 1..min(4, GPU_MAX_HW_QUEUES)");
+    }
+    return std::max(1, std::min(L, used));
+  }
+
+  // Groups to lanes by estimated work (X elements times components over the
+  // group's fits), largest first, each to the least-loaded lane; groups no
+  // fit uses stay with lane 0. Fits keep their order within a lane.
+  void partition(int L) {
+    const size_t ng = groups_.size();
+    std::vector<double> work(ng, 0.0);
+    std::vector<char> used(ng, 0);
+    for (const FitIn& f : fits_) {
+      const GroupIn& g = groups_[size_t(f.group)];
+      work[size_t(f.group)] += double(g.n) * double(g.p) * double(std::max(f.A, 1));
+      used[size_t(f.group)] = 1;
+    }
+    std::vector<int> order(ng);
+    for (size_t g = 0; g < ng; ++g) order[g] = int(g);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return work[size_t(a)] > work[size_t(b)]; });
+    std::vector<int> lane_of_group(ng, 0), local_group(ng, 0);
+    std::vector<double> load(size_t(L), 0.0);
+    lane_groups_.assign(size_t(L), {});
+    for (int g : order) {
+      int l = 0;
+      if (used[size_t(g)])
+        for (int k = 1; k < L; ++k)
+          if (load[size_t(k)] < load[size_t(l)]) l = k;
+      load[size_t(l)] += work[size_t(g)];
+      lane_of_group[size_t(g)] = l;
+      local_group[size_t(g)] = int(lane_groups_[size_t(l)].size());
+      lane_groups_[size_t(l)].push_back(groups_[size_t(g)]);
+    }
+    lane_fits_.assign(size_t(L), {});
+    lane_of_.resize(fits_.size());
+    local_.resize(fits_.size());
+    for (size_t f = 0; f < fits_.size(); ++f) {
+      const int l = lane_of_group[size_t(fits_[f].group)];
+      FitIn in = fits_[f];
+      in.group = local_group[size_t(fits_[f].group)];
+      lane_of_[f] = l;
+      local_[f] = int(lane_fits_[size_t(l)].size());
+      lane_fits_[size_t(l)].push_back(in);
+    }
+  }
+
+  std::unique_ptr<Plan> build(const BatchSplits& cfg) {
+    auto P = std::make_unique<Plan>();
+    const int L = int(lane_groups_.size());
+    if (L > 1) P->streams = stream_set_take(L);
+    for (int l = 0; l < L; ++l)
+      P->b.push_back(std::make_unique<SimplsBatch<T>>(
+          lane_groups_[size_t(l)], lane_fits_[size_t(l)], scaling_, oversample_, power_, reorth_,
+          cfg, L > 1 ? P->streams[size_t(l)] : nullptr, true));
+    return P;
+  }
+
+  // Every lane launched (after one event, so the device time counts from a
+  // common start), then waited for; the plan's device seconds run from that
+  // event to the last lane's end.
+  static void execute(Plan& P, bool graph) {
+    if (P.b.size() == 1) {
+      P.b[0]->launch(nullptr, graph);
+      P.b[0]->finish();
+      P.seconds = P.b[0]->seconds();
+      return;
+    }
+    Event fork;
+    FPLS_HIP(hipEventRecord(fork.e, P.streams[0]));
+    for (auto& b : P.b) b->launch(fork.e, graph);
+    double s = 0.0;
+    for (auto& b : P.b) {
+      b->finish();
+      s = std::max(s, b->seconds_since(fork.e));
+    }
+    P.seconds = s;
+  }
+
+  const SimplsBatch<T>& lane(int f) const {
+    if (f < 0 || f >= int(fits_.size())) throw std::invalid_argument("fit index outside the batch");
+    return *plan_->b[size_t(lane_of_[size_t(f)])];
+  }
+  SimplsBatch<T>& lane(int f) {
+    if (f < 0 || f >= int(fits_.size())) throw std::invalid_argument("fit index outside the batch");
+    return *plan_->b[size_t(lane_of_[size_t(f)])];
+  }
+
   std::vector<GroupIn> groups_;
   std::vector<FitIn> fits_;
   int scaling_, oversample_, power_, reorth_;
-  std::unique_ptr<SimplsBatch<T>> b_;
+  std::vector<std::vector<GroupIn>> lane_groups_;
+  std::vector<std::vector<FitIn>> lane_fits_;
+  std::vector<int> lane_of_, local_;   // per fit: its lane and index there
+  std::unique_ptr<Plan> plan_;
 };
 
 // ------------------------------------------------------------------- CV
@@ -1278,6 +1455,13 @@ int fastpls_hip_batch_splits(void* batch, long long* lengths, char* error,
     const fpls::BatchSplits b =
         batch ? static_cast<BatchHandle*>(batch)->splits() : fpls::default_splits();
     for (int i = 0; i < 4; ++i) lengths[i] = b.get(i);
+  });
+}
+
+int fastpls_hip_batch_lanes(void* batch, int* lanes, char* error, size_t error_capacity) {
+  return fpls::guarded(error, error_capacity, [&] {
+    if (!batch || !lanes) throw std::invalid_argument("null batch or lane output");
+    *lanes = static_cast<BatchHandle*>(batch)->lanes();
   });
 }
 

@@ -7,9 +7,11 @@
 #include <algorithm>
 #include <cstddef>
 #include <cstdio>
+#include <cstdlib>
 #include <exception>
 #include <random>
 #include <stdexcept>
+#include <vector>
 
 namespace fpls {
 
@@ -36,6 +38,22 @@ void stream_give(hipStream_t stream) noexcept;
 // stream.
 hipStream_t stream_create();
 void stream_cache_drop() noexcept;
+// Hardware queues HIP gives the process (GPU_MAX_HW_QUEUES, 4 unless set).
+inline int hardware_queues() {
+  const char* v = std::getenv("GPU_MAX_HW_QUEUES");
+  const int n = v ? std::atoi(v) : 0;
+  return n > 1 ? n : 4;
+}
+// `count` streams (at most hardware_queues()) that share no hardware queue,
+// for work split over streams that should run at the same time, owned by the
+// caller until stream_set_give drains them and keeps the set for the next
+// caller asking for as many. A new set is the last `count` of a run of
+// streams created back to back (the runtime binds a new stream to the
+// hardware queue the fewest live streams use, so the first ones of the run
+// even the usage out and consecutive ones then land on consecutive queues);
+// the rest of the run, which never ran work, is destroyed.
+std::vector<hipStream_t> stream_set_take(int count);
+void stream_set_give(std::vector<hipStream_t>& set) noexcept;
 
 // Device buffer. Unbound buffers own their memory directly (hipMalloc /
 // hipFree); buffers bound to a stream take it from the block cache and give

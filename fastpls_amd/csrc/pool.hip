@@ -195,6 +195,62 @@ void stream_cache_drop() noexcept {
   for (hipStream_t s : idle) (void)hipStreamDestroy(s);
 }
 
+namespace {
+
+struct StreamSets {
+  std::mutex mu;
+  std::vector<std::vector<hipStream_t>> idle;
+};
+
+StreamSets& stream_sets() {
+  static StreamSets* c = new StreamSets();   // lives until the process ends
+  return *c;
+}
+
+}  // namespace
+
+std::vector<hipStream_t> stream_set_take(int count) {
+  if (count < 1 || count > hardware_queues())
+    throw std::invalid_argument("a stream set holds 1 to GPU_MAX_HW_QUEUES streams");
+  StreamSets& C = stream_sets();
+  {
+    std::lock_guard<std::mutex> lock(C.mu);
+    for (size_t i = 0; i < C.idle.size(); ++i)
+      if (int(C.idle[i].size()) == count) {
+        std::vector<hipStream_t> set = std::move(C.idle[i]);
+        C.idle.erase(C.idle.begin() + long(i));
+        return set;
+      }
+  }
+  const int run = std::max(count, 8 * hardware_queues());
+  std::vector<hipStream_t> made;
+  try {
+    for (int i = 0; i < run; ++i) made.push_back(stream_create());
+  } catch (...) {
+    for (hipStream_t st : made) (void)hipStreamDestroy(st);
+    throw;
+  }
+  for (int i = 0; i < run - count; ++i) (void)hipStreamDestroy(made[size_t(i)]);
+  made.erase(made.begin(), made.begin() + (run - count));
+  return made;
+}
+
+void stream_set_give(std::vector<hipStream_t>& set) noexcept {
+  if (set.empty()) return;
+  for (hipStream_t s : set) {
+    (void)hipStreamSynchronize(s);
+    pool_forget_stream(s);
+  }
+  StreamSets& C = stream_sets();
+  std::lock_guard<std::mutex> lock(C.mu);
+  if (C.idle.size() < 8) {
+    C.idle.push_back(std::move(set));
+  } else {
+    for (hipStream_t s : set) (void)hipStreamDestroy(s);
+  }
+  set.clear();
+}
+
 void stream_give(hipStream_t stream) noexcept {
   if (!stream) return;
   (void)hipStreamSynchronize(stream);

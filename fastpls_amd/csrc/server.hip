@@ -292,12 +292,6 @@ class Server {
   static constexpr int kMaxFitStreams = 4;
   static constexpr int kMaxQueryStreams = 2;
 
-  static int hardware_queues() {
-    const char* v = std::getenv("GPU_MAX_HW_QUEUES");
-    const int n = v ? std::atoi(v) : 0;
-    return n > 1 ? n : 4;
-  }
-
   void give_streams() noexcept {
     for (hipStream_t st : fit_streams_) stream_give(st);
     for (hipStream_t st : query_streams_) stream_give(st);

@@ -87,6 +87,7 @@ SIGNATURES = {
     "fastpls_hip_batch_rerun": (C.c_int, [_vp, _cp, _sz]),
     "fastpls_hip_batch_autotune": (C.c_int, [_vp, C.c_int, _dp, C.c_int, _cp, _sz]),
     "fastpls_hip_batch_splits": (C.c_int, [_vp, C.POINTER(C.c_longlong), _cp, _sz]),
+    "fastpls_hip_batch_lanes": (C.c_int, [_vp, C.POINTER(C.c_int), _cp, _sz]),
     "fastpls_hip_batch_set_splits": (C.c_int, [C.POINTER(C.c_longlong), _cp, _sz]),
     "fastpls_hip_batch_destroy": (None, [_vp]),
     "fastpls_hip_simpls_cv_regression": _CV_REGRESSION,
@@ -531,6 +532,14 @@ class SimplsBatch:
 
     def splits(self):
         return batch_splits(self._h)
+
+    def lanes(self):
+        """Lane count of the batch (fastpls_hip_batch_lanes)."""
+        out = C.c_int(0)
+        e = _err()
+        if load_library().fastpls_hip_batch_lanes(self._h, C.byref(out), e, 1024) != 0:
+            _raise(e)
+        return out.value
 
     def status(self):
         F = len(self.fits)

@@ -263,6 +263,16 @@ int fastpls_hip_batch_autotune(void* batch, int trials, double* report, int repo
 int fastpls_hip_batch_splits(void* batch, long long* lengths, char* error,
                              size_t error_capacity);
 int fastpls_hip_batch_set_splits(const long long* lengths, char* error, size_t error_capacity);
+/* Lanes of a batch (no reference counterpart): the shared-X groups are dealt
+ * to sub-batches by estimated work (X elements times components), each run on
+ * its own stream of a set that shares no hardware queue, so the per-fit
+ * refresh and update kernels of one lane overlap the streaming passes of the
+ * others. Each fit's arithmetic, and so its results, are those of a one-lane
+ * batch. 2 lanes for 32 fits or more over at least two groups, else 1; the
+ * environment variable FASTPLS_HIP_BATCH_LANES (1..min(4, GPU_MAX_HW_QUEUES))
+ * sets the count at creation; 1 with FASTPLS_BENCH_PHASE_TIMING=1. Writes the
+ * batch's lane count. */
+int fastpls_hip_batch_lanes(void* batch, int* lanes, char* error, size_t error_capacity);
 void fastpls_hip_batch_destroy(void* batch);
 
 /* fastpls_resident_simpls_cv_regression (src/cuda_resident_api.h:20-26):
