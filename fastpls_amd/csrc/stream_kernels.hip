@@ -16,6 +16,7 @@
 #include "stream.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace fpls {
 
@@ -968,15 +969,22 @@ __global__ __launch_bounds__(kBf6Threads) void k_xw_bf6(const XwTask<float>* tas
 // ------------------------------------------------------ xtw, FP64 MFMA
 // Same tiling as the bf16x6 kernel (128 columns, 64-row steps, lane rows
 // 16v + 4g + e); MFMA call (v, e) pairs row 16v + 4g + e of A with the same
-// row of W, which is staged in LDS as FP64. The FP64 matrix cores bound this
-// kernel, so A is single-buffered. A is FP64, or FP32 widened to FP64 as it
-// is loaded (every product and sum is then FP64: the accurate route for
-// FP32-stored X).
+// row of W. The loop is software-pipelined one step deep: the A values and
+// the W tile of step s + 1 are loaded into registers while step s runs its
+// matrix products (A from registers, W from a double-buffered LDS tile), so
+// a wave never waits for global memory between steps. The W tile is laid
+// out row by row with one padding column: the staging stores (consecutive
+// rows per lane) and the operand reads (16 consecutive W columns per lane
+// group) are both free of bank conflicts. A is FP64, or FP32 widened to
+// FP64 as it is used (every product and sum is then FP64: the accurate route
+// for FP32-stored X). Each accumulator sums its products in the same order
+// (steps ascending, then v, then e), whatever the pipelining.
 template<int NB, class TX>
 __global__ __launch_bounds__(kWG) void k_xtw_f64(const XtwTask<TX>* tasks, int count,
                                                  XtwTask<TX> single) {
   constexpr int NBLK = NB / 16;
-  constexpr int LD = 64 + 2;
+  constexpr int WLD = NB + 1;                 // LDS row length of the W tile
+  constexpr int WPT = 64 * NB / kWG;          // W values per thread and step
   const XtwTask<TX> t = load_task(tasks, count, single);
   const long long ctiles = (t.k + kXtwCols - 1) / kXtwCols;
   const long long total = ctiles * t.splits;
@@ -989,8 +997,9 @@ __global__ __launch_bounds__(kWG) void k_xtw_f64(const XtwTask<TX>* tasks, int c
   const long long r0 = split * per;
   const long long r1 = r0 + per < t.m ? r0 + per : t.m;
 
-  __shared__ double ws[NB][LD];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __shared__ double ws[2][64][WLD];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
   const int m16 = lane & 15, g = lane >> 4;
   const bool stats = t.mode == kShiftStats;
   const double FPLS_GLOBAL* W = gp(t.W);
@@ -1018,48 +1027,100 @@ __global__ __launch_bounds__(kWG) void k_xtw_f64(const XtwTask<TX>* tasks, int c
     for (int nb = 0; nb < NBLK; ++nb) acc[mb][nb] = f64x4{0.0, 0.0, 0.0, 0.0};
   double s1[2] = {0.0, 0.0}, s2[2] = {0.0, 0.0};
 
-  for (long long i0 = r0; i0 < r1; i0 += 64) {
-    double x[2][16];
-    const bool full = i0 + 64 <= r1;
-#pragma unroll
-    for (int mb = 0; mb < 2; ++mb) {
-      if (full) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) load4_wide(xcol[mb] + i0 + 16 * v + 4 * g, &x[mb][4 * v]);
-      } else {
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const long long row = i0 + 16 * v + 4 * g + e;
-            x[mb][4 * v + e] = (cvalid[mb] && row < r1) ? double(xcol[mb][row]) : centre[mb];
-          }
-      }
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < 64 * NB; e += kWG) {
-      const int r = e & 63, n = e >> 6;
-      const long long wr = i0 + r;
-      ws[n][r] = (wr < r1 && n < t.N) ? W[wr + (long long)n * t.ldw] : 0.0;
-    }
-    __syncthreads();
+  // A values of a full 64-row step (raw; widened where they are used)
+  auto load_x = [&](TX (&x)[2][16], long long i0) __attribute__((always_inline)) {
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-      for (int u = 0; u < 16; ++u) {
-        x[mb][u] -= centre[mb];
-        if (stats) { s1[mb] += x[mb][u]; s2[mb] = fma(x[mb][u], x[mb][u], s2[mb]); }
-      }
+      for (int v = 0; v < 4; ++v) load4_raw(xcol[mb] + i0 + 16 * v + 4 * g, &x[mb][4 * v]);
+  };
+  // W rows of a step: value u of thread tid is row (tid + 256 u) % 64 of
+  // column (tid + 256 u) / 64; rows past r1 and columns past N read zero
+  auto load_w = [&](double (&w)[WPT], long long i0) __attribute__((always_inline)) {
 #pragma unroll
-    for (int nb = 0; nb < NBLK; ++nb)
+    for (int u = 0; u < WPT; ++u) {
+      const int e = tid + u * kWG;
+      const int r = e & 63, n = e >> 6;
+      const long long wr = i0 + r;
+      w[u] = (wr < r1 && n < t.N) ? W[wr + (long long)n * t.ldw] : 0.0;
+    }
+  };
+  auto store_w = [&](const double (&w)[WPT], int b) __attribute__((always_inline)) {
+#pragma unroll
+    for (int u = 0; u < WPT; ++u) {
+      const int e = tid + u * kWG;
+      ws[b][e & 63][e >> 6] = w[u];
+    }
+  };
+  // one step: the raw A values are widened and centred one reduction row
+  // (v, e) at a time, right before the products that use them
+  // (`edge` steps: rows at or past `lim` are zero after centring)
+  auto compute = [&](const TX (&xr)[2][16], int b, auto edge, long long lim)
+      __attribute__((always_inline)) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        double x[2];
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb) {
+          x[mb] = double(xr[mb][4 * v + e]) - centre[mb];
+          if constexpr (decltype(edge)::value) {
+            if (16 * v + 4 * g + e >= lim) x[mb] = 0.0;
+          }
+          if (stats) { s1[mb] += x[mb]; s2[mb] = fma(x[mb], x[mb], s2[mb]); }
+        }
+#pragma unroll
+        for (int nb = 0; nb < NBLK; ++nb) {
+          const double bv = ws[b][16 * v + 4 * g + e][nb * 16 + m16];
+#pragma unroll
+          for (int mb = 0; mb < 2; ++mb) acc[mb][nb] = mma64(x[mb], bv, acc[mb][nb]);
+        }
+      }
+  };
+
+  const long long nfull = (r1 - r0) / 64;
+  if (nfull > 0) {
+    // xn / w hold the loads of the next step while the current one (xc)
+    // multiplies; LDS buffer s & 1 holds the W tile of step s
+    TX xn[2][16];
+    double w[WPT];
+    load_x(xn, r0);
+    load_w(w, r0);
+    store_w(w, 0);
+    __syncthreads();
+    for (long long s = 0; s < nfull; ++s) {
+      TX xc[2][16];
+#pragma unroll
+      for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int u = 0; u < 16; ++u) xc[mb][u] = xn[mb][u];
+      const bool more = s + 1 < nfull;
+      if (more) { load_x(xn, r0 + 64 * (s + 1)); load_w(w, r0 + 64 * (s + 1)); }
+      compute(xc, int(s & 1), std::false_type{}, 0);
+      if (more) store_w(w, int((s + 1) & 1));
+      __syncthreads();
+    }
+  }
+  // the split's ragged last step: rows past r1 read the centre (zero after
+  // centring) and their W rows are zero
+  const long long it = r0 + 64 * nfull;
+  if (it < r1) {
+    TX x[2][16];
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
       for (int v = 0; v < 4; ++v)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          const double b = ws[nb * 16 + m16][16 * v + 4 * g + e];
-#pragma unroll
-          for (int mb = 0; mb < 2; ++mb) acc[mb][nb] = mma64(x[mb][4 * v + e], b, acc[mb][nb]);
+          const long long row = it + 16 * v + 4 * g + e;
+          x[mb][4 * v + e] = (cvalid[mb] && row < r1) ? xcol[mb][row] : TX(0);
         }
+    double w[WPT];
+    load_w(w, it);
+    store_w(w, 0);
+    __syncthreads();
+    compute(x, 0, std::true_type{}, r1 - it);
   }
 
   double FPLS_GLOBAL* o = gpw(t.out) + (long long)split * t.slab;
@@ -1095,12 +1156,17 @@ __global__ __launch_bounds__(kWG) void k_xtw_f64(const XtwTask<TX>* tasks, int c
 // ------------------------------------------------------- xw, FP64 MFMA
 // Same tiling as the bf16x6 kernel (256 rows, 32-column steps, lane rows
 // 4r..4r+3 of columns 8g..8g+7); MFMA call (e, j) pairs column 8g + j of row
-// 4r + e with row 8g + j of W, read straight from W's columns in L2. A is
-// FP64, or FP32 widened to FP64 as it is loaded.
+// 4r + e with row 8g + j of W. Pipelined like the xtw kernel above: the A
+// values of step s + 1 and its W rows (and centres) are loaded while step s
+// multiplies; W is staged in a double-buffered LDS tile shared by the four
+// waves (row length NB + 1: conflict-free stores and operand reads). Each
+// accumulator sums its products in the order steps, then j.
 template<int NB, class TX>
 __global__ __launch_bounds__(kWG) void k_xw_f64(const XwTask<TX>* tasks, int count,
                                                 XwTask<TX> single) {
   constexpr int NBLK = NB / 16;
+  constexpr int WLD = NB + 1;
+  constexpr int WPT = 32 * NB / kWG;          // W values per thread and step
   const XwTask<TX> t = load_task(tasks, count, single);
   long long local = (long long)blockIdx.x - t.tile0;
   const long long rtiles = xw_row_tiles(t.m);
@@ -1109,7 +1175,8 @@ __global__ __launch_bounds__(kWG) void k_xw_f64(const XwTask<TX>* tasks, int cou
   // (columns of any length start unaligned): on one XCD, one L2 fetches it
   local = xcd_local(local, rtiles * t.splits);
   const int split = int(local / rtiles);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
   const int m16 = lane & 15, g = lane >> 4;
   const long long i0w = (local % rtiles) * kWG + wave * 64;
   const long long rowb = i0w + 4 * m16;
@@ -1121,40 +1188,99 @@ __global__ __launch_bounds__(kWG) void k_xw_f64(const XwTask<TX>* tasks, int cou
   const double FPLS_GLOBAL* W = gp(t.W);
   const double FPLS_GLOBAL* C = t.center ? gp(t.center) : nullptr;
 
+  __shared__ double ws[2][32][WLD];
+  __shared__ double cs[2][32];
+
   f64x4 acc[4][NBLK];
 #pragma unroll
   for (int e = 0; e < 4; ++e)
 #pragma unroll
     for (int nb = 0; nb < NBLK; ++nb) acc[e][nb] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-  for (long long jc = j0; jc < j1; jc += 32) {
+  // A values of a step (columns past j1 and rows past m read zero)
+  auto load_x = [&](TX (&x)[8][4], long long jc, bool full) __attribute__((always_inline)) {
     const long long cb = jc + 8 * g;
-    const bool full = jc + 32 <= j1;
-    double x[8][4], c[8];
     if (rows_full && full) {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) load4_wide(xr + (cb + j) * t.m, x[j]);
+      for (int j = 0; j < 8; ++j) load4_raw(xr + (cb + j) * t.m, x[j]);
     } else {
 #pragma unroll
       for (int j = 0; j < 8; ++j)
 #pragma unroll
         for (int e = 0; e < 4; ++e)
-          x[j][e] = (cb + j < j1 && rowb + e < t.m) ? double(xr[(cb + j) * t.m + e]) : 0.0;
+          x[j][e] = (cb + j < j1 && rowb + e < t.m) ? xr[(cb + j) * t.m + e] : TX(0);
     }
+  };
+  // W rows of a step: value u of thread tid is row (tid + 256 u) % 32 of
+  // column (tid + 256 u) / 32; threads below 32 also load the centres
+  auto load_w = [&](double (&w)[WPT], double& c, long long jc) __attribute__((always_inline)) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) c[j] = (C != nullptr && cb + j < j1) ? C[cb + j] : 0.0;
+    for (int u = 0; u < WPT; ++u) {
+      const int e = tid + u * kWG;
+      const int r = e & 31, n = e >> 5;
+      w[u] = (jc + r < j1 && n < t.N) ? W[(jc + r) + (long long)n * t.ldw] : 0.0;
+    }
+    if (tid < 32) c = (C != nullptr && jc + tid < j1) ? C[jc + tid] : 0.0;
+  };
+  auto store_w = [&](const double (&w)[WPT], double c, int b) __attribute__((always_inline)) {
 #pragma unroll
-    for (int nb = 0; nb < NBLK; ++nb) {
-      const int n = nb * 16 + m16;
-      double w[8];
+    for (int u = 0; u < WPT; ++u) {
+      const int e = tid + u * kWG;
+      ws[b][e & 31][e >> 5] = w[u];
+    }
+    if (tid < 32) cs[b][tid] = c;
+  };
+  // one step: the raw A values are widened and centred one column j at a
+  // time, right before the products that use them
+  auto compute = [&](const TX (&xr8)[8][4], int b) __attribute__((always_inline)) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const double c = cs[b][8 * g + j];
+      double x[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) x[e] = double(xr8[j][e]) - c;
+#pragma unroll
+      for (int nb = 0; nb < NBLK; ++nb) {
+        const double bv = ws[b][8 * g + j][nb * 16 + m16];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc[e][nb] = mma64(x[e], bv, acc[e][nb]);
+      }
+    }
+  };
+
+  const long long nfull = (j1 - j0) / 32;
+  if (nfull > 0) {
+    // xn / w / c hold the loads of the next step while the current one (xc)
+    // multiplies; LDS buffer s & 1 holds the W rows of step s
+    TX xn[8][4];
+    double w[WPT], c = 0.0;
+    load_x(xn, j0, true);
+    load_w(w, c, j0);
+    store_w(w, c, 0);
+    __syncthreads();
+    for (long long s = 0; s < nfull; ++s) {
+      TX xc[8][4];
 #pragma unroll
       for (int j = 0; j < 8; ++j)
-        w[j] = (n < t.N && cb + j < j1) ? W[(cb + j) + (long long)n * t.ldw] : 0.0;
 #pragma unroll
-      for (int e = 0; e < 4; ++e)
-#pragma unroll
-        for (int j = 0; j < 8; ++j) acc[e][nb] = mma64(x[j][e] - c[j], w[j], acc[e][nb]);
+        for (int e = 0; e < 4; ++e) xc[j][e] = xn[j][e];
+      const bool more = s + 1 < nfull;
+      if (more) { load_x(xn, j0 + 32 * (s + 1), true); load_w(w, c, j0 + 32 * (s + 1)); }
+      compute(xc, int(s & 1));
+      if (more) store_w(w, c, int((s + 1) & 1));
+      __syncthreads();
     }
+  }
+  // the split's ragged last step
+  const long long jt = j0 + 32 * nfull;
+  if (jt < j1) {
+    TX x[8][4];
+    double w[WPT], c = 0.0;
+    load_x(x, jt, false);
+    load_w(w, c, jt);
+    store_w(w, c, 0);
+    __syncthreads();
+    compute(x, 0);
   }
 
   double FPLS_GLOBAL* o = gpw(t.out) + (long long)split * t.slab;
