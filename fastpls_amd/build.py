@@ -27,7 +27,8 @@ CFLAGS = [
     "-Wall",
 ]
 SOURCES = ["stream_kernels.hip", "small_kernels.hip", "prep_kernels.hip", "engine.hip", "batch.hip",
-           "cv.hip", "server.hip", "pool.hip", "svd.hip", "core.hip", "operators.hip"]
+           "cv.hip", "server.hip", "pool.hip", "svd.hip", "core.hip", "operators.hip",
+           "special.hip"]
 
 
 def _compile(src: str) -> str:

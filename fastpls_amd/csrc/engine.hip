@@ -198,10 +198,20 @@ class Model final : public Handle, private StreamOps {
 
   void response_sums(const void* x, const void* y, const int* labels, int rows, int prefix,
                      void* out) override {
+    response_sums_rows(x, y, labels, rows, prefix, out, false);
+  }
+
+  void response_sums_device(const void* x, const void* y, const int* labels, int rows,
+                            int prefix, void* out) override {
+    response_sums_rows(x, y, labels, rows, prefix, out, true);
+  }
+
+  void response_sums_rows(const void* x, const void* y, const int* labels, int rows, int prefix,
+                          void* out, bool device_rows) {
     if (!out || (!y && !labels) || (y && labels) || (classification_ != (labels != nullptr)))
       throw std::invalid_argument("invalid resident response representation");
     if (!x || rows < 1 || prefix < 1 || prefix > A_) throw std::invalid_argument("invalid resident prediction input");
-    project_rows(load_rows(static_cast<const T*>(x), rows), rows, prefix);
+    project_rows(load_rows(static_cast<const T*>(x), rows, device_rows), rows, prefix);
     response_block(rows, prefix, false, nullptr, 0);
     metric_.reserve(sizeof(double) * 3 * q_);
     flags_.reserve(sizeof(int));
@@ -304,8 +314,18 @@ class Model final : public Handle, private StreamOps {
   }
 
   void project(const void* x, int rows, int prefix, void* out) override {
+    project_any(x, rows, prefix, out, false);
+  }
+
+  void project_device(const void* x, int rows, int prefix, void* out) override {
+    project_any(x, rows, prefix, out, true);
+  }
+
+  const double* device_weights() const override { return R_.d(); }
+
+  void project_any(const void* x, int rows, int prefix, void* out, bool device_rows) {
     if (!x || !out || rows < 1 || prefix < 1 || prefix > A_) throw std::invalid_argument("invalid resident score request");
-    project_rows(load_rows(static_cast<const T*>(x), rows), rows, prefix);
+    project_rows(load_rows(static_cast<const T*>(x), rows, device_rows), rows, prefix);
     outT_.reserve(sizeof(T) * (size_t)rows * prefix);
     launch_convert<T>(scores_.d(), outT_.as<T>(), (long long)rows * prefix, stream_);
     FPLS_HIP(hipMemcpyAsync(out, outT_.as<T>(), sizeof(T) * (size_t)rows * prefix,
@@ -1347,18 +1367,37 @@ void* fastpls_hip_plssvd_create(const void* x, const void* y, const int* labels,
   return h;
 }
 
-void* fastpls_hip_opls_create(const void*, const void*, const int*, int, int, int, int, int, int,
-                              int, int, int, unsigned long long, int, char* error,
-                              size_t error_capacity) {
-  fpls::write_error(error, error_capacity, "OPLS is not supported by the HIP backend");
-  return nullptr;
+void* fastpls_hip_opls_create(const void* x, const void* y, const int* labels, int precision,
+                              int n, int p, int q, int components, int scaling, int oversample,
+                              int power, int retain_scores, unsigned long long seed, int north,
+                              char* error, size_t error_capacity) {
+  Handle* h = nullptr;
+  fpls::guarded(error, error_capacity, [&] {
+    if (!x || (precision != 32 && precision != 64) || north < 0)
+      throw std::invalid_argument("invalid resident HIP OPLS input");
+    if (retain_scores != 0 && retain_scores != 1)
+      throw std::invalid_argument("invalid resident score-retention request");
+    h = fpls::create_opls(x, y, labels, precision, n, p, q, components, scaling, oversample,
+                          power, seed, north, retain_scores == 1);
+  });
+  return h;
 }
 
-void* fastpls_hip_kernelpls_create(const void*, const void*, const int*, int, int, int, int, int,
-                                   int, int, int, int, unsigned long long, int, double, int,
-                                   double, char* error, size_t error_capacity) {
-  fpls::write_error(error, error_capacity, "kernel PLS is not supported by the HIP backend");
-  return nullptr;
+void* fastpls_hip_kernelpls_create(const void* x, const void* y, const int* labels, int precision,
+                                   int n, int p, int q, int components, int scaling,
+                                   int oversample, int power, int retain_scores,
+                                   unsigned long long seed, int kernel, double gamma, int degree,
+                                   double coef0, char* error, size_t error_capacity) {
+  Handle* h = nullptr;
+  fpls::guarded(error, error_capacity, [&] {
+    if (!x || (precision != 32 && precision != 64) || (kernel != 2 && kernel != 3))
+      throw std::invalid_argument("invalid resident HIP nonlinear kernel PLS input");
+    if (retain_scores != 0 && retain_scores != 1)
+      throw std::invalid_argument("invalid resident score-retention request");
+    h = fpls::create_kernelpls(x, y, labels, precision, n, p, q, components, scaling, oversample,
+                               power, seed, kernel, gamma, degree, coef0, retain_scores == 1);
+  });
+  return h;
 }
 
 void* fastpls_hip_fit_device(int method, const void* x_device, const void* y_device,

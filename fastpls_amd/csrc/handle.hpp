@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstddef>
+#include <stdexcept>
 
 #include <hip/hip_runtime.h>
 
@@ -52,6 +53,20 @@ class Handle {
   // or another attach): the current stream is drained first, and a stream
   // the handle took from the stream cache goes back to it.
   virtual void attach_stream(hipStream_t stream) = 0;
+  // The FP64 weights R (features x components, column-major) in device
+  // memory, valid while the handle lives; null when the handle keeps none.
+  virtual const double* device_weights() const { return nullptr; }
+  // `project` and `response_sums` for query rows x already in device memory
+  // (y and labels stay host arrays).
+  virtual void project_device(const void* x, int rows, int prefix, void* out) {
+    (void)x; (void)rows; (void)prefix; (void)out;
+    throw std::invalid_argument("this handle takes host query rows only");
+  }
+  virtual void response_sums_device(const void* x, const void* y, const int* labels, int rows,
+                                    int prefix, void* out) {
+    (void)x; (void)y; (void)labels; (void)rows; (void)prefix; (void)out;
+    throw std::invalid_argument("this handle takes host query rows only");
+  }
 };
 
 // method 0 = SIMPLS, 1 = PLS-SVD; precision 32 or 64. With `device_inputs`
@@ -64,5 +79,17 @@ Handle* create(int method, const void* x, const void* y, const int* labels, int 
                bool device_inputs, int n, int p, int q, int components, int scaling,
                int oversample, int power, unsigned long long seed,
                hipStream_t stream = nullptr, bool retain_scores = true);
+
+// Resident OPLS (src/cuda_resident_special.cuh:155-358): `north` orthogonal
+// filter components, then SIMPLS on the filtered predictors. Host inputs.
+Handle* create_opls(const void* x, const void* y, const int* labels, int precision, int n, int p,
+                    int q, int components, int scaling, int oversample, int power,
+                    unsigned long long seed, int north, bool retain_scores);
+// Resident nonlinear kernel PLS (src/cuda_resident_special.cuh:360-544): kind
+// 2 radial basis, 3 polynomial. Host inputs.
+Handle* create_kernelpls(const void* x, const void* y, const int* labels, int precision, int n,
+                         int p, int q, int components, int scaling, int oversample, int power,
+                         unsigned long long seed, int kind, double gamma, int degree,
+                         double coef0, bool retain_scores);
 
 }  // namespace fpls

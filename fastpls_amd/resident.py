@@ -20,6 +20,8 @@ LIB_PATH = os.path.join(HERE, "lib", "libfastpls_hip.so")
 
 SIMPLS = 0
 PLSSVD = 1
+OPLS = 2       # handle tags of the special models (not method codes of the C ABI)
+KERNELPLS = 3
 
 FIELDS = {"R": 0, "Q": 1, "T": 2, "mX": 3, "vX": 4, "mY": 5, "P": 6, "variance": 7}
 
@@ -193,22 +195,21 @@ def gemm(left, right, precision: int = 64):
 class ResidentModel:
     """A fitted model resident in HBM (handle of the C ABI)."""
 
-    def __init__(self, handle, method, precision, n, p, q, components, classification):
+    def __init__(self, handle, method, precision, n, p, q, components, classification,
+                 feature_p=None):
         self._h = handle
         self.method = method
         self.precision = precision
         self.dtype = _dtype(precision)
         self.n, self.p, self.q = n, p, q
+        # columns of the weights R and loadings P: p, or n for kernel PLS
+        self.feature_p = p if feature_p is None else feature_p
         self.components = components
         self.classification = classification
 
     # ----------------------------------------------------------- creation
-    @classmethod
-    def fit(cls, x, y=None, labels=None, classes=None, method=SIMPLS, precision=64,
-            components=2, scaling=2, oversample=32, power=5, seed=1, retain_scores=True):
-        """Fit SIMPLS (method=0) or PLS-SVD (method=1) from host arrays."""
-        lib = load_library()
-        dt = _dtype(precision)
+    @staticmethod
+    def _inputs(x, y, labels, classes, dt):
         xa = _fmat(x, dt)
         n, p = xa.shape
         if (y is None) == (labels is None):
@@ -223,6 +224,14 @@ class ResidentModel:
                 ya = ya.reshape((-1, 1), order="F")
             q = ya.shape[1]
             lab = None
+        return xa, ya, lab, n, p, q
+
+    @classmethod
+    def fit(cls, x, y=None, labels=None, classes=None, method=SIMPLS, precision=64,
+            components=2, scaling=2, oversample=32, power=5, seed=1, retain_scores=True):
+        """Fit SIMPLS (method=0) or PLS-SVD (method=1) from host arrays."""
+        lib = load_library()
+        xa, ya, lab, n, p, q = cls._inputs(x, y, labels, classes, _dtype(precision))
         e = _err()
         fn = lib.fastpls_hip_simpls_create if method == SIMPLS else lib.fastpls_hip_plssvd_create
         h = fn(_ptr(xa), _ptr(ya), _iptr(lab), precision, n, p, q, components, scaling,
@@ -230,6 +239,38 @@ class ResidentModel:
         if not h:
             _raise(e)
         return cls(h, method, precision, n, p, q, components, lab is not None)
+
+    @classmethod
+    def fit_opls(cls, x, y=None, labels=None, classes=None, precision=64, components=2, north=1,
+                 scaling=2, oversample=32, power=5, seed=1, retain_scores=True):
+        """Resident OPLS (fastpls_hip_opls_create): `north` orthogonal filter
+        components, then SIMPLS on the filtered predictors."""
+        lib = load_library()
+        xa, ya, lab, n, p, q = cls._inputs(x, y, labels, classes, _dtype(precision))
+        e = _err()
+        h = lib.fastpls_hip_opls_create(_ptr(xa), _ptr(ya), _iptr(lab), precision, n, p, q,
+                                        components, scaling, oversample, power,
+                                        int(bool(retain_scores)), seed, north, e, 1024)
+        if not h:
+            _raise(e)
+        return cls(h, OPLS, precision, n, p, q, components, lab is not None)
+
+    @classmethod
+    def fit_kernelpls(cls, x, y=None, labels=None, classes=None, precision=64, components=2,
+                      kernel=2, gamma=1.0, degree=3, coef0=1.0, scaling=2, oversample=32,
+                      power=5, seed=1, retain_scores=True):
+        """Resident nonlinear kernel PLS (fastpls_hip_kernelpls_create):
+        kernel 2 radial basis, 3 polynomial; SIMPLS on the centred kernel."""
+        lib = load_library()
+        xa, ya, lab, n, p, q = cls._inputs(x, y, labels, classes, _dtype(precision))
+        e = _err()
+        h = lib.fastpls_hip_kernelpls_create(_ptr(xa), _ptr(ya), _iptr(lab), precision, n, p, q,
+                                             components, scaling, oversample, power,
+                                             int(bool(retain_scores)), seed, kernel,
+                                             float(gamma), degree, float(coef0), e, 1024)
+        if not h:
+            _raise(e)
+        return cls(h, KERNELPLS, precision, n, p, q, components, lab is not None, feature_p=n)
 
     @classmethod
     def fit_device(cls, x_ptr, y_ptr, labels_ptr, n, p, q, method=SIMPLS, precision=32,
@@ -347,8 +388,9 @@ class ResidentModel:
     def export(self, field):
         f = FIELDS[field] if isinstance(field, str) else int(field)
         a = self.components
-        shapes = {0: (self.p, a), 1: (self.q, a), 2: (self.n, a), 3: (self.p,), 4: (self.p,),
-                  5: (self.q,), 6: (self.p, a), 7: (a + 1,)}
+        fp = self.feature_p
+        shapes = {0: (fp, a), 1: (self.q, a), 2: (self.n, a), 3: (self.p,), 4: (self.p,),
+                  5: (self.q,), 6: (fp, a), 7: (a + 1,)}
         shape = shapes[f]
         out = np.empty(shape, dtype=self.dtype, order="F")
         e = _err()

@@ -69,9 +69,18 @@ void* fastpls_hip_plssvd_create(const void* x, const void* y, const int* labels,
                                 int retain_scores, unsigned long long seed,
                                 char* error, size_t error_capacity);
 
-/* fastpls_resident_opls_create / fastpls_resident_kernelpls_create
- * (src/cuda_resident_api.h:48-56): not part of this backend; they always
- * return NULL with an "unsupported" message. */
+/* fastpls_resident_opls_create (src/cuda_resident_api.h:48-51): resident OPLS,
+ * `north` >= 0 orthogonal filter components (rank-one randomized direction of
+ * the current cross-covariance, seed + component), then SIMPLS on the
+ * filtered predictors with the same seed; query rows are standardised and
+ * filtered on the device. Export field 0 holds R_eff, the weights that take
+ * standardised rows to the scores; fields 3/4 the outer centre and scale.
+ * fastpls_resident_kernelpls_create (src/cuda_resident_api.h:52-56): resident
+ * nonlinear kernel PLS, kernel 2 radial basis exp(-gamma |a - b|^2) or 3
+ * polynomial (gamma a.b + coef0)^degree on the standardised predictors,
+ * double-centred, then SIMPLS on the n x n kernel; fields 0 and 6 have n
+ * rows. The n x n kernel plus the predictors must fit in 3/5 of the free
+ * device memory, as must a query's rows x n test kernel. */
 void* fastpls_hip_opls_create(const void* x, const void* y, const int* labels,
                               int precision, int n, int p, int q, int components,
                               int scaling, int oversample, int power, int retain_scores,

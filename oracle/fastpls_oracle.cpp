@@ -1413,6 +1413,179 @@ PlssvdModel fit_plssvd_operator(const Mat& Xs, const Mat& Y, const std::vector<d
                          operator_svd_op(crosscov, int(retained), oversample, power, seed, false));
 }
 
+// ------------------------------------------------------------ OPLS
+// The orthogonal filter of opls.hpp:123-221 (fit_preprocessed_filter_inplace)
+// with the randomized leading direction of fit_opls_filter_rsvd
+// (opls.hpp:314-363) and fit_opls_filter_labels_rsvd (opls.hpp:418-465): a
+// rank-one rSVD of the current cross-covariance with seed + component, left
+// vector only, normalised. The resident device engine takes this randomized
+// route at both precisions (src/cuda_resident_special.cuh:270-301).
+struct OplsFilter {
+  Prepared pr;        // outer statistics; pr.Xs is filtered in place
+  Mat W, P;           // orthogonal weights and loadings, p x completed
+  size_t completed = 0;
+};
+
+// opls.hpp:34-61 (vector_norm / normalize): plain double sums, a zero or
+// non-finite norm fails
+bool opls_normalize(Mat& v) {
+  double sum = 0.0;
+  for (size_t i = 0; i < v.r; ++i) sum += v(i, 0) * v(i, 0);
+  const double nrm = std::sqrt(std::max(sum, 0.0));
+  if (!std::isfinite(nrm) || nrm <= 0.0) return false;
+  for (size_t i = 0; i < v.r; ++i) v(i, 0) /= nrm;
+  return true;
+}
+
+double opls_dot(const Mat& a, const Mat& b) {
+  double sum = 0.0;
+  for (size_t i = 0; i < a.r; ++i) sum += a(i, 0) * b(i, 0);
+  return sum;
+}
+
+// classification.hpp:341-385 (centered_label_crossprod) with offsets -prior
+Mat label_crossprod(const Mat& X, const int* labels1, const std::vector<double>& prior) {
+  const size_t K = prior.size();
+  Mat S(X.c, K);
+  std::vector<double> sums(K);
+  for (size_t j = 0; j < X.c; ++j) {
+    std::fill(sums.begin(), sums.end(), 0.0);
+    double total = 0.0;
+    for (size_t i = 0; i < X.r; ++i) {
+      const double v = X(i, j);
+      total += v;
+      sums[size_t(labels1[i] - 1)] += v;
+    }
+    for (size_t c = 0; c < K; ++c) S(j, c) = sums[c] + total * (-prior[c]);
+  }
+  return S;
+}
+
+OplsFilter opls_filter(const Mat& X, const Mat* Y, const int* labels1, size_t q, size_t north,
+                       int scaling, int oversample, int power, unsigned int seed) {
+  OplsFilter f;
+  f.pr = labels1 ? prepare_labels(X, labels1, q, scaling) : prepare_dense(X, *Y, scaling);
+  Mat& Xf = f.pr.Xs;
+  const size_t n = Xf.r, p = Xf.c;
+  Mat Yc;
+  if (!labels1) {  // opls.hpp:283-289: the centred response of the refreshes
+    Yc.resize(n, q);
+    for (size_t c = 0; c < q; ++c)
+      for (size_t i = 0; i < n; ++i) Yc(i, c) = (*Y)(i, c) - f.pr.my[c];
+  }
+  Mat S = f.pr.S;
+  f.W.resize(p, north);
+  f.P.resize(p, north);
+  for (size_t comp = 0; comp < north; ++comp) {
+    if (comp > 0) S = labels1 ? label_crossprod(Xf, labels1, f.pr.my) : mul(Xf, true, Yc, false);
+    Triplets tr = randomized_svd(S, 1, oversample, power, seed + unsigned(comp), true);
+    if (tr.U.c == 0) break;
+    Mat w(p, 1);
+    for (size_t j = 0; j < p; ++j) w(j, 0) = tr.U(j, 0);
+    if (!opls_normalize(w)) break;
+    Mat t = mul(Xf, false, w, false);
+    const double tt = opls_dot(t, t);
+    if (!std::isfinite(tt) || tt <= 0.0) break;
+    Mat loading = mul(Xf, true, t, false);
+    for (size_t j = 0; j < p; ++j) loading(j, 0) /= tt;
+    const double ww = opls_dot(w, w);
+    if (!std::isfinite(ww) || ww <= 0.0) break;
+    const double proj = opls_dot(w, loading) / ww;
+    Mat wo(p, 1);
+    for (size_t j = 0; j < p; ++j) wo(j, 0) = loading(j, 0) - w(j, 0) * proj;
+    if (!opls_normalize(wo)) break;
+    Mat to = mul(Xf, false, wo, false);
+    const double too = opls_dot(to, to);
+    if (!std::isfinite(too) || too <= 0.0) break;
+    Mat po = mul(Xf, true, to, false);
+    for (size_t j = 0; j < p; ++j) {
+      po(j, 0) /= too;
+      f.W(j, f.completed) = wo(j, 0);
+      f.P(j, f.completed) = po(j, 0);
+    }
+    Mat corr = mul(to, false, po, true);
+    for (size_t e = 0; e < Xf.d.size(); ++e) Xf.d[e] -= corr.d[e];
+    ++f.completed;
+  }
+  return f;
+}
+
+// Weights that take standardised rows straight to the scores of the inner
+// model: R_eff = (I - w_1 p_1ᵀ) ... (I - w_m p_mᵀ) R (the filtered rows are
+// x_s (I - w_1 p_1ᵀ) ... (I - w_m p_mᵀ), opls.hpp:467-509), applied from the
+// last filter component to the first (src/cuda_resident_special.cuh:306-312).
+Mat opls_effective_weights(const Mat& R, const Mat& W, const Mat& P, size_t completed) {
+  Mat out = R;
+  for (size_t c = completed; c > 0; --c) {
+    const size_t k = c - 1;
+    for (size_t a = 0; a < out.c; ++a) {
+      double coef = 0.0;
+      for (size_t j = 0; j < out.r; ++j) coef += P(j, k) * out(j, a);
+      for (size_t j = 0; j < out.r; ++j) out(j, a) -= W(j, k) * coef;
+    }
+  }
+  return out;
+}
+
+// ------------------------------------------------------------ kernel PLS
+// kernels.hpp:20-62 (kernel_from_dots): 2 = radial basis exp(-γ d) with d =
+// |a|² + |b|² - 2 a·b (small negative d, above -1e-10, clamped to zero),
+// 3 = polynomial (γ a·b + offset)^degree.
+Mat kernel_dots(const Mat& A, const Mat& B, int kind, double gamma, int degree, double offset) {
+  Mat K = mul(A, false, B, true);
+  if (kind == 3) {
+    for (double& v : K.d) v = std::pow(gamma * v + offset, degree);
+    return K;
+  }
+  std::vector<double> ln(A.r, 0.0), rn(B.r, 0.0);
+  for (size_t f = 0; f < A.c; ++f) {
+    for (size_t i = 0; i < A.r; ++i) ln[i] += A(i, f) * A(i, f);
+    for (size_t i = 0; i < B.r; ++i) rn[i] += B(i, f) * B(i, f);
+  }
+  for (size_t j = 0; j < K.c; ++j)
+    for (size_t i = 0; i < K.r; ++i) {
+      double d = ln[i] + rn[j] - 2.0 * K(i, j);
+      if (d < 0.0 && d > -1e-10) d = 0.0;
+      K(i, j) = std::exp(-gamma * d);
+    }
+  return K;
+}
+
+struct KernelCentre {
+  std::vector<double> colmeans;
+  double grand = 0.0;
+};
+
+// kernels.hpp:128-154 (center_kernel_train)
+KernelCentre centre_train_kernel(Mat& K) {
+  KernelCentre kc;
+  kc.colmeans.assign(K.c, 0.0);
+  std::vector<double> rowmeans(K.r, 0.0);
+  for (size_t j = 0; j < K.c; ++j) {
+    for (size_t i = 0; i < K.r; ++i) {
+      kc.colmeans[j] += K(i, j);
+      rowmeans[i] += K(i, j);
+    }
+    kc.colmeans[j] /= double(K.r);
+    kc.grand += kc.colmeans[j];
+  }
+  kc.grand /= double(K.c);
+  for (double& v : rowmeans) v /= double(K.c);
+  for (size_t j = 0; j < K.c; ++j)
+    for (size_t i = 0; i < K.r; ++i) K(i, j) += kc.grand - rowmeans[i] - kc.colmeans[j];
+  return kc;
+}
+
+// kernels.hpp:156-178 (center_kernel_test)
+void centre_test_kernel(Mat& K, const KernelCentre& kc) {
+  for (size_t i = 0; i < K.r; ++i) {
+    double rm = 0.0;
+    for (size_t j = 0; j < K.c; ++j) rm += K(i, j);
+    rm /= double(K.c);
+    for (size_t j = 0; j < K.c; ++j) K(i, j) += kc.grand - rm - kc.colmeans[j];
+  }
+}
+
 // ------------------------------------------------------------ LDA
 struct Lda { Mat means, linear; std::vector<double> constants, priors; double ridge = 0, relative = 0; };
 
@@ -2048,6 +2221,102 @@ int orc_variance(const double* xs, const double* T, int n, int p, int a,
 
 // General product with the reference's fastpls_cuda_gemm contract
 // (src/cuda_resident_api.h:11-13): out (rows x cols) = left (rows x inner) * right.
+// OPLS: orthogonal filter (north components) + SIMPLS on the filtered
+// predictors (scaling none, the same controls and seed; the resident
+// engine's inner fit, src/cuda_resident_special.cuh:302-305). y (n x q) or
+// one-based labels. Outputs: W, P (p x north; *filtered components), the
+// inner R (p x A), Q (q x A), T (n x A), R_eff (p x A), outer mx, sx (p),
+// my (q), *completed inner components.
+int orc_opls_fit(const double* x, const double* y, const int* labels, int n, int p, int q,
+                 int ncomp, int north, int scaling, int oversample, int power,
+                 unsigned int seed, double* W, double* P, int* filtered, double* R,
+                 double* Q, double* T, double* Reff, double* mx, double* sx, double* my,
+                 int* completed) {
+  return guarded([&] {
+    const Mat X = wrap(x, n, p);
+    Mat Y;
+    if (!labels) Y = wrap(y, n, q);
+    OplsFilter f = opls_filter(X, labels ? nullptr : &Y, labels, size_t(q), size_t(north),
+                               scaling, oversample, power, seed);
+    Prepared inner = labels ? prepare_labels(f.pr.Xs, labels, size_t(q), 3)
+                            : prepare_dense(f.pr.Xs, Y, 3);
+    const size_t cap = std::min<size_t>(size_t(p), std::max<size_t>(size_t(n) - 1, 1));
+    const size_t A = std::max<size_t>(1, std::min<size_t>(size_t(ncomp), cap));
+    SimplsControls ctl = simpls_controls(n, p, q, A, labels != nullptr, oversample, power, seed);
+    SimplsModel m = fit_simpls(inner.Xs, inner.S, ctl);
+    for (size_t c = 0; c < f.completed; ++c)
+      for (int j = 0; j < p; ++j) {
+        W[j + c * size_t(p)] = f.W(size_t(j), c);
+        P[j + c * size_t(p)] = f.P(size_t(j), c);
+      }
+    *filtered = int(f.completed);
+    copy_out(m.R, R); copy_out(m.Q, Q);
+    if (T) copy_out(m.T, T);
+    copy_out(opls_effective_weights(m.R, f.W, f.P, f.completed), Reff);
+    std::copy(f.pr.mx.begin(), f.pr.mx.end(), mx);
+    std::copy(f.pr.sx.begin(), f.pr.sx.end(), sx);
+    std::copy(inner.my.begin(), inner.my.end(), my);
+    *completed = int(m.completed);
+  });
+}
+
+// Kernel PLS (kernelpls.hpp:87-153; with one-based labels the inner SIMPLS
+// takes the label cross-product of the centred kernel, the resident
+// engine's classification route, src/cuda_resident_special.cuh:491-503):
+// standardised X (scaling), the training kernel (kind 2 radial basis, 3
+// polynomial), its double centring, SIMPLS on it (scaling none). Outputs:
+// R (n x A), Q (q x A), T (n x A), mx, sx (p), my (q), kernel column means
+// (n) and grand mean, *completed. Predictions of `rows` test rows at
+// `prefix` components go to pred (rows x q) when rows > 0.
+int orc_kernelpls_fit(const double* x, const double* y, const int* labels, int n, int p, int q,
+                      int ncomp, int scaling, int kind, double gamma, int degree,
+                      double offset, int oversample, int power, unsigned int seed, double* R,
+                      double* Q, double* T, double* mx, double* sx, double* my,
+                      double* colmeans, double* grand, int* completed, const double* xtest,
+                      int rows, int prefix, double* pred) {
+  return guarded([&] {
+    if (kind != 2 && kind != 3) throw std::invalid_argument("oracle kernel must be 2 or 3");
+    const Mat X = wrap(x, n, p);
+    Mat Y;
+    if (!labels) Y = wrap(y, n, q);
+    Prepared outer = labels ? prepare_labels(X, labels, size_t(q), scaling)
+                            : prepare_dense(X, Y, scaling, false);
+    Mat K = kernel_dots(outer.Xs, outer.Xs, kind, gamma, degree, offset);
+    KernelCentre kc = centre_train_kernel(K);
+    Prepared inner = labels ? prepare_labels(K, labels, size_t(q), 3) : prepare_dense(K, Y, 3);
+    const size_t cap = std::max<size_t>(size_t(n) - 1, 1);
+    const size_t A = std::max<size_t>(1, std::min<size_t>(size_t(ncomp), cap));
+    SimplsControls ctl = simpls_controls(n, n, q, A, labels != nullptr, oversample, power, seed);
+    SimplsModel m = fit_simpls(inner.Xs, inner.S, ctl);
+    copy_out(m.R, R); copy_out(m.Q, Q);
+    if (T) copy_out(m.T, T);
+    std::copy(outer.mx.begin(), outer.mx.end(), mx);
+    std::copy(outer.sx.begin(), outer.sx.end(), sx);
+    std::copy(inner.my.begin(), inner.my.end(), my);
+    std::copy(kc.colmeans.begin(), kc.colmeans.end(), colmeans);
+    *grand = kc.grand;
+    *completed = int(m.completed);
+    if (rows > 0) {
+      // kernelpls.hpp:155-189 (predict_kernelpls)
+      Mat Xt{size_t(rows), size_t(p)};
+      for (int j = 0; j < p; ++j)
+        for (int i = 0; i < rows; ++i)
+          Xt(size_t(i), size_t(j)) = (xtest[i + size_t(j) * rows] - outer.mx[j]) / outer.sx[j];
+      Mat Kt = kernel_dots(Xt, outer.Xs, kind, gamma, degree, offset);
+      centre_test_kernel(Kt, kc);
+      Mat Rk{size_t(n), size_t(prefix)}, Qk{size_t(q), size_t(prefix)};
+      for (int k = 0; k < prefix; ++k) {
+        for (int i = 0; i < n; ++i) Rk(size_t(i), size_t(k)) = m.R(size_t(i), size_t(k));
+        for (int c = 0; c < q; ++c) Qk(size_t(c), size_t(k)) = m.Q(size_t(c), size_t(k));
+      }
+      Mat Yh = mul(mul(Kt, false, Rk, false), false, Qk, true);
+      for (int c = 0; c < q; ++c)
+        for (int i = 0; i < rows; ++i) Yh(size_t(i), size_t(c)) += inner.my[c];
+      copy_out(Yh, pred);
+    }
+  });
+}
+
 void orc_gemm(const double* left, const double* right, int rows, int inner,
               int cols, double* out) {
   gemm(left, size_t(rows), false, right, size_t(inner), false, size_t(rows),

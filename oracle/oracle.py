@@ -88,6 +88,72 @@ def _audited(lib_fn, check, a, k, oversample, power, seed, left_only):
     return out
 
 
+def _opls_call(fn, check, x, y, labels, classes, ncomp, north, scaling, oversample, power, seed,
+               extra=(), with_reff=False, xtest=None, prefix=None):
+    x = _f64(x); n, p = x.shape
+    lab = _labels(labels)
+    if lab is not None:
+        q = int(classes); yy = None
+    else:
+        yy = _f64(y); q = yy.shape[1]
+    A = max(1, min(ncomp, p, max(n - 1, 1)))
+    W = np.zeros((p, max(north, 1)), order="F"); P = np.zeros((p, max(north, 1)), order="F")
+    R = np.zeros((p, A), order="F"); Q = np.zeros((q, A), order="F"); T = np.zeros((n, A), order="F")
+    Reff = np.zeros((p, A), order="F")
+    mx = np.empty(p); sx = np.empty(p); my = np.empty(q)
+    filt = C.c_int(0); done = C.c_int(0)
+    args = [_d(x), _d(yy), _i(lab), n, p, q, ncomp, north, scaling, oversample, power,
+            C.c_uint(seed), *extra, _d(W), _d(P), C.byref(filt), _d(R), _d(Q), _d(T)]
+    if with_reff:
+        args.append(_d(Reff))
+    args += [_d(mx), _d(sx), _d(my), C.byref(done)]
+    pred = None
+    if not with_reff:
+        xt = _f64(xtest) if xtest is not None else None
+        rows = 0 if xt is None else xt.shape[0]
+        pred = np.zeros((max(rows, 1), q), order="F")
+        args += [_d(xt), rows, int(prefix or A), _d(pred)]
+        if rows == 0:
+            pred = None
+    check(fn(*args))
+    f = filt.value
+    out = dict(W=W[:, :f], P=P[:, :f], filtered=f, R=R, Q=Q, T=T, mx=mx, sx=sx, my=my,
+               completed=done.value)
+    if with_reff:
+        out["Reff"] = Reff
+    if pred is not None:
+        out["pred"] = pred
+    return out
+
+
+def _kernelpls_call(fn, check, x, y, labels, classes, ncomp, scaling, kernel, gamma, degree,
+                    coef0, oversample, power, seed, extra=(), xtest=None, prefix=None,
+                    with_labels=True):
+    x = _f64(x); n, p = x.shape
+    lab = _labels(labels)
+    if lab is not None:
+        q = int(classes); yy = None
+    else:
+        yy = _f64(y); q = yy.shape[1]
+    A = max(1, min(ncomp, max(n - 1, 1)))
+    R = np.zeros((n, A), order="F"); Q = np.zeros((q, A), order="F"); T = np.zeros((n, A), order="F")
+    mx = np.empty(p); sx = np.empty(p); my = np.empty(q); cm = np.empty(n)
+    grand = C.c_double(0); done = C.c_int(0)
+    xt = _f64(xtest) if xtest is not None else None
+    rows = 0 if xt is None else xt.shape[0]
+    pred = np.zeros((max(rows, 1), q), order="F")
+    head = [_d(x), _d(yy)] + ([_i(lab)] if with_labels else [])
+    check(fn(*head, n, p, q, ncomp, scaling, kernel, C.c_double(gamma), degree,
+             C.c_double(coef0), oversample, power, C.c_uint(seed), *extra, _d(R), _d(Q), _d(T),
+             _d(mx), _d(sx), _d(my), _d(cm), C.byref(grand), C.byref(done), _d(xt), rows,
+             int(prefix or A), _d(pred)))
+    out = dict(R=R, Q=Q, T=T, mx=mx, sx=sx, my=my, colmeans=cm, grand=grand.value,
+               completed=done.value)
+    if rows:
+        out["pred"] = pred
+    return out
+
+
 class Oracle:
     """FP64 restatement of the reference algorithms (column-major)."""
 
@@ -324,6 +390,20 @@ class Oracle:
         return _lda_moments(self.lib.orc_lda_moments, self._check, gram, class_sums,
                             np.ascontiguousarray(counts, dtype=np.int32), _i, n, prefix)
 
+    def opls_fit(self, x, y=None, labels=None, classes=None, ncomp=2, north=1, scaling=2,
+                 oversample=32, power=5, seed=1):
+        """OPLS (orthogonal filter + SIMPLS on the filtered predictors) with
+        R_eff, the weights that take standardised rows to the scores."""
+        return _opls_call(self.lib.orc_opls_fit, self._check, x, y, labels, classes, ncomp,
+                          north, scaling, oversample, power, seed, with_reff=True)
+
+    def kernelpls_fit(self, x, y=None, labels=None, classes=None, ncomp=2, scaling=2, kernel=2,
+                      gamma=1.0, degree=3, coef0=1.0, oversample=32, power=5, seed=1,
+                      xtest=None, prefix=None):
+        return _kernelpls_call(self.lib.orc_kernelpls_fit, self._check, x, y, labels, classes,
+                               ncomp, scaling, kernel, gamma, degree, coef0, oversample, power,
+                               seed, xtest=xtest, prefix=prefix)
+
     def plssvd_weights(self, gram, sigma, Q):
         """PLS-SVD prefix weights (plssvd.hpp:138-164) and the solve route per prefix."""
         return _plssvd_weights(self.lib.orc_plssvd_weights, self._check, gram, sigma, Q)
@@ -474,6 +554,22 @@ class Reference:
     def lda_moments(self, gram, class_sums, counts, n, prefix):
         return _lda_moments(self.lib.ref_lda_moments, self._check, gram, class_sums,
                             np.ascontiguousarray(counts, dtype=np.float64), _d, n, prefix)
+
+    def opls_fit(self, x, y=None, labels=None, classes=None, ncomp=2, north=1, scaling=2,
+                 oversample=32, power=5, seed=1, block=1, cache=0, reorth=0, batch_geometry=0,
+                 xtest=None, prefix=None):
+        return _opls_call(self.lib.ref_opls_fit, self._check, x, y, labels, classes, ncomp,
+                          north, scaling, oversample, power, seed,
+                          extra=(block, cache, reorth, batch_geometry), xtest=xtest,
+                          prefix=prefix)
+
+    def kernelpls_fit(self, x, y, ncomp=2, scaling=2, kernel=2, gamma=1.0, degree=3, coef0=1.0,
+                      oversample=32, power=5, seed=1, block=1, cache=0, reorth=0,
+                      batch_geometry=0, xtest=None, prefix=None):
+        return _kernelpls_call(self.lib.ref_kernelpls_fit, self._check, x, y, None, None, ncomp,
+                               scaling, kernel, gamma, degree, coef0, oversample, power, seed,
+                               extra=(block, cache, reorth, batch_geometry), xtest=xtest,
+                               prefix=prefix, with_labels=False)
 
     def plssvd_weights(self, gram, sigma, Q):
         return _plssvd_weights(self.lib.ref_plssvd_weights, self._check, gram, sigma, Q)

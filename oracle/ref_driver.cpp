@@ -239,6 +239,127 @@ int ref_plssvd_operator_fit(const double* x, const double* y, int n, int p, int 
   });
 }
 
+// OPLS as the reference core composes it. Dense responses: fit_opls and
+// predict_opls (oplspls.hpp:33-122: the randomized orthogonal filter of
+// opls.hpp:314-363, then SIMPLS on the filtered predictors). One-based
+// labels: the label filter of opls.hpp:418-465, then SIMPLS on the label
+// cross-product of the filtered predictors (scaling none), as the resident
+// engine composes them (src/cuda_resident_special.cuh:302-305). Outputs: W,
+// P (p x north), *filtered, inner R, Q, T (p x A, q x A, n x A), outer mx,
+// sx, my, *completed; dense fits also predict `rows` test rows at `prefix`
+// components into pred (rows x q).
+int ref_opls_fit(const double* x, const double* y, const int* labels, int n, int p, int q,
+                 int A, int north, int scaling, int oversample, int power, unsigned int seed,
+                 int block, int cache, int reorth, int batch_geometry, double* W, double* P,
+                 int* filtered, double* R, double* Q, double* T, double* mx, double* sx,
+                 double* my, int* completed, const double* xtest, int rows, int prefix,
+                 double* pred) {
+  return guarded([&] {
+    ReferenceBackend<double> backend;
+    fastpls::core::RsvdControls rc;
+    rc.oversample = oversample; rc.power = power; rc.seed = seed;
+    fastpls::core::SimplsControls controls;
+    controls.components = std::size_t(A);
+    controls.maximum_block = std::size_t(block);
+    controls.cache_predictor_crossprod = cache != 0;
+    controls.batch_candidate_geometry = batch_geometry != 0;
+    controls.reorthogonalize = reorth != 0;
+    controls.store_scores = true;
+    controls.use_right_gram = true;
+    controls.rsvd = rc;
+    if (!labels) {
+      Matrix<double> Y = load(y, n, q);
+      fastpls::core::OplsControls oc;
+      oc.orthogonal_components = std::size_t(north);
+      oc.scaling = static_cast<PredictorScaling>(scaling);
+      oc.randomized_filter = true;
+      oc.filter_rsvd = rc;
+      oc.simpls = controls;
+      auto model = fastpls::core::fit_opls(load(x, n, p), ConstMatrixView<double>(Y.view()), oc,
+                                           backend);
+      store(model.filter.weights, W); store(model.filter.loadings, P);
+      *filtered = int(model.filter.completed_components);
+      store(model.inner.weights, R); store(model.inner.response_loadings, Q);
+      store(model.inner.scores, T);
+      std::copy(model.filter.predictor_center.begin(), model.filter.predictor_center.end(), mx);
+      std::copy(model.filter.predictor_scale.begin(), model.filter.predictor_scale.end(), sx);
+      std::copy(model.response_mean.begin(), model.response_mean.end(), my);
+      *completed = int(model.inner.completed_components);
+      if (rows > 0) {
+        auto yh = fastpls::core::predict_opls(model, load(xtest, rows, p), std::size_t(prefix),
+                                              backend);
+        store(yh, pred);
+      }
+      return;
+    }
+    std::vector<std::size_t> zero_based(n);
+    for (int i = 0; i < n; ++i) zero_based[i] = std::size_t(labels[i] - 1);
+    auto filter = fastpls::core::fit_opls_filter_labels_rsvd(
+      load(x, n, p), zero_based.data(), std::size_t(n), std::size_t(q), std::size_t(north),
+      static_cast<PredictorScaling>(scaling), rc, backend);
+    auto pr = fastpls::core::prepare_scaled_label_crossprod(
+      filter.predictors.view(), zero_based.data(), std::size_t(n), std::size_t(q),
+      PredictorScaling::none, backend);
+    fastpls::core::SimplsWorkspace<double> workspace;
+    auto model = fastpls::core::fit_simpls_preprocessed<double>(
+      ConstMatrixView<double>(filter.predictors.view()), ConstMatrixView<double>(pr.crossprod.view()),
+      controls, backend, workspace);
+    store(filter.weights, W); store(filter.loadings, P);
+    *filtered = int(filter.completed_components);
+    store(model.weights, R); store(model.response_loadings, Q); store(model.scores, T);
+    std::copy(filter.predictor_center.begin(), filter.predictor_center.end(), mx);
+    std::copy(filter.predictor_scale.begin(), filter.predictor_scale.end(), sx);
+    std::copy(pr.response_mean.begin(), pr.response_mean.end(), my);
+    *completed = int(model.completed_components);
+  });
+}
+
+// Kernel PLS (kernelpls.hpp:87-189): fit with the given kernel controls and
+// SIMPLS controls, then predict `rows` test rows at `prefix` components.
+// Outputs: R (n x A), Q, T, mx, sx, my, kernel column means (n), grand mean,
+// *completed, pred (rows x q).
+int ref_kernelpls_fit(const double* x, const double* y, int n, int p, int q, int A, int scaling,
+                      int kind, double gamma, int degree, double offset, int oversample,
+                      int power, unsigned int seed, int block, int cache, int reorth,
+                      int batch_geometry, double* R, double* Q, double* T, double* mx,
+                      double* sx, double* my, double* colmeans, double* grand,
+                      int* completed, const double* xtest, int rows, int prefix,
+                      double* pred) {
+  return guarded([&] {
+    ReferenceBackend<double> backend;
+    Matrix<double> Y = load(y, n, q);
+    fastpls::core::KernelPlsControls controls;
+    controls.kernel = static_cast<fastpls::core::KernelType>(kind);
+    controls.gamma = gamma; controls.degree = degree; controls.offset = offset;
+    controls.scaling = static_cast<PredictorScaling>(scaling);
+    controls.simpls.components = std::size_t(A);
+    controls.simpls.maximum_block = std::size_t(block);
+    controls.simpls.cache_predictor_crossprod = cache != 0;
+    controls.simpls.batch_candidate_geometry = batch_geometry != 0;
+    controls.simpls.reorthogonalize = reorth != 0;
+    controls.simpls.store_scores = true;
+    controls.simpls.use_right_gram = true;
+    controls.simpls.rsvd.oversample = oversample;
+    controls.simpls.rsvd.power = power;
+    controls.simpls.rsvd.seed = seed;
+    auto model = fastpls::core::fit_kernelpls(load(x, n, p), ConstMatrixView<double>(Y.view()),
+                                              controls, backend);
+    store(model.inner.weights, R); store(model.inner.response_loadings, Q);
+    store(model.inner.scores, T);
+    std::copy(model.predictor_center.begin(), model.predictor_center.end(), mx);
+    std::copy(model.predictor_scale.begin(), model.predictor_scale.end(), sx);
+    std::copy(model.response_mean.begin(), model.response_mean.end(), my);
+    std::copy(model.kernel_column_means.begin(), model.kernel_column_means.end(), colmeans);
+    *grand = model.kernel_grand_mean;
+    *completed = int(model.inner.completed_components);
+    if (rows > 0) {
+      auto yh = fastpls::core::predict_kernelpls(model, load(xtest, rows, p),
+                                                 std::size_t(prefix), backend);
+      store(yh, pred);
+    }
+  });
+}
+
 // Explicit randomized SVD of a dense matrix (rsvd.hpp:185-237).
 int ref_rsvd(const double* a, int m, int n, int k, int oversample, int power,
              unsigned int seed, double* U, double* s, double* Vt, int* got) {

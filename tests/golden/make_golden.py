@@ -190,6 +190,49 @@ def edge_cases(ref):
     save("plssvd_pivoted", **arrays)
 
 
+def special_cases(ref):
+    """OPLS (the reference core's fit_opls / predict_opls for dense responses,
+    the label filter + label SIMPLS for classes) and kernel PLS
+    (fit_kernelpls / predict_kernelpls): the dense-SVD and sketch routes of
+    the filter direction, radial-basis and polynomial kernels."""
+    cases = [("opls_dense", dict(n=80, p=30, q=2, factors=6, seed=3, ncomp=4, north=2,
+                                 oversample=4, power=2, rsvd_seed=7, scaling=2)),
+             ("opls_sketch", dict(n=80, p=30, q=6, factors=6, seed=3, ncomp=4, north=2,
+                                  oversample=2, power=1, rsvd_seed=7, scaling=1))]
+    for name, c in cases:
+        X, Y = latent_data(c["n"], c["p"], c["q"], factors=c["factors"], seed=c["seed"])
+        Xt, _ = latent_data(20, c["p"], c["q"], factors=c["factors"], seed=c["seed"] + 1)
+        m = ref.opls_fit(X, Y, ncomp=c["ncomp"], north=c["north"], scaling=c["scaling"],
+                         oversample=c["oversample"], power=c["power"], seed=c["rsvd_seed"],
+                         xtest=Xt, prefix=3)
+        save(name, X=X, Y=Y, Xt=Xt, W=m["W"], P=m["P"], R=m["R"], Q=m["Q"], T=m["T"],
+             mx=m["mx"], sx=m["sx"], my=m["my"], pred=m["pred"], prefix=np.int64(3),
+             filtered=np.int64(m["filtered"]), completed=np.int64(m["completed"]),
+             ncomp=np.int64(c["ncomp"]), north=np.int64(c["north"]),
+             scaling=np.int64(c["scaling"]), oversample=np.int64(c["oversample"]),
+             power=np.int64(c["power"]), seed=np.int64(c["rsvd_seed"]))
+    X, lab = class_data(90, 25, 4, latent=4, seed=5)
+    m = ref.opls_fit(X, labels=lab, classes=4, ncomp=3, north=2, oversample=1, power=1, seed=2)
+    save("opls_labels", X=X, labels=np.asarray(lab, dtype=np.int32), classes=np.int64(4),
+         W=m["W"], P=m["P"], R=m["R"], Q=m["Q"], T=m["T"], mx=m["mx"], sx=m["sx"], my=m["my"],
+         filtered=np.int64(m["filtered"]), completed=np.int64(m["completed"]),
+         ncomp=np.int64(3), north=np.int64(2), scaling=np.int64(2), oversample=np.int64(1),
+         power=np.int64(1), seed=np.int64(2))
+    X, Y = latent_data(70, 12, 2, factors=5, seed=11)
+    Xt, _ = latent_data(15, 12, 2, factors=5, seed=12)
+    for name, kernel, gamma, degree, coef0, scaling in (("kernelpls_rbf", 2, 0.05, 3, 1.0, 2),
+                                                        ("kernelpls_poly", 3, 0.1, 2, 1.0, 1)):
+        m = ref.kernelpls_fit(X, Y, ncomp=4, scaling=scaling, kernel=kernel, gamma=gamma,
+                              degree=degree, coef0=coef0, oversample=4, power=2, seed=5,
+                              xtest=Xt, prefix=3)
+        save(name, X=X, Y=Y, Xt=Xt, R=m["R"], Q=m["Q"], T=m["T"], mx=m["mx"], sx=m["sx"],
+             my=m["my"], colmeans=m["colmeans"], grand=np.float64(m["grand"]), pred=m["pred"],
+             prefix=np.int64(3), completed=np.int64(m["completed"]), ncomp=np.int64(4),
+             scaling=np.int64(scaling), kernel=np.int64(kernel), gamma=np.float64(gamma),
+             degree=np.int64(degree), coef0=np.float64(coef0), oversample=np.int64(4),
+             power=np.int64(2), seed=np.int64(5))
+
+
 def main():
     build(reference=True)
     ref = Reference()
@@ -202,9 +245,13 @@ def main():
     if len(sys.argv) > 1 and sys.argv[1] == "edges":
         edge_cases(ref)
         return
+    if len(sys.argv) > 1 and sys.argv[1] == "special":
+        special_cases(ref)
+        return
     audited_cases(ref)
     operator_cases(ref)
     edge_cases(ref)
+    special_cases(ref)
 
     # SIMPLS, dense responses, explicit width = q (direction independent of the sketch)
     X, Y = latent_data(64, 20, 3, factors=5, noise=0.1, seed=11)

@@ -528,3 +528,94 @@ def test_oracle_plssvd_pivoted_golden(oracle):
         np.testing.assert_allclose(o["W"][k - 1], g[f"W{k}"], rtol=1e-12, atol=1e-12)
     with pytest.raises(RuntimeError, match="latent solve failed"):
         oracle.plssvd_weights(g["G_singular"], g["sigma"][:2], g["Q"][:, :2])
+
+
+# ------------------------------------------------------- OPLS and kernel PLS
+def opls_predict(m, x, prefix):
+    """Ŷ of an oracle OPLS fit by filtering the standardised rows explicitly
+    (opls.hpp:467-509) and applying the inner SIMPLS model."""
+    xs = (x - m["mx"]) / m["sx"]
+    for c in range(m["W"].shape[1]):
+        s = xs @ m["W"][:, c]
+        xs = xs - np.outer(s, m["P"][:, c])
+    return xs @ m["R"][:, :prefix] @ m["Q"][:, :prefix].T + m["my"]
+
+
+@pytest.mark.parametrize("name", ["opls_dense", "opls_sketch", "opls_labels"])
+def test_opls_matches_reference_core(oracle, name):
+    g = load_golden(name)
+    kw = dict(ncomp=int(g["ncomp"]), north=int(g["north"]), scaling=int(g["scaling"]),
+              oversample=int(g["oversample"]), power=int(g["power"]), seed=int(g["seed"]))
+    if "labels" in g:
+        m = oracle.opls_fit(g["X"], labels=g["labels"], classes=int(g["classes"]), **kw)
+    else:
+        m = oracle.opls_fit(g["X"], g["Y"], **kw)
+    assert m["filtered"] == int(g["filtered"]) and m["completed"] == int(g["completed"])
+    np.testing.assert_allclose(m["mx"], g["mx"], rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(m["sx"], g["sx"], rtol=1e-13)
+    np.testing.assert_allclose(m["my"], g["my"], rtol=1e-13, atol=1e-15)
+    # the filter's weights and loadings, up to a common sign per component
+    assert rel(align_columns(m["W"], g["W"]), g["W"]) < 1e-10
+    assert rel(align_columns(m["P"], g["P"]), g["P"]) < 1e-10
+    for k in range(1, int(g["ncomp"]) + 1):
+        assert rel(coef(m, k), coef(g, k)) < 1e-9, k
+    if "pred" in g:
+        k = int(g["prefix"])
+        assert rel(opls_predict(m, g["Xt"], k), g["pred"]) < 1e-10
+        # R_eff takes standardised rows straight to the inner scores
+        xs = (g["Xt"] - m["mx"]) / m["sx"]
+        pe = xs @ m["Reff"][:, :k] @ m["Q"][:, :k].T + m["my"]
+        assert rel(pe, g["pred"]) < 1e-10
+
+
+def test_opls_filter_invariants(oracle):
+    """The filtered predictors annihilate every orthogonal weight
+    (X_f w_o = X w_o (1 - p_oᵀ w_o) with p_oᵀ w_o = 1), and with no filter
+    component OPLS is SIMPLS on the same standardised predictors."""
+    X, Y = latent_data(120, 25, 3, factors=5, seed=21)
+    m = oracle.opls_fit(X, Y, ncomp=3, north=2, oversample=4, power=2, seed=5)
+    prep = oracle.prepare(X, Y)
+    xf = prep["xs"].copy()
+    for c in range(2):
+        wo, po = m["W"][:, c], m["P"][:, c]
+        assert abs(po @ wo - 1.0) < 1e-12
+        xf = xf - np.outer(xf @ wo, po)
+    for c in range(2):
+        assert np.abs(xf @ m["W"][:, c]).max() < 1e-10 * np.abs(xf).max()
+    m0 = oracle.opls_fit(X, Y, ncomp=3, north=0, oversample=4, power=2, seed=5)
+    s = oracle.simpls_fit(X, Y, ncomp=3, oversample=4, power=2, seed=5)
+    for k in (1, 3):
+        assert rel(coef(m0, k), coef(s, k)) < 1e-10
+    np.testing.assert_allclose(m0["Reff"], m0["R"], rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("name", ["kernelpls_rbf", "kernelpls_poly"])
+def test_kernelpls_matches_reference_core(oracle, name):
+    g = load_golden(name)
+    m = oracle.kernelpls_fit(g["X"], g["Y"], ncomp=int(g["ncomp"]), scaling=int(g["scaling"]),
+                             kernel=int(g["kernel"]), gamma=float(g["gamma"]),
+                             degree=int(g["degree"]), coef0=float(g["coef0"]),
+                             oversample=int(g["oversample"]), power=int(g["power"]),
+                             seed=int(g["seed"]), xtest=g["Xt"], prefix=int(g["prefix"]))
+    assert m["completed"] == int(g["completed"])
+    np.testing.assert_allclose(m["mx"], g["mx"], rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(m["sx"], g["sx"], rtol=1e-13)
+    np.testing.assert_allclose(m["colmeans"], g["colmeans"], rtol=1e-12, atol=1e-14)
+    assert abs(m["grand"] - float(g["grand"])) < 1e-12 * max(1.0, abs(float(g["grand"])))
+    for k in range(1, int(g["ncomp"]) + 1):
+        assert rel(coef(m, k), coef(g, k)) < 1e-9, k
+    assert rel(m["pred"], g["pred"]) < 1e-10
+
+
+def test_kernelpls_closed_forms(oracle):
+    """The centred training kernel has zero row and column sums, so the
+    training scores T = K_c R have zero column means; predicting the
+    training rows through the test-kernel path (kernel against the training
+    rows, test centring) reproduces the fitted values T_k Q_kᵀ + μ_y."""
+    X, Y = latent_data(60, 8, 2, factors=4, seed=31)
+    for kernel, gamma, degree in ((2, 0.1, 3), (3, 0.2, 2)):
+        m = oracle.kernelpls_fit(X, Y, ncomp=3, kernel=kernel, gamma=gamma, degree=degree,
+                                 oversample=4, power=2, seed=3, xtest=X, prefix=3)
+        assert np.abs(m["T"].mean(axis=0)).max() < 1e-12 * np.abs(m["T"]).max()
+        fitted = m["T"] @ m["Q"].T + m["my"]
+        assert rel(m["pred"], fitted) < 1e-12
