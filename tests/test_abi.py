@@ -61,3 +61,67 @@ def test_no_cpu_fallback_without_a_device():
     with pytest.raises(fastpls_amd.HipError, match="no HIP device"):
         fastpls_amd.ResidentModel.fit([[1.0, 2.0], [3.0, 5.0], [1.0, 0.0]], [[1.0], [2.0], [0.0]],
                                       components=1)
+
+
+# The reference's resident ABI (src/cuda_resident_api.h:11-77), restated as
+# the interface a caller such as src/cuda_resident_r.cpp is written against:
+# the same names, return types and parameter lists.
+REFERENCE_PROTOTYPES = r"""
+int fastpls_cuda_gemm(const void*, const void*, int, int, int, int, void*, char*, size_t);
+int fastpls_resident_simpls_cv_classification(const void*, const int*, const int*, int, int,
+    int, int, const int*, int, int, int, int, int, unsigned long long, int, int, int*, void*,
+    int*, double*, char*, size_t);
+int fastpls_resident_simpls_cv_regression(const void*, const void*, const int*, int, int, int,
+    int, const int*, int, int, int, int, int, unsigned long long, int, double*, int*, double*,
+    double*, double*, double*, char*, size_t);
+int fastpls_resident_plssvd_cv_classification(const void*, const int*, const int*, int, int,
+    int, int, const int*, int, int, int, int, int, unsigned long long, int, int, int*, void*,
+    int*, double*, char*, size_t);
+int fastpls_resident_plssvd_cv_regression(const void*, const void*, const int*, int, int, int,
+    int, const int*, int, int, int, int, int, unsigned long long, int, double*, int*, double*,
+    double*, double*, double*, char*, size_t);
+void* fastpls_resident_simpls_create(const void*, const void*, const int*, int, int, int, int,
+    int, int, int, int, int, unsigned long long, char*, size_t);
+void* fastpls_resident_plssvd_create(const void*, const void*, const int*, int, int, int, int,
+    int, int, int, int, int, unsigned long long, char*, size_t);
+void* fastpls_resident_opls_create(const void*, const void*, const int*, int, int, int, int,
+    int, int, int, int, int, unsigned long long, int, char*, size_t);
+void* fastpls_resident_kernelpls_create(const void*, const void*, const int*, int, int, int,
+    int, int, int, int, int, int, unsigned long long, int, double, int, double, char*, size_t);
+int fastpls_resident_predict_path(void*, const void*, int, const int*, int, int, void*, char*,
+    size_t);
+void fastpls_resident_simpls_destroy(void*);
+int fastpls_resident_export(void*, int, void*, size_t, char*, size_t);
+int fastpls_resident_classify_path(void*, const void*, int, const int*, int, int, int, int*,
+    char*, size_t);
+int fastpls_resident_classify_response_path(void*, const void*, int, const int*, int, int, int,
+    int*, void*, char*, size_t);
+int fastpls_resident_response_sums(void*, const void*, const void*, const int*, int, int, void*,
+    char*, size_t);
+int fastpls_resident_project(void*, const void*, int, int, void*, char*, size_t);
+int fastpls_resident_controls(void*, int*, int*, int*, int*, int*, int*, char*, size_t);
+int fastpls_resident_compact(void*, int, char*, size_t);
+"""
+
+
+def test_reference_named_caller_links_through_the_alias_header(tmp_path):
+    """A C++ caller written against the reference's names compiles with the
+    alias header (every reference prototype, redeclared after it, must match
+    the HIP declaration it maps to, or the compiler rejects the conflicting
+    extern "C" declaration) and links against libfastpls_hip.so."""
+    if not os.path.exists(LIB):
+        import fastpls_amd.build as b
+        b.build()
+    names = re.findall(r"\b(fastpls_(?:cuda|resident)_[a-z_]+)\s*\(", REFERENCE_PROTOTYPES)
+    assert len(names) == 18
+    src = tmp_path / "caller.cpp"
+    src.write_text(
+        "#include <cstddef>\n#include \"fastpls_resident_alias.h\"\n"
+        "extern \"C\" {\n" + REFERENCE_PROTOTYPES + "}\n"
+        "int main() {\n  void* table[] = {" + ", ".join(f"(void*)&{n}" for n in names) + "};\n"
+        "  return table[0] == nullptr;\n}\n")
+    exe = tmp_path / "caller"
+    subprocess.run(["g++", "-std=c++17", "-Werror", f"-I{os.path.join(ROOT, 'include')}", str(src),
+                    f"-L{os.path.dirname(LIB)}", "-lfastpls_hip",
+                    f"-Wl,-rpath,{os.path.dirname(LIB)}", "-Wl,--unresolved-symbols=ignore-in-shared-libs",
+                    "-o", str(exe)], check=True)
