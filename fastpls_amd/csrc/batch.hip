@@ -506,6 +506,7 @@ class SimplsBatch {
       o[7] = arena_.add(8 * (size_t)simpls_scratch_doubles(G.p, F.q, F.wmax));
       o[8] = 0;                                                   // state: in the status block
       o[9] = arena_.add(sizeof(int) * q);                         // counts
+      o[10] = reorth_ == 1 ? arena_.add(8 * p * A) : 0;           // P (re-orthogonalising)
       o[11] = fits_[f].perm && fits_[f].labels ? arena_.add(sizeof(int) * n) : 0;  // permuted labels
       o[12] = up_.add(sizeof(RefreshStep) * F.host_steps.size());  // uploaded
       o[13] = up_.add(8 * F.host_omega.size());                    // uploaded
@@ -579,6 +580,7 @@ class SimplsBatch {
       d.ts_part = nullptr;
       d.ls_part = G.loads + (size_t)F.boff * G.p;
       d.R = F.R; d.Q = F.Q; d.V = F.V; d.T = F.Tsc;
+      d.P = reorth_ == 1 ? arena_.at<double>(o[10]) : nullptr;
       d.scratch = F.scratch; d.state = F.state;
     }
     // a non-finite cross-product column flags the fit that owns it

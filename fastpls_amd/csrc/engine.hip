@@ -100,6 +100,10 @@ class Model final : public Handle, private StreamOps {
       prepare_crossprod();
       if (method_ == kSimpls) fit_simpls(); else fit_plssvd();
       finish_fit();
+      if (!P_.empty()) {  // the re-orthogonalisation's loadings serve the fit only
+        FPLS_HIP(hipStreamSynchronize(stream_));
+        P_.release();
+      }
       if (!keep_scores_) {
         ensure_lda_moments();  // classification only: its moments come from the scores
         FPLS_HIP(hipStreamSynchronize(stream_));
@@ -379,7 +383,7 @@ class Model final : public Handle, private StreamOps {
       ensure_lda_moments();
     }
     FPLS_HIP(hipStreamSynchronize(stream_));
-    for (Buf* b : {&xown_, &yown_, &Yc_, &S_, &S0_, &G_, &V_, &Tsc_, &xgram_, &simpls_scratch_, &cand_,
+    for (Buf* b : {&xown_, &yown_, &Yc_, &S_, &S0_, &G_, &V_, &Tsc_, &P_, &xgram_, &simpls_scratch_, &cand_,
                    &cand_scaled_, &t_raw_, &ts_part_, &ls_part_, &omega_, &steps_, &fitdev_,
                    &part_, &tmp1_, &tmp2_, &tmp3_, &H_, &labels_dev_, &frag_, &cp_part_,
                    &cp_stats_, &qr_gram_, &qr_rinv_, &qr_tmp_, &qr_fail_, &qraw_, &op_sig_,
@@ -397,7 +401,7 @@ class Model final : public Handle, private StreamOps {
 
   std::vector<Buf*> all_bufs() {
     std::vector<Buf*> v = {&xown_, &yown_, &labels_dev_, &counts_, &Yc_, &H_, &ysum_, &my_, &mx_,
-                           &sx_, &colss_, &S_, &S0_, &G_, &R_, &Q_, &V_, &Tsc_, &xgram_, &Rs_, &sigma_,
+                           &sx_, &colss_, &S_, &S0_, &G_, &R_, &Q_, &V_, &Tsc_, &P_, &xgram_, &Rs_, &sigma_,
                            &gram_, &W_, &state_, &omega_, &steps_, &fitdev_, &simpls_scratch_,
                            &cand_, &cand_scaled_, &t_raw_, &ts_part_, &ls_part_, &tmp1_,
                            &tmp2_, &tmp3_, &cp_part_, &cp_stats_, &usable_, &rows_, &scores_,
@@ -672,11 +676,12 @@ class Model final : public Handle, private StreamOps {
     SimplsFitDev f{};
     f.n = n_; f.p = p_; f.q = q_; f.A = A_;
     f.power = power_; f.use_right_gram = right_gram ? 1 : 0;
-    // the rank-one route re-orthogonalises twice (src/r_api.cpp:1491); the
-    // reference's float core does too (src/r_api.cpp:1416) to offset float
-    // arithmetic, which this engine's FP32-stored fits do not have: their
-    // products and small math are FP64, so both precisions take the FP64
-    // routing (one pass) and give the same fit on the same values
+    // the rank-one route re-orthogonalises (src/r_api.cpp:1491): scores and
+    // weights against the earlier scores, loadings twice; the reference's
+    // float core does too (src/r_api.cpp:1416) to offset float arithmetic,
+    // which this engine's FP32-stored fits do not have: their products and
+    // small math are FP64, so both precisions take the FP64 routing (one
+    // pass) and give the same fit on the same values
     f.reorth = rank_one_ ? 1 : 0;
     f.nsteps = int(steps_ref.size()); f.bmax = bmax; f.ts_splits = xs; f.ls_splits = ls;
     f.ls_slab = (long long)p_ * bmax;
@@ -692,6 +697,8 @@ class Model final : public Handle, private StreamOps {
     f.ts_part = ts_part_.d(); f.ls_part = ls_part_.d();
     f.R = R_.d(); f.Q = Q_.d(); f.V = V_.d(); f.T = Tsc_.d();
     f.xgram = cache_gram_ ? xgram_.d() : nullptr;
+    if (f.reorth) P_.alloc(sizeof(double) * (size_t)p_ * A_);
+    f.P = f.reorth ? P_.d() : nullptr;
     f.scratch = simpls_scratch_.d(); f.state = state_.as<int>();
     fitdev_host_ = f;
     fitdev_.alloc(sizeof(SimplsFitDev));
@@ -1193,6 +1200,7 @@ class Model final : public Handle, private StreamOps {
   const T* X_ = nullptr;
   Buf xown_, yown_, labels_dev_, counts_, Yc_, H_, ysum_, my_, mx_, sx_, colss_, S_, S0_, G_;
   Buf R_, Q_, V_, Tsc_, Rs_, sigma_, gram_, W_, state_;
+  Buf P_;  // loadings X_sᵀ T of a re-orthogonalising fit (fit time only)
   Buf omega_, steps_, fitdev_, simpls_scratch_, cand_, cand_scaled_, t_raw_, ts_part_, ls_part_;
   Buf tmp1_, tmp2_, tmp3_;
   // fit workspaces that outlive the kernels reading them (no mid-fit frees)

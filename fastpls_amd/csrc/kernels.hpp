@@ -46,6 +46,10 @@ struct SimplsFitDev {
   // the update forms the loading X_sᵀ X_s r and |t|² = rᵀ X_sᵀ X_s r from it
   // (no loading pass; T, when not null, is t_raw / |t|)
   const double* xgram;
+  // reorth: p x A loadings X_sᵀ T of the components so far (before their
+  // orthogonalisation), so that the loading of a score with the earlier
+  // scores projected out follows without another pass over X; needs T
+  double* P;
 };
 
 // Error flags written to state[2].

@@ -33,7 +33,7 @@ struct WgScratch {
 
 // Scratch layout of one SIMPLS fit (doubles).
 struct RefreshLayout {
-  long long Z, Y2, sample, Qb, Bm, Gm, Ve, vtmp, evals, wvec, proj, row, total;
+  long long Z, Y2, sample, Qb, Bm, Gm, Ve, vtmp, evals, wvec, proj, ptot, row, total;
   __host__ __device__ RefreshLayout(int p, int q, int w, int A) {
     long long o = 0;
     Z = o; o += (long long)q * w;
@@ -47,6 +47,7 @@ struct RefreshLayout {
     evals = o; o += w;
     wvec = o; o += p;
     proj = o; o += (A > 0 ? A : 1);
+    ptot = o; o += (A > 0 ? A : 1);
     row = o; o += q;
     total = o;
   }
@@ -442,16 +443,20 @@ __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits,
   double* row = f.scratch + L.row;
   // out (m) = Xᵀ v for X (p x m): one thread per row of X, 32 columns of X
   // per pass (more partial sums per thread spill registers) when m <= 128
-  auto dots = [&](int m, const double* X, const double* v, double* out) {
+  // (len rows: p, or n for the scores)
+  auto dots_len = [&](int m, int len, const double* X, const double* v, double* out) {
     if (m <= 16) {
-      wg::gemm_tv_rows<16>(m, p, X, p, v, out, red);
+      wg::gemm_tv_rows<16>(m, len, X, len, v, out, red);
     } else if (m <= 128) {
       for (int c0 = 0; c0 < m; c0 += 32)
-        wg::gemm_tv_rows<32>(m - c0 < 32 ? m - c0 : 32, p, X + (long long)c0 * p, p, v, out + c0,
-                             red);
+        wg::gemm_tv_rows<32>(m - c0 < 32 ? m - c0 : 32, len, X + (long long)c0 * len, len, v,
+                             out + c0, red);
     } else {
-      wg::gemm_tn_long(m, 1, p, X, p, v, p, out, m);
+      wg::gemm_tn_long(m, 1, len, X, len, v, len, out, m);
     }
+  };
+  auto dots = [&](int m, const double* X, const double* v, double* out) {
+    dots_len(m, p, X, v, out);
   };
   auto stop = [&](double x) {
     set_flag(f.state, isfinite(x) ? kFlagZeroNorm : kFlagNonFinite);
@@ -461,7 +466,7 @@ __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits,
   for (int j = 0; j < st.b; ++j) {
     const int comp = st.a + j;
     if (comp >= f.A) break;
-    const double* t = f.t_raw + (long long)j * n;
+    double* t = f.t_raw + (long long)j * n;
     double* Rc = f.R + (long long)comp * p;
     double* Vc = f.V + (long long)comp * p;
     double* Qc = f.Q + (long long)comp * q;
@@ -491,13 +496,32 @@ __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits,
       scale_copy(Vc, Vc, p, inv);
       __syncthreads();
     } else {
+      // re-orthogonalising (simpls.hpp:161-200, 635-642): twice, the
+      // projection of t on the earlier scores T is removed from t, and the
+      // same combination of the earlier weights R from r; the loading
+      // X_sᵀ t then is the block's loading minus P (the earlier loadings
+      // X_sᵀ T) times the summed projections
+      double* cand = f.cand + (long long)j * p;
+      const bool rsb = f.reorth && comp > 0 && f.T != nullptr && f.P != nullptr;
+      double* ptot = f.scratch + L.ptot;
+      if (rsb) {
+        for (int pass = 0; pass < 2; ++pass) {
+          dots_len(comp, n, f.T, t, proj);
+          __syncthreads();
+          orth_pass(t, n, f.T, proj, comp, false);
+          orth_pass(cand, p, f.R, proj, comp, false);
+          for (int c = threadIdx.x; c < comp; c += kWG) ptot[c] = pass ? ptot[c] + proj[c] : proj[c];
+          __syncthreads();
+        }
+      }
       const double tn = sqrt(fmax(sumsq_stream(t, n, sh.red), 0.0));
       if (!(tn > 0.0) || !isfinite(tn)) return stop(tn);
       inv = 1.0 / tn;
       if (f.T) scale_copy(f.T + (long long)comp * n, t, n, inv);
       // one pass over the rows: R = r / |t| and loading = (sum of the
-      // row-split partials, in split order) / sx / |t|
-      const double* cand = f.cand + (long long)j * p;
+      // row-split partials, in split order) / sx / |t| (the division by |t|
+      // follows the correction when re-orthogonalising)
+      const double lscale = rsb ? 1.0 : inv;
       const double* part = f.ls_part + (long long)j * p;
       const int splits = f.ls_splits;
       for (int i0 = threadIdx.x; i0 < p; i0 += 4 * kWG) {
@@ -527,15 +551,34 @@ __global__ __launch_bounds__(kWG) void k_simpls_update(const SimplsFitDev* fits,
 #pragma unroll
         for (int u = 0; u < 4; ++u)
           if (i0 + u * kWG < p) {
-            Vc[i0 + u * kWG] = acc[u] / sx[u] * inv;
+            Vc[i0 + u * kWG] = acc[u] / sx[u] * lscale;
             Rc[i0 + u * kWG] = r[u] * inv;
           }
       }
       __syncthreads();
+      if (rsb) {
+        orth_pass(Vc, p, f.P, ptot, comp, false);
+        __syncthreads();
+        scale_copy(Vc, Vc, p, inv);
+        __syncthreads();
+      }
+      if (f.P) {
+        scale_copy(f.P + (long long)comp * p, Vc, p, 1.0);
+        __syncthreads();
+      }
     }
     // response loadings q_a = S0ᵀ r (= Ỹᵀ t on the matrix-free route)
     if (f.qraw) {
-      for (int c = threadIdx.x; c < q; c += kWG) Qc[c] = f.qraw[c + (long long)j * q] * inv;
+      // Ỹᵀ t; with the earlier scores projected out, minus Q (= Ỹᵀ T) times
+      // the summed projections
+      const bool rsb = !f.xgram && f.reorth && comp > 0 && f.T != nullptr && f.P != nullptr;
+      const double* ptot = f.scratch + L.ptot;
+      for (int c = threadIdx.x; c < q; c += kWG) {
+        double v = f.qraw[c + (long long)j * q];
+        if (rsb)
+          for (int l = 0; l < comp; ++l) v -= f.Q[c + (long long)l * q] * ptot[l];
+        Qc[c] = v * inv;
+      }
       __syncthreads();
     } else {
       dots(q, f.S0, Rc, Qc);

@@ -202,9 +202,12 @@ int fastpls_hip_fit_phases(void* model, double* out, int count, char* error,
  * per step; the call returns when all fits are done. Arrays of pointers are
  * host arrays; y_device / labels_device / perm_device may be NULL as a whole.
  * reorth: -1 follows the FP64 routing (one Gram-Schmidt pass of the
- * loadings; precision 32 computes in FP64 too, so the reference's doubled
- * passes for its float core, src/r_api.cpp:1416, do not apply), 0 / 1 force
- * one / two passes. */
+ * loadings; precision 32 computes in FP64 too, so the reference's
+ * re-orthogonalisation for its float core, src/r_api.cpp:1416, does not
+ * apply); 0 forces that routing, 1 the float core's: twice per component the
+ * score's projection on the earlier scores is removed from the score and the
+ * weight (simpls.hpp:161-200, 635-642) and the loadings are orthogonalised
+ * twice. */
 void* fastpls_hip_simpls_batch_create(int precision, int group_count, const void* const* x_device,
                                       const int* n, const int* p, int fit_count,
                                       const int* group, const void* const* y_device,
