@@ -50,6 +50,13 @@ class Arena {
 struct GroupIn {
   const void* x;
   int n, p;
+  // row view (all null / 0: every row): the group's fits use the rows whose
+  // mask (device, n) is 1; hold (device, ascending) lists the other hold_n
+  // rows and hold_x (device, hold_n x p) holds their copy
+  const int* mask = nullptr;
+  const int* hold = nullptr;
+  const void* hold_x = nullptr;
+  int hold_n = 0;
 };
 
 struct FitIn {
@@ -332,6 +339,13 @@ class SimplsBatch {
   struct Group {
     const T* X;
     int n, p, qcat = 0, bcat = 0, max_steps = 0;
+    // row view: see GroupIn; hstats (2 x p) holds the held-out rows' shifted sums
+    const int* mask = nullptr;
+    const int* hold = nullptr;
+    const T* hold_x = nullptr;
+    int hold_n = 0;
+    double* hstats = nullptr;
+    int used() const { return n - hold_n; }  // rows the fits see
     double *Yc, *ysum, *S, *S0, *mx, *sx, *colss, *stats, *spart;
     double *cand, *cand_scaled, *t_raw, *tpart, *loads;
     int splits = 1, ls_splits = 1, xs_splits = 1;
@@ -389,6 +403,14 @@ class SimplsBatch {
       gplan_[g].X = static_cast<const T*>(in.x);
       gplan_[g].n = in.n;
       gplan_[g].p = in.p;
+      if (in.hold_n != 0) {
+        if (in.hold_n < 0 || in.n - in.hold_n < 2 || !in.mask || !in.hold || !in.hold_x)
+          throw std::invalid_argument("invalid batch row view");
+        gplan_[g].mask = in.mask;
+        gplan_[g].hold = in.hold;
+        gplan_[g].hold_x = static_cast<const T*>(in.hold_x);
+        gplan_[g].hold_n = in.hold_n;
+      }
     }
     plan_.resize(fits_.size());
     for (size_t f = 0; f < fits_.size(); ++f) {
@@ -396,13 +418,15 @@ class SimplsBatch {
       if (in.group < 0 || in.group >= int(groups_.size())) throw std::invalid_argument("fit refers to a missing group");
       if ((in.y == nullptr) == (in.labels == nullptr)) throw std::invalid_argument("provide responses or labels, not both");
       Group& G = gplan_[in.group];
+      if (G.hold_n > 0 && (in.labels || in.perm))
+        throw std::invalid_argument("row views take unpermuted dense responses");
       Fit& F = plan_[f];
       F.q = in.q;
       if (F.q < 1 || (in.labels && F.q < 2)) throw std::invalid_argument("invalid response count");
       if (F.q > 512) throw std::invalid_argument("more than 512 responses is not supported");
-      if (in.A < 1 || in.A > std::min(G.n - 1, G.p)) throw std::invalid_argument("invalid batch SIMPLS dimensions");
+      if (in.A < 1 || in.A > std::min(G.used() - 1, G.p)) throw std::invalid_argument("invalid batch SIMPLS dimensions");
       F.A = in.A;
-      const int block = candidate_block_size(F.A, G.p, F.q, in.labels != nullptr, G.n, 64);
+      const int block = candidate_block_size(F.A, G.p, F.q, in.labels != nullptr, G.used(), 64);
       const int maxdim = std::min(G.p, F.q);
       F.right_gram = F.q <= G.p && F.q <= 512;
       F.bmax = 1; F.wmax = 1;
@@ -462,7 +486,7 @@ class SimplsBatch {
     }
     // arena layout; the status words (zero at upload) lead the uploaded arena
     status_off_ = up_.add(sizeof(int) * 4 * (gplan_.size() + plan_.size()));
-    constexpr int kGo = 16, kFo = 15;
+    constexpr int kGo = 17, kFo = 15;
     std::vector<size_t> go(gplan_.size() * kGo), fo(plan_.size() * kFo);
     for (size_t g = 0; g < gplan_.size(); ++g) {
       const Group& G = gplan_[g];
@@ -490,6 +514,7 @@ class SimplsBatch {
                          xt_chunks * xtw_frag_bytes<T>(G.n, maxn_step, kF64) +
                          pre_chunks * xtw_frag_bytes<T>(G.n, maxn_pre, kF64) +
                          256 * (xw_chunks + xt_chunks + pre_chunks));
+      o[16] = G.hold_n > 0 ? arena_.add(8 * 2 * p) : 0;  // held-out sums
     }
     for (size_t f = 0; f < plan_.size(); ++f) {
       const Fit& F = plan_[f];
@@ -538,6 +563,7 @@ class SimplsBatch {
       G.loads = arena_.at<double>(o[12]); G.flags = up_.at<int>(status_off_) + 4 * g;
       G.col_flags = up_.at<int>(colflag_off[g]);
       G.tpart = G.xs_splits > 1 ? arena_.at<double>(o[14]) : nullptr;
+      G.hstats = G.hold_n > 0 ? arena_.at<double>(o[16]) : nullptr;
       frag_cursor[g] = arena_.at<char>(o[15]);
     }
     auto take_frag = [&](int g, size_t bytes) -> void* {
@@ -617,6 +643,8 @@ class SimplsBatch {
         cl->yc = G.Yc + (size_t)(F.off + c) * G.n;
         cl->mean = F.my + c;
         cl->ysum = G.ysum + F.off + c;
+        cl->mask = G.hold_n > 0 ? G.mask : nullptr;
+        cl->used = G.used();
       }
     }
     FPLS_HIP(hipMemcpy(up_.at<char>(0), image.data(), image.size(), hipMemcpyHostToDevice));
@@ -817,10 +845,16 @@ class SimplsBatch {
           launch_xtw<T>(pre_dev_[c].as<XtwTask<T>>(), int(pre_[c].size()), pre_tiles_[c],
                         pre_maxn_[c], stream_, kF64, /*stats=*/true);
         });
-    for (Group& G : gplan_)
+    for (Group& G : gplan_) {
+      // a row view: the pass ran over every row (zero responses on the
+      // held-out ones); its column sums lose the held-out rows'
+      if (G.hold_n > 0)
+        launch_holdout_stats<T>(G.hold_x, G.hold_n, G.p, G.X, G.n, G.hstats, stream_);
       launch_finalize_crossprod<T>(G.X, G.n, G.p, G.qcat, scaling_, G.stats, G.spart, G.splits,
                                    G.ysum, G.mx, G.sx, G.colss, G.S, G.S0, G.flags, stream_,
-                                   status_base_, G.col_flags);
+                                   status_base_, G.col_flags, G.hstats,
+                                   G.hold_n > 0 ? G.used() : -1);
+    }
     // right Grams SᵀS of every fit: one launch
     if (gram_count_ > 0)
       launch_xtw<double>(gram_dev_.as<XtwTask<double>>(), gram_count_, gram_tiles_, gram_maxn_,
@@ -845,6 +879,11 @@ class SimplsBatch {
         timed(4, 0.0, [&] {
           launch_sum_slabs_list(sums_.as<SlabSum>(), L.sum_count, L.sum_max, stream_);
         });
+      // row views: the held-out rows' scores are zeroed, so that the loading
+      // pass and the score norms sum the view's rows only
+      for (const Group& G : gplan_)
+        if (G.hold_n > 0 && G.max_steps > s)
+          launch_zero_rows(G.t_raw, G.n, G.bcat, G.hold, G.hold_n, stream_);
       for (int c = 0; c < kWidthClasses; ++c) {
         const auto& P = L.pass[c];
         if (P.xt_count > 0)
@@ -1245,8 +1284,6 @@ void simpls_cv_regression(const T* predictors, const T* responses, const int* fo
     throw std::invalid_argument("invalid HIP regression CV output or rank");
   split.require_usable();
   const std::vector<std::vector<int>>& test_rows = split.test;
-  std::vector<std::vector<int>> train_rows(fold_count);
-  for (int f = 0; f < fold_count; ++f) train_rows[f] = split.training(f, n);
   const long double total_ss = cv_total_ss(responses, n, q);
   hipStream_t stream = stream_take();
   struct StreamGuard {
@@ -1259,22 +1296,24 @@ void simpls_cv_regression(const T* predictors, const T* responses, const int* fo
   FPLS_HIP(hipMemcpy(xfull.as<T>(), predictors, sizeof(T) * (size_t)n * p, hipMemcpyHostToDevice));
   FPLS_HIP(hipMemcpy(yfull.as<T>(), responses, sizeof(T) * (size_t)n * q, hipMemcpyHostToDevice));
   std::vector<int> all_idx;
-  std::vector<size_t> tr_off(fold_count), te_off(fold_count);
+  std::vector<size_t> te_off(fold_count);
   for (int f = 0; f < fold_count; ++f) {
-    tr_off[f] = all_idx.size();
-    all_idx.insert(all_idx.end(), train_rows[f].begin(), train_rows[f].end());
     te_off[f] = all_idx.size();
     all_idx.insert(all_idx.end(), test_rows[f].begin(), test_rows[f].end());
   }
   idx.alloc(sizeof(int) * all_idx.size());
   FPLS_HIP(hipMemcpy(idx.as<int>(), all_idx.data(), sizeof(int) * all_idx.size(), hipMemcpyHostToDevice));
-  // Folds are fitted together in batches of as many folds as fit: a fold holds
-  // gathered copies of its training and held-out rows (n rows of X and Y in
-  // total) and its fit's device state. The reference fits one fold at a time
-  // (one training copy resident, src/cuda_resident_api.cuh:770-1023); here
-  // the folds of a sub-batch share every launch, and a sub-batch holds at
-  // most 2/5 of the free device memory (as the reference's query workspace
-  // rule, :259-297), down to one fold per sub-batch.
+  // Folds are fitted together in batches of as many folds as fit. A fold is a
+  // row view of the one resident X: its fit runs over every row of X with
+  // zero responses and zero scores on its held-out rows, and its column
+  // statistics are the full ones minus the held-out rows'. A fold holds a
+  // gathered copy of its held-out rows only (for those statistics and its
+  // test predictions), a row mask and its fit's device state. The reference
+  // fits one fold at a time on a gathered training copy
+  // (src/cuda_resident_api.cuh:770-1023); here the folds of a sub-batch
+  // share every launch, and a sub-batch holds at most 2/5 of the free device
+  // memory (as the reference's query workspace rule, :259-297), down to one
+  // fold per sub-batch.
   // FASTPLS_HIP_CV_FREE_BYTES, when set, stands for the free memory.
   std::vector<int> usable;
   for (int f = 0; f < fold_count; ++f) {
@@ -1282,13 +1321,15 @@ void simpls_cv_regression(const T* predictors, const T* responses, const int* fo
       status[f] = 2;
       continue;
     }
-    if (maxp > std::min(int(train_rows[f].size()) - 1, p))
+    if (maxp > std::min(n - int(test_rows[f].size()) - 1, p))
       throw std::invalid_argument("HIP CV component count exceeds a fold rank");
     usable.push_back(f);
   }
   if (usable.empty()) throw std::invalid_argument("HIP CV has no non-empty fold");
+  size_t max_test = 0;
+  for (int f : usable) max_test = std::max(max_test, test_rows[f].size());
   const double fold_bytes =
-      1.25 * (double(sizeof(T)) * double(n) * double(p + q) +
+      1.25 * (double(sizeof(T)) * double(max_test) * double(p + q) + 4.0 * double(n) +
               8.0 * double(n) * double(2 * maxp + 2 * q + 4) +
               8.0 * double(p) * double(4 * maxp + 3 * q + 96) + 8.0 * double(q) * double(q));
   size_t free_b = 0, total_b = 0;
@@ -1301,25 +1342,34 @@ void simpls_cv_regression(const T* predictors, const T* responses, const int* fo
   RegressionTally tally(prefix_count);
   for (size_t b0 = 0; b0 < usable.size(); b0 += size_t(per_batch)) {
     const size_t b1 = std::min(usable.size(), b0 + size_t(per_batch));
-    std::vector<std::unique_ptr<Buf>> xtr, ytr, xte, yte;
+    std::vector<std::unique_ptr<Buf>> masks, xte, yte;
     std::vector<GroupIn> groups;
     std::vector<FitIn> fits;
     std::vector<int> fold_of;
+    std::vector<int> host_mask(n);
     for (size_t u = b0; u < b1; ++u) {
       const int f = usable[u];
-      const int ntr = int(train_rows[f].size()), nte = int(test_rows[f].size());
-      for (auto* v : {&xtr, &ytr, &xte, &yte}) v->emplace_back(new Buf);
-      xtr.back()->alloc(sizeof(T) * (size_t)ntr * p);
-      ytr.back()->alloc(sizeof(T) * (size_t)ntr * q);
+      const int nte = int(test_rows[f].size());
+      for (auto* v : {&masks, &xte, &yte}) v->emplace_back(new Buf);
+      masks.back()->alloc(sizeof(int) * (size_t)n);
       xte.back()->alloc(sizeof(T) * (size_t)nte * p);
       yte.back()->alloc(sizeof(T) * (size_t)nte * q);
-      launch_gather_rows<T>(xfull.as<T>(), n, p, idx.as<int>() + tr_off[f], ntr, xtr.back()->as<T>(), stream);
-      launch_gather_rows<T>(yfull.as<T>(), n, q, idx.as<int>() + tr_off[f], ntr, ytr.back()->as<T>(), stream);
+      std::fill(host_mask.begin(), host_mask.end(), 1);
+      for (int r : test_rows[f]) host_mask[size_t(r)] = 0;
+      FPLS_HIP(hipMemcpyAsync(masks.back()->as<int>(), host_mask.data(), sizeof(int) * (size_t)n,
+                              hipMemcpyHostToDevice, stream));
+      FPLS_HIP(hipStreamSynchronize(stream));  // host_mask is refilled for the next fold
       launch_gather_rows<T>(xfull.as<T>(), n, p, idx.as<int>() + te_off[f], nte, xte.back()->as<T>(), stream);
       launch_gather_rows<T>(yfull.as<T>(), n, q, idx.as<int>() + te_off[f], nte, yte.back()->as<T>(), stream);
-      groups.push_back({xtr.back()->as<T>(), ntr, p});
+      GroupIn gi{};
+      gi.x = xfull.as<T>(); gi.n = n; gi.p = p;
+      gi.mask = masks.back()->as<int>();
+      gi.hold = idx.as<int>() + te_off[f];
+      gi.hold_x = xte.back()->as<T>();
+      gi.hold_n = nte;
+      groups.push_back(gi);
       FitIn fi{};
-      fi.group = int(groups.size()) - 1; fi.q = q; fi.A = maxp; fi.y = ytr.back()->as<T>();
+      fi.group = int(groups.size()) - 1; fi.q = q; fi.A = maxp; fi.y = yfull.as<T>();
       fi.labels = nullptr; fi.perm = nullptr; fi.seed = seed;  // SIMPLS folds share the seed
       fits.push_back(fi);
       fold_of.push_back(f);

@@ -140,6 +140,10 @@ struct CenterColumn {
   double* yc;
   double* mean;
   double* ysum;
+  // row view (null: every row): rows with mask 0 are left out of the mean and
+  // get a zero centred value; used = the number of rows with mask 1
+  const int* mask;
+  long long used;
 };
 template<class T>
 void launch_center_columns(const CenterColumn<T>* device_list, int count, hipStream_t s);
@@ -159,7 +163,19 @@ void launch_finalize_crossprod(const T* X, long long n, int p, int q, int scalin
                                const double* ysum, double* mean, double* scale,
                                double* colss, double* S, double* S0, int* flags,
                                hipStream_t s, int* cross_flags = nullptr,
-                               const int* cross_offsets = nullptr);
+                               const int* cross_offsets = nullptr,
+                               const double* hold_stats = nullptr, long long used = -1);
+// Row views: the shifted column sums and sums of squares (2 x p: sum d, sum
+// d^2 with d = x - X[0, j], the shift of the statistics pass over the full X,
+// ld n) of the held-out rows, from their gathered copy hx (hn x p); the
+// statistics of the view are the full ones minus these.
+template<class T>
+void launch_holdout_stats(const T* hx, long long hn, int p, const T* X, long long n, double* out,
+                          hipStream_t s);
+// t[rows[i] + c ld] = 0 for i < count, c < cols (the held-out rows of a row
+// view's scores, so that the products and norms that follow sum its rows only)
+void launch_zero_rows(double* t, long long ld, int cols, const int* rows, int count,
+                      hipStream_t s);
 // out (n x cols, FP64) = (X[:, c0 + c] - mean[c0 + c]) / scale[c0 + c]: columns
 // c0 .. c0 + cols of the standardised X (leading dimension n)
 template<class T>

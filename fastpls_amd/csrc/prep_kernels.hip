@@ -73,15 +73,28 @@ __global__ __launch_bounds__(kWG) void k_center_columns(const CenterColumn<T>* l
   __shared__ double red[kWaves];
   const CenterColumn<T> c = list[blockIdx.x];
   double s = 0.0;
+  if (c.mask) {
+    for (long long i = threadIdx.x; i < c.n; i += kWG)
+      if (c.mask[i]) s += double(c.y[c.perm ? c.perm[i] : i]);
+  } else {
 #pragma unroll 4
-  for (long long i = threadIdx.x; i < c.n; i += kWG) s += double(c.y[c.perm ? c.perm[i] : i]);
-  const double mu = wg_sum(s, red) / double(c.n);
+    for (long long i = threadIdx.x; i < c.n; i += kWG) s += double(c.y[c.perm ? c.perm[i] : i]);
+  }
+  const double mu = wg_sum(s, red) / double(c.mask ? c.used : c.n);
   double cs = 0.0;
+  if (c.mask) {
+    for (long long i = threadIdx.x; i < c.n; i += kWG) {
+      const double v = c.mask[i] ? double(c.y[c.perm ? c.perm[i] : i]) - mu : 0.0;
+      c.yc[i] = v;
+      cs += v;
+    }
+  } else {
 #pragma unroll 4
-  for (long long i = threadIdx.x; i < c.n; i += kWG) {
-    const double v = double(c.y[c.perm ? c.perm[i] : i]) - mu;
-    c.yc[i] = v;
-    cs += v;
+    for (long long i = threadIdx.x; i < c.n; i += kWG) {
+      const double v = double(c.y[c.perm ? c.perm[i] : i]) - mu;
+      c.yc[i] = v;
+      cs += v;
+    }
   }
   cs = wg_sum(cs, red);
   if (threadIdx.x == 0) {
@@ -159,19 +172,25 @@ __global__ __launch_bounds__(kWG) void k_label_fill(const int* labels, long long
 template<class T>
 __global__ __launch_bounds__(kWG) void k_finalize_stats(
     const T* X, long long n, int p, int scaling, const double* stats_part, int splits,
-    double* mean, double* scale, double* colss, int* flags) {
+    double* mean, double* scale, double* colss, int* flags, const double* hold_stats,
+    long long used) {
   const int j = blockIdx.x * kWG + threadIdx.x;
   if (j >= p) return;
-  const double s1 = sum_strided(stats_part + j, 2LL * p, splits);
-  const double s2 = sum_strided(stats_part + p + j, 2LL * p, splits);
+  double s1 = sum_strided(stats_part + j, 2LL * p, splits);
+  double s2 = sum_strided(stats_part + p + j, 2LL * p, splits);
+  if (hold_stats) {  // a row view: the full sums minus the held-out rows'
+    s1 -= hold_stats[j];
+    s2 -= hold_stats[p + j];
+  }
   const double shift = double(X[(long long)j * n]);
-  const double dn = double(n);
+  const long long rows = used >= 0 ? used : n;
+  const double dn = double(rows);
   double mu = 0.0, sd = 1.0;
   if (scaling != 3) {
     mu = shift + s1 / dn;
     if (scaling == 2) {
       const double css = fmax(0.0, s2 - s1 * s1 / dn);
-      sd = sqrt(css / double(n > 1 ? n - 1 : 1));
+      sd = sqrt(css / double(rows > 1 ? rows - 1 : 1));
       if (!isfinite(sd) || sd <= 0.0) sd = 1.0;
     }
   }
@@ -200,6 +219,41 @@ __global__ __launch_bounds__(kWG) void k_finalize_cross(
   S[j + (long long)c * p] = v;
   if (S0) S0[j + (long long)c * p] = v;
   if (!isfinite(v)) atomicOr(flags, kFlagNonFinite);
+}
+
+// Shifted sums of the held-out rows of a row view: one wave per column j,
+// over the gathered copy hx (hn x p), with the shift X[0, j] of the full X.
+template<class T>
+__global__ __launch_bounds__(kWG) void k_holdout_stats(const T* hx, long long hn, int p,
+                                                       const T* X, long long n, double* out) {
+  const int lane = threadIdx.x & 63;
+  const int j = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  if (j >= p) return;
+  const double shift = double(X[(long long)j * n]);
+  const T* c = hx + (long long)j * hn;
+  double s1 = 0.0, s2 = 0.0;
+  for (long long i = lane; i < hn; i += 64) {
+    const double d = double(c[i]) - shift;
+    s1 += d;
+    s2 = fma(d, d, s2);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    s1 += __shfl_xor(s1, off, 64);
+    s2 += __shfl_xor(s2, off, 64);
+  }
+  if (lane == 0) {
+    out[j] = s1;
+    out[p + j] = s2;
+  }
+}
+
+__global__ __launch_bounds__(kWG) void k_zero_rows(double* t, long long ld, int cols,
+                                                   const int* rows, int count) {
+  for (long long e = (long long)blockIdx.x * kWG + threadIdx.x; e < (long long)count * cols;
+       e += (long long)gridDim.x * kWG) {
+    const long long c = e / count, i = e % count;
+    t[rows[i] + c * ld] = 0.0;
+  }
 }
 
 // dst (count x cols) = src[idx[i], j] for a column-major src with n rows.
@@ -487,15 +541,34 @@ void launch_finalize_crossprod(const T* X, long long n, int p, int q, int scalin
                                const double* stats_part, const double* s_part, int splits,
                                const double* ysum, double* mean, double* scale,
                                double* colss, double* S, double* S0, int* flags,
-                               hipStream_t s, int* cross_flags, const int* cross_offsets) {
+                               hipStream_t s, int* cross_flags, const int* cross_offsets,
+                               const double* hold_stats, long long used) {
   const unsigned blocks = unsigned((p + kWG - 1) / kWG);
   hipLaunchKernelGGL((k_finalize_stats<T>), dim3(blocks), dim3(kWG), 0, s, X, n, p, scaling,
-                     stats_part, splits, mean, scale, colss, flags);
+                     stats_part, splits, mean, scale, colss, flags, hold_stats, used);
   if (q > 0)
     hipLaunchKernelGGL((k_finalize_cross<T>), dim3(blocks, q), dim3(kWG), 0, s, X, n, p, q, s_part,
                        splits, ysum, mean, scale, S, S0,
                        cross_offsets ? cross_flags : flags, cross_offsets);
   FPLS_LAUNCHED("k_finalize_cross");
+}
+
+template<class T>
+void launch_holdout_stats(const T* hx, long long hn, int p, const T* X, long long n, double* out,
+                          hipStream_t s) {
+  if (p <= 0) return;
+  hipLaunchKernelGGL((k_holdout_stats<T>), dim3(unsigned((p + kWaves - 1) / kWaves)), dim3(kWG), 0,
+                     s, hx, hn, p, X, n, out);
+  FPLS_LAUNCHED("k_holdout_stats");
+}
+
+void launch_zero_rows(double* t, long long ld, int cols, const int* rows, int count,
+                      hipStream_t s) {
+  if (count <= 0 || cols <= 0) return;
+  const long long total = (long long)count * cols;
+  const unsigned blocks = unsigned(std::min<long long>((total + kWG - 1) / kWG, 1024));
+  hipLaunchKernelGGL(k_zero_rows, dim3(blocks), dim3(kWG), 0, s, t, ld, cols, rows, count);
+  FPLS_LAUNCHED("k_zero_rows");
 }
 
 template<class T>
@@ -621,7 +694,9 @@ void launch_convert(const double* in, T* out, long long count, hipStream_t s) {
                                              const double*, const double*, int,              \
                                              const double*, double*, double*, double*,       \
                                              double*, double*, int*, hipStream_t, int*,      \
-                                             const int*);                                    \
+                                             const int*, const double*, long long);          \
+  template void launch_holdout_stats<T>(const T*, long long, int, const T*, long long,       \
+                                        double*, hipStream_t);                               \
   template void launch_apply_small<T>(const double*, long long, int, const double*,          \
                                       long long, long long, int, const double*, T*, double*, \
                                       hipStream_t);                                          \

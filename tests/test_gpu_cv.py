@@ -124,6 +124,37 @@ def test_simpls_cv_regression_matches_oracle(hip, oracle, metric):
     assert rel(got["predictions"], want["predictions"]) < 1e-9
 
 
+@pytest.mark.parametrize("precision,scaling", [(64, 1), (64, 2), (64, 3), (32, 2)])
+def test_simpls_cv_regression_row_views(hip, oracle, precision, scaling):
+    """The batched regression CV runs each fold as a row view of the one
+    resident X: every pass covers all rows with zero responses and zero
+    scores on the fold's held-out rows, and the fold's column statistics are
+    the full sums minus the held-out rows' (same shift). Uneven folds,
+    column means far from zero (the subtraction of shifted sums), ten
+    responses (the statistics pass on the FP64 matrix cores); against
+    per-fold oracle fits on the gathered training rows (on the FP32 values
+    widened to FP64 at precision 32)."""
+    fp = hip[0]
+    rng = np.random.default_rng(12)
+    X, Y = latent_data(700, 300, 10, factors=6, noise=0.2, seed=12)
+    X = X + 40.0 * rng.standard_normal(300)
+    folds = rng.choice([1, 2, 3, 4], size=700, p=[0.1, 0.2, 0.3, 0.4]).astype(np.int32)
+    dt = np.float64 if precision == 64 else np.float32
+    Xp, Yp = X.astype(dt), Y.astype(dt)
+    prefixes = [1, 4, 8]
+    got = fp.simpls_cv_regression(Xp, Yp, folds, prefixes, precision=precision, scaling=scaling,
+                                  metric=3, oversample=6, power=2, seed=5)
+    want = _oracle_cv_regression(oracle, 0, Xp.astype(np.float64), Yp.astype(np.float64), folds,
+                                 prefixes, seed=5, oversample=6, power=2, scaling=scaling)
+    np.testing.assert_array_equal(got["status"], want["status"])
+    err = rel(got["predictions"], want["predictions"])
+    print(f"precision {precision} scaling {scaling}: predictions {err:.2e}")
+    tol = 1e-9 if precision == 64 else 1e-7
+    assert err < tol
+    for key in ("q2", "rmsd", "observed_r2"):
+        np.testing.assert_allclose(got[key], want[key], rtol=tol)
+
+
 def test_simpls_cv_regression_empty_fold_status(hip):
     fp = hip[0]
     X, Y = latent_data(120, 15, 1, factors=3, seed=7)
