@@ -647,7 +647,7 @@ class SimplsBatch {
         cl->used = G.used();
       }
     }
-    FPLS_HIP(hipMemcpy(up_.at<char>(0), image.data(), image.size(), hipMemcpyHostToDevice));
+    upload(up_.at<char>(0), image.data(), image.size());
     // task lists of the per-step passes (identical for every step), in group
     // order; prefix bookkeeping for the active groups of each step. Tasks are
     // kept in width classes, each launched on its own: narrow right-hand sides
@@ -743,13 +743,11 @@ class SimplsBatch {
       }
       if (!xw[c].empty()) {
         xw_tasks_[c].alloc(sizeof(XwTask<T>) * xw[c].size());
-        FPLS_HIP(hipMemcpy(xw_tasks_[c].as<XwTask<T>>(), xw[c].data(),
-                           sizeof(XwTask<T>) * xw[c].size(), hipMemcpyHostToDevice));
+        upload(xw_tasks_[c].as<XwTask<T>>(), xw[c].data(), sizeof(XwTask<T>) * xw[c].size());
       }
       if (!xt[c].empty()) {
         xt_tasks_[c].alloc(sizeof(XtwTask<T>) * xt[c].size());
-        FPLS_HIP(hipMemcpy(xt_tasks_[c].as<XtwTask<T>>(), xt[c].data(),
-                           sizeof(XtwTask<T>) * xt[c].size(), hipMemcpyHostToDevice));
+        upload(xt_tasks_[c].as<XtwTask<T>>(), xt[c].data(), sizeof(XtwTask<T>) * xt[c].size());
       }
     }
     for (size_t na = 1; na <= ng; ++na) {
@@ -758,7 +756,7 @@ class SimplsBatch {
     }
     if (!sums.empty()) {
       sums_.alloc(sizeof(SlabSum) * sums.size());
-      FPLS_HIP(hipMemcpy(sums_.as<SlabSum>(), sums.data(), sizeof(SlabSum) * sums.size(), hipMemcpyHostToDevice));
+      upload(sums_.as<SlabSum>(), sums.data(), sizeof(SlabSum) * sums.size());
     }
     // preprocessing task lists: column statistics + cross-products of every
     // group, one launch per width class
@@ -785,8 +783,7 @@ class SimplsBatch {
       pre_tiles_[c] = pack_xtw(pre_[c].data(), int(pre_[c].size()), &pre_maxn_[c], kF64,
                                c == 0 ? 0 : kMfmaMinN);
       pre_dev_[c].alloc(sizeof(XtwTask<T>) * pre_[c].size());
-      FPLS_HIP(hipMemcpy(pre_dev_[c].as<XtwTask<T>>(), pre_[c].data(),
-                         sizeof(XtwTask<T>) * pre_[c].size(), hipMemcpyHostToDevice));
+      upload(pre_dev_[c].as<XtwTask<T>>(), pre_[c].data(), sizeof(XtwTask<T>) * pre_[c].size());
     }
     // right Grams SᵀS of every fit that works in the response space: one
     // task list, launched after the cross-products
@@ -808,9 +805,19 @@ class SimplsBatch {
       gram_count_ = int(gram.size());
       gram_tiles_ = pack_xtw(gram.data(), gram_count_, &gram_maxn_);
       gram_dev_.alloc(sizeof(XtwTask<double>) * gram.size());
-      FPLS_HIP(hipMemcpy(gram_dev_.as<XtwTask<double>>(), gram.data(),
-                         sizeof(XtwTask<double>) * gram.size(), hipMemcpyHostToDevice));
+      upload(gram_dev_.as<XtwTask<double>>(), gram.data(), sizeof(XtwTask<double>) * gram.size());
     }
+  }
+
+  // Host-to-device upload of planning data on the batch's own stream, ahead
+  // of its launches. (A synchronous copy runs on the null stream, whose
+  // hardware queue may be the one another lane's running kernels hold, and
+  // would wait behind them.) The host bytes are kept until the batch goes.
+  void upload(void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return;
+    const char* b = static_cast<const char*>(src);
+    staged_.emplace_back(b, b + bytes);
+    FPLS_HIP(hipMemcpyAsync(dst, staged_.back().data(), bytes, hipMemcpyHostToDevice, stream_));
   }
 
   // The whole execution as launches on stream_ (no host synchronisation, no
@@ -931,8 +938,10 @@ class SimplsBatch {
     // group flags then fit states, one copy
     const size_t ng = gplan_.size(), nf = plan_.size();
     std::vector<int> status(4 * (ng + nf));
-    FPLS_HIP(hipMemcpy(status.data(), up_.at<int>(status_off_), sizeof(int) * status.size(),
-                       hipMemcpyDeviceToHost));
+    // on the batch's stream (after its work), not behind another lane's
+    FPLS_HIP(hipMemcpyAsync(status.data(), up_.at<int>(status_off_), sizeof(int) * status.size(),
+                            hipMemcpyDeviceToHost, stream_));
+    FPLS_HIP(hipStreamSynchronize(stream_));
     completed_.assign(nf, 0);
     flags_.assign(nf, 0);
     for (size_t f = 0; f < nf; ++f) {
@@ -977,6 +986,7 @@ class SimplsBatch {
   int gram_count_ = 0, gram_maxn_ = 0;
   long long gram_tiles_ = 0;
   hipGraphExec_t graph_ = nullptr;  // run() captured by the first rerun
+  std::vector<std::vector<char>> staged_;  // host sources of the uploads
   BatchSplits splits_;
   bool has_wide_ = false, has_narrow_ = false;
   long long pre_tiles_[kWidthClasses] = {};
@@ -1052,8 +1062,7 @@ class TypedBatch final : public BatchHandle {
       if (f.group < 0 || f.group >= int(groups_.size()))
         throw std::invalid_argument("fit refers to a missing group");
     partition(lane_count());
-    plan_ = build(default_splits());
-    execute(*plan_, false);
+    plan_ = build_launch(default_splits());
   }
   int count() const override { return int(fits_.size()); }
   double seconds() const override { return plan_->seconds; }
@@ -1219,6 +1228,37 @@ class TypedBatch final : public BatchHandle {
       P->b.push_back(std::make_unique<SimplsBatch<T>>(
           lane_groups_[size_t(l)], lane_fits_[size_t(l)], scaling_, oversample_, power_, reorth_,
           cfg, L > 1 ? P->streams[size_t(l)] : nullptr, true));
+    return P;
+  }
+
+  // The first execution of a plan: each lane is planned and at once launched,
+  // so the host planning of the later lanes (sizes, sketches, uploads)
+  // overlaps the device work of the earlier ones instead of delaying the
+  // first launch. The device seconds run from one event recorded before the
+  // first launch to the last lane's end.
+  std::unique_ptr<Plan> build_launch(const BatchSplits& cfg) {
+    const int L = int(lane_groups_.size());
+    if (L == 1) {
+      auto P = build(cfg);
+      execute(*P, false);
+      return P;
+    }
+    Event fork;  // declared first: destroyed after the plan has drained its streams
+    auto P = std::make_unique<Plan>();
+    P->streams = stream_set_take(L);
+    for (int l = 0; l < L; ++l) {
+      P->b.push_back(std::make_unique<SimplsBatch<T>>(
+          lane_groups_[size_t(l)], lane_fits_[size_t(l)], scaling_, oversample_, power_, reorth_,
+          cfg, P->streams[size_t(l)], true));
+      if (l == 0) FPLS_HIP(hipEventRecord(fork.e, P->streams[0]));
+      P->b.back()->launch(fork.e, false);
+    }
+    double s = 0.0;
+    for (auto& b : P->b) {
+      b->finish();
+      s = std::max(s, b->seconds_since(fork.e));
+    }
+    P->seconds = s;
     return P;
   }
 
