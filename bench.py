@@ -160,7 +160,7 @@ def calibrate_copy(fp, mib):
           flush=True)
 
 
-# --------------------------------------------------------------- C3 (head)
+# ------------------- headline: batched SIMPLS, 256 fits (section key c3)
 C3_WORKLOAD = ("batched_simpls 256 fits: 2 x (5-fold CV x 20 response-row permutations) + 56 "
                "singletons; n~U[500,2e4], p~U[100,5e3], q~U[1,10], ncomp~U[2,30]; latent-factor "
                "data (10 factors, noise 0.1); oversample 10, power 2; X, Y FP32 in HBM")
@@ -362,7 +362,7 @@ def c3_batched(fp, steps, warmup, cpu, cpu_budget):
     }
 
 
-# ------------------------------------------------------------------- C1
+# ------------------ plain SIMPLS, 2000 x 1000 x 3, 10 comps (section c1)
 def c1_plain(fp, steps, cpu):
     from oracle.oracle import latent_data
     n, p, q, A = 2000, 1000, 3, 10
@@ -397,7 +397,7 @@ def c1_plain(fp, steps, cpu):
                                    note="(1+A) n p 8 algorithmic bytes over the fit's device time: "
                                         "X (16 MB) sits in the 256 MB MALL and the fit is bound by "
                                         "the latency of its 10 dependent component steps")
-    # 64 concurrent C1-shaped fits (64 data sets) in one batch
+    # 64 concurrent fits of the 2000 x 1000 x 3 shape (64 data sets) in one batch
     bufs, groups, fits = [], [], []
     for d in range(64):
         Xd, Yd = latent_data(n, p, q, factors=10, noise=0.1, seed=d)
@@ -440,7 +440,7 @@ def c1_plain(fp, steps, cpu):
     return res
 
 
-# ------------------------------------------------------------------- C2
+# ---------------- PLS-SVD, 1e5 x 2e4 x 50, 20 comps, FP32 X (section c2)
 def c2_plssvd(fp, steps, warmup, plain_steps, cpu, cpu_rows, cpu_budget):
     n, p, q, k = 100000, 20000, 50, 20
     rows = 4096
@@ -533,7 +533,7 @@ def c2_plssvd(fp, steps, warmup, plain_steps, cpu, cpu_rows, cpu_budget):
     return res
 
 
-# ------------------------------------------------------------------- C4
+# ------------- PLS-DA 5e4 x 1e4, 20 classes, 15 comps + LDA (section c4)
 def class_split(n, ntest, p, K, latent=15, spread=0.5, seed=2):
     """Balanced PLS-DA data: class means spread * N(0, I) in a latent space
     embedded through a p x latent loading, plus N(0, 1) noise; training and
@@ -629,7 +629,7 @@ def c4_plsda(fp, steps, cpu):
     return res
 
 
-# ------------------------------------------------------------------- C5
+# --------------------- serving stream, 10000 requests at 500/s (section c5)
 def c5_serving(fp, requests=10000, rate=500.0, seed=3, workers=32, max_fits=0):
     """Serving stream: `requests` requests with Poisson arrivals at `rate`/s
     (seed 3) against the request server (HBM budget 200 GB, FP32 storage,
