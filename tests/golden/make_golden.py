@@ -233,9 +233,30 @@ def special_cases(ref):
              power=np.int64(2), seed=np.int64(5))
 
 
+def reorth_cases(ref):
+    """Dense SIMPLS with the re-orthogonalisation forced (the reference float
+    core's routing: scores and weights against the earlier scores twice,
+    simpls.hpp:161-200, 635-642, loadings twice): 30 components of a rank-8
+    problem, where it and the single-pass routing part by about 5e-7 in B.
+    Inputs are stored as their generator recipe and checksum."""
+    n, p, q, factors, noise, seed, decay = 200, 300, 3, 8, 0.05, 7, 0.85
+    X, Y = latent_data(n, p, q, factors=factors, noise=noise, seed=seed, decay=decay)
+    ncomp, oversample, power, rseed = 30, 10, 2, 3
+    m = ref.simpls_fit(X, Y, ncomp=ncomp, oversample=oversample, power=power, seed=rseed,
+                       block=1, cache=0, reorth=1, batch_geometry=0)
+    save("simpls_dense_reorth", R=m["R"], Q=m["Q"], V=m["V"], T=m["T"], mx=m["mx"], sx=m["sx"],
+         my=m["my"], completed=np.int64(m["completed"]), ncomp=np.int64(ncomp),
+         oversample=np.int64(oversample), power=np.int64(power), seed=np.int64(rseed),
+         reorth=np.int64(1), recipe=np.asarray([n, p, q, factors, noise, seed, decay]),
+         x_sha256=digest(X))
+
+
 def main():
     build(reference=True)
     ref = Reference()
+    if len(sys.argv) > 1 and sys.argv[1] == "reorth":
+        reorth_cases(ref)
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "audited":
         audited_cases(ref)
         return
@@ -252,6 +273,7 @@ def main():
     operator_cases(ref)
     edge_cases(ref)
     special_cases(ref)
+    reorth_cases(ref)
 
     # SIMPLS, dense responses, explicit width = q (direction independent of the sketch)
     X, Y = latent_data(64, 20, 3, factors=5, noise=0.1, seed=11)

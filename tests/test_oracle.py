@@ -59,6 +59,30 @@ def test_simpls_c1_workload_matches_reference_core(oracle):
         assert rel(coef(m, k), coef(g, k)) < 1e-9
 
 
+def test_simpls_reorth_matches_reference_core(oracle):
+    """Dense SIMPLS with the re-orthogonalisation forced (the reference float
+    core's routing: scores and weights against the earlier scores twice,
+    simpls.hpp:161-200, 635-642, loadings twice) against the compiled
+    reference core, at 30 components of a rank-8 problem. The single-pass
+    routing parts from the fixture by far more than the tolerance there, so the
+    check pins the re-orthogonalisation itself."""
+    g = load_golden("simpls_dense_reorth")
+    n, p, q, factors, noise, seed, decay = g["recipe"]
+    X, Y = latent_data(int(n), int(p), int(q), factors=int(factors), noise=float(noise),
+                       seed=int(seed), decay=float(decay))
+    import hashlib
+    assert hashlib.sha256(np.asfortranarray(X).tobytes(order="F")).hexdigest() == str(g["x_sha256"])
+    A = int(g["ncomp"])
+    kw = dict(ncomp=A, oversample=int(g["oversample"]), power=int(g["power"]), seed=int(g["seed"]))
+    m1 = oracle.simpls_fit(X, Y, reorth=1, **kw)
+    m0 = oracle.simpls_fit(X, Y, reorth=0, **kw)
+    assert m1["completed"] == int(g["completed"]) == A
+    for k in range(1, A + 1):
+        assert rel(coef(m1, k), coef(g, k)) < 1e-10, k
+    assert rel(coef(m0, A), coef(g, A)) > 1e-8
+    np.testing.assert_allclose(m1["T"].T @ m1["T"], np.eye(A), atol=1e-13)
+
+
 @pytest.mark.parametrize("name", ["plssvd_sketch", "plssvd_dense", "plssvd_labels"])
 def test_plssvd_matches_reference_core(oracle, name):
     g = load_golden(name)
